@@ -1,0 +1,193 @@
+/*
+ * sw4hip.h -- C ABI of the MI355X (gfx950) implementation of the sw4lite
+ * elastic-wave time-step hot path.
+ *
+ * Two levels:
+ *  1. Kernel-level entry points with the argument lists of the reference
+ *     kernels (index bounds, raw arrays, coefficient tables), operating on
+ *     DEVICE arrays in the Sarray layout, stream-ordered and asynchronous.
+ *  2. An EW-level object that owns the device state of one Cartesian grid and
+ *     runs the reference time-step loop (EW::timesteploop, EW.C:2339-2928),
+ *     either stage by stage (the reference order) or with the fused sweeps.
+ *
+ * Conventions: every function returns SW4HIP_OK (0) or an error code; the
+ * message of the last error of the calling thread is sw4hip_last_error().
+ * Index boxes include the two ghost points (ifirst = 1-2 on a single grid).
+ * corder = 1 selects the component-blocked layout u[i + ni j + nij k +
+ * npts c] (Sarray m_corder, Sarray.C:753-778), corder = 0 the interleaved one
+ * u[c + 3 (i + ni j + nij k)].  `stream` is a hipStream_t (NULL = default).
+ * Host pointers are caller-owned and only read during the call.
+ */
+#ifndef SW4HIP_H
+#define SW4HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+   SW4HIP_OK = 0,
+   SW4HIP_EINVAL = 1, /* bad argument */
+   SW4HIP_EHIP = 2,   /* HIP runtime error */
+   SW4HIP_ENOMEM = 3, /* device allocation failed */
+   SW4HIP_ENODEV = 4, /* no usable GPU */
+   SW4HIP_ERANGE = 5, /* size beyond a stated limit */
+   SW4HIP_ESTATE = 6  /* call out of order (e.g. step before setup) */
+};
+
+/* boundaryConditionType of the reference (src/double/sw4.h:37-38) */
+enum { SW4HIP_BC_FREE = 0, SW4HIP_BC_DIRICHLET = 1, SW4HIP_BC_SUPERGRID = 2, SW4HIP_BC_PERIODIC = 3 };
+/* time functions (timeDep subset, src/double/sw4.h:40-41) */
+enum { SW4HIP_TF_RICKER = 0, SW4HIP_TF_GAUSSIAN = 1, SW4HIP_TF_C6SMOOTHBUMP = 2 };
+
+const char* sw4hip_last_error(void);
+int sw4hip_version(void);
+
+/* ----------------------------------------------------------------------- */
+/* Host-side set-up                                                          */
+/* ----------------------------------------------------------------------- */
+
+/* Replaces EW::GetStencilCoefficients (ew-cfromfort.C:1487-1944): acof[384],
+ * ghcof[6], bope[48], sbop[5] in the reference's layout. */
+int sw4hip_get_stencil_coefficients(double* acof, double* ghcof, double* bope, double* sbop);
+
+/* One Cartesian grid: nx,ny,nz interior points, arrays over [-1,n+2]. */
+typedef struct {
+   int nx, ny, nz;
+   double h;
+   double zmin;         /* z of k=1 (0 for a single grid) */
+   int corder;          /* Sarray layout, see above */
+   int bc[6];           /* SW4HIP_BC_* per side x-,x+,y-,y+,z-,z+ */
+   int sg_gp_thickness; /* supergrid width in points (reference default 20) */
+   double sg_damping;   /* supergrid damping coefficient (default 0.02) */
+   double cfl;          /* CFL number (default 1.3) */
+} sw4hip_grid_desc;
+
+/* Replaces EW::setup_supergrid + assign_supergrid_damping_arrays
+ * (EW.C:4636-4801, SuperGrid.C:54-194): 1-D arrays of length n+4 per axis. */
+int sw4hip_host_supergrid(const sw4hip_grid_desc* g, double* dcx, double* dcy, double* dcz,
+                          double* strx, double* stry, double* strz, double* cox, double* coy,
+                          double* coz);
+
+/* Replaces EW::setup_boundary_arrays / side_plane (EW.C:3347-3443). */
+int sw4hip_host_boundary_windows(const sw4hip_grid_desc* g, int wind[36], int npts[6]);
+
+/* Replaces Source::set_grid_point_sources4 for a Cartesian moment-tensor
+ * source (Source.C:1074-2026).  m = {xx,xy,xz,yy,yz,zz}.  Writes at most
+ * maxn grid-point sources: ijk[3n], force[3n]; *n = number produced. */
+int sw4hip_host_moment_source(const sw4hip_grid_desc* g, double x, double y, double z,
+                              const double m[6], int maxn, int* n, int* ijk, double* force);
+
+/* ----------------------------------------------------------------------- */
+/* Kernel-level entry points (device arrays)                                */
+/* ----------------------------------------------------------------------- */
+
+/* Replaces rhs4sg (rhs4sg.C:38-41, corder 0) and rhs4sg_rev
+ * (rhs4sg_rev.C:43-48, corder 1); prototypes EW.C:141-149.  lu = L(u).
+ * onesided[6], acof, bope, ghcof are HOST tables (as in the reference). */
+int sw4hip_rhs4sg(int corder, int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast,
+                  int nk, const int* onesided, const double* acof, const double* bope,
+                  const double* ghcof, double* lu, const double* u, const double* mu,
+                  const double* lambda, double h, const double* strx, const double* stry,
+                  const double* strz, void* stream);
+
+/* Replaces EW::predfort (ew-cfromfort.C:70-113): up = 2u - um + dt2/rho (lu + fo). */
+int sw4hip_predfort(int corder, int ib, int ie, int jb, int je, int kb, int ke, double* up,
+                    const double* u, const double* um, const double* lu, const double* fo,
+                    const double* rho, double dt2, void* stream);
+
+/* Replaces EW::corrfort (ew-cfromfort.C:40-67): up += dt4/(12 rho) (lu + fo). */
+int sw4hip_corrfort(int corder, int ib, int ie, int jb, int je, int kb, int ke, double* up,
+                    const double* lu, const double* fo, const double* rho, double dt4,
+                    void* stream);
+
+/* Replaces EW::dpdmtfort (ew-cfromfort.C:116-141): uacc = dt2i (up - 2u + um). */
+int sw4hip_dpdmtfort(int ib, int ie, int jb, int je, int kb, int ke, const double* up,
+                     const double* u, const double* um, double* uacc, double dt2i, void* stream);
+
+/* Replaces EW::addsgd4fort / addsgd4fort_indrev (ew-cfromfort.C:748-849,
+ * :952-1056).  1-D arrays are device arrays. */
+int sw4hip_addsgd4(int corder, int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast,
+                   double* up, const double* u, const double* um, const double* rho,
+                   const double* dcx, const double* dcy, const double* dcz, const double* strx,
+                   const double* stry, const double* strz, const double* cox, const double* coy,
+                   const double* coz, double beta, void* stream);
+
+/* Replaces EW::bcfortsg / bcfortsg_indrev (ew-cfromfort.C:205-747).
+ * wind[36], bccnd[6], sbop[5] are HOST tables; bforce[6] is a host array of
+ * device pointers (NULL entries mean zero forcing).  t, om, ph, cv of the
+ * reference are unused by the routine and omitted. */
+int sw4hip_bcfortsg(int corder, int ib, int ie, int jb, int je, int kb, int ke, const int* wind,
+                    int nx, int ny, int nz, double* u, double h, const int* bccnd,
+                    const double* sbop, const double* mu, const double* lambda,
+                    const double* const* bforce, const double* strx, const double* stry,
+                    void* stream);
+
+/* ----------------------------------------------------------------------- */
+/* EW-level object                                                           */
+/* ----------------------------------------------------------------------- */
+typedef struct sw4hip_ew sw4hip_ew;
+
+typedef struct {
+   double dt;
+   int nsteps;
+   double t;         /* current time */
+   int cycle;        /* steps taken */
+   int onesided[6];
+   int wind[36];
+   long npts;        /* points per component incl. ghosts */
+   int nsources;     /* grid-point sources */
+   int nunique;      /* distinct source points */
+} sw4hip_ew_info;
+
+/* The EW constructor + set-up of one grid (EW::EW / setupRun, EW.C:162-224,
+ * 2288-2336) on the current HIP device. */
+int sw4hip_ew_create(const sw4hip_grid_desc* g, sw4hip_ew** ew);
+int sw4hip_ew_destroy(sw4hip_ew* ew);
+/* Stream used by every later call on this object (hipStream_t). */
+int sw4hip_ew_set_stream(sw4hip_ew* ew, void* stream);
+/* rho, mu, lambda over the whole box incl. ghosts; on_device selects whether
+ * the pointers are device (1) or host (0) arrays. */
+int sw4hip_ew_set_material(sw4hip_ew* ew, const double* rho, const double* mu,
+                           const double* lambda, int on_device);
+/* testpointsource material: constant rho, cs, cp (EW.C:893-950). */
+int sw4hip_ew_set_material_const(sw4hip_ew* ew, double rho, double cs, double cp);
+/* A moment-tensor source (EW::processSource, EW.C:951-1313), m already
+ * multiplied by m0. */
+int sw4hip_ew_add_moment_source(sw4hip_ew* ew, double x, double y, double z, const double m[6],
+                                double t0, double freq, int tfkind);
+/* Supergrid arrays, boundary windows, time step (EW::computeDT with
+ * t = tmax; nsteps > 0 overrides the step count with dt from CFL), source
+ * discretisation.  Must follow the material and the sources. */
+int sw4hip_ew_setup(sw4hip_ew* ew, double tmax, int nsteps);
+int sw4hip_ew_get_info(sw4hip_ew* ew, sw4hip_ew_info* info);
+/* Copies of the set-up products (host buffers). */
+int sw4hip_ew_get_supergrid(sw4hip_ew* ew, double* dcx, double* dcy, double* dcz, double* strx,
+                            double* stry, double* strz, double* cox, double* coy, double* coz);
+int sw4hip_ew_get_sources(sw4hip_ew* ew, int* ijk, double* force, double* freq, double* t0,
+                          int* tfkind, int* first);
+/* U (time t) and Um (t - dt), 3 components each, in the grid's layout. */
+int sw4hip_ew_set_solution(sw4hip_ew* ew, const double* u, const double* um, int on_device);
+int sw4hip_ew_get_solution(sw4hip_ew* ew, double* u, double* um, int on_device);
+/* Advance nsteps steps.  mode 0: fused sweeps (fast path, corder 1 only);
+ * mode 1: the reference's stage order (Force, evalRHS, evalPredictor,
+ * enforceBC, Force_tt, evalDpDmInTime, evalRHS, evalCorrector,
+ * addSuperGridDamping, enforceBC; EW.C:2527-2842). */
+int sw4hip_ew_step(sw4hip_ew* ew, int nsteps, int mode);
+/* The fused step with per-stage GPU times: hipEvents recorded on the
+ * object's stream; stage_ms[6] receives the summed milliseconds of
+ * (predictor interior kernel, predictor boundary rows + ghost shell,
+ * predictor sources + ghost-point BC, corrector interior kernel, corrector
+ * boundary rows + ghost shell, corrector sources + BC).  Synchronises. */
+int sw4hip_ew_step_timed(sw4hip_ew* ew, int nsteps, double* stage_ms);
+/* Analytic point-source error of the current U (EW::exactSol +
+ * normOfDifference, EW.C:3650-3669, 4526-4610). */
+int sw4hip_ew_point_source_error(sw4hip_ew* ew, double* linf, double* l2, double* solinf);
+/* Device pointers of the state: which = 0 U, 1 Um, 2 rho, 3 mu, 4 lambda. */
+int sw4hip_ew_device_array(sw4hip_ew* ew, int which, double** ptr);
+int sw4hip_ew_sync(sw4hip_ew* ew);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,453 @@
+// abi.cpp -- extern "C" entry points declared in include/sw4hip.h.
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/sw4hip.h"
+#include "ew.h"
+
+namespace sw4amd {
+
+namespace {
+thread_local std::string g_last_error;
+
+struct SbpCache {
+   SbpTables host;
+   SbpTables* dev = nullptr;
+   bool valid = false;
+};
+std::mutex g_sbp_mutex;
+SbpCache g_sbp_cache[64];
+
+// Device copy of an SBP table set on the current device; re-uploaded only
+// when the caller's tables differ from the cached ones.
+int device_sbp(const SbpTables& t, const SbpTables** out)
+{
+   int dev = 0;
+   int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+   if (rc) return rc;
+   if (dev < 0 || dev >= 64) return set_error(E_RANGE, "device ordinal beyond 63");
+   std::lock_guard<std::mutex> lock(g_sbp_mutex);
+   SbpCache& c = g_sbp_cache[dev];
+   if (!c.dev) {
+      void* p = nullptr;
+      if (hipMalloc(&p, sizeof(SbpTables)) != hipSuccess) return set_error(E_NOMEM, "SBP table allocation failed");
+      c.dev = static_cast<SbpTables*>(p);
+   }
+   if (!c.valid || std::memcmp(&c.host, &t, sizeof(t)) != 0) {
+      if ((rc = check_hip(hipMemcpy(c.dev, &t, sizeof(t), hipMemcpyHostToDevice), "SBP table upload"))) return rc;
+      c.host = t;
+      c.valid = true;
+   }
+   *out = c.dev;
+   return OK;
+}
+
+SbpTables default_tables()
+{
+   SbpTables t;
+   fill_stencil_coefficients(t.acof, t.ghcof, t.bope, t.sbop);
+   return t;
+}
+
+bool box_ok(int ib, int ie, int jb, int je, int kb, int ke)
+{
+   return ie - ib >= 4 && je - jb >= 4 && ke - kb >= 4;
+}
+
+GridSetup to_grid(const sw4hip_grid_desc* d)
+{
+   GridSetup g;
+   g.nx = d->nx; g.ny = d->ny; g.nz = d->nz;
+   g.h = d->h;
+   g.zmin = d->zmin;
+   for (int s = 0; s < 6; s++) g.bc[s] = d->bc[s];
+   g.sg_gp_thickness = d->sg_gp_thickness > 0 ? d->sg_gp_thickness : 20;
+   return g;
+}
+
+int check_desc(const sw4hip_grid_desc* d)
+{
+   if (!d) return set_error(E_INVAL, "null grid descriptor");
+   if (d->nx < 12 || d->ny < 12 || d->nz < 12) return set_error(E_INVAL, "grid needs at least 12 points per axis");
+   if (!(d->h > 0)) return set_error(E_INVAL, "grid spacing must be positive");
+   if (d->corder != 0 && d->corder != 1) return set_error(E_INVAL, "corder must be 0 or 1");
+   for (int s = 0; s < 6; s++)
+      if (d->bc[s] < 0 || d->bc[s] > 3) return set_error(E_INVAL, "bc must be one of SW4HIP_BC_*");
+   return OK;
+}
+}  // namespace
+
+int set_error(int code, const std::string& msg)
+{
+   g_last_error = msg;
+   return code;
+}
+
+int check_hip(hipError_t e, const char* what)
+{
+   if (e == hipSuccess) return OK;
+   return set_error(e == hipErrorOutOfMemory ? E_NOMEM : E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace sw4amd
+
+using namespace sw4amd;
+
+struct sw4hip_ew {
+   EW* ew;
+};
+
+extern "C" {
+
+const char* sw4hip_last_error(void) { return g_last_error.c_str(); }
+int sw4hip_version(void) { return 1; }
+
+int sw4hip_get_stencil_coefficients(double* acof, double* ghcof, double* bope, double* sbop)
+{
+   if (!acof || !ghcof || !bope || !sbop) return set_error(E_INVAL, "null coefficient buffer");
+   fill_stencil_coefficients(acof, ghcof, bope, sbop);
+   return OK;
+}
+
+int sw4hip_host_supergrid(const sw4hip_grid_desc* d, double* dcx, double* dcy, double* dcz, double* strx,
+                          double* stry, double* strz, double* cox, double* coy, double* coz)
+{
+   int rc;
+   if ((rc = check_desc(d))) return rc;
+   SupergridArrays a;
+   make_supergrid_arrays(to_grid(d), a);
+   double* dst[9] = {dcx, dcy, dcz, strx, stry, strz, cox, coy, coz};
+   const std::vector<double>* src[9] = {&a.dcx, &a.dcy, &a.dcz, &a.strx, &a.stry, &a.strz, &a.cox, &a.coy, &a.coz};
+   for (int q = 0; q < 9; q++) {
+      if (!dst[q]) return set_error(E_INVAL, "null supergrid buffer");
+      std::memcpy(dst[q], src[q]->data(), sizeof(double) * src[q]->size());
+   }
+   return OK;
+}
+
+int sw4hip_host_boundary_windows(const sw4hip_grid_desc* d, int wind[36], int npts[6])
+{
+   int rc;
+   if ((rc = check_desc(d))) return rc;
+   if (!wind || !npts) return set_error(E_INVAL, "null window buffer");
+   make_boundary_windows(to_grid(d), wind, npts);
+   return OK;
+}
+
+int sw4hip_host_moment_source(const sw4hip_grid_desc* d, double x, double y, double z, const double m[6], int maxn,
+                              int* n, int* ijk, double* force)
+{
+   int rc;
+   if ((rc = check_desc(d))) return rc;
+   if (!m || !n || !ijk || !force) return set_error(E_INVAL, "null source buffer");
+   MomentSource s{x, y, z, {m[0], m[1], m[2], m[3], m[4], m[5]}, 0.0, 1.0, TF_GAUSSIAN};
+   std::vector<GridPointSource> pts;
+   if (moment_source_points(to_grid(d), s, pts)) return set_error(E_INVAL, "grid too small for the source stencil");
+   if ((int)pts.size() > maxn) return set_error(E_RANGE, "more grid-point sources than maxn");
+   *n = (int)pts.size();
+   for (size_t q = 0; q < pts.size(); q++) {
+      ijk[3 * q] = pts[q].i; ijk[3 * q + 1] = pts[q].j; ijk[3 * q + 2] = pts[q].k;
+      for (int c = 0; c < 3; c++) force[3 * q + c] = pts[q].f[c];
+   }
+   return OK;
+}
+
+// ---------------------------------------------------------------------------
+// kernel level
+// ---------------------------------------------------------------------------
+int sw4hip_rhs4sg(int corder, int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast, int nk,
+                  const int* onesided, const double* acof, const double* bope, const double* ghcof, double* lu,
+                  const double* u, const double* mu, const double* lambda, double h, const double* strx,
+                  const double* stry, const double* strz, void* stream)
+{
+   if (corder != 0 && corder != 1) return set_error(E_INVAL, "corder must be 0 or 1");
+   if (!box_ok(ifirst, ilast, jfirst, jlast, kfirst, klast)) return set_error(E_INVAL, "rhs4sg: box too small");
+   if (!onesided || !acof || !bope || !ghcof || !lu || !u || !mu || !lambda || !strx || !stry || !strz)
+      return set_error(E_INVAL, "rhs4sg: null argument");
+   if ((onesided[4] == 1 || onesided[5] == 1) && (nk < 12 || kfirst > -1 || klast < nk + 2))
+      return set_error(E_INVAL, "rhs4sg: one-sided closures need nk >= 12 and ghost points at both ends");
+   if (!(h > 0)) return set_error(E_INVAL, "rhs4sg: h must be positive");
+   SbpTables t = default_tables();
+   std::memcpy(t.acof, acof, sizeof(t.acof));
+   std::memcpy(t.bope, bope, sizeof(t.bope));
+   std::memcpy(t.ghcof, ghcof, sizeof(t.ghcof));
+   const SbpTables* d = nullptr;
+   int rc;
+   if ((rc = device_sbp(t, &d))) return rc;
+   return launch_rhs4sg(corder, make_box(ifirst, ilast, jfirst, jlast, kfirst, klast), nk, onesided, d, lu, u, mu,
+                        lambda, h, strx, stry, strz, (hipStream_t)stream);
+}
+
+int sw4hip_predfort(int corder, int ib, int ie, int jb, int je, int kb, int ke, double* up, const double* u,
+                    const double* um, const double* lu, const double* fo, const double* rho, double dt2, void* stream)
+{
+   if (corder != 0 && corder != 1) return set_error(E_INVAL, "corder must be 0 or 1");
+   if (ie < ib || je < jb || ke < kb || !up || !u || !um || !lu || !fo || !rho) return set_error(E_INVAL, "predfort: bad argument");
+   return launch_pred(corder, make_box(ib, ie, jb, je, kb, ke), up, u, um, lu, fo, rho, dt2, (hipStream_t)stream);
+}
+
+int sw4hip_corrfort(int corder, int ib, int ie, int jb, int je, int kb, int ke, double* up, const double* lu,
+                    const double* fo, const double* rho, double dt4, void* stream)
+{
+   if (corder != 0 && corder != 1) return set_error(E_INVAL, "corder must be 0 or 1");
+   if (ie < ib || je < jb || ke < kb || !up || !lu || !fo || !rho) return set_error(E_INVAL, "corrfort: bad argument");
+   return launch_corr(corder, make_box(ib, ie, jb, je, kb, ke), up, lu, fo, rho, dt4, (hipStream_t)stream);
+}
+
+int sw4hip_dpdmtfort(int ib, int ie, int jb, int je, int kb, int ke, const double* up, const double* u,
+                     const double* um, double* uacc, double dt2i, void* stream)
+{
+   if (ie < ib || je < jb || ke < kb || !up || !u || !um || !uacc) return set_error(E_INVAL, "dpdmtfort: bad argument");
+   return launch_dpdmt(make_box(ib, ie, jb, je, kb, ke), up, u, um, uacc, dt2i, (hipStream_t)stream);
+}
+
+int sw4hip_addsgd4(int corder, int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast, double* up,
+                   const double* u, const double* um, const double* rho, const double* dcx, const double* dcy,
+                   const double* dcz, const double* strx, const double* stry, const double* strz, const double* cox,
+                   const double* coy, const double* coz, double beta, void* stream)
+{
+   if (corder != 0 && corder != 1) return set_error(E_INVAL, "corder must be 0 or 1");
+   if (!box_ok(ifirst, ilast, jfirst, jlast, kfirst, klast)) return set_error(E_INVAL, "addsgd4: box too small");
+   if (!up || !u || !um || !rho || !dcx || !dcy || !dcz || !strx || !stry || !strz || !cox || !coy || !coz)
+      return set_error(E_INVAL, "addsgd4: null argument");
+   SgCoef c{dcx, dcy, dcz, strx, stry, strz, cox, coy, coz};
+   return launch_addsgd4(corder, make_box(ifirst, ilast, jfirst, jlast, kfirst, klast), up, u, um, rho, c, beta,
+                         (hipStream_t)stream);
+}
+
+int sw4hip_bcfortsg(int corder, int ib, int ie, int jb, int je, int kb, int ke, const int* wind, int nx, int ny,
+                    int nz, double* u, double h, const int* bccnd, const double* sbop, const double* mu,
+                    const double* lambda, const double* const* bforce, const double* strx, const double* stry,
+                    void* stream)
+{
+   if (corder != 0 && corder != 1) return set_error(E_INVAL, "corder must be 0 or 1");
+   if (!box_ok(ib, ie, jb, je, kb, ke)) return set_error(E_INVAL, "bcfortsg: box too small");
+   if (!wind || !u || !bccnd || !sbop || !mu || !lambda || !bforce || !strx || !stry)
+      return set_error(E_INVAL, "bcfortsg: null argument");
+   for (int s = 0; s < 6; s++) {
+      const int* w = wind + 6 * s;
+      const bool active = bccnd[s] >= 0 && bccnd[s] <= 3;
+      if (!active || w[1] < w[0] || w[3] < w[2] || w[5] < w[4]) continue;
+      if (w[0] < ib || w[1] > ie || w[2] < jb || w[3] > je || w[4] < kb || w[5] > ke)
+         return set_error(E_INVAL, "bcfortsg: window outside the array");
+      if (bccnd[s] == 3) {
+         const int pi = s == 0 ? nx : s == 1 ? -nx : 0, pj = s == 2 ? ny : s == 3 ? -ny : 0,
+                   pk = s == 4 ? nz : s == 5 ? -nz : 0;
+         if (w[0] + pi < ib || w[1] + pi > ie || w[2] + pj < jb || w[3] + pj > je || w[4] + pk < kb || w[5] + pk > ke)
+            return set_error(E_INVAL, "bcfortsg: periodic source outside the array");
+      }
+      if (bccnd[s] == 0) {
+         if (s != 4 && s != 5) return set_error(E_INVAL, "bcfortsg: free surface only on sides 4 and 5");
+         // top: reads planes 1..4 and writes 0; bottom: reads nz-3..nz, writes nz+1
+         const bool fits = s == 4 ? (kb <= 0 && ke >= 4) : (nz - 3 >= kb && nz + 1 <= ke);
+         if (!fits) return set_error(E_INVAL, "bcfortsg: free-surface plane too close to the array end");
+      }
+   }
+   SbpTables t = default_tables();
+   std::memcpy(t.sbop, sbop, sizeof(t.sbop));
+   const SbpTables* d = nullptr;
+   int rc;
+   if ((rc = device_sbp(t, &d))) return rc;
+   return launch_bcfortsg(corder, make_box(ib, ie, jb, je, kb, ke), wind, nx, ny, nz, u, h, bccnd, d, mu, lambda,
+                          bforce, strx, stry, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// EW level
+// ---------------------------------------------------------------------------
+int sw4hip_ew_create(const sw4hip_grid_desc* d, sw4hip_ew** out)
+{
+   int rc;
+   if (!out) return set_error(E_INVAL, "null output handle");
+   *out = nullptr;
+   if ((rc = check_desc(d))) return rc;
+   int ndev = 0;
+   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+      (void)hipGetLastError();
+      return set_error(E_NODEV, "no HIP device available");
+   }
+   EWOptions o;
+   o.grid = to_grid(d);
+   o.corder = d->corder;
+   o.beta = d->sg_damping >= 0 ? d->sg_damping : 0.02;
+   o.cfl = d->cfl > 0 ? d->cfl : 1.3;
+   EW* ew = new EW(o);
+   if ((rc = ew->init())) {
+      delete ew;
+      return rc;
+   }
+   *out = new sw4hip_ew{ew};
+   return OK;
+}
+
+int sw4hip_ew_destroy(sw4hip_ew* h)
+{
+   if (!h) return OK;
+   delete h->ew;
+   delete h;
+   return OK;
+}
+
+#define EWCHK(h)                                                  \
+   if (!(h) || !(h)->ew) return set_error(E_INVAL, "null EW handle")
+
+int sw4hip_ew_set_stream(sw4hip_ew* h, void* stream)
+{
+   EWCHK(h);
+   h->ew->set_stream((hipStream_t)stream);
+   return OK;
+}
+
+int sw4hip_ew_set_material(sw4hip_ew* h, const double* rho, const double* mu, const double* la, int on_device)
+{
+   EWCHK(h);
+   if (!rho || !mu || !la) return set_error(E_INVAL, "null material array");
+   return h->ew->set_material(rho, mu, la, on_device != 0);
+}
+
+int sw4hip_ew_set_material_const(sw4hip_ew* h, double rho, double cs, double cp)
+{
+   EWCHK(h);
+   if (!(rho > 0) || !(cs > 0) || !(cp > cs)) return set_error(E_INVAL, "need rho > 0 and cp > cs > 0");
+   return h->ew->set_material_const(rho, cs, cp);
+}
+
+int sw4hip_ew_add_moment_source(sw4hip_ew* h, double x, double y, double z, const double m[6], double t0, double freq,
+                                int tfkind)
+{
+   EWCHK(h);
+   if (!m) return set_error(E_INVAL, "null moment tensor");
+   if (!(freq > 0)) return set_error(E_INVAL, "source frequency must be positive");
+   const GridSetup& g = h->ew->m_opt.grid;
+   if (x < 0 || x > (g.nx - 1) * g.h || y < 0 || y > (g.ny - 1) * g.h || z < g.zmin || z > g.zmin + (g.nz - 1) * g.h)
+      return set_error(E_INVAL, "source positioned outside the grid");
+   MomentSource s{x, y, z, {m[0], m[1], m[2], m[3], m[4], m[5]}, t0, freq, tfkind};
+   return h->ew->add_moment_source(s);
+}
+
+int sw4hip_ew_setup(sw4hip_ew* h, double tmax, int nsteps)
+{
+   EWCHK(h);
+   if (!(tmax > 0) && nsteps <= 0) return set_error(E_INVAL, "give tmax > 0 or nsteps > 0");
+   return h->ew->setup(tmax, nsteps);
+}
+
+int sw4hip_ew_get_info(sw4hip_ew* h, sw4hip_ew_info* info)
+{
+   EWCHK(h);
+   if (!info) return set_error(E_INVAL, "null info");
+   EW& e = *h->ew;
+   info->dt = e.mDt;
+   info->nsteps = e.mNumberOfTimeSteps;
+   info->t = e.mT;
+   info->cycle = e.mCycle;
+   for (int s = 0; s < 6; s++) info->onesided[s] = e.m_onesided[s];
+   for (int q = 0; q < 36; q++) info->wind[q] = e.m_wind[q];
+   info->npts = e.m_box.npts;
+   info->nsources = (int)e.m_gps.size();
+   info->nunique = e.m_first.empty() ? 0 : (int)e.m_first.size() - 1;
+   return OK;
+}
+
+int sw4hip_ew_get_supergrid(sw4hip_ew* h, double* dcx, double* dcy, double* dcz, double* strx, double* stry,
+                            double* strz, double* cox, double* coy, double* coz)
+{
+   EWCHK(h);
+   const SupergridArrays& a = h->ew->m_sg_host;
+   if (a.dcx.empty()) return set_error(E_STATE, "supergrid arrays exist after setup");
+   double* dst[9] = {dcx, dcy, dcz, strx, stry, strz, cox, coy, coz};
+   const std::vector<double>* src[9] = {&a.dcx, &a.dcy, &a.dcz, &a.strx, &a.stry, &a.strz, &a.cox, &a.coy, &a.coz};
+   for (int q = 0; q < 9; q++) {
+      if (!dst[q]) return set_error(E_INVAL, "null supergrid buffer");
+      std::memcpy(dst[q], src[q]->data(), sizeof(double) * src[q]->size());
+   }
+   return OK;
+}
+
+int sw4hip_ew_get_sources(sw4hip_ew* h, int* ijk, double* force, double* freq, double* t0, int* tfkind, int* first)
+{
+   EWCHK(h);
+   const EW& e = *h->ew;
+   for (size_t s = 0; s < e.m_gps.size(); s++) {
+      if (ijk) { ijk[3 * s] = e.m_gps[s].i; ijk[3 * s + 1] = e.m_gps[s].j; ijk[3 * s + 2] = e.m_gps[s].k; }
+      if (force) for (int c = 0; c < 3; c++) force[3 * s + c] = e.m_gps[s].f[c];
+      if (freq) freq[s] = e.m_gps[s].freq;
+      if (t0) t0[s] = e.m_gps[s].t0;
+      if (tfkind) tfkind[s] = e.m_gps[s].tfkind;
+   }
+   if (first) for (size_t r = 0; r < e.m_first.size(); r++) first[r] = e.m_first[r];
+   return OK;
+}
+
+int sw4hip_ew_set_solution(sw4hip_ew* h, const double* u, const double* um, int on_device)
+{
+   EWCHK(h);
+   EW& e = *h->ew;
+   const size_t bytes = sizeof(double) * e.mU.size();
+   const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+   int rc;
+   if (u && (rc = check_hip(hipMemcpyAsync(e.mU.dev, u, bytes, kind, e.m_stream), "U copy"))) return rc;
+   if (um && (rc = check_hip(hipMemcpyAsync(e.mUm.dev, um, bytes, kind, e.m_stream), "Um copy"))) return rc;
+   return check_hip(hipStreamSynchronize(e.m_stream), "solution copy");
+}
+
+int sw4hip_ew_get_solution(sw4hip_ew* h, double* u, double* um, int on_device)
+{
+   EWCHK(h);
+   EW& e = *h->ew;
+   const size_t bytes = sizeof(double) * e.mU.size();
+   const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+   int rc;
+   if (u && (rc = check_hip(hipMemcpyAsync(u, e.mU.dev, bytes, kind, e.m_stream), "U copy"))) return rc;
+   if (um && (rc = check_hip(hipMemcpyAsync(um, e.mUm.dev, bytes, kind, e.m_stream), "Um copy"))) return rc;
+   return check_hip(hipStreamSynchronize(e.m_stream), "solution copy");
+}
+
+int sw4hip_ew_step(sw4hip_ew* h, int nsteps, int mode)
+{
+   EWCHK(h);
+   if (nsteps < 0) return set_error(E_INVAL, "negative step count");
+   if (mode == 0) return h->ew->step_fused(nsteps);
+   if (mode == 1) return h->ew->step_reference(nsteps);
+   return set_error(E_INVAL, "step mode must be 0 (fused) or 1 (reference order)");
+}
+
+int sw4hip_ew_step_timed(sw4hip_ew* h, int nsteps, double* stage_ms)
+{
+   EWCHK(h);
+   if (nsteps < 0) return set_error(E_INVAL, "negative step count");
+   if (!stage_ms) return set_error(E_INVAL, "null stage_ms");
+   return h->ew->step_fused(nsteps, stage_ms);
+}
+
+int sw4hip_ew_point_source_error(sw4hip_ew* h, double* linf, double* l2, double* solinf)
+{
+   EWCHK(h);
+   if (!linf || !l2 || !solinf) return set_error(E_INVAL, "null output");
+   return h->ew->point_source_error(*linf, *l2, *solinf);
+}
+
+int sw4hip_ew_device_array(sw4hip_ew* h, int which, double** ptr)
+{
+   EWCHK(h);
+   if (!ptr) return set_error(E_INVAL, "null output");
+   EW& e = *h->ew;
+   switch (which) {
+   case 0: *ptr = e.mU.dev; break;
+   case 1: *ptr = e.mUm.dev; break;
+   case 2: *ptr = e.mRho.dev; break;
+   case 3: *ptr = e.mMu.dev; break;
+   case 4: *ptr = e.mLambda.dev; break;
+   default: return set_error(E_INVAL, "which must be 0..4");
+   }
+   return OK;
+}
+
+int sw4hip_ew_sync(sw4hip_ew* h)
+{
+   EWCHK(h);
+   return check_hip(hipStreamSynchronize(h->ew->m_stream), "stream synchronize");
+}
+
+}  // extern "C"

@@ -1,0 +1,415 @@
+// ew.cpp -- the EW facade: set-up of one Cartesian grid and the time-step
+// loop on the GPU.  Stage order and formulas follow EW::timesteploop
+// (EW.C:2339-2928) and the EW step methods (EW.C:3085-3636).
+#include "ew.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace sw4amd {
+
+int launch_fill(double* a, long n, double v, hipStream_t s);
+
+EW::EW(const EWOptions& opt) : m_opt(opt)
+{
+   const GridSetup& g = opt.grid;
+   m_box = make_box(-1, g.nx + 2, -1, g.ny + 2, -1, g.nz + 2);
+   for (int q = 0; q < 36; q++) m_wind[q] = 0;
+   for (int q = 0; q < 6; q++) m_nbcpts[q] = 0;
+}
+
+EW::~EW()
+{
+   for (void* p : m_allocs) (void)hipFree(p);
+}
+
+int EW::alloc(Sarray& a, int nc)
+{
+   a.m_nc = nc;
+   a.m_ib = m_box.ib; a.m_ie = m_box.ie;
+   a.m_jb = m_box.jb; a.m_je = m_box.je;
+   a.m_kb = m_box.kb; a.m_ke = m_box.ke;
+   a.m_corder = m_opt.corder != 0;
+   void* p = nullptr;
+   if (hipMalloc(&p, sizeof(double) * a.size()) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(E_NOMEM, "EW: device allocation of " + std::to_string(sizeof(double) * a.size()) + " bytes failed");
+   }
+   m_allocs.push_back(p);
+   a.dev = static_cast<double*>(p);
+   return launch_fill(a.dev, a.size(), 0.0, m_stream);
+}
+
+int EW::init()
+{
+   int rc;
+   const GridSetup& g = m_opt.grid;
+   if (g.nx < 12 || g.ny < 12 || g.nz < 12 || !(g.h > 0))
+      return set_error(E_INVAL, "EW: need nx,ny,nz >= 12 and h > 0");
+   if ((rc = alloc(mU, 3)) || (rc = alloc(mUm, 3)) || (rc = alloc(mUp, 3)) || (rc = alloc(mUp2, 3)))
+      return rc;
+   if ((rc = alloc(mRho, 1)) || (rc = alloc(mMu, 1)) || (rc = alloc(mLambda, 1))) return rc;
+   void* p = nullptr;
+   if (hipMalloc(&p, sizeof(SbpTables)) != hipSuccess) return set_error(E_NOMEM, "EW: SBP table allocation failed");
+   m_allocs.push_back(p);
+   m_dsbp = static_cast<SbpTables*>(p);
+   SbpTables t;
+   fill_stencil_coefficients(t.acof, t.ghcof, t.bope, t.sbop);
+   if ((rc = check_hip(hipMemcpy(m_dsbp, &t, sizeof(t), hipMemcpyHostToDevice), "SBP upload"))) return rc;
+   const long nsg = 3 * (m_box.ni + (m_box.je - m_box.jb + 1) + (m_box.ke - m_box.kb + 1));
+   if (hipMalloc(&p, sizeof(double) * nsg) != hipSuccess) return set_error(E_NOMEM, "EW: supergrid allocation failed");
+   m_allocs.push_back(p);
+   m_dsg = static_cast<double*>(p);
+   if (hipMalloc(&p, sizeof(double) * 3 * m_nred + 64) != hipSuccess) return set_error(E_NOMEM, "EW: reduction buffer allocation failed");
+   m_allocs.push_back(p);
+   m_dred = static_cast<double*>(p);
+   return OK;
+}
+
+int EW::set_material(const double* rho, const double* mu, const double* la, bool on_device)
+{
+   const size_t bytes = sizeof(double) * m_box.npts;
+   const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+   int rc;
+   if ((rc = check_hip(hipMemcpyAsync(mRho.dev, rho, bytes, kind, m_stream), "rho copy")) ||
+       (rc = check_hip(hipMemcpyAsync(mMu.dev, mu, bytes, kind, m_stream), "mu copy")) ||
+       (rc = check_hip(hipMemcpyAsync(mLambda.dev, la, bytes, kind, m_stream), "lambda copy")))
+      return rc;
+   if (!on_device && (rc = check_hip(hipStreamSynchronize(m_stream), "material copy"))) return rc;
+   m_material_set = true;
+   return OK;
+}
+
+// EW::processTestPointSource (EW.C:893-950): mu = rho cs^2, la = rho cp^2 - 2 mu
+int EW::set_material_const(double rho, double cs, double cp)
+{
+   const double mu = rho * cs * cs, la = rho * cp * cp - 2 * mu;
+   int rc;
+   if ((rc = launch_fill(mRho.dev, m_box.npts, rho, m_stream)) || (rc = launch_fill(mMu.dev, m_box.npts, mu, m_stream)) ||
+       (rc = launch_fill(mLambda.dev, m_box.npts, la, m_stream)))
+      return rc;
+   m_material_set = true;
+   return OK;
+}
+
+int EW::add_moment_source(const MomentSource& s)
+{
+   if (m_setup_done) return set_error(E_STATE, "EW: sources must be added before setup");
+   if (s.tfkind < TF_RICKER || s.tfkind > TF_C6SMOOTHBUMP) return set_error(E_INVAL, "EW: unknown time function");
+   m_sources.push_back(s);
+   return OK;
+}
+
+SgCoef EW::sg() const
+{
+   const long ni = m_box.ni, nj = m_box.je - m_box.jb + 1, nk = m_box.ke - m_box.kb + 1;
+   const double* d = m_dsg;
+   SgCoef c;
+   c.dcx = d; c.strx = d + ni; c.cox = d + 2 * ni;
+   d += 3 * ni;
+   c.dcy = d; c.stry = d + nj; c.coy = d + 2 * nj;
+   d += 3 * nj;
+   c.dcz = d; c.strz = d + nk; c.coz = d + 2 * nk;
+   return c;
+}
+
+int EW::upload_sources()
+{
+   // packed: first[nu+1] | ijk[3n] | tfkind[n] | force[3n] | freq[n] | t0[n]
+   const int n = (int)m_gps.size(), nu = (int)m_first.size() - 1;
+   const size_t ibytes = sizeof(int) * (nu + 1 + 4 * n);
+   const size_t ioff = (ibytes + 15) / 16 * 16;
+   const size_t total = ioff + sizeof(double) * 5 * n + 16;
+   std::vector<char> host(total, 0);
+   int* ip = reinterpret_cast<int*>(host.data());
+   std::memcpy(ip, m_first.data(), sizeof(int) * (nu + 1));
+   int* ijk = ip + nu + 1;
+   int* tf = ijk + 3 * n;
+   double* dp = reinterpret_cast<double*>(host.data() + ioff);
+   double *force = dp, *freq = dp + 3 * n, *t0 = dp + 4 * n;
+   for (int s = 0; s < n; s++) {
+      ijk[3 * s] = m_gps[s].i; ijk[3 * s + 1] = m_gps[s].j; ijk[3 * s + 2] = m_gps[s].k;
+      tf[s] = m_gps[s].tfkind;
+      for (int c = 0; c < 3; c++) force[3 * s + c] = m_gps[s].f[c];
+      freq[s] = m_gps[s].freq;
+      t0[s] = m_gps[s].t0;
+   }
+   void* p = nullptr;
+   if (hipMalloc(&p, total) != hipSuccess) return set_error(E_NOMEM, "EW: source table allocation failed");
+   m_allocs.push_back(p);
+   m_dsrc = p;
+   return check_hip(hipMemcpy(p, host.data(), total, hipMemcpyHostToDevice), "source upload");
+}
+
+SourceTable EW::sources() const
+{
+   SourceTable st;
+   const int n = (int)m_gps.size(), nu = (int)m_first.size() - 1;
+   st.nunique = m_dsrc ? nu : 0;
+   const size_t ibytes = sizeof(int) * (nu + 1 + 4 * n);
+   const size_t ioff = (ibytes + 15) / 16 * 16;
+   const int* ip = static_cast<const int*>(m_dsrc);
+   const double* dp = reinterpret_cast<const double*>(static_cast<const char*>(m_dsrc) + ioff);
+   st.first = ip;
+   st.ijk = ip + nu + 1;
+   st.tfkind = ip + nu + 1 + 3 * n;
+   st.force = dp;
+   st.freq = dp + 3 * n;
+   st.t0 = dp + 4 * n;
+   return st;
+}
+
+int EW::setup(double tmax, int nsteps)
+{
+   if (!m_material_set) return set_error(E_STATE, "EW: set the material before setup");
+   if (m_setup_done) return set_error(E_STATE, "EW: setup called twice");
+   const GridSetup& g = m_opt.grid;
+   int rc;
+   // one-sided SBP closures where the top/bottom is a free surface
+   // (EW::assign_local_bcs, EW.C:4863-4867)
+   for (int s = 0; s < 6; s++) m_onesided[s] = 0;
+   m_onesided[4] = g.bc[4] == BC_FREE;
+   m_onesided[5] = g.bc[5] == BC_FREE;
+   for (int s = 0; s < 4; s++)
+      if (g.bc[s] == BC_FREE) return set_error(E_INVAL, "EW: free surface is only supported on sides 4 and 5");
+   // supergrid arrays
+   make_supergrid_arrays(g, m_sg_host);
+   {
+      std::vector<double> all;
+      for (auto* v : {&m_sg_host.dcx, &m_sg_host.strx, &m_sg_host.cox, &m_sg_host.dcy, &m_sg_host.stry,
+                      &m_sg_host.coy, &m_sg_host.dcz, &m_sg_host.strz, &m_sg_host.coz})
+         all.insert(all.end(), v->begin(), v->end());
+      if ((rc = check_hip(hipMemcpy(m_dsg, all.data(), sizeof(double) * all.size(), hipMemcpyHostToDevice), "supergrid upload")))
+         return rc;
+   }
+   make_boundary_windows(g, m_wind, m_nbcpts);
+   // EW::computeDT (EW.C:5041-5146), Cartesian grid
+   if ((rc = launch_maxeig(m_box.npts, mMu.dev, mLambda.dev, mRho.dev, m_dred, m_nred, m_dred + 3 * m_nred, m_stream)))
+      return rc;
+   double eigmax = 0;
+   if ((rc = check_hip(hipMemcpyAsync(&eigmax, m_dred + 3 * m_nred, sizeof(double), hipMemcpyDeviceToHost, m_stream), "dt readback")) ||
+       (rc = check_hip(hipStreamSynchronize(m_stream), "dt readback")))
+      return rc;
+   if (!(eigmax > 0)) return set_error(E_INVAL, "EW: material gives a non-positive (4mu+lambda)/rho");
+   const double ieigmax = 1 / std::sqrt(eigmax);
+   mDt = std::min(1e10, m_opt.cfl * g.h * ieigmax);
+   if (tmax > 0) {
+      mNumberOfTimeSteps = static_cast<int>(tmax / mDt + 0.5);
+      if (mNumberOfTimeSteps == 0) mNumberOfTimeSteps = 1;
+      mDt = tmax / mNumberOfTimeSteps;
+   } else {
+      mNumberOfTimeSteps = nsteps;
+   }
+   // grid-point sources
+   m_gps.clear();
+   for (const MomentSource& s : m_sources)
+      if (moment_source_points(g, s, m_gps)) return set_error(E_INVAL, "EW: grid too small for the source stencil");
+   sort_grid_point_sources(g, m_gps, m_first);
+   if (!m_gps.empty() && (rc = upload_sources())) return rc;
+   mT = 0;
+   mCycle = 0;
+   m_setup_done = true;
+   return OK;
+}
+
+// ---------------------------------------------------------------------------
+// reference step methods
+// ---------------------------------------------------------------------------
+int EW::Force(double t, Sarray& F, bool tt)
+{
+   return launch_force(m_opt.corder, m_box, F.dev, sources(), t, tt ? 1 : 0, true, m_stream);
+}
+
+int EW::evalRHS(Sarray& U, Sarray& Mu, Sarray& Lambda, Sarray& Lu)
+{
+   const SgCoef c = sg();
+   return launch_rhs4sg(m_opt.corder, m_box, m_opt.grid.nz, m_onesided, m_dsbp, Lu.dev, U.dev, Mu.dev,
+                        Lambda.dev, m_opt.grid.h, c.strx, c.stry, c.strz, m_stream);
+}
+
+int EW::evalPredictor(Sarray& Up, Sarray& U, Sarray& Um, Sarray& Rho, Sarray& Lu, Sarray& F)
+{
+   return launch_pred(m_opt.corder, m_box, Up.dev, U.dev, Um.dev, Lu.dev, F.dev, Rho.dev, mDt * mDt, m_stream);
+}
+
+int EW::evalCorrector(Sarray& Up, Sarray& Rho, Sarray& Lu, Sarray& F)
+{
+   return launch_corr(m_opt.corder, m_box, Up.dev, Lu.dev, F.dev, Rho.dev, mDt * mDt * mDt * mDt, m_stream);
+}
+
+int EW::evalDpDmInTime(Sarray& Up, Sarray& U, Sarray& Um, Sarray& Uacc)
+{
+   return launch_dpdmt(m_box, Up.dev, U.dev, Um.dev, Uacc.dev, 1. / (mDt * mDt), m_stream);
+}
+
+int EW::addSuperGridDamping(Sarray& Up, Sarray& U, Sarray& Um, Sarray& Rho)
+{
+   return launch_addsgd4(m_opt.corder, m_box, Up.dev, U.dev, Um.dev, Rho.dev, sg(), m_opt.beta, m_stream);
+}
+
+// EW::enforceBC with cartesian_bc_forcing giving zero boundary data
+// (EW.C:3320-3344, 3446-3501)
+int EW::enforceBC(Sarray& U, Sarray& Mu, Sarray& Lambda, double t)
+{
+   (void)t;
+   const GridSetup& g = m_opt.grid;
+   const double* bforce[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+   const SgCoef c = sg();
+   return launch_bcfortsg(m_opt.corder, m_box, m_wind, g.nx, g.ny, g.nz, U.dev, g.h, g.bc, m_dsbp, Mu.dev,
+                          Lambda.dev, bforce, c.strx, c.stry, m_stream);
+}
+
+// EW::cycleSolutionArrays (EW.C:3060-3082): pointer rotation, no copy
+void EW::cycleSolutionArrays()
+{
+   double* tmp = mUm.dev;
+   mUm.dev = mU.dev;
+   mU.dev = mUp.dev;
+   mUp.dev = tmp;
+}
+
+int EW::ensure_reference_arrays()
+{
+   int rc;
+   if (!mLu.dev && (rc = alloc(mLu, 3))) return rc;
+   if (!mF.dev && (rc = alloc(mF, 3))) return rc;
+   if (!mUacc.dev && (rc = alloc(mUacc, 3))) return rc;
+   return OK;
+}
+
+int EW::step_reference(int n)
+{
+   if (!m_setup_done) return set_error(E_STATE, "EW: step before setup");
+   int rc;
+   if ((rc = ensure_reference_arrays())) return rc;
+   for (int s = 0; s < n; s++) {
+      const double t = mT;
+      if ((rc = Force(t, mF, false)) || (rc = evalRHS(mU, mMu, mLambda, mLu)) ||
+          (rc = evalPredictor(mUp, mU, mUm, mRho, mLu, mF)) || (rc = enforceBC(mUp, mMu, mLambda, t + mDt)) ||
+          (rc = Force(t, mF, true)) || (rc = evalDpDmInTime(mUp, mU, mUm, mUacc)) ||
+          (rc = evalRHS(mUacc, mMu, mLambda, mLu)) || (rc = evalCorrector(mUp, mRho, mLu, mF)) ||
+          (rc = addSuperGridDamping(mUp, mU, mUm, mRho)) || (rc = enforceBC(mUp, mMu, mLambda, t + mDt)))
+         return rc;
+      mT += mDt;
+      mCycle++;
+      cycleSolutionArrays();
+   }
+   return OK;
+}
+
+int EW::step_fused(int n, double* stage_ms)
+{
+   if (!m_setup_done) return set_error(E_STATE, "EW: step before setup");
+   if (!m_opt.corder) return set_error(E_INVAL, "EW: the fused step needs the component-blocked layout (corder=1)");
+   const GridSetup& g = m_opt.grid;
+   const SgCoef c = sg();
+   const SourceTable st = sources();
+   int rc = OK;
+   // optional per-stage timing: one event before the step and after each stage
+   std::vector<hipEvent_t> ev;
+   if (stage_ms) {
+      ev.resize((size_t)n * (kStages + 1));
+      for (auto& e : ev)
+         if ((rc = check_hip(hipEventCreate(&e), "hipEventCreate"))) return rc;
+   }
+   auto mark = [&](int s, int q) {
+      return stage_ms ? check_hip(hipEventRecord(ev[(size_t)s * (kStages + 1) + q], m_stream), "hipEventRecord") : OK;
+   };
+   for (int s = 0; s < n && rc == OK; s++) {
+      const double t = mT, dt = mDt;
+      if ((rc = mark(s, 0)) ||
+          (rc = launch_pred_fused(m_box, g.nz, m_onesided, m_dsbp, mU.dev, mUm.dev, mMu.dev, mLambda.dev, mRho.dev,
+                                  c, mUp.dev, g.h, dt, m_stream, [&]() { return mark(s, 1); })) ||
+          (rc = mark(s, 2)) || (rc = launch_src_apply(m_box, st, t, 0, dt * dt, mRho.dev, mUp.dev, m_stream)) ||
+          (rc = enforceBC(mUp, mMu, mLambda, t + dt)) || (rc = mark(s, 3)) ||
+          (rc = launch_corr_fused(m_box, g.nz, m_onesided, m_dsbp, mUp.dev, mU.dev, mUm.dev, mMu.dev, mLambda.dev,
+                                  mRho.dev, c, m_opt.beta, mUp2.dev, g.h, dt, m_stream, [&]() { return mark(s, 4); })) ||
+          (rc = mark(s, 5)) ||
+          (rc = launch_src_apply(m_box, st, t, 1, dt * dt * dt * dt / 12, mRho.dev, mUp2.dev, m_stream)) ||
+          (rc = enforceBC(mUp2, mMu, mLambda, t + dt)) || (rc = mark(s, 6)))
+         break;
+      mT += dt;
+      mCycle++;
+      // Um <- U, U <- corrected, the predictor buffer and old Um are scratch
+      double* old_um = mUm.dev;
+      mUm.dev = mU.dev;
+      mU.dev = mUp2.dev;
+      mUp2.dev = old_um;
+   }
+   if (stage_ms) {
+      if (rc == OK) rc = check_hip(hipStreamSynchronize(m_stream), "timed step synchronize");
+      for (int q = 0; q < kStages; q++) stage_ms[q] = 0;
+      for (int s = 0; s < n && rc == OK; s++)
+         for (int q = 0; q < kStages; q++) {
+            float ms = 0;
+            rc = check_hip(hipEventElapsedTime(&ms, ev[(size_t)s * (kStages + 1) + q], ev[(size_t)s * (kStages + 1) + q + 1]),
+                           "hipEventElapsedTime");
+            if (rc) break;
+            stage_ms[q] += ms;
+         }
+      for (auto& e : ev) (void)hipEventDestroy(e);
+   }
+   return rc;
+}
+
+// EW::exactSol + normOfDifference for the first source (EW.C:3650-3669,
+// 3899-4523, 4526-4610), written to the scratch array mUp.
+int EW::point_source_error(double& linf, double& l2, double& solinf)
+{
+   if (!m_setup_done || m_sources.empty()) return set_error(E_STATE, "EW: point-source error needs setup and a source");
+   const GridSetup& g = m_opt.grid;
+   const MomentSource& s0 = m_sources[0];
+   if (s0.tfkind != TF_GAUSSIAN && s0.tfkind != TF_C6SMOOTHBUMP)
+      return set_error(E_INVAL, "EW: analytic solution needs a Gaussian or C6SmoothBump source");
+   int rc;
+   // material at the grid midpoint
+   const int im = (m_box.ib + m_box.ie) / 2, jm = (m_box.jb + m_box.je) / 2, km = (m_box.kb + m_box.ke) / 2;
+   const long pm = sidx(m_box, im, jm, km);
+   double rmu[3];
+   if ((rc = check_hip(hipMemcpyAsync(&rmu[0], mRho.dev + pm, sizeof(double), hipMemcpyDeviceToHost, m_stream), "readback")) ||
+       (rc = check_hip(hipMemcpyAsync(&rmu[1], mMu.dev + pm, sizeof(double), hipMemcpyDeviceToHost, m_stream), "readback")) ||
+       (rc = check_hip(hipMemcpyAsync(&rmu[2], mLambda.dev + pm, sizeof(double), hipMemcpyDeviceToHost, m_stream), "readback")) ||
+       (rc = check_hip(hipStreamSynchronize(m_stream), "readback")))
+      return rc;
+   ExactParams P;
+   P.x0 = s0.x; P.y0 = s0.y; P.z0 = s0.z;
+   P.rho = rmu[0];
+   P.beta = std::sqrt(rmu[1] / P.rho);
+   P.alpha = std::sqrt((2 * rmu[1] + rmu[2]) / P.rho);
+   P.fr = s0.freq;
+   P.time = (mT - s0.t0) * s0.freq;
+   if (s0.tfkind == TF_GAUSSIAN) {
+      P.fr = 1 / P.fr;
+      P.time = P.time * P.fr;
+   }
+   for (int q = 0; q < 6; q++) P.m[q] = s0.m[q];
+   P.tfkind = s0.tfkind;
+   P.h = g.h;
+   P.zmin = g.zmin;
+   P.eps = 1e-3 * g.h;
+   if ((rc = launch_exact(m_box, P, mUp.dev, m_stream))) return rc;
+   // window without the supergrid layers (EW.C:4537-4570)
+   const int nsg = (int)(0.5 + g.sg_gp_thickness * g.h / g.h);
+   NormWindow w;
+   w.imin = g.bc[0] == BC_SUPERGRID ? std::max(1, nsg + 1) : 1;
+   w.imax = g.bc[1] == BC_SUPERGRID ? std::min(g.nx, g.nx - nsg) : g.nx;
+   w.jmin = g.bc[2] == BC_SUPERGRID ? std::max(1, nsg + 1) : 1;
+   w.jmax = g.bc[3] == BC_SUPERGRID ? std::min(g.ny, g.ny - nsg) : g.ny;
+   w.kmin = g.bc[4] == BC_SUPERGRID ? std::max(1, nsg + 1) : 1;
+   w.kmax = g.bc[5] == BC_SUPERGRID ? std::min(g.nz, g.nz - nsg) : g.nz;
+   w.x0 = s0.x; w.y0 = s0.y; w.z0 = s0.z;
+   w.r2 = (4 * g.h) * (4 * g.h);
+   w.h = g.h;
+   w.zmin = g.zmin;
+   if ((rc = launch_solerr(m_box, w, mUp.dev, mU.dev, m_dred, m_nred, m_dred + 3 * m_nred, m_stream))) return rc;
+   double out[3];
+   if ((rc = check_hip(hipMemcpyAsync(out, m_dred + 3 * m_nred, sizeof(out), hipMemcpyDeviceToHost, m_stream), "norm readback")) ||
+       (rc = check_hip(hipStreamSynchronize(m_stream), "norm readback")))
+      return rc;
+   linf = out[0];
+   l2 = std::sqrt(out[1]);
+   solinf = out[2];
+   return OK;
+}
+
+}  // namespace sw4amd

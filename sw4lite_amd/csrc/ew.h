@@ -1,0 +1,105 @@
+// ew.h -- C++ host facade with the reference's EW time-step methods, backed
+// by device-resident Sarray data on one MI355X.  The method names and their
+// argument meaning follow EW.h:93-127 so the reference time loop can be kept
+// as is; the bodies launch gfx950 kernels instead of OpenMP loops.
+#pragma once
+#include <vector>
+
+#include "kernels_common.h"
+#include "kernels_step.h"
+#include "setup.h"
+
+namespace sw4amd {
+
+// Sarray with the reference's index arithmetic (Sarray.h:54-187) whose data
+// lives in device memory.  nc = 1 (material) or 3 (displacement).
+struct Sarray {
+   int m_nc = 0, m_ib = 0, m_ie = -1, m_jb = 0, m_je = -1, m_kb = 0, m_ke = -1;
+   bool m_corder = true;
+   double* dev = nullptr;  // owned by EW
+   long npts() const { return (long)(m_ie - m_ib + 1) * (m_je - m_jb + 1) * (m_ke - m_kb + 1); }
+   long size() const { return npts() * m_nc; }
+   // offset of (c,i,j,k), c = 1..m_nc as in the reference
+   long index(int c, int i, int j, int k) const
+   {
+      const long ni = m_ie - m_ib + 1, nij = ni * (m_je - m_jb + 1);
+      const long p = (i - m_ib) + ni * (j - m_jb) + nij * (k - m_kb);
+      return m_corder ? p + npts() * (c - 1) : (long)m_nc * p + (c - 1);
+   }
+};
+
+struct EWOptions {
+   GridSetup grid;
+   int corder = 1;
+   double beta = 0.02;  // m_supergrid_damping_coefficient (EW.C:174)
+   double cfl = 1.3;    // mCFL (EW.C:163)
+};
+
+class EW {
+public:
+   explicit EW(const EWOptions& opt);
+   ~EW();
+   EW(const EW&) = delete;
+   EW& operator=(const EW&) = delete;
+
+   int init();  // device allocation; returns status
+   void set_stream(hipStream_t s) { m_stream = s; }
+
+   int set_material(const double* rho, const double* mu, const double* la, bool on_device);
+   int set_material_const(double rho, double cs, double cp);
+   int add_moment_source(const MomentSource& s);
+   int setup(double tmax, int nsteps);  // setupRun + computeDT
+
+   // --- reference step methods (EW.h:93-127), single Cartesian grid ---
+   int Force(double t, Sarray& F, bool tt);
+   int evalRHS(Sarray& U, Sarray& Mu, Sarray& Lambda, Sarray& Lu);
+   int evalPredictor(Sarray& Up, Sarray& U, Sarray& Um, Sarray& Rho, Sarray& Lu, Sarray& F);
+   int evalCorrector(Sarray& Up, Sarray& Rho, Sarray& Lu, Sarray& F);
+   int evalDpDmInTime(Sarray& Up, Sarray& U, Sarray& Um, Sarray& Uacc);
+   int addSuperGridDamping(Sarray& Up, Sarray& U, Sarray& Um, Sarray& Rho);
+   int enforceBC(Sarray& U, Sarray& Mu, Sarray& Lambda, double t);
+   void cycleSolutionArrays();
+
+   // --- time loop ---
+   int step_reference(int n);  // stage order of EW::timesteploop (EW.C:2527-2842)
+   // two fused sweeps per step; with stage_ms, kStages accumulated stage
+   // times (ms): predictor interior, predictor boundary rows + ghost shell,
+   // predictor sources + BC, corrector interior, corrector boundary rows +
+   // ghost shell, corrector sources + BC
+   static constexpr int kStages = 6;
+   int step_fused(int n, double* stage_ms = nullptr);
+   int point_source_error(double& linf, double& l2, double& solinf);
+
+   // state access
+   Sarray mU, mUm, mUp, mUp2, mRho, mMu, mLambda;
+   Sarray mLu, mF, mUacc;  // only for step_reference
+   double mDt = 0, mT = 0;
+   int mNumberOfTimeSteps = 0, mCycle = 0;
+   int m_onesided[6] = {0, 0, 0, 0, 0, 0};
+   int m_wind[36];
+   int m_nbcpts[6];
+   Box m_box;
+   EWOptions m_opt;
+   SupergridArrays m_sg_host;
+   std::vector<MomentSource> m_sources;
+   std::vector<GridPointSource> m_gps;
+   std::vector<int> m_first;
+   hipStream_t m_stream = nullptr;
+
+private:
+   int alloc(Sarray& a, int nc);
+   int upload_sources();
+   int ensure_reference_arrays();
+   SgCoef sg() const;
+   SourceTable sources() const;
+
+   double* m_dsg = nullptr;  // 9 supergrid arrays back to back
+   SbpTables* m_dsbp = nullptr;
+   void* m_dsrc = nullptr;   // packed source table
+   double* m_dred = nullptr; // reduction partials
+   int m_nred = 1024;
+   bool m_setup_done = false, m_material_set = false;
+   std::vector<void*> m_allocs;
+};
+
+}  // namespace sw4amd

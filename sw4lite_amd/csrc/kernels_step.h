@@ -1,0 +1,40 @@
+// kernels_step.h -- launchers of the fused step and helper kernels
+// (kernels_step.hip).
+#pragma once
+#include <functional>
+
+#include "kernels_common.h"
+
+namespace sw4amd {
+
+struct ExactParams {
+   double x0, y0, z0, rho, alpha, beta, time, fr;
+   double m[6];  // xx xy xz yy yz zz
+   int tfkind;   // 1 Gaussian, 2 C6SmoothBump
+   double h, zmin, eps;
+};
+
+struct NormWindow {
+   int imin, imax, jmin, jmax, kmin, kmax;
+   double x0, y0, z0, r2, h, zmin;
+};
+
+int launch_pred_fused(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp,
+                      const double* u, const double* um, const double* mu, const double* la,
+                      const double* rho, const SgCoef& sg, double* up, double h, double dt,
+                      hipStream_t s, const std::function<int()>& after_interior);
+int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp,
+                      const double* up, const double* u, const double* um, const double* mu,
+                      const double* la, const double* rho, const SgCoef& sg, double beta,
+                      double* upo, double h, double dt, hipStream_t s,
+                      const std::function<int()>& after_interior);
+int launch_src_apply(const Box& b, const SourceTable& st, double t, int deriv2, double coef,
+                     const double* rho, double* up, hipStream_t s);
+int launch_maxeig(long n, const double* mu, const double* la, const double* rho, double* partial,
+                  int npartial, double* out, hipStream_t s);
+int launch_exact(const Box& b, const ExactParams& P, double* out, hipStream_t s);
+int launch_solerr(const Box& b, const NormWindow& w, const double* uex, const double* u,
+                  double* partial, int npartial, double* out, hipStream_t s);
+long shell_count(const Box& b);
+
+}  // namespace sw4amd

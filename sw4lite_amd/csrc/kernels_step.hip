@@ -1,0 +1,515 @@
+// kernels_step.hip -- kernels of the fused time step and its helpers:
+//   * predictor sweep:  Up  = 2U - Um + dt^2/rho L(U)            (+ sources)
+//   * corrector sweep:  Up' = Up + dt^4/(12 rho) L((Up-2U+Um)/dt^2)
+//                             - supergrid damping(U, Um)           (+ sources)
+//     i.e. evalRHS+evalPredictor (EW.C:2569-2584) and evalDpDmInTime+
+//     evalRHS+evalCorrector+addSuperGridDamping (EW.C:2662-2715) in two
+//     passes over HBM instead of eight;
+//   * ghost-shell updates that reproduce what the unfused path leaves in the
+//     ghost points;
+//   * the sparse point-source update, the CFL reduction (EW::computeDT,
+//     EW.C:5041-5146), the analytic point-source solution
+//     (EW::get_exact_point_source, EW.C:3899-4523) and the error norms
+//     (EW::normOfDifference, EW.C:4526-4610 / solerr3fort).
+#include "kernels_common.h"
+#include "kernels_step.h"
+
+namespace sw4amd {
+
+namespace {
+inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
+}
+
+// ---------------------------------------------------------------------------
+// Fused sweeps, thread per point (component-blocked layout)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_pred_interior(Box b, int klo, int khi, const double* __restrict__ u, const double* __restrict__ um,
+                const double* __restrict__ mu, const double* __restrict__ la,
+                const double* __restrict__ rho, const double* __restrict__ strx,
+                const double* __restrict__ stry, const double* __restrict__ strz,
+                double* __restrict__ up, double cof, double dt2)
+{
+   const int i = b.ib + 2 + blockIdx.x * blockDim.x + threadIdx.x;
+   const int j = b.jb + 2 + blockIdx.y * blockDim.y + threadIdx.y;
+   const int k = klo + blockIdx.z;
+   if (i > b.ie - 2 || j > b.je - 2 || k > khi) return;
+   double r[3];
+   rhs_interior_point<1>(b, i, j, k, Plain{u}, mu, la, strx, stry, strz, r);
+   const long p = sidx(b, i, j, k);
+   const double s = dt2 / rho[p];
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      const long q = p + b.npts * c;
+      up[q] = 2 * u[q] - um[q] + s * (cof * r[c]);
+   }
+}
+
+__global__ void __launch_bounds__(256)
+k_pred_sbp(Box b, int top, int nk, const SbpTables* __restrict__ sbp, const double* __restrict__ u,
+           const double* __restrict__ um, const double* __restrict__ mu,
+           const double* __restrict__ la, const double* __restrict__ rho,
+           const double* __restrict__ strx, const double* __restrict__ stry,
+           double* __restrict__ up, double cof, double dt2)
+{
+   const int i = b.ib + 2 + blockIdx.x * blockDim.x + threadIdx.x;
+   const int j = b.jb + 2 + blockIdx.y * blockDim.y + threadIdx.y;
+   const int kr = 1 + blockIdx.z;
+   if (i > b.ie - 2 || j > b.je - 2) return;
+   const int k = top ? kr : nk - kr + 1;
+   double r[3];
+   rhs_sbp_point<1>(b, i, j, k, kr, top ? 1 : nk, top ? 1 : -1, sbp, Plain{u}, mu, la, strx, stry, r);
+   const long p = sidx(b, i, j, k);
+   const double s = dt2 / rho[p];
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      const long q = p + b.npts * c;
+      up[q] = 2 * u[q] - um[q] + s * (cof * r[c]);
+   }
+}
+
+__global__ void __launch_bounds__(256)
+k_corr_interior(Box b, int klo, int khi, const double* __restrict__ up, const double* __restrict__ u,
+                const double* __restrict__ um, const double* __restrict__ mu,
+                const double* __restrict__ la, const double* __restrict__ rho, SgCoef sg,
+                double* __restrict__ upo, double cof, double dt2i, double dt4i12, double beta)
+{
+   const int i = b.ib + 2 + blockIdx.x * blockDim.x + threadIdx.x;
+   const int j = b.jb + 2 + blockIdx.y * blockDim.y + threadIdx.y;
+   const int k = klo + blockIdx.z;
+   if (i > b.ie - 2 || j > b.je - 2 || k > khi) return;
+   double r[3], d[3];
+   rhs_interior_point<1>(b, i, j, k, Accel{up, u, um, dt2i}, mu, la, sg.strx, sg.stry, sg.strz, r);
+   addsgd4_point<1>(b, i, j, k, u, um, rho, sg, beta, d);
+   const long p = sidx(b, i, j, k);
+   const double s = dt4i12 / rho[p];
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      const long q = p + b.npts * c;
+      upo[q] = (up[q] + s * (cof * r[c])) - d[c];
+   }
+}
+
+__global__ void __launch_bounds__(256)
+k_corr_sbp(Box b, int top, int nk, const SbpTables* __restrict__ sbp, const double* __restrict__ up,
+           const double* __restrict__ u, const double* __restrict__ um,
+           const double* __restrict__ mu, const double* __restrict__ la,
+           const double* __restrict__ rho, SgCoef sg, double* __restrict__ upo, double cof,
+           double dt2i, double dt4i12, double beta)
+{
+   const int i = b.ib + 2 + blockIdx.x * blockDim.x + threadIdx.x;
+   const int j = b.jb + 2 + blockIdx.y * blockDim.y + threadIdx.y;
+   const int kr = 1 + blockIdx.z;
+   if (i > b.ie - 2 || j > b.je - 2) return;
+   const int k = top ? kr : nk - kr + 1;
+   double r[3], d[3];
+   rhs_sbp_point<1>(b, i, j, k, kr, top ? 1 : nk, top ? 1 : -1, sbp, Accel{up, u, um, dt2i}, mu, la,
+                    sg.strx, sg.stry, r);
+   addsgd4_point<1>(b, i, j, k, u, um, rho, sg, beta, d);
+   const long p = sidx(b, i, j, k);
+   const double s = dt4i12 / rho[p];
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      const long q = p + b.npts * c;
+      upo[q] = (up[q] + s * (cof * r[c])) - d[c];
+   }
+}
+
+// ---------------------------------------------------------------------------
+// Ghost shell: the 2-point frame outside [1,nx]x[1,ny]x[1,nz].  mode 0:
+// up = 2u - um (the predictor where L(u) and F vanish); mode 1: out = in.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool shell_point(const Box& b, long q, int& i, int& j, int& k)
+{
+   const long ni = b.ni, nj = b.je - b.jb + 1;
+   const long nxi = ni - 4, nyi = nj - 4, nzi = (b.ke - b.kb + 1) - 4;
+   const long nk_planes = 4 * ni * nj;      // k in {kb,kb+1,ke-1,ke}
+   const long nj_planes = 4 * ni * nzi;     // j in {jb,jb+1,je-1,je}, k interior
+   const long ni_planes = 4 * nyi * nzi;    // i in {ib,ib+1,ie-1,ie}, j,k interior
+   if (q < nk_planes) {
+      const long pl = q / (ni * nj), r = q % (ni * nj);
+      k = pl < 2 ? b.kb + (int)pl : b.ke - 3 + (int)pl;
+      j = b.jb + (int)(r / ni);
+      i = b.ib + (int)(r % ni);
+      return true;
+   }
+   q -= nk_planes;
+   if (q < nj_planes) {
+      const long pl = q / (ni * nzi), r = q % (ni * nzi);
+      j = pl < 2 ? b.jb + (int)pl : b.je - 3 + (int)pl;
+      k = b.kb + 2 + (int)(r / ni);
+      i = b.ib + (int)(r % ni);
+      return true;
+   }
+   q -= nj_planes;
+   if (q < ni_planes) {
+      const long pl = q / (nyi * nzi), r = q % (nyi * nzi);
+      i = pl < 2 ? b.ib + (int)pl : b.ie - 3 + (int)pl;
+      k = b.kb + 2 + (int)(r / nyi);
+      j = b.jb + 2 + (int)(r % nyi);
+      return true;
+   }
+   (void)nxi;
+   return false;
+}
+
+__global__ void k_shell(Box b, int mode, const double* __restrict__ a, const double* __restrict__ u,
+                        const double* __restrict__ um, double* __restrict__ out)
+{
+   const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   int i, j, k;
+   if (!shell_point(b, q, i, j, k)) return;
+   const long p = sidx(b, i, j, k);
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      const long v = p + b.npts * c;
+      out[v] = mode == 0 ? 2 * u[v] - um[v] : a[v];
+   }
+}
+
+long shell_count(const Box& b)
+{
+   const long ni = b.ni, nj = b.je - b.jb + 1, nk = b.ke - b.kb + 1;
+   return 4 * ni * nj + 4 * ni * (nk - 4) + 4 * (nj - 4) * (nk - 4);
+}
+
+// ---------------------------------------------------------------------------
+// Sparse source update: up(:,p) += coef/rho(p) * sum_s f_s g_s(t)
+// ---------------------------------------------------------------------------
+__global__ void k_src_apply(Box b, SourceTable st, double t, int deriv2, double coef,
+                            const double* __restrict__ rho, double* __restrict__ up)
+{
+   const int r = blockIdx.x * blockDim.x + threadIdx.x;
+   if (r >= st.nunique) return;
+   const int s0 = st.first[r], s1 = st.first[r + 1];
+   const int i = st.ijk[3 * s0], j = st.ijk[3 * s0 + 1], k = st.ijk[3 * s0 + 2];
+   double f0 = 0, f1 = 0, f2 = 0;
+   for (int s = s0; s < s1; s++) {
+      const double g = time_function(st.tfkind[s], deriv2, st.freq[s], t - st.t0[s]);
+      f0 += st.force[3 * s] * g;
+      f1 += st.force[3 * s + 1] * g;
+      f2 += st.force[3 * s + 2] * g;
+   }
+   const long p = sidx(b, i, j, k);
+   const double sc = coef / rho[p];
+   up[p] += sc * f0;
+   up[p + b.npts] += sc * f1;
+   up[p + 2 * b.npts] += sc * f2;
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic two-stage reductions
+// ---------------------------------------------------------------------------
+constexpr int RED_BLOCK = 256;
+
+__device__ __forceinline__ double block_max(double v, double* sh)
+{
+   sh[threadIdx.x] = v;
+   __syncthreads();
+   for (int o = RED_BLOCK / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+      __syncthreads();
+   }
+   const double r = sh[0];
+   __syncthreads();
+   return r;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh)
+{
+   sh[threadIdx.x] = v;
+   __syncthreads();
+   for (int o = RED_BLOCK / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+      __syncthreads();
+   }
+   const double r = sh[0];
+   __syncthreads();
+   return r;
+}
+
+// max over the whole box of (4 mu + la)/rho  (EW.C:5050-5058)
+__global__ void __launch_bounds__(RED_BLOCK)
+k_maxeig(long n, const double* __restrict__ mu, const double* __restrict__ la,
+         const double* __restrict__ rho, double* __restrict__ partial)
+{
+   __shared__ double sh[RED_BLOCK];
+   double m = -1;
+   for (long q = blockIdx.x * (long)RED_BLOCK + threadIdx.x; q < n; q += (long)gridDim.x * RED_BLOCK)
+      m = fmax(m, (4 * mu[q] + la[q]) / rho[q]);
+   m = block_max(m, sh);
+   if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+
+__global__ void __launch_bounds__(RED_BLOCK) k_final_max(int n, const double* __restrict__ partial, double* __restrict__ out)
+{
+   __shared__ double sh[RED_BLOCK];
+   double m = -1;
+   for (int q = threadIdx.x; q < n; q += RED_BLOCK) m = fmax(m, partial[q]);
+   m = block_max(m, sh);
+   if (threadIdx.x == 0) out[0] = m;
+}
+
+// ---------------------------------------------------------------------------
+// Analytic whole-space solution of a moment-tensor point source
+// ---------------------------------------------------------------------------
+// Time functions of the exact solution, with s = time - R/c and "time" the
+// scaled time of EW.C:3929-3934.  The C6 bump and its antiderivative moments
+// are evaluated in Horner form; the Gaussian uses erf.
+__device__ __forceinline__ double c6_g(double s)
+{
+   if (!(s > 0 && s < 1)) return 0.0;
+   const double p = s * (1 - s), p2 = p * p, p4 = p2 * p2;
+   return 51480.0 * (p4 * p2 * p);
+}
+__device__ __forceinline__ double c6_gp(double s)
+{
+   if (!(s > 0 && s < 1)) return 0.0;
+   const double p = s * (1 - s), p2 = p * p;
+   return 51480.0 * 7 * (1 - 2 * s) * (p2 * p2 * p2);
+}
+// G1(x) = int_0^x g, G2(x) = int_0^x s g(s) ds for g = 51480 s^7 (1-s)^7
+// (unit mass, mean 1/2), x in [0,1].  The binomial series
+// (1-s)^7 = sum_m (-1)^m C(7,m) s^m alternates, so it is only summed for
+// arguments <= 1/2; larger x use g(s) = g(1-s):
+//   G1(x) = 1 - G1(1-x),  G2(x) = 1/2 - G1(1-x) + G2(1-x).
+__device__ __forceinline__ void c6_moments(double x, double& g1, double& g2)
+{
+   const bool flip = x > 0.5;
+   const double y = flip ? 1 - x : x;  // exact for x in [1/2,1]
+   const double binom[8] = {1, 7, 21, 35, 35, 21, 7, 1};
+   double a1 = 0, a2 = 0;
+   for (int m = 7; m >= 0; m--) {  // Horner over y^m/(8+m) and y^m/(9+m)
+      const double sg = (m & 1) ? -binom[m] : binom[m];
+      a1 = a1 * y + sg / (8 + m);
+      a2 = a2 * y + sg / (9 + m);
+   }
+   const double y2 = y * y, y4 = y2 * y2, y8 = y4 * y4;
+   const double G1 = 51480.0 * y8 * a1, G2 = 51480.0 * y8 * y * a2;
+   if (flip) {
+      g1 = 1 - G1;
+      g2 = 0.5 - G1 + G2;
+   } else {
+      g1 = G1;
+      g2 = G2;
+   }
+}
+// int_{lo}^{hi} tau g(time - tau) dtau, clipped to the support of g
+__device__ __forceinline__ double c6_tau_integral(double time, double R, double alpha, double beta)
+{
+   const double lo = fmax(R / alpha, time - 1), hi = fmin(R / beta, time);
+   if (!(lo < time && hi > time - 1)) return 0.0;
+   // = int_{time-hi}^{time-lo} (time - s) g(s) ds
+   double g1a, g2a, g1b, g2b;
+   c6_moments(time - hi, g1a, g2a);
+   c6_moments(time - lo, g1b, g2b);
+   return time * (g1b - g1a) - (g2b - g2a);
+}
+
+__device__ void exact_moment(const ExactParams& P, double x, double y, double z, double out[3])
+{
+   const double X[3] = {x - P.x0, y - P.y0, z - P.z0};
+   const double R = sqrt(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]);
+   out[0] = out[1] = out[2] = 0;
+   if (R < P.eps) return;
+   const double al2 = P.alpha * P.alpha, be2 = P.beta * P.beta;
+   double A, B, C, D, E;
+   if (P.tfkind == 2) {
+      A = c6_g(P.time - R / P.alpha);
+      B = c6_g(P.time - R / P.beta);
+      C = c6_tau_integral(P.time, R, P.alpha, P.beta);
+      D = c6_gp(P.time - R / P.alpha) / (al2 * P.alpha) / R;
+      E = c6_gp(P.time - R / P.beta) / (be2 * P.beta) / R;
+   } else {
+      const double f = P.fr, nrm = 1 / (f * sqrt(2 * M_PI));
+      const double sa = P.time - R / P.alpha, sb = P.time - R / P.beta;
+      A = nrm * exp(-sa * sa / (2 * f * f));
+      B = nrm * exp(-sb * sb / (2 * f * f));
+      C = -0.5 * P.time * (erf(sb / (sqrt(2.0) * f)) - erf(sa / (sqrt(2.0) * f))) -
+          f / sqrt(2 * M_PI) * (exp(-sb * sb / (2 * f * f)) - exp(-sa * sa / (2 * f * f)));
+      D = -A * sa / (f * f) / (al2 * P.alpha) / R;
+      E = -B * sb / (f * f) / (be2 * P.beta) / R;
+   }
+   const double Aa = A / al2, Bb = B / be2, AmB = Aa - Bb;
+   const double r2 = R * R, ir = 1 / R, ir3 = ir / r2, ir5 = ir3 / r2, ir7 = ir5 / r2;
+   // u_i = -1/(4 pi rho) sum_kj M_kj d_j G_ik, G_ik = T1 + T2 + T3:
+   //  T1 = (3 X_i X_k/r^5 - d_ik/r^3) C,  T2 = X_i X_k/r^3 A/alpha^2,
+   //  T3 = -(X_i X_k/r^3 - d_ik/r) B/beta^2,
+   //  d_j C = X_j (B/beta^2 - A/alpha^2), d_j(A/alpha^2) = -X_j D, d_j(B/beta^2) = -X_j E
+   const double Mt[3][3] = {{P.m[0], P.m[1], P.m[2]}, {P.m[1], P.m[3], P.m[4]}, {P.m[2], P.m[4], P.m[5]}};
+   for (int i = 0; i < 3; i++) {
+      double acc = 0;
+      for (int k = 0; k < 3; k++)
+         for (int j = 0; j < 3; j++) {
+            const double dij = i == j, dkj = k == j, dik = i == k;
+            const double xx = X[i] * X[k];
+            const double dxx = dij * X[k] + dkj * X[i];
+            const double t1 = (3 * dxx * ir5 - 15 * xx * X[j] * ir7 + 3 * dik * X[j] * ir5) * C -
+                              (3 * xx * ir5 - dik * ir3) * X[j] * AmB;
+            const double t2 = (dxx * ir3 - 3 * xx * X[j] * ir5) * Aa - xx * ir3 * X[j] * D;
+            const double t3 = -(dxx * ir3 - 3 * xx * X[j] * ir5 + dik * X[j] * ir3) * Bb +
+                              (xx * ir3 - dik * ir) * X[j] * E;
+            acc += Mt[k][j] * (t1 + t2 + t3);
+         }
+      out[i] = -acc / (4 * M_PI * P.rho);
+   }
+}
+
+__global__ void k_exact(Box b, ExactParams P, double* __restrict__ out)
+{
+   const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (p >= b.npts) return;
+   const int i = b.ib + (int)(p % b.ni);
+   const int j = b.jb + (int)((p / b.ni) % (b.je - b.jb + 1));
+   const int k = b.kb + (int)(p / b.nij);
+   double e[3];
+   exact_moment(P, (i - 1) * P.h, (j - 1) * P.h, (k - 1) * P.h + P.zmin, e);
+   out[p] = e[0];
+   out[p + b.npts] = e[1];
+   out[p + 2 * b.npts] = e[2];
+}
+
+// Linf, h^3 L2 and max(uex) of u - uex over a window, excluding a ball
+// (solerr3fort, ew-cfromfort.C:144-202).  Partials per block, 3 per block.
+__global__ void __launch_bounds__(RED_BLOCK)
+k_solerr(Box b, NormWindow w, const double* __restrict__ uex, const double* __restrict__ u,
+         double* __restrict__ partial)
+{
+   __shared__ double sh[RED_BLOCK];
+   const long wi = w.imax - w.imin + 1, wj = w.jmax - w.jmin + 1, wk = w.kmax - w.kmin + 1;
+   const long n = wi * wj * wk;
+   const double h3 = w.h * w.h * w.h;
+   double li = 0, l2 = 0, xl = 0;
+   for (long q = blockIdx.x * (long)RED_BLOCK + threadIdx.x; q < n; q += (long)gridDim.x * RED_BLOCK) {
+      const int i = w.imin + (int)(q % wi), j = w.jmin + (int)((q / wi) % wj), k = w.kmin + (int)(q / (wi * wj));
+      const double dx = (i - 1) * w.h - w.x0, dy = (j - 1) * w.h - w.y0, dz = (k - 1) * w.h + w.zmin - w.z0;
+      if (dx * dx + dy * dy + dz * dz <= w.r2) continue;
+      const long p = sidx(b, i, j, k);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+         const double ex = uex[p + b.npts * c], e = u[p + b.npts * c] - ex;
+         li = fmax(li, fabs(e));
+         xl = fmax(xl, ex);
+         l2 += h3 * e * e;
+      }
+   }
+   li = block_max(li, sh);
+   xl = block_max(xl, sh);
+   l2 = block_sum(l2, sh);
+   if (threadIdx.x == 0) {
+      partial[3 * blockIdx.x] = li;
+      partial[3 * blockIdx.x + 1] = l2;
+      partial[3 * blockIdx.x + 2] = xl;
+   }
+}
+
+__global__ void __launch_bounds__(RED_BLOCK) k_final_norm(int n, const double* __restrict__ partial, double* __restrict__ out)
+{
+   __shared__ double sh[RED_BLOCK];
+   double li = 0, l2 = 0, xl = 0;
+   for (int q = threadIdx.x; q < n; q += RED_BLOCK) {
+      li = fmax(li, partial[3 * q]);
+      l2 += partial[3 * q + 1];
+      xl = fmax(xl, partial[3 * q + 2]);
+   }
+   li = block_max(li, sh);
+   xl = block_max(xl, sh);
+   l2 = block_sum(l2, sh);
+   if (threadIdx.x == 0) {
+      out[0] = li;
+      out[1] = l2;
+      out[2] = xl;
+   }
+}
+
+// ===========================================================================
+// launchers
+// ===========================================================================
+int launch_pred_fused(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp,
+                      const double* u, const double* um, const double* mu, const double* la,
+                      const double* rho, const SgCoef& sg, double* up, double h, double dt,
+                      hipStream_t s, const std::function<int()>& after_interior)
+{
+   const double cof = 1.0 / (h * h), dt2 = dt * dt;
+   const int k1 = onesided[4] == 1 ? 7 : b.kb + 2;
+   const int k2 = onesided[5] == 1 ? nk - 6 : b.ke - 2;
+   const int nii = b.ie - b.ib - 3, njj = b.je - b.jb - 3;
+   dim3 blk(64, 4, 1);
+   if (k2 >= k1) {
+      dim3 grd(cdiv(nii, 64), cdiv(njj, 4), k2 - k1 + 1);
+      k_pred_interior<<<grd, blk, 0, s>>>(b, k1, k2, u, um, mu, la, rho, sg.strx, sg.stry, sg.strz, up, cof, dt2);
+   }
+   if (after_interior) {
+      const int rc = after_interior();
+      if (rc) return rc;
+   }
+   for (int side = 4; side <= 5; side++) {
+      if (onesided[side] != 1) continue;
+      dim3 grd(cdiv(nii, 64), cdiv(njj, 4), 6);
+      k_pred_sbp<<<grd, blk, 0, s>>>(b, side == 4, nk, d_sbp, u, um, mu, la, rho, sg.strx, sg.stry, up, cof, dt2);
+   }
+   const long ns = shell_count(b);
+   k_shell<<<cdiv(ns, 256), 256, 0, s>>>(b, 0, nullptr, u, um, up);
+   return check_hip(hipGetLastError(), "fused predictor launch");
+}
+
+int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp,
+                      const double* up, const double* u, const double* um, const double* mu,
+                      const double* la, const double* rho, const SgCoef& sg, double beta,
+                      double* upo, double h, double dt, hipStream_t s,
+                      const std::function<int()>& after_interior)
+{
+   const double cof = 1.0 / (h * h), dt2i = 1.0 / (dt * dt), dt4i12 = dt * dt * dt * dt / 12;
+   const int k1 = onesided[4] == 1 ? 7 : b.kb + 2;
+   const int k2 = onesided[5] == 1 ? nk - 6 : b.ke - 2;
+   const int nii = b.ie - b.ib - 3, njj = b.je - b.jb - 3;
+   dim3 blk(64, 4, 1);
+   if (k2 >= k1) {
+      dim3 grd(cdiv(nii, 64), cdiv(njj, 4), k2 - k1 + 1);
+      k_corr_interior<<<grd, blk, 0, s>>>(b, k1, k2, up, u, um, mu, la, rho, sg, upo, cof, dt2i, dt4i12, beta);
+   }
+   if (after_interior) {
+      const int rc = after_interior();
+      if (rc) return rc;
+   }
+   for (int side = 4; side <= 5; side++) {
+      if (onesided[side] != 1) continue;
+      dim3 grd(cdiv(nii, 64), cdiv(njj, 4), 6);
+      k_corr_sbp<<<grd, blk, 0, s>>>(b, side == 4, nk, d_sbp, up, u, um, mu, la, rho, sg, upo, cof, dt2i, dt4i12, beta);
+   }
+   const long ns = shell_count(b);
+   k_shell<<<cdiv(ns, 256), 256, 0, s>>>(b, 1, up, nullptr, nullptr, upo);
+   return check_hip(hipGetLastError(), "fused corrector launch");
+}
+
+int launch_src_apply(const Box& b, const SourceTable& st, double t, int deriv2, double coef,
+                     const double* rho, double* up, hipStream_t s)
+{
+   if (st.nunique <= 0) return OK;
+   k_src_apply<<<cdiv(st.nunique, 128), 128, 0, s>>>(b, st, t, deriv2, coef, rho, up);
+   return check_hip(hipGetLastError(), "source update launch");
+}
+
+int launch_maxeig(long n, const double* mu, const double* la, const double* rho, double* partial,
+                  int npartial, double* out, hipStream_t s)
+{
+   k_maxeig<<<npartial, RED_BLOCK, 0, s>>>(n, mu, la, rho, partial);
+   k_final_max<<<1, RED_BLOCK, 0, s>>>(npartial, partial, out);
+   return check_hip(hipGetLastError(), "maxeig launch");
+}
+
+int launch_exact(const Box& b, const ExactParams& P, double* out, hipStream_t s)
+{
+   k_exact<<<cdiv(b.npts, 256), 256, 0, s>>>(b, P, out);
+   return check_hip(hipGetLastError(), "exact solution launch");
+}
+
+int launch_solerr(const Box& b, const NormWindow& w, const double* uex, const double* u,
+                  double* partial, int npartial, double* out, hipStream_t s)
+{
+   k_solerr<<<npartial, RED_BLOCK, 0, s>>>(b, w, uex, u, partial);
+   k_final_norm<<<1, RED_BLOCK, 0, s>>>(npartial, partial, out);
+   return check_hip(hipGetLastError(), "error norm launch");
+}
+
+}  // namespace sw4amd

@@ -1,0 +1,235 @@
+"""Python mirror of the reference ``EW`` time-step interface over the C ABI.
+
+:class:`EW` owns one Cartesian grid on the current GPU (through
+``sw4hip_ew_*``) and exposes the reference's driver-level operations:
+material and source set-up (EW::setupRun, EW.C:2288-2336), the time-step loop
+(EW::timesteploop, EW.C:2339-2928) and the point-source error
+(EW::normOfDifference, EW.C:4526-4610).  The kernel-level functions below
+(:func:`rhs4sg`, :func:`predfort`, ...) take the reference kernels' argument
+lists with device tensors.  Device memory comes from PyTorch; all arithmetic
+runs in the HIP library.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import lib as L
+
+
+def _ptr(a):
+    """Device or host address of a torch tensor / numpy array."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return C.c_void_p(a.data_ptr())
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data_as(C.c_void_p)
+    raise TypeError("expected a torch tensor or numpy array")
+
+
+def _is_dev(a):
+    return hasattr(a, "is_cuda") and a.is_cuda
+
+
+def _stream(stream):
+    if stream is not None:
+        return C.c_void_p(stream)
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def stencil_coefficients():
+    """acof[384], ghcof[6], bope[48], sbop[5] (EW::GetStencilCoefficients)."""
+    lib = L.load()
+    acof, ghcof, bope, sbop = np.zeros(384), np.zeros(6), np.zeros(48), np.zeros(5)
+    L.check(lib.sw4hip_get_stencil_coefficients(_ptr(acof), _ptr(ghcof), _ptr(bope), _ptr(sbop)))
+    return acof, ghcof, bope, sbop
+
+
+def host_supergrid(desc):
+    lib = L.load()
+    n = (desc.nx + 4, desc.ny + 4, desc.nz + 4)
+    out = {k: np.zeros(n["xyz".index(k[-1])]) for k in
+           ("dcx", "dcy", "dcz", "strx", "stry", "strz", "cox", "coy", "coz")}
+    L.check(lib.sw4hip_host_supergrid(C.byref(desc), *[_ptr(out[k]) for k in
+                                                        ("dcx", "dcy", "dcz", "strx", "stry", "strz",
+                                                         "cox", "coy", "coz")]))
+    return out
+
+
+def host_boundary_windows(desc):
+    lib = L.load()
+    wind = np.zeros(36, dtype=np.int32)
+    npts = np.zeros(6, dtype=np.int32)
+    L.check(lib.sw4hip_host_boundary_windows(C.byref(desc), _ptr(wind), _ptr(npts)))
+    return wind, npts
+
+
+def host_moment_source(desc, x, y, z, m):
+    lib = L.load()
+    maxn = 216
+    ijk = np.zeros(3 * maxn, dtype=np.int32)
+    force = np.zeros(3 * maxn)
+    n = C.c_int(0)
+    mm = np.asarray(m, dtype=np.float64)
+    L.check(lib.sw4hip_host_moment_source(C.byref(desc), x, y, z, _ptr(mm), maxn, C.byref(n),
+                                          ijk.ctypes.data_as(L.IP), _ptr(force)))
+    return ijk[:3 * n.value].reshape(-1, 3), force[:3 * n.value].reshape(-1, 3)
+
+
+# --------------------------------------------------------------------------
+# kernel level (device tensors, reference argument lists)
+# --------------------------------------------------------------------------
+def rhs4sg(corder, box, nk, onesided, acof, bope, ghcof, lu, u, mu, la, h, strx, stry, strz, stream=None):
+    lib = L.load()
+    os_ = np.ascontiguousarray(onesided, dtype=np.int32)
+    L.check(lib.sw4hip_rhs4sg(corder, *box, nk, _ptr(os_), _ptr(np.ascontiguousarray(acof)),
+                              _ptr(np.ascontiguousarray(bope)), _ptr(np.ascontiguousarray(ghcof)),
+                              _ptr(lu), _ptr(u), _ptr(mu), _ptr(la), h, _ptr(strx), _ptr(stry), _ptr(strz),
+                              _stream(stream)))
+
+
+def predfort(corder, box, up, u, um, lu, fo, rho, dt2, stream=None):
+    L.check(L.load().sw4hip_predfort(corder, *box, _ptr(up), _ptr(u), _ptr(um), _ptr(lu), _ptr(fo), _ptr(rho),
+                                     dt2, _stream(stream)))
+
+
+def corrfort(corder, box, up, lu, fo, rho, dt4, stream=None):
+    L.check(L.load().sw4hip_corrfort(corder, *box, _ptr(up), _ptr(lu), _ptr(fo), _ptr(rho), dt4, _stream(stream)))
+
+
+def dpdmtfort(box, up, u, um, uacc, dt2i, stream=None):
+    L.check(L.load().sw4hip_dpdmtfort(*box, _ptr(up), _ptr(u), _ptr(um), _ptr(uacc), dt2i, _stream(stream)))
+
+
+def addsgd4(corder, box, up, u, um, rho, dcx, dcy, dcz, strx, stry, strz, cox, coy, coz, beta, stream=None):
+    L.check(L.load().sw4hip_addsgd4(corder, *box, _ptr(up), _ptr(u), _ptr(um), _ptr(rho), _ptr(dcx), _ptr(dcy),
+                                    _ptr(dcz), _ptr(strx), _ptr(stry), _ptr(strz), _ptr(cox), _ptr(coy), _ptr(coz),
+                                    beta, _stream(stream)))
+
+
+def bcfortsg(corder, box, wind, nx, ny, nz, u, h, bccnd, sbop, mu, la, bforce, strx, stry, stream=None):
+    w = np.ascontiguousarray(wind, dtype=np.int32)
+    bc = np.ascontiguousarray(bccnd, dtype=np.int32)
+    sb = np.ascontiguousarray(sbop, dtype=np.float64)
+    bf = (C.c_void_p * 6)(*[(_ptr(b).value if b is not None else None) for b in bforce])
+    L.check(L.load().sw4hip_bcfortsg(corder, *box, _ptr(w), nx, ny, nz, _ptr(u), h, _ptr(bc), _ptr(sb), _ptr(mu),
+                                     _ptr(la), C.cast(bf, C.c_void_p), _ptr(strx), _ptr(stry), _stream(stream)))
+
+
+# --------------------------------------------------------------------------
+# EW object
+# --------------------------------------------------------------------------
+class EW:
+    """One Cartesian grid with the reference's time-stepping driver.
+
+    Arrays span [-1, n+2] on every axis (two ghost points).  bc lists the
+    boundary type of the six sides x-, x+, y-, y+, z-, z+.
+    """
+
+    def __init__(self, nx, ny, nz, h, bc, corder=1, sg_gp_thickness=20, sg_damping=0.02, cfl=1.3,
+                 zmin=0.0, stream=None):
+        self.lib = L.load()
+        self.desc = L.grid_desc(nx, ny, nz, h, bc, corder=corder, zmin=zmin, sg_gp_thickness=sg_gp_thickness,
+                                sg_damping=sg_damping, cfl=cfl)
+        self.h = C.c_void_p()
+        L.check(self.lib.sw4hip_ew_create(C.byref(self.desc), C.byref(self.h)))
+        self.shape = (nz + 4, ny + 4, nx + 4)  # (k, j, i), i fastest
+        self.corder = corder
+        if stream is not None:
+            L.check(self.lib.sw4hip_ew_set_stream(self.h, C.c_void_p(stream)))
+
+    # set-up -------------------------------------------------------------
+    def set_material(self, rho, mu, la):
+        on_dev = _is_dev(rho)
+        L.check(self.lib.sw4hip_ew_set_material(self.h, _ptr(rho), _ptr(mu), _ptr(la), int(on_dev)))
+
+    def set_material_const(self, rho, cs, cp):
+        L.check(self.lib.sw4hip_ew_set_material_const(self.h, rho, cs, cp))
+
+    def add_moment_source(self, x, y, z, m, t0, freq, tfkind):
+        mm = np.asarray(m, dtype=np.float64)
+        L.check(self.lib.sw4hip_ew_add_moment_source(self.h, x, y, z, _ptr(mm), t0, freq, tfkind))
+
+    def setup(self, tmax=0.0, nsteps=0):
+        L.check(self.lib.sw4hip_ew_setup(self.h, tmax, nsteps))
+
+    def info(self):
+        inf = L.EWInfo()
+        L.check(self.lib.sw4hip_ew_get_info(self.h, C.byref(inf)))
+        return inf
+
+    def supergrid(self):
+        n = {"x": self.desc.nx + 4, "y": self.desc.ny + 4, "z": self.desc.nz + 4}
+        keys = ("dcx", "dcy", "dcz", "strx", "stry", "strz", "cox", "coy", "coz")
+        out = {k: np.zeros(n[k[-1]]) for k in keys}
+        L.check(self.lib.sw4hip_ew_get_supergrid(self.h, *[_ptr(out[k]) for k in keys]))
+        return out
+
+    def sources(self):
+        inf = self.info()
+        n, nu = inf.nsources, inf.nunique
+        ijk = np.zeros(3 * n, dtype=np.int32)
+        force = np.zeros(3 * n)
+        freq = np.zeros(n)
+        t0 = np.zeros(n)
+        tf = np.zeros(n, dtype=np.int32)
+        first = np.zeros(nu + 1, dtype=np.int32)
+        L.check(self.lib.sw4hip_ew_get_sources(self.h, _ptr(ijk), _ptr(force), _ptr(freq), _ptr(t0), _ptr(tf),
+                                               _ptr(first)))
+        return {"ijk": ijk.reshape(-1, 3), "force": force.reshape(-1, 3), "freq": freq, "t0": t0,
+                "tfkind": tf, "first": first}
+
+    # state ---------------------------------------------------------------
+    def set_solution(self, u, um):
+        on_dev = _is_dev(u)
+        L.check(self.lib.sw4hip_ew_set_solution(self.h, _ptr(u), _ptr(um), int(on_dev)))
+
+    def get_solution(self):
+        n = 3 * int(np.prod(self.shape))
+        u, um = np.empty(n), np.empty(n)
+        L.check(self.lib.sw4hip_ew_get_solution(self.h, _ptr(u), _ptr(um), 0))
+        return u, um
+
+    def device_array(self, which):
+        p = C.c_void_p()
+        L.check(self.lib.sw4hip_ew_device_array(self.h, which, C.byref(p)))
+        return p.value
+
+    # time stepping -------------------------------------------------------
+    def timesteploop(self, nsteps=None, mode=0):
+        """Advance nsteps (default: all remaining) steps; mode 0 fused sweeps,
+        mode 1 the reference stage order."""
+        inf = self.info()
+        if nsteps is None:
+            nsteps = inf.nsteps - inf.cycle
+        L.check(self.lib.sw4hip_ew_step(self.h, int(nsteps), int(mode)))
+
+    def timesteploop_timed(self, nsteps):
+        """Fused steps with per-stage GPU times (ms, summed over the steps)."""
+        ms = np.zeros(6)
+        L.check(self.lib.sw4hip_ew_step_timed(self.h, int(nsteps), _ptr(ms)))
+        return ms
+
+    def sync(self):
+        L.check(self.lib.sw4hip_ew_sync(self.h))
+
+    def point_source_error(self):
+        li, l2, xi = C.c_double(), C.c_double(), C.c_double()
+        L.check(self.lib.sw4hip_ew_point_source_error(self.h, C.byref(li), C.byref(l2), C.byref(xi)))
+        return li.value, l2.value, xi.value
+
+    def close(self):
+        if self.h:
+            self.lib.sw4hip_ew_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,134 @@
+"""The reference time-step loop (EW::timesteploop, EW.C:2527-2842) driven on
+the CPU with the oracle kernels.  Set-up products (supergrid arrays, ghost
+windows, grid-point sources) come from the library's host-side set-up calls,
+which need no GPU.  Test infrastructure only."""
+import numpy as np
+
+from common import grid_box, npts
+from oracle import oracle as O
+from sw4lite_amd import ew
+from sw4lite_amd import lib as L
+
+
+class OracleRun:
+    def __init__(self, nx, ny, nz, h, bc, rho, mu, la, sources, corder=1, sg_gp=20, beta=0.02, cfl=1.3,
+                 tmax=0.0, nsteps=0):
+        """rho, mu, la: arrays over the box (npts) or scalars.  sources:
+        list of dicts x,y,z,m(6),t0,freq,tfkind."""
+        self.desc = L.grid_desc(nx, ny, nz, h, bc, corder=corder, sg_gp_thickness=sg_gp, sg_damping=beta, cfl=cfl)
+        self.nx, self.ny, self.nz, self.h, self.bc, self.corder, self.beta = nx, ny, nz, h, list(bc), corder, beta
+        self.box = grid_box(nx, ny, nz)
+        n = self.n = npts(self.box)
+        self.rho = np.broadcast_to(np.asarray(rho, float), (n,)).copy()
+        self.mu = np.broadcast_to(np.asarray(mu, float), (n,)).copy()
+        self.la = np.broadcast_to(np.asarray(la, float), (n,)).copy()
+        self.sg = ew.host_supergrid(self.desc)
+        self.wind, _ = ew.host_boundary_windows(self.desc)
+        self.onesided = [0, 0, 0, 0, int(bc[4] == L.BC_FREE), int(bc[5] == L.BC_FREE)]
+        self.acof, self.ghcof, self.bope, self.sbop = ew.stencil_coefficients()
+        # EW::computeDT (EW.C:5041-5146)
+        eig = np.max((4 * self.mu + self.la) / self.rho)
+        dt = min(1e10, cfl * h * (1 / np.sqrt(eig)))
+        if tmax > 0:
+            nst = int(tmax / dt + 0.5)
+            nst = max(nst, 1)
+            dt = tmax / nst
+        else:
+            nst = nsteps
+        self.dt, self.nsteps = dt, nst
+        # grid-point sources, sorted by grid index (stable)
+        ijk, frc, fq, t0, tf = [], [], [], [], []
+        for s in sources:
+            a, b = ew.host_moment_source(self.desc, s["x"], s["y"], s["z"], s["m"])
+            ijk.append(a)
+            frc.append(b)
+            fq += [s["freq"]] * len(a)
+            t0 += [s["t0"]] * len(a)
+            tf += [s["tfkind"]] * len(a)
+        if ijk:
+            ijk = np.concatenate(ijk)
+            frc = np.concatenate(frc)
+            ni, nij = nx + 4, (nx + 4) * (ny + 4)
+            key = (ijk[:, 0] + 1) + ni * (ijk[:, 1] + 1) + nij * (ijk[:, 2] + 1)
+            order = np.argsort(key, kind="stable")
+            self.src = (ijk[order], frc[order], np.array(fq)[order], np.array(t0)[order], np.array(tf)[order])
+        else:
+            self.src = None
+        self.U = np.zeros(3 * n)
+        self.Um = np.zeros(3 * n)
+        self.Up = np.zeros(3 * n)
+        self.Lu = np.zeros(3 * n)
+        self.F = np.zeros(3 * n)
+        self.Uacc = np.zeros(3 * n)
+        self.t = 0.0
+        self.cycle = 0
+        ni, nj = nx + 4, ny + 4
+        self.bforce = []
+        wind = self.wind.reshape(6, 6)
+        for s in range(6):
+            w = wind[s]
+            m = max(0, (w[1] - w[0] + 1) * (w[3] - w[2] + 1) * (w[5] - w[4] + 1))
+            if bc[s] == L.BC_FREE:
+                m = ni * nj
+            self.bforce.append(np.zeros(3 * max(m, 1)))
+
+    def force(self, t, tt):
+        if self.src is None:
+            self.F[:] = 0
+            return
+        ijk, frc, fq, t0, tf = self.src
+        O.force(self.corder, self.box, self.F, ijk.ravel(), frc.ravel(), fq, t0, tf, t, int(tt))
+
+    def rhs(self, u):
+        sg = self.sg
+        O.rhs4sg(self.corder, self.box, self.nz, self.onesided, self.acof, self.bope, self.ghcof, self.Lu, u, self.mu,
+                 self.la, self.h, sg["strx"], sg["stry"], sg["strz"])
+
+    def enforce_bc(self, u):
+        O.bcfortsg(self.corder, self.box, self.wind, self.nx, self.ny, self.nz, u, self.h, self.bc, self.sbop,
+                   self.mu, self.la, self.bforce, self.sg["strx"], self.sg["stry"])
+
+    def step(self, n=1):
+        sg = self.sg
+        for _ in range(n):
+            t, dt = self.t, self.dt
+            self.force(t, False)
+            self.rhs(self.U)
+            O.predfort(self.corder, self.box, self.Up, self.U, self.Um, self.Lu, self.F, self.rho, dt * dt)
+            self.enforce_bc(self.Up)
+            self.force(t, True)
+            O.dpdmtfort(self.box, self.Up, self.U, self.Um, self.Uacc, 1.0 / (dt * dt))
+            self.rhs(self.Uacc)
+            O.corrfort(self.corder, self.box, self.Up, self.Lu, self.F, self.rho, dt * dt * dt * dt)
+            O.addsgd4(self.corder, self.box, self.Up, self.U, self.Um, self.rho, sg["dcx"], sg["dcy"], sg["dcz"],
+                      sg["strx"], sg["stry"], sg["strz"], sg["cox"], sg["coy"], sg["coz"], self.beta)
+            self.enforce_bc(self.Up)
+            self.t += dt
+            self.cycle += 1
+            self.Um, self.U, self.Up = self.U, self.Up, self.Um
+
+    def point_source_error(self, src):
+        """EW::exactSol + normOfDifference for source src (EW.C:3650-3669, 4526-4610)."""
+        n = self.n
+        nx, ny, nz, h = self.nx, self.ny, self.nz, self.h
+        im = (self.box[0] + self.box[1]) // 2
+        jm = (self.box[2] + self.box[3]) // 2
+        km = (self.box[4] + self.box[5]) // 2
+        p = (im - self.box[0]) + (nx + 4) * ((jm - self.box[2]) + (ny + 4) * (km - self.box[4]))
+        rho, mu, la = self.rho[p], self.mu[p], self.la[p]
+        beta, alpha = np.sqrt(mu / rho), np.sqrt((2 * mu + la) / rho)
+        fr = src["freq"]
+        time = (self.t - src["t0"]) * src["freq"]
+        if src["tfkind"] == L.TF_GAUSSIAN:
+            fr = 1 / fr
+            time = time * fr
+        uex = np.zeros(3 * n)
+        O.exact_moment(self.box, h, 0.0, src["x"], src["y"], src["z"], rho, alpha, beta, time, fr, src["m"],
+                       src["tfkind"], uex, self.corder)
+        sgp = self.desc.sg_gp_thickness
+        bc = self.bc
+        win = [max(1, sgp + 1) if bc[0] == L.BC_SUPERGRID else 1, min(nx, nx - sgp) if bc[1] == L.BC_SUPERGRID else nx,
+               max(1, sgp + 1) if bc[2] == L.BC_SUPERGRID else 1, min(ny, ny - sgp) if bc[3] == L.BC_SUPERGRID else ny,
+               max(1, sgp + 1) if bc[4] == L.BC_SUPERGRID else 1, min(nz, nz - sgp) if bc[5] == L.BC_SUPERGRID else nz]
+        li, l2, xli = O.solerr3(self.box, h, uex, self.U, 0.0, src["x"], src["y"], src["z"], 4 * h, win, self.corder)
+        return li, np.sqrt(l2), xli, uex

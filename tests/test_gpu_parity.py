@@ -1,0 +1,179 @@
+"""GPU parity: every kernel-level entry point of libsw4hip.so and the EW time
+loop (fused sweeps and the reference stage order) against the CPU oracle on
+the same seeded inputs.  Bar: max|gpu - oracle| / max|oracle| <= 1e-12 (FP64;
+the GPU contracts multiply-adds into FMAs, the oracle does not)."""
+import numpy as np
+import pytest
+
+from common import grid_box, npts, random_inputs, relerr, to_layout
+from oracle import oracle as O
+from sw4lite_amd import lib as L
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    assert _t.cuda.is_available(), "GPU tests need a visible GPU"
+    L.load()
+    return _t
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    import torch as _t
+    _t.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("corder", [0, 1])
+@pytest.mark.parametrize("os45", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_rhs4sg(torch, coefs, corder, os45):
+    from sw4lite_amd import ew
+    acof, ghcof, bope, _ = coefs
+    nx, ny, nz = 37, 21, 19
+    box, f = random_inputs(100 + corder, nx, ny, nz)
+    u = to_layout(f["u"], box, corder)
+    want = np.zeros_like(u)
+    os_ = [0, 0, 0, 0, *os45]
+    O.rhs4sg(corder, box, nz, os_, acof, bope, ghcof, want, u, f["mu"], f["la"], 0.3, f["strx"], f["stry"], f["strz"])
+    lu = dev(torch, np.zeros_like(u))
+    ew.rhs4sg(corder, box, nz, os_, acof, bope, ghcof, lu, dev(torch, u), dev(torch, f["mu"]), dev(torch, f["la"]),
+              0.3, dev(torch, f["strx"]), dev(torch, f["stry"]), dev(torch, f["strz"]))
+    got = host(lu)
+    print("rhs4sg corder", corder, "onesided", os45, "relerr", relerr(got, want), flush=True)
+    assert relerr(got, want) < TOL
+
+
+@pytest.mark.parametrize("corder", [0, 1])
+def test_pointwise_updates(torch, corder):
+    from sw4lite_amd import ew
+    box, f = random_inputs(7, 20, 18, 16)
+    g = {k: to_layout(f[k], box, corder) for k in ("u", "um", "up")}
+    _, f2 = random_inputs(8, 20, 18, 16)
+    lu, fo = to_layout(f2["u"], box, corder), to_layout(f2["um"], box, corder)
+    dt = 0.013
+    # predictor
+    want = np.zeros_like(g["u"])
+    O.predfort(corder, box, want, g["u"], g["um"], lu, fo, f["rho"], dt * dt)
+    up = dev(torch, np.zeros_like(want))
+    ew.predfort(corder, box, up, dev(torch, g["u"]), dev(torch, g["um"]), dev(torch, lu), dev(torch, fo),
+                dev(torch, f["rho"]), dt * dt)
+    assert relerr(host(up), want) < TOL
+    # corrector
+    want = g["up"].copy()
+    O.corrfort(corder, box, want, lu, fo, f["rho"], dt ** 4)
+    up = dev(torch, g["up"])
+    ew.corrfort(corder, box, up, dev(torch, lu), dev(torch, fo), dev(torch, f["rho"]), dt ** 4)
+    assert relerr(host(up) - g["up"], want - g["up"]) < TOL
+    # second time difference
+    want = np.zeros_like(g["u"])
+    O.dpdmtfort(box, g["up"], g["u"], g["um"], want, 1 / dt ** 2)
+    ua = dev(torch, np.zeros_like(want))
+    ew.dpdmtfort(box, dev(torch, g["up"]), dev(torch, g["u"]), dev(torch, g["um"]), ua, 1 / dt ** 2)
+    assert relerr(host(ua), want) < TOL
+
+
+@pytest.mark.parametrize("corder", [0, 1])
+def test_addsgd4(torch, corder):
+    from sw4lite_amd import ew
+    box, f = random_inputs(9, 33, 17, 21)
+    u, um, up0 = (to_layout(f[k], box, corder) for k in ("u", "um", "up"))
+    one = [f[k] for k in ("dcx", "dcy", "dcz", "strx", "stry", "strz", "cox", "coy", "coz")]
+    want = up0.copy()
+    O.addsgd4(corder, box, want, u, um, f["rho"], *one, 0.02)
+    up = dev(torch, up0)
+    ew.addsgd4(corder, box, up, dev(torch, u), dev(torch, um), dev(torch, f["rho"]), *[dev(torch, a) for a in one],
+               0.02)
+    got = host(up)
+    print("addsgd4 corder", corder, "relerr", relerr(got - up0, want - up0), flush=True)
+    assert relerr(got - up0, want - up0) < TOL
+
+
+@pytest.mark.parametrize("corder", [0, 1])
+@pytest.mark.parametrize("bc", [[2, 2, 2, 2, 0, 2], [1, 1, 2, 2, 2, 0], [3, 3, 3, 3, 0, 0], [2, 2, 1, 1, 1, 1]])
+def test_bcfortsg(torch, coefs, corder, bc):
+    from sw4lite_amd import ew
+    nx, ny, nz = 16, 15, 14
+    box, f = random_inputs(10, nx, ny, nz)
+    sbop = coefs[3]
+    desc = L.grid_desc(nx, ny, nz, 0.1, bc, corder=corder)
+    wind, nbc = ew.host_boundary_windows(desc)
+    rng = np.random.default_rng(11)
+    ni, nj = nx + 4, ny + 4
+    bforce = []
+    for s in range(6):
+        m = ni * nj if bc[s] == 0 else max(int(nbc[s]), 1)
+        bforce.append(rng.uniform(-1, 1, 3 * m))
+    u0 = to_layout(f["u"], box, corder)
+    want = u0.copy()
+    O.bcfortsg(corder, box, wind, nx, ny, nz, want, 0.1, bc, sbop, f["mu"], f["la"], bforce, f["strx"], f["stry"])
+    u = dev(torch, u0)
+    ew.bcfortsg(corder, box, wind, nx, ny, nz, u, 0.1, bc, sbop, dev(torch, f["mu"]), dev(torch, f["la"]),
+                [dev(torch, b) for b in bforce], dev(torch, f["strx"]), dev(torch, f["stry"]))
+    got = host(u)
+    assert np.count_nonzero(got != u0) > 0
+    assert relerr(got, want) < TOL
+
+
+# ---------------------------------------------------------------------------
+def layered_material(nx, ny, nz, h, seed):
+    rng = np.random.default_rng(seed)
+    box = grid_box(nx, ny, nz)
+    k = np.arange(box[4], box[5] + 1)
+    z = (k - 1) * h
+    cs = 1.0 + 1.5 * z / z.max()
+    cs3 = np.broadcast_to(cs[:, None, None], (nz + 4, ny + 4, nx + 4))
+    p = np.exp(0.05 * rng.standard_normal((nz + 4, ny + 4, nx + 4)))
+    cs3 = cs3 * p
+    cp3 = 1.8 * cs3
+    rho = 1.7 * cp3 ** 0.25
+    mu = rho * cs3 ** 2
+    la = rho * cp3 ** 2 - 2 * mu
+    return rho.ravel().copy(), mu.ravel().copy(), la.ravel().copy()
+
+
+SOURCES = [dict(x=1.2, y=1.1, z=0.7, m=[0.3, 1.0, -0.2, -0.5, 0.4, 0.2], t0=0.12, freq=6.0, tfkind=L.TF_RICKER),
+           dict(x=0.9, y=1.5, z=0.35, m=[1.0, 0.0, 0.5, -1.0, 0.0, 0.0], t0=0.1, freq=20.0, tfkind=L.TF_GAUSSIAN),
+           dict(x=0.9, y=1.5, z=0.35, m=[0.2, 0.1, 0.0, 0.3, 0.0, -0.5], t0=0.0, freq=8.0, tfkind=L.TF_C6SMOOTHBUMP)]
+
+
+@pytest.mark.parametrize("mode,corder,bc", [
+    (0, 1, [2, 2, 2, 2, 0, 2]),
+    (1, 1, [2, 2, 2, 2, 0, 2]),
+    (1, 0, [2, 2, 2, 2, 0, 2]),
+    (0, 1, [2, 2, 2, 2, 0, 0]),
+    (0, 1, [2, 2, 2, 2, 2, 2]),
+    (0, 1, [3, 3, 3, 3, 0, 1]),
+])
+def test_ew_steps_match_oracle(torch, mode, corder, bc):
+    from oracle_loop import OracleRun
+    from sw4lite_amd import EW
+    nx, ny, nz, h = 30, 28, 26, 0.08
+    rho, mu, la = layered_material(nx, ny, nz, h, 5)
+    ref = OracleRun(nx, ny, nz, h, bc, rho, mu, la, SOURCES, corder=corder, sg_gp=6, nsteps=12)
+    e = EW(nx, ny, nz, h, bc, corder=corder, sg_gp_thickness=6)
+    e.set_material(rho, mu, la)
+    for s in SOURCES:
+        e.add_moment_source(s["x"], s["y"], s["z"], s["m"], s["t0"], s["freq"], s["tfkind"])
+    e.setup(nsteps=12)
+    inf = e.info()
+    assert inf.dt == pytest.approx(ref.dt, rel=1e-15)
+    src = e.sources()
+    assert np.array_equal(src["ijk"], ref.src[0])
+    assert relerr(src["force"], ref.src[1]) < 1e-15
+    for _ in range(3):
+        e.timesteploop(4, mode=mode)
+        ref.step(4)
+        u, um = e.get_solution()
+        err = relerr(u, ref.U)
+        print("EW mode", mode, "corder", corder, "bc", bc, "cycle", ref.cycle, "relerr", err, flush=True)
+        assert err < TOL
+        assert relerr(um, ref.Um) < TOL
+    e.close()
