@@ -1,20 +1,37 @@
 """The reference time-step loop (EW::timesteploop, EW.C:2527-2842) driven on
 the CPU with the oracle kernels.  Set-up products (supergrid arrays, ghost
 windows, grid-point sources) come from the library's host-side set-up calls,
-which need no GPU.  Test infrastructure only."""
+which need no GPU.
+
+TEST INFRASTRUCTURE ONLY (tests/ and the cpu_baseline leg of bench.py).
+With kernels="reference" the two stencil stages (rhs4sg_rev, addsgd4) run
+the reference's own code compiled by oracle/Makefile into oracle/_ref."""
 import numpy as np
 
-from common import grid_box, npts
 from oracle import oracle as O
 from sw4lite_amd import ew
 from sw4lite_amd import lib as L
 
 
+def grid_box(nx, ny, nz):
+    return (-1, nx + 2, -1, ny + 2, -1, nz + 2)
+
+
+def npts(box):
+    ib, ie, jb, je, kb, ke = box
+    return (ie - ib + 1) * (je - jb + 1) * (ke - kb + 1)
+
+
 class OracleRun:
     def __init__(self, nx, ny, nz, h, bc, rho, mu, la, sources, corder=1, sg_gp=20, beta=0.02, cfl=1.3,
-                 tmax=0.0, nsteps=0):
+                 tmax=0.0, nsteps=0, kernels="oracle"):
         """rho, mu, la: arrays over the box (npts) or scalars.  sources:
         list of dicts x,y,z,m(6),t0,freq,tfkind."""
+        self.stencil_lib = None
+        if kernels == "reference":
+            self.stencil_lib = O.reference()
+            if self.stencil_lib is None:
+                raise RuntimeError("oracle/_ref/libsw4ref.so is not built")
         self.desc = L.grid_desc(nx, ny, nz, h, bc, corder=corder, sg_gp_thickness=sg_gp, sg_damping=beta, cfl=cfl)
         self.nx, self.ny, self.nz, self.h, self.bc, self.corder, self.beta = nx, ny, nz, h, list(bc), corder, beta
         self.box = grid_box(nx, ny, nz)
@@ -54,12 +71,13 @@ class OracleRun:
             self.src = (ijk[order], frc[order], np.array(fq)[order], np.array(t0)[order], np.array(tf)[order])
         else:
             self.src = None
-        self.U = np.zeros(3 * n)
-        self.Um = np.zeros(3 * n)
-        self.Up = np.zeros(3 * n)
-        self.Lu = np.zeros(3 * n)
-        self.F = np.zeros(3 * n)
-        self.Uacc = np.zeros(3 * n)
+        # every array touched in parallel before stepping (EW.C:2390-2398)
+        self.U = O.filled(3 * n)
+        self.Um = O.filled(3 * n)
+        self.Up = O.filled(3 * n)
+        self.Lu = O.filled(3 * n)
+        self.F = O.filled(3 * n)
+        self.Uacc = O.filled(3 * n)
         self.t = 0.0
         self.cycle = 0
         ni, nj = nx + 4, ny + 4
@@ -82,7 +100,7 @@ class OracleRun:
     def rhs(self, u):
         sg = self.sg
         O.rhs4sg(self.corder, self.box, self.nz, self.onesided, self.acof, self.bope, self.ghcof, self.Lu, u, self.mu,
-                 self.la, self.h, sg["strx"], sg["stry"], sg["strz"])
+                 self.la, self.h, sg["strx"], sg["stry"], sg["strz"], lib=self.stencil_lib)
 
     def enforce_bc(self, u):
         O.bcfortsg(self.corder, self.box, self.wind, self.nx, self.ny, self.nz, u, self.h, self.bc, self.sbop,
@@ -101,7 +119,8 @@ class OracleRun:
             self.rhs(self.Uacc)
             O.corrfort(self.corder, self.box, self.Up, self.Lu, self.F, self.rho, dt * dt * dt * dt)
             O.addsgd4(self.corder, self.box, self.Up, self.U, self.Um, self.rho, sg["dcx"], sg["dcy"], sg["dcz"],
-                      sg["strx"], sg["stry"], sg["strz"], sg["cox"], sg["coy"], sg["coz"], self.beta)
+                      sg["strx"], sg["stry"], sg["strz"], sg["cox"], sg["coy"], sg["coz"], self.beta,
+                      lib=self.stencil_lib)
             self.enforce_bc(self.Up)
             self.t += dt
             self.cycle += 1

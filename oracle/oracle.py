@@ -67,6 +67,13 @@ def rhs4sg(corder, box, nk, onesided, acof, bope, ghcof, lu, u, mu, la, h, strx,
                        D(h), _p(strx), _p(stry), _p(strz))
 
 
+def filled(n, v=0.0):
+    """A float64 array of n entries, first touched in parallel."""
+    a = np.empty(n)
+    oracle().orc_fill(C.c_ssize_t(n), _p(a), D(v))
+    return a
+
+
 def predfort(corder, box, up, u, um, lu, fo, rho, dt2):
     oracle().orc_predfort(*box, _p(up), _p(u), _p(um), _p(lu), _p(fo), _p(rho), D(dt2), corder)
 

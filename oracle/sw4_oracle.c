@@ -641,3 +641,11 @@ void orc_exact_moment(int ib, int ie, int jb, int je, int kb, int ke, double h, 
             for (int c = 0; c < 3; c++) out[V(&b, c, i, j, k)] = res[c];
          }
 }
+
+/* Parallel first-touch fill, as EW::timesteploop does with set_value(0.0)
+ * on every array before stepping (EW.C:2390-2398). */
+void orc_fill(ptrdiff_t n, double *a, double v)
+{
+#pragma omp parallel for schedule(static)
+   for (ptrdiff_t q = 0; q < n; q++) a[q] = v;
+}

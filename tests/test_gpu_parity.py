@@ -153,7 +153,7 @@ SOURCES = [dict(x=1.2, y=1.1, z=0.7, m=[0.3, 1.0, -0.2, -0.5, 0.4, 0.2], t0=0.12
     (0, 1, [3, 3, 3, 3, 0, 1]),
 ])
 def test_ew_steps_match_oracle(torch, mode, corder, bc):
-    from oracle_loop import OracleRun
+    from oracle.loop import OracleRun
     from sw4lite_amd import EW
     nx, ny, nz, h = 30, 28, 26, 0.08
     rho, mu, la = layered_material(nx, ny, nz, h, 5)

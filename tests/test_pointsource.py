@@ -34,7 +34,7 @@ def test_deck_parameters_match_fixture():
 
 
 def test_pointsource_error_cpu_oracle():
-    from oracle_loop import OracleRun
+    from oracle.loop import OracleRun
     d = DECK
     mu = d["rho"] * d["cs"] ** 2
     la = d["rho"] * d["cp"] ** 2 - 2 * mu
