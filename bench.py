@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-nz", type=int, default=128, help="depth of the CPU-baseline slab")
     p.add_argument("--cpu-steps", type=int, default=3)
+    p.add_argument("--variant", type=int, default=1, help="interior sweep kernels: 0 thread/point, 1 LDS k-march")
+    p.add_argument("--kchunk", type=int, default=32, help="planes per workgroup of the LDS sweeps")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also launch a plain device copy of 1 GiB (PMC scaling)")
     return p.parse_args()
@@ -161,6 +163,7 @@ def main():
     srcs = sources(n, h, seed=2)
     rho, mu, la = make_material(torch, n, h, seed=2)
     e = EW(n, n, n, h, [L.BC_SUPERGRID] * 4 + [L.BC_FREE, L.BC_SUPERGRID], corder=1, sg_gp_thickness=30)
+    e.set_kernel_variant(a.variant, a.kchunk)
     e.set_material(rho, mu, la)
     del rho, mu, la
     torch.cuda.empty_cache()
@@ -208,7 +211,7 @@ def main():
         "config": {
             "workload": "%d^3 Cartesian FP64 step, h=20 m, layered+lognormal material, free top, supergrid 30, "
                         "4 double couples (Ricker 5 Hz)" % n,
-            "grid": [n, n, n], "h": h, "dt": inf.dt, "sources": len(srcs), "grid_point_sources": inf.nsources,
+            "grid": [n, n, n], "h": h, "dt": inf.dt, "kernel_variant": a.variant, "kchunk": a.kchunk, "sources": len(srcs), "grid_point_sources": inf.nsources,
             "step_hbm_fraction": value * STEP_BYTES_PER_PT / (HBM_PEAK_GBS * 1e9),
             "stage_ms_per_step": {k: v / a.steps for k, v in zip(
                 ["pred_interior", "pred_boundary_rows_shell", "pred_sources_bc", "corr_interior",
@@ -218,7 +221,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "k_corr_interior", "bytes_per_launch": corr_pts * CORR_BYTES_PER_PT,
+            "kernel": "k_sweep_lds<true>" if a.variant == 1 else "k_corr_interior", "bytes_per_launch": corr_pts * CORR_BYTES_PER_PT,
             "avg_launch_ms": corr_ms,
         },
         "cpu_baseline": None,

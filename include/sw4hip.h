@@ -146,6 +146,10 @@ int sw4hip_ew_create(const sw4hip_grid_desc* g, sw4hip_ew** ew);
 int sw4hip_ew_destroy(sw4hip_ew* ew);
 /* Stream used by every later call on this object (hipStream_t). */
 int sw4hip_ew_set_stream(sw4hip_ew* ew, void* stream);
+/* Interior kernels of the fused sweeps: variant 0 one thread per point,
+ * variant 1 (default) LDS-staged k-marching tiles with kchunk planes per
+ * workgroup (default 32).  Both give the same results to rounding. */
+int sw4hip_ew_set_kernel_variant(sw4hip_ew* ew, int variant, int kchunk);
 /* rho, mu, lambda over the whole box incl. ghosts; on_device selects whether
  * the pointers are device (1) or host (0) arrays. */
 int sw4hip_ew_set_material(sw4hip_ew* ew, const double* rho, const double* mu,
@@ -183,6 +187,14 @@ int sw4hip_ew_step_timed(sw4hip_ew* ew, int nsteps, double* stage_ms);
 /* Analytic point-source error of the current U (EW::exactSol +
  * normOfDifference, EW.C:3650-3669, 4526-4610). */
 int sw4hip_ew_point_source_error(sw4hip_ew* ew, double* linf, double* l2, double* solinf);
+/* A receiver at the grid point nearest to (x,y,z) (EW::computeNearestGridPoint,
+ * EW.C:5149-5180); mode 0 records displacement, 1 velocity (up-um)/(2dt)
+ * (EW::extractRecordData, EW.C:5854-5875).  One sample at t=0 and one per
+ * step, up to the step count given to setup.  Must precede setup. */
+int sw4hip_ew_add_receiver(sw4hip_ew* ew, double x, double y, double z, int mode, int* index);
+/* Samples recorded so far: out[sample][receiver][3] (host buffer of
+ * nsamples*nreceivers*3 doubles, or NULL to query *nsamples). */
+int sw4hip_ew_get_receivers(sw4hip_ew* ew, int* nsamples, double* out);
 /* Device pointers of the state: which = 0 U, 1 Um, 2 rho, 3 mu, 4 lambda. */
 int sw4hip_ew_device_array(sw4hip_ew* ew, int which, double** ptr);
 int sw4hip_ew_sync(sw4hip_ew* ew);

@@ -17,6 +17,7 @@ LIB = os.path.join(HERE, "libsw4hip.so")
 SOURCES = [
     "kernels_basic.hip",
     "kernels_step.hip",
+    "kernels_fast.hip",
     "sbp_coefficients.cpp",
     "setup.cpp",
     "ew.cpp",

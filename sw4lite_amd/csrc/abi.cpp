@@ -299,6 +299,16 @@ int sw4hip_ew_set_stream(sw4hip_ew* h, void* stream)
    return OK;
 }
 
+int sw4hip_ew_set_kernel_variant(sw4hip_ew* h, int variant, int kchunk)
+{
+   EWCHK(h);
+   if (variant != 0 && variant != 1) return set_error(E_INVAL, "variant must be 0 or 1");
+   if (kchunk < 1 || kchunk > 4096) return set_error(E_INVAL, "kchunk must be in [1, 4096]");
+   h->ew->m_variant = variant;
+   h->ew->m_kchunk = kchunk;
+   return OK;
+}
+
 int sw4hip_ew_set_material(sw4hip_ew* h, const double* rho, const double* mu, const double* la, int on_device)
 {
    EWCHK(h);
@@ -426,6 +436,18 @@ int sw4hip_ew_point_source_error(sw4hip_ew* h, double* linf, double* l2, double*
    EWCHK(h);
    if (!linf || !l2 || !solinf) return set_error(E_INVAL, "null output");
    return h->ew->point_source_error(*linf, *l2, *solinf);
+}
+
+int sw4hip_ew_add_receiver(sw4hip_ew* h, double x, double y, double z, int mode, int* index)
+{
+   EWCHK(h);
+   return h->ew->add_receiver(x, y, z, mode, index);
+}
+
+int sw4hip_ew_get_receivers(sw4hip_ew* h, int* nsamples, double* out)
+{
+   EWCHK(h);
+   return h->ew->get_receivers(nsamples, out);
 }
 
 int sw4hip_ew_device_array(sw4hip_ew* h, int which, double** ptr)

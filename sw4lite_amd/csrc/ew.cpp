@@ -209,7 +209,68 @@ int EW::setup(double tmax, int nsteps)
    mT = 0;
    mCycle = 0;
    m_setup_done = true;
+   // receivers: one sample for the initial data and one per step
+   // (EW.C:2463-2480 records U at t = 0)
+   const int nrec = (int)m_rec.size() / 4;
+   if (nrec > 0) {
+      m_maxsamples = std::max(mNumberOfTimeSteps, 0) + 1;
+      void* p = nullptr;
+      if (hipMalloc(&p, sizeof(int) * m_rec.size()) != hipSuccess) return set_error(E_NOMEM, "EW: receiver table");
+      m_allocs.push_back(p);
+      m_drec = static_cast<int*>(p);
+      if ((rc = check_hip(hipMemcpy(m_drec, m_rec.data(), sizeof(int) * m_rec.size(), hipMemcpyHostToDevice),
+                          "receiver upload")))
+         return rc;
+      if (hipMalloc(&p, sizeof(double) * 3 * nrec * (size_t)m_maxsamples) != hipSuccess)
+         return set_error(E_NOMEM, "EW: receiver buffer");
+      m_allocs.push_back(p);
+      m_drecdata = static_cast<double*>(p);
+      m_nsamples = 0;
+      if ((rc = record(mU, mUm))) return rc;
+   }
    return OK;
+}
+
+int EW::add_receiver(double x, double y, double z, int mode, int* index)
+{
+   if (m_setup_done) return set_error(E_STATE, "EW: receivers must be added before setup");
+   if (mode != 0 && mode != 1) return set_error(E_INVAL, "EW: receiver mode must be 0 (displacement) or 1 (velocity)");
+   const GridSetup& g = m_opt.grid;
+   // EW::computeNearestGridPoint (EW.C:5149-5180), one Cartesian grid
+   int i = (int)std::floor(x / g.h) + 1;
+   if (x - (i - 0.5) * g.h > 0.) i++;
+   int j = (int)std::floor(y / g.h) + 1;
+   if (y - (j - 0.5) * g.h > 0.) j++;
+   int k = (int)std::floor((z - g.zmin) / g.h) + 1;
+   if (z - (g.zmin + (k - 0.5) * g.h) > 0.) k++;
+   if (i < 1 || i > g.nx || j < 1 || j > g.ny || k < 1 || k > g.nz)
+      return set_error(E_INVAL, "EW: receiver outside the grid");
+   if (index) *index = (int)m_rec.size() / 4;
+   m_rec.insert(m_rec.end(), {i, j, k, mode});
+   return OK;
+}
+
+int EW::record(const Sarray& Up, const Sarray& Um)
+{
+   const int nrec = (int)m_rec.size() / 4;
+   if (nrec == 0 || !m_drecdata) return OK;
+   if (m_nsamples >= m_maxsamples) return OK;  // steps beyond the set-up count are not recorded
+   const int rc = launch_record(m_box, nrec, m_drec, Up.dev, Um.dev, mDt,
+                                m_drecdata + (size_t)3 * nrec * m_nsamples, m_stream);
+   if (rc == OK) m_nsamples++;
+   return rc;
+}
+
+int EW::get_receivers(int* nsamples, double* out)
+{
+   const int nrec = (int)m_rec.size() / 4;
+   if (nsamples) *nsamples = m_nsamples;
+   if (!out || nrec == 0 || m_nsamples == 0) return OK;
+   int rc;
+   if ((rc = check_hip(hipMemcpyAsync(out, m_drecdata, sizeof(double) * 3 * nrec * (size_t)m_nsamples,
+                                      hipMemcpyDeviceToHost, m_stream), "receiver readback")))
+      return rc;
+   return check_hip(hipStreamSynchronize(m_stream), "receiver readback");
 }
 
 // ---------------------------------------------------------------------------
@@ -288,7 +349,8 @@ int EW::step_reference(int n)
           (rc = evalPredictor(mUp, mU, mUm, mRho, mLu, mF)) || (rc = enforceBC(mUp, mMu, mLambda, t + mDt)) ||
           (rc = Force(t, mF, true)) || (rc = evalDpDmInTime(mUp, mU, mUm, mUacc)) ||
           (rc = evalRHS(mUacc, mMu, mLambda, mLu)) || (rc = evalCorrector(mUp, mRho, mLu, mF)) ||
-          (rc = addSuperGridDamping(mUp, mU, mUm, mRho)) || (rc = enforceBC(mUp, mMu, mLambda, t + mDt)))
+          (rc = addSuperGridDamping(mUp, mU, mUm, mRho)) || (rc = enforceBC(mUp, mMu, mLambda, t + mDt)) ||
+          (rc = record(mUp, mUm)))
          return rc;
       mT += mDt;
       mCycle++;
@@ -319,14 +381,15 @@ int EW::step_fused(int n, double* stage_ms)
       const double t = mT, dt = mDt;
       if ((rc = mark(s, 0)) ||
           (rc = launch_pred_fused(m_box, g.nz, m_onesided, m_dsbp, mU.dev, mUm.dev, mMu.dev, mLambda.dev, mRho.dev,
-                                  c, mUp.dev, g.h, dt, m_stream, [&]() { return mark(s, 1); })) ||
+                                  c, mUp.dev, g.h, dt, m_stream, [&]() { return mark(s, 1); }, m_variant, m_kchunk)) ||
           (rc = mark(s, 2)) || (rc = launch_src_apply(m_box, st, t, 0, dt * dt, mRho.dev, mUp.dev, m_stream)) ||
           (rc = enforceBC(mUp, mMu, mLambda, t + dt)) || (rc = mark(s, 3)) ||
           (rc = launch_corr_fused(m_box, g.nz, m_onesided, m_dsbp, mUp.dev, mU.dev, mUm.dev, mMu.dev, mLambda.dev,
-                                  mRho.dev, c, m_opt.beta, mUp2.dev, g.h, dt, m_stream, [&]() { return mark(s, 4); })) ||
+                                  mRho.dev, c, m_opt.beta, mUp2.dev, g.h, dt, m_stream, [&]() { return mark(s, 4); },
+                                  m_variant, m_kchunk)) ||
           (rc = mark(s, 5)) ||
           (rc = launch_src_apply(m_box, st, t, 1, dt * dt * dt * dt / 12, mRho.dev, mUp2.dev, m_stream)) ||
-          (rc = enforceBC(mUp2, mMu, mLambda, t + dt)) || (rc = mark(s, 6)))
+          (rc = enforceBC(mUp2, mMu, mLambda, t + dt)) || (rc = mark(s, 6)) || (rc = record(mUp2, mUm)))
          break;
       mT += dt;
       mCycle++;

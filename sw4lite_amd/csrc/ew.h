@@ -70,6 +70,13 @@ public:
    int step_fused(int n, double* stage_ms = nullptr);
    int point_source_error(double& linf, double& l2, double& solinf);
 
+   // receivers (TimeSeries, EW.C:5854-5875): mode 0 displacement, 1 velocity
+   int add_receiver(double x, double y, double z, int mode, int* index);
+   int record(const Sarray& Up, const Sarray& Um);  // one sample, after a step
+   int get_receivers(int* nsamples, double* out);   // out[sample][rec][3]
+   std::vector<int> m_rec;                          // i, j, k, mode per receiver
+   int m_nsamples = 0;
+
    // state access
    Sarray mU, mUm, mUp, mUp2, mRho, mMu, mLambda;
    Sarray mLu, mF, mUacc;  // only for step_reference
@@ -85,6 +92,8 @@ public:
    std::vector<GridPointSource> m_gps;
    std::vector<int> m_first;
    hipStream_t m_stream = nullptr;
+   int m_variant = 1;   // interior sweep kernels: 0 thread per point, 1 LDS-staged k-march
+   int m_kchunk = 32;   // planes per workgroup of the LDS sweeps
 
 private:
    int alloc(Sarray& a, int nc);
@@ -97,6 +106,9 @@ private:
    SbpTables* m_dsbp = nullptr;
    void* m_dsrc = nullptr;   // packed source table
    double* m_dred = nullptr; // reduction partials
+   int* m_drec = nullptr;     // receiver table (device)
+   double* m_drecdata = nullptr;
+   int m_maxsamples = 0;
    int m_nred = 1024;
    bool m_setup_done = false, m_material_set = false;
    std::vector<void*> m_allocs;

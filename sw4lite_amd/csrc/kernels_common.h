@@ -3,6 +3,7 @@
 // reference arithmetic of one stencil (file:line cited) for a single grid
 // point; layouts are selected at compile time by CO.
 #pragma once
+#include "stencil_body.h"
 #include "sw4hip_internal.h"
 
 namespace sw4amd {
@@ -148,6 +149,28 @@ __device__ __forceinline__ double mixed(const Box& b, int i, int j, int k,
    return d4(g[0], g[1], g[2], g[3]);
 }
 
+// Global-memory accessor of rhs4_interior_body (stencil_body.h).
+template <int CO, class UA>
+struct GlobalSten {
+   const Box& b;
+   int i, j, k;
+   const UA& uu;
+   const double* __restrict__ mu_;
+   const double* __restrict__ la_;
+   const double* __restrict__ strx_;
+   const double* __restrict__ stry_;
+   const double* __restrict__ strz_;
+   __device__ __forceinline__ double u(int c, int di, int dj, int dk) const
+   {
+      return uu[vidx<CO>(b, c, i + di, j + dj, k + dk)];
+   }
+   __device__ __forceinline__ double mu(int di, int dj, int dk) const { return mu_[sidx(b, i + di, j + dj, k + dk)]; }
+   __device__ __forceinline__ double la(int di, int dj, int dk) const { return la_[sidx(b, i + di, j + dj, k + dk)]; }
+   __device__ __forceinline__ double sx(int d) const { return strx_[i + d - b.ib]; }
+   __device__ __forceinline__ double sy(int d) const { return stry_[j + d - b.jb]; }
+   __device__ __forceinline__ double sz(int d) const { return strz_[k + d - b.kb]; }
+};
+
 // Interior point of L(u) without the 1/h^2 factor (rhs4sg_rev.C:111-347).
 template <int CO, class UA>
 __device__ __forceinline__ void rhs_interior_point(const Box& b, int i, int j, int k, const UA& u,
@@ -157,16 +180,8 @@ __device__ __forceinline__ void rhs_interior_point(const Box& b, int i, int j, i
                                                    const double* __restrict__ stry,
                                                    const double* __restrict__ strz, double r[3])
 {
-   const double i144 = 1.0 / 144;
-   const double sx = strx[i - b.ib], sy = stry[j - b.jb], sz = strz[k - b.kb];
-   second_derivs<CO, true, UA>(b, i, j, k, u, mu, la, strx, stry, strz, r);
-   const double sxy = sx * sy * i144, sxz = sx * sz * i144, syz = sy * sz * i144;
-   r[0] += sxy * mixed<CO, UA>(b, i, j, k, la, u, 1, 0, 1) + sxz * mixed<CO, UA>(b, i, j, k, la, u, 2, 0, 2) +
-           sxy * mixed<CO, UA>(b, i, j, k, mu, u, 1, 1, 0) + sxz * mixed<CO, UA>(b, i, j, k, mu, u, 2, 2, 0);
-   r[1] += sxy * mixed<CO, UA>(b, i, j, k, mu, u, 0, 0, 1) + sxy * mixed<CO, UA>(b, i, j, k, la, u, 0, 1, 0) +
-           syz * mixed<CO, UA>(b, i, j, k, la, u, 2, 1, 2) + syz * mixed<CO, UA>(b, i, j, k, mu, u, 2, 2, 1);
-   r[2] += sxz * mixed<CO, UA>(b, i, j, k, mu, u, 0, 0, 2) + syz * mixed<CO, UA>(b, i, j, k, mu, u, 1, 1, 2) +
-           sxz * mixed<CO, UA>(b, i, j, k, la, u, 0, 2, 0) + syz * mixed<CO, UA>(b, i, j, k, la, u, 1, 2, 1);
+   GlobalSten<CO, UA> s{b, i, j, k, u, mu, la, strx, stry, strz};
+   rhs4_interior_body(s, r);
 }
 
 // SBP boundary row kr (1..6) at grid plane k; kbase/sgn map boundary-relative

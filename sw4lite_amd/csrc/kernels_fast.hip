@@ -1,0 +1,176 @@
+// kernels_fast.hip -- LDS-staged sweeps of the fused time step.
+//
+// A workgroup owns a 32 x 8 tile of (i,j) columns and marches along k over a
+// chunk of planes.  The five displacement planes k-2..k+2 the 4th-order
+// stencil touches are kept, with their 2-point (i,j) halo, in an LDS ring;
+// each new plane is loaded once from HBM (coalesced rows of 36 doubles) and
+// read ~135 times from LDS.  mu and lambda of the current plane sit in LDS for
+// the in-plane terms, the column values k+-1, k+-2 rotate in registers.  The
+// corrector stages (Up - 2U + Um)/dt^2 instead of U, so the acceleration is
+// never stored.  The arithmetic is rhs4_interior_body, shared with the
+// global-memory kernels (same results to rounding).
+#include "kernels_common.h"
+#include "kernels_fast.h"
+#include "stencil_body.h"
+
+namespace sw4amd {
+
+namespace {
+constexpr int TX = 32, TY = 8, HX = TX + 4, HY = TY + 4, NT = TX * TY, PL = HX * HY;
+inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
+}  // namespace
+
+struct LdsSten {
+   const double* p0;  // ring slot of plane k-2, offset to this thread's point
+   const double* p1;
+   const double* p2;
+   const double* p3;
+   const double* p4;
+   const double* pm;  // mu of plane k at this point
+   const double* pl;
+   double muc[5], lac[5];
+   double sxv[5], syv[5], szv[5];
+   __device__ __forceinline__ double u(int c, int di, int dj, int dk) const
+   {
+      const double* p = dk == -2 ? p0 : dk == -1 ? p1 : dk == 0 ? p2 : dk == 1 ? p3 : p4;
+      return p[c * PL + dj * HX + di];
+   }
+   __device__ __forceinline__ double mu(int di, int dj, int dk) const { return dk ? muc[dk + 2] : pm[dj * HX + di]; }
+   __device__ __forceinline__ double la(int di, int dj, int dk) const { return dk ? lac[dk + 2] : pl[dj * HX + di]; }
+   __device__ __forceinline__ double sx(int d) const { return sxv[d + 2]; }
+   __device__ __forceinline__ double sy(int d) const { return syv[d + 2]; }
+   __device__ __forceinline__ double sz(int d) const { return szv[d + 2]; }
+};
+
+// CORR = false: out = 2u - um + dt2/rho cof L(u)                (predictor)
+// CORR = true:  out = up + dt4/(12 rho) cof L((up-2u+um) dt2i) - damping(u,um)
+template <bool CORR>
+__global__ void __launch_bounds__(NT, CORR ? 1 : 2)
+k_sweep_lds(Box b, int kbeg, int kend, int kc, const double* __restrict__ upin, const double* __restrict__ u,
+            const double* __restrict__ um, const double* __restrict__ mu, const double* __restrict__ la,
+            const double* __restrict__ rho, SgCoef sg, double* __restrict__ out, double cof, double dt2i,
+            double cupd, double beta)
+{
+   __shared__ double su[5 * 3 * PL];
+   __shared__ double sm[PL], sl[PL];
+   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+   const int ia = b.ib + blockIdx.x * TX, ja = b.jb + blockIdx.y * TY;  // halo corner of the tile
+   const int i = ia + 2 + tx, j = ja + 2 + ty;
+   const bool active = i <= b.ie - 2 && j <= b.je - 2;
+   const int k0 = kbeg + blockIdx.z * kc;
+   const int k1 = min(kend, k0 + kc - 1);
+   const long ni = b.ni, nij = b.nij, npts = b.npts;
+   const long col = active ? (i - b.ib) + ni * (j - b.jb) : 0;
+   const int o = (ty + 2) * HX + tx + 2;
+
+   auto stage = [&](int kp, int slot) {
+      const long pb = nij * (kp - b.kb);
+      double* dst = su + slot * 3 * PL;
+      for (int e = tid; e < 3 * PL; e += NT) {
+         const int c = e / PL, r = e - c * PL, jj = r / HX, ii = r - jj * HX;
+         const int gi = ia + ii, gj = ja + jj;
+         double v = 0.0;
+         if (gi <= b.ie && gj <= b.je) {
+            const long q = (gi - b.ib) + ni * (gj - b.jb) + pb + npts * c;
+            v = CORR ? dt2i * (upin[q] - 2 * u[q] + um[q]) : u[q];
+         }
+         dst[e] = v;
+      }
+   };
+
+   LdsSten s;
+#pragma unroll
+   for (int q = 0; q < 4; q++) {
+      stage(k0 - 2 + q, q);
+      const long pq = nij * (k0 - 2 + q - b.kb);
+      s.muc[q] = active ? mu[col + pq] : 0.0;
+      s.lac[q] = active ? la[col + pq] : 0.0;
+      s.szv[q] = sg.strz[k0 - 2 + q - b.kb];
+   }
+#pragma unroll
+   for (int q = 0; q < 5; q++) {
+      s.sxv[q] = active ? sg.strx[i - b.ib + q - 2] : 1.0;
+      s.syv[q] = active ? sg.stry[j - b.jb + q - 2] : 1.0;
+   }
+   s.pm = sm + o;
+   s.pl = sl + o;
+
+   for (int k = k0; k <= k1; k++) {
+      const int n = k - k0;
+      const int sl0 = n % 5;
+      stage(k + 2, (sl0 + 4) % 5);
+      const long pbk = nij * (k - b.kb);
+      for (int e = tid; e < PL; e += NT) {
+         const int jj = e / HX, ii = e - jj * HX;
+         const int gi = ia + ii, gj = ja + jj;
+         double vm = 0.0, vl = 0.0;
+         if (gi <= b.ie && gj <= b.je) {
+            const long q = (gi - b.ib) + ni * (gj - b.jb) + pbk;
+            vm = mu[q];
+            vl = la[q];
+         }
+         sm[e] = vm;
+         sl[e] = vl;
+      }
+      {
+         const long pq = nij * (k + 2 - b.kb);
+         s.muc[4] = active ? mu[col + pq] : 0.0;
+         s.lac[4] = active ? la[col + pq] : 0.0;
+         s.szv[4] = sg.strz[k + 2 - b.kb];
+      }
+      __syncthreads();
+      if (active) {
+         s.p0 = su + (sl0 * 3) * PL + o;
+         s.p1 = su + (((sl0 + 1) % 5) * 3) * PL + o;
+         s.p2 = su + (((sl0 + 2) % 5) * 3) * PL + o;
+         s.p3 = su + (((sl0 + 3) % 5) * 3) * PL + o;
+         s.p4 = su + (((sl0 + 4) % 5) * 3) * PL + o;
+         double r[3];
+         rhs4_interior_body(s, r);
+         const long p = col + pbk;
+         const double sc = cupd / rho[p];
+         if (!CORR) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+               const long q = p + npts * c;
+               out[q] = 2 * s.p2[c * PL] - um[q] + sc * (cof * r[c]);
+            }
+         } else {
+            double d[3];
+            addsgd4_point<1>(b, i, j, k, u, um, rho, sg, beta, d);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+               const long q = p + npts * c;
+               out[q] = (upin[q] + sc * (cof * r[c])) - d[c];
+            }
+         }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+         s.muc[q] = s.muc[q + 1];
+         s.lac[q] = s.lac[q + 1];
+         s.szv[q] = s.szv[q + 1];
+      }
+   }
+}
+
+int launch_sweep_lds(bool corr, const Box& b, int k1, int k2, int kc, const double* upin, const double* u,
+                     const double* um, const double* mu, const double* la, const double* rho, const SgCoef& sg,
+                     double* out, double h, double dt, double beta, hipStream_t s)
+{
+   if (k2 < k1) return OK;
+   const long nx = b.ie - b.ib - 3, ny = b.je - b.jb - 3;
+   if (kc <= 0) kc = 32;
+   dim3 grd(cdiv(nx, TX), cdiv(ny, TY), cdiv(k2 - k1 + 1, kc));
+   const double cof = 1.0 / (h * h), dt2i = 1.0 / (dt * dt);
+   if (corr)
+      k_sweep_lds<true><<<grd, NT, 0, s>>>(b, k1, k2, kc, upin, u, um, mu, la, rho, sg, out, cof, dt2i,
+                                           dt * dt * dt * dt / 12, beta);
+   else
+      k_sweep_lds<false><<<grd, NT, 0, s>>>(b, k1, k2, kc, nullptr, u, um, mu, la, rho, sg, out, cof, dt2i,
+                                            dt * dt, beta);
+   return check_hip(hipGetLastError(), corr ? "LDS corrector launch" : "LDS predictor launch");
+}
+
+}  // namespace sw4amd

@@ -22,12 +22,13 @@ struct NormWindow {
 int launch_pred_fused(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp,
                       const double* u, const double* um, const double* mu, const double* la,
                       const double* rho, const SgCoef& sg, double* up, double h, double dt,
-                      hipStream_t s, const std::function<int()>& after_interior);
+                      hipStream_t s, const std::function<int()>& after_interior, int variant = 1,
+                      int kchunk = 32);
 int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp,
                       const double* up, const double* u, const double* um, const double* mu,
                       const double* la, const double* rho, const SgCoef& sg, double beta,
                       double* upo, double h, double dt, hipStream_t s,
-                      const std::function<int()>& after_interior);
+                      const std::function<int()>& after_interior, int variant = 1, int kchunk = 32);
 int launch_src_apply(const Box& b, const SourceTable& st, double t, int deriv2, double coef,
                      const double* rho, double* up, hipStream_t s);
 int launch_maxeig(long n, const double* mu, const double* la, const double* rho, double* partial,
@@ -36,5 +37,7 @@ int launch_exact(const Box& b, const ExactParams& P, double* out, hipStream_t s)
 int launch_solerr(const Box& b, const NormWindow& w, const double* uex, const double* u,
                   double* partial, int npartial, double* out, hipStream_t s);
 long shell_count(const Box& b);
+int launch_record(const Box& b, int nrec, const int* rec, const double* up, const double* um, double dt,
+                  double* out, hipStream_t s);
 
 }  // namespace sw4amd

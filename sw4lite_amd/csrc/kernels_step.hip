@@ -12,6 +12,7 @@
 //     (EW::get_exact_point_source, EW.C:3899-4523) and the error norms
 //     (EW::normOfDifference, EW.C:4526-4610 / solerr3fort).
 #include "kernels_common.h"
+#include "kernels_fast.h"
 #include "kernels_step.h"
 
 namespace sw4amd {
@@ -422,20 +423,50 @@ __global__ void __launch_bounds__(RED_BLOCK) k_final_norm(int n, const double* _
    }
 }
 
+// ---------------------------------------------------------------------------
+// Receivers, EW::extractRecordData (EW.C:5854-5875): displacement (mode 0) or
+// centred velocity (up - um)/(2 dt) (mode 1) at the nearest grid point.
+// rec[4r..4r+3] = i, j, k, mode; out[3r+c] for this sample.
+// ---------------------------------------------------------------------------
+__global__ void k_record(Box b, int nrec, const int* __restrict__ rec, const double* __restrict__ up,
+                         const double* __restrict__ um, double dt, double* __restrict__ out)
+{
+   const int r = blockIdx.x * blockDim.x + threadIdx.x;
+   if (r >= nrec) return;
+   const long p = sidx(b, rec[4 * r], rec[4 * r + 1], rec[4 * r + 2]);
+   const int mode = rec[4 * r + 3];
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      const long q = p + b.npts * c;
+      out[3 * r + c] = mode == 1 ? (up[q] - um[q]) / (2 * dt) : up[q];
+   }
+}
+
 // ===========================================================================
 // launchers
 // ===========================================================================
+int launch_record(const Box& b, int nrec, const int* rec, const double* up, const double* um, double dt,
+                  double* out, hipStream_t s)
+{
+   if (nrec <= 0) return OK;
+   k_record<<<cdiv(nrec, 64), 64, 0, s>>>(b, nrec, rec, up, um, dt, out);
+   return check_hip(hipGetLastError(), "receiver launch");
+}
+
 int launch_pred_fused(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp,
                       const double* u, const double* um, const double* mu, const double* la,
                       const double* rho, const SgCoef& sg, double* up, double h, double dt,
-                      hipStream_t s, const std::function<int()>& after_interior)
+                      hipStream_t s, const std::function<int()>& after_interior, int variant, int kchunk)
 {
    const double cof = 1.0 / (h * h), dt2 = dt * dt;
    const int k1 = onesided[4] == 1 ? 7 : b.kb + 2;
    const int k2 = onesided[5] == 1 ? nk - 6 : b.ke - 2;
    const int nii = b.ie - b.ib - 3, njj = b.je - b.jb - 3;
    dim3 blk(64, 4, 1);
-   if (k2 >= k1) {
+   if (k2 >= k1 && variant == 1) {
+      const int rc = launch_sweep_lds(false, b, k1, k2, kchunk, nullptr, u, um, mu, la, rho, sg, up, h, dt, 0.0, s);
+      if (rc) return rc;
+   } else if (k2 >= k1) {
       dim3 grd(cdiv(nii, 64), cdiv(njj, 4), k2 - k1 + 1);
       k_pred_interior<<<grd, blk, 0, s>>>(b, k1, k2, u, um, mu, la, rho, sg.strx, sg.stry, sg.strz, up, cof, dt2);
    }
@@ -457,14 +488,17 @@ int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables
                       const double* up, const double* u, const double* um, const double* mu,
                       const double* la, const double* rho, const SgCoef& sg, double beta,
                       double* upo, double h, double dt, hipStream_t s,
-                      const std::function<int()>& after_interior)
+                      const std::function<int()>& after_interior, int variant, int kchunk)
 {
    const double cof = 1.0 / (h * h), dt2i = 1.0 / (dt * dt), dt4i12 = dt * dt * dt * dt / 12;
    const int k1 = onesided[4] == 1 ? 7 : b.kb + 2;
    const int k2 = onesided[5] == 1 ? nk - 6 : b.ke - 2;
    const int nii = b.ie - b.ib - 3, njj = b.je - b.jb - 3;
    dim3 blk(64, 4, 1);
-   if (k2 >= k1) {
+   if (k2 >= k1 && variant == 1) {
+      const int rc = launch_sweep_lds(true, b, k1, k2, kchunk, up, u, um, mu, la, rho, sg, upo, h, dt, beta, s);
+      if (rc) return rc;
+   } else if (k2 >= k1) {
       dim3 grd(cdiv(nii, 64), cdiv(njj, 4), k2 - k1 + 1);
       k_corr_interior<<<grd, blk, 0, s>>>(b, k1, k2, up, u, um, mu, la, rho, sg, upo, cof, dt2i, dt4i12, beta);
    }

@@ -144,6 +144,10 @@ class EW:
             L.check(self.lib.sw4hip_ew_set_stream(self.h, C.c_void_p(stream)))
 
     # set-up -------------------------------------------------------------
+    def set_kernel_variant(self, variant, kchunk=32):
+        """0: one thread per point; 1: LDS-staged k-marching tiles (default)."""
+        L.check(self.lib.sw4hip_ew_set_kernel_variant(self.h, int(variant), int(kchunk)))
+
     def set_material(self, rho, mu, la):
         on_dev = _is_dev(rho)
         L.check(self.lib.sw4hip_ew_set_material(self.h, _ptr(rho), _ptr(mu), _ptr(la), int(on_dev)))
@@ -154,6 +158,23 @@ class EW:
     def add_moment_source(self, x, y, z, m, t0, freq, tfkind):
         mm = np.asarray(m, dtype=np.float64)
         L.check(self.lib.sw4hip_ew_add_moment_source(self.h, x, y, z, _ptr(mm), t0, freq, tfkind))
+
+    def add_receiver(self, x, y, z, mode=0):
+        """Receiver at the nearest grid point; mode 0 displacement, 1 velocity."""
+        idx = C.c_int(0)
+        L.check(self.lib.sw4hip_ew_add_receiver(self.h, x, y, z, int(mode), C.byref(idx)))
+        self._nrec = getattr(self, "_nrec", 0) + 1
+        return idx.value
+
+    def receivers(self):
+        """Recorded samples, array [nsamples, nreceivers, 3]."""
+        n = C.c_int(0)
+        L.check(self.lib.sw4hip_ew_get_receivers(self.h, C.byref(n), None))
+        nrec = getattr(self, "_nrec", 0)
+        out = np.zeros((n.value, nrec, 3))
+        if n.value and nrec:
+            L.check(self.lib.sw4hip_ew_get_receivers(self.h, C.byref(n), _ptr(out)))
+        return out
 
     def setup(self, tmax=0.0, nsteps=0):
         L.check(self.lib.sw4hip_ew_setup(self.h, tmax, nsteps))

@@ -46,6 +46,7 @@ _SIGS = {
     "sw4hip_ew_create": (I, [C.POINTER(GridDesc), C.POINTER(P)]),
     "sw4hip_ew_destroy": (I, [P]),
     "sw4hip_ew_set_stream": (I, [P, P]),
+    "sw4hip_ew_set_kernel_variant": (I, [P, I, I]),
     "sw4hip_ew_set_material": (I, [P, P, P, P, I]),
     "sw4hip_ew_set_material_const": (I, [P, D, D, D]),
     "sw4hip_ew_add_moment_source": (I, [P, D, D, D, P, D, D, I]),
@@ -58,6 +59,8 @@ _SIGS = {
     "sw4hip_ew_step": (I, [P, I, I]),
     "sw4hip_ew_step_timed": (I, [P, I, P]),
     "sw4hip_ew_point_source_error": (I, [P, DP, DP, DP]),
+    "sw4hip_ew_add_receiver": (I, [P, D, D, D, I, IP]),
+    "sw4hip_ew_get_receivers": (I, [P, IP, P]),
     "sw4hip_ew_device_array": (I, [P, I, C.POINTER(P)]),
     "sw4hip_ew_sync": (I, [P]),
 }
