@@ -129,3 +129,38 @@ def solerr3(box, h, uex, u, zmin, x0, y0, z0, radius, win, corder):
     oracle().orc_solerr3(*box, D(h), _p(uex), _p(u), C.byref(li), C.byref(l2), C.byref(xli), D(zmin), D(x0),
                          D(y0), D(z0), D(radius), *win, corder)
     return li.value, l2.value, xli.value
+
+
+# ---------------------------------------------------------------- curvilinear
+def metric(corder, box, h, z, met, jac, lib=None):
+    """met (4 components), jac from the grid depth z; returns the kernel's
+    status (0, or -1 where z is not increasing in k).  With lib (the reference
+    build) x and y are made from h as the reference reads h off x."""
+    if lib is None:
+        return oracle().orc_metric(*box, D(h), _p(z), _p(met), _p(jac), corder)
+    ib, ie, jb, je, kb, ke = box
+    ni, nj, nk = ie - ib + 1, je - jb + 1, ke - kb + 1
+    x = np.broadcast_to((np.arange(ib, ie + 1) - 1) * h, (nk, nj, ni)).copy()
+    y = np.broadcast_to(((np.arange(jb, je + 1) - 1) * h)[:, None], (nk, nj, ni)).copy()
+    lib.ref_metric.restype = C.c_int
+    return lib.ref_metric(corder, *box, _p(x), _p(y), _p(z), _p(met), _p(jac))
+
+
+def rhs4sgcurv(corder, box, u, mu, la, met, jac, lu, onesided, acof, bope, ghcof, strx, stry, lib=None):
+    os_ = np.ascontiguousarray(onesided, dtype=np.int32)
+    args = [_p(u), _p(mu), _p(la), _p(met), _p(jac), _p(lu), _p(os_), _p(acof), _p(bope), _p(ghcof), _p(strx),
+            _p(stry)]
+    if lib is None:
+        oracle().orc_rhs4sgcurv(*box, *args, corder)
+    else:
+        lib.ref_rhs4sgcurv(corder, *box, *args)
+
+
+def addsgd4c(corder, box, up, u, um, rho, dcx, dcy, strx, stry, jac, cox, coy, beta):
+    oracle().orc_addsgd4c(*box, _p(up), _p(u), _p(um), _p(rho), _p(dcx), _p(dcy), _p(strx), _p(stry), _p(jac),
+                          _p(cox), _p(coy), D(beta), corder)
+
+
+def freesurfcurvisg(corder, box, nz, side, u, mu, la, met, sbop, forcing, strx, stry):
+    oracle().orc_freesurfcurvisg(*box, nz, side, _p(u), _p(mu), _p(la), _p(met), _p(np.ascontiguousarray(sbop)),
+                                 _p(forcing), _p(strx), _p(stry), corder)

@@ -6,12 +6,20 @@
 //   rhs4sg_rev       src/rhs4sg_rev.C:43-48
 //   addsgd4fort      tests/testil/addsgd4fort.C:4-11
 //   addsgd4fort_rev  tests/testil/addsgd4fort_rev.C:4-11
+//   rhs4sgcurv       src/rhs4sgcurv.C:34-39
+//   rhs4sgcurv_rev   src/rhs4sgcurv_rev.C:34-40
+//   metric           tests/testil/metricC.C:5-6
+//   metric_rev       tests/testil/metricC.C:127-128
 typedef double R;
 
 void rhs4sg(int, int, int, int, int, int, int, int*, R*, R*, R*, R*, R*, R*, R*, R, R*, R*, R*);
 void rhs4sg_rev(int, int, int, int, int, int, int, int*, R*, R*, R*, R*, R*, R*, R*, R, R*, R*, R*);
 void addsgd4fort(int, int, int, int, int, int, R*, R*, R*, R*, R*, R*, R*, R*, R*, R*, R*, R*, R*, R);
 void addsgd4fort_rev(int, int, int, int, int, int, R*, R*, R*, R*, R*, R*, R*, R*, R*, R*, R*, R*, R*, R);
+void rhs4sgcurv(int, int, int, int, int, int, R*, R*, R*, R*, R*, R*, int*, R*, R*, R*, R*, R*);
+void rhs4sgcurv_rev(int, int, int, int, int, int, R*, R*, R*, R*, R*, R*, int*, R*, R*, R*, R*, R*);
+int metric(int, int, int, int, int, int, R*, R*, R*, R*, R*);
+int metric_rev(int, int, int, int, int, int, R*, R*, R*, R*, R*);
 
 extern "C" {
 void ref_rhs4sg(int corder, int ib, int ie, int jb, int je, int kb, int ke, int nk, int* onesided,
@@ -29,5 +37,18 @@ void ref_addsgd4(int corder, int ib, int ie, int jb, int je, int kb, int ke, R* 
       addsgd4fort_rev(ib, ie, jb, je, kb, ke, up, u, um, rho, dcx, dcy, dcz, strx, stry, strz, cox, coy, coz, beta);
    else
       addsgd4fort(ib, ie, jb, je, kb, ke, up, u, um, rho, dcx, dcy, dcz, strx, stry, strz, cox, coy, coz, beta);
+}
+void ref_rhs4sgcurv(int corder, int ib, int ie, int jb, int je, int kb, int ke, R* u, R* mu, R* la, R* met,
+                    R* jac, R* lu, int* onesided, R* acof, R* bope, R* ghcof, R* strx, R* stry)
+{
+   if (corder)
+      rhs4sgcurv_rev(ib, ie, jb, je, kb, ke, u, mu, la, met, jac, lu, onesided, acof, bope, ghcof, strx, stry);
+   else
+      rhs4sgcurv(ib, ie, jb, je, kb, ke, u, mu, la, met, jac, lu, onesided, acof, bope, ghcof, strx, stry);
+}
+int ref_metric(int corder, int ib, int ie, int jb, int je, int kb, int ke, R* x, R* y, R* z, R* met, R* jac)
+{
+   return corder ? metric_rev(ib, ie, jb, je, kb, ke, x, y, z, met, jac)
+                 : metric(ib, ie, jb, je, kb, ke, x, y, z, met, jac);
 }
 }

@@ -53,3 +53,23 @@ def relerr(a, b):
     d = np.max(np.abs(np.asarray(a) - np.asarray(b)))
     s = np.max(np.abs(np.asarray(b)))
     return d / s if s > 0 else d
+
+
+def hills_depth(nx, ny, nz, h, seed, nhill=3, amp=0.15):
+    """Grid depth z(i,j,k) over the box: the top is a sum of Gaussian hills
+    (height <= amp * depth), mapped linearly in k to a flat bottom."""
+    rng = np.random.default_rng(seed)
+    box = grid_box(nx, ny, nz)
+    ib, ie, jb, je, kb, ke = box
+    x = (np.arange(ib, ie + 1) - 1) * h
+    y = (np.arange(jb, je + 1) - 1) * h
+    depth = (nz - 1) * h
+    X, Y = np.meshgrid(x, y)  # (nj, ni)
+    top = np.zeros_like(X)
+    for _ in range(nhill):
+        xc, yc = rng.uniform(0.2, 0.8) * x[-1], rng.uniform(0.2, 0.8) * y[-1]
+        L = rng.uniform(0.15, 0.35) * min(x[-1], y[-1])
+        top -= rng.uniform(0.3, 1.0) * amp * depth / nhill * np.exp(-((X - xc) ** 2 + (Y - yc) ** 2) / L ** 2)
+    zeta = (np.arange(kb, ke + 1) - 1) / (nz - 1)
+    z = top[None, :, :] * (1 - zeta[:, None, None]) + depth * zeta[:, None, None]
+    return box, np.ascontiguousarray(z.reshape(-1))

@@ -10,6 +10,10 @@ Run in a container that holds the reference source tree (default
     stored;
   * extracts the exact SBP coefficients from GetStencilCoefficients
     (ew-cfromfort.C:1487-1944) as rational numbers;
+  * evaluates the reference metric (tests/testil/metricC.C:127) and
+    rhs4sgcurv_rev (rhs4sgcurv_rev.C:34) on a grid under seeded Gaussian
+    hills (tests/common.py ``hills_depth``), with and without the top SBP
+    rows (curv.npz / curv.json);
   * copies the point-source accuracy fixture
     (pytest/reference/pointsource/pointsource-h0p04/PointSourceErr.txt) and
     the parameters of its deck (pytest/reference/pointsource/pointsource.in).
@@ -29,7 +33,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
-from common import random_inputs  # noqa: E402
+from common import hills_depth, npts, random_inputs  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
 GRID = (8, 9, 13)
@@ -89,6 +93,30 @@ def checksums(f):
     return {k: float(np.sum(v)) for k, v in sorted(f.items())}
 
 
+CURV_GRID = (10, 9, 13)
+
+
+def curv_golden(lib, acof, bope, ghcof):
+    nx, ny, nz = CURV_GRID
+    out, meta = {}, {"grid": CURV_GRID, "h": H, "cases": {}}
+    for top in (0, 1):
+        seed = 41 + top
+        box, f = random_inputs(seed, nx, ny, nz)
+        _, z = hills_depth(nx, ny, nz, H, seed + 100)
+        n = npts(box)
+        met, jac = np.zeros(4 * n), np.zeros(n)
+        assert O.metric(1, box, H, z, met, jac, lib=lib) == 0
+        lu = np.zeros_like(f["u"])
+        O.rhs4sgcurv(1, box, f["u"], f["mu"], f["la"], met, jac, lu, [0, 0, 0, 0, top, 0], acof, bope, ghcof,
+                     f["strx"], f["stry"], lib=lib)
+        name = "rhs4sgcurv_rev_top%d" % top
+        out[name] = lu
+        meta["cases"][name] = {"seed": seed, "met_sum": float(np.sum(met)), "checksums": checksums(f)}
+    np.savez_compressed(os.path.join(HERE, "curv.npz"), **out)
+    with open(os.path.join(HERE, "curv.json"), "w") as fh:
+        json.dump(meta, fh, indent=1)
+
+
 def main(ref):
     O.build(ref=True)
     lib = O.reference()
@@ -114,6 +142,7 @@ def main(ref):
     out["addsgd4fort_rev"] = up
     meta["cases"]["addsgd4fort_rev"] = {"seed": 21, "checksums": checksums(f)}
     np.savez_compressed(os.path.join(HERE, "kernels.npz"), **out)
+    curv_golden(lib, acof, bope, ghcof)
     with open(os.path.join(HERE, "kernels.json"), "w") as fh:
         json.dump(meta, fh, indent=1)
     # point-source accuracy fixture and its deck parameters
