@@ -5,7 +5,8 @@ Workload (the reference path at scale): FP64 Cartesian 4th-order SBP solver
 on a 512^3 grid, h = 20 m; layered material cs = 1000 -> 3500 m/s linear in
 depth, cp = 1.8 cs, rho = 1700 (cp / 1 km/s)^0.25 kg/m^3, times a seeded
 lognormal perturbation exp(0.05 g) of cs and cp, g a unit-variance field of
-white noise smoothed by a Gaussian of 8 cells; free surface on top, supergrid
+white noise smoothed by three passes of a 17-point box filter per axis (a
+near-Gaussian of 8.5 cells); free surface on top, supergrid
 of 30 points on the other five faces; four seeded random double-couple
 sources with 5 Hz Ricker time functions.  All inputs are synthetic and built
 on the GPU before timing.
@@ -50,37 +51,49 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-nz", type=int, default=128, help="depth of the CPU-baseline slab")
     p.add_argument("--cpu-steps", type=int, default=3)
-    p.add_argument("--variant", type=int, default=1, help="interior sweep kernels: 0 thread/point, 1 LDS k-march")
+    p.add_argument("--variant", type=int, default=1, help="interior sweep kernels: 0 thread/point, 1 LDS k-march, 2 with prefetch, "
+                        "3 prefetch without register cap")
     p.add_argument("--kchunk", type=int, default=32, help="planes per workgroup of the LDS sweeps")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also launch a plain device copy of 1 GiB (PMC scaling)")
+    p.add_argument("--workload", choices=["cartesian", "curvilinear"], default="cartesian",
+                   help="cartesian: the full time step at N^3 (default); curvilinear: the curvilinear-grid "
+                        "kernels (rhs4sgcurv, addsgd4c, free-surface closure) on a 384x384x256 grid under "
+                        "Gaussian hills")
     return p.parse_args()
 
 
-def make_material(torch, n, h, seed):
-    """rho, mu, lambda over the (n+4)^3 box (two ghost points per side)."""
-    dev = torch.device("cuda")
-    m = n + 4
+def make_material(torch, n, h, seed, device="cuda", z=None):
+    """rho, mu, lambda over the box of n = N or (nx, ny, nz) interior points
+    (two ghost points per side); depth from the grid depth field z (same box,
+    flattened) when given, else z = (k-1) h."""
+    dev = torch.device(device)
+    nx, ny, nz = (n, n, n) if isinstance(n, int) else n
+    shp = (nz + 4, ny + 4, nx + 4)
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
-    g = torch.randn((m, m, m), generator=gen, device=dev, dtype=torch.float32)
-    sig = 8.0
-    r = int(4 * sig)
-    x = torch.arange(-r, r + 1, device=dev, dtype=torch.float32)
-    w = torch.exp(-0.5 * (x / sig) ** 2)
-    w = (w / w.sum()).view(1, 1, -1)
+    g = torch.randn(shp, generator=gen, device=dev, dtype=torch.float64)
     F = torch.nn.functional
-    for axis in range(3):  # separable Gaussian smoothing along k, j, i
-        g = g.movedim(axis, 2).contiguous()
-        sh = g.shape
-        g = F.conv1d(g.reshape(-1, 1, sh[2]), w, padding=r).reshape(sh)
-        g = g.movedim(2, axis).contiguous()
+    W = 17  # three passes of a W-point box filter: a near-Gaussian of 8.5 cells
+
+    def box(x):  # along the last axis, zero padded
+        c = F.pad(x, (W // 2 + 1, W // 2)).cumsum(-1)
+        return (c[..., W:] - c[..., :-W]) / W
+
+    for axis in range(3):
+        g = g.movedim(axis, 2)
+        for _ in range(3):
+            g = box(g)
+        g = g.movedim(2, axis)
     g = (g - g.mean()) / g.std()
-    p = torch.exp(0.05 * g.double())
+    p = torch.exp(0.05 * g)
     del g
-    k = torch.arange(-1, n + 3, device=dev, dtype=torch.float64)
-    z = (k - 1) * h
-    cs = (1000.0 + 2500.0 * z / ((n - 1) * h)).view(m, 1, 1) * p
+    if z is None:
+        k = torch.arange(-1, nz + 3, device=dev, dtype=torch.float64)
+        depth = ((k - 1) * h).view(-1, 1, 1)
+    else:
+        depth = z.view(shp).clamp(min=0.0)
+    cs = (1000.0 + 2500.0 * depth / ((nz - 1) * h)) * p
     cp = 1.8 * cs
     rho = 1700.0 * (cp / 1000.0) ** 0.25
     mu = rho * cs * cs
@@ -148,6 +161,153 @@ def cpu_baseline(n, h, nz, steps, srcs):
     }
 
 
+CURV_FLOP_INTERIOR = 2126   # rhs4sgcurv.C:47-50, arithmetic ops per interior point
+CURV_FLOP_BOUNDARY = 6049   # per point of the six SBP rows
+CURV_WORDS = 13             # u(3), mu, lambda, met(4), jac in; lu(3) out
+FP64_PEAK_TFLOPS = 78.6
+
+
+def hills_depth(torch, nx, ny, nz, h, seed, nhill=6):
+    """Grid depth over the box: the top is a sum of nhill seeded Gaussian
+    hills (width 20-60 cells, total height 15% of the depth), mapped linearly
+    in k to the flat bottom z = (nz-1) h."""
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda")
+    depth = (nz - 1) * h
+    x = (torch.arange(-1, nx + 3, device=dev, dtype=torch.float64) - 1) * h
+    y = (torch.arange(-1, ny + 3, device=dev, dtype=torch.float64) - 1) * h
+    Y, X = torch.meshgrid(y, x, indexing="ij")
+    top = torch.zeros_like(X)
+    for _ in range(nhill):
+        xc, yc = rng.uniform(0.15, 0.85) * (nx - 1) * h, rng.uniform(0.15, 0.85) * (ny - 1) * h
+        w = rng.uniform(20, 60) * h
+        top -= rng.uniform(0.3, 1.0) * torch.exp(-((X - xc) ** 2 + (Y - yc) ** 2) / (2 * w * w))
+    top *= 0.15 * depth / top.abs().max()
+    k = torch.arange(-1, nz + 3, device=dev, dtype=torch.float64)
+    zeta = ((k - 1) / (nz - 1)).view(-1, 1, 1)
+    z = top.unsqueeze(0) * (1 - zeta) + depth * zeta
+    xs = X.unsqueeze(0).expand(nz + 4, -1, -1).contiguous().view(-1)
+    ys = Y.unsqueeze(0).expand(nz + 4, -1, -1).contiguous().view(-1)
+    return xs, ys, z.contiguous().view(-1)
+
+
+def cpu_baseline_curv(nx, ny, nz_slab):
+    """The oracle's rhs4sgcurv (OpenMP) on a bounded slab of the workload's
+    shape: nx x ny x nz_slab interior points under one hill, 3 evaluations."""
+    from oracle import oracle as O
+    from sw4lite_amd import ew
+    acof, ghcof, bope, _ = ew.stencil_coefficients()
+    h = 20.0
+    ib, ie, jb, je, kb, ke = -1, nx + 2, -1, ny + 2, -1, nz_slab + 2
+    box = (ib, ie, jb, je, kb, ke)
+    shp = (ke - kb + 1, je - jb + 1, ie - ib + 1)
+    n = shp[0] * shp[1] * shp[2]
+    x = (np.arange(ib, ie + 1) - 1) * h
+    y = (np.arange(jb, je + 1) - 1) * h
+    X, Y = np.meshgrid(x, y)
+    top = -0.1 * 255 * h * np.exp(-((X - x.mean()) ** 2 + (Y - y.mean()) ** 2) / (2 * (40 * h) ** 2))
+    zeta = (np.arange(kb, ke + 1) - 1) / 255.0
+    z = np.ascontiguousarray((top[None] * (1 - zeta[:, None, None]) + 255 * h * zeta[:, None, None]).reshape(-1))
+    met, jac = np.zeros(4 * n), np.zeros(n)
+    O.metric(1, box, h, z, met, jac)
+    rng = np.random.default_rng(0)
+    u = rng.uniform(-1, 1, 3 * n)
+    mu, la = np.full(n, 3e10), np.full(n, 4e10)
+    strx, stry = np.ones(shp[2]), np.ones(shp[1])
+    lu = O.filled(3 * n)
+    O.rhs4sgcurv(1, box, u, mu, la, met, jac, lu, [0, 0, 0, 0, 1, 0], acof, bope, ghcof, strx, stry)
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        O.rhs4sgcurv(1, box, u, mu, la, met, jac, lu, [0, 0, 0, 0, 1, 0], acof, bope, ghcof, strx, stry)
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": nx * ny * nz_slab * reps / dt, "unit": "grid-pt evaluations/s", "cores": threads,
+            "kind": "port",
+            "sample": "oracle rhs4sgcurv_rev restatement, %dx%dx%d slab under one hill, top SBP rows, %d "
+                      "evaluations after 1 warm-up, %.1f s, OMP threads %d" % (nx, ny, nz_slab, reps, dt, threads)}
+
+
+def main_curvilinear(a, torch):
+    from sw4lite_amd import ew
+    from sw4lite_amd import lib as L
+    nx, ny, nz, h = 384, 384, 256, 20.0
+    box = (-1, nx + 2, -1, ny + 2, -1, nz + 2)
+    npts = (nx + 4) * (ny + 4) * (nz + 4)
+    dev = torch.device("cuda")
+    x, y, z = hills_depth(torch, nx, ny, nz, h, seed=3)
+    met = torch.empty(4 * npts, dtype=torch.float64, device=dev)
+    jac = torch.empty(npts, dtype=torch.float64, device=dev)
+    ew.metric(1, box, x, y, z, met, jac)
+    del x, y
+    rho, mu, la = make_material(torch, (nx, ny, nz), h, seed=3, z=z)
+    del z
+    desc = L.grid_desc(nx, ny, nz, h, [L.BC_SUPERGRID] * 4 + [L.BC_FREE, L.BC_SUPERGRID], corder=1,
+                       sg_gp_thickness=30)
+    sg = {k: torch.from_numpy(v).to(dev) for k, v in ew.host_supergrid(desc).items()}
+    acof, ghcof, bope, sbop = ew.stencil_coefficients()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(3)
+    u = 1e-3 * torch.randn(3 * npts, generator=gen, device=dev, dtype=torch.float64)
+    um = u + 1e-5 * torch.randn(3 * npts, generator=gen, device=dev, dtype=torch.float64)
+    lu = torch.zeros_like(u)
+    up = torch.zeros_like(u)
+    onesided = [0, 0, 0, 0, 1, 0]
+
+    def rhs():
+        ew.rhs4sgcurv(1, box, u, mu, la, met, jac, lu, onesided, acof, bope, ghcof, sg["strx"], sg["stry"])
+
+    def damp():
+        ew.addsgd4c(1, box, up, u, um, rho, sg["dcx"], sg["dcy"], sg["strx"], sg["stry"], jac, sg["cox"], sg["coy"],
+                    0.02)
+
+    def bc():
+        ew.freesurfcurvisg(1, box, nz, 5, u, mu, la, met, sbop, None, sg["strx"], sg["stry"])
+
+    def timed(fn, reps):
+        for _ in range(a.warmup):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    t0 = time.perf_counter()
+    ms_rhs = timed(rhs, a.steps)
+    ms_damp = timed(damp, a.steps)
+    ms_bc = timed(bc, a.steps)
+    wall = time.perf_counter() - t0
+    pts = float(nx) * ny * nz
+    flop = float(nx) * ny * ((nz - 6) * CURV_FLOP_INTERIOR + 6 * CURV_FLOP_BOUNDARY)
+    tflops = flop / (ms_rhs * 1e-3) / 1e12
+    gbs = pts * CURV_WORDS * 8 / (ms_rhs * 1e-3) / 1e9
+    line = {
+        "metric": "rhs4sgcurv grid-pt evaluations/s (curvilinear divergence of stress, FP64)",
+        "value": pts / (ms_rhs * 1e-3),
+        "unit": "grid-pt evaluations/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_rhs,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {
+            "workload": "%dx%dx%d curvilinear grid under 6 seeded Gaussian hills (15%% of depth, widths 20-60 "
+                        "cells), linear vertical stretching, h=20 m, layered+lognormal material, top SBP rows; "
+                        "one rhs4sgcurv evaluation per step" % (nx, ny, nz),
+            "addsgd4c_ms": ms_damp, "freesurfcurvisg_ms": ms_bc, "wall_s": wall,
+            "hbm_GBps_at_13_words": gbs,
+        },
+        "roofline": {"bound": "fp64", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_rhs4curv",
+                     "flop_per_launch": flop, "avg_launch_ms": ms_rhs},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline_curv(nx, ny, 24)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -159,6 +319,8 @@ def main():
     if not torch.cuda.is_available():
         sys.stderr.write("bench.py: no GPU visible\n")
         sys.exit(1)
+    if a.workload == "curvilinear":
+        return main_curvilinear(a, torch)
     n, h = a.n, 20.0
     srcs = sources(n, h, seed=2)
     rho, mu, la = make_material(torch, n, h, seed=2)
@@ -215,13 +377,13 @@ def main():
             "step_hbm_fraction": value * STEP_BYTES_PER_PT / (HBM_PEAK_GBS * 1e9),
             "stage_ms_per_step": {k: v / a.steps for k, v in zip(
                 ["pred_interior", "pred_boundary_rows_shell", "pred_sources_bc", "corr_interior",
-                 "corr_boundary_rows_shell", "corr_sources_bc"], stage_ms.tolist())},
+                 "corr_interior_damping", "corr_boundary_rows_shell", "corr_sources_bc"], stage_ms.tolist())},
             "pred_interior_GBps": corr_pts * PRED_BYTES_PER_PT / (pred_ms * 1e-3) / 1e9,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "k_sweep_lds<true>" if a.variant == 1 else "k_corr_interior", "bytes_per_launch": corr_pts * CORR_BYTES_PER_PT,
+            "kernel": "k_sweep_lds<true,...>" if a.variant >= 1 else "k_corr_interior", "bytes_per_launch": corr_pts * CORR_BYTES_PER_PT,
             "avg_launch_ms": corr_ms,
         },
         "cpu_baseline": None,

@@ -124,6 +124,42 @@ int sw4hip_bcfortsg(int corder, int ib, int ie, int jb, int je, int kb, int ke, 
                     void* stream);
 
 /* ----------------------------------------------------------------------- */
+/* Curvilinear (topography) grid, kernel level                              */
+/* ----------------------------------------------------------------------- */
+/* Replaces EW::metric / metric_rev (curvilinear-c.C:53-172, :175-295): the
+ * metric met (4 components (sqrt(z_r), -z_p/sqrt(z_r), -z_q/sqrt(z_r),
+ * h/sqrt(z_r)), same layout as u) and jac = h^2 z_r from the grid
+ * coordinates x, y, z (device arrays over the box; h = x(ib+1,jb,kb) -
+ * x(ib,jb,kb) as in the reference, y unused).  SW4HIP_EINVAL where z_r <= 0
+ * (the reference returns -1).  Synchronises the stream. */
+int sw4hip_metric(int corder, int ib, int ie, int jb, int je, int kb, int ke, const double* x,
+                  const double* y, const double* z, double* met, double* jac, void* stream);
+/* Replaces rhs4sgcurv (rhs4sgcurv.C:34-39, corder 0) and rhs4sgcurv_rev
+ * (rhs4sgcurv_rev.C:34-40, corder 1); called EW.C:3221-3243.  lu = L(u) on
+ * the curvilinear grid; SBP rows 1..6 when onesided[4] == 1, interior rows
+ * up to klast-2.  onesided, acof, bope, ghcof are HOST tables. */
+int sw4hip_rhs4sgcurv(int corder, int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast,
+                      const double* u, const double* mu, const double* lambda, const double* met,
+                      const double* jac, double* lu, const int* onesided, const double* acof,
+                      const double* bope, const double* ghcof, const double* strx,
+                      const double* stry, void* stream);
+/* Replaces EW::addsgd4cfort / addsgd4cfort_indrev (ew-cfromfort.C:1160-1240,
+ * :1323-1404): x and y supergrid damping with rho -> rho jac. */
+int sw4hip_addsgd4c(int corder, int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast,
+                    double* up, const double* u, const double* um, const double* rho,
+                    const double* dcx, const double* dcy, const double* strx, const double* stry,
+                    const double* jac, const double* cox, const double* coy, double beta,
+                    void* stream);
+/* Replaces EW::freesurfcurvisg / freesurfcurvisg_rev (curvilinear-c.C:465-618,
+ * :621-764): traction-free ghost points on side 5 (k = 1, the top) or 6
+ * (k = nz), sides numbered from 1 as in the reference.  forcing: device
+ * array of 3 interleaved values per (i,j) of the box plane, or NULL for zero
+ * traction; sbop is a HOST table. */
+int sw4hip_freesurfcurvisg(int corder, int ib, int ie, int jb, int je, int kb, int ke, int nz,
+                           int side, double* u, const double* mu, const double* lambda,
+                           const double* met, const double* sbop, const double* forcing,
+                           const double* strx, const double* stry, void* stream);
+/* ----------------------------------------------------------------------- */
 /* EW-level object                                                           */
 /* ----------------------------------------------------------------------- */
 typedef struct sw4hip_ew sw4hip_ew;
@@ -146,9 +182,10 @@ int sw4hip_ew_create(const sw4hip_grid_desc* g, sw4hip_ew** ew);
 int sw4hip_ew_destroy(sw4hip_ew* ew);
 /* Stream used by every later call on this object (hipStream_t). */
 int sw4hip_ew_set_stream(sw4hip_ew* ew, void* stream);
-/* Interior kernels of the fused sweeps: variant 0 one thread per point,
- * variant 1 (default) LDS-staged k-marching tiles with kchunk planes per
- * workgroup (default 32).  Both give the same results to rounding. */
+/* Interior kernels of the fused sweeps: variant 0 one thread per point;
+ * variants 1-3 LDS-staged k-marching tiles with kchunk planes per workgroup
+ * (1: plain, 2: register prefetch of the next plane, 3: prefetch without a
+ * register cap).  All give the same results to rounding. */
 int sw4hip_ew_set_kernel_variant(sw4hip_ew* ew, int variant, int kchunk);
 /* rho, mu, lambda over the whole box incl. ghosts; on_device selects whether
  * the pointers are device (1) or host (0) arrays. */
@@ -179,10 +216,11 @@ int sw4hip_ew_get_solution(sw4hip_ew* ew, double* u, double* um, int on_device);
  * addSuperGridDamping, enforceBC; EW.C:2527-2842). */
 int sw4hip_ew_step(sw4hip_ew* ew, int nsteps, int mode);
 /* The fused step with per-stage GPU times: hipEvents recorded on the
- * object's stream; stage_ms[6] receives the summed milliseconds of
+ * object's stream; stage_ms[7] receives the summed milliseconds of
  * (predictor interior kernel, predictor boundary rows + ghost shell,
  * predictor sources + ghost-point BC, corrector interior kernel, corrector
- * boundary rows + ghost shell, corrector sources + BC).  Synchronises. */
+ * interior supergrid damping, corrector boundary rows + ghost shell,
+ * corrector sources + BC).  Synchronises. */
 int sw4hip_ew_step_timed(sw4hip_ew* ew, int nsteps, double* stage_ms);
 /* Analytic point-source error of the current U (EW::exactSol +
  * normOfDifference, EW.C:3650-3669, 4526-4610). */

@@ -18,6 +18,7 @@ SOURCES = [
     "kernels_basic.hip",
     "kernels_step.hip",
     "kernels_fast.hip",
+    "kernels_curv.hip",
     "sbp_coefficients.cpp",
     "setup.cpp",
     "ew.cpp",

@@ -5,6 +5,7 @@
 
 #include "../../include/sw4hip.h"
 #include "ew.h"
+#include "kernels_curv.h"
 
 namespace sw4amd {
 
@@ -254,6 +255,114 @@ int sw4hip_bcfortsg(int corder, int ib, int ie, int jb, int je, int kb, int ke, 
 }
 
 // ---------------------------------------------------------------------------
+// Curvilinear grid
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_flag_mutex;
+int* g_flag[64] = {nullptr};
+
+// A device int on the current device for kernels that report a condition.
+int device_flag(int** out)
+{
+   int dev = 0;
+   int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+   if (rc) return rc;
+   if (dev < 0 || dev >= 64) return set_error(E_RANGE, "device ordinal beyond 63");
+   std::lock_guard<std::mutex> lock(g_flag_mutex);
+   if (!g_flag[dev]) {
+      void* p = nullptr;
+      if (hipMalloc(&p, sizeof(int)) != hipSuccess) return set_error(E_NOMEM, "flag allocation failed");
+      g_flag[dev] = static_cast<int*>(p);
+   }
+   *out = g_flag[dev];
+   return OK;
+}
+}  // namespace
+
+int sw4hip_metric(int corder, int ib, int ie, int jb, int je, int kb, int ke, const double* x, const double* y,
+                  const double* z, double* met, double* jac, void* stream)
+{
+   (void)y;
+   if (corder != 0 && corder != 1) return set_error(E_INVAL, "corder must be 0 or 1");
+   if (ie - ib < 5 || je - jb < 5 || ke - kb < 5) return set_error(E_INVAL, "metric: need 6 points per direction");
+   if (!x || !z || !met || !jac) return set_error(E_INVAL, "metric: null argument");
+   hipStream_t s = (hipStream_t)stream;
+   int rc;
+   double x01[2];
+   int* flag = nullptr;
+   if ((rc = device_flag(&flag))) return rc;
+   if ((rc = check_hip(hipMemcpyAsync(x01, x, 2 * sizeof(double), hipMemcpyDeviceToHost, s), "metric: x readback")) ||
+       (rc = check_hip(hipStreamSynchronize(s), "metric: x readback")))
+      return rc;
+   const double h = x01[1] - x01[0];  // h = x(ib+1,jb,kb) - x(ib,jb,kb)
+   if ((rc = check_hip(hipMemsetAsync(flag, 0, sizeof(int), s), "metric: flag clear")) ||
+       (rc = launch_metric(corder, make_box(ib, ie, jb, je, kb, ke), h, z, met, jac, flag, s)))
+      return rc;
+   int bad = 0;
+   if ((rc = check_hip(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, s), "metric: flag readback")) ||
+       (rc = check_hip(hipStreamSynchronize(s), "metric: flag readback")))
+      return rc;
+   if (bad) return set_error(E_INVAL, "metric: z is not increasing in k somewhere (z_r <= 0)");
+   return OK;
+}
+
+int sw4hip_rhs4sgcurv(int corder, int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast,
+                      const double* u, const double* mu, const double* lambda, const double* met, const double* jac,
+                      double* lu, const int* onesided, const double* acof, const double* bope, const double* ghcof,
+                      const double* strx, const double* stry, void* stream)
+{
+   if (corder != 0 && corder != 1) return set_error(E_INVAL, "corder must be 0 or 1");
+   if (!box_ok(ifirst, ilast, jfirst, jlast, kfirst, klast)) return set_error(E_INVAL, "rhs4sgcurv: box too small");
+   if (!u || !mu || !lambda || !met || !jac || !lu || !onesided || !acof || !bope || !ghcof || !strx || !stry)
+      return set_error(E_INVAL, "rhs4sgcurv: null argument");
+   // the top closure reads planes 0..8 and writes rows 1..6
+   if (onesided[4] == 1 && (kfirst > 0 || klast < 8))
+      return set_error(E_INVAL, "rhs4sgcurv: the top closure needs planes 0..8 in the array");
+   SbpTables t = default_tables();
+   std::memcpy(t.acof, acof, sizeof(t.acof));
+   std::memcpy(t.bope, bope, sizeof(t.bope));
+   std::memcpy(t.ghcof, ghcof, sizeof(t.ghcof));
+   const SbpTables* d = nullptr;
+   int rc;
+   if ((rc = device_sbp(t, &d))) return rc;
+   return launch_rhs4sgcurv(corder, make_box(ifirst, ilast, jfirst, jlast, kfirst, klast), onesided, d, u, mu,
+                            lambda, met, jac, strx, stry, lu, (hipStream_t)stream);
+}
+
+int sw4hip_addsgd4c(int corder, int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast, double* up,
+                    const double* u, const double* um, const double* rho, const double* dcx, const double* dcy,
+                    const double* strx, const double* stry, const double* jac, const double* cox, const double* coy,
+                    double beta, void* stream)
+{
+   if (corder != 0 && corder != 1) return set_error(E_INVAL, "corder must be 0 or 1");
+   if (!box_ok(ifirst, ilast, jfirst, jlast, kfirst, klast)) return set_error(E_INVAL, "addsgd4c: box too small");
+   if (!up || !u || !um || !rho || !dcx || !dcy || !strx || !stry || !jac || !cox || !coy)
+      return set_error(E_INVAL, "addsgd4c: null argument");
+   SgCoef c{dcx, dcy, nullptr, strx, stry, nullptr, cox, coy, nullptr};
+   return launch_addsgd4c(corder, make_box(ifirst, ilast, jfirst, jlast, kfirst, klast), up, u, um, rho, jac, c, beta,
+                          (hipStream_t)stream);
+}
+
+int sw4hip_freesurfcurvisg(int corder, int ib, int ie, int jb, int je, int kb, int ke, int nz, int side, double* u,
+                           const double* mu, const double* lambda, const double* met, const double* sbop,
+                           const double* forcing, const double* strx, const double* stry, void* stream)
+{
+   if (corder != 0 && corder != 1) return set_error(E_INVAL, "corder must be 0 or 1");
+   if (!box_ok(ib, ie, jb, je, kb, ke)) return set_error(E_INVAL, "freesurfcurvisg: box too small");
+   if (!u || !mu || !lambda || !met || !sbop || !strx || !stry) return set_error(E_INVAL, "freesurfcurvisg: null argument");
+   if (side != 5 && side != 6) return set_error(E_INVAL, "freesurfcurvisg: side must be 5 (k = 1) or 6 (k = nz)");
+   const bool fits = side == 5 ? (kb <= 0 && ke >= 4) : (nz - 3 >= kb && nz + 1 <= ke);
+   if (!fits) return set_error(E_INVAL, "freesurfcurvisg: boundary plane too close to the array end");
+   SbpTables t = default_tables();
+   std::memcpy(t.sbop, sbop, sizeof(t.sbop));
+   const SbpTables* d = nullptr;
+   int rc;
+   if ((rc = device_sbp(t, &d))) return rc;
+   return launch_freesurf_curv(corder, make_box(ib, ie, jb, je, kb, ke), nz, side - 1, u, mu, lambda, met, d, forcing,
+                               strx, stry, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
 // EW level
 // ---------------------------------------------------------------------------
 int sw4hip_ew_create(const sw4hip_grid_desc* d, sw4hip_ew** out)
@@ -302,7 +411,7 @@ int sw4hip_ew_set_stream(sw4hip_ew* h, void* stream)
 int sw4hip_ew_set_kernel_variant(sw4hip_ew* h, int variant, int kchunk)
 {
    EWCHK(h);
-   if (variant != 0 && variant != 1) return set_error(E_INVAL, "variant must be 0 or 1");
+   if (variant < 0 || variant > 3) return set_error(E_INVAL, "variant must be 0..3");
    if (kchunk < 1 || kchunk > 4096) return set_error(E_INVAL, "kchunk must be in [1, 4096]");
    h->ew->m_variant = variant;
    h->ew->m_kchunk = kchunk;

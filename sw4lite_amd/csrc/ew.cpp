@@ -63,6 +63,10 @@ int EW::init()
    if (hipMalloc(&p, sizeof(double) * 3 * m_nred + 64) != hipSuccess) return set_error(E_NOMEM, "EW: reduction buffer allocation failed");
    m_allocs.push_back(p);
    m_dred = static_cast<double*>(p);
+   if (hipMalloc(&p, sizeof(double) * accel_slab_size(m_box)) != hipSuccess)
+      return set_error(E_NOMEM, "EW: acceleration slab allocation failed");
+   m_allocs.push_back(p);
+   m_dacc = static_cast<double*>(p);
    return OK;
 }
 
@@ -385,11 +389,11 @@ int EW::step_fused(int n, double* stage_ms)
           (rc = mark(s, 2)) || (rc = launch_src_apply(m_box, st, t, 0, dt * dt, mRho.dev, mUp.dev, m_stream)) ||
           (rc = enforceBC(mUp, mMu, mLambda, t + dt)) || (rc = mark(s, 3)) ||
           (rc = launch_corr_fused(m_box, g.nz, m_onesided, m_dsbp, mUp.dev, mU.dev, mUm.dev, mMu.dev, mLambda.dev,
-                                  mRho.dev, c, m_opt.beta, mUp2.dev, g.h, dt, m_stream, [&]() { return mark(s, 4); },
-                                  m_variant, m_kchunk)) ||
-          (rc = mark(s, 5)) ||
+                                  mRho.dev, c, m_opt.beta, mUp2.dev, m_dacc, g.h, dt, m_stream, [&]() { return mark(s, 4); },
+                                  [&]() { return mark(s, 5); }, m_variant, m_kchunk)) ||
+          (rc = mark(s, 6)) ||
           (rc = launch_src_apply(m_box, st, t, 1, dt * dt * dt * dt / 12, mRho.dev, mUp2.dev, m_stream)) ||
-          (rc = enforceBC(mUp2, mMu, mLambda, t + dt)) || (rc = mark(s, 6)) || (rc = record(mUp2, mUm)))
+          (rc = enforceBC(mUp2, mMu, mLambda, t + dt)) || (rc = mark(s, 7)) || (rc = record(mUp2, mUm)))
          break;
       mT += dt;
       mCycle++;

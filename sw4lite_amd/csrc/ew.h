@@ -64,9 +64,10 @@ public:
    int step_reference(int n);  // stage order of EW::timesteploop (EW.C:2527-2842)
    // two fused sweeps per step; with stage_ms, kStages accumulated stage
    // times (ms): predictor interior, predictor boundary rows + ghost shell,
-   // predictor sources + BC, corrector interior, corrector boundary rows +
-   // ghost shell, corrector sources + BC
-   static constexpr int kStages = 6;
+   // predictor sources + BC, corrector interior, corrector interior damping
+   // (LDS variant; 0 otherwise), corrector boundary rows + ghost shell,
+   // corrector sources + BC
+   static constexpr int kStages = 7;
    int step_fused(int n, double* stage_ms = nullptr);
    int point_source_error(double& linf, double& l2, double& solinf);
 
@@ -80,6 +81,7 @@ public:
    // state access
    Sarray mU, mUm, mUp, mUp2, mRho, mMu, mLambda;
    Sarray mLu, mF, mUacc;  // only for step_reference
+   double* m_dacc = nullptr;  // acceleration on the planes the corrector's SBP rows read
    double mDt = 0, mT = 0;
    int mNumberOfTimeSteps = 0, mCycle = 0;
    int m_onesided[6] = {0, 0, 0, 0, 0, 0};

@@ -43,6 +43,19 @@ struct Accel {
    double dt2i;
    __device__ __forceinline__ double operator[](long q) const { return dt2i * (up[q] - 2 * u[q] + um[q]); }
 };
+// A few k-planes ks..ks+npl-1 of a component-blocked array over the full
+// (i,j) extent, addressed with the full box's component-blocked indices.
+struct Slab {
+   const double* __restrict__ p;
+   long npts;   // points of the full box
+   long shift;  // nij * (ks - kb)
+   long nslab;  // nij * npl
+   __device__ __forceinline__ double operator[](long q) const
+   {
+      const int c = (q >= npts) + (q >= 2 * npts);
+      return p[q - c * npts - shift + c * nslab];
+   }
+};
 
 // 4th-order centred first difference times 12h: f(-2)-f(+2)+8(f(+1)-f(-1))
 __device__ __forceinline__ double d4(double fm2, double fm1, double fp1, double fp2)
@@ -202,6 +215,7 @@ __device__ __forceinline__ void rhs_sbp_point(const Box& b, int i, int j, int k,
    second_derivs<CO, false, UA>(b, i, j, k, u, mu, la, strx, stry, nullptr, r);
 
    double zz0 = 0, zz1 = 0, zz2 = 0;
+#pragma unroll 1
    for (int q = 1; q <= 8; q++) {
       double cmu = 0, c2ml = 0;
 #pragma unroll
@@ -229,6 +243,7 @@ __device__ __forceinline__ void rhs_sbp_point(const Box& b, int i, int j, int k,
 #pragma unroll
    for (int s = 0; s < 4; s++) {
       double a3x = 0, a3y = 0, a1x = 0, a2y = 0;
+#pragma unroll 1
       for (int q = 1; q <= 8; q++) {
          const double bq = sgn * sbp->bope[(kr - 1) + 6 * (q - 1)];
          a3x += bq * u[vidx<CO>(b, 2, i + off[s], j, KK(q))];
@@ -242,6 +257,7 @@ __device__ __forceinline__ void rhs_sbp_point(const Box& b, int i, int j, int k,
       w2y[s] = mu[sidx(b, i, j + off[s], k)] * a2y;
    }
    double t3x = 0, t3y = 0, t1x = 0, t2y = 0;
+#pragma unroll 1
    for (int q = 1; q <= 8; q++) {
       const double bq = sgn * sbp->bope[(kr - 1) + 6 * (q - 1)];
       const int kq = KK(q);

@@ -7,8 +7,10 @@ namespace sw4amd {
 // Interior planes k1..k2 of the predictor (corr = false: out = Up from u, um)
 // or corrector sweep (corr = true: out = Up' from upin = Up, u, um), chunks of
 // kc planes per workgroup.
-int launch_sweep_lds(bool corr, const Box& b, int k1, int k2, int kc, const double* upin, const double* u,
-                     const double* um, const double* mu, const double* la, const double* rho, const SgCoef& sg,
-                     double* out, double h, double dt, double beta, hipStream_t s);
+// variant 1: plain k-march, 2 waves/SIMD; 2: register prefetch of the next
+// plane, 2 waves/SIMD; 3: prefetch, 1 wave/SIMD.
+int launch_sweep_lds(bool corr, int variant, const Box& b, int k1, int k2, int kc, const double* upin,
+                     const double* u, const double* um, const double* mu, const double* la, const double* rho,
+                     const SgCoef& sg, double* out, double h, double dt, double beta, hipStream_t s);
 
 }  // namespace sw4amd

@@ -8,7 +8,8 @@
 // the in-plane terms, the column values k+-1, k+-2 rotate in registers.  The
 // corrector stages (Up - 2U + Um)/dt^2 instead of U, so the acceleration is
 // never stored.  The arithmetic is rhs4_interior_body, shared with the
-// global-memory kernels (same results to rounding).
+// global-memory kernels (same results to rounding).  The supergrid damping
+// of the corrector sweep runs as k_addsgd4_planes right after it.
 #include "kernels_common.h"
 #include "kernels_fast.h"
 #include "stencil_body.h"
@@ -42,15 +43,21 @@ struct LdsSten {
    __device__ __forceinline__ double sz(int d) const { return szv[d + 2]; }
 };
 
-// CORR = false: out = 2u - um + dt2/rho cof L(u)                (predictor)
-// CORR = true:  out = up + dt4/(12 rho) cof L((up-2u+um) dt2i) - damping(u,um)
-template <bool CORR>
-__global__ void __launch_bounds__(NT, CORR ? 1 : 2)
+// CORR = false: out = 2u - um + dt2/rho cof L(u)                  (predictor)
+// CORR = true:  out = up + dt4/(12 rho) cof L((up - 2u + um) dt2i)  (corrector;
+//               the supergrid damping is applied by k_addsgd4_planes)
+// Software pipeline: the global loads of plane k+3 (and of the material of
+// plane k+1) are issued before the stencil of plane k is evaluated and written
+// to LDS at the top of the next iteration, so their latency overlaps compute.
+template <bool CORR, bool PIPE, int MINW>
+__global__ void __launch_bounds__(NT, MINW)
 k_sweep_lds(Box b, int kbeg, int kend, int kc, const double* __restrict__ upin, const double* __restrict__ u,
             const double* __restrict__ um, const double* __restrict__ mu, const double* __restrict__ la,
             const double* __restrict__ rho, SgCoef sg, double* __restrict__ out, double cof, double dt2i,
-            double cupd, double beta)
+            double cupd)
 {
+   constexpr int NU = (3 * PL + NT - 1) / NT;  // staged displacement values per thread
+   constexpr int NM = (PL + NT - 1) / NT;      // staged material values per thread
    __shared__ double su[5 * 3 * PL];
    __shared__ double sm[PL], sl[PL];
    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
@@ -63,25 +70,71 @@ k_sweep_lds(Box b, int kbeg, int kend, int kc, const double* __restrict__ upin, 
    const long col = active ? (i - b.ib) + ni * (j - b.jb) : 0;
    const int o = (ty + 2) * HX + tx + 2;
 
-   auto stage = [&](int kp, int slot) {
+   // element e = tid + p*NT of the staged (c, jj, ii) tile, its global offset
+   long goff[NU];
+   bool gok[NU];
+#pragma unroll
+   for (int p = 0; p < NU; p++) {
+      const int e = tid + p * NT;
+      const int c = e / PL, r = e - c * PL, jj = r / HX, ii = r - jj * HX;
+      const int gi = ia + ii, gj = ja + jj;
+      gok[p] = e < 3 * PL && gi <= b.ie && gj <= b.je;
+      goff[p] = gok[p] ? (gi - b.ib) + ni * (gj - b.jb) + npts * c : 0;
+   }
+   long moff[NM];
+   bool mok[NM];
+#pragma unroll
+   for (int p = 0; p < NM; p++) {
+      const int e = tid + p * NT;
+      const int jj = e / HX, ii = e - jj * HX;
+      const int gi = ia + ii, gj = ja + jj;
+      mok[p] = e < PL && gi <= b.ie && gj <= b.je;
+      moff[p] = mok[p] ? (gi - b.ib) + ni * (gj - b.jb) : 0;
+   }
+   double ru[NU], rp[NU], rq[NU], rmu[NM], rla[NM];
+   auto load_plane = [&](int kp) {
       const long pb = nij * (kp - b.kb);
-      double* dst = su + slot * 3 * PL;
-      for (int e = tid; e < 3 * PL; e += NT) {
-         const int c = e / PL, r = e - c * PL, jj = r / HX, ii = r - jj * HX;
-         const int gi = ia + ii, gj = ja + jj;
-         double v = 0.0;
-         if (gi <= b.ie && gj <= b.je) {
-            const long q = (gi - b.ib) + ni * (gj - b.jb) + pb + npts * c;
-            v = CORR ? dt2i * (upin[q] - 2 * u[q] + um[q]) : u[q];
+#pragma unroll
+      for (int p = 0; p < NU; p++) {
+         ru[p] = gok[p] ? u[pb + goff[p]] : 0.0;
+         if (CORR) {
+            rp[p] = gok[p] ? upin[pb + goff[p]] : 0.0;
+            rq[p] = gok[p] ? um[pb + goff[p]] : 0.0;
          }
-         dst[e] = v;
+      }
+   };
+   auto store_plane = [&](int slot) {
+      double* dst = su + slot * 3 * PL;
+#pragma unroll
+      for (int p = 0; p < NU; p++) {
+         const int e = tid + p * NT;
+         if (e < 3 * PL) dst[e] = CORR ? dt2i * (rp[p] - 2 * ru[p] + rq[p]) : ru[p];
+      }
+   };
+   auto load_mat = [&](int kp) {
+      const long pb = nij * (kp - b.kb);
+#pragma unroll
+      for (int p = 0; p < NM; p++) {
+         rmu[p] = mok[p] ? mu[pb + moff[p]] : 0.0;
+         rla[p] = mok[p] ? la[pb + moff[p]] : 0.0;
+      }
+   };
+   auto store_mat = [&]() {
+#pragma unroll
+      for (int p = 0; p < NM; p++) {
+         const int e = tid + p * NT;
+         if (e < PL) {
+            sm[e] = rmu[p];
+            sl[e] = rla[p];
+         }
       }
    };
 
    LdsSten s;
 #pragma unroll
    for (int q = 0; q < 4; q++) {
-      stage(k0 - 2 + q, q);
+      load_plane(k0 - 2 + q);
+      store_plane(q);
       const long pq = nij * (k0 - 2 + q - b.kb);
       s.muc[q] = active ? mu[col + pq] : 0.0;
       s.lac[q] = active ? la[col + pq] : 0.0;
@@ -94,31 +147,39 @@ k_sweep_lds(Box b, int kbeg, int kend, int kc, const double* __restrict__ upin, 
    }
    s.pm = sm + o;
    s.pl = sl + o;
+   // PIPE: prefetch plane k0+2, material of plane k0, column values of k0+2
+   double cmu = 0, cla = 0, csz = 0;
+   if (PIPE) {
+      load_plane(k0 + 2);
+      load_mat(k0);
+      cmu = active ? mu[col + nij * (k0 + 2 - b.kb)] : 0.0;
+      cla = active ? la[col + nij * (k0 + 2 - b.kb)] : 0.0;
+      csz = sg.strz[k0 + 2 - b.kb];
+   }
 
    for (int k = k0; k <= k1; k++) {
-      const int n = k - k0;
-      const int sl0 = n % 5;
-      stage(k + 2, (sl0 + 4) % 5);
-      const long pbk = nij * (k - b.kb);
-      for (int e = tid; e < PL; e += NT) {
-         const int jj = e / HX, ii = e - jj * HX;
-         const int gi = ia + ii, gj = ja + jj;
-         double vm = 0.0, vl = 0.0;
-         if (gi <= b.ie && gj <= b.je) {
-            const long q = (gi - b.ib) + ni * (gj - b.jb) + pbk;
-            vm = mu[q];
-            vl = la[q];
-         }
-         sm[e] = vm;
-         sl[e] = vl;
+      const int sl0 = (k - k0) % 5;
+      if (!PIPE) {
+         load_plane(k + 2);
+         load_mat(k);
+         cmu = active ? mu[col + nij * (k + 2 - b.kb)] : 0.0;
+         cla = active ? la[col + nij * (k + 2 - b.kb)] : 0.0;
+         csz = sg.strz[k + 2 - b.kb];
       }
-      {
-         const long pq = nij * (k + 2 - b.kb);
-         s.muc[4] = active ? mu[col + pq] : 0.0;
-         s.lac[4] = active ? la[col + pq] : 0.0;
-         s.szv[4] = sg.strz[k + 2 - b.kb];
-      }
+      store_plane((sl0 + 4) % 5);
+      store_mat();
+      s.muc[4] = cmu;
+      s.lac[4] = cla;
+      s.szv[4] = csz;
       __syncthreads();
+      if (PIPE && k < k1) {
+         load_plane(k + 3);
+         load_mat(k + 1);
+         const long pq = nij * (k + 3 - b.kb);
+         cmu = active ? mu[col + pq] : 0.0;
+         cla = active ? la[col + pq] : 0.0;
+         csz = sg.strz[k + 3 - b.kb];
+      }
       if (active) {
          s.p0 = su + (sl0 * 3) * PL + o;
          s.p1 = su + (((sl0 + 1) % 5) * 3) * PL + o;
@@ -127,22 +188,15 @@ k_sweep_lds(Box b, int kbeg, int kend, int kc, const double* __restrict__ upin, 
          s.p4 = su + (((sl0 + 4) % 5) * 3) * PL + o;
          double r[3];
          rhs4_interior_body(s, r);
-         const long p = col + pbk;
+         const long p = col + nij * (k - b.kb);
          const double sc = cupd / rho[p];
-         if (!CORR) {
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-               const long q = p + npts * c;
+         for (int c = 0; c < 3; c++) {
+            const long q = p + npts * c;
+            if (!CORR)
                out[q] = 2 * s.p2[c * PL] - um[q] + sc * (cof * r[c]);
-            }
-         } else {
-            double d[3];
-            addsgd4_point<1>(b, i, j, k, u, um, rho, sg, beta, d);
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-               const long q = p + npts * c;
-               out[q] = (upin[q] + sc * (cof * r[c])) - d[c];
-            }
+            else
+               out[q] = upin[q] + sc * (cof * r[c]);
          }
       }
       __syncthreads();
@@ -155,21 +209,31 @@ k_sweep_lds(Box b, int kbeg, int kend, int kc, const double* __restrict__ upin, 
    }
 }
 
-int launch_sweep_lds(bool corr, const Box& b, int k1, int k2, int kc, const double* upin, const double* u,
-                     const double* um, const double* mu, const double* la, const double* rho, const SgCoef& sg,
-                     double* out, double h, double dt, double beta, hipStream_t s)
+int launch_sweep_lds(bool corr, int variant, const Box& b, int k1, int k2, int kc, const double* upin,
+                     const double* u, const double* um, const double* mu, const double* la, const double* rho,
+                     const SgCoef& sg, double* out, double h, double dt, double beta, hipStream_t s)
 {
    if (k2 < k1) return OK;
    const long nx = b.ie - b.ib - 3, ny = b.je - b.jb - 3;
    if (kc <= 0) kc = 32;
    dim3 grd(cdiv(nx, TX), cdiv(ny, TY), cdiv(k2 - k1 + 1, kc));
    const double cof = 1.0 / (h * h), dt2i = 1.0 / (dt * dt);
-   if (corr)
-      k_sweep_lds<true><<<grd, NT, 0, s>>>(b, k1, k2, kc, upin, u, um, mu, la, rho, sg, out, cof, dt2i,
-                                           dt * dt * dt * dt / 12, beta);
-   else
-      k_sweep_lds<false><<<grd, NT, 0, s>>>(b, k1, k2, kc, nullptr, u, um, mu, la, rho, sg, out, cof, dt2i,
-                                            dt * dt, beta);
+   (void)beta;
+   const double c4 = dt * dt * dt * dt / 12, c2 = dt * dt;
+#define SW4_SWEEP(C, P, W) k_sweep_lds<C, P, W><<<grd, NT, 0, s>>>(b, k1, k2, kc, C ? upin : nullptr, u, um, mu, \
+                                                                  la, rho, sg, out, cof, dt2i, C ? c4 : c2)
+   switch (variant) {
+   case 2:  // prefetch, 2 waves/SIMD
+      if (corr) SW4_SWEEP(true, true, 2); else SW4_SWEEP(false, true, 2);
+      break;
+   case 3:  // prefetch, 1 wave/SIMD (no register cap)
+      if (corr) SW4_SWEEP(true, true, 1); else SW4_SWEEP(false, true, 1);
+      break;
+   default:  // 1: no prefetch, 2 waves/SIMD
+      if (corr) SW4_SWEEP(true, false, 2); else SW4_SWEEP(false, false, 2);
+      break;
+   }
+#undef SW4_SWEEP
    return check_hip(hipGetLastError(), corr ? "LDS corrector launch" : "LDS predictor launch");
 }
 

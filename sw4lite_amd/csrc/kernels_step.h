@@ -24,11 +24,14 @@ int launch_pred_fused(const Box& b, int nk, const int* onesided, const SbpTables
                       const double* rho, const SgCoef& sg, double* up, double h, double dt,
                       hipStream_t s, const std::function<int()>& after_interior, int variant = 1,
                       int kchunk = 32);
+// Doubles of the acceleration slab launch_corr_fused needs as scratch (acc).
+long accel_slab_size(const Box& b);
 int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp,
                       const double* up, const double* u, const double* um, const double* mu,
                       const double* la, const double* rho, const SgCoef& sg, double beta,
-                      double* upo, double h, double dt, hipStream_t s,
-                      const std::function<int()>& after_interior, int variant = 1, int kchunk = 32);
+                      double* upo, double* acc, double h, double dt, hipStream_t s,
+                      const std::function<int()>& after_interior, const std::function<int()>& after_damping,
+                      int variant = 1, int kchunk = 32);
 int launch_src_apply(const Box& b, const SourceTable& st, double t, int deriv2, double coef,
                      const double* rho, double* up, hipStream_t s);
 int launch_maxeig(long n, const double* mu, const double* la, const double* rho, double* partial,

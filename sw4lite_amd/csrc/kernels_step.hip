@@ -91,12 +91,28 @@ k_corr_interior(Box b, int klo, int khi, const double* __restrict__ up, const do
    }
 }
 
+// Acceleration (up - 2u + um)/dt^2 on planes ks..ks+SLAB_PLANES-1, all (i,j)
+// of the box, component-blocked: the values the SBP rows of the corrector
+// read (evalDpDmInTime restricted to those planes, same arithmetic).
+constexpr int SLAB_PLANES = 9;
+__global__ void __launch_bounds__(256)
+k_accel_slab(Box b, int ks, const double* __restrict__ up, const double* __restrict__ u,
+             const double* __restrict__ um, double dt2i, double* __restrict__ acc)
+{
+   const long ns = b.nij * SLAB_PLANES;
+   const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (e >= 3 * ns) return;
+   const int c = (int)(e / ns);
+   const long r = e - c * ns;
+   const long q = r + b.nij * (ks - b.kb) + b.npts * c;
+   acc[e] = dt2i * (up[q] - 2 * u[q] + um[q]);
+}
+
 __global__ void __launch_bounds__(256)
 k_corr_sbp(Box b, int top, int nk, const SbpTables* __restrict__ sbp, const double* __restrict__ up,
-           const double* __restrict__ u, const double* __restrict__ um,
-           const double* __restrict__ mu, const double* __restrict__ la,
-           const double* __restrict__ rho, SgCoef sg, double* __restrict__ upo, double cof,
-           double dt2i, double dt4i12, double beta)
+           const double* __restrict__ acc, int ks, const double* __restrict__ u, const double* __restrict__ um,
+           const double* __restrict__ mu, const double* __restrict__ la, const double* __restrict__ rho, SgCoef sg,
+           double* __restrict__ upo, double cof, double dt4i12, double beta)
 {
    const int i = b.ib + 2 + blockIdx.x * blockDim.x + threadIdx.x;
    const int j = b.jb + 2 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -104,8 +120,8 @@ k_corr_sbp(Box b, int top, int nk, const SbpTables* __restrict__ sbp, const doub
    if (i > b.ie - 2 || j > b.je - 2) return;
    const int k = top ? kr : nk - kr + 1;
    double r[3], d[3];
-   rhs_sbp_point<1>(b, i, j, k, kr, top ? 1 : nk, top ? 1 : -1, sbp, Accel{up, u, um, dt2i}, mu, la,
-                    sg.strx, sg.stry, r);
+   const Slab a{acc, b.npts, b.nij * (ks - b.kb), b.nij * SLAB_PLANES};
+   rhs_sbp_point<1>(b, i, j, k, kr, top ? 1 : nk, top ? 1 : -1, sbp, a, mu, la, sg.strx, sg.stry, r);
    addsgd4_point<1>(b, i, j, k, u, um, rho, sg, beta, d);
    const long p = sidx(b, i, j, k);
    const double s = dt4i12 / rho[p];
@@ -114,6 +130,23 @@ k_corr_sbp(Box b, int top, int nk, const SbpTables* __restrict__ sbp, const doub
       const long q = p + b.npts * c;
       upo[q] = (up[q] + s * (cof * r[c])) - d[c];
    }
+}
+
+// Supergrid damping of the corrector sweep on planes klo..khi (the planes the
+// LDS corrector covers): up -= damping(u, um) (ew-cfromfort.C:985-1041).
+__global__ void __launch_bounds__(256)
+k_addsgd4_planes(Box b, int klo, int khi, double* __restrict__ up, const double* __restrict__ u,
+                 const double* __restrict__ um, const double* __restrict__ rho, SgCoef sg, double beta)
+{
+   const int i = b.ib + 2 + blockIdx.x * blockDim.x + threadIdx.x;
+   const int j = b.jb + 2 + blockIdx.y * blockDim.y + threadIdx.y;
+   const int k = klo + blockIdx.z;
+   if (i > b.ie - 2 || j > b.je - 2 || k > khi) return;
+   double d[3];
+   addsgd4_point<1>(b, i, j, k, u, um, rho, sg, beta, d);
+   const long p = sidx(b, i, j, k);
+#pragma unroll
+   for (int c = 0; c < 3; c++) up[p + b.npts * c] -= d[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -463,8 +496,9 @@ int launch_pred_fused(const Box& b, int nk, const int* onesided, const SbpTables
    const int k2 = onesided[5] == 1 ? nk - 6 : b.ke - 2;
    const int nii = b.ie - b.ib - 3, njj = b.je - b.jb - 3;
    dim3 blk(64, 4, 1);
-   if (k2 >= k1 && variant == 1) {
-      const int rc = launch_sweep_lds(false, b, k1, k2, kchunk, nullptr, u, um, mu, la, rho, sg, up, h, dt, 0.0, s);
+   if (k2 >= k1 && variant >= 1) {
+      const int rc = launch_sweep_lds(false, variant, b, k1, k2, kchunk, nullptr, u, um, mu, la, rho, sg, up, h, dt,
+                                      0.0, s);
       if (rc) return rc;
    } else if (k2 >= k1) {
       dim3 grd(cdiv(nii, 64), cdiv(njj, 4), k2 - k1 + 1);
@@ -484,32 +518,47 @@ int launch_pred_fused(const Box& b, int nk, const int* onesided, const SbpTables
    return check_hip(hipGetLastError(), "fused predictor launch");
 }
 
+long accel_slab_size(const Box& b) { return 3 * b.nij * SLAB_PLANES; }
+
 int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp,
                       const double* up, const double* u, const double* um, const double* mu,
                       const double* la, const double* rho, const SgCoef& sg, double beta,
-                      double* upo, double h, double dt, hipStream_t s,
-                      const std::function<int()>& after_interior, int variant, int kchunk)
+                      double* upo, double* acc, double h, double dt, hipStream_t s,
+                      const std::function<int()>& after_interior, const std::function<int()>& after_damping,
+                      int variant, int kchunk)
 {
    const double cof = 1.0 / (h * h), dt2i = 1.0 / (dt * dt), dt4i12 = dt * dt * dt * dt / 12;
    const int k1 = onesided[4] == 1 ? 7 : b.kb + 2;
    const int k2 = onesided[5] == 1 ? nk - 6 : b.ke - 2;
    const int nii = b.ie - b.ib - 3, njj = b.je - b.jb - 3;
    dim3 blk(64, 4, 1);
-   if (k2 >= k1 && variant == 1) {
-      const int rc = launch_sweep_lds(true, b, k1, k2, kchunk, up, u, um, mu, la, rho, sg, upo, h, dt, beta, s);
+   if (k2 >= k1 && variant >= 1) {
+      int rc = launch_sweep_lds(true, variant, b, k1, k2, kchunk, up, u, um, mu, la, rho, sg, upo, h, dt, beta, s);
       if (rc) return rc;
-   } else if (k2 >= k1) {
-      dim3 grd(cdiv(nii, 64), cdiv(njj, 4), k2 - k1 + 1);
-      k_corr_interior<<<grd, blk, 0, s>>>(b, k1, k2, up, u, um, mu, la, rho, sg, upo, cof, dt2i, dt4i12, beta);
+      if (after_interior && (rc = after_interior())) return rc;
+      if (beta != 0) {
+         dim3 grd(cdiv(nii, 64), cdiv(njj, 4), k2 - k1 + 1);
+         k_addsgd4_planes<<<grd, blk, 0, s>>>(b, k1, k2, upo, u, um, rho, sg, beta);
+      }
+   } else {
+      if (k2 >= k1) {
+         dim3 grd(cdiv(nii, 64), cdiv(njj, 4), k2 - k1 + 1);
+         k_corr_interior<<<grd, blk, 0, s>>>(b, k1, k2, up, u, um, mu, la, rho, sg, upo, cof, dt2i, dt4i12, beta);
+      }
+      int rc;
+      if (after_interior && (rc = after_interior())) return rc;
    }
-   if (after_interior) {
-      const int rc = after_interior();
+   if (after_damping) {
+      const int rc = after_damping();
       if (rc) return rc;
    }
    for (int side = 4; side <= 5; side++) {
       if (onesided[side] != 1) continue;
+      const int ks = side == 4 ? 0 : nk - SLAB_PLANES + 2;  // planes KK(0)..KK(8) of the rows
+      k_accel_slab<<<cdiv(accel_slab_size(b), 256), 256, 0, s>>>(b, ks, up, u, um, dt2i, acc);
       dim3 grd(cdiv(nii, 64), cdiv(njj, 4), 6);
-      k_corr_sbp<<<grd, blk, 0, s>>>(b, side == 4, nk, d_sbp, up, u, um, mu, la, rho, sg, upo, cof, dt2i, dt4i12, beta);
+      k_corr_sbp<<<grd, blk, 0, s>>>(b, side == 4, nk, d_sbp, up, acc, ks, u, um, mu, la, rho, sg, upo, cof,
+                                     dt4i12, beta);
    }
    const long ns = shell_count(b);
    k_shell<<<cdiv(ns, 256), 256, 0, s>>>(b, 1, up, nullptr, nullptr, upo);

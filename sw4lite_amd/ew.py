@@ -112,6 +112,33 @@ def addsgd4(corder, box, up, u, um, rho, dcx, dcy, dcz, strx, stry, strz, cox, c
                                     beta, _stream(stream)))
 
 
+def metric(corder, box, x, y, z, met, jac, stream=None):
+    """EW::metric / metric_rev: met (4 components) and jac from the grid
+    coordinates; raises where z is not increasing in k."""
+    L.check(L.load().sw4hip_metric(corder, *box, _ptr(x), _ptr(y), _ptr(z), _ptr(met), _ptr(jac), _stream(stream)))
+
+
+def rhs4sgcurv(corder, box, u, mu, la, met, jac, lu, onesided, acof, bope, ghcof, strx, stry, stream=None):
+    os_ = np.ascontiguousarray(onesided, dtype=np.int32)
+    L.check(L.load().sw4hip_rhs4sgcurv(corder, *box, _ptr(u), _ptr(mu), _ptr(la), _ptr(met), _ptr(jac), _ptr(lu),
+                                       _ptr(os_), _ptr(np.ascontiguousarray(acof)), _ptr(np.ascontiguousarray(bope)),
+                                       _ptr(np.ascontiguousarray(ghcof)), _ptr(strx), _ptr(stry), _stream(stream)))
+
+
+def addsgd4c(corder, box, up, u, um, rho, dcx, dcy, strx, stry, jac, cox, coy, beta, stream=None):
+    L.check(L.load().sw4hip_addsgd4c(corder, *box, _ptr(up), _ptr(u), _ptr(um), _ptr(rho), _ptr(dcx), _ptr(dcy),
+                                     _ptr(strx), _ptr(stry), _ptr(jac), _ptr(cox), _ptr(coy), beta,
+                                     _stream(stream)))
+
+
+def freesurfcurvisg(corder, box, nz, side, u, mu, la, met, sbop, forcing, strx, stry, stream=None):
+    """side 5 (k = 1) or 6 (k = nz), numbered from 1 as in the reference;
+    forcing None means zero traction."""
+    sb = np.ascontiguousarray(sbop, dtype=np.float64)
+    L.check(L.load().sw4hip_freesurfcurvisg(corder, *box, nz, side, _ptr(u), _ptr(mu), _ptr(la), _ptr(met), _ptr(sb),
+                                            _ptr(forcing), _ptr(strx), _ptr(stry), _stream(stream)))
+
+
 def bcfortsg(corder, box, wind, nx, ny, nz, u, h, bccnd, sbop, mu, la, bforce, strx, stry, stream=None):
     w = np.ascontiguousarray(wind, dtype=np.int32)
     bc = np.ascontiguousarray(bccnd, dtype=np.int32)
@@ -145,7 +172,8 @@ class EW:
 
     # set-up -------------------------------------------------------------
     def set_kernel_variant(self, variant, kchunk=32):
-        """0: one thread per point; 1: LDS-staged k-marching tiles (default)."""
+        """0: one thread per point; 1-3: LDS-staged k-marching tiles (1 plain,
+        2 with register prefetch, 3 prefetch without a register cap)."""
         L.check(self.lib.sw4hip_ew_set_kernel_variant(self.h, int(variant), int(kchunk)))
 
     def set_material(self, rho, mu, la):
@@ -232,7 +260,7 @@ class EW:
 
     def timesteploop_timed(self, nsteps):
         """Fused steps with per-stage GPU times (ms, summed over the steps)."""
-        ms = np.zeros(6)
+        ms = np.zeros(7)
         L.check(self.lib.sw4hip_ew_step_timed(self.h, int(nsteps), _ptr(ms)))
         return ms
 

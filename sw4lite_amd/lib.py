@@ -43,6 +43,10 @@ _SIGS = {
     "sw4hip_dpdmtfort": (I, [I] * 6 + [P] * 4 + [D, P]),
     "sw4hip_addsgd4": (I, [I] * 7 + [P] * 13 + [D, P]),
     "sw4hip_bcfortsg": (I, [I] * 7 + [P, I, I, I, P, D, P, P, P, P, P, P, P, P]),
+    "sw4hip_metric": (I, [I] * 7 + [P] * 5 + [P]),
+    "sw4hip_rhs4sgcurv": (I, [I] * 7 + [P] * 12 + [P]),
+    "sw4hip_addsgd4c": (I, [I] * 7 + [P] * 11 + [D, P]),
+    "sw4hip_freesurfcurvisg": (I, [I] * 9 + [P] * 8 + [P]),
     "sw4hip_ew_create": (I, [C.POINTER(GridDesc), C.POINTER(P)]),
     "sw4hip_ew_destroy": (I, [P]),
     "sw4hip_ew_set_stream": (I, [P, P]),

@@ -212,6 +212,9 @@ def test_full_size_fused_matches_reference_order(torch):
     (1, 32, [2, 2, 2, 2, 0, 2]),
     (1, 7, [2, 2, 2, 2, 0, 0]),
     (1, 5, [2, 2, 2, 2, 2, 2]),
+    (2, 7, [2, 2, 2, 2, 0, 2]),
+    (2, 32, [2, 2, 2, 2, 0, 0]),
+    (3, 5, [2, 2, 2, 2, 0, 2]),
     (0, 32, [2, 2, 2, 2, 0, 2]),
 ])
 def test_sweep_variants_match_oracle(torch, variant, kchunk, bc):
