@@ -399,7 +399,8 @@ void orc_addsgd4c(int ifirst, int ilast, int jfirst, int jlast, int kfirst, int 
 /* Traction-free ghost points on the curved top (or bottom) plane,     */
 /* curvilinear-c.C:465-618 (freesurfcurvisg) / :621-764 (_rev)         */
 /* ------------------------------------------------------------------ */
-/* side 4 (k = 1, ghost k = 0) or 5 (k = nz, ghost nz+1); forcing holds the
+/* side 5 (k = 1, ghost k = 0) or 6 (k = nz, ghost nz+1), numbered from 1
+ * as in the reference; forcing holds the
  * boundary traction, 3 interleaved values per (i,j) of the plane.  The
  * tangential part g of the traction comes from centred differences in the
  * plane; the normal part is N (sbop . u along k) with
@@ -414,7 +415,7 @@ void orc_freesurfcurvisg(int ib, int ie, int jb, int je, int kb, int ke, int nz,
    curv_t s;
    s.b = cmkbox(ib, ie, jb, je, kb, ke, corder);
    s.u = u; s.mu = mu; s.la = la; s.met = met; s.strx = strx; s.stry = stry;
-   const int k = side == 4 ? 1 : nz, kl = side == 4 ? 1 : -1;
+   const int k = side == 5 ? 1 : nz, kl = side == 5 ? 1 : -1;
    const ptrdiff_t ni = ie - ib + 1;
 #pragma omp parallel for schedule(static)
    for (int j = jb + 2; j <= je - 2; j++)
