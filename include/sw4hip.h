@@ -78,6 +78,12 @@ int sw4hip_host_boundary_windows(const sw4hip_grid_desc* g, int wind[36], int np
 int sw4hip_host_moment_source(const sw4hip_grid_desc* g, double x, double y, double z,
                               const double m[6], int maxn, int* n, int* ijk, double* force);
 
+/* The same for a source on a curvilinear grid whose depth z (host array over
+ * the box, i fastest, increasing in k) is given: Source.C:1605-1800 with
+ * the grid derivatives at the source interpolated from the grid. */
+int sw4hip_host_moment_source_curv(const sw4hip_grid_desc* g, const double* zgrid, double x,
+                                   double y, double z, const double m[6], int maxn, int* n,
+                                   int* ijk, double* force);
 /* ----------------------------------------------------------------------- */
 /* Kernel-level entry points (device arrays)                                */
 /* ----------------------------------------------------------------------- */
@@ -197,6 +203,14 @@ int sw4hip_ew_set_material_const(sw4hip_ew* ew, double rho, double cs, double cp
  * multiplied by m0. */
 int sw4hip_ew_add_moment_source(sw4hip_ew* ew, double x, double y, double z, const double m[6],
                                 double t0, double freq, int tfkind);
+/* Makes the grid curvilinear (EW::generate_grid + setup_metric,
+ * EW.C:6164-6300, for a single grid): z is the grid depth over the box,
+ * increasing in k (host or device array per on_device).  Computes the metric
+ * and Jacobian on the device.  Must precede setup and receivers.  The time
+ * step then uses rhs4sgcurv, addsgd4cfort (x and y only) and the curved
+ * free-surface closure, in the reference stage order (sw4hip_ew_step mode 1);
+ * dt comes from the curvilinear eigenvalue bound (EW.C:5063-5136). */
+int sw4hip_ew_set_topography(sw4hip_ew* ew, const double* z, int on_device);
 /* Supergrid arrays, boundary windows, time step (EW::computeDT with
  * t = tmax; nsteps > 0 overrides the step count with dt from CFL), source
  * discretisation.  Must follow the material and the sources. */
@@ -233,7 +247,8 @@ int sw4hip_ew_add_receiver(sw4hip_ew* ew, double x, double y, double z, int mode
 /* Samples recorded so far: out[sample][receiver][3] (host buffer of
  * nsamples*nreceivers*3 doubles, or NULL to query *nsamples). */
 int sw4hip_ew_get_receivers(sw4hip_ew* ew, int* nsamples, double* out);
-/* Device pointers of the state: which = 0 U, 1 Um, 2 rho, 3 mu, 4 lambda. */
+/* Device pointers of the state: which = 0 U, 1 Um, 2 rho, 3 mu, 4 lambda,
+ * 5 metric, 6 Jacobian (curvilinear grid). */
 int sw4hip_ew_device_array(sw4hip_ew* ew, int which, double** ptr);
 int sw4hip_ew_sync(sw4hip_ew* ew);
 

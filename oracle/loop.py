@@ -24,9 +24,11 @@ def npts(box):
 
 class OracleRun:
     def __init__(self, nx, ny, nz, h, bc, rho, mu, la, sources, corder=1, sg_gp=20, beta=0.02, cfl=1.3,
-                 tmax=0.0, nsteps=0, kernels="oracle"):
+                 tmax=0.0, nsteps=0, kernels="oracle", zgrid=None):
         """rho, mu, la: arrays over the box (npts) or scalars.  sources:
-        list of dicts x,y,z,m(6),t0,freq,tfkind."""
+        list of dicts x,y,z,m(6),t0,freq,tfkind.  zgrid: grid depth over the
+        box for a curvilinear grid (rhs4sgcurv, addsgd4cfort, freesurfcurvisg;
+        EW.C:3218-3243, 3479-3497, 3560-3582)."""
         self.stencil_lib = None
         if kernels == "reference":
             self.stencil_lib = O.reference()
@@ -43,9 +45,35 @@ class OracleRun:
         self.wind, _ = ew.host_boundary_windows(self.desc)
         self.onesided = [0, 0, 0, 0, int(bc[4] == L.BC_FREE), int(bc[5] == L.BC_FREE)]
         self.acof, self.ghcof, self.bope, self.sbop = ew.stencil_coefficients()
-        # EW::computeDT (EW.C:5041-5146)
-        eig = np.max((4 * self.mu + self.la) / self.rho)
-        dt = min(1e10, cfl * h * (1 / np.sqrt(eig)))
+        self.zgrid = None if zgrid is None else np.ascontiguousarray(zgrid, dtype=np.float64)
+        if self.zgrid is not None:
+            self.met, self.jac = np.zeros(4 * n), np.zeros(n)
+            if O.metric(1, self.box, h, self.zgrid, self.met, self.jac) != 0:
+                raise ValueError("folded curvilinear grid")
+            if corder == 0:
+                self.met = np.ascontiguousarray(self.met.reshape(4, n).T.reshape(-1))
+            self.onesided[5] = 0
+        # EW::computeDT (EW.C:5041-5146); curvilinear: most negative eigenvalue
+        # of the principal-part matrix per point (EW.C:5063-5136), here from
+        # numpy's symmetric eigensolver
+        if self.zgrid is None:
+            eig = np.max((4 * self.mu + self.la) / self.rho)
+            dt = min(1e10, cfl * h * (1 / np.sqrt(eig)))
+        else:
+            m = self.met.reshape(4, n) if corder == 1 else self.met.reshape(n, 4).T
+            la_, mu_ = self.la, self.mu
+            l2, ji = la_ + 2 * mu_, 1 / self.jac
+            s1, s2, s3, s4 = (m[c] ** 2 for c in range(4))
+            A = np.empty((n, 3, 3))
+            A[:, 0, 0] = -4 * (s1 * l2 + s1 * mu_ + s2 * l2 + s3 * mu_ + s4 * mu_) * ji
+            A[:, 0, 1] = A[:, 1, 0] = -4 * m[1] * m[2] * (mu_ + la_) * ji
+            A[:, 0, 2] = A[:, 2, 0] = -4 * m[1] * m[3] * (mu_ + la_) * ji
+            A[:, 1, 1] = -4 * (s1 * mu_ + s1 * l2 + s2 * mu_ + s3 * l2 + s4 * mu_) * ji
+            A[:, 1, 2] = A[:, 2, 1] = -4 * m[2] * m[3] * (mu_ + la_) * ji
+            A[:, 2, 2] = -4 * (2 * s1 * mu_ + s2 * mu_ + s3 * mu_ + s4 * l2) * ji
+            w0 = np.linalg.eigvalsh(A)[:, 0]
+            eig = np.max(-w0 / (4 * self.rho))
+            dt = min(1e10, cfl * (1 / np.sqrt(eig)))
         if tmax > 0:
             nst = int(tmax / dt + 0.5)
             nst = max(nst, 1)
@@ -56,7 +84,10 @@ class OracleRun:
         # grid-point sources, sorted by grid index (stable)
         ijk, frc, fq, t0, tf = [], [], [], [], []
         for s in sources:
-            a, b = ew.host_moment_source(self.desc, s["x"], s["y"], s["z"], s["m"])
+            if self.zgrid is None:
+                a, b = ew.host_moment_source(self.desc, s["x"], s["y"], s["z"], s["m"])
+            else:
+                a, b = ew.host_moment_source_curv(self.desc, self.zgrid, s["x"], s["y"], s["z"], s["m"])
             ijk.append(a)
             frc.append(b)
             fq += [s["freq"]] * len(a)
@@ -99,12 +130,36 @@ class OracleRun:
 
     def rhs(self, u):
         sg = self.sg
+        if self.zgrid is not None:
+            O.rhs4sgcurv(self.corder, self.box, u, self.mu, self.la, self.met, self.jac, self.Lu, self.onesided,
+                         self.acof, self.bope, self.ghcof, sg["strx"], sg["stry"], lib=self.stencil_lib)
+            return
         O.rhs4sg(self.corder, self.box, self.nz, self.onesided, self.acof, self.bope, self.ghcof, self.Lu, u, self.mu,
                  self.la, self.h, sg["strx"], sg["stry"], sg["strz"], lib=self.stencil_lib)
 
     def enforce_bc(self, u):
-        O.bcfortsg(self.corder, self.box, self.wind, self.nx, self.ny, self.nz, u, self.h, self.bc, self.sbop,
+        if self.zgrid is None:
+            O.bcfortsg(self.corder, self.box, self.wind, self.nx, self.ny, self.nz, u, self.h, self.bc, self.sbop,
+                       self.mu, self.la, self.bforce, self.sg["strx"], self.sg["stry"])
+            return
+        # EW::enforceBC (EW.C:3446-3501): bcfortsg, then the curved closure
+        # on the top, which replaces the Cartesian one at the same points
+        bc = [8 if b == L.BC_FREE else b for b in self.bc]
+        O.bcfortsg(self.corder, self.box, self.wind, self.nx, self.ny, self.nz, u, self.h, bc, self.sbop,
                    self.mu, self.la, self.bforce, self.sg["strx"], self.sg["stry"])
+        if self.bc[4] == L.BC_FREE:
+            O.freesurfcurvisg(self.corder, self.box, self.nz, 5, u, self.mu, self.la, self.met, self.sbop,
+                              np.zeros(3 * (self.nx + 4) * (self.ny + 4)), self.sg["strx"], self.sg["stry"])
+
+    def damp(self):
+        sg = self.sg
+        if self.zgrid is not None:
+            O.addsgd4c(self.corder, self.box, self.Up, self.U, self.Um, self.rho, sg["dcx"], sg["dcy"], sg["strx"],
+                       sg["stry"], self.jac, sg["cox"], sg["coy"], self.beta)
+            return
+        O.addsgd4(self.corder, self.box, self.Up, self.U, self.Um, self.rho, sg["dcx"], sg["dcy"], sg["dcz"],
+                  sg["strx"], sg["stry"], sg["strz"], sg["cox"], sg["coy"], sg["coz"], self.beta,
+                  lib=self.stencil_lib)
 
     def step(self, n=1):
         sg = self.sg
@@ -118,9 +173,7 @@ class OracleRun:
             O.dpdmtfort(self.box, self.Up, self.U, self.Um, self.Uacc, 1.0 / (dt * dt))
             self.rhs(self.Uacc)
             O.corrfort(self.corder, self.box, self.Up, self.Lu, self.F, self.rho, dt * dt * dt * dt)
-            O.addsgd4(self.corder, self.box, self.Up, self.U, self.Um, self.rho, sg["dcx"], sg["dcy"], sg["dcz"],
-                      sg["strx"], sg["stry"], sg["strz"], sg["cox"], sg["coy"], sg["coz"], self.beta,
-                      lib=self.stencil_lib)
+            self.damp()
             self.enforce_bc(self.Up)
             self.t += dt
             self.cycle += 1

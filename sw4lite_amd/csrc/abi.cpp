@@ -154,6 +154,27 @@ int sw4hip_host_moment_source(const sw4hip_grid_desc* d, double x, double y, dou
    return OK;
 }
 
+int sw4hip_host_moment_source_curv(const sw4hip_grid_desc* d, const double* zgrid, double x, double y, double z,
+                                   const double m[6], int maxn, int* n, int* ijk, double* force)
+{
+   int rc;
+   if ((rc = check_desc(d))) return rc;
+   if (!zgrid || !m || !n || !ijk || !force) return set_error(E_INVAL, "null source buffer");
+   MomentSource s{x, y, z, {m[0], m[1], m[2], m[3], m[4], m[5]}, 0.0, 1.0, TF_GAUSSIAN};
+   std::vector<GridPointSource> pts;
+   const int q = moment_source_points_curv(to_grid(d), zgrid, s, pts);
+   if (q == -1) return set_error(E_INVAL, "grid too small for the source stencil");
+   if (q == -2) return set_error(E_INVAL, "source outside the curvilinear grid");
+   if (q) return set_error(E_INVAL, "folded grid at the source");
+   if ((int)pts.size() > maxn) return set_error(E_RANGE, "more grid-point sources than maxn");
+   *n = (int)pts.size();
+   for (size_t k = 0; k < pts.size(); k++) {
+      ijk[3 * k] = pts[k].i; ijk[3 * k + 1] = pts[k].j; ijk[3 * k + 2] = pts[k].k;
+      for (int c = 0; c < 3; c++) force[3 * k + c] = pts[k].f[c];
+   }
+   return OK;
+}
+
 // ---------------------------------------------------------------------------
 // kernel level
 // ---------------------------------------------------------------------------
@@ -439,10 +460,19 @@ int sw4hip_ew_add_moment_source(sw4hip_ew* h, double x, double y, double z, cons
    if (!m) return set_error(E_INVAL, "null moment tensor");
    if (!(freq > 0)) return set_error(E_INVAL, "source frequency must be positive");
    const GridSetup& g = h->ew->m_opt.grid;
-   if (x < 0 || x > (g.nx - 1) * g.h || y < 0 || y > (g.ny - 1) * g.h || z < g.zmin || z > g.zmin + (g.nz - 1) * g.h)
+   if (x < 0 || x > (g.nx - 1) * g.h || y < 0 || y > (g.ny - 1) * g.h)
+      return set_error(E_INVAL, "source positioned outside the grid");
+   // on a curvilinear grid the depth is checked against the grid at setup
+   if (!h->ew->curvilinear() && (z < g.zmin || z > g.zmin + (g.nz - 1) * g.h))
       return set_error(E_INVAL, "source positioned outside the grid");
    MomentSource s{x, y, z, {m[0], m[1], m[2], m[3], m[4], m[5]}, t0, freq, tfkind};
    return h->ew->add_moment_source(s);
+}
+
+int sw4hip_ew_set_topography(sw4hip_ew* h, const double* z, int on_device)
+{
+   EWCHK(h);
+   return h->ew->set_topography(z, on_device != 0);
 }
 
 int sw4hip_ew_setup(sw4hip_ew* h, double tmax, int nsteps)
@@ -570,7 +600,12 @@ int sw4hip_ew_device_array(sw4hip_ew* h, int which, double** ptr)
    case 2: *ptr = e.mRho.dev; break;
    case 3: *ptr = e.mMu.dev; break;
    case 4: *ptr = e.mLambda.dev; break;
-   default: return set_error(E_INVAL, "which must be 0..4");
+   case 5:
+   case 6:
+      if (!e.curvilinear()) return set_error(E_STATE, "no topography set: the grid has no metric");
+      *ptr = which == 5 ? e.mMetric.dev : e.mJ.dev;
+      break;
+   default: return set_error(E_INVAL, "which must be 0..6");
    }
    return OK;
 }

@@ -104,6 +104,39 @@ int EW::add_moment_source(const MomentSource& s)
    return OK;
 }
 
+int EW::set_topography(const double* z, bool on_device)
+{
+   if (m_setup_done) return set_error(E_STATE, "EW: topography must be set before setup");
+   if (!z) return set_error(E_INVAL, "EW: null topography");
+   const GridSetup& g = m_opt.grid;
+   for (int s = 0; s < 4; s++)
+      if (g.bc[s] == BC_PERIODIC) return set_error(E_INVAL, "EW: periodic sides are not supported on a curvilinear grid");
+   if (g.bc[5] == BC_FREE) return set_error(E_INVAL, "EW: the curvilinear grid has no free surface at the bottom");
+   const long n = m_box.npts;
+   int rc;
+   m_zhost.resize(n);
+   if ((rc = check_hip(hipMemcpy(m_zhost.data(), z, sizeof(double) * n,
+                                 on_device ? hipMemcpyDeviceToHost : hipMemcpyHostToHost), "topography copy")))
+      return rc;
+   if (!mMetric.dev && ((rc = alloc(mMetric, 4)) || (rc = alloc(mJ, 1)))) return rc;
+   // the metric kernel reads z from the device: stage it in the Lu array
+   if (!mLu.dev && (rc = alloc(mLu, 3))) return rc;
+   if ((rc = check_hip(hipMemcpyAsync(mLu.dev, m_zhost.data(), sizeof(double) * n, hipMemcpyHostToDevice, m_stream),
+                       "topography upload")))
+      return rc;
+   int* flag = reinterpret_cast<int*>(m_dred);
+   if ((rc = check_hip(hipMemsetAsync(flag, 0, sizeof(int), m_stream), "metric flag")) ||
+       (rc = launch_metric(m_opt.corder, m_box, g.h, mLu.dev, mMetric.dev, mJ.dev, flag, m_stream)))
+      return rc;
+   int bad = 0;
+   if ((rc = check_hip(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, m_stream), "metric flag")) ||
+       (rc = check_hip(hipStreamSynchronize(m_stream), "metric flag")))
+      return rc;
+   if (bad) return set_error(E_INVAL, "EW: topography grid is folded (z not increasing in k)");
+   m_curv = true;
+   return OK;
+}
+
 SgCoef EW::sg() const
 {
    const long ni = m_box.ni, nj = m_box.je - m_box.jb + 1, nk = m_box.ke - m_box.kb + 1;
@@ -187,16 +220,21 @@ int EW::setup(double tmax, int nsteps)
          return rc;
    }
    make_boundary_windows(g, m_wind, m_nbcpts);
-   // EW::computeDT (EW.C:5041-5146), Cartesian grid
-   if ((rc = launch_maxeig(m_box.npts, mMu.dev, mLambda.dev, mRho.dev, m_dred, m_nred, m_dred + 3 * m_nred, m_stream)))
-      return rc;
+   // EW::computeDT (EW.C:5041-5146): Cartesian grid, or the curvilinear
+   // grid's principal-part eigenvalues (EW.C:5063-5136, dt = CFL/sqrt(max))
+   if (m_curv)
+      rc = launch_maxeig_curv(m_opt.corder, m_box, mMetric.dev, mJ.dev, mMu.dev, mLambda.dev, mRho.dev, m_dred, m_nred,
+                              m_dred + 3 * m_nred, m_stream);
+   else
+      rc = launch_maxeig(m_box.npts, mMu.dev, mLambda.dev, mRho.dev, m_dred, m_nred, m_dred + 3 * m_nred, m_stream);
+   if (rc) return rc;
    double eigmax = 0;
    if ((rc = check_hip(hipMemcpyAsync(&eigmax, m_dred + 3 * m_nred, sizeof(double), hipMemcpyDeviceToHost, m_stream), "dt readback")) ||
        (rc = check_hip(hipStreamSynchronize(m_stream), "dt readback")))
       return rc;
    if (!(eigmax > 0)) return set_error(E_INVAL, "EW: material gives a non-positive (4mu+lambda)/rho");
    const double ieigmax = 1 / std::sqrt(eigmax);
-   mDt = std::min(1e10, m_opt.cfl * g.h * ieigmax);
+   mDt = std::min(1e10, m_opt.cfl * (m_curv ? 1.0 : g.h) * ieigmax);
    if (tmax > 0) {
       mNumberOfTimeSteps = static_cast<int>(tmax / mDt + 0.5);
       if (mNumberOfTimeSteps == 0) mNumberOfTimeSteps = 1;
@@ -206,8 +244,12 @@ int EW::setup(double tmax, int nsteps)
    }
    // grid-point sources
    m_gps.clear();
-   for (const MomentSource& s : m_sources)
-      if (moment_source_points(g, s, m_gps)) return set_error(E_INVAL, "EW: grid too small for the source stencil");
+   for (const MomentSource& s : m_sources) {
+      const int q = m_curv ? moment_source_points_curv(g, m_zhost.data(), s, m_gps) : moment_source_points(g, s, m_gps);
+      if (q == -1) return set_error(E_INVAL, "EW: grid too small for the source stencil");
+      if (q == -2) return set_error(E_INVAL, "EW: source outside the curvilinear grid");
+      if (q) return set_error(E_INVAL, "EW: folded grid at the source");
+   }
    sort_grid_point_sources(g, m_gps, m_first);
    if (!m_gps.empty() && (rc = upload_sources())) return rc;
    mT = 0;
@@ -247,6 +289,14 @@ int EW::add_receiver(double x, double y, double z, int mode, int* index)
    if (y - (j - 0.5) * g.h > 0.) j++;
    int k = (int)std::floor((z - g.zmin) / g.h) + 1;
    if (z - (g.zmin + (k - 0.5) * g.h) > 0.) k++;
+   if (m_curv && i >= 1 && i <= g.nx && j >= 1 && j <= g.ny) {
+      // nearest grid point along the column (i, j) of the curvilinear grid
+      const long ni = g.nx + 4, nij = ni * (g.ny + 4);
+      const double* col = m_zhost.data() + (i + 1) + ni * (j + 1);
+      k = 1;
+      for (int kk = 2; kk <= g.nz; kk++)
+         if (std::fabs(col[nij * (kk + 1)] - z) < std::fabs(col[nij * (k + 1)] - z)) k = kk;
+   }
    if (i < 1 || i > g.nx || j < 1 || j > g.ny || k < 1 || k > g.nz)
       return set_error(E_INVAL, "EW: receiver outside the grid");
    if (index) *index = (int)m_rec.size() / 4;
@@ -288,6 +338,9 @@ int EW::Force(double t, Sarray& F, bool tt)
 int EW::evalRHS(Sarray& U, Sarray& Mu, Sarray& Lambda, Sarray& Lu)
 {
    const SgCoef c = sg();
+   if (m_curv)  // EW.C:3218-3243
+      return launch_rhs4sgcurv(m_opt.corder, m_box, m_onesided, m_dsbp, U.dev, Mu.dev, Lambda.dev, mMetric.dev, mJ.dev,
+                               c.strx, c.stry, Lu.dev, m_stream);
    return launch_rhs4sg(m_opt.corder, m_box, m_opt.grid.nz, m_onesided, m_dsbp, Lu.dev, U.dev, Mu.dev,
                         Lambda.dev, m_opt.grid.h, c.strx, c.stry, c.strz, m_stream);
 }
@@ -309,6 +362,8 @@ int EW::evalDpDmInTime(Sarray& Up, Sarray& U, Sarray& Um, Sarray& Uacc)
 
 int EW::addSuperGridDamping(Sarray& Up, Sarray& U, Sarray& Um, Sarray& Rho)
 {
+   if (m_curv)  // no damping along k on the curvilinear grid (EW.C:3560-3582, 4745)
+      return launch_addsgd4c(m_opt.corder, m_box, Up.dev, U.dev, Um.dev, Rho.dev, mJ.dev, sg(), m_opt.beta, m_stream);
    return launch_addsgd4(m_opt.corder, m_box, Up.dev, U.dev, Um.dev, Rho.dev, sg(), m_opt.beta, m_stream);
 }
 
@@ -320,8 +375,18 @@ int EW::enforceBC(Sarray& U, Sarray& Mu, Sarray& Lambda, double t)
    const GridSetup& g = m_opt.grid;
    const double* bforce[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
    const SgCoef c = sg();
-   return launch_bcfortsg(m_opt.corder, m_box, m_wind, g.nx, g.ny, g.nz, U.dev, g.h, g.bc, m_dsbp, Mu.dev,
-                          Lambda.dev, bforce, c.strx, c.stry, m_stream);
+   if (!m_curv)
+      return launch_bcfortsg(m_opt.corder, m_box, m_wind, g.nx, g.ny, g.nz, U.dev, g.h, g.bc, m_dsbp, Mu.dev,
+                             Lambda.dev, bforce, c.strx, c.stry, m_stream);
+   // curvilinear top: the Cartesian closure of bcfortsg is overwritten by
+   // freesurfcurvisg at the same points (EW.C:3479-3497), so it is skipped
+   int bc[6];
+   for (int s = 0; s < 6; s++) bc[s] = g.bc[s] == BC_FREE ? BC_NONE : g.bc[s];
+   int rc = launch_bcfortsg(m_opt.corder, m_box, m_wind, g.nx, g.ny, g.nz, U.dev, g.h, bc, m_dsbp, Mu.dev, Lambda.dev,
+                            bforce, c.strx, c.stry, m_stream);
+   if (rc || g.bc[4] != BC_FREE) return rc;
+   return launch_freesurf_curv(m_opt.corder, m_box, g.nz, 4, U.dev, Mu.dev, Lambda.dev, mMetric.dev, m_dsbp, nullptr,
+                               c.strx, c.stry, m_stream);
 }
 
 // EW::cycleSolutionArrays (EW.C:3060-3082): pointer rotation, no copy
@@ -367,6 +432,7 @@ int EW::step_fused(int n, double* stage_ms)
 {
    if (!m_setup_done) return set_error(E_STATE, "EW: step before setup");
    if (!m_opt.corder) return set_error(E_INVAL, "EW: the fused step needs the component-blocked layout (corder=1)");
+   if (m_curv) return set_error(E_INVAL, "EW: the fused step is Cartesian; step a curvilinear grid in the reference order");
    const GridSetup& g = m_opt.grid;
    const SgCoef c = sg();
    const SourceTable st = sources();
@@ -424,6 +490,7 @@ int EW::step_fused(int n, double* stage_ms)
 int EW::point_source_error(double& linf, double& l2, double& solinf)
 {
    if (!m_setup_done || m_sources.empty()) return set_error(E_STATE, "EW: point-source error needs setup and a source");
+   if (m_curv) return set_error(E_INVAL, "EW: the analytic point-source solution is for a Cartesian grid");
    const GridSetup& g = m_opt.grid;
    const MomentSource& s0 = m_sources[0];
    if (s0.tfkind != TF_GAUSSIAN && s0.tfkind != TF_C6SMOOTHBUMP)

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "kernels_common.h"
+#include "kernels_curv.h"
 #include "kernels_step.h"
 #include "setup.h"
 
@@ -48,6 +49,9 @@ public:
    int set_material(const double* rho, const double* mu, const double* la, bool on_device);
    int set_material_const(double rho, double cs, double cp);
    int add_moment_source(const MomentSource& s);
+   // Curvilinear grid (EW::generate_grid + setup_metric, EW.C:6164-6300):
+   // z = grid depth over the box, increasing with k; before setup.
+   int set_topography(const double* z, bool on_device);
    int setup(double tmax, int nsteps);  // setupRun + computeDT
 
    // --- reference step methods (EW.h:93-127), single Cartesian grid ---
@@ -59,6 +63,8 @@ public:
    int addSuperGridDamping(Sarray& Up, Sarray& U, Sarray& Um, Sarray& Rho);
    int enforceBC(Sarray& U, Sarray& Mu, Sarray& Lambda, double t);
    void cycleSolutionArrays();
+
+   bool curvilinear() const { return m_curv; }
 
    // --- time loop ---
    int step_reference(int n);  // stage order of EW::timesteploop (EW.C:2527-2842)
@@ -81,6 +87,8 @@ public:
    // state access
    Sarray mU, mUm, mUp, mUp2, mRho, mMu, mLambda;
    Sarray mLu, mF, mUacc;  // only for step_reference
+   Sarray mMetric, mJ;     // curvilinear grid only
+   std::vector<double> m_zhost;
    double* m_dacc = nullptr;  // acceleration on the planes the corrector's SBP rows read
    double mDt = 0, mT = 0;
    int mNumberOfTimeSteps = 0, mCycle = 0;
@@ -112,7 +120,7 @@ private:
    double* m_drecdata = nullptr;
    int m_maxsamples = 0;
    int m_nred = 1024;
-   bool m_setup_done = false, m_material_set = false;
+   bool m_setup_done = false, m_material_set = false, m_curv = false;
    std::vector<void*> m_allocs;
 };
 

@@ -23,4 +23,10 @@ int launch_freesurf_curv(int co, const Box& b, int nz, int side, double* u, cons
                          const double* met, const SbpTables* d_sbp, const double* forcing, const double* strx,
                          const double* stry, hipStream_t s);
 
+// max over the box of -lambda_min(A)/(4 rho), A the principal-part matrix
+// of the curvilinear operator (EW.C:5063-5136); npartial blocks, result in
+// out[0].
+int launch_maxeig_curv(int co, const Box& b, const double* met, const double* jac, const double* mu, const double* la,
+                       const double* rho, double* partial, int npartial, double* out, hipStream_t s);
+
 }  // namespace sw4amd

@@ -406,6 +406,80 @@ k_freesurf_curv(Box b, int k, int kl, CurvG<CO> s, const SbpTables* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Time step of the curvilinear grid (EW::computeDT, EW.C:5063-5136)
+// ---------------------------------------------------------------------------
+// Per point the symmetric matrix A (EW.C:5089-5102) of the curvilinear
+// operator's principal part; the time step follows from its most negative
+// eigenvalue: max over points of -lambda_min(A)/(4 rho).  The reference calls
+// LAPACK dspev; here the eigenvalue comes from the closed form for symmetric
+// 3x3 matrices (O. K. Smith, "Eigenvalues of a symmetric 3x3 matrix",
+// Comm. ACM 4(4):168, 1961).
+constexpr int CRED = 256;
+
+__device__ __forceinline__ double sym3_min_eig(double a11, double a12, double a13, double a22, double a23,
+                                               double a33)
+{
+   const double p1 = a12 * a12 + a13 * a13 + a23 * a23;
+   const double q = (a11 + a22 + a33) / 3;
+   if (p1 == 0) return fmin(a11, fmin(a22, a33));
+   const double b11 = a11 - q, b22 = a22 - q, b33 = a33 - q;
+   const double p2 = b11 * b11 + b22 * b22 + b33 * b33 + 2 * p1;
+   const double p = sqrt(p2 / 6);
+   const double ip = 1 / p;
+   // r = det(B)/2 with B = (A - qI)/p
+   const double det = b11 * (b22 * b33 - a23 * a23) - a12 * (a12 * b33 - a23 * a13) + a13 * (a12 * a23 - b22 * a13);
+   const double r = fmin(1.0, fmax(-1.0, det * ip * ip * ip / 2));
+   const double phi = acos(r) / 3;
+   // eigenvalues q + 2p cos(phi + 2 pi m / 3); the smallest is m = 1
+   return q + 2 * p * cos(phi + 2.0943951023931954923);
+}
+
+template <int CO>
+__global__ void __launch_bounds__(CRED)
+k_maxeig_curv(Box b, const double* __restrict__ met, const double* __restrict__ jac, const double* __restrict__ mu,
+              const double* __restrict__ la, const double* __restrict__ rho, double* __restrict__ partial)
+{
+   __shared__ double sh[CRED];
+   double m = -1;
+   for (long p = blockIdx.x * (long)CRED + threadIdx.x; p < b.npts; p += (long)gridDim.x * CRED) {
+      const long m0 = CO ? p : 4 * p, ms = CO ? b.npts : 1;
+      const double m1 = met[m0], m2 = met[m0 + ms], m3 = met[m0 + 2 * ms], m4 = met[m0 + 3 * ms];
+      const double l = la[p], u = mu[p], l2 = l + 2 * u, ji = 1 / jac[p];
+      const double s1 = m1 * m1, s2 = m2 * m2, s3 = m3 * m3, s4 = m4 * m4;
+      const double a11 = -4 * (s1 * l2 + s1 * u + s2 * l2 + s3 * u + s4 * u) * ji;
+      const double a12 = -4 * m2 * m3 * (u + l) * ji;
+      const double a13 = -4 * m2 * m4 * (u + l) * ji;
+      const double a22 = -4 * (s1 * u + s1 * l2 + s2 * u + s3 * l2 + s4 * u) * ji;
+      const double a23 = -4 * m3 * m4 * (u + l) * ji;
+      const double a33 = -4 * (s1 * u + s1 * u + s2 * u + s3 * u + s4 * l2) * ji;
+      const double w0 = sym3_min_eig(a11, a12, a13, a22, a23, a33);
+      m = fmax(m, -w0 / (4 * rho[p]));
+   }
+   sh[threadIdx.x] = m;
+   __syncthreads();
+   for (int o = CRED / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(CRED) k_final_max_curv(int n, const double* __restrict__ partial,
+                                                         double* __restrict__ out)
+{
+   __shared__ double sh[CRED];
+   double m = -1;
+   for (int q = threadIdx.x; q < n; q += CRED) m = fmax(m, partial[q]);
+   sh[threadIdx.x] = m;
+   __syncthreads();
+   for (int o = CRED / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 int launch_metric(int co, const Box& b, double h, const double* z, double* met, double* jac, int* bad,
@@ -469,6 +543,17 @@ int launch_freesurf_curv(int co, const Box& b, int nz, int side, double* u, cons
    else
       k_freesurf_curv<0><<<grd, blk, 0, s>>>(b, k, kl, CurvG<0>{b, u, mu, la, met, strx, stry}, d_sbp, forcing, u);
    return check_hip(hipGetLastError(), "curvilinear free-surface launch");
+}
+
+int launch_maxeig_curv(int co, const Box& b, const double* met, const double* jac, const double* mu, const double* la,
+                       const double* rho, double* partial, int npartial, double* out, hipStream_t s)
+{
+   if (co)
+      k_maxeig_curv<1><<<npartial, CRED, 0, s>>>(b, met, jac, mu, la, rho, partial);
+   else
+      k_maxeig_curv<0><<<npartial, CRED, 0, s>>>(b, met, jac, mu, la, rho, partial);
+   k_final_max_curv<<<1, CRED, 0, s>>>(npartial, partial, out);
+   return check_hip(hipGetLastError(), "curvilinear maxeig launch");
 }
 
 }  // namespace sw4amd

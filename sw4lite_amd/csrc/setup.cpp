@@ -254,6 +254,185 @@ int moment_source_points(const GridSetup& g, const MomentSource& src, std::vecto
    return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Moment source on a curvilinear grid (Source.C:1605-1700 with the forcing
+// of :1724-1800 and jaci = 1/J(i,j,k)).  The grid is given by its depth z
+// over the box; the reference evaluates z and its derivatives at the source
+// from its analytic grid mapping (Source.C:2191-2330), here they come from
+// the grid itself: the 4th-order grid derivatives (as in the metric)
+// interpolated to the source with the 6-point delta weights, which are exact
+// for polynomials of degree <= 4.
+// ---------------------------------------------------------------------------
+namespace {
+
+// derivative along a grid line, same closures as the metric
+// (curvilinear-c.C:78-146); f(t) = f[t * st], position s of n
+double line_deriv(const double* f, long st, int s, int n)
+{
+   const double c1 = 2.0 / 3, c2 = -1.0 / 12;
+   auto F = [&](int t) { return f[(long)t * st]; };
+   if (s >= 2 && s <= n - 3) return c2 * (F(s + 2) - F(s - 2)) + c1 * (F(s + 1) - F(s - 1));
+   if (s == 0)
+      return -2.25 * F(0) + (4 + 5.0 / 6) * F(1) - (14.0 / 3) * F(2) + 3 * F(3) - (1 + 1.0 / 12) * F(4) +
+             (1.0 / 6) * F(5);
+   if (s == 1)
+      return -(1.0 / 6) * F(0) - 1.25 * F(1) + (1 + 4.0 / 3) * F(2) - (4.0 / 3) * F(3) + 0.5 * F(4) -
+             (1.0 / 12) * F(5);
+   if (s == n - 2)
+      return (1.0 / 6) * F(n - 1) + 1.25 * F(n - 2) - (1 + 4.0 / 3) * F(n - 3) + (4.0 / 3) * F(n - 4) -
+             0.5 * F(n - 5) + (1.0 / 12) * F(n - 6);
+   return 2.25 * F(n - 1) - (4 + 5.0 / 6) * F(n - 2) + (14.0 / 3) * F(n - 3) - 3 * F(n - 4) +
+          (1 + 1.0 / 12) * F(n - 5) - (1.0 / 6) * F(n - 6);
+}
+
+struct CurvGrid {
+   const GridSetup& g;
+   const double* z;
+   int ni, nj, nk;
+   long nij;
+   long p(int i, int j, int k) const { return (i + 1) + (long)ni * (j + 1) + nij * (k + 1); }
+   double Z(int i, int j, int k) const { return z[p(i, j, k)]; }
+   double zp(int i, int j, int k) const { return line_deriv(z + p(-1, j, k), 1, i + 1, ni); }
+   double zq(int i, int j, int k) const { return line_deriv(z + p(i, -1, k), ni, j + 1, nj); }
+   double zr(int i, int j, int k) const { return line_deriv(z + p(i, j, -1), nij, k + 1, nk); }
+};
+
+// Interpolated grid depth and k-derivative at (q0, r0, s) with the delta
+// weights centred at (ic, jc, kc).
+void column_at(const CurvGrid& G, const double wi[6], const double wj[6], int ic, int jc, double s, double& z,
+               double& zs)
+{
+   const int kc = std::min(std::max((int)std::floor(s), 1), G.g.nz - 1);
+   double wk[6];
+   delta_weights(s - kc, wk);
+   z = zs = 0;
+   for (int c = 0; c < 6; c++) {
+      const int k = kc - 2 + c;
+      if (k < -1 || k > G.g.nz + 2) continue;
+      for (int b = 0; b < 6; b++)
+         for (int a = 0; a < 6; a++) {
+            const double w = wi[a] * wj[b] * wk[c];
+            z += w * G.Z(ic - 2 + a, jc - 2 + b, k);
+            zs += w * G.zr(ic - 2 + a, jc - 2 + b, k);
+         }
+   }
+}
+
+}  // namespace
+
+int moment_source_points_curv(const GridSetup& g, const double* zgrid, const MomentSource& src,
+                              std::vector<GridPointSource>& out)
+{
+   const double h = g.h;
+   const int Ni = g.nx, Nj = g.ny, Nz = g.nz;
+   if (Ni < 6 || Nj < 6 || Nz < 6) return -1;
+   CurvGrid G{g, zgrid, Ni + 4, Nj + 4, Nz + 4, (long)(Ni + 4) * (Nj + 4)};
+   const double q0 = src.x / h + 1, r0 = src.y / h + 1;
+   int ic = (int)std::floor(q0), jc = (int)std::floor(r0);
+   if (ic <= 2) ic = 3;
+   if (ic >= Ni - 2) ic = Ni - 3;
+   if (jc <= 2) jc = 3;
+   if (jc >= Nj - 2) jc = Nj - 3;
+   const double ai = q0 - ic, bi = r0 - jc;
+   double wi[6], wj[6], dwi[6], dwj[6];
+   delta_weights(ai, wi);
+   delta_weights(bi, wj);
+   ddelta_weights(ai, dwi);
+   ddelta_weights(bi, dwj);
+   // s0 with z(q0, r0, s0) = src.z: bracket on the interpolated column, then
+   // Newton on the delta-weight interpolant (EW::invert_curvilinear_grid_mapping)
+   double s0 = -1;
+   {
+      double zlo = 0, dum = 0;
+      column_at(G, wi, wj, ic, jc, 1.0, zlo, dum);
+      if (src.z < zlo) return -2;  // above the surface
+      for (int k = 1; k < Nz; k++) {
+         double za, zb;
+         column_at(G, wi, wj, ic, jc, (double)k, za, dum);
+         column_at(G, wi, wj, ic, jc, (double)(k + 1), zb, dum);
+         if (src.z >= za && src.z <= zb) {
+            s0 = k + (src.z - za) / (zb - za);
+            break;
+         }
+      }
+      if (s0 < 0) return -2;  // below the grid
+      for (int it = 0; it < 8; it++) {
+         double zz, zs;
+         column_at(G, wi, wj, ic, jc, s0, zz, zs);
+         if (!(zs > 0)) return -3;
+         const double ds = (zz - src.z) / zs;
+         s0 = std::min(std::max(s0 - ds, 1.0), (double)Nz);
+         if (std::fabs(ds) < 1e-14) break;
+      }
+   }
+   int kc = (int)std::floor(s0);
+   if (kc >= Nz) kc = Nz - 1;
+   if (kc < 1) kc = 1;
+   if (kc <= 2) kc = 3;
+   if (kc >= Nz - 2) kc = Nz - 3;
+   const double ci = s0 - kc;
+   double wk[6], dwk[6];
+   delta_weights(ci, wk);
+   ddelta_weights(ci, dwk);
+   if (kc == 3 && ci <= 0) {
+      delta_weights_low(ci, wk);
+      ddelta_weights_low(ci, dwk);
+   }
+   // grid derivatives at the source (index units), interpolated before the
+   // SBP norm corrections are applied to the k weights
+   double zq = 0, zr = 0, zs = 0;
+   {
+      double wk0[6];
+      delta_weights(ci, wk0);
+      if (kc == 3 && ci <= 0) delta_weights_low(ci, wk0);
+      for (int c = 0; c < 6; c++)
+         for (int b = 0; b < 6; b++)
+            for (int a = 0; a < 6; a++) {
+               const int i = ic - 2 + a, j = jc - 2 + b, k = kc - 2 + c;
+               const double w = wi[a] * wj[b] * wk0[c];
+               zq += w * G.zp(i, j, k);
+               zr += w * G.zq(i, j, k);
+               zs += w * G.zr(i, j, k);
+            }
+   }
+   if (!(zs > 0)) return -3;
+   static const double normwgh[4] = {17.0 / 48.0, 59.0 / 48.0, 43.0 / 48.0, 49.0 / 48.0};
+   if (g.bc[4] == BC_FREE)
+      for (int m = 0; m <= 5; m++) {
+         const int kk = m + kc - 2;
+         if (kk >= 1 && kk <= 4) {
+            wk[m] /= normwgh[kk - 1];
+            dwk[m] /= normwgh[kk - 1];
+         }
+      }
+   // gradient of s with respect to (x, y, z) at the source (Source.C:1646-1651)
+   const double sx0 = -zq / (h * zs), sy0 = -zr / (h * zs), sz0 = 1 / zs;
+   const double hi = 1 / h;
+   const double* M = src.m;
+   for (int k = kc - 2; k <= kc + 3; k++)
+      for (int j = jc - 2; j <= jc + 3; j++)
+         for (int i = ic - 2; i <= ic + 3; i++) {
+            const int a = i - ic + 2, b = j - jc + 2, c = k - kc + 2;
+            const double wFx = hi * dwi[a] * wj[b] * wk[c] + wi[a] * wj[b] * sx0 * dwk[c];
+            const double wFy = wi[a] * hi * dwj[b] * wk[c] + wi[a] * wj[b] * sy0 * dwk[c];
+            const double wFz = wi[a] * wj[b] * sz0 * dwk[c];
+            // J = h^2 z_r (curvilinear-c.C:159)
+            const double jaci = 1.0 / (h * h * G.zr(i, j, k));
+            GridPointSource p;
+            p.i = i;
+            p.j = j;
+            p.k = k;
+            p.f[0] = -(M[0] * wFx + M[1] * wFy + M[2] * wFz) * jaci;
+            p.f[1] = -(M[1] * wFx + M[3] * wFy + M[4] * wFz) * jaci;
+            p.f[2] = -(M[2] * wFx + M[4] * wFy + M[5] * wFz) * jaci;
+            p.freq = src.freq;
+            p.t0 = src.t0;
+            p.tfkind = src.tfkind;
+            if (k >= 1 && k <= Nz) out.push_back(p);
+         }
+   return 0;
+}
+
 // EW::sort_grid_point_sources (EW.C:6370-6433): order by grid index and mark
 // runs that share one point.  A stable sort keeps the input order within a
 // run, so the summation order at a shared point is deterministic.

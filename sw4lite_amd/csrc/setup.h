@@ -52,6 +52,10 @@ struct GridPointSource {
 void make_supergrid_arrays(const GridSetup& g, SupergridArrays& a);
 void make_boundary_windows(const GridSetup& g, int wind[36], int npts[6]);
 int moment_source_points(const GridSetup& g, const MomentSource& src, std::vector<GridPointSource>& out);
+// Curvilinear grid of depth zgrid (over the box, i fastest): 0, or -1 grid
+// too small, -2 source outside the grid, -3 folded grid at the source.
+int moment_source_points_curv(const GridSetup& g, const double* zgrid, const MomentSource& src,
+                              std::vector<GridPointSource>& out);
 void sort_grid_point_sources(const GridSetup& g, std::vector<GridPointSource>& src, std::vector<int>& first);
 
 }  // namespace sw4amd

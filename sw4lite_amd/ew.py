@@ -81,6 +81,21 @@ def host_moment_source(desc, x, y, z, m):
     return ijk[:3 * n.value].reshape(-1, 3), force[:3 * n.value].reshape(-1, 3)
 
 
+def host_moment_source_curv(desc, zgrid, x, y, z, m):
+    """Grid-point sources of a moment source on the curvilinear grid of depth
+    zgrid (host array over the box)."""
+    lib = L.load()
+    maxn = 216
+    ijk = np.zeros(3 * maxn, dtype=np.int32)
+    force = np.zeros(3 * maxn)
+    n = C.c_int(0)
+    mm = np.asarray(m, dtype=np.float64)
+    zg = np.ascontiguousarray(zgrid, dtype=np.float64)
+    L.check(lib.sw4hip_host_moment_source_curv(C.byref(desc), _ptr(zg), x, y, z, _ptr(mm), maxn, C.byref(n),
+                                               ijk.ctypes.data_as(L.IP), _ptr(force)))
+    return ijk[:3 * n.value].reshape(-1, 3), force[:3 * n.value].reshape(-1, 3)
+
+
 # --------------------------------------------------------------------------
 # kernel level (device tensors, reference argument lists)
 # --------------------------------------------------------------------------
@@ -250,6 +265,11 @@ class EW:
         return p.value
 
     # time stepping -------------------------------------------------------
+    def set_topography(self, z):
+        """Curvilinear grid of depth z over the box (numpy array or device
+        tensor); before setup and receivers."""
+        L.check(self.lib.sw4hip_ew_set_topography(self.h, _ptr(z), int(_is_dev(z))))
+
     def timesteploop(self, nsteps=None, mode=0):
         """Advance nsteps (default: all remaining) steps; mode 0 fused sweeps,
         mode 1 the reference stage order."""

@@ -43,6 +43,8 @@ _SIGS = {
     "sw4hip_dpdmtfort": (I, [I] * 6 + [P] * 4 + [D, P]),
     "sw4hip_addsgd4": (I, [I] * 7 + [P] * 13 + [D, P]),
     "sw4hip_bcfortsg": (I, [I] * 7 + [P, I, I, I, P, D, P, P, P, P, P, P, P, P]),
+    "sw4hip_host_moment_source_curv": (I, [C.POINTER(GridDesc), P, D, D, D, P, I, IP, P, P]),
+    "sw4hip_ew_set_topography": (I, [P, P, I]),
     "sw4hip_metric": (I, [I] * 7 + [P] * 5 + [P]),
     "sw4hip_rhs4sgcurv": (I, [I] * 7 + [P] * 12 + [P]),
     "sw4hip_addsgd4c": (I, [I] * 7 + [P] * 11 + [D, P]),

@@ -140,3 +140,39 @@ def test_freesurfcurvisg(torch, coefs, corder, side, forced):
     print("freesurfcurvisg corder", corder, "side", side, "forced", forced, "relerr", err, flush=True)
     assert err < TOL
     assert np.array_equal(got != u0, want != u0)
+
+
+@pytest.mark.parametrize("corder,bc", [(1, [2, 2, 2, 2, 0, 2]), (0, [2, 2, 2, 2, 0, 1]), (1, [1, 2, 2, 1, 2, 2])])
+def test_curvilinear_time_loop(torch, corder, bc):
+    """The EW time loop on a grid under Gaussian hills (reference stage order
+    with rhs4sgcurv, addsgd4cfort and the curved free-surface closure) against
+    the oracle's loop: same dt, sources, and displacements after 8 steps."""
+    from oracle.loop import OracleRun
+    from sw4lite_amd import EW
+    nx, ny, nz, h = 33, 29, 24, 0.05
+    _, z = hills_depth(nx, ny, nz, h, 12)
+    box, f = random_inputs(13, nx, ny, nz)
+    rho = 1.5 + 0.2 * f["rho"]
+    mu, la = 1.0 + 0.1 * f["mu"], 1.2 + 0.1 * f["la"]
+    src = [dict(x=0.81, y=0.74, z=0.52, m=[0.3, 0.8, -0.45, -0.6, 0.25, 0.9], t0=0.12, freq=18.0,
+                tfkind=L.TF_GAUSSIAN)]
+    ref = OracleRun(nx, ny, nz, h, bc, rho, mu, la, src, corder=corder, sg_gp=6, nsteps=8, zgrid=z)
+    e = EW(nx, ny, nz, h, bc, corder=corder, sg_gp_thickness=6)
+    e.set_material(rho, mu, la)
+    e.set_topography(z)
+    for s in src:
+        e.add_moment_source(s["x"], s["y"], s["z"], s["m"], s["t0"], s["freq"], s["tfkind"])
+    e.add_receiver(0.8, 0.7, 0.0, 0)
+    e.setup(nsteps=8)
+    inf = e.info()
+    assert abs(inf.dt - ref.dt) < 1e-13 * ref.dt
+    e.timesteploop(8, mode=1)
+    ref.step(8)
+    u, um = e.get_solution()
+    rec = e.receivers()
+    e.close()
+    err = relerr(u, ref.U)
+    print("curvilinear loop corder", corder, "bc", bc, "relerr", err, "dt", inf.dt, flush=True)
+    assert np.max(np.abs(ref.U)) > 0
+    assert err < TOL and relerr(um, ref.Um) < TOL
+    assert rec.shape == (9, 1, 3) and np.max(np.abs(rec[-1])) > 0
