@@ -51,8 +51,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-nz", type=int, default=128, help="depth of the CPU-baseline slab")
     p.add_argument("--cpu-steps", type=int, default=3)
-    p.add_argument("--variant", type=int, default=1, help="interior sweep kernels: 0 thread/point, 1 LDS k-march, 2 with prefetch, "
-                        "3 prefetch without register cap")
+    p.add_argument("--variant", type=int, default=4, help="interior sweep kernels: 0 thread/point, 1 LDS k-march, 2 with prefetch, "
+                        "3 prefetch without register cap, 4 as 3 with the damping in the corrector sweep")
     p.add_argument("--kchunk", type=int, default=32, help="planes per workgroup of the LDS sweeps")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also launch a plain device copy of 1 GiB (PMC scaling)")

@@ -133,6 +133,8 @@ int EW::set_topography(const double* z, bool on_device)
        (rc = check_hip(hipStreamSynchronize(m_stream), "metric flag")))
       return rc;
    if (bad) return set_error(E_INVAL, "EW: topography grid is folded (z not increasing in k)");
+   // Lu is read at every point by the pointwise updates: clear the staging
+   if ((rc = launch_fill(mLu.dev, mLu.size(), 0.0, m_stream))) return rc;
    m_curv = true;
    return OK;
 }

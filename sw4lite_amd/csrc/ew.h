@@ -102,7 +102,7 @@ public:
    std::vector<GridPointSource> m_gps;
    std::vector<int> m_first;
    hipStream_t m_stream = nullptr;
-   int m_variant = 1;   // interior sweep kernels: 0 thread per point, 1 LDS-staged k-march
+   int m_variant = 4;   // interior sweep kernels: 0 thread per point, 1-4 LDS-staged k-march
    int m_kchunk = 32;   // planes per workgroup of the LDS sweeps
 
 private:

@@ -8,7 +8,8 @@ namespace sw4amd {
 // or corrector sweep (corr = true: out = Up' from upin = Up, u, um), chunks of
 // kc planes per workgroup.
 // variant 1: plain k-march, 2 waves/SIMD; 2: register prefetch of the next
-// plane, 2 waves/SIMD; 3: prefetch, 1 wave/SIMD.
+// plane, 2 waves/SIMD; 3: prefetch, 1 wave/SIMD; 4: as 3 with the corrector's
+// supergrid damping (beta != 0) applied in the same sweep.
 int launch_sweep_lds(bool corr, int variant, const Box& b, int k1, int k2, int kc, const double* upin,
                      const double* u, const double* um, const double* mu, const double* la, const double* rho,
                      const SgCoef& sg, double* out, double h, double dt, double beta, hipStream_t s);

@@ -21,6 +21,10 @@ constexpr int TX = 32, TY = 8, HX = TX + 4, HY = TY + 4, NT = TX * TY, PL = HX *
 inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 }  // namespace
 
+struct TileMap {
+   int ntx, nty, ntiles;  // tiles along i and j, tiles in all
+};
+
 struct LdsSten {
    const double* p0;  // ring slot of plane k-2, offset to this thread's point
    const double* p1;
@@ -49,22 +53,38 @@ struct LdsSten {
 // Software pipeline: the global loads of plane k+3 (and of the material of
 // plane k+1) are issued before the stencil of plane k is evaluated and written
 // to LDS at the top of the next iteration, so their latency overlaps compute.
-template <bool CORR, bool PIPE, int MINW>
+// DAMP (corrector only): the supergrid damping of the corrector is applied in
+// the same sweep.  d = U - Um is kept for the planes k..k+2 in an LDS ring
+// (in-plane stencil at plane k), the thread's own column of d for k-2..k+2
+// and rho at k-1, k+1 in registers, rho of plane k in LDS.
+template <bool CORR, bool PIPE, int MINW, bool DAMP = false>
 __global__ void __launch_bounds__(NT, MINW)
-k_sweep_lds(Box b, int kbeg, int kend, int kc, const double* __restrict__ upin, const double* __restrict__ u,
+k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __restrict__ upin,
+            const double* __restrict__ u,
             const double* __restrict__ um, const double* __restrict__ mu, const double* __restrict__ la,
             const double* __restrict__ rho, SgCoef sg, double* __restrict__ out, double cof, double dt2i,
-            double cupd)
+            double cupd, double beta)
 {
+   static_assert(!DAMP || CORR, "damping belongs to the corrector");
    constexpr int NU = (3 * PL + NT - 1) / NT;  // staged displacement values per thread
    constexpr int NM = (PL + NT - 1) / NT;      // staged material values per thread
    __shared__ double su[5 * 3 * PL];
    __shared__ double sm[PL], sl[PL];
+   __shared__ double sd[DAMP ? 3 * 3 * PL : 1];  // d = U - Um, planes k..k+2 (slot (kp - k0) % 3)
+   __shared__ double sr[DAMP ? PL : 1];          // rho of plane k
+   // XCD-aware tile order: workgroups go to the 8 XCDs round-robin by their
+   // id, so id -> (xcd = id % 8, slot = id / 8) and each XCD takes one
+   // contiguous run of tiles (i fastest, then j, then k-chunk).  Tiles that
+   // share halos then run on the same XCD at the same time and meet in its L2.
+   const int per = (int)((map.ntiles + 7) / 8);
+   const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+   if (t >= map.ntiles) return;
+   const int tix = t % map.ntx, tr = t / map.ntx, tjy = tr % map.nty, tkz = tr / map.nty;
    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
-   const int ia = b.ib + blockIdx.x * TX, ja = b.jb + blockIdx.y * TY;  // halo corner of the tile
+   const int ia = b.ib + tix * TX, ja = b.jb + tjy * TY;  // halo corner of the tile
    const int i = ia + 2 + tx, j = ja + 2 + ty;
    const bool active = i <= b.ie - 2 && j <= b.je - 2;
-   const int k0 = kbeg + blockIdx.z * kc;
+   const int k0 = kbeg + tkz * kc;
    const int k1 = min(kend, k0 + kc - 1);
    const long ni = b.ni, nij = b.nij, npts = b.npts;
    const long col = active ? (i - b.ib) + ni * (j - b.jb) : 0;
@@ -91,7 +111,7 @@ k_sweep_lds(Box b, int kbeg, int kend, int kc, const double* __restrict__ upin, 
       mok[p] = e < PL && gi <= b.ie && gj <= b.je;
       moff[p] = mok[p] ? (gi - b.ib) + ni * (gj - b.jb) : 0;
    }
-   double ru[NU], rp[NU], rq[NU], rmu[NM], rla[NM];
+   double ru[NU], rp[NU], rq[NU], rmu[NM], rla[NM], rrh[DAMP ? NM : 1];
    auto load_plane = [&](int kp) {
       const long pb = nij * (kp - b.kb);
 #pragma unroll
@@ -103,12 +123,15 @@ k_sweep_lds(Box b, int kbeg, int kend, int kc, const double* __restrict__ upin, 
          }
       }
    };
-   auto store_plane = [&](int slot) {
+   auto store_plane = [&](int slot, int dslot) {
       double* dst = su + slot * 3 * PL;
 #pragma unroll
       for (int p = 0; p < NU; p++) {
          const int e = tid + p * NT;
-         if (e < 3 * PL) dst[e] = CORR ? dt2i * (rp[p] - 2 * ru[p] + rq[p]) : ru[p];
+         if (e < 3 * PL) {
+            dst[e] = CORR ? dt2i * (rp[p] - 2 * ru[p] + rq[p]) : ru[p];
+            if (DAMP) sd[dslot * 3 * PL + e] = ru[p] - rq[p];
+         }
       }
    };
    auto load_mat = [&](int kp) {
@@ -117,6 +140,7 @@ k_sweep_lds(Box b, int kbeg, int kend, int kc, const double* __restrict__ upin, 
       for (int p = 0; p < NM; p++) {
          rmu[p] = mok[p] ? mu[pb + moff[p]] : 0.0;
          rla[p] = mok[p] ? la[pb + moff[p]] : 0.0;
+         if (DAMP) rrh[p] = mok[p] ? rho[pb + moff[p]] : 0.0;
       }
    };
    auto store_mat = [&]() {
@@ -126,15 +150,29 @@ k_sweep_lds(Box b, int kbeg, int kend, int kc, const double* __restrict__ upin, 
          if (e < PL) {
             sm[e] = rmu[p];
             sl[e] = rla[p];
+            if (DAMP) sr[e] = rrh[p];
          }
       }
    };
 
    LdsSten s;
+   // damping: own column of d at k-2..k+2 and rho at k-1, k+1
+   double dcol[5][3], rkm = 0, rkp = 0, rk0 = 0;
+   if (DAMP) {
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+         for (int c = 0; c < 3; c++) {
+            const long g = col + nij * (k0 - 2 + q - b.kb) + npts * c;
+            dcol[q][c] = active ? u[g] - um[g] : 0.0;
+         }
+      rkm = active ? rho[col + nij * (k0 - 1 - b.kb)] : 0.0;
+      rkp = active ? rho[col + nij * (k0 + 1 - b.kb)] : 0.0;
+   }
 #pragma unroll
    for (int q = 0; q < 4; q++) {
       load_plane(k0 - 2 + q);
-      store_plane(q);
+      store_plane(q, q >= 2 ? q - 2 : 2);  // d ring: planes k0 (slot 0), k0+1 (slot 1)
       const long pq = nij * (k0 - 2 + q - b.kb);
       s.muc[q] = active ? mu[col + pq] : 0.0;
       s.lac[q] = active ? la[col + pq] : 0.0;
@@ -166,7 +204,7 @@ k_sweep_lds(Box b, int kbeg, int kend, int kc, const double* __restrict__ upin, 
          cla = active ? la[col + nij * (k + 2 - b.kb)] : 0.0;
          csz = sg.strz[k + 2 - b.kb];
       }
-      store_plane((sl0 + 4) % 5);
+      store_plane((sl0 + 4) % 5, (k + 2 - k0) % 3);
       store_mat();
       s.muc[4] = cmu;
       s.lac[4] = cla;
@@ -189,12 +227,43 @@ k_sweep_lds(Box b, int kbeg, int kend, int kc, const double* __restrict__ upin, 
          double r[3];
          rhs4_interior_body(s, r);
          const long p = col + nij * (k - b.kb);
-         const double sc = cupd / rho[p];
+         const double r0 = DAMP ? sr[o] : rho[p];
+         if (DAMP) rk0 = r0;
+         const double sc = cupd / r0;
+         double dmp[3] = {0, 0, 0};
+         if (DAMP) {
+            const double* dpl = sd + ((k - k0) % 3) * 3 * PL + o;  // d of plane k at this point
+#pragma unroll
+            for (int c = 0; c < 3; c++) dcol[4][c] = sd[((k + 2 - k0) % 3) * 3 * PL + c * PL + o];
+            const int ii = i - b.ib, jj = j - b.jb, kk = k - b.kb;
+            const double birho = beta / r0;
+            const double rxp = sr[o + 1] * sg.dcx[ii + 1], rx0 = r0 * sg.dcx[ii], rxm = sr[o - 1] * sg.dcx[ii - 1];
+            const double ryp = sr[o + HX] * sg.dcy[jj + 1], ry0 = r0 * sg.dcy[jj],
+                         rym = sr[o - HX] * sg.dcy[jj - 1];
+            const double rzp = rkp * sg.dcz[kk + 1], rz0 = r0 * sg.dcz[kk], rzm = rkm * sg.dcz[kk - 1];
+            const double wx = sg.strx[ii] * sg.coy[jj] * sg.coz[kk];
+            const double wy = sg.stry[jj] * sg.cox[ii] * sg.coz[kk];
+            const double wz = sg.strz[kk] * sg.cox[ii] * sg.coy[jj];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+               const double* q = dpl + c * PL;
+               const double ax = rxp * (q[2] - 2 * q[1] + q[0]) - 2 * rx0 * (q[1] - 2 * q[0] + q[-1]) +
+                                 rxm * (q[0] - 2 * q[-1] + q[-2]);
+               const double ay = ryp * (q[2 * HX] - 2 * q[HX] + q[0]) - 2 * ry0 * (q[HX] - 2 * q[0] + q[-HX]) +
+                                 rym * (q[0] - 2 * q[-HX] + q[-2 * HX]);
+               const double az = rzp * (dcol[4][c] - 2 * dcol[3][c] + dcol[2][c]) -
+                                 2 * rz0 * (dcol[3][c] - 2 * dcol[2][c] + dcol[1][c]) +
+                                 rzm * (dcol[2][c] - 2 * dcol[1][c] + dcol[0][c]);
+               dmp[c] = birho * (wx * ax + wy * ay + wz * az);
+            }
+         }
 #pragma unroll
          for (int c = 0; c < 3; c++) {
             const long q = p + npts * c;
             if (!CORR)
                out[q] = 2 * s.p2[c * PL] - um[q] + sc * (cof * r[c]);
+            else if (DAMP)
+               out[q] = (upin[q] + sc * (cof * r[c])) - dmp[c];
             else
                out[q] = upin[q] + sc * (cof * r[c]);
          }
@@ -206,6 +275,14 @@ k_sweep_lds(Box b, int kbeg, int kend, int kc, const double* __restrict__ upin, 
          s.lac[q] = s.lac[q + 1];
          s.szv[q] = s.szv[q + 1];
       }
+      if (DAMP) {
+#pragma unroll
+         for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) dcol[q][c] = dcol[q + 1][c];
+         rkm = rk0;  // rho of plane k becomes k-1 of the next row (0 where inactive)
+         rkp = active && k < k1 ? rho[col + nij * (k + 2 - b.kb)] : 0.0;
+      }
    }
 }
 
@@ -216,21 +293,28 @@ int launch_sweep_lds(bool corr, int variant, const Box& b, int k1, int k2, int k
    if (k2 < k1) return OK;
    const long nx = b.ie - b.ib - 3, ny = b.je - b.jb - 3;
    if (kc <= 0) kc = 32;
-   dim3 grd(cdiv(nx, TX), cdiv(ny, TY), cdiv(k2 - k1 + 1, kc));
+   TileMap map;
+   map.ntx = (int)cdiv(nx, TX);
+   map.nty = (int)cdiv(ny, TY);
+   map.ntiles = map.ntx * map.nty * (int)cdiv(k2 - k1 + 1, kc);
+   const unsigned grd = 8 * cdiv(map.ntiles, 8);
    const double cof = 1.0 / (h * h), dt2i = 1.0 / (dt * dt);
-   (void)beta;
    const double c4 = dt * dt * dt * dt / 12, c2 = dt * dt;
-#define SW4_SWEEP(C, P, W) k_sweep_lds<C, P, W><<<grd, NT, 0, s>>>(b, k1, k2, kc, C ? upin : nullptr, u, um, mu, \
-                                                                  la, rho, sg, out, cof, dt2i, C ? c4 : c2)
+#define SW4_SWEEP(C, P, W, D)                                                                                    \
+   k_sweep_lds<C, P, W, D><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, C ? upin : nullptr, u, um, mu, la, rho, sg, out, \
+                                              cof, dt2i, C ? c4 : c2, beta)
    switch (variant) {
    case 2:  // prefetch, 2 waves/SIMD
-      if (corr) SW4_SWEEP(true, true, 2); else SW4_SWEEP(false, true, 2);
+      if (corr) SW4_SWEEP(true, true, 2, false); else SW4_SWEEP(false, true, 2, false);
       break;
    case 3:  // prefetch, 1 wave/SIMD (no register cap)
-      if (corr) SW4_SWEEP(true, true, 1); else SW4_SWEEP(false, true, 1);
+      if (corr) SW4_SWEEP(true, true, 1, false); else SW4_SWEEP(false, true, 1, false);
+      break;
+   case 4:  // as 3, the corrector with the supergrid damping in the same sweep
+      if (corr) SW4_SWEEP(true, true, 1, true); else SW4_SWEEP(false, true, 1, false);
       break;
    default:  // 1: no prefetch, 2 waves/SIMD
-      if (corr) SW4_SWEEP(true, false, 2); else SW4_SWEEP(false, false, 2);
+      if (corr) SW4_SWEEP(true, false, 2, false); else SW4_SWEEP(false, false, 2, false);
       break;
    }
 #undef SW4_SWEEP

@@ -187,8 +187,9 @@ class EW:
 
     # set-up -------------------------------------------------------------
     def set_kernel_variant(self, variant, kchunk=32):
-        """0: one thread per point; 1-3: LDS-staged k-marching tiles (1 plain,
-        2 with register prefetch, 3 prefetch without a register cap)."""
+        """0: one thread per point; 1-4: LDS-staged k-marching tiles (1 plain,
+        2 with register prefetch, 3 prefetch without a register cap, 4 as 3
+        with the corrector's damping in the same sweep, the default)."""
         L.check(self.lib.sw4hip_ew_set_kernel_variant(self.h, int(variant), int(kchunk)))
 
     def set_material(self, rho, mu, la):

@@ -66,7 +66,7 @@ def test_metric(torch, corder):
     ew.metric(corder, box, dev(torch, x), dev(torch, y), dev(torch, z), gm, gj)
     e1, e2 = relerr(host(gm), met_layout(met, box, corder)), relerr(host(gj), jac)
     print("metric corder", corder, "relerr", e1, e2, flush=True)
-    assert e1 < 1e-14 and e2 < 1e-14
+    assert e1 < TOL and e2 < TOL
 
 
 def test_metric_rejects_folded_grid(torch):

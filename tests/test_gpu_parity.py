@@ -215,6 +215,9 @@ def test_full_size_fused_matches_reference_order(torch):
     (2, 7, [2, 2, 2, 2, 0, 2]),
     (2, 32, [2, 2, 2, 2, 0, 0]),
     (3, 5, [2, 2, 2, 2, 0, 2]),
+    (4, 32, [2, 2, 2, 2, 0, 2]),
+    (4, 7, [2, 2, 2, 2, 0, 0]),
+    (4, 5, [2, 2, 2, 2, 2, 2]),
     (0, 32, [2, 2, 2, 2, 0, 2]),
 ])
 def test_sweep_variants_match_oracle(torch, variant, kchunk, bc):
