@@ -19,7 +19,8 @@ The JSON line adds
   roofline: the dominant kernel (corrector sweep, interior points), its
     algorithmic bytes per launch (15 FP64 words per point: read Up, U, Um (3
     components each), mu, lambda, rho, write Up') over its average launch
-    time from hipEvents on the stream it runs on, against 8 TB/s;
+    time from hipEvents on the stream it runs on, against 8 TB/s; traffic is
+    the HBM bytes per launch measured with rocprofv3 counters;
   cpu_baseline: the same step on the host cores for a bounded slab of the
     workload, with the stencil stages (rhs4sg_rev, addsgd4) from the
     reference's own sources (oracle/_ref) when they are present.
@@ -37,7 +38,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "grid-pt updates/s per RK step (FP64 vs FP32-comp, %HBM roofline); adjoint/fwd ratio"
 HBM_PEAK_GBS = 8000.0
-CORR_BYTES_PER_PT = 15 * 8     # corrector + damping sweep, FP64
+CORR_BYTES_PER_PT = 15 * 8     # corrector sweep: Up, U, Um, mu, lambda, rho in, Up' out (FP64)
 PRED_BYTES_PER_PT = 12 * 8     # predictor sweep, FP64
 STEP_BYTES_PER_PT = 27 * 8     # whole step, compulsory traffic
 
@@ -51,9 +52,10 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-nz", type=int, default=128, help="depth of the CPU-baseline slab")
     p.add_argument("--cpu-steps", type=int, default=3)
-    p.add_argument("--variant", type=int, default=4, help="interior sweep kernels: 0 thread/point, 1 LDS k-march, 2 with prefetch, "
-                        "3 prefetch without register cap, 4 as 3 with the damping in the corrector sweep")
-    p.add_argument("--kchunk", type=int, default=32, help="planes per workgroup of the LDS sweeps")
+    p.add_argument("--variant", type=int, default=6, help="interior sweep kernels: 0 thread/point, 1 LDS k-march, 2 with prefetch, "
+                        "3 prefetch without register cap, 4 as 3 with the damping in the corrector sweep, "
+                        "5 corrector as 4 and predictor as 1, 6 as 1 with an LDS damping sweep")
+    p.add_argument("--kchunk", type=int, default=64, help="planes per workgroup of the LDS sweeps")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also launch a plain device copy of 1 GiB (PMC scaling)")
     p.add_argument("--workload", choices=["cartesian", "curvilinear"], default="cartesian",

@@ -102,8 +102,8 @@ public:
    std::vector<GridPointSource> m_gps;
    std::vector<int> m_first;
    hipStream_t m_stream = nullptr;
-   int m_variant = 4;   // interior sweep kernels: 0 thread per point, 1-4 LDS-staged k-march
-   int m_kchunk = 32;   // planes per workgroup of the LDS sweeps
+   int m_variant = 6;   // interior sweep kernels: 0 thread per point, 1-6 LDS-staged k-march
+   int m_kchunk = 64;   // planes per workgroup of the LDS sweeps
 
 private:
    int alloc(Sarray& a, int nc);

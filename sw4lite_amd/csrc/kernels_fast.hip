@@ -33,8 +33,10 @@ struct LdsSten {
    const double* p4;
    const double* pm;  // mu of plane k at this point
    const double* pl;
+   const double* psx;  // strx at this thread's i (LDS copy over the tile)
+   const double* psy;
    double muc[5], lac[5];
-   double sxv[5], syv[5], szv[5];
+   double szv[5];
    __device__ __forceinline__ double u(int c, int di, int dj, int dk) const
    {
       const double* p = dk == -2 ? p0 : dk == -1 ? p1 : dk == 0 ? p2 : dk == 1 ? p3 : p4;
@@ -42,36 +44,40 @@ struct LdsSten {
    }
    __device__ __forceinline__ double mu(int di, int dj, int dk) const { return dk ? muc[dk + 2] : pm[dj * HX + di]; }
    __device__ __forceinline__ double la(int di, int dj, int dk) const { return dk ? lac[dk + 2] : pl[dj * HX + di]; }
-   __device__ __forceinline__ double sx(int d) const { return sxv[d + 2]; }
-   __device__ __forceinline__ double sy(int d) const { return syv[d + 2]; }
+   __device__ __forceinline__ double sx(int d) const { return psx[d]; }
+   __device__ __forceinline__ double sy(int d) const { return psy[d]; }
    __device__ __forceinline__ double sz(int d) const { return szv[d + 2]; }
 };
 
 // CORR = false: out = 2u - um + dt2/rho cof L(u)                  (predictor)
-// CORR = true:  out = up + dt4/(12 rho) cof L((up - 2u + um) dt2i)  (corrector;
-//               the supergrid damping is applied by k_addsgd4_planes)
-// Software pipeline: the global loads of plane k+3 (and of the material of
-// plane k+1) are issued before the stencil of plane k is evaluated and written
-// to LDS at the top of the next iteration, so their latency overlaps compute.
+// CORR = true:  out = up + dt4/(12 rho) cof L((up - 2u + um) dt2i)  (corrector)
 // DAMP (corrector only): the supergrid damping of the corrector is applied in
 // the same sweep.  d = U - Um is kept for the planes k..k+2 in an LDS ring
 // (in-plane stencil at plane k), the thread's own column of d for k-2..k+2
 // and rho at k-1, k+1 in registers, rho of plane k in LDS.
+// PIPE: software pipeline.  Every global load an iteration needs (plane k+3
+// of the staged arrays, the material of plane k+1, the column and centre
+// values of the next plane) is issued right after the barrier and consumed in
+// the next iteration.  The compute phase itself reads only LDS and registers:
+// the load counter is in order, so a global load issued after the prefetch
+// and waited on in the same iteration would wait for the prefetch as well.
 template <bool CORR, bool PIPE, int MINW, bool DAMP = false>
 __global__ void __launch_bounds__(NT, MINW)
 k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __restrict__ upin,
-            const double* __restrict__ u,
-            const double* __restrict__ um, const double* __restrict__ mu, const double* __restrict__ la,
-            const double* __restrict__ rho, SgCoef sg, double* __restrict__ out, double cof, double dt2i,
-            double cupd, double beta)
+            const double* __restrict__ u, const double* __restrict__ um, const double* __restrict__ mu,
+            const double* __restrict__ la, const double* __restrict__ rho, SgCoef sg, double* __restrict__ out,
+            double cof, double dt2i, double cupd, double beta)
 {
    static_assert(!DAMP || CORR, "damping belongs to the corrector");
    constexpr int NU = (3 * PL + NT - 1) / NT;  // staged displacement values per thread
    constexpr int NM = (PL + NT - 1) / NT;      // staged material values per thread
-   __shared__ double su[5 * 3 * PL];
+   __shared__ double su[5 * 3 * PL];            // ring: plane kp in slot (kp - k0 + 2) % 5
    __shared__ double sm[PL], sl[PL];
-   __shared__ double sd[DAMP ? 3 * 3 * PL : 1];  // d = U - Um, planes k..k+2 (slot (kp - k0) % 3)
+   __shared__ double sd[DAMP ? 3 * 3 * PL : 1];  // d = U - Um, plane kp in slot (kp - k0) % 3
    __shared__ double sr[DAMP ? PL : 1];          // rho of plane k
+   __shared__ double ssx[HX], ssy[HY];            // strx, stry over the tile with halo
+   __shared__ double sgx[DAMP ? 5 * TX : 1], sgy[DAMP ? 5 * TY : 1];  // dc(-1,0,+1), str, co per column
+   __shared__ double sct[3 * NT];                 // centre values of plane k (see load_centre)
    // XCD-aware tile order: workgroups go to the 8 XCDs round-robin by their
    // id, so id -> (xcd = id % 8, slot = id / 8) and each XCD takes one
    // contiguous run of tiles (i fastest, then j, then k-chunk).  Tiles that
@@ -112,14 +118,18 @@ k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __rest
       moff[p] = mok[p] ? (gi - b.ib) + ni * (gj - b.jb) : 0;
    }
    double ru[NU], rp[NU], rq[NU], rmu[NM], rla[NM], rrh[DAMP ? NM : 1];
+   // Out-of-tile elements have offset 0 (a valid address): every load is
+   // issued unconditionally and masked afterwards, so no branches.
    auto load_plane = [&](int kp) {
       const long pb = nij * (kp - b.kb);
 #pragma unroll
       for (int p = 0; p < NU; p++) {
-         ru[p] = gok[p] ? u[pb + goff[p]] : 0.0;
+         const double vu = u[pb + goff[p]];
+         ru[p] = gok[p] ? vu : 0.0;
          if (CORR) {
-            rp[p] = gok[p] ? upin[pb + goff[p]] : 0.0;
-            rq[p] = gok[p] ? um[pb + goff[p]] : 0.0;
+            const double vp = upin[pb + goff[p]], vq = um[pb + goff[p]];
+            rp[p] = gok[p] ? vp : 0.0;
+            rq[p] = gok[p] ? vq : 0.0;
          }
       }
    };
@@ -138,9 +148,13 @@ k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __rest
       const long pb = nij * (kp - b.kb);
 #pragma unroll
       for (int p = 0; p < NM; p++) {
-         rmu[p] = mok[p] ? mu[pb + moff[p]] : 0.0;
-         rla[p] = mok[p] ? la[pb + moff[p]] : 0.0;
-         if (DAMP) rrh[p] = mok[p] ? rho[pb + moff[p]] : 0.0;
+         const double vm = mu[pb + moff[p]], vl = la[pb + moff[p]];
+         rmu[p] = mok[p] ? vm : 0.0;
+         rla[p] = mok[p] ? vl : 0.0;
+         if (DAMP) {
+            const double vr = rho[pb + moff[p]];
+            rrh[p] = mok[p] ? vr : 0.0;
+         }
       }
    };
    auto store_mat = [&]() {
@@ -154,9 +168,22 @@ k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __rest
          }
       }
    };
+   // values of plane kp at this thread's point used after the stencil: the
+   // centre of up (corrector) or um (predictor), rho (without DAMP)
+   double ctr[3] = {0, 0, 0}, crho = 1.0;
+   auto load_centre = [&](int kp, double c3[3], double& r) {
+      const long p = col + nij * (kp - b.kb);  // col = 0 where inactive: a valid address
+#pragma unroll
+      for (int c = 0; c < 3; c++) c3[c] = (CORR ? upin : um)[p + npts * c];
+      if (!DAMP) {
+         const double v = rho[p];
+         r = active ? v : 1.0;
+      }
+   };
 
    LdsSten s;
-   // damping: own column of d at k-2..k+2 and rho at k-1, k+1
+   // damping: own column of d at k-2..k+2, rho at k-1 and k+1, and the x/y
+   // supergrid coefficients of this column (constant over the sweep)
    double dcol[5][3], rkm = 0, rkp = 0, rk0 = 0;
    if (DAMP) {
 #pragma unroll
@@ -168,7 +195,28 @@ k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __rest
          }
       rkm = active ? rho[col + nij * (k0 - 1 - b.kb)] : 0.0;
       rkp = active ? rho[col + nij * (k0 + 1 - b.kb)] : 0.0;
+      // x/y supergrid coefficients of the tile's columns (interior columns only)
+      if (tid < TX) {
+         const int ii = ia + 2 + tid - b.ib;
+         const bool ok = ia + 2 + tid <= b.ie - 2;
+         sgx[0 * TX + tid] = ok ? sg.dcx[ii - 1] : 0.0;
+         sgx[1 * TX + tid] = ok ? sg.dcx[ii] : 0.0;
+         sgx[2 * TX + tid] = ok ? sg.dcx[ii + 1] : 0.0;
+         sgx[3 * TX + tid] = ok ? sg.strx[ii] : 0.0;
+         sgx[4 * TX + tid] = ok ? sg.cox[ii] : 0.0;
+      } else if (tid >= 64 && tid < 64 + TY) {
+         const int q = tid - 64, jj = ja + 2 + q - b.jb;
+         const bool ok = ja + 2 + q <= b.je - 2;
+         sgy[0 * TY + q] = ok ? sg.dcy[jj - 1] : 0.0;
+         sgy[1 * TY + q] = ok ? sg.dcy[jj] : 0.0;
+         sgy[2 * TY + q] = ok ? sg.dcy[jj + 1] : 0.0;
+         sgy[3 * TY + q] = ok ? sg.stry[jj] : 0.0;
+         sgy[4 * TY + q] = ok ? sg.coy[jj] : 0.0;
+      }
    }
+   // stretching over the tile with its halo (1 beyond the array ends)
+   if (tid >= 128 && tid < 128 + HX) ssx[tid - 128] = ia + tid - 128 <= b.ie ? sg.strx[ia + tid - 128 - b.ib] : 1.0;
+   if (tid >= 192 && tid < 192 + HY) ssy[tid - 192] = ja + tid - 192 <= b.je ? sg.stry[ja + tid - 192 - b.jb] : 1.0;
 #pragma unroll
    for (int q = 0; q < 4; q++) {
       load_plane(k0 - 2 + q);
@@ -178,45 +226,60 @@ k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __rest
       s.lac[q] = active ? la[col + pq] : 0.0;
       s.szv[q] = sg.strz[k0 - 2 + q - b.kb];
    }
-#pragma unroll
-   for (int q = 0; q < 5; q++) {
-      s.sxv[q] = active ? sg.strx[i - b.ib + q - 2] : 1.0;
-      s.syv[q] = active ? sg.stry[j - b.jb + q - 2] : 1.0;
-   }
    s.pm = sm + o;
    s.pl = sl + o;
-   // PIPE: prefetch plane k0+2, material of plane k0, column values of k0+2
+   s.psx = ssx + tx + 2;
+   s.psy = ssy + ty + 2;
+   // PIPE: prefetch plane k0+2, material of plane k0, column values of k0+2,
+   // centre values of plane k0
    double cmu = 0, cla = 0, csz = 0;
+   double nctr[3] = {0, 0, 0}, ncrho = 1.0;
    if (PIPE) {
       load_plane(k0 + 2);
       load_mat(k0);
       cmu = active ? mu[col + nij * (k0 + 2 - b.kb)] : 0.0;
       cla = active ? la[col + nij * (k0 + 2 - b.kb)] : 0.0;
       csz = sg.strz[k0 + 2 - b.kb];
+      load_centre(k0, nctr, ncrho);
    }
 
    for (int k = k0; k <= k1; k++) {
-      const int sl0 = (k - k0) % 5;
+      const int sl0 = (k - k0) % 5;  // slot of plane k-2
       if (!PIPE) {
          load_plane(k + 2);
          load_mat(k);
          cmu = active ? mu[col + nij * (k + 2 - b.kb)] : 0.0;
          cla = active ? la[col + nij * (k + 2 - b.kb)] : 0.0;
          csz = sg.strz[k + 2 - b.kb];
+         load_centre(k, ctr, crho);
+      } else {
+#pragma unroll
+         for (int c = 0; c < 3; c++) sct[c * NT + tid] = nctr[c];
+         crho = ncrho;
       }
+      if (!PIPE)
+#pragma unroll
+         for (int c = 0; c < 3; c++) sct[c * NT + tid] = ctr[c];
       store_plane((sl0 + 4) % 5, (k + 2 - k0) % 3);
       store_mat();
       s.muc[4] = cmu;
       s.lac[4] = cla;
       s.szv[4] = csz;
+      // z supergrid coefficients of plane k (uniform over the workgroup)
+      const int kz = k - b.kb;
+      double gz[5] = {0, 0, 0, 0, 0};
+      if (DAMP) {
+         gz[0] = sg.dcz[kz - 1]; gz[1] = sg.dcz[kz]; gz[2] = sg.dcz[kz + 1]; gz[3] = sg.strz[kz]; gz[4] = sg.coz[kz];
+      }
       __syncthreads();
       if (PIPE && k < k1) {
          load_plane(k + 3);
          load_mat(k + 1);
          const long pq = nij * (k + 3 - b.kb);
-         cmu = active ? mu[col + pq] : 0.0;
-         cla = active ? la[col + pq] : 0.0;
+         cmu = mu[col + pq];  // col = 0 where inactive: a valid address, value unused
+         cla = la[col + pq];
          csz = sg.strz[k + 3 - b.kb];
+         load_centre(k + 1, nctr, ncrho);
       }
       if (active) {
          s.p0 = su + (sl0 * 3) * PL + o;
@@ -227,7 +290,7 @@ k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __rest
          double r[3];
          rhs4_interior_body(s, r);
          const long p = col + nij * (k - b.kb);
-         const double r0 = DAMP ? sr[o] : rho[p];
+         const double r0 = DAMP ? sr[o] : crho;
          if (DAMP) rk0 = r0;
          const double sc = cupd / r0;
          double dmp[3] = {0, 0, 0};
@@ -235,15 +298,15 @@ k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __rest
             const double* dpl = sd + ((k - k0) % 3) * 3 * PL + o;  // d of plane k at this point
 #pragma unroll
             for (int c = 0; c < 3; c++) dcol[4][c] = sd[((k + 2 - k0) % 3) * 3 * PL + c * PL + o];
-            const int ii = i - b.ib, jj = j - b.jb, kk = k - b.kb;
             const double birho = beta / r0;
-            const double rxp = sr[o + 1] * sg.dcx[ii + 1], rx0 = r0 * sg.dcx[ii], rxm = sr[o - 1] * sg.dcx[ii - 1];
-            const double ryp = sr[o + HX] * sg.dcy[jj + 1], ry0 = r0 * sg.dcy[jj],
-                         rym = sr[o - HX] * sg.dcy[jj - 1];
-            const double rzp = rkp * sg.dcz[kk + 1], rz0 = r0 * sg.dcz[kk], rzm = rkm * sg.dcz[kk - 1];
-            const double wx = sg.strx[ii] * sg.coy[jj] * sg.coz[kk];
-            const double wy = sg.stry[jj] * sg.cox[ii] * sg.coz[kk];
-            const double wz = sg.strz[kk] * sg.cox[ii] * sg.coy[jj];
+            const double* gx = sgx + tx;
+            const double* gy = sgy + ty;
+            const double rxp = sr[o + 1] * gx[2 * TX], rx0 = r0 * gx[TX], rxm = sr[o - 1] * gx[0];
+            const double ryp = sr[o + HX] * gy[2 * TY], ry0 = r0 * gy[TY], rym = sr[o - HX] * gy[0];
+            const double rzp = rkp * gz[2], rz0 = r0 * gz[1], rzm = rkm * gz[0];
+            const double wx = gx[3 * TX] * gy[4 * TY] * gz[4];
+            const double wy = gy[3 * TY] * gx[4 * TX] * gz[4];
+            const double wz = gz[3] * gx[4 * TX] * gy[4 * TY];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                const double* q = dpl + c * PL;
@@ -260,12 +323,13 @@ k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __rest
 #pragma unroll
          for (int c = 0; c < 3; c++) {
             const long q = p + npts * c;
+            const double cv = sct[c * NT + tid];
             if (!CORR)
-               out[q] = 2 * s.p2[c * PL] - um[q] + sc * (cof * r[c]);
+               out[q] = 2 * s.p2[c * PL] - cv + sc * (cof * r[c]);
             else if (DAMP)
-               out[q] = (upin[q] + sc * (cof * r[c])) - dmp[c];
+               out[q] = (cv + sc * (cof * r[c])) - dmp[c];
             else
-               out[q] = upin[q] + sc * (cof * r[c]);
+               out[q] = cv + sc * (cof * r[c]);
          }
       }
       __syncthreads();
@@ -284,6 +348,130 @@ k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __rest
          rkp = active && k < k1 ? rho[col + nij * (k + 2 - b.kb)] : 0.0;
       }
    }
+}
+
+// Supergrid damping of the corrector on planes kbeg..kend as a k-marching LDS
+// sweep (ew-cfromfort.C:985-1041): up -= damping(u - um).  d = u - um of plane
+// k and rho of plane k are staged with their (i,j) halo in LDS; the thread's
+// own column of d for k-2..k+2 and rho at k+-1 live in registers.  Small LDS
+// and register footprint, so several workgroups share a CU and hide latency.
+__global__ void __launch_bounds__(NT, 2)
+k_damp_lds(Box b, TileMap map, int kbeg, int kend, int kc, double* __restrict__ up, const double* __restrict__ u,
+           const double* __restrict__ um, const double* __restrict__ rho, SgCoef sg, double beta)
+{
+   constexpr int NU = (3 * PL + NT - 1) / NT;
+   constexpr int NM = (PL + NT - 1) / NT;
+   __shared__ double sdp[3 * PL];
+   __shared__ double srp[PL];
+   const int per = (int)((map.ntiles + 7) / 8);
+   const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+   if (t >= map.ntiles) return;
+   const int tix = t % map.ntx, tr = t / map.ntx, tjy = tr % map.nty, tkz = tr / map.nty;
+   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+   const int ia = b.ib + tix * TX, ja = b.jb + tjy * TY;
+   const int i = ia + 2 + tx, j = ja + 2 + ty;
+   const bool active = i <= b.ie - 2 && j <= b.je - 2;
+   const int k0 = kbeg + tkz * kc;
+   const int k1 = min(kend, k0 + kc - 1);
+   const long ni = b.ni, nij = b.nij, npts = b.npts;
+   const long col = active ? (i - b.ib) + ni * (j - b.jb) : 0;
+   const int o = (ty + 2) * HX + tx + 2;
+   const int ii = i - b.ib, jj = j - b.jb;
+   double dq[5][3];
+   double rkm = 0, rkp = 0;
+#pragma unroll
+   for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+         const long g = col + nij * (k0 - 2 + q - b.kb) + npts * c;
+         const double a = u[g], e = um[g];
+         dq[q][c] = active ? a - e : 0.0;
+      }
+   {
+      const double a = rho[col + nij * (k0 - 1 - b.kb)];
+      rkm = active ? a : 0.0;
+   }
+   double gx[5] = {0, 0, 0, 0, 0}, gy[5] = {0, 0, 0, 0, 0};
+   if (active) {
+      gx[0] = sg.dcx[ii - 1]; gx[1] = sg.dcx[ii]; gx[2] = sg.dcx[ii + 1]; gx[3] = sg.strx[ii]; gx[4] = sg.cox[ii];
+      gy[0] = sg.dcy[jj - 1]; gy[1] = sg.dcy[jj]; gy[2] = sg.dcy[jj + 1]; gy[3] = sg.stry[jj]; gy[4] = sg.coy[jj];
+   }
+   for (int k = k0; k <= k1; k++) {
+      const long pb = nij * (k - b.kb);
+#pragma unroll
+      for (int p = 0; p < NU; p++) {
+         const int e = tid + p * NT;
+         const int c = e / PL, r = e - c * PL, y = r / HX, x = r - y * HX;
+         const int gi = ia + x, gj = ja + y;
+         const bool ok = e < 3 * PL && gi <= b.ie && gj <= b.je;
+         const long g = ok ? pb + (gi - b.ib) + ni * (gj - b.jb) + npts * c : 0;
+         const double a = u[g], m = um[g];
+         if (e < 3 * PL) sdp[e] = ok ? a - m : 0.0;
+      }
+#pragma unroll
+      for (int p = 0; p < NM; p++) {
+         const int e = tid + p * NT;
+         const int y = e / HX, x = e - y * HX;
+         const int gi = ia + x, gj = ja + y;
+         const bool ok = e < PL && gi <= b.ie && gj <= b.je;
+         const double a = rho[ok ? pb + (gi - b.ib) + ni * (gj - b.jb) : 0];
+         if (e < PL) srp[e] = ok ? a : 0.0;
+      }
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+         const long g = col + nij * (k + 2 - b.kb) + npts * c;
+         const double a = u[g], m = um[g];
+         dq[4][c] = active ? a - m : 0.0;
+      }
+      {
+         const double a = rho[col + nij * (k + 1 - b.kb)];
+         rkp = active ? a : 0.0;
+      }
+      const int kz = k - b.kb;
+      const double gz0 = sg.dcz[kz - 1], gz1 = sg.dcz[kz], gz2 = sg.dcz[kz + 1], gz3 = sg.strz[kz], gz4 = sg.coz[kz];
+      __syncthreads();
+      double r0 = 0;
+      if (active) {
+         r0 = srp[o];
+         const double birho = beta / r0;
+         const double rxp = srp[o + 1] * gx[2], rx0 = r0 * gx[1], rxm = srp[o - 1] * gx[0];
+         const double ryp = srp[o + HX] * gy[2], ry0 = r0 * gy[1], rym = srp[o - HX] * gy[0];
+         const double rzp = rkp * gz2, rz0 = r0 * gz1, rzm = rkm * gz0;
+         const double wx = gx[3] * gy[4] * gz4, wy = gy[3] * gx[4] * gz4, wz = gz3 * gx[4] * gy[4];
+#pragma unroll
+         for (int c = 0; c < 3; c++) {
+            const double* q = sdp + c * PL + o;
+            const double ax = rxp * (q[2] - 2 * q[1] + q[0]) - 2 * rx0 * (q[1] - 2 * q[0] + q[-1]) +
+                              rxm * (q[0] - 2 * q[-1] + q[-2]);
+            const double ay = ryp * (q[2 * HX] - 2 * q[HX] + q[0]) - 2 * ry0 * (q[HX] - 2 * q[0] + q[-HX]) +
+                              rym * (q[0] - 2 * q[-HX] + q[-2 * HX]);
+            const double az = rzp * (dq[4][c] - 2 * dq[3][c] + dq[2][c]) -
+                              2 * rz0 * (dq[3][c] - 2 * dq[2][c] + dq[1][c]) +
+                              rzm * (dq[2][c] - 2 * dq[1][c] + dq[0][c]);
+            up[col + pb + npts * c] -= birho * (wx * ax + wy * ay + wz * az);
+         }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+         for (int c = 0; c < 3; c++) dq[q][c] = dq[q + 1][c];
+      rkm = r0;
+   }
+}
+
+int launch_damp_lds(const Box& b, int k1, int k2, int kc, double* up, const double* u, const double* um,
+                    const double* rho, const SgCoef& sg, double beta, hipStream_t s)
+{
+   if (k2 < k1 || beta == 0) return OK;
+   const long nx = b.ie - b.ib - 3, ny = b.je - b.jb - 3;
+   if (kc <= 0) kc = 64;
+   TileMap map;
+   map.ntx = (int)cdiv(nx, TX);
+   map.nty = (int)cdiv(ny, TY);
+   map.ntiles = map.ntx * map.nty * (int)cdiv(k2 - k1 + 1, kc);
+   k_damp_lds<<<8 * cdiv(map.ntiles, 8), NT, 0, s>>>(b, map, k1, k2, kc, up, u, um, rho, sg, beta);
+   return check_hip(hipGetLastError(), "damping sweep launch");
 }
 
 int launch_sweep_lds(bool corr, int variant, const Box& b, int k1, int k2, int kc, const double* upin,
@@ -312,6 +500,12 @@ int launch_sweep_lds(bool corr, int variant, const Box& b, int k1, int k2, int k
       break;
    case 4:  // as 3, the corrector with the supergrid damping in the same sweep
       if (corr) SW4_SWEEP(true, true, 1, true); else SW4_SWEEP(false, true, 1, false);
+      break;
+   case 5:  // corrector as 4; predictor without prefetch at 2 waves/SIMD
+      if (corr) SW4_SWEEP(true, true, 1, true); else SW4_SWEEP(false, false, 2, false);
+      break;
+   case 6:  // both sweeps without prefetch at 2 waves/SIMD; damping by launch_damp_lds
+      if (corr) SW4_SWEEP(true, false, 2, false); else SW4_SWEEP(false, false, 2, false);
       break;
    default:  // 1: no prefetch, 2 waves/SIMD
       if (corr) SW4_SWEEP(true, false, 2, false); else SW4_SWEEP(false, false, 2, false);

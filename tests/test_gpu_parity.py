@@ -218,6 +218,13 @@ def test_full_size_fused_matches_reference_order(torch):
     (4, 32, [2, 2, 2, 2, 0, 2]),
     (4, 7, [2, 2, 2, 2, 0, 0]),
     (4, 5, [2, 2, 2, 2, 2, 2]),
+    (4, 1, [2, 2, 2, 2, 0, 2]),
+    (4, 64, [2, 2, 2, 2, 0, 2]),
+    (5, 64, [2, 2, 2, 2, 0, 2]),
+    (5, 5, [2, 2, 2, 2, 2, 2]),
+    (6, 64, [2, 2, 2, 2, 0, 2]),
+    (6, 7, [2, 2, 2, 2, 0, 0]),
+    (6, 5, [2, 2, 2, 2, 2, 2]),
     (0, 32, [2, 2, 2, 2, 0, 2]),
 ])
 def test_sweep_variants_match_oracle(torch, variant, kchunk, bc):
