@@ -510,6 +510,11 @@ int launch_pred_fused(const Box& b, int nk, const int* onesided, const SbpTables
    }
    for (int side = 4; side <= 5; side++) {
       if (onesided[side] != 1) continue;
+      if (variant >= 1) {  // LDS slab of the six rows
+         const int rc = launch_sbp_rows(false, b, side, nk, d_sbp, nullptr, u, um, mu, la, rho, sg, up, h, dt, 0.0, s);
+         if (rc) return rc;
+         continue;
+      }
       dim3 grd(cdiv(nii, 64), cdiv(njj, 4), 6);
       k_pred_sbp<<<grd, blk, 0, s>>>(b, side == 4, nk, d_sbp, u, um, mu, la, rho, sg.strx, sg.stry, up, cof, dt2);
    }
@@ -556,6 +561,11 @@ int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables
    }
    for (int side = 4; side <= 5; side++) {
       if (onesided[side] != 1) continue;
+      if (variant >= 1) {  // LDS slab of the six rows, acceleration formed while staging
+         const int rc = launch_sbp_rows(true, b, side, nk, d_sbp, up, u, um, mu, la, rho, sg, upo, h, dt, beta, s);
+         if (rc) return rc;
+         continue;
+      }
       const int ks = side == 4 ? 0 : nk - SLAB_PLANES + 2;  // planes KK(0)..KK(8) of the rows
       k_accel_slab<<<cdiv(accel_slab_size(b), 256), 256, 0, s>>>(b, ks, up, u, um, dt2i, acc);
       dim3 grd(cdiv(nii, 64), cdiv(njj, 4), 6);
