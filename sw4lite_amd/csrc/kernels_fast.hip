@@ -355,10 +355,13 @@ k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __rest
 // k and rho of plane k are staged with their (i,j) halo in LDS; the thread's
 // own column of d for k-2..k+2 and rho at k+-1 live in registers.  Small LDS
 // and register footprint, so several workgroups share a CU and hide latency.
-__global__ void __launch_bounds__(NT, 2)
+// Tiles of 32 x DTY columns (DTY * 32 threads); taller tiles re-read less halo.
+template <int DTY>
+__global__ void __launch_bounds__(32 * DTY)
 k_damp_lds(Box b, TileMap map, int kbeg, int kend, int kc, double* __restrict__ up, const double* __restrict__ u,
            const double* __restrict__ um, const double* __restrict__ rho, SgCoef sg, double beta)
 {
+   constexpr int TY = DTY, HY = TY + 4, NT = TX * TY, PL = HX * HY;
    constexpr int NU = (3 * PL + NT - 1) / NT;
    constexpr int NM = (PL + NT - 1) / NT;
    __shared__ double sdp[3 * PL];
@@ -466,11 +469,12 @@ int launch_damp_lds(const Box& b, int k1, int k2, int kc, double* up, const doub
    if (k2 < k1 || beta == 0) return OK;
    const long nx = b.ie - b.ib - 3, ny = b.je - b.jb - 3;
    if (kc <= 0) kc = 64;
+   constexpr int DTY = 8;
    TileMap map;
    map.ntx = (int)cdiv(nx, TX);
-   map.nty = (int)cdiv(ny, TY);
+   map.nty = (int)cdiv(ny, DTY);
    map.ntiles = map.ntx * map.nty * (int)cdiv(k2 - k1 + 1, kc);
-   k_damp_lds<<<8 * cdiv(map.ntiles, 8), NT, 0, s>>>(b, map, k1, k2, kc, up, u, um, rho, sg, beta);
+   k_damp_lds<DTY><<<8 * cdiv(map.ntiles, 8), 32 * DTY, 0, s>>>(b, map, k1, k2, kc, up, u, um, rho, sg, beta);
    return check_hip(hipGetLastError(), "damping sweep launch");
 }
 
