@@ -41,6 +41,14 @@ HBM_PEAK_GBS = 8000.0
 CORR_BYTES_PER_PT = 15 * 8     # corrector sweep: Up, U, Um, mu, lambda, rho in, Up' out (FP64)
 PRED_BYTES_PER_PT = 12 * 8     # predictor sweep, FP64
 STEP_BYTES_PER_PT = 27 * 8     # whole step, compulsory traffic
+# FP32 form with the compensated update (float words): the predictor reads U,
+# mu, lambda, rho and writes A; the corrector (damping included) reads A, D,
+# U, C, mu, lambda, rho and writes Dn, U, C
+COMP_CORR_BYTES_PER_PT = 24 * 4
+COMP_PRED_BYTES_PER_PT = 9 * 4
+COMP_STEP_BYTES_PER_PT = 33 * 4
+STAGES = ["pred_interior", "pred_boundary_rows_shell", "pred_sources_bc", "corr_interior",
+          "corr_interior_damping", "corr_boundary_rows_shell", "corr_sources_bc"]
 
 
 def parse():
@@ -50,6 +58,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--n", type=int, default=512, help="grid points per axis")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-fp32", action="store_true", help="skip the FP32 compensated form")
     p.add_argument("--cpu-nz", type=int, default=128, help="depth of the CPU-baseline slab")
     p.add_argument("--cpu-steps", type=int, default=3)
     p.add_argument("--variant", type=int, default=6, help="interior sweep kernels: 0 thread/point, 1 LDS k-march, 2 with prefetch, "
@@ -337,6 +346,35 @@ def main_curvilinear(a, torch):
     print(json.dumps(line), flush=True)
 
 
+def bench_fp32(a, e, torch, n, fp64_value):
+    """The same steps in the FP32 form with the compensated time update
+    (EW step mode 2), continuing from the FP64 state."""
+    e.timesteploop(a.warmup, mode=2)
+    e.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stage_ms = e.timesteploop_timed(a.steps, fp32=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    pts = float(n) ** 3
+    value = pts * a.steps / wall
+    corr_pts = float(n) * n * (n - 6)
+    corr_ms = stage_ms[3] / a.steps
+    pred_ms = stage_ms[0] / a.steps
+    achieved = corr_pts * COMP_CORR_BYTES_PER_PT / (corr_ms * 1e-3) / 1e9
+    return {
+        "value": value, "unit": "grid-pt updates/s", "ms_per_step": 1e3 * wall / a.steps,
+        "dtype": "f32 stencils, compensated f32 pair update, f64 boundary closure and sources",
+        "speedup_vs_fp64": value / fp64_value,
+        "step_hbm_fraction": value * COMP_STEP_BYTES_PER_PT / (HBM_PEAK_GBS * 1e9),
+        "stage_ms_per_step": {k: v / a.steps for k, v in zip(STAGES, stage_ms.tolist())},
+        "pred_interior_GBps": corr_pts * COMP_PRED_BYTES_PER_PT / (pred_ms * 1e-3) / 1e9,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "kernel": "k_comp_sweep<true>",
+                     "bytes_per_launch": corr_pts * COMP_CORR_BYTES_PER_PT, "avg_launch_ms": corr_ms},
+    }
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -404,9 +442,7 @@ def main():
                         "4 double couples (Ricker 5 Hz)" % n,
             "grid": [n, n, n], "h": h, "dt": inf.dt, "kernel_variant": a.variant, "kchunk": a.kchunk, "sources": len(srcs), "grid_point_sources": inf.nsources,
             "step_hbm_fraction": value * STEP_BYTES_PER_PT / (HBM_PEAK_GBS * 1e9),
-            "stage_ms_per_step": {k: v / a.steps for k, v in zip(
-                ["pred_interior", "pred_boundary_rows_shell", "pred_sources_bc", "corr_interior",
-                 "corr_interior_damping", "corr_boundary_rows_shell", "corr_sources_bc"], stage_ms.tolist())},
+            "stage_ms_per_step": {k: v / a.steps for k, v in zip(STAGES, stage_ms.tolist())},
             "pred_interior_GBps": corr_pts * PRED_BYTES_PER_PT / (pred_ms * 1e-3) / 1e9,
         },
         "roofline": {
@@ -417,6 +453,8 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not a.no_fp32:
+        line["fp32_comp"] = bench_fp32(a, e, torch, n, value)
     e.close()
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(n, h, a.cpu_nz, a.cpu_steps, srcs)

@@ -230,7 +230,14 @@ int sw4hip_ew_get_solution(sw4hip_ew* ew, double* u, double* um, int on_device);
 /* Advance nsteps steps.  mode 0: fused sweeps (fast path, corder 1 only);
  * mode 1: the reference's stage order (Force, evalRHS, evalPredictor,
  * enforceBC, Force_tt, evalDpDmInTime, evalRHS, evalCorrector,
- * addSuperGridDamping, enforceBC; EW.C:2527-2842). */
+ * addSuperGridDamping, enforceBC; EW.C:2527-2842);
+ * mode 2: the fused sweeps in FP32 with a compensated time update, the
+ * same stage order as EW::timesteploop.  The solution is carried as
+ * u = U + C (two floats), D = u - um and the predicted acceleration; the
+ * ghost-point closures, source time functions and receivers use FP64.  The
+ * float state is built from U, Um on the first mode-2 step and written back
+ * to U, Um before any FP64 step or solution access.  The reference's float
+ * build (src/float/sw4.h:35) is the uncompensated counterpart. */
 int sw4hip_ew_step(sw4hip_ew* ew, int nsteps, int mode);
 /* The fused step with per-stage GPU times: hipEvents recorded on the
  * object's stream; stage_ms[7] receives the summed milliseconds of
@@ -239,6 +246,9 @@ int sw4hip_ew_step(sw4hip_ew* ew, int nsteps, int mode);
  * interior supergrid damping, corrector boundary rows + ghost shell,
  * corrector sources + BC).  Synchronises. */
 int sw4hip_ew_step_timed(sw4hip_ew* ew, int nsteps, double* stage_ms);
+/* Mode 2 (FP32, compensated) with the same per-stage times; the supergrid
+ * damping is part of the corrector sweep, so stage 5 reads 0. */
+int sw4hip_ew_step_comp_timed(sw4hip_ew* ew, int nsteps, double* stage_ms);
 /* Analytic point-source error of the current U (EW::exactSol +
  * normOfDifference, EW.C:3650-3669, 4526-4610). */
 int sw4hip_ew_point_source_error(sw4hip_ew* ew, double* linf, double* l2, double* solinf);

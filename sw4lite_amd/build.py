@@ -19,6 +19,7 @@ SOURCES = [
     "kernels_step.hip",
     "kernels_fast.hip",
     "kernels_curv.hip",
+    "kernels_comp.hip",
     "sbp_coefficients.cpp",
     "setup.cpp",
     "ew.cpp",

@@ -536,6 +536,7 @@ int sw4hip_ew_set_solution(sw4hip_ew* h, const double* u, const double* um, int 
    const size_t bytes = sizeof(double) * e.mU.size();
    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
    int rc;
+   if ((rc = e.comp_end())) return rc;
    if (u && (rc = check_hip(hipMemcpyAsync(e.mU.dev, u, bytes, kind, e.m_stream), "U copy"))) return rc;
    if (um && (rc = check_hip(hipMemcpyAsync(e.mUm.dev, um, bytes, kind, e.m_stream), "Um copy"))) return rc;
    return check_hip(hipStreamSynchronize(e.m_stream), "solution copy");
@@ -548,6 +549,7 @@ int sw4hip_ew_get_solution(sw4hip_ew* h, double* u, double* um, int on_device)
    const size_t bytes = sizeof(double) * e.mU.size();
    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
    int rc;
+   if ((rc = e.comp_end())) return rc;
    if (u && (rc = check_hip(hipMemcpyAsync(u, e.mU.dev, bytes, kind, e.m_stream), "U copy"))) return rc;
    if (um && (rc = check_hip(hipMemcpyAsync(um, e.mUm.dev, bytes, kind, e.m_stream), "Um copy"))) return rc;
    return check_hip(hipStreamSynchronize(e.m_stream), "solution copy");
@@ -559,7 +561,8 @@ int sw4hip_ew_step(sw4hip_ew* h, int nsteps, int mode)
    if (nsteps < 0) return set_error(E_INVAL, "negative step count");
    if (mode == 0) return h->ew->step_fused(nsteps);
    if (mode == 1) return h->ew->step_reference(nsteps);
-   return set_error(E_INVAL, "step mode must be 0 (fused) or 1 (reference order)");
+   if (mode == 2) return h->ew->step_comp(nsteps);
+   return set_error(E_INVAL, "step mode must be 0 (fused), 1 (reference order) or 2 (fused FP32, compensated)");
 }
 
 int sw4hip_ew_step_timed(sw4hip_ew* h, int nsteps, double* stage_ms)
@@ -568,6 +571,14 @@ int sw4hip_ew_step_timed(sw4hip_ew* h, int nsteps, double* stage_ms)
    if (nsteps < 0) return set_error(E_INVAL, "negative step count");
    if (!stage_ms) return set_error(E_INVAL, "null stage_ms");
    return h->ew->step_fused(nsteps, stage_ms);
+}
+
+int sw4hip_ew_step_comp_timed(sw4hip_ew* h, int nsteps, double* stage_ms)
+{
+   EWCHK(h);
+   if (nsteps < 0) return set_error(E_INVAL, "negative step count");
+   if (!stage_ms) return set_error(E_INVAL, "null stage_ms");
+   return h->ew->step_comp(nsteps, stage_ms);
 }
 
 int sw4hip_ew_point_source_error(sw4hip_ew* h, double* linf, double* l2, double* solinf)
@@ -594,6 +605,10 @@ int sw4hip_ew_device_array(sw4hip_ew* h, int which, double** ptr)
    EWCHK(h);
    if (!ptr) return set_error(E_INVAL, "null output");
    EW& e = *h->ew;
+   if (which <= 1) {
+      const int rc = e.comp_end();
+      if (rc) return rc;
+   }
    switch (which) {
    case 0: *ptr = e.mU.dev; break;
    case 1: *ptr = e.mUm.dev; break;

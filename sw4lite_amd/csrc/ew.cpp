@@ -413,6 +413,7 @@ int EW::step_reference(int n)
 {
    if (!m_setup_done) return set_error(E_STATE, "EW: step before setup");
    int rc;
+   if ((rc = comp_end())) return rc;
    if ((rc = ensure_reference_arrays())) return rc;
    for (int s = 0; s < n; s++) {
       const double t = mT;
@@ -433,6 +434,7 @@ int EW::step_reference(int n)
 int EW::step_fused(int n, double* stage_ms)
 {
    if (!m_setup_done) return set_error(E_STATE, "EW: step before setup");
+   if (int rc = comp_end()) return rc;
    if (!m_opt.corder) return set_error(E_INVAL, "EW: the fused step needs the component-blocked layout (corder=1)");
    if (m_curv) return set_error(E_INVAL, "EW: the fused step is Cartesian; step a curvilinear grid in the reference order");
    const GridSetup& g = m_opt.grid;
@@ -487,6 +489,153 @@ int EW::step_fused(int n, double* stage_ms)
    return rc;
 }
 
+// ---------------------------------------------------------------------------
+// FP32 form with the compensated time update
+// ---------------------------------------------------------------------------
+CompState EW::comp_state() const
+{
+   const long n = m_box.npts, n3 = 3 * n;
+   CompState c;
+   c.U = m_cbuf;
+   c.C = m_cbuf + n3;
+   c.D = m_cbuf + (2 + m_cd) * n3;
+   c.Dn = m_cbuf + (3 - m_cd) * n3;
+   c.A = m_cbuf + 4 * n3;
+   c.mu = m_cbuf + 5 * n3;
+   c.la = m_cbuf + 5 * n3 + n;
+   c.rho = m_cbuf + 5 * n3 + 2 * n;
+   return c;
+}
+
+SgCoefF EW::sgf() const
+{
+   const long ni = m_box.ni, nj = m_box.je - m_box.jb + 1, nk = m_box.ke - m_box.kb + 1;
+   const float* d = m_dsgf;
+   SgCoefF c;
+   c.dcx = d; c.strx = d + ni; c.cox = d + 2 * ni;
+   d += 3 * ni;
+   c.dcy = d; c.stry = d + nj; c.coy = d + 2 * nj;
+   d += 3 * nj;
+   c.dcz = d; c.strz = d + nk; c.coz = d + 2 * nk;
+   return c;
+}
+
+int EW::comp_begin()
+{
+   if (m_comp) return OK;
+   if (!m_opt.corder) return set_error(E_INVAL, "EW: the FP32 step needs the component-blocked layout (corder=1)");
+   if (m_curv) return set_error(E_INVAL, "EW: the FP32 step is Cartesian");
+   const long n = m_box.npts;
+   int rc;
+   if (!m_cbuf) {
+      const long nsg = 3 * (m_box.ni + (m_box.je - m_box.jb + 1) + (m_box.ke - m_box.kb + 1));
+      void* p = nullptr;
+      if (hipMalloc(&p, sizeof(float) * 18 * n) != hipSuccess) {
+         (void)hipGetLastError();
+         return set_error(E_NOMEM, "EW: FP32 state allocation of " + std::to_string(sizeof(float) * 18 * n) +
+                                       " bytes failed");
+      }
+      m_allocs.push_back(p);
+      m_cbuf = static_cast<float*>(p);
+      if (hipMalloc(&p, sizeof(float) * nsg + sizeof(SbpTablesT<float>)) != hipSuccess) {
+         (void)hipGetLastError();
+         return set_error(E_NOMEM, "EW: FP32 coefficient allocation failed");
+      }
+      m_allocs.push_back(p);
+      m_dsbpf = static_cast<SbpTablesT<float>*>(p);
+      m_dsgf = reinterpret_cast<float*>(static_cast<char*>(p) + sizeof(SbpTablesT<float>));
+      SbpTables t;
+      fill_stencil_coefficients(t.acof, t.ghcof, t.bope, t.sbop);
+      SbpTablesT<float> tf;
+      for (int q = 0; q < 384; q++) tf.acof[q] = (float)t.acof[q];
+      for (int q = 0; q < 48; q++) tf.bope[q] = (float)t.bope[q];
+      for (int q = 0; q < 6; q++) tf.ghcof[q] = (float)t.ghcof[q];
+      for (int q = 0; q < 5; q++) tf.sbop[q] = (float)t.sbop[q];
+      if ((rc = check_hip(hipMemcpy(m_dsbpf, &tf, sizeof(tf), hipMemcpyHostToDevice), "FP32 SBP upload"))) return rc;
+      if ((rc = launch_to_float(nsg, m_dsg, m_dsgf, m_stream))) return rc;
+   }
+   float* mat = m_cbuf + 15 * n;
+   if ((rc = launch_to_float(n, mMu.dev, mat, m_stream)) || (rc = launch_to_float(n, mLambda.dev, mat + n, m_stream)) ||
+       (rc = launch_to_float(n, mRho.dev, mat + 2 * n, m_stream)))
+      return rc;
+   const CompState cs = comp_state();
+   if ((rc = launch_comp_init(3 * n, mU.dev, mUm.dev, cs.U, cs.C, cs.D, m_stream))) return rc;
+   m_comp = true;
+   return OK;
+}
+
+int EW::comp_end()
+{
+   if (!m_comp) return OK;
+   m_comp = false;
+   const CompState cs = comp_state();
+   return launch_comp_flush(3 * m_box.npts, cs.U, cs.C, cs.D, mU.dev, mUm.dev, m_stream);
+}
+
+int EW::step_comp(int n, double* stage_ms)
+{
+   if (!m_setup_done) return set_error(E_STATE, "EW: step before setup");
+   int rc = comp_begin();
+   if (rc) return rc;
+   const GridSetup& g = m_opt.grid;
+   const SgCoef c = sg();
+   const SgCoefF cf = sgf();
+   const SourceTable st = sources();
+   const int nrec = (int)m_rec.size() / 4;
+   std::vector<hipEvent_t> ev;
+   if (stage_ms) {
+      ev.resize((size_t)n * (kStages + 1));
+      for (auto& e : ev)
+         if ((rc = check_hip(hipEventCreate(&e), "hipEventCreate"))) return rc;
+   }
+   auto mark = [&](int s, int q) {
+      return stage_ms ? check_hip(hipEventRecord(ev[(size_t)s * (kStages + 1) + q], m_stream), "hipEventRecord") : OK;
+   };
+   for (int s = 0; s < n && rc == OK; s++) {
+      const double t = mT, dt = mDt;
+      const CompState cs = comp_state();
+      // the damping is part of the corrector sweep: stages 4 and 5 coincide
+      if ((rc = mark(s, 0)) ||
+          (rc = launch_comp_pred(m_box, g.nz, m_onesided, m_dsbpf, cs, cf, g.h, m_stream,
+                                 [&]() { return mark(s, 1); }, m_kchunk)) ||
+          (rc = mark(s, 2)) || (rc = launch_comp_src(false, m_box, st, t, 1.0, cs, m_stream)) ||
+          (rc = launch_comp_bc(false, m_box, m_wind, g.nx, g.ny, g.nz, g.bc, cs, g.h, dt, m_dsbp, mMu.dev,
+                               mLambda.dev, c.strx, c.stry, m_stream)) ||
+          (rc = mark(s, 3)) ||
+          (rc = launch_comp_corr(m_box, g.nz, m_onesided, m_dsbpf, cs, cf, m_opt.beta, g.h, dt, m_stream,
+                                 [&]() { const int r = mark(s, 4); return r ? r : mark(s, 5); }, m_kchunk)) ||
+          (rc = mark(s, 6)) ||
+          (rc = launch_comp_src(true, m_box, st, t, dt * dt * dt * dt / 12, cs, m_stream)) ||
+          (rc = launch_comp_bc(true, m_box, m_wind, g.nx, g.ny, g.nz, g.bc, cs, g.h, dt, m_dsbp, mMu.dev,
+                               mLambda.dev, c.strx, c.stry, m_stream)) ||
+          (rc = mark(s, 7)))
+         break;
+      if (nrec > 0 && m_drecdata && m_nsamples < m_maxsamples) {
+         if ((rc = launch_comp_record(m_box, nrec, m_drec, cs, dt, m_drecdata + (size_t)3 * nrec * m_nsamples,
+                                      m_stream)))
+            break;
+         m_nsamples++;
+      }
+      mT += dt;
+      mCycle++;
+      m_cd ^= 1;  // Dn becomes D
+   }
+   if (stage_ms) {
+      if (rc == OK) rc = check_hip(hipStreamSynchronize(m_stream), "timed step synchronize");
+      for (int q = 0; q < kStages; q++) stage_ms[q] = 0;
+      for (int s = 0; s < n && rc == OK; s++)
+         for (int q = 0; q < kStages; q++) {
+            float ms = 0;
+            rc = check_hip(hipEventElapsedTime(&ms, ev[(size_t)s * (kStages + 1) + q], ev[(size_t)s * (kStages + 1) + q + 1]),
+                           "hipEventElapsedTime");
+            if (rc) break;
+            stage_ms[q] += ms;
+         }
+      for (auto& e : ev) (void)hipEventDestroy(e);
+   }
+   return rc;
+}
+
 // EW::exactSol + normOfDifference for the first source (EW.C:3650-3669,
 // 3899-4523, 4526-4610), written to the scratch array mUp.
 int EW::point_source_error(double& linf, double& l2, double& solinf)
@@ -498,6 +647,7 @@ int EW::point_source_error(double& linf, double& l2, double& solinf)
    if (s0.tfkind != TF_GAUSSIAN && s0.tfkind != TF_C6SMOOTHBUMP)
       return set_error(E_INVAL, "EW: analytic solution needs a Gaussian or C6SmoothBump source");
    int rc;
+   if ((rc = comp_end())) return rc;
    // material at the grid midpoint
    const int im = (m_box.ib + m_box.ie) / 2, jm = (m_box.jb + m_box.je) / 2, km = (m_box.kb + m_box.ke) / 2;
    const long pm = sidx(m_box, im, jm, km);

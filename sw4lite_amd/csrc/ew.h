@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "kernels_common.h"
+#include "kernels_comp.h"
 #include "kernels_curv.h"
 #include "kernels_step.h"
 #include "setup.h"
@@ -75,6 +76,13 @@ public:
    // corrector sources + BC
    static constexpr int kStages = 7;
    int step_fused(int n, double* stage_ms = nullptr);
+   // The fused step in FP32 with the compensated time update
+   // (kernels_comp.h), same stage times.  The float state is built from U, Um
+   // on the first such step; comp_end writes it back to U, Um and every FP64
+   // access to the solution calls it first.
+   int step_comp(int n, double* stage_ms = nullptr);
+   int comp_end();
+   bool comp_active() const { return m_comp; }
    int point_source_error(double& linf, double& l2, double& solinf);
 
    // receivers (TimeSeries, EW.C:5854-5875): mode 0 displacement, 1 velocity
@@ -111,6 +119,9 @@ private:
    int ensure_reference_arrays();
    SgCoef sg() const;
    SourceTable sources() const;
+   int comp_begin();
+   CompState comp_state() const;
+   SgCoefF sgf() const;
 
    double* m_dsg = nullptr;  // 9 supergrid arrays back to back
    SbpTables* m_dsbp = nullptr;
@@ -120,6 +131,11 @@ private:
    double* m_drecdata = nullptr;
    int m_maxsamples = 0;
    int m_nred = 1024;
+   float* m_cbuf = nullptr;  // FP32 form: U, C, D0, D1, A (3 components each), mu, lambda, rho
+   float* m_dsgf = nullptr;  // float copies of the supergrid arrays (layout of m_dsg)
+   SbpTablesT<float>* m_dsbpf = nullptr;
+   int m_cd = 0;             // D0 (0) or D1 (1) holds u^n - u^{n-1}
+   bool m_comp = false;      // the float state holds the solution
    bool m_setup_done = false, m_material_set = false, m_curv = false;
    std::vector<void*> m_allocs;
 };

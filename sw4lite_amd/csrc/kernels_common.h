@@ -58,22 +58,25 @@ struct Slab {
 };
 
 // 4th-order centred first difference times 12h: f(-2)-f(+2)+8(f(+1)-f(-1))
-__device__ __forceinline__ double d4(double fm2, double fm1, double fp1, double fp2)
+template <class R>
+__device__ __forceinline__ R d4(R fm2, R fm1, R fp1, R fp2)
 {
    return fm2 - fp2 + 8 * (fp1 - fm1);
 }
 
 // Coefficients of the variable-coefficient second difference (a u_x)_x on
 // a 5-point window of a = coefficient*stretching (rhs4sg_rev.C:120-145).
-__device__ __forceinline__ void varcoef4(const double a[5], double m[4])
+template <class R>
+__device__ __forceinline__ void varcoef4(const R a[5], R m[4])
 {
-   m[0] = a[1] - 0.75 * (a[2] + a[0]);
+   m[0] = a[1] - R(0.75) * (a[2] + a[0]);
    m[1] = a[0] + a[3] + 3 * (a[2] + a[1]);
    m[2] = a[1] + a[4] + 3 * (a[3] + a[2]);
-   m[3] = a[3] - 0.75 * (a[2] + a[4]);
+   m[3] = a[3] - R(0.75) * (a[2] + a[4]);
 }
 
-__device__ __forceinline__ double varsum4(const double m[4], const double f[5])
+template <class R>
+__device__ __forceinline__ R varsum4(const R m[4], const R f[5])
 {
    return m[0] * (f[0] - f[2]) + m[1] * (f[1] - f[2]) + m[2] * (f[3] - f[2]) + m[3] * (f[4] - f[2]);
 }
@@ -286,6 +289,43 @@ __device__ __forceinline__ void rhs_sbp_point(const Box& b, int i, int j, int k,
 #undef KK
 }
 
+// Point q of the ghost shell, the 2-point frame outside [1,nx]x[1,ny]x[1,nz]
+// (the k planes kb, kb+1, ke-1, ke first, then the j and i frames of the
+// remaining planes); false past the last shell point.
+__device__ __forceinline__ bool shell_point(const Box& b, long q, int& i, int& j, int& k)
+{
+   const long ni = b.ni, nj = b.je - b.jb + 1;
+   const long nxi = ni - 4, nyi = nj - 4, nzi = (b.ke - b.kb + 1) - 4;
+   const long nk_planes = 4 * ni * nj;      // k in {kb,kb+1,ke-1,ke}
+   const long nj_planes = 4 * ni * nzi;     // j in {jb,jb+1,je-1,je}, k interior
+   const long ni_planes = 4 * nyi * nzi;    // i in {ib,ib+1,ie-1,ie}, j,k interior
+   if (q < nk_planes) {
+      const long pl = q / (ni * nj), r = q % (ni * nj);
+      k = pl < 2 ? b.kb + (int)pl : b.ke - 3 + (int)pl;
+      j = b.jb + (int)(r / ni);
+      i = b.ib + (int)(r % ni);
+      return true;
+   }
+   q -= nk_planes;
+   if (q < nj_planes) {
+      const long pl = q / (ni * nzi), r = q % (ni * nzi);
+      j = pl < 2 ? b.jb + (int)pl : b.je - 3 + (int)pl;
+      k = b.kb + 2 + (int)(r / ni);
+      i = b.ib + (int)(r % ni);
+      return true;
+   }
+   q -= nj_planes;
+   if (q < ni_planes) {
+      const long pl = q / (nyi * nzi), r = q % (nyi * nzi);
+      i = pl < 2 ? b.ib + (int)pl : b.ie - 3 + (int)pl;
+      k = b.kb + 2 + (int)(r / nyi);
+      j = b.jb + 2 + (int)(r % nyi);
+      return true;
+   }
+   (void)nxi;
+   return false;
+}
+
 // Damping increment (to be subtracted from up) of the 4th-order supergrid
 // term at one point, all three components (ew-cfromfort.C:985-1041).
 template <int CO>
@@ -331,10 +371,11 @@ __device__ __forceinline__ void addsgd4_point(const Box& b, int i, int j, int k,
 
 // Ghost value below (kl=1, top) or above (kl=-1, bottom) the free surface at
 // plane k, from zero-traction minus the boundary forcing t[3]
-// (ew-cfromfort.C:683-742).
-template <int CO>
+// (ew-cfromfort.C:683-742).  u is an array or a read accessor (operator[]
+// on the layout's offsets returning double).
+template <int CO, class UA>
 __device__ __forceinline__ void freesurf_point(const Box& b, int i, int j, int k, int kl, double h,
-                                               const double* sbop, const double* __restrict__ u,
+                                               const double* sbop, const UA& u,
                                                const double* __restrict__ mu,
                                                const double* __restrict__ la, const double t[3],
                                                const double* __restrict__ strx,

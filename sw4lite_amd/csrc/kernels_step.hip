@@ -150,43 +150,9 @@ k_addsgd4_planes(Box b, int klo, int khi, double* __restrict__ up, const double*
 }
 
 // ---------------------------------------------------------------------------
-// Ghost shell: the 2-point frame outside [1,nx]x[1,ny]x[1,nz].  mode 0:
-// up = 2u - um (the predictor where L(u) and F vanish); mode 1: out = in.
+// Ghost shell (shell_point, kernels_common.h).  mode 0: up = 2u - um (the
+// predictor where L(u) and F vanish); mode 1: out = in.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool shell_point(const Box& b, long q, int& i, int& j, int& k)
-{
-   const long ni = b.ni, nj = b.je - b.jb + 1;
-   const long nxi = ni - 4, nyi = nj - 4, nzi = (b.ke - b.kb + 1) - 4;
-   const long nk_planes = 4 * ni * nj;      // k in {kb,kb+1,ke-1,ke}
-   const long nj_planes = 4 * ni * nzi;     // j in {jb,jb+1,je-1,je}, k interior
-   const long ni_planes = 4 * nyi * nzi;    // i in {ib,ib+1,ie-1,ie}, j,k interior
-   if (q < nk_planes) {
-      const long pl = q / (ni * nj), r = q % (ni * nj);
-      k = pl < 2 ? b.kb + (int)pl : b.ke - 3 + (int)pl;
-      j = b.jb + (int)(r / ni);
-      i = b.ib + (int)(r % ni);
-      return true;
-   }
-   q -= nk_planes;
-   if (q < nj_planes) {
-      const long pl = q / (ni * nzi), r = q % (ni * nzi);
-      j = pl < 2 ? b.jb + (int)pl : b.je - 3 + (int)pl;
-      k = b.kb + 2 + (int)(r / ni);
-      i = b.ib + (int)(r % ni);
-      return true;
-   }
-   q -= nj_planes;
-   if (q < ni_planes) {
-      const long pl = q / (nyi * nzi), r = q % (nyi * nzi);
-      i = pl < 2 ? b.ib + (int)pl : b.ie - 3 + (int)pl;
-      k = b.kb + 2 + (int)(r / nyi);
-      j = b.jb + 2 + (int)(r % nyi);
-      return true;
-   }
-   (void)nxi;
-   return false;
-}
-
 __global__ void k_shell(Box b, int mode, const double* __restrict__ a, const double* __restrict__ u,
                         const double* __restrict__ um, double* __restrict__ out)
 {

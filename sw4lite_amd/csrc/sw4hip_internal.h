@@ -42,12 +42,14 @@ __host__ __device__ __forceinline__ long vidx(const Box& b, int c, int i, int j,
 // SBP coefficient block kept in device memory; uniform loads become scalar
 // (constant-cache) loads.  Layout is the reference's: acof(k,q,m) at
 // (k-1)+6(q-1)+48(m-1), bope(k,q) at (k-1)+6(q-1).
-struct SbpTables {
-   double acof[384];
-   double bope[48];
-   double ghcof[6];
-   double sbop[5];
+template <class R>
+struct SbpTablesT {
+   R acof[384];
+   R bope[48];
+   R ghcof[6];
+   R sbop[5];
 };
+using SbpTables = SbpTablesT<double>;
 
 // Status codes of the C ABI (mirrored in include/sw4hip.h).
 enum Status : int {

@@ -275,16 +275,19 @@ class EW:
 
     def timesteploop(self, nsteps=None, mode=0):
         """Advance nsteps (default: all remaining) steps; mode 0 fused sweeps,
-        mode 1 the reference stage order."""
+        mode 1 the reference stage order, mode 2 the fused sweeps in FP32 with
+        the compensated time update."""
         inf = self.info()
         if nsteps is None:
             nsteps = inf.nsteps - inf.cycle
         L.check(self.lib.sw4hip_ew_step(self.h, int(nsteps), int(mode)))
 
-    def timesteploop_timed(self, nsteps):
-        """Fused steps with per-stage GPU times (ms, summed over the steps)."""
+    def timesteploop_timed(self, nsteps, fp32=False):
+        """Fused steps (FP64, or FP32 compensated) with per-stage GPU times
+        (ms, summed over the steps)."""
         ms = np.zeros(7)
-        L.check(self.lib.sw4hip_ew_step_timed(self.h, int(nsteps), _ptr(ms)))
+        fn = self.lib.sw4hip_ew_step_comp_timed if fp32 else self.lib.sw4hip_ew_step_timed
+        L.check(fn(self.h, int(nsteps), _ptr(ms)))
         return ms
 
     def sync(self):

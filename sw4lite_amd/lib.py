@@ -64,6 +64,7 @@ _SIGS = {
     "sw4hip_ew_get_solution": (I, [P, P, P, I]),
     "sw4hip_ew_step": (I, [P, I, I]),
     "sw4hip_ew_step_timed": (I, [P, I, P]),
+    "sw4hip_ew_step_comp_timed": (I, [P, I, P]),
     "sw4hip_ew_point_source_error": (I, [P, DP, DP, DP]),
     "sw4hip_ew_add_receiver": (I, [P, D, D, D, I, IP]),
     "sw4hip_ew_get_receivers": (I, [P, IP, P]),
