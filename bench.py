@@ -54,9 +54,9 @@ STAGES = ["pred_interior", "pred_boundary_rows_shell", "pred_sources_bc", "corr_
 def parse():
     p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=100)
+    p.add_argument("--steps", type=int, default=None, help="timed steps (default 100; 800 for the precision workload)")
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--n", type=int, default=512, help="grid points per axis")
+    p.add_argument("--n", type=int, default=None, help="grid points per axis (default 512; 1024 for the precision workload)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-fp32", action="store_true", help="skip the FP32 compensated form")
     p.add_argument("--cpu-nz", type=int, default=128, help="depth of the CPU-baseline slab")
@@ -67,17 +67,25 @@ def parse():
     p.add_argument("--kchunk", type=int, default=64, help="planes per workgroup of the LDS sweeps")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also launch a plain device copy of 1 GiB (PMC scaling)")
-    p.add_argument("--workload", choices=["cartesian", "curvilinear"], default="cartesian",
+    p.add_argument("--workload", choices=["cartesian", "curvilinear", "precision"], default="cartesian",
                    help="cartesian: the full time step at N^3 (default); curvilinear: the curvilinear-grid "
                         "kernels (rhs4sgcurv, addsgd4c, free-surface closure) on a 384x384x256 grid under "
-                        "Gaussian hills")
-    return p.parse_args()
+                        "Gaussian hills; precision: FP32 compensated against FP64 on 64 surface receivers "
+                        "(1024^3, h=10 m, 800 steps)")
+    a = p.parse_args()
+    if a.n is None:
+        a.n = 1024 if a.workload == "precision" else 512
+    if a.steps is None:
+        a.steps = 800 if a.workload == "precision" else 100
+    return a
 
 
-def make_material(torch, n, h, seed, device="cuda", z=None):
+def make_material(torch, n, h, seed, device="cuda", z=None, width=17):
     """rho, mu, lambda over the box of n = N or (nx, ny, nz) interior points
     (two ghost points per side); depth from the grid depth field z (same box,
-    flattened) when given, else z = (k-1) h."""
+    flattened) when given, else z = (k-1) h.  The perturbation is smoothed by
+    three passes of a `width`-point box filter per axis (17: a near-Gaussian
+    of 8.5 cells)."""
     dev = torch.device(device)
     nx, ny, nz = (n, n, n) if isinstance(n, int) else n
     shp = (nz + 4, ny + 4, nx + 4)
@@ -85,7 +93,7 @@ def make_material(torch, n, h, seed, device="cuda", z=None):
     gen.manual_seed(seed)
     g = torch.randn(shp, generator=gen, device=dev, dtype=torch.float64)
     F = torch.nn.functional
-    W = 17  # three passes of a W-point box filter: a near-Gaussian of 8.5 cells
+    W = width
 
     def box(x):  # along the last axis, zero padded
         c = F.pad(x, (W // 2 + 1, W // 2)).cumsum(-1)
@@ -375,6 +383,83 @@ def bench_fp32(a, e, torch, n, fp64_value):
     }
 
 
+def main_precision(a, torch):
+    """The workload model at twice the resolution (h = 10 m, the smoothing
+    length of the perturbation kept at 170 m), run once with the FP64 fused
+    step and once with the FP32 form with the compensated update from the
+    same zero initial state; 64 seeded surface receivers record displacement
+    after every step.  The four double couples sit 200-800 m deep and the
+    receivers within 1.5 km of their epicentres, so that P and S arrivals
+    reach the receivers within the 800 steps (1.07 s).  Reported: per
+    component, the relative L2 error of the FP32 traces against the FP64
+    traces over all receivers and samples, and the maximum error relative to
+    the largest FP64 amplitude."""
+    from sw4lite_amd import EW
+    from sw4lite_amd import lib as L
+    n, h, steps = a.n, 10.0 * 1024 / a.n, a.steps
+    span = (n - 1) * h
+    rng = np.random.default_rng(4)
+    srcs = []
+    for _ in range(4):
+        x, y = rng.uniform(0.2 * span, 0.8 * span), rng.uniform(0.2 * span, 0.8 * span)
+        srcs.append(dict(x=x, y=y, z=rng.uniform(200.0, 800.0), m=double_couple(rng, 1e15), t0=0.24, freq=5.0,
+                         tfkind=0))
+    recs = []
+    for r in range(64):
+        s0 = srcs[r % 4]
+        rad, ang = rng.uniform(0.0, 1500.0), rng.uniform(0.0, 2 * np.pi)
+        recs.append((s0["x"] + rad * np.cos(ang), s0["y"] + rad * np.sin(ang), 0.0))
+    traces, secs = {}, {}
+    for mode in (0, 2):
+        rho, mu, la = make_material(torch, n, h, seed=4, width=33)
+        torch.cuda.empty_cache()
+        e = EW(n, n, n, h, [L.BC_SUPERGRID] * 4 + [L.BC_FREE, L.BC_SUPERGRID], corder=1, sg_gp_thickness=30)
+        e.set_material(rho, mu, la)
+        del rho, mu, la
+        torch.cuda.empty_cache()
+        for s in srcs:
+            e.add_moment_source(s["x"], s["y"], s["z"], s["m"], s["t0"], s["freq"], s["tfkind"])
+        for x, y, z in recs:
+            e.add_receiver(x, y, z, 0)
+        e.setup(nsteps=steps)
+        dt = e.info().dt
+        done, t0 = 0, time.perf_counter()
+        while done < steps:
+            chunk = min(100, steps - done)
+            e.timesteploop(chunk, mode=mode)
+            e.sync()
+            done += chunk
+            sys.stderr.write("precision: mode %d, %d/%d steps, %.1f s\n" % (mode, done, steps, time.perf_counter() - t0))
+            sys.stderr.flush()
+        secs[mode] = time.perf_counter() - t0
+        traces[mode] = e.receivers()
+        e.close()
+        torch.cuda.empty_cache()
+    ref, got = traces[0], traces[2]
+    d = got - ref
+    rel_l2 = [float(np.linalg.norm(d[..., c]) / np.linalg.norm(ref[..., c])) for c in range(3)]
+    max_err = [float(np.max(np.abs(d[..., c])) / np.max(np.abs(ref[..., c]))) for c in range(3)]
+    pts = float(n) ** 3
+    line = {
+        "metric": "FP32-compensated seismogram error vs FP64 (relative L2, worst component)",
+        "value": max(rel_l2), "unit": "relative L2", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": 1e3 * secs[2] / steps, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 stencils, compensated f32 pair update, f64 boundary closure and sources",
+        "data": "synthetic",
+        "config": {
+            "workload": "%d^3 Cartesian step, h=%g m, layered+lognormal material (smoothing 170 m), free top, "
+                        "supergrid 30, 4 double couples 200-800 m deep (Ricker 5 Hz), 64 surface receivers "
+                        "within 1.5 km of the epicentres" % (n, h),
+            "grid": [n, n, n], "h": h, "dt": dt, "receivers": len(recs),
+            "rel_l2_per_component": rel_l2, "max_err_per_component": max_err, "tolerance": 1e-5,
+            "fp64_ms_per_step": 1e3 * secs[0] / steps, "fp32_ms_per_step": 1e3 * secs[2] / steps,
+            "fp64_grid_pt_updates_per_s": pts * steps / secs[0], "fp32_grid_pt_updates_per_s": pts * steps / secs[2],
+            "max_abs_fp64_trace": float(np.max(np.abs(ref))),
+        },
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -388,6 +473,8 @@ def main():
         sys.exit(1)
     if a.workload == "curvilinear":
         return main_curvilinear(a, torch)
+    if a.workload == "precision":
+        return main_precision(a, torch)
     n, h = a.n, 20.0
     srcs = sources(n, h, seed=2)
     rho, mu, la = make_material(torch, n, h, seed=2)
