@@ -119,6 +119,38 @@ int sw4hip_addsgd4(int corder, int ifirst, int ilast, int jfirst, int jlast, int
                    const double* stry, const double* strz, const double* cox, const double* coy,
                    const double* coz, double beta, void* stream);
 
+/* Transposes of the kernels above (no counterpart in the reference: they
+ * are the adjoints of sw4hip_rhs4sg, sw4hip_addsgd4 and sw4hip_bcfortsg, for
+ * the component-blocked layout and the index box [-1, n+2] of one grid).
+ * sw4hip_rhs4sg_adjoint: out = L^T z on every point, z read on the rows of L
+ *   (the interior, with the SBP rows where onesided).
+ * sw4hip_rhs4sg_matgrad: gmu += d/dmu (z . L(u)), glambda += d/dlambda (z . L(u)).
+ * sw4hip_addsgd4_adjoint: for up -= D(u - um; rho): dbar = -D^T y on every
+ *   point, grho += d/drho (-y . D).
+ * sw4hip_bcfortsg_adjoint: y <- BC^T y for zero boundary data (sides in
+ *   reverse order; Dirichlet/supergrid ghosts dropped, free-surface ghosts
+ *   pushed into the points their closure reads), gmu, glambda += the closure's
+ *   derivatives evaluated on fwd, the field the BC acted on.  Periodic sides
+ *   are rejected.  Synchronises. */
+int sw4hip_rhs4sg_adjoint(int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast, int nk,
+                          const int* onesided, const double* acof, const double* bope, const double* ghcof,
+                          double* out, const double* z, const double* mu, const double* lambda, double h,
+                          const double* strx, const double* stry, const double* strz, void* stream);
+int sw4hip_rhs4sg_matgrad(int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast, int nk,
+                          const int* onesided, const double* acof, const double* bope, const double* ghcof,
+                          const double* z, const double* u, const double* mu, const double* lambda, double h,
+                          const double* strx, const double* stry, const double* strz, double* gmu,
+                          double* glambda, void* stream);
+int sw4hip_addsgd4_adjoint(int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast,
+                           const double* y, const double* u, const double* um, const double* rho,
+                           const double* dcx, const double* dcy, const double* dcz, const double* strx,
+                           const double* stry, const double* strz, const double* cox, const double* coy,
+                           const double* coz, double beta, double* dbar, double* grho, void* stream);
+int sw4hip_bcfortsg_adjoint(int ib, int ie, int jb, int je, int kb, int ke, const int* wind, int nx, int ny,
+                            int nz, double* y, const double* fwd, double h, const int* bccnd, const double* sbop,
+                            const double* mu, const double* lambda, const double* strx, const double* stry,
+                            double* gmu, double* glambda, void* stream);
+
 /* Replaces EW::bcfortsg / bcfortsg_indrev (ew-cfromfort.C:205-747).
  * wind[36], bccnd[6], sbop[5] are HOST tables; bforce[6] is a host array of
  * device pointers (NULL entries mean zero forcing).  t, om, ph, cv of the
@@ -252,6 +284,23 @@ int sw4hip_ew_step_comp_timed(sw4hip_ew* ew, int nsteps, double* stage_ms);
 /* Analytic point-source error of the current U (EW::exactSol +
  * normOfDifference, EW.C:3650-3669, 4526-4610). */
 int sw4hip_ew_point_source_error(sw4hip_ew* ew, double* linf, double* l2, double* solinf);
+
+/* ---- Discrete adjoint of the FP64 step (no counterpart in the reference,
+ * which has no adjoint: these are the exact transposes of the maps of
+ * EW::timesteploop, EW.C:2527-2842, for component-blocked Cartesian grids
+ * with supergrid, Dirichlet and free-surface sides) ---- */
+/* Zero U, Um at t = 0 and restart the receivers with the t = 0 sample. */
+int sw4hip_ew_reset(sw4hip_ew* ew);
+/* Gradient of a receiver misfit J with respect to rho, mu and lambda on
+ * every point of the box (ghost points included).  Runs nsteps fused steps
+ * from zero initial data, saving (U, Um) every ckpt steps (ckpt <= 0:
+ * sqrt(nsteps)), then the reverse sweep, recomputing one checkpoint segment
+ * at a time.  grad_traces: HOST array [nsteps+1][nreceivers][3] of dJ/d(sample)
+ * in the order of sw4hip_ew_get_receivers.  dt stays the one of setup (set
+ * a new material with sw4hip_ew_set_material after setup to keep it).
+ * Outputs overwritten; device pointers when on_device. */
+int sw4hip_ew_gradient(sw4hip_ew* ew, int nsteps, int ckpt, const double* grad_traces, double* grho,
+                       double* gmu, double* glambda, int on_device);
 /* A receiver at the grid point nearest to (x,y,z) (EW::computeNearestGridPoint,
  * EW.C:5149-5180); mode 0 records displacement, 1 velocity (up-um)/(2dt)
  * (EW::extractRecordData, EW.C:5854-5875).  One sample at t=0 and one per

@@ -20,9 +20,11 @@ SOURCES = [
     "kernels_fast.hip",
     "kernels_curv.hip",
     "kernels_comp.hip",
+    "kernels_adj.hip",
     "sbp_coefficients.cpp",
     "setup.cpp",
     "ew.cpp",
+    "adjoint.cpp",
     "abi.cpp",
 ]
 

@@ -5,6 +5,7 @@
 
 #include "../../include/sw4hip.h"
 #include "ew.h"
+#include "kernels_adj.h"
 #include "kernels_curv.h"
 
 namespace sw4amd {
@@ -273,6 +274,121 @@ int sw4hip_bcfortsg(int corder, int ib, int ie, int jb, int je, int kb, int ke, 
    if ((rc = device_sbp(t, &d))) return rc;
    return launch_bcfortsg(corder, make_box(ib, ie, jb, je, kb, ke), wind, nx, ny, nz, u, h, bccnd, d, mu, lambda,
                           bforce, strx, stry, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// Transposed kernels (discrete adjoint)
+// ---------------------------------------------------------------------------
+namespace {
+int sbp_from(const double* acof, const double* bope, const double* ghcof, const SbpTables** d)
+{
+   SbpTables t = default_tables();
+   if (acof) std::memcpy(t.acof, acof, sizeof(t.acof));
+   if (bope) std::memcpy(t.bope, bope, sizeof(t.bope));
+   if (ghcof) std::memcpy(t.ghcof, ghcof, sizeof(t.ghcof));
+   return device_sbp(t, d);
+}
+int rows_ok(int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast, int nk, const int* onesided)
+{
+   if (!box_ok(ifirst, ilast, jfirst, jlast, kfirst, klast)) return set_error(E_INVAL, "box too small");
+   if (ifirst != -1 || jfirst != -1 || kfirst != -1 || klast != nk + 2)
+      return set_error(E_INVAL, "transposed kernels need the index box [-1, n+2] of one grid");
+   if (onesided && (onesided[4] == 1 || onesided[5] == 1) && nk < 12)
+      return set_error(E_INVAL, "one-sided closures need nk >= 12");
+   return OK;
+}
+}  // namespace
+
+int sw4hip_rhs4sg_adjoint(int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast, int nk,
+                          const int* onesided, const double* acof, const double* bope, const double* ghcof, double* out,
+                          const double* z, const double* mu, const double* lambda, double h, const double* strx,
+                          const double* stry, const double* strz, void* stream)
+{
+   int rc;
+   if ((rc = rows_ok(ifirst, ilast, jfirst, jlast, kfirst, klast, nk, onesided))) return rc;
+   if (!onesided || !acof || !bope || !ghcof || !out || !z || !mu || !lambda || !strx || !stry || !strz)
+      return set_error(E_INVAL, "rhs4sg_adjoint: null argument");
+   if (!(h > 0)) return set_error(E_INVAL, "rhs4sg_adjoint: h must be positive");
+   const SbpTables* d = nullptr;
+   if ((rc = sbp_from(acof, bope, ghcof, &d))) return rc;
+   const Box b = make_box(ifirst, ilast, jfirst, jlast, kfirst, klast);
+   void* work = nullptr;
+   if (hipMalloc(&work, sizeof(double) * 3 * b.npts) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(E_NOMEM, "rhs4sg_adjoint: scratch allocation failed");
+   }
+   rc = launch_rhs4sg_T(b, make_rowmap(b, nk, onesided), d, z, mu, lambda, h, strx, stry, strz, out,
+                        static_cast<double*>(work), (hipStream_t)stream);
+   const int rs = check_hip(hipStreamSynchronize((hipStream_t)stream), "rhs4sg_adjoint");
+   (void)hipFree(work);
+   return rc ? rc : rs;
+}
+
+int sw4hip_rhs4sg_matgrad(int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast, int nk,
+                          const int* onesided, const double* acof, const double* bope, const double* ghcof,
+                          const double* z, const double* u, const double* mu, const double* lambda, double h,
+                          const double* strx, const double* stry, const double* strz, double* gmu, double* glambda,
+                          void* stream)
+{
+   int rc;
+   if ((rc = rows_ok(ifirst, ilast, jfirst, jlast, kfirst, klast, nk, onesided))) return rc;
+   if (!onesided || !acof || !bope || !ghcof || !z || !u || !mu || !lambda || !strx || !stry || !strz || !gmu ||
+       !glambda)
+      return set_error(E_INVAL, "rhs4sg_matgrad: null argument");
+   if (!(h > 0)) return set_error(E_INVAL, "rhs4sg_matgrad: h must be positive");
+   const SbpTables* d = nullptr;
+   if ((rc = sbp_from(acof, bope, ghcof, &d))) return rc;
+   const Box b = make_box(ifirst, ilast, jfirst, jlast, kfirst, klast);
+   return launch_rhs4sg_matgrad(b, make_rowmap(b, nk, onesided), d, z, u, mu, lambda, h, strx, stry, strz, gmu,
+                                glambda, (hipStream_t)stream);
+}
+
+int sw4hip_addsgd4_adjoint(int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast, const double* y,
+                           const double* u, const double* um, const double* rho, const double* dcx, const double* dcy,
+                           const double* dcz, const double* strx, const double* stry, const double* strz,
+                           const double* cox, const double* coy, const double* coz, double beta, double* dbar,
+                           double* grho, void* stream)
+{
+   if (!box_ok(ifirst, ilast, jfirst, jlast, kfirst, klast)) return set_error(E_INVAL, "addsgd4_adjoint: box too small");
+   if (!y || !u || !um || !rho || !dcx || !dcy || !dcz || !strx || !stry || !strz || !cox || !coy || !coz || !dbar ||
+       !grho)
+      return set_error(E_INVAL, "addsgd4_adjoint: null argument");
+   SgCoef c{dcx, dcy, dcz, strx, stry, strz, cox, coy, coz};
+   return launch_addsgd4_T(make_box(ifirst, ilast, jfirst, jlast, kfirst, klast), y, u, um, rho, c, beta, dbar, grho,
+                           (hipStream_t)stream);
+}
+
+int sw4hip_bcfortsg_adjoint(int ib, int ie, int jb, int je, int kb, int ke, const int* wind, int nx, int ny, int nz,
+                            double* y, const double* fwd, double h, const int* bccnd, const double* sbop,
+                            const double* mu, const double* lambda, const double* strx, const double* stry,
+                            double* gmu, double* glambda, void* stream)
+{
+   int rc;
+   if ((rc = rows_ok(ib, ie, jb, je, kb, ke, nz, nullptr))) return rc;
+   if (ie != nx + 2 || je != ny + 2) return set_error(E_INVAL, "bcfortsg_adjoint: box must be [-1, n+2]");
+   if (!wind || !y || !fwd || !bccnd || !sbop || !mu || !lambda || !strx || !stry || !gmu || !glambda)
+      return set_error(E_INVAL, "bcfortsg_adjoint: null argument");
+   for (int s = 0; s < 6; s++) {
+      const int* w = wind + 6 * s;
+      if (bccnd[s] < 0 || bccnd[s] > 2 || w[1] < w[0] || w[3] < w[2] || w[5] < w[4]) continue;
+      if (w[0] < ib || w[1] > ie || w[2] < jb || w[3] > je || w[4] < kb || w[5] > ke)
+         return set_error(E_INVAL, "bcfortsg_adjoint: window outside the array");
+   }
+   SbpTables t = default_tables();
+   std::memcpy(t.sbop, sbop, sizeof(t.sbop));
+   const SbpTables* d = nullptr;
+   if ((rc = device_sbp(t, &d))) return rc;
+   const Box b = make_box(ib, ie, jb, je, kb, ke);
+   void* plane = nullptr;
+   if (hipMalloc(&plane, sizeof(double) * 3 * b.nij) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(E_NOMEM, "bcfortsg_adjoint: scratch allocation failed");
+   }
+   rc = launch_bcfortsg_T(b, wind, nx, ny, nz, bccnd, y, fwd, h, d, mu, lambda, strx, stry, gmu, glambda,
+                          static_cast<double*>(plane), (hipStream_t)stream);
+   const int rs = check_hip(hipStreamSynchronize((hipStream_t)stream), "bcfortsg_adjoint");
+   (void)hipFree(plane);
+   return rc ? rc : rs;
 }
 
 // ---------------------------------------------------------------------------
@@ -579,6 +695,19 @@ int sw4hip_ew_step_comp_timed(sw4hip_ew* h, int nsteps, double* stage_ms)
    if (nsteps < 0) return set_error(E_INVAL, "negative step count");
    if (!stage_ms) return set_error(E_INVAL, "null stage_ms");
    return h->ew->step_comp(nsteps, stage_ms);
+}
+
+int sw4hip_ew_reset(sw4hip_ew* h)
+{
+   EWCHK(h);
+   return h->ew->reset();
+}
+
+int sw4hip_ew_gradient(sw4hip_ew* h, int nsteps, int ckpt, const double* grad_traces, double* grho, double* gmu,
+                       double* glambda, int on_device)
+{
+   EWCHK(h);
+   return h->ew->gradient(nsteps, ckpt, grad_traces, grho, gmu, glambda, on_device != 0);
 }
 
 int sw4hip_ew_point_source_error(sw4hip_ew* h, double* linf, double* l2, double* solinf)

@@ -1,0 +1,284 @@
+// adjoint.cpp -- discrete adjoint of the FP64 time step: material gradients
+// of a receiver misfit by a reverse sweep over checkpointed forward states.
+//
+// One step (EW::timesteploop, EW.C:2527-2842) maps (U^n, U^{n-1}) to U^{n+1}:
+//   Up  = BC(2U - Um + dt^2/rho (L(U) + F))                     predictor
+//   A   = (Up - 2U + Um) / dt^2                                 dpdmt
+//   U^{n+1} = BC(Up + dt^4/(12 rho) (L(A) + F_tt) - D(U - Um))  corrector, damping
+// with L = rhs4sg on the rows (zero on the ghost points), D the supergrid
+// damping on the interior and BC = bcfortsg with zero boundary data.  The
+// reverse step applies the transposes of these maps in the opposite order
+// (kernels_adj.hip) and accumulates the derivatives of L, D, BC and the 1/rho
+// factors with respect to the material.  The reverse sweep recomputes the
+// forward states of one checkpoint segment at a time.
+#include <cmath>
+#include <vector>
+
+#include "ew.h"
+#include "kernels_adj.h"
+
+namespace sw4amd {
+
+int launch_fill(double* a, long n, double v, hipStream_t s);
+
+int EW::reset()
+{
+   if (!m_setup_done) return set_error(E_STATE, "EW: reset before setup");
+   int rc;
+   if ((rc = comp_end())) return rc;
+   m_comp = false;
+   if ((rc = launch_fill(mU.dev, mU.size(), 0.0, m_stream)) || (rc = launch_fill(mUm.dev, mUm.size(), 0.0, m_stream)))
+      return rc;
+   mT = 0;
+   mCycle = 0;
+   m_nsamples = 0;
+   return record(mU, mUm);
+}
+
+int EW::fused_once(double t)
+{
+   const GridSetup& g = m_opt.grid;
+   const SgCoef c = sg();
+   const SourceTable st = sources();
+   const double dt = mDt;
+   int rc;
+   if ((rc = launch_pred_fused(m_box, g.nz, m_onesided, m_dsbp, mU.dev, mUm.dev, mMu.dev, mLambda.dev, mRho.dev, c,
+                               mUp.dev, g.h, dt, m_stream, nullptr, m_variant, m_kchunk)) ||
+       (rc = launch_src_apply(m_box, st, t, 0, dt * dt, mRho.dev, mUp.dev, m_stream)) ||
+       (rc = enforceBC(mUp, mMu, mLambda, t + dt)) ||
+       (rc = launch_corr_fused(m_box, g.nz, m_onesided, m_dsbp, mUp.dev, mU.dev, mUm.dev, mMu.dev, mLambda.dev,
+                               mRho.dev, c, m_opt.beta, mUp2.dev, m_dacc, g.h, dt, m_stream, nullptr, nullptr, m_variant,
+                               m_kchunk)) ||
+       (rc = launch_src_apply(m_box, st, t, 1, dt * dt * dt * dt / 12, mRho.dev, mUp2.dev, m_stream)) ||
+       (rc = enforceBC(mUp2, mMu, mLambda, t + dt)))
+      return rc;
+   return OK;
+}
+
+// Reverse of step n: U = U^n, Um = U^{n-1}; lam_a (adjoint of U^{n+1}) is
+// consumed; lam_b += dJ/dU^n and lam_c += dJ/dU^{n-1} through this step.
+// tmp: 13 work fields of 3 components.
+int EW::step_vjp(int n, double* u, double* um, double* lam_a, double* lam_b, double* lam_c, double* grho,
+                 double* gmu, double* gla, double* const* tmp, double* plane)
+{
+   const GridSetup& g = m_opt.grid;
+   const SgCoef c = sg();
+   const double dt = mDt, t = n * dt, dt2 = dt * dt, c4 = dt2 * dt2 / 12;
+   const long n3 = 3 * m_box.npts;
+   const RowMap rm = make_rowmap(m_box, g.nz, m_onesided);
+   auto view = [&](double* p) {
+      Sarray a = mU;
+      a.dev = p;
+      return a;
+   };
+   Sarray U = view(u), Um = view(um);
+   Sarray F = view(tmp[0]), Lu = view(tmp[1]), Up = view(tmp[2]), Ftt = view(tmp[3]), Acc = view(tmp[4]),
+          Lu2 = view(tmp[5]), Up2 = view(tmp[6]);
+   double *Y = tmp[7], *Db = tmp[8], *Yp = tmp[9], *Z = tmp[10], *LT = tmp[11];
+   int rc;
+   // forward recompute in the reference stage order (rows of Lu only; ghosts 0)
+   if ((rc = Force(t, F, false)) || (rc = launch_zero(Lu.dev, n3, m_stream)) || (rc = evalRHS(U, mMu, mLambda, Lu)) ||
+       (rc = evalPredictor(Up, U, Um, mRho, Lu, F)) || (rc = enforceBC(Up, mMu, mLambda, t + dt)) ||
+       (rc = Force(t, Ftt, true)) || (rc = evalDpDmInTime(Up, U, Um, Acc)) ||
+       (rc = launch_zero(Lu2.dev, n3, m_stream)) || (rc = evalRHS(Acc, mMu, mLambda, Lu2)) ||
+       (rc = check_hip(hipMemcpyAsync(Up2.dev, Up.dev, sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream),
+                       "adjoint copy")) ||
+       (rc = evalCorrector(Up2, mRho, Lu2, Ftt)) || (rc = addSuperGridDamping(Up2, U, Um, mRho)) ||
+       (rc = enforceBC(Up2, mMu, mLambda, t + dt)))
+      return rc;
+   // corrector BC, transposed (acting on Up2)
+   if ((rc = check_hip(hipMemcpyAsync(Y, lam_a, sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream), "adjoint copy")) ||
+       (rc = launch_bcfortsg_T(m_box, m_wind, g.nx, g.ny, g.nz, g.bc, Y, Up2.dev, g.h, m_dsbp, mMu.dev, mLambda.dev,
+                               c.strx, c.stry, gmu, gla, plane, m_stream)))
+      return rc;
+   // damping: Up2 -= D(U - Um)
+   if ((rc = launch_addsgd4_T(m_box, Y, u, um, mRho.dev, c, m_opt.beta, Db, grho, m_stream)) ||
+       (rc = launch_axpy(n3, 1.0, Db, lam_b, m_stream)) || (rc = launch_axpy(n3, -1.0, Db, lam_c, m_stream)))
+      return rc;
+   // corrector: Up2 = Up + c4/rho (L(A) + F_tt)
+   if ((rc = check_hip(hipMemcpyAsync(Yp, Y, sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream), "adjoint copy")) ||
+       (rc = launch_rows_scale(m_box, c4, mRho.dev, Y, Z, m_stream)) ||
+       (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, LT, tmp[12], m_stream)) ||
+       (rc = launch_rhs4sg_matgrad(m_box, rm, m_dsbp, Z, Acc.dev, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz,
+                                   gmu, gla, m_stream)) ||
+       (rc = launch_rho_grad(m_box, c4, Y, Lu2.dev, Ftt.dev, mRho.dev, grho, m_stream)))
+      return rc;
+   // A = (Up - 2U + Um)/dt^2
+   const double i2 = 1.0 / dt2;
+   if ((rc = launch_axpy(n3, i2, LT, Yp, m_stream)) || (rc = launch_axpy(n3, -2 * i2, LT, lam_b, m_stream)) ||
+       (rc = launch_axpy(n3, i2, LT, lam_c, m_stream)))
+      return rc;
+   // predictor BC, transposed (acting on Up)
+   if ((rc = launch_bcfortsg_T(m_box, m_wind, g.nx, g.ny, g.nz, g.bc, Yp, Up.dev, g.h, m_dsbp, mMu.dev, mLambda.dev,
+                               c.strx, c.stry, gmu, gla, plane, m_stream)))
+      return rc;
+   // predictor: Up = 2U - Um + dt^2/rho (L(U) + F)
+   if ((rc = launch_axpy(n3, 2.0, Yp, lam_b, m_stream)) || (rc = launch_axpy(n3, -1.0, Yp, lam_c, m_stream)) ||
+       (rc = launch_rows_scale(m_box, dt2, mRho.dev, Yp, Z, m_stream)) ||
+       (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, LT, tmp[12], m_stream)) ||
+       (rc = launch_axpy(n3, 1.0, LT, lam_b, m_stream)) ||
+       (rc = launch_rhs4sg_matgrad(m_box, rm, m_dsbp, Z, u, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, gmu,
+                                   gla, m_stream)) ||
+       (rc = launch_rho_grad(m_box, dt2, Yp, Lu.dev, F.dev, mRho.dev, grho, m_stream)))
+      return rc;
+   return OK;
+}
+
+int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* gmu, double* gla, bool on_device)
+{
+   if (!m_setup_done) return set_error(E_STATE, "EW: gradient before setup");
+   if (!m_opt.corder) return set_error(E_INVAL, "EW: the adjoint needs the component-blocked layout (corder=1)");
+   if (m_curv) return set_error(E_INVAL, "EW: the adjoint is Cartesian");
+   const GridSetup& g = m_opt.grid;
+   for (int s = 0; s < 6; s++)
+      if (g.bc[s] == BC_PERIODIC) return set_error(E_INVAL, "EW: the adjoint does not support periodic sides");
+   const int nrec = (int)m_rec.size() / 4;
+   if (nsteps < 1 || nsteps + 1 > m_maxsamples || nrec == 0)
+      return set_error(E_INVAL, "EW: gradient needs receivers and 1 <= nsteps <= the set-up step count");
+   if (!gtr || !grho || !gmu || !gla) return set_error(E_INVAL, "EW: null gradient argument");
+   if (ckpt < 1) ckpt = std::max(1, (int)std::lround(std::sqrt((double)nsteps)));
+   const long n1 = m_box.npts, n3 = 3 * n1;
+   const int ncp = (nsteps + ckpt - 1) / ckpt;
+   // work memory: 13 fields, 3 adjoint fields, ckpt+1 segment states, 2 per checkpoint, 3 gradients
+   std::vector<void*> mine;
+   auto release = [&]() {
+      for (void* p : mine) (void)hipFree(p);
+      mine.clear();
+   };
+   auto grab = [&](size_t n, double** out) {
+      void* p = nullptr;
+      if (hipMalloc(&p, sizeof(double) * n) != hipSuccess) {
+         (void)hipGetLastError();
+         return set_error(E_NOMEM, "EW: adjoint work allocation failed");
+      }
+      mine.push_back(p);
+      *out = static_cast<double*>(p);
+      return (int)OK;
+   };
+   int rc = OK;
+   double* tmp[13];
+   double *lam[3], *gr[3], *plane = nullptr;
+   std::vector<double*> seg(ckpt + 1), cp(2 * ncp);
+   for (int q = 0; q < 13 && rc == OK; q++) rc = grab(n3, &tmp[q]);
+   for (int q = 0; q < 3 && rc == OK; q++) rc = grab(n3, &lam[q]);
+   for (int q = 0; q < 3 && rc == OK; q++) rc = grab(n1, &gr[q]);
+   for (int q = 0; q <= ckpt && rc == OK; q++) rc = grab(n3, &seg[q]);
+   for (int q = 0; q < 2 * ncp && rc == OK; q++) rc = grab(n3, &cp[q]);
+   if (rc == OK) rc = grab(3 * m_box.nij, &plane);
+   int* didx = nullptr;
+   if (rc == OK) {
+      void* p = nullptr;
+      if (hipMalloc(&p, sizeof(int) * 6 * nrec + sizeof(double) * 6 * nrec) != hipSuccess) {
+         (void)hipGetLastError();
+         rc = set_error(E_NOMEM, "EW: adjoint source allocation failed");
+      } else {
+         mine.push_back(p);
+         didx = static_cast<int*>(p);
+      }
+   }
+   if (rc) {
+      release();
+      return rc;
+   }
+   double* dval = reinterpret_cast<double*>(didx + 6 * nrec);
+   // direct adjoint source of state s: displacement receivers g[s], velocity
+   // receivers (U^s - U^{s-2}) / (2 dt): +g[s]/(2dt) on U^s, -g[s+2]/(2dt) on U^s
+   std::vector<int> hidx;
+   std::vector<double> hval;
+   auto add_direct = [&](int s, double* lamp) {
+      if (s < 1 || s > nsteps) return (int)OK;
+      hidx.clear();
+      hval.clear();
+      for (int r = 0; r < nrec; r++) {
+         const int i = m_rec[4 * r], j = m_rec[4 * r + 1], k = m_rec[4 * r + 2], mode = m_rec[4 * r + 3];
+         const long p = sidx(m_box, i, j, k);
+         for (int c = 0; c < 3; c++) {
+            double v;
+            if (mode == 0)
+               v = gtr[((size_t)s * nrec + r) * 3 + c];
+            else {
+               v = gtr[((size_t)s * nrec + r) * 3 + c] / (2 * mDt);
+               if (s + 2 <= nsteps) v -= gtr[((size_t)(s + 2) * nrec + r) * 3 + c] / (2 * mDt);
+            }
+            hidx.push_back((int)(p + n1 * c));
+            hval.push_back(v);
+         }
+      }
+      // component offsets fit an int for the grids this path is meant for
+      if (n3 > 0x7fffffff) return set_error(E_RANGE, "EW: adjoint source index beyond 32 bits");
+      int e;
+      if ((e = check_hip(hipMemcpyAsync(didx, hidx.data(), sizeof(int) * hidx.size(), hipMemcpyHostToDevice, m_stream),
+                         "adjoint source upload")) ||
+          (e = check_hip(hipMemcpyAsync(dval, hval.data(), sizeof(double) * hval.size(), hipMemcpyHostToDevice, m_stream),
+                         "adjoint source upload")) ||
+          (e = launch_scatter_add((int)hidx.size(), didx, dval, lamp, m_stream)) ||
+          (e = check_hip(hipStreamSynchronize(m_stream), "adjoint source upload")))
+         return e;
+      return (int)OK;
+   };
+   // forward pass with checkpoints (U^{n}, U^{n-1}) at n = 0, ckpt, 2 ckpt, ...
+   if ((rc = reset())) {
+      release();
+      return rc;
+   }
+   for (int n = 0; n < nsteps && rc == OK; n++) {
+      if (n % ckpt == 0) {
+         const int q = n / ckpt;
+         if ((rc = check_hip(hipMemcpyAsync(cp[2 * q], mU.dev, sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream),
+                             "checkpoint")) ||
+             (rc = check_hip(hipMemcpyAsync(cp[2 * q + 1], mUm.dev, sizeof(double) * n3, hipMemcpyDeviceToDevice,
+                                            m_stream), "checkpoint")))
+            break;
+      }
+      rc = step_fused(1);
+   }
+   // reverse sweep
+   for (int q = 0; q < 3 && rc == OK; q++) rc = launch_zero(gr[q], n1, m_stream);
+   for (int q = 0; q < 3 && rc == OK; q++) rc = launch_zero(lam[q], n3, m_stream);
+   double *la = lam[0], *lb = lam[1], *lc = lam[2];
+   if (rc == OK) rc = add_direct(nsteps, la);
+   if (rc == OK) rc = add_direct(nsteps - 1, lb);
+   if (rc == OK) rc = add_direct(nsteps - 2, lc);
+   // the segment recompute runs the fused step on the EW buffers; keep them
+   double *sU = mU.dev, *sUm = mUm.dev, *sUp = mUp.dev, *sUp2 = mUp2.dev;
+   for (int cq = ncp - 1; cq >= 0 && rc == OK; cq--) {
+      const int s0 = cq * ckpt, s1 = std::min(nsteps, s0 + ckpt);
+      // seg[m] = U^{s0-1+m}, m = 0 .. s1-s0
+      if ((rc = check_hip(hipMemcpyAsync(seg[0], cp[2 * cq + 1], sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream),
+                          "checkpoint restore")) ||
+          (rc = check_hip(hipMemcpyAsync(seg[1], cp[2 * cq], sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream),
+                          "checkpoint restore")))
+         break;
+      for (int m = 1; m < s1 - s0 && rc == OK; m++) {
+         mU.dev = seg[m];
+         mUm.dev = seg[m - 1];
+         mUp.dev = tmp[0];
+         mUp2.dev = seg[m + 1];
+         rc = fused_once((s0 + m - 1) * mDt);
+      }
+      mU.dev = sU;
+      mUm.dev = sUm;
+      mUp.dev = sUp;
+      mUp2.dev = sUp2;
+      for (int n = s1 - 1; n >= s0 && rc == OK; n--) {
+         rc = step_vjp(n, seg[n - s0 + 1], seg[n - s0], la, lb, lc, gr[0], gr[1], gr[2], tmp, plane);
+         if (rc) break;
+         // rotate: U^{n} becomes the complete adjoint, U^{n-2} starts
+         double* old = la;
+         la = lb;
+         lb = lc;
+         lc = old;
+         if ((rc = launch_zero(lc, n3, m_stream))) break;
+         rc = add_direct(n - 2, lc);
+      }
+   }
+   const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+   double* outs[3] = {grho, gmu, gla};
+   for (int q = 0; q < 3 && rc == OK; q++)
+      rc = check_hip(hipMemcpyAsync(outs[q], gr[q], sizeof(double) * n1, kind, m_stream), "gradient copy");
+   const int rs = check_hip(hipStreamSynchronize(m_stream), "gradient synchronize");
+   release();
+   return rc ? rc : rs;
+}
+
+}  // namespace sw4amd

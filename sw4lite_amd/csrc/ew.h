@@ -85,6 +85,16 @@ public:
    bool comp_active() const { return m_comp; }
    int point_source_error(double& linf, double& l2, double& solinf);
 
+   // --- discrete adjoint of the FP64 step (adjoint.cpp) ---
+   // Zero initial data at t = 0, receivers restarted with the t = 0 sample.
+   int reset();
+   // Gradient of a receiver misfit with respect to rho, mu, lambda (whole
+   // box): forward nsteps fused steps from zero data with checkpoints every
+   // ckpt steps, then the reverse sweep.  gtr: host array
+   // [nsteps+1][nreceivers][3], the misfit's derivative with respect to each
+   // recorded sample.  Outputs are overwritten (device pointers if on_device).
+   int gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* gmu, double* gla, bool on_device);
+
    // receivers (TimeSeries, EW.C:5854-5875): mode 0 displacement, 1 velocity
    int add_receiver(double x, double y, double z, int mode, int* index);
    int record(const Sarray& Up, const Sarray& Um);  // one sample, after a step
@@ -119,6 +129,9 @@ private:
    int ensure_reference_arrays();
    SgCoef sg() const;
    SourceTable sources() const;
+   int fused_once(double t);  // one fused step U, Um -> Up2 (scratch Up), no rotation
+   int step_vjp(int n, double* u, double* um, double* lam_a, double* lam_b, double* lam_c, double* grho,
+                double* gmu, double* gla, double* const* tmp, double* plane);
    int comp_begin();
    CompState comp_state() const;
    SgCoefF sgf() const;

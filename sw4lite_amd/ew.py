@@ -163,6 +163,44 @@ def bcfortsg(corder, box, wind, nx, ny, nz, u, h, bccnd, sbop, mu, la, bforce, s
                                      _ptr(la), C.cast(bf, C.c_void_p), _ptr(strx), _ptr(stry), _stream(stream)))
 
 
+# transposed kernels of the discrete adjoint (component-blocked layout, box [-1, n+2])
+def rhs4sg_adjoint(box, nk, onesided, acof, bope, ghcof, out, z, mu, la, h, strx, stry, strz, stream=None):
+    """out = L^T z on every point (z read on the rows of rhs4sg)."""
+    os_ = np.ascontiguousarray(onesided, dtype=np.int32)
+    L.check(L.load().sw4hip_rhs4sg_adjoint(*box, nk, _ptr(os_), _ptr(np.ascontiguousarray(acof)),
+                                           _ptr(np.ascontiguousarray(bope)), _ptr(np.ascontiguousarray(ghcof)),
+                                           _ptr(out), _ptr(z), _ptr(mu), _ptr(la), h, _ptr(strx), _ptr(stry),
+                                           _ptr(strz), _stream(stream)))
+
+
+def rhs4sg_matgrad(box, nk, onesided, acof, bope, ghcof, z, u, mu, la, h, strx, stry, strz, gmu, gla, stream=None):
+    """gmu += d/dmu (z . L(u)), gla += d/dlambda (z . L(u))."""
+    os_ = np.ascontiguousarray(onesided, dtype=np.int32)
+    L.check(L.load().sw4hip_rhs4sg_matgrad(*box, nk, _ptr(os_), _ptr(np.ascontiguousarray(acof)),
+                                           _ptr(np.ascontiguousarray(bope)), _ptr(np.ascontiguousarray(ghcof)),
+                                           _ptr(z), _ptr(u), _ptr(mu), _ptr(la), h, _ptr(strx), _ptr(stry),
+                                           _ptr(strz), _ptr(gmu), _ptr(gla), _stream(stream)))
+
+
+def addsgd4_adjoint(box, y, u, um, rho, dcx, dcy, dcz, strx, stry, strz, cox, coy, coz, beta, dbar, grho,
+                    stream=None):
+    """For up -= D(u - um): dbar = -D^T y, grho += d/drho (-y . D)."""
+    L.check(L.load().sw4hip_addsgd4_adjoint(*box, _ptr(y), _ptr(u), _ptr(um), _ptr(rho), _ptr(dcx), _ptr(dcy),
+                                            _ptr(dcz), _ptr(strx), _ptr(stry), _ptr(strz), _ptr(cox), _ptr(coy),
+                                            _ptr(coz), beta, _ptr(dbar), _ptr(grho), _stream(stream)))
+
+
+def bcfortsg_adjoint(box, wind, nx, ny, nz, y, fwd, h, bccnd, sbop, mu, la, strx, stry, gmu, gla, stream=None):
+    """y <- BC^T y (zero boundary data); gmu, gla += the free-surface closure's
+    derivatives evaluated on fwd."""
+    w = np.ascontiguousarray(wind, dtype=np.int32)
+    bc = np.ascontiguousarray(bccnd, dtype=np.int32)
+    sb = np.ascontiguousarray(sbop, dtype=np.float64)
+    L.check(L.load().sw4hip_bcfortsg_adjoint(*box, _ptr(w), nx, ny, nz, _ptr(y), _ptr(fwd), h, _ptr(bc), _ptr(sb),
+                                             _ptr(mu), _ptr(la), _ptr(strx), _ptr(stry), _ptr(gmu), _ptr(gla),
+                                             _stream(stream)))
+
+
 # --------------------------------------------------------------------------
 # EW object
 # --------------------------------------------------------------------------
@@ -292,6 +330,30 @@ class EW:
 
     def sync(self):
         L.check(self.lib.sw4hip_ew_sync(self.h))
+
+    # discrete adjoint ------------------------------------------------------
+    def reset(self):
+        """Zero initial data at t = 0; receivers restart with the t = 0 sample."""
+        L.check(self.lib.sw4hip_ew_reset(self.h))
+
+    def gradient(self, grad_traces, nsteps=None, ckpt=0, out=None):
+        """Gradient of a receiver misfit with respect to (rho, mu, lambda) by
+        the discrete adjoint of the fused FP64 step.  grad_traces: the
+        misfit's derivative with respect to every recorded sample, array
+        [nsteps+1, nreceivers, 3].  Runs the forward from zero data itself.
+        out: three device tensors to fill (else numpy arrays are returned)."""
+        g = np.ascontiguousarray(np.asarray(grad_traces, dtype=np.float64))
+        if nsteps is None:
+            nsteps = g.shape[0] - 1
+        if out is None:
+            n = int(np.prod(self.shape))
+            res = [np.empty(n), np.empty(n), np.empty(n)]
+            on_dev = 0
+        else:
+            res, on_dev = list(out), 1
+        L.check(self.lib.sw4hip_ew_gradient(self.h, int(nsteps), int(ckpt), _ptr(g), _ptr(res[0]), _ptr(res[1]),
+                                            _ptr(res[2]), on_dev))
+        return tuple(res)
 
     def point_source_error(self):
         li, l2, xi = C.c_double(), C.c_double(), C.c_double()

@@ -43,6 +43,12 @@ _SIGS = {
     "sw4hip_dpdmtfort": (I, [I] * 6 + [P] * 4 + [D, P]),
     "sw4hip_addsgd4": (I, [I] * 7 + [P] * 13 + [D, P]),
     "sw4hip_bcfortsg": (I, [I] * 7 + [P, I, I, I, P, D, P, P, P, P, P, P, P, P]),
+    "sw4hip_rhs4sg_adjoint": (I, [I] * 7 + [P] * 8 + [D, P, P, P, P]),
+    "sw4hip_rhs4sg_matgrad": (I, [I] * 7 + [P] * 8 + [D, P, P, P, P, P, P]),
+    "sw4hip_addsgd4_adjoint": (I, [I] * 6 + [P] * 13 + [D, P, P, P]),
+    "sw4hip_bcfortsg_adjoint": (I, [I] * 6 + [P, I, I, I, P, P, D, P, P, P, P, P, P, P, P, P]),
+    "sw4hip_ew_reset": (I, [P]),
+    "sw4hip_ew_gradient": (I, [P, I, I, P, P, P, P, I]),
     "sw4hip_host_moment_source_curv": (I, [C.POINTER(GridDesc), P, D, D, D, P, I, IP, P, P]),
     "sw4hip_ew_set_topography": (I, [P, P, I]),
     "sw4hip_metric": (I, [I] * 7 + [P] * 5 + [P]),
