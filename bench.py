@@ -67,16 +67,18 @@ def parse():
     p.add_argument("--kchunk", type=int, default=64, help="planes per workgroup of the LDS sweeps")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also launch a plain device copy of 1 GiB (PMC scaling)")
-    p.add_argument("--workload", choices=["cartesian", "curvilinear", "precision"], default="cartesian",
+    p.add_argument("--workload", choices=["cartesian", "curvilinear", "precision", "adjoint"], default="cartesian",
                    help="cartesian: the full time step at N^3 (default); curvilinear: the curvilinear-grid "
                         "kernels (rhs4sgcurv, addsgd4c, free-surface closure) on a 384x384x256 grid under "
                         "Gaussian hills; precision: FP32 compensated against FP64 on 64 surface receivers "
-                        "(1024^3, h=10 m, 800 steps)")
+                        "(1024^3, h=10 m, 800 steps); adjoint: gradient of a receiver misfit against the forward run "
+                        "(256^3, h=25 m, 600 steps, 128 surface receivers)")
+    p.add_argument("--ckpt", type=int, default=50, help="adjoint workload: steps between checkpoints")
     a = p.parse_args()
     if a.n is None:
-        a.n = 1024 if a.workload == "precision" else 512
+        a.n = {"precision": 1024, "adjoint": 256}.get(a.workload, 512)
     if a.steps is None:
-        a.steps = 800 if a.workload == "precision" else 100
+        a.steps = {"precision": 800, "adjoint": 600}.get(a.workload, 100)
     return a
 
 
@@ -460,6 +462,65 @@ def main_precision(a, torch):
     print(json.dumps(line), flush=True)
 
 
+def main_adjoint(a, torch):
+    """One shot of the batched-adjoint workload: n^3 grid at h = 25 m with the
+    workload material, a shallow double couple (Ricker 5 Hz), 128 seeded
+    surface receivers, `steps` steps.  Times the forward run (fused FP64
+    steps recording the receivers) and the gradient of J = 1/2 |d - d_obs|^2
+    with respect to rho, mu, lambda (sw4hip_ew_gradient: forward with
+    checkpoints every --ckpt steps, reverse sweep recomputing one segment at
+    a time).  value = gradient seconds / forward seconds."""
+    from sw4lite_amd import EW
+    from sw4lite_amd import lib as L
+    n, h, steps = a.n, 25.0, a.steps
+    span = (n - 1) * h
+    rng = np.random.default_rng(6)
+    rho, mu, la = make_material(torch, n, h, seed=6)
+    torch.cuda.empty_cache()
+    e = EW(n, n, n, h, [L.BC_SUPERGRID] * 4 + [L.BC_FREE, L.BC_SUPERGRID], corder=1, sg_gp_thickness=30)
+    e.set_material(rho, mu, la)
+    e.sync()
+    e.add_moment_source(0.5 * span, 0.5 * span, 500.0, double_couple(rng, 1e15), 0.24, 5.0, 0)
+    for _ in range(128):
+        e.add_receiver(rng.uniform(0.15 * span, 0.85 * span), rng.uniform(0.15 * span, 0.85 * span), 0.0, 0)
+    e.setup(nsteps=steps)
+    e.reset()
+    e.timesteploop(min(5, steps), mode=0)
+    e.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.reset()
+    e.timesteploop(steps, mode=0)
+    e.sync()
+    t_fwd = time.perf_counter() - t0
+    tr = e.receivers()
+    dobs = 0.9 * tr
+    out = [torch.empty_like(rho), torch.empty_like(mu), torch.empty_like(la)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.gradient(tr - dobs, steps, ckpt=a.ckpt, out=out)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    gn = [float(o.abs().max()) for o in out]
+    e.close()
+    pts = float(n) ** 3
+    line = {
+        "metric": "adjoint/forward cost ratio (gradient of a receiver misfit w.r.t. rho, mu, lambda)",
+        "value": t_grad / t_fwd, "unit": "ratio", "n_gpus": 1, "steps": steps, "warmup": min(5, steps),
+        "ms_per_step": 1e3 * t_grad / steps, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {
+            "workload": "%d^3 Cartesian FP64 step, h=%g m, layered+lognormal material, free top, supergrid 30, "
+                        "one double couple at 500 m (Ricker 5 Hz), 128 surface receivers, one shot" % (n, h),
+            "grid": [n, n, n], "h": h, "checkpoint_every": a.ckpt,
+            "forward_s": t_fwd, "gradient_s": t_grad,
+            "forward_grid_pt_updates_per_s": pts * steps / t_fwd,
+            "max_abs_gradient": {"rho": gn[0], "mu": gn[1], "lambda": gn[2]},
+        },
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -475,6 +536,8 @@ def main():
         return main_curvilinear(a, torch)
     if a.workload == "precision":
         return main_precision(a, torch)
+    if a.workload == "adjoint":
+        return main_adjoint(a, torch)
     n, h = a.n, 20.0
     srcs = sources(n, h, seed=2)
     rho, mu, la = make_material(torch, n, h, seed=2)

@@ -313,12 +313,15 @@ int sw4hip_rhs4sg_adjoint(int ifirst, int ilast, int jfirst, int jlast, int kfir
    if ((rc = sbp_from(acof, bope, ghcof, &d))) return rc;
    const Box b = make_box(ifirst, ilast, jfirst, jlast, kfirst, klast);
    void* work = nullptr;
-   if (hipMalloc(&work, sizeof(double) * 3 * b.npts) != hipSuccess) {
+   if (hipMalloc(&work, sizeof(double) * adj_pad_doubles(b)) != hipSuccess) {
       (void)hipGetLastError();
       return set_error(E_NOMEM, "rhs4sg_adjoint: scratch allocation failed");
    }
-   rc = launch_rhs4sg_T(b, make_rowmap(b, nk, onesided), d, z, mu, lambda, h, strx, stry, strz, out,
-                        static_cast<double*>(work), (hipStream_t)stream);
+   const AdjPad pad = make_adj_pad(b, static_cast<double*>(work));
+   rc = launch_adj_pad_material(b, mu, lambda, strx, stry, strz, pad, (hipStream_t)stream);
+   if (rc == OK)
+      rc = launch_rhs4sg_T(b, make_rowmap(b, nk, onesided), d, z, mu, lambda, h, strx, stry, strz, out, pad,
+                           (hipStream_t)stream);
    const int rs = check_hip(hipStreamSynchronize((hipStream_t)stream), "rhs4sg_adjoint");
    (void)hipFree(work);
    return rc ? rc : rs;
@@ -339,8 +342,17 @@ int sw4hip_rhs4sg_matgrad(int ifirst, int ilast, int jfirst, int jlast, int kfir
    const SbpTables* d = nullptr;
    if ((rc = sbp_from(acof, bope, ghcof, &d))) return rc;
    const Box b = make_box(ifirst, ilast, jfirst, jlast, kfirst, klast);
-   return launch_rhs4sg_matgrad(b, make_rowmap(b, nk, onesided), d, z, u, mu, lambda, h, strx, stry, strz, gmu,
-                                glambda, (hipStream_t)stream);
+   void* work = nullptr;
+   if (hipMalloc(&work, sizeof(double) * adj_pad_doubles(b)) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(E_NOMEM, "rhs4sg_matgrad: scratch allocation failed");
+   }
+   const AdjPad pad = make_adj_pad(b, static_cast<double*>(work));
+   rc = launch_rhs4sg_matgrad(b, make_rowmap(b, nk, onesided), d, z, u, mu, lambda, h, strx, stry, strz, gmu, glambda,
+                              pad, (hipStream_t)stream);
+   const int rs = check_hip(hipStreamSynchronize((hipStream_t)stream), "rhs4sg_matgrad");
+   (void)hipFree(work);
+   return rc ? rc : rs;
 }
 
 int sw4hip_addsgd4_adjoint(int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast, const double* y,

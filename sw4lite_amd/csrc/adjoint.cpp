@@ -55,53 +55,50 @@ int EW::fused_once(double t)
    return OK;
 }
 
-// Reverse of step n: U = U^n, Um = U^{n-1}; lam_a (adjoint of U^{n+1}) is
-// consumed; lam_b += dJ/dU^n and lam_c += dJ/dU^{n-1} through this step.
-// tmp: 13 work fields of 3 components.
-int EW::step_vjp(int n, double* u, double* um, double* lam_a, double* lam_b, double* lam_c, double* grho,
-                 double* gmu, double* gla, double* const* tmp, double* plane)
+// Reverse of step n: U = U^n, Um = U^{n-1}, U1 = U^{n+1}; lam_a (adjoint of
+// U^{n+1}) is consumed; lam_b += dJ/dU^n and lam_c += dJ/dU^{n-1} through
+// this step.  Of the forward step only Up is recomputed (fused predictor,
+// sources, BC).  The 1/rho derivatives use state differences on the
+// interior, where every grid-point source lies:
+//   dt^2/rho (L(U) + F)        = Up - 2U + Um = dt^2 A,
+//   c4/rho (L(A) + F_tt) - D   = U^{n+1} - Up,
+// so d/drho [y . c4/rho (...)] + d/drho [-y . D]|_(1/rho) = -y . (U^{n+1} - Up)/rho.
+// tmp: 7 work fields of 3 components; pad: padded material (kernels_adj.h).
+int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double* lam_b, double* lam_c,
+                 double* grho, double* gmu, double* gla, double* const* tmp, double* plane, const AdjPad& pad)
 {
    const GridSetup& g = m_opt.grid;
    const SgCoef c = sg();
+   const SourceTable st = sources();
    const double dt = mDt, t = n * dt, dt2 = dt * dt, c4 = dt2 * dt2 / 12;
    const long n3 = 3 * m_box.npts;
    const RowMap rm = make_rowmap(m_box, g.nz, m_onesided);
-   auto view = [&](double* p) {
-      Sarray a = mU;
-      a.dev = p;
-      return a;
-   };
-   Sarray U = view(u), Um = view(um);
-   Sarray F = view(tmp[0]), Lu = view(tmp[1]), Up = view(tmp[2]), Ftt = view(tmp[3]), Acc = view(tmp[4]),
-          Lu2 = view(tmp[5]), Up2 = view(tmp[6]);
-   double *Y = tmp[7], *Db = tmp[8], *Yp = tmp[9], *Z = tmp[10], *LT = tmp[11];
+   double *Up = tmp[0], *Acc = tmp[1], *Y = tmp[2], *Db = tmp[3], *Yp = tmp[4], *Z = tmp[5], *LT = tmp[6];
+   Sarray UpA = mU;
+   UpA.dev = Up;
    int rc;
-   // forward recompute in the reference stage order (rows of Lu only; ghosts 0)
-   if ((rc = Force(t, F, false)) || (rc = launch_zero(Lu.dev, n3, m_stream)) || (rc = evalRHS(U, mMu, mLambda, Lu)) ||
-       (rc = evalPredictor(Up, U, Um, mRho, Lu, F)) || (rc = enforceBC(Up, mMu, mLambda, t + dt)) ||
-       (rc = Force(t, Ftt, true)) || (rc = evalDpDmInTime(Up, U, Um, Acc)) ||
-       (rc = launch_zero(Lu2.dev, n3, m_stream)) || (rc = evalRHS(Acc, mMu, mLambda, Lu2)) ||
-       (rc = check_hip(hipMemcpyAsync(Up2.dev, Up.dev, sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream),
-                       "adjoint copy")) ||
-       (rc = evalCorrector(Up2, mRho, Lu2, Ftt)) || (rc = addSuperGridDamping(Up2, U, Um, mRho)) ||
-       (rc = enforceBC(Up2, mMu, mLambda, t + dt)))
+   // forward: Up = BC(2U - Um + dt^2/rho (L(U) + F)), A = (Up - 2U + Um)/dt^2
+   if ((rc = launch_pred_fused(m_box, g.nz, m_onesided, m_dsbp, u, um, mMu.dev, mLambda.dev, mRho.dev, c, Up, g.h, dt,
+                               m_stream, nullptr, m_variant, m_kchunk)) ||
+       (rc = launch_src_apply(m_box, st, t, 0, dt2, mRho.dev, Up, m_stream)) ||
+       (rc = enforceBC(UpA, mMu, mLambda, t + dt)) || (rc = launch_dpdmt(m_box, Up, u, um, Acc, 1.0 / dt2, m_stream)))
       return rc;
-   // corrector BC, transposed (acting on Up2)
+   // corrector BC, transposed (acting on U^{n+1})
    if ((rc = check_hip(hipMemcpyAsync(Y, lam_a, sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream), "adjoint copy")) ||
-       (rc = launch_bcfortsg_T(m_box, m_wind, g.nx, g.ny, g.nz, g.bc, Y, Up2.dev, g.h, m_dsbp, mMu.dev, mLambda.dev,
-                               c.strx, c.stry, gmu, gla, plane, m_stream)))
+       (rc = launch_bcfortsg_T(m_box, m_wind, g.nx, g.ny, g.nz, g.bc, Y, u1, g.h, m_dsbp, mMu.dev, mLambda.dev, c.strx,
+                               c.stry, gmu, gla, plane, m_stream)))
       return rc;
-   // damping: Up2 -= D(U - Um)
-   if ((rc = launch_addsgd4_T(m_box, Y, u, um, mRho.dev, c, m_opt.beta, Db, grho, m_stream)) ||
+   // damping: U^{n+1} -= D(U - Um)
+   if ((rc = launch_addsgd4_T_pad(m_box, Y, u, um, mRho.dev, c, m_opt.beta, Db, grho, pad, m_stream, false)) ||
        (rc = launch_axpy(n3, 1.0, Db, lam_b, m_stream)) || (rc = launch_axpy(n3, -1.0, Db, lam_c, m_stream)))
       return rc;
-   // corrector: Up2 = Up + c4/rho (L(A) + F_tt)
+   // corrector: Up + c4/rho (L(A) + F_tt)
    if ((rc = check_hip(hipMemcpyAsync(Yp, Y, sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream), "adjoint copy")) ||
        (rc = launch_rows_scale(m_box, c4, mRho.dev, Y, Z, m_stream)) ||
-       (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, LT, tmp[12], m_stream)) ||
-       (rc = launch_rhs4sg_matgrad(m_box, rm, m_dsbp, Z, Acc.dev, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz,
-                                   gmu, gla, m_stream)) ||
-       (rc = launch_rho_grad(m_box, c4, Y, Lu2.dev, Ftt.dev, mRho.dev, grho, m_stream)))
+       (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, LT, pad, m_stream)) ||
+       (rc = launch_rhs4sg_matgrad(m_box, rm, m_dsbp, Z, Acc, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, gmu,
+                                   gla, pad, m_stream)) ||
+       (rc = launch_rho_diff(m_box, 1.0, Y, u1, Up, mRho.dev, grho, m_stream)))
       return rc;
    // A = (Up - 2U + Um)/dt^2
    const double i2 = 1.0 / dt2;
@@ -109,17 +106,17 @@ int EW::step_vjp(int n, double* u, double* um, double* lam_a, double* lam_b, dou
        (rc = launch_axpy(n3, i2, LT, lam_c, m_stream)))
       return rc;
    // predictor BC, transposed (acting on Up)
-   if ((rc = launch_bcfortsg_T(m_box, m_wind, g.nx, g.ny, g.nz, g.bc, Yp, Up.dev, g.h, m_dsbp, mMu.dev, mLambda.dev,
-                               c.strx, c.stry, gmu, gla, plane, m_stream)))
+   if ((rc = launch_bcfortsg_T(m_box, m_wind, g.nx, g.ny, g.nz, g.bc, Yp, Up, g.h, m_dsbp, mMu.dev, mLambda.dev, c.strx,
+                               c.stry, gmu, gla, plane, m_stream)))
       return rc;
-   // predictor: Up = 2U - Um + dt^2/rho (L(U) + F)
+   // predictor: 2U - Um + dt^2/rho (L(U) + F)
    if ((rc = launch_axpy(n3, 2.0, Yp, lam_b, m_stream)) || (rc = launch_axpy(n3, -1.0, Yp, lam_c, m_stream)) ||
        (rc = launch_rows_scale(m_box, dt2, mRho.dev, Yp, Z, m_stream)) ||
-       (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, LT, tmp[12], m_stream)) ||
+       (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, LT, pad, m_stream)) ||
        (rc = launch_axpy(n3, 1.0, LT, lam_b, m_stream)) ||
        (rc = launch_rhs4sg_matgrad(m_box, rm, m_dsbp, Z, u, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, gmu,
-                                   gla, m_stream)) ||
-       (rc = launch_rho_grad(m_box, dt2, Yp, Lu.dev, F.dev, mRho.dev, grho, m_stream)))
+                                   gla, pad, m_stream)) ||
+       (rc = launch_rho_diff(m_box, dt2, Yp, Acc, nullptr, mRho.dev, grho, m_stream)))
       return rc;
    return OK;
 }
@@ -139,7 +136,7 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
    if (ckpt < 1) ckpt = std::max(1, (int)std::lround(std::sqrt((double)nsteps)));
    const long n1 = m_box.npts, n3 = 3 * n1;
    const int ncp = (nsteps + ckpt - 1) / ckpt;
-   // work memory: 13 fields, 3 adjoint fields, ckpt+1 segment states, 2 per checkpoint, 3 gradients
+   // work memory: 7 fields, 3 adjoint fields, ckpt+2 segment states, 2 per checkpoint, 3 gradients, pad
    std::vector<void*> mine;
    auto release = [&]() {
       for (void* p : mine) (void)hipFree(p);
@@ -156,13 +153,15 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
       return (int)OK;
    };
    int rc = OK;
-   double* tmp[13];
+   double* tmp[7];
    double *lam[3], *gr[3], *plane = nullptr;
-   std::vector<double*> seg(ckpt + 1), cp(2 * ncp);
-   for (int q = 0; q < 13 && rc == OK; q++) rc = grab(n3, &tmp[q]);
+   std::vector<double*> seg(ckpt + 2), cp(2 * ncp);
+   for (int q = 0; q < 7 && rc == OK; q++) rc = grab(n3, &tmp[q]);
+   double* padmem = nullptr;
+   if (rc == OK) rc = grab(adj_pad_doubles(m_box), &padmem);
    for (int q = 0; q < 3 && rc == OK; q++) rc = grab(n3, &lam[q]);
    for (int q = 0; q < 3 && rc == OK; q++) rc = grab(n1, &gr[q]);
-   for (int q = 0; q <= ckpt && rc == OK; q++) rc = grab(n3, &seg[q]);
+   for (int q = 0; q <= ckpt + 1 && rc == OK; q++) rc = grab(n3, &seg[q]);
    for (int q = 0; q < 2 * ncp && rc == OK; q++) rc = grab(n3, &cp[q]);
    if (rc == OK) rc = grab(3 * m_box.nij, &plane);
    int* didx = nullptr;
@@ -216,52 +215,64 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
          return e;
       return (int)OK;
    };
-   // forward pass with checkpoints (U^{n}, U^{n-1}) at n = 0, ckpt, 2 ckpt, ...
+   // forward pass: segment by segment into seg (seg[m] = U^{s0-1+m}), saving
+   // the checkpoint (U^{s0}, U^{s0-1}) of each; seg ends up holding the last
+   // segment, which the reverse sweep then uses without recomputing it
+   double *sU = mU.dev, *sUm = mUm.dev, *sUp = mUp.dev, *sUp2 = mUp2.dev;
+   auto fill_segment = [&](int cq) {
+      const int s0 = cq * ckpt, s1 = std::min(nsteps, s0 + ckpt);
+      int e;
+      if ((e = check_hip(hipMemcpyAsync(seg[0], cp[2 * cq + 1], sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream),
+                         "checkpoint restore")) ||
+          (e = check_hip(hipMemcpyAsync(seg[1], cp[2 * cq], sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream),
+                         "checkpoint restore")))
+         return e;
+      for (int m = 1; m <= s1 - s0 && e == OK; m++) {
+         mU.dev = seg[m];
+         mUm.dev = seg[m - 1];
+         mUp.dev = tmp[0];
+         mUp2.dev = seg[m + 1];
+         e = fused_once((s0 + m - 1) * mDt);
+      }
+      mU.dev = sU;
+      mUm.dev = sUm;
+      mUp.dev = sUp;
+      mUp2.dev = sUp2;
+      return e;
+   };
    if ((rc = reset())) {
       release();
       return rc;
    }
-   for (int n = 0; n < nsteps && rc == OK; n++) {
-      if (n % ckpt == 0) {
-         const int q = n / ckpt;
-         if ((rc = check_hip(hipMemcpyAsync(cp[2 * q], mU.dev, sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream),
-                             "checkpoint")) ||
-             (rc = check_hip(hipMemcpyAsync(cp[2 * q + 1], mUm.dev, sizeof(double) * n3, hipMemcpyDeviceToDevice,
+   rc = check_hip(hipMemcpyAsync(cp[0], mU.dev, sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream), "checkpoint");
+   if (rc == OK)
+      rc = check_hip(hipMemcpyAsync(cp[1], mUm.dev, sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream), "checkpoint");
+   for (int cq = 0; cq < ncp && rc == OK; cq++) {
+      if ((rc = fill_segment(cq))) break;
+      if (cq + 1 < ncp) {
+         const int len = std::min(nsteps, (cq + 1) * ckpt) - cq * ckpt;  // U^{s1} is seg[len+1]
+         if ((rc = check_hip(hipMemcpyAsync(cp[2 * cq + 2], seg[len + 1], sizeof(double) * n3, hipMemcpyDeviceToDevice,
+                                            m_stream), "checkpoint")) ||
+             (rc = check_hip(hipMemcpyAsync(cp[2 * cq + 3], seg[len], sizeof(double) * n3, hipMemcpyDeviceToDevice,
                                             m_stream), "checkpoint")))
             break;
       }
-      rc = step_fused(1);
    }
    // reverse sweep
+   const SgCoef sgc = sg();
+   const AdjPad pad = make_adj_pad(m_box, padmem);
+   if (rc == OK) rc = launch_adj_pad_material(m_box, mMu.dev, mLambda.dev, sgc.strx, sgc.stry, sgc.strz, pad, m_stream);
    for (int q = 0; q < 3 && rc == OK; q++) rc = launch_zero(gr[q], n1, m_stream);
    for (int q = 0; q < 3 && rc == OK; q++) rc = launch_zero(lam[q], n3, m_stream);
    double *la = lam[0], *lb = lam[1], *lc = lam[2];
    if (rc == OK) rc = add_direct(nsteps, la);
    if (rc == OK) rc = add_direct(nsteps - 1, lb);
    if (rc == OK) rc = add_direct(nsteps - 2, lc);
-   // the segment recompute runs the fused step on the EW buffers; keep them
-   double *sU = mU.dev, *sUm = mUm.dev, *sUp = mUp.dev, *sUp2 = mUp2.dev;
    for (int cq = ncp - 1; cq >= 0 && rc == OK; cq--) {
       const int s0 = cq * ckpt, s1 = std::min(nsteps, s0 + ckpt);
-      // seg[m] = U^{s0-1+m}, m = 0 .. s1-s0
-      if ((rc = check_hip(hipMemcpyAsync(seg[0], cp[2 * cq + 1], sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream),
-                          "checkpoint restore")) ||
-          (rc = check_hip(hipMemcpyAsync(seg[1], cp[2 * cq], sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream),
-                          "checkpoint restore")))
-         break;
-      for (int m = 1; m < s1 - s0 && rc == OK; m++) {
-         mU.dev = seg[m];
-         mUm.dev = seg[m - 1];
-         mUp.dev = tmp[0];
-         mUp2.dev = seg[m + 1];
-         rc = fused_once((s0 + m - 1) * mDt);
-      }
-      mU.dev = sU;
-      mUm.dev = sUm;
-      mUp.dev = sUp;
-      mUp2.dev = sUp2;
+      if (cq < ncp - 1 && (rc = fill_segment(cq))) break;  // the last one is still in seg
       for (int n = s1 - 1; n >= s0 && rc == OK; n--) {
-         rc = step_vjp(n, seg[n - s0 + 1], seg[n - s0], la, lb, lc, gr[0], gr[1], gr[2], tmp, plane);
+         rc = step_vjp(n, seg[n - s0 + 1], seg[n - s0], seg[n - s0 + 2], la, lb, lc, gr[0], gr[1], gr[2], tmp, plane, pad);
          if (rc) break;
          // rotate: U^{n} becomes the complete adjoint, U^{n-2} starts
          double* old = la;

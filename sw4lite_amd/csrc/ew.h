@@ -5,6 +5,7 @@
 #pragma once
 #include <vector>
 
+#include "kernels_adj.h"
 #include "kernels_common.h"
 #include "kernels_comp.h"
 #include "kernels_curv.h"
@@ -130,8 +131,8 @@ private:
    SgCoef sg() const;
    SourceTable sources() const;
    int fused_once(double t);  // one fused step U, Um -> Up2 (scratch Up), no rotation
-   int step_vjp(int n, double* u, double* um, double* lam_a, double* lam_b, double* lam_c, double* grho,
-                double* gmu, double* gla, double* const* tmp, double* plane);
+   int step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double* lam_b, double* lam_c,
+                double* grho, double* gmu, double* gla, double* const* tmp, double* plane, const AdjPad& pad);
    int comp_begin();
    CompState comp_state() const;
    SgCoefF sgf() const;

@@ -11,6 +11,7 @@
 // direction of the SBP rows (acof, ghcof, bope terms, rhs4sg_rev.C:353-629)
 // is transposed term by term.
 #include "kernels_adj.h"
+#include "kernels_fast.h"
 #include "kernels_step.h"
 #include "stencil_body.h"
 
@@ -59,6 +60,7 @@ struct AdjCtx {
    const double* strx;
    const double* stry;
    const double* strz;
+   Box fb;  // index box of the padded fields read by the unchecked gathers
    __device__ __forceinline__ double vmu(int i, int j, int k) const { return inbox(b, i, j, k) ? mu[sidx(b, i, j, k)] : 0.0; }
    __device__ __forceinline__ double vla(int i, int j, int k) const { return inbox(b, i, j, k) ? la[sidx(b, i, j, k)] : 0.0; }
    __device__ __forceinline__ double sx(int i) const { return i >= b.ib && i <= b.ie ? strx[i - b.ib] : 1.0; }
@@ -78,26 +80,6 @@ struct AdjCtx {
    {
       return sbp_row(m, i, j, k) ? z[sidx(b, i, j, k) + b.npts * c] : 0.0;
    }
-};
-
-// Accessor of rhs4_interior_body / rhs4_xy_body at (i,j,k) over S z:
-// INPLANE = false: interior rows, S = sx sy sz; true: SBP rows, S = sx sy.
-template <bool INPLANE>
-struct TSten {
-   const AdjCtx& A;
-   const double* z;
-   int i, j, k;
-   __device__ __forceinline__ double u(int c, int di, int dj, int dk) const
-   {
-      const int ii = i + di, jj = j + dj, kk = k + dk;
-      const double v = INPLANE ? A.zb(z, c, ii, jj, kk) : A.zi(z, c, ii, jj, kk);
-      return v * A.sx(ii) * A.sy(jj) * (INPLANE ? 1.0 : A.sz(kk));
-   }
-   __device__ __forceinline__ double mu(int di, int dj, int dk) const { return A.vmu(i + di, j + dj, k + dk); }
-   __device__ __forceinline__ double la(int di, int dj, int dk) const { return A.vla(i + di, j + dj, k + dk); }
-   __device__ __forceinline__ double sx(int d) const { return A.sx(i + d); }
-   __device__ __forceinline__ double sy(int d) const { return A.sy(j + d); }
-   __device__ __forceinline__ double sz(int d) const { return A.sz(k + d); }
 };
 
 // x and y parts of the interior formula with the xy mixed terms: the
@@ -205,82 +187,153 @@ __device__ void sbp_col_T(const AdjCtx& A, const double* z, int i, int j, int k,
    }
 }
 
-// W = S zI: the adjoint field on the interior-formula rows times S = sx sy sz
-__global__ void k_adj_weight(AdjCtx A, const double* __restrict__ z, double* __restrict__ w)
+Box pad_box(const Box& b) { return make_box(b.ib - 2, b.ie + 2, b.jb - 2, b.je + 2, b.kb - 2, b.ke + 2); }
+
+namespace {
+Box pad4_box(const Box& b) { return make_box(b.ib - 4, b.ie + 4, b.jb - 4, b.je + 4, b.kb - 4, b.ke + 4); }
+}  // namespace
+
+long adj_pad_doubles(const Box& b)
 {
-   const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   const Box& b = A.b;
-   if (p >= b.npts) return;
-   const int i = b.ib + (int)(p % b.ni);
-   const int j = b.jb + (int)((p / b.ni) % (b.je - b.jb + 1));
-   const int k = b.kb + (int)(p / b.nij);
-   const double s = int_row(A.m, i, j, k) ? A.sx(i) * A.sy(j) * A.sz(k) : 0.0;
-   for (int c = 0; c < 3; c++) w[p + b.npts * c] = s * z[p + b.npts * c];
+   const Box pb = pad_box(b);
+   return 12 * pb.npts + pb.ni + (pb.je - pb.jb + 1) + (pb.ke - pb.kb + 1) + 7 * b.npts + 6 * pad4_box(b).npts;
 }
 
-// out /= S on the interior points (where the forward interior kernel wrote)
-__global__ void k_adj_unweight(AdjCtx A, double* __restrict__ out)
+AdjPad make_adj_pad(const Box& b, double* mem)
 {
-   const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   const Box& b = A.b;
-   if (p >= b.npts) return;
-   const int i = b.ib + (int)(p % b.ni);
-   const int j = b.jb + (int)((p / b.ni) % (b.je - b.jb + 1));
-   const int k = b.kb + (int)(p / b.nij);
-   if (i < b.ib + 2 || i > b.ie - 2 || j < b.jb + 2 || j > b.je - 2 || k < b.kb + 2 || k > b.ke - 2) return;
-   const double s = 1.0 / (A.sx(i) * A.sy(j) * A.sz(k));
-   for (int c = 0; c < 3; c++) out[p + b.npts * c] *= s;
+   AdjPad p;
+   p.pb = pad_box(b);
+   p.w = mem;
+   p.mu = mem + 3 * p.pb.npts;
+   p.la = p.mu + p.pb.npts;
+   p.sx = p.la + p.pb.npts;
+   p.sy = p.sx + p.pb.ni;
+   p.sz = p.sy + (p.pb.je - p.pb.jb + 1);
+   p.wb = p.sz + (p.pb.ke - p.pb.kb + 1);
+   p.w2b = p.wb + 3 * b.npts;
+   p.sb = p.w2b + 3 * b.npts;
+   p.pb4 = pad4_box(b);
+   p.u4 = p.sb + b.npts;
+   p.z4 = p.u4 + 3 * p.pb4.npts;
+   p.gy = p.z4 + 3 * p.pb4.npts;
+   p.dd = p.gy + 3 * p.pb.npts;
+   p.rh = p.dd + 3 * p.pb.npts;
+   return p;
 }
 
-// Accessor of rhs4_interior_body over a weighted field W, 0 outside the box.
-struct WSten {
-   const AdjCtx& A;
-   const double* w;
-   int i, j, k;
-   __device__ __forceinline__ double u(int c, int di, int dj, int dk) const { return A.f(w, c, i + di, j + dj, k + dk); }
-   __device__ __forceinline__ double mu(int di, int dj, int dk) const { return A.vmu(i + di, j + dj, k + dk); }
-   __device__ __forceinline__ double la(int di, int dj, int dk) const { return A.vla(i + di, j + dj, k + dk); }
-   __device__ __forceinline__ double sx(int d) const { return A.sx(i + d); }
-   __device__ __forceinline__ double sy(int d) const { return A.sy(j + d); }
-   __device__ __forceinline__ double sz(int d) const { return A.sz(k + d); }
-};
-
-// the interior-row part of L^T on the ghost shell: cof S^{-1} L(W)
-__global__ void __launch_bounds__(256) k_rhs4sg_T_shell(AdjCtx A, const double* __restrict__ w,
-                                                        double* __restrict__ out, double cof)
+__global__ void k_adj_pad_material(Box b, Box pb, const double* __restrict__ mu, const double* __restrict__ la,
+                                   const double* __restrict__ strx, const double* __restrict__ stry,
+                                   const double* __restrict__ strz, double* __restrict__ mup, double* __restrict__ lap,
+                                   double* __restrict__ sxp, double* __restrict__ syp, double* __restrict__ szp)
 {
    const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   int i, j, k;
-   if (!shell_point(A.b, q, i, j, k)) return;
-   const WSten s{A, w, i, j, k};
-   double r[3];
-   rhs4_interior_body(s, r);
-   const double sc = cof / (A.sx(i) * A.sy(j) * A.sz(k));
-   const long p = sidx(A.b, i, j, k);
-   for (int c = 0; c < 3; c++) out[p + A.b.npts * c] = sc * r[c];
+   if (q < pb.npts) {
+      const int i = pb.ib + (int)(q % pb.ni);
+      const int j = pb.jb + (int)((q / pb.ni) % (pb.je - pb.jb + 1));
+      const int k = pb.kb + (int)(q / pb.nij);
+      const bool in = inbox(b, i, j, k);
+      mup[q] = in ? mu[sidx(b, i, j, k)] : 0.0;
+      lap[q] = in ? la[sidx(b, i, j, k)] : 0.0;
+   }
+   if (q < pb.ni) sxp[q] = (pb.ib + q >= b.ib && pb.ib + q <= b.ie) ? strx[pb.ib + q - b.ib] : 1.0;
+   if (q < pb.je - pb.jb + 1) syp[q] = (pb.jb + q >= b.jb && pb.jb + q <= b.je) ? stry[pb.jb + q - b.jb] : 1.0;
+   if (q < pb.ke - pb.kb + 1) szp[q] = (pb.kb + q >= b.kb && pb.kb + q <= b.ke) ? strz[pb.kb + q - b.kb] : 1.0;
 }
 
-// SBP rows of one side: out += in-plane part on the row planes and the k
-// terms on the planes 0..8 of the row space (thread per point of those planes)
-__global__ void __launch_bounds__(256) k_rhs4sg_T_sbp(AdjCtx A, int kbase, int sgn, const double* __restrict__ z,
+__global__ void k_adj_sfield(Box b, const double* __restrict__ strx, const double* __restrict__ stry,
+                             const double* __restrict__ strz, double* __restrict__ sb)
+{
+   const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (p >= b.npts) return;
+   const long i = p % b.ni, j = (p / b.ni) % (b.je - b.jb + 1), k = p / b.nij;
+   sb[p] = strx[i] * stry[j] * strz[k];
+}
+
+// padded W: INPLANE = false: S zI (interior-formula rows, S = sx sy sz);
+// true: sx sy zB (SBP rows)
+template <bool INPLANE>
+__global__ void k_adj_weight(AdjCtx A, AdjPad P, const double* __restrict__ z)
+{
+   const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   const Box& pb = P.pb;
+   if (q >= pb.npts) return;
+   const int i = pb.ib + (int)(q % pb.ni);
+   const int j = pb.jb + (int)((q / pb.ni) % (pb.je - pb.jb + 1));
+   const int k = pb.kb + (int)(q / pb.nij);
+   const bool row = INPLANE ? sbp_row(A.m, i, j, k) : int_row(A.m, i, j, k);
+   const double s = row ? A.sx(i) * A.sy(j) * (INPLANE ? 1.0 : A.sz(k)) : 0.0;
+   const long p = row ? sidx(A.b, i, j, k) : 0;
+   const bool in = inbox(A.b, i, j, k);
+   const long pbx = in ? sidx(A.b, i, j, k) : 0;
+   for (int c = 0; c < 3; c++) {
+      const double v = row ? s * z[p + A.b.npts * c] : 0.0;
+      P.w[q + pb.npts * c] = v;
+      if (!INPLANE && in) {
+         P.wb[pbx + A.b.npts * c] = v;
+         P.w2b[pbx + A.b.npts * c] = 2 * v;
+      }
+   }
+}
+
+// interior rows on the ghost shell: out = cof S^{-1} L(W) (the forward
+// interior formula on the padded arrays); the interior points are done by
+// the LDS predictor sweep
+__global__ void __launch_bounds__(256) k_rhs4sg_T_shell(AdjCtx A, AdjPad P, double* __restrict__ out, double cof)
+{
+   const Box& b = A.b;
+   const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   int i, j, k;
+   if (!shell_point(b, q, i, j, k)) return;
+   double r[3];
+   rhs_interior_point<1>(P.pb, i, j, k, Plain{P.w}, P.mu, P.la, P.sx, P.sy, P.sz, r);
+   const double sc = cof / (A.sx(i) * A.sy(j) * A.sz(k));
+   const long p = sidx(b, i, j, k);
+   for (int c = 0; c < 3; c++) out[p + b.npts * c] = sc * r[c];
+}
+
+// Accessor of rhs4_xy_body over the padded arrays (no checks).
+struct PadSten {
+   const AdjPad& P;
+   int i, j, k;
+   __device__ __forceinline__ double u(int c, int di, int dj, int dk) const
+   {
+      return P.w[sidx(P.pb, i + di, j + dj, k + dk) + P.pb.npts * c];
+   }
+   __device__ __forceinline__ double mu(int di, int dj, int dk) const { return P.mu[sidx(P.pb, i + di, j + dj, k + dk)]; }
+   __device__ __forceinline__ double la(int di, int dj, int dk) const { return P.la[sidx(P.pb, i + di, j + dj, k + dk)]; }
+   __device__ __forceinline__ double sx(int d) const { return P.sx[i + d - P.pb.ib]; }
+   __device__ __forceinline__ double sy(int d) const { return P.sy[j + d - P.pb.jb]; }
+   __device__ __forceinline__ double sz(int d) const { return P.sz[k + d - P.pb.kb]; }
+};
+
+// SBP rows, in-plane part on the row planes of one side: out += cof
+// (sx sy)^{-1} Lxy(W), W = sx sy zB
+__global__ void __launch_bounds__(256) k_rhs4sg_T_xy(AdjCtx A, AdjPad P, int kbase, int sgn, double* __restrict__ out,
+                                                     double cof)
+{
+   const Box& b = A.b;
+   const int i = b.ib + blockIdx.x * blockDim.x + threadIdx.x;
+   const int j = b.jb + blockIdx.y * blockDim.y + threadIdx.y;
+   const int k = kbase + sgn * (int)blockIdx.z;  // rows 1..6
+   if (i > b.ie || j > b.je) return;
+   const PadSten s{P, i, j, k};
+   double r[3];
+   rhs4_xy_body(s, r);
+   const double sc = cof / (A.sx(i) * A.sy(j));
+   const long p = sidx(b, i, j, k);
+   for (int c = 0; c < 3; c++) out[p + b.npts * c] += sc * r[c];
+}
+
+// SBP rows, k-direction terms on the planes 0..8 of one side
+__global__ void __launch_bounds__(256) k_rhs4sg_T_col(AdjCtx A, int kbase, int sgn, const double* __restrict__ z,
                                                       double* __restrict__ out, double cof)
 {
    const Box& b = A.b;
-   const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   if (e >= 9 * b.nij) return;
-   const int q = (int)(e / b.nij);
-   const long r0 = e - q * b.nij;
-   const int i = b.ib + (int)(r0 % b.ni), j = b.jb + (int)(r0 / b.ni);
-   const int k = kbase + sgn * (q - 1);
-   if (k < b.kb || k > b.ke) return;
+   const int i = b.ib + blockIdx.x * blockDim.x + threadIdx.x;
+   const int j = b.jb + blockIdx.y * blockDim.y + threadIdx.y;
+   const int k = kbase + sgn * ((int)blockIdx.z - 1);  // planes q = 0..8
+   if (i > b.ie || j > b.je || k < b.kb || k > b.ke) return;
    double r[3] = {0, 0, 0};
-   if (q >= 1 && q <= 6) {
-      const TSten<true> s{A, z, i, j, k};
-      double t[3];
-      rhs4_xy_body(s, t);
-      const double wgt = 1.0 / (A.sx(i) * A.sy(j));
-      for (int c = 0; c < 3; c++) r[c] += wgt * t[c];
-   }
    sbp_col_T(A, z, i, j, k, kbase, sgn, r);
    const long p = sidx(b, i, j, k);
    for (int c = 0; c < 3; c++) out[p + b.npts * c] += cof * r[c];
@@ -305,16 +358,19 @@ __device__ __forceinline__ double psi(int t, const double f[5])
 
 // Second-difference terms along axis AX of the rows selected by ROWS (0
 // interior formula, 1 SBP rows): contribution to (gmu, gla) at p.
-template <int AX, int ROWS>
+// FAST: every row and every read of the neighbourhood is an interior-formula
+// row / inside the box, so nothing is checked.
+template <int AX, int ROWS, bool FAST = false>
 __device__ __forceinline__ void grad_second(const AdjCtx& A, const double* z, const double* u, int i, int j, int k,
                                             double& gm, double& gl)
 {
+   const Box& b = A.fb;
    const double sp = AX == 0 ? A.sx(i) : AX == 1 ? A.sy(j) : A.sz(k);
-#pragma unroll
+#pragma unroll 1
    for (int t = 0; t < 5; t++) {
       const int o = 2 - t;  // row = p + o e_AX
       const int ir = i + (AX == 0) * o, jr = j + (AX == 1) * o, kr = k + (AX == 2) * o;
-      const bool ok = ROWS == 0 ? int_row(A.m, ir, jr, kr) : sbp_row(A.m, ir, jr, kr);
+      const bool ok = FAST || (ROWS == 0 ? int_row(A.m, ir, jr, kr) : sbp_row(A.m, ir, jr, kr));
       if (!ok) continue;
       const double srow = AX == 0 ? A.sx(ir) : AX == 1 ? A.sy(jr) : A.sz(kr);
       const double w = srow * sp * (1.0 / 6);
@@ -322,9 +378,12 @@ __device__ __forceinline__ void grad_second(const AdjCtx& A, const double* z, co
       for (int c = 0; c < 3; c++) {
          double f[5];
 #pragma unroll
-         for (int s = 0; s < 5; s++)
-            f[s] = A.f(u, c, ir + (AX == 0) * (s - 2), jr + (AX == 1) * (s - 2), kr + (AX == 2) * (s - 2));
-         const double val = z[sidx(A.b, ir, jr, kr) + A.b.npts * c] * w * psi(t, f);
+         for (int s = 0; s < 5; s++) {
+            const int ii = ir + (AX == 0) * (s - 2), jj = jr + (AX == 1) * (s - 2), kk = kr + (AX == 2) * (s - 2);
+            f[s] = FAST ? u[sidx(b, ii, jj, kk) + b.npts * c] : A.f(u, c, ii, jj, kk);
+         }
+         const double val = (FAST ? z[sidx(b, ir, jr, kr) + b.npts * c] : z[sidx(A.b, ir, jr, kr) + A.b.npts * c]) * w *
+                            psi(t, f);
          gm += (c == AX ? 2.0 : 1.0) * val;
          if (c == AX) gl += val;
       }
@@ -332,16 +391,19 @@ __device__ __forceinline__ void grad_second(const AdjCtx& A, const double* z, co
 }
 
 // Mixed term (coef D_DB u_C)_DA of row component RC: coef at p.
-template <int RC, int DA, int DB, int C, int ROWS>
+template <int RC, int DA, int DB, int C, int ROWS, bool FAST = false>
 __device__ __forceinline__ double grad_mixed(const AdjCtx& A, const double* z, const double* u, int i, int j, int k)
 {
+   const Box& b = A.fb;
    double D = 0, W = 0;
 #pragma unroll
    for (int t = 0; t < 4; t++) {
       const int a = t < 2 ? t - 2 : t - 1;
-      D += cd4(a) * A.f(u, C, i + (DB == 0) * a, j + (DB == 1) * a, k + (DB == 2) * a);
+      const int iu = i + (DB == 0) * a, ju = j + (DB == 1) * a, ku = k + (DB == 2) * a;
+      D += cd4(a) * (FAST ? u[sidx(b, iu, ju, ku) + b.npts * C] : A.f(u, C, iu, ju, ku));
       const int ir = i - (DA == 0) * a, jr = j - (DA == 1) * a, kr = k - (DA == 2) * a;
-      const double zr = ROWS == 0 ? A.zi(z, RC, ir, jr, kr) : A.zb(z, RC, ir, jr, kr);
+      const double zr = FAST ? z[sidx(b, ir, jr, kr) + b.npts * RC]
+                             : (ROWS == 0 ? A.zi(z, RC, ir, jr, kr) : A.zb(z, RC, ir, jr, kr));
       const double sa = DA == 0 ? A.sx(ir) : DA == 1 ? A.sy(jr) : A.sz(kr);
       const double sb = DB == 0 ? A.sx(ir) : DB == 1 ? A.sy(jr) : A.sz(kr);
       W += cd4(a) * sa * sb * zr;
@@ -426,36 +488,225 @@ __device__ void sbp_col_grad(const AdjCtx& A, const double* z, const double* u, 
    }
 }
 
-__global__ void __launch_bounds__(256) k_rhs4sg_matgrad(AdjCtx A, const double* __restrict__ z,
-                                                        const double* __restrict__ u, double* __restrict__ gmu,
-                                                        double* __restrict__ gla, double cof)
+// u and the interior-row part of z on the box grown by 4 (zeros outside)
+__global__ void k_adj_pad4(AdjCtx A, AdjPad P, const double* __restrict__ z, const double* __restrict__ u)
 {
-   const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   const Box& p4 = P.pb4;
+   if (q >= p4.npts) return;
+   const int i = p4.ib + (int)(q % p4.ni);
+   const int j = p4.jb + (int)((q / p4.ni) % (p4.je - p4.jb + 1));
+   const int k = p4.kb + (int)(q / p4.nij);
+   const bool in = inbox(A.b, i, j, k), row = int_row(A.m, i, j, k);
+   const long p = in ? sidx(A.b, i, j, k) : 0;
+   for (int c = 0; c < 3; c++) {
+      P.u4[q + p4.npts * c] = in ? u[p + A.b.npts * c] : 0.0;
+      P.z4[q + p4.npts * c] = row ? z[p + A.b.npts * c] : 0.0;
+   }
+}
+
+// Interior-row terms of G_mu, G_la at every point of the box from the padded
+// fields (no checks: z4 vanishes off the rows, u4 off the box).
+__global__ void __launch_bounds__(256) k_rhs4sg_matgrad_int(AdjCtx A, AdjPad P, double* __restrict__ gmu,
+                                                            double* __restrict__ gla, double cof)
+{
    const Box& b = A.b;
-   if (p >= b.npts) return;
-   const int i = b.ib + (int)(p % b.ni);
-   const int j = b.jb + (int)((p / b.ni) % (b.je - b.jb + 1));
-   const int k = b.kb + (int)(p / b.nij);
+   const int i = b.ib + blockIdx.x * blockDim.x + threadIdx.x;
+   const int j = b.jb + blockIdx.y * blockDim.y + threadIdx.y;
+   const int k = b.kb + blockIdx.z;
+   if (i > b.ie || j > b.je) return;
+   AdjCtx Q = A;
+   Q.fb = P.pb4;  // the fields are read on the padded box
    double gm = 0, gl = 0;
-   // interior rows
-   grad_second<0, 0>(A, z, u, i, j, k, gm, gl);
-   grad_second<1, 0>(A, z, u, i, j, k, gm, gl);
-   grad_second<2, 0>(A, z, u, i, j, k, gm, gl);
-   gl += grad_mixed<0, 0, 1, 1, 0>(A, z, u, i, j, k) + grad_mixed<0, 0, 2, 2, 0>(A, z, u, i, j, k) +
-         grad_mixed<1, 1, 0, 0, 0>(A, z, u, i, j, k) + grad_mixed<1, 1, 2, 2, 0>(A, z, u, i, j, k) +
-         grad_mixed<2, 2, 0, 0, 0>(A, z, u, i, j, k) + grad_mixed<2, 2, 1, 1, 0>(A, z, u, i, j, k);
-   gm += grad_mixed<0, 1, 0, 1, 0>(A, z, u, i, j, k) + grad_mixed<0, 2, 0, 2, 0>(A, z, u, i, j, k) +
-         grad_mixed<1, 0, 1, 0, 0>(A, z, u, i, j, k) + grad_mixed<1, 2, 1, 2, 0>(A, z, u, i, j, k) +
-         grad_mixed<2, 0, 2, 0, 0>(A, z, u, i, j, k) + grad_mixed<2, 1, 2, 1, 0>(A, z, u, i, j, k);
-   // SBP rows: in-plane part, then the k-direction terms
-   if (A.m.top || A.m.bot) {
+   grad_second<0, 0, true>(Q, P.z4, P.u4, i, j, k, gm, gl);
+   grad_second<1, 0, true>(Q, P.z4, P.u4, i, j, k, gm, gl);
+   grad_second<2, 0, true>(Q, P.z4, P.u4, i, j, k, gm, gl);
+   gl += grad_mixed<0, 0, 1, 1, 0, true>(Q, P.z4, P.u4, i, j, k) + grad_mixed<0, 0, 2, 2, 0, true>(Q, P.z4, P.u4, i, j, k) +
+         grad_mixed<1, 1, 0, 0, 0, true>(Q, P.z4, P.u4, i, j, k) + grad_mixed<1, 1, 2, 2, 0, true>(Q, P.z4, P.u4, i, j, k) +
+         grad_mixed<2, 2, 0, 0, 0, true>(Q, P.z4, P.u4, i, j, k) + grad_mixed<2, 2, 1, 1, 0, true>(Q, P.z4, P.u4, i, j, k);
+   gm += grad_mixed<0, 1, 0, 1, 0, true>(Q, P.z4, P.u4, i, j, k) + grad_mixed<0, 2, 0, 2, 0, true>(Q, P.z4, P.u4, i, j, k) +
+         grad_mixed<1, 0, 1, 0, 0, true>(Q, P.z4, P.u4, i, j, k) + grad_mixed<1, 2, 1, 2, 0, true>(Q, P.z4, P.u4, i, j, k) +
+         grad_mixed<2, 0, 2, 0, 0, true>(Q, P.z4, P.u4, i, j, k) + grad_mixed<2, 1, 2, 1, 0, true>(Q, P.z4, P.u4, i, j, k);
+   const long p = sidx(b, i, j, k);
+   gmu[p] += cof * gm;
+   gla[p] += cof * gl;
+}
+
+// LDS k-march version of k_rhs4sg_matgrad_int.  A workgroup owns a 32 x 8
+// tile of (i,j) columns of the box (ghosts included) and marches along k.
+// Plane k of u (halo 4) and of z (halo 2) sit in LDS for the x and y terms;
+// the thread's own column of u (k-4..k+4) and z (k-2..k+2) rotates in
+// registers for the z terms.  Reads the padded fields (kernels_adj.h).
+namespace {
+constexpr int MTX = 32, MTY = 8, MNT = MTX * MTY;
+constexpr int MHX4 = MTX + 8, MHY4 = MTY + 8, MPL4 = MHX4 * MHY4;  // u plane, halo 4
+constexpr int MHX2 = MTX + 4, MHY2 = MTY + 4, MPL2 = MHX2 * MHY2;  // z plane, halo 2
+}  // namespace
+
+__global__ void __launch_bounds__(MNT, 2) k_rhs4sg_matgrad_lds(AdjCtx A, AdjPad P, int ntx, int nty, int kc,
+                                                               double* __restrict__ gmu, double* __restrict__ gla,
+                                                               double cof)
+{
+   __shared__ double su[3 * MPL4];
+   __shared__ double sz[3 * MPL2];
+   const Box& b = A.b;
+   const Box& q4 = P.pb4;
+   const int t = blockIdx.x;
+   const int tix = t % ntx, tr = t / ntx, tjy = tr % nty, tkz = tr / nty;
+   const int tid = threadIdx.x, tx = tid % MTX, ty = tid / MTX;
+   const int ia = b.ib + tix * MTX, ja = b.jb + tjy * MTY;
+   const int i = ia + tx, j = ja + ty;
+   const bool active = i <= b.ie && j <= b.je;
+   const int k0 = b.kb + tkz * kc, k1 = min(b.ke, k0 + kc - 1);
+   const double* u4 = P.u4;
+   const double* z4 = P.z4;
+   const long n4 = q4.npts;
+   // own column (inactive threads read column (ib, jb): inside the pad box)
+   const long col = sidx(q4, active ? i : b.ib, active ? j : b.jb, 0) - sidx(q4, 0, 0, 0);
+   auto at4 = [&](int kk) { return col + (long)q4.nij * (kk - q4.kb) + (sidx(q4, 0, 0, q4.kb) - sidx(q4, 0, 0, q4.kb)); };
+   double ucol[9][3], zcol[5][3];
+#pragma unroll
+   for (int q = 0; q < 8; q++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) ucol[q][c] = u4[sidx(q4, 0, 0, k0 - 4 + q) + col + n4 * c];
+#pragma unroll
+   for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) zcol[q][c] = z4[sidx(q4, 0, 0, k0 - 2 + q) + col + n4 * c];
+   (void)at4;
+   const double sxi = A.sx(i), syj = A.sy(j);
+   double sxo[5], syo[5];
+#pragma unroll
+   for (int o = 0; o < 5; o++) {
+      sxo[o] = A.sx(i + o - 2);
+      syo[o] = A.sy(j + o - 2);
+   }
+   const double i6 = 1.0 / 6, i144 = 1.0 / 144;
+   for (int k = k0; k <= k1; k++) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+         ucol[8][c] = u4[sidx(q4, 0, 0, k + 4) + col + n4 * c];
+         zcol[4][c] = z4[sidx(q4, 0, 0, k + 2) + col + n4 * c];
+      }
+      // stage plane k: u with halo 4, z with halo 2 (0 beyond the pad box)
+      for (int e = tid; e < 3 * MPL4; e += MNT) {
+         const int c = e / MPL4, r = e - c * MPL4, yy = r / MHX4, xx = r - yy * MHX4;
+         const int gi = ia - 4 + xx, gj = ja - 4 + yy;
+         const bool ok = gi <= q4.ie && gj <= q4.je;
+         su[e] = ok ? u4[sidx(q4, gi, gj, k) + n4 * c] : 0.0;
+      }
+      for (int e = tid; e < 3 * MPL2; e += MNT) {
+         const int c = e / MPL2, r = e - c * MPL2, yy = r / MHX2, xx = r - yy * MHX2;
+         const int gi = ia - 2 + xx, gj = ja - 2 + yy;
+         const bool ok = gi <= q4.ie && gj <= q4.je;
+         sz[e] = ok ? z4[sidx(q4, gi, gj, k) + n4 * c] : 0.0;
+      }
+      __syncthreads();
+      if (active) {
+         auto U = [&](int c, int dx, int dy) { return su[c * MPL4 + (ty + 4 + dy) * MHX4 + tx + 4 + dx]; };
+         auto Z = [&](int c, int dx, int dy) { return sz[c * MPL2 + (ty + 2 + dy) * MHX2 + tx + 2 + dx]; };
+         const double szk = A.sz(k);
+         double gm = 0, gl = 0;
+         // second differences along x and y (plane k, LDS)
+#pragma unroll 1
+         for (int tt = 0; tt < 5; tt++) {
+            const int o = 2 - tt;
+            const double wx = A.sx(i + o) * sxi * i6, wy = A.sy(j + o) * syj * i6;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+               double f[5];
+#pragma unroll
+               for (int q = 0; q < 5; q++) f[q] = U(c, o + q - 2, 0);
+               const double vx = Z(c, o, 0) * wx * psi(tt, f);
+#pragma unroll
+               for (int q = 0; q < 5; q++) f[q] = U(c, 0, o + q - 2);
+               const double vy = Z(c, 0, o) * wy * psi(tt, f);
+               gm += (c == 0 ? 2.0 : 1.0) * vx + (c == 1 ? 2.0 : 1.0) * vy;
+               gl += (c == 0 ? vx : 0.0) + (c == 1 ? vy : 0.0);
+            }
+         }
+         // along z (own column, registers)
+#pragma unroll
+         for (int tt = 0; tt < 5; tt++) {
+            const int o = 2 - tt;
+            const double wz = A.sz(k + o) * szk * i6;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+               double f[5];
+#pragma unroll
+               for (int q = 0; q < 5; q++) f[q] = ucol[o + q + 2][c];
+               const double vz = zcol[o + 2][c] * wz * psi(tt, f);
+               gm += (c == 2 ? 2.0 : 1.0) * vz;
+               gl += c == 2 ? vz : 0.0;
+            }
+         }
+         // mixed terms (coef D_DB u_C)_DA of row component RC, coef at this point
+         auto mixed = [&](int RC, int DA, int DB, int C) {
+            double D = 0, W = 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+               const int a = q < 2 ? q - 2 : q - 1;
+               const double ca = cd4(a);
+               D += ca * (DB == 0 ? U(C, a, 0) : DB == 1 ? U(C, 0, a) : ucol[a + 4][C]);
+               double zr, sa, sb;
+               if (DA == 0) {
+                  zr = Z(RC, -a, 0);
+                  sa = sxo[2 - a];
+                  sb = DB == 1 ? syj : szk;
+               } else if (DA == 1) {
+                  zr = Z(RC, 0, -a);
+                  sa = syo[2 - a];
+                  sb = DB == 0 ? sxi : szk;
+               } else {
+                  zr = zcol[2 - a][RC];
+                  sa = A.sz(k - a);
+                  sb = DB == 0 ? sxi : syj;
+               }
+               W += ca * sa * sb * zr;
+            }
+            return D * W * i144;
+         };
+         gl += mixed(0, 0, 1, 1) + mixed(0, 0, 2, 2) + mixed(1, 1, 0, 0) + mixed(1, 1, 2, 2) + mixed(2, 2, 0, 0) +
+               mixed(2, 2, 1, 1);
+         gm += mixed(0, 1, 0, 1) + mixed(0, 2, 0, 2) + mixed(1, 0, 1, 0) + mixed(1, 2, 1, 2) + mixed(2, 0, 2, 0) +
+               mixed(2, 1, 2, 1);
+         const long p = sidx(b, i, j, k);
+         gmu[p] += cof * gm;
+         gla[p] += cof * gl;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+#pragma unroll
+         for (int c = 0; c < 3; c++) ucol[q][c] = ucol[q + 1][c];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+         for (int c = 0; c < 3; c++) zcol[q][c] = zcol[q + 1][c];
+   }
+}
+
+// SBP rows of one side on its planes 0..8: in-plane part (row planes) and
+// the k-direction terms
+__global__ void __launch_bounds__(256) k_rhs4sg_matgrad_sbp(AdjCtx A, int kbase, int sgn, const double* __restrict__ z,
+                                                            const double* __restrict__ u, double* __restrict__ gmu,
+                                                            double* __restrict__ gla, double cof)
+{
+   const Box& b = A.b;
+   const int i = b.ib + blockIdx.x * blockDim.x + threadIdx.x;
+   const int j = b.jb + blockIdx.y * blockDim.y + threadIdx.y;
+   const int q = blockIdx.z;  // plane 0..8
+   const int k = kbase + sgn * (q - 1);
+   if (i > b.ie || j > b.je || k < b.kb || k > b.ke) return;
+   double gm = 0, gl = 0;
+   if (q >= 1 && q <= 6) {
       grad_second<0, 1>(A, z, u, i, j, k, gm, gl);
       grad_second<1, 1>(A, z, u, i, j, k, gm, gl);
       gl += grad_mixed<0, 0, 1, 1, 1>(A, z, u, i, j, k) + grad_mixed<1, 1, 0, 0, 1>(A, z, u, i, j, k);
       gm += grad_mixed<0, 1, 0, 1, 1>(A, z, u, i, j, k) + grad_mixed<1, 0, 1, 0, 1>(A, z, u, i, j, k);
-      if (A.m.top) sbp_col_grad(A, z, u, i, j, k, 1, 1, gm, gl);
-      if (A.m.bot) sbp_col_grad(A, z, u, i, j, k, A.m.nz, -1, gm, gl);
    }
+   sbp_col_grad(A, z, u, i, j, k, kbase, sgn, gm, gl);
+   const long p = sidx(b, i, j, k);
    gmu[p] += cof * gm;
    gla[p] += cof * gl;
 }
@@ -474,9 +725,10 @@ struct DampCtx {
       return i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 && k <= nz;
    }
    // g^ax = beta w_ax / rho y at interior points (w_x = strx coy coz, ...)
+   template <bool FAST = false>
    __device__ __forceinline__ double g(int ax, const double* y, int c, int i, int j, int k) const
    {
-      if (!interior(i, j, k)) return 0.0;
+      if (!FAST && !interior(i, j, k)) return 0.0;
       const int ii = i - b.ib, jj = j - b.jb, kk = k - b.kb;
       const double w = ax == 0 ? sg.strx[ii] * sg.coy[jj] * sg.coz[kk]
                      : ax == 1 ? sg.stry[jj] * sg.cox[ii] * sg.coz[kk]
@@ -485,9 +737,10 @@ struct DampCtx {
       return beta * w / rho[p] * y[p + b.npts * c];
    }
    // r^ax = rho dc_ax, 0 outside the box
+   template <bool FAST = false>
    __device__ __forceinline__ double r(int ax, int i, int j, int k) const
    {
-      if (!inbox(b, i, j, k)) return 0.0;
+      if (!FAST && !inbox(b, i, j, k)) return 0.0;
       const double dc = ax == 0 ? sg.dcx[i - b.ib] : ax == 1 ? sg.dcy[j - b.jb] : sg.dcz[k - b.kb];
       return rho[sidx(b, i, j, k)] * dc;
    }
@@ -502,18 +755,54 @@ __device__ __forceinline__ void step3(int& i, int& j, int& k, int d)
 }
 
 // (D2 g)(l) along AX
-template <int AX>
+template <int AX, bool FAST = false>
 __device__ __forceinline__ double d2g(const DampCtx& D, const double* y, int c, int i, int j, int k)
 {
    int im = i, jm = j, km = k, ip = i, jp = j, kp = k;
    step3<AX>(im, jm, km, -1);
    step3<AX>(ip, jp, kp, 1);
-   return D.g(AX, y, c, ip, jp, kp) - 2 * D.g(AX, y, c, i, j, k) + D.g(AX, y, c, im, jm, km);
+   return D.g<FAST>(AX, y, c, ip, jp, kp) - 2 * D.g<FAST>(AX, y, c, i, j, k) + D.g<FAST>(AX, y, c, im, jm, km);
+}
+
+// D^T y at p (dbar = -D^T y) and the derivative through r = rho dc at p;
+// FAST: p and its neighbours within 2 are interior points (no checks)
+template <bool FAST>
+__device__ __forceinline__ double damp_T_point(const DampCtx& D, const double* __restrict__ y,
+                                               const double* __restrict__ u, const double* __restrict__ um,
+                                               double* __restrict__ dbar, long p, int i, int j, int k)
+{
+   const Box& b = D.b;
+   auto dd = [&](int c, int ii, int jj, int kk) {
+      if (!FAST && !inbox(b, ii, jj, kk)) return 0.0;
+      const long q = sidx(b, ii, jj, kk) + b.npts * c;
+      return u[q] - um[q];
+   };
+   double gr = 0;
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      // D^T y = sum_ax D2 (r D2 g) at p
+      const double gx = d2g<0, FAST>(D, y, c, i, j, k), gy = d2g<1, FAST>(D, y, c, i, j, k),
+                   gz = d2g<2, FAST>(D, y, c, i, j, k);
+      double s = 0;
+      s += D.r<FAST>(0, i + 1, j, k) * d2g<0, FAST>(D, y, c, i + 1, j, k) - 2 * D.r<FAST>(0, i, j, k) * gx +
+           D.r<FAST>(0, i - 1, j, k) * d2g<0, FAST>(D, y, c, i - 1, j, k);
+      s += D.r<FAST>(1, i, j + 1, k) * d2g<1, FAST>(D, y, c, i, j + 1, k) - 2 * D.r<FAST>(1, i, j, k) * gy +
+           D.r<FAST>(1, i, j - 1, k) * d2g<1, FAST>(D, y, c, i, j - 1, k);
+      s += D.r<FAST>(2, i, j, k + 1) * d2g<2, FAST>(D, y, c, i, j, k + 1) - 2 * D.r<FAST>(2, i, j, k) * gz +
+           D.r<FAST>(2, i, j, k - 1) * d2g<2, FAST>(D, y, c, i, j, k - 1);
+      dbar[p + b.npts * c] = -s;
+      const double d0 = dd(c, i, j, k);
+      const double dxx = dd(c, i + 1, j, k) - 2 * d0 + dd(c, i - 1, j, k);
+      const double dyy = dd(c, i, j + 1, k) - 2 * d0 + dd(c, i, j - 1, k);
+      const double dzz = dd(c, i, j, k + 1) - 2 * d0 + dd(c, i, j, k - 1);
+      gr -= D.sg.dcx[i - b.ib] * dxx * gx + D.sg.dcy[j - b.jb] * dyy * gy + D.sg.dcz[k - b.kb] * dzz * gz;
+   }
+   return gr;
 }
 
 __global__ void __launch_bounds__(256) k_addsgd4_T(DampCtx D, const double* __restrict__ y,
                                                    const double* __restrict__ u, const double* __restrict__ um,
-                                                   double* __restrict__ dbar, double* __restrict__ grho)
+                                                   double* __restrict__ dbar, double* __restrict__ grho, int inv_rho)
 {
    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
    const Box& b = D.b;
@@ -521,41 +810,104 @@ __global__ void __launch_bounds__(256) k_addsgd4_T(DampCtx D, const double* __re
    const int i = b.ib + (int)(p % b.ni);
    const int j = b.jb + (int)((p / b.ni) % (b.je - b.jb + 1));
    const int k = b.kb + (int)(p / b.nij);
-   auto dd = [&](int c, int ii, int jj, int kk) {
-      if (!inbox(b, ii, jj, kk)) return 0.0;
-      const long q = sidx(b, ii, jj, kk) + b.npts * c;
-      return u[q] - um[q];
-   };
-   double gr = 0;
-#pragma unroll
-   for (int c = 0; c < 3; c++) {
-      double acc = 0;
-      // D^T y = sum_ax D2 (r D2 g) at p; rho derivative through r at p
-      {
-         double s = 0;
-         s += D.r(0, i + 1, j, k) * d2g<0>(D, y, c, i + 1, j, k) - 2 * D.r(0, i, j, k) * d2g<0>(D, y, c, i, j, k) +
-              D.r(0, i - 1, j, k) * d2g<0>(D, y, c, i - 1, j, k);
-         s += D.r(1, i, j + 1, k) * d2g<1>(D, y, c, i, j + 1, k) - 2 * D.r(1, i, j, k) * d2g<1>(D, y, c, i, j, k) +
-              D.r(1, i, j - 1, k) * d2g<1>(D, y, c, i, j - 1, k);
-         s += D.r(2, i, j, k + 1) * d2g<2>(D, y, c, i, j, k + 1) - 2 * D.r(2, i, j, k) * d2g<2>(D, y, c, i, j, k) +
-              D.r(2, i, j, k - 1) * d2g<2>(D, y, c, i, j, k - 1);
-         acc = s;
-      }
-      dbar[p + b.npts * c] = -acc;
-      const double d0 = dd(c, i, j, k);
-      const double dxx = dd(c, i + 1, j, k) - 2 * d0 + dd(c, i - 1, j, k);
-      const double dyy = dd(c, i, j + 1, k) - 2 * d0 + dd(c, i, j - 1, k);
-      const double dzz = dd(c, i, j, k + 1) - 2 * d0 + dd(c, i, j, k - 1);
-      gr -= D.sg.dcx[i - b.ib] * dxx * d2g<0>(D, y, c, i, j, k) + D.sg.dcy[j - b.jb] * dyy * d2g<1>(D, y, c, i, j, k) +
-            D.sg.dcz[k - b.kb] * dzz * d2g<2>(D, y, c, i, j, k);
-   }
-   if (D.interior(i, j, k)) {
+   const bool fast = i >= 3 && i <= D.nx - 2 && j >= 3 && j <= D.ny - 2 && k >= 3 && k <= D.nz - 2;
+   double gr = fast ? damp_T_point<true>(D, y, u, um, dbar, p, i, j, k) : damp_T_point<false>(D, y, u, um, dbar, p, i, j, k);
+   if (inv_rho && D.interior(i, j, k)) {
       double dm[3];
       addsgd4_point<1>(b, i, j, k, u, um, D.rho, D.sg, D.beta, dm);
       const double rp = D.rho[p];
       gr += (y[p] * dm[0] + y[p + b.npts] * dm[1] + y[p + 2 * b.npts] * dm[2]) / rp;
    }
    grho[p] += gr;
+}
+
+// padded inputs of the damping transpose
+__global__ void k_damp_prep(DampCtx D, AdjPad P, const double* __restrict__ y, const double* __restrict__ u,
+                            const double* __restrict__ um)
+{
+   const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   const Box& pb = P.pb;
+   if (q >= pb.npts) return;
+   const int i = pb.ib + (int)(q % pb.ni);
+   const int j = pb.jb + (int)((q / pb.ni) % (pb.je - pb.jb + 1));
+   const int k = pb.kb + (int)(q / pb.nij);
+   const Box& b = D.b;
+   const bool in = inbox(b, i, j, k), itr = D.interior(i, j, k);
+   const long p = in ? sidx(b, i, j, k) : 0;
+   const double rp = D.rho[p];
+   P.rh[q] = in ? rp : 0.0;
+   for (int c = 0; c < 3; c++) {
+      const long v = p + b.npts * c;
+      P.gy[q + pb.npts * c] = itr ? D.beta * y[v] / rp : 0.0;
+      P.dd[q + pb.npts * c] = in ? u[v] - um[v] : 0.0;
+   }
+}
+
+// D^T y and the r = rho dc derivative at every box point from the padded
+// fields; along each axis the weight w_ax = str_ax co_b co_c factors into
+// co_b co_c (constant along the axis) and str_ax (inside the differences).
+__global__ void __launch_bounds__(256) k_addsgd4_T_pad(DampCtx D, AdjPad P, double* __restrict__ dbar,
+                                                       double* __restrict__ grho)
+{
+   const Box& b = D.b;
+   const Box& pb = P.pb;
+   const int i = b.ib + blockIdx.x * blockDim.x + threadIdx.x;
+   const int j = b.jb + blockIdx.y * blockDim.y + threadIdx.y;
+   const int k = b.kb + blockIdx.z;
+   if (i > b.ie || j > b.je) return;
+   const long q0 = sidx(pb, i, j, k);
+   const long sxs = 1, sys = pb.ni, szs = pb.nij;
+   const int ii = i - b.ib, jj = j - b.jb, kk = k - b.kb;
+   const SgCoef& sg = D.sg;
+   const double* sx = P.sx + (i - pb.ib);  // stretching at i (padded tables, 1 in the pad)
+   const double* sy = P.sy + (j - pb.jb);
+   const double* sz = P.sz + (k - pb.kb);
+   auto cl = [](int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; };
+   const int ni = b.ie - b.ib, nj = b.je - b.jb, nk = b.ke - b.kb;
+   // r = rho dc at -1, 0, +1 along each axis (rho is 0 in the pad)
+   const double rx[3] = {P.rh[q0 - sxs] * sg.dcx[cl(ii - 1, 0, ni)], P.rh[q0] * sg.dcx[ii], P.rh[q0 + sxs] * sg.dcx[cl(ii + 1, 0, ni)]};
+   const double ry[3] = {P.rh[q0 - sys] * sg.dcy[cl(jj - 1, 0, nj)], P.rh[q0] * sg.dcy[jj], P.rh[q0 + sys] * sg.dcy[cl(jj + 1, 0, nj)]};
+   const double rz[3] = {P.rh[q0 - szs] * sg.dcz[cl(kk - 1, 0, nk)], P.rh[q0] * sg.dcz[kk], P.rh[q0 + szs] * sg.dcz[cl(kk + 1, 0, nk)]};
+   const double wx = sg.coy[jj] * sg.coz[kk], wy = sg.cox[ii] * sg.coz[kk], wz = sg.cox[ii] * sg.coy[jj];
+   double gr = 0;
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      const double* g = P.gy + pb.npts * c + q0;
+      const double* d = P.dd + pb.npts * c + q0;
+      double hx[5], hy[5], hz[5];
+#pragma unroll
+      for (int o = 0; o < 5; o++) {
+         hx[o] = sx[o - 2] * g[(o - 2) * sxs];
+         hy[o] = sy[o - 2] * g[(o - 2) * sys];
+         hz[o] = sz[o - 2] * g[(o - 2) * szs];
+      }
+      const double ax[3] = {hx[2] - 2 * hx[1] + hx[0], hx[3] - 2 * hx[2] + hx[1], hx[4] - 2 * hx[3] + hx[2]};
+      const double ay[3] = {hy[2] - 2 * hy[1] + hy[0], hy[3] - 2 * hy[2] + hy[1], hy[4] - 2 * hy[3] + hy[2]};
+      const double az[3] = {hz[2] - 2 * hz[1] + hz[0], hz[3] - 2 * hz[2] + hz[1], hz[4] - 2 * hz[3] + hz[2]};
+      const double tx = wx * (rx[2] * ax[2] - 2 * rx[1] * ax[1] + rx[0] * ax[0]);
+      const double ty = wy * (ry[2] * ay[2] - 2 * ry[1] * ay[1] + ry[0] * ay[0]);
+      const double tz = wz * (rz[2] * az[2] - 2 * rz[1] * az[1] + rz[0] * az[0]);
+      dbar[sidx(b, i, j, k) + b.npts * c] = -(tx + ty + tz);
+      const double dxx = d[sxs] - 2 * d[0] + d[-sxs];
+      const double dyy = d[sys] - 2 * d[0] + d[-sys];
+      const double dzz = d[szs] - 2 * d[0] + d[-szs];
+      gr -= sg.dcx[ii] * dxx * wx * ax[1] + sg.dcy[jj] * dyy * wy * ay[1] + sg.dcz[kk] * dzz * wz * az[1];
+   }
+   grho[sidx(b, i, j, k)] += gr;
+}
+
+__global__ void k_damp_invrho(DampCtx D, const double* __restrict__ y, const double* __restrict__ u,
+                              const double* __restrict__ um, double* __restrict__ grho)
+{
+   const Box& b = D.b;
+   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+   const int k = 1 + blockIdx.z;
+   if (i > D.nx || j > D.ny) return;
+   double dm[3];
+   addsgd4_point<1>(b, i, j, k, u, um, D.rho, D.sg, D.beta, dm);
+   const long p = sidx(b, i, j, k);
+   grho[p] += (y[p] * dm[0] + y[p + b.npts] * dm[1] + y[p + 2 * b.npts] * dm[2]) / D.rho[p];
 }
 
 // ---------------------------------------------------------------------------
@@ -687,6 +1039,23 @@ __global__ void k_rho_grad(Box b, double coef, const double* __restrict__ y, con
    grho[p] -= coef * s / (rho[p] * rho[p]);
 }
 
+__global__ void k_rho_diff(Box b, double coef, const double* __restrict__ y, const double* __restrict__ a,
+                           const double* __restrict__ bs, const double* __restrict__ rho, double* __restrict__ grho)
+{
+   const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (p >= b.npts) return;
+   const int i = b.ib + (int)(p % b.ni);
+   const int j = b.jb + (int)((p / b.ni) % (b.je - b.jb + 1));
+   const int k = b.kb + (int)(p / b.nij);
+   if (i < b.ib + 2 || i > b.ie - 2 || j < b.jb + 2 || j > b.je - 2 || k < b.kb + 2 || k > b.ke - 2) return;
+   double s = 0;
+   for (int c = 0; c < 3; c++) {
+      const long q = p + b.npts * c;
+      s += y[q] * (a[q] - (bs ? bs[q] : 0.0));
+   }
+   grho[p] -= coef * s / rho[p];
+}
+
 __global__ void k_scatter_add(int n, const int* __restrict__ idx, const double* __restrict__ val, double* __restrict__ y)
 {
    const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -696,41 +1065,83 @@ __global__ void k_scatter_add(int n, const int* __restrict__ idx, const double* 
 // ===========================================================================
 // launchers
 // ===========================================================================
+int launch_adj_pad_material(const Box& b, const double* mu, const double* la, const double* strx, const double* stry,
+                            const double* strz, const AdjPad& P, hipStream_t s)
+{
+   k_adj_pad_material<<<cdiv(P.pb.npts, 256), 256, 0, s>>>(b, P.pb, mu, la, strx, stry, strz, P.mu, P.la, P.sx, P.sy,
+                                                           P.sz);
+   k_adj_sfield<<<cdiv(b.npts, 256), 256, 0, s>>>(b, strx, stry, strz, P.sb);
+   return check_hip(hipGetLastError(), "adjoint pad launch");
+}
+
 int launch_rhs4sg_T(const Box& b, const RowMap& m, const SbpTables* d_sbp, const double* z, const double* mu,
                     const double* la, double h, const double* strx, const double* stry, const double* strz,
-                    double* out, double* work, hipStream_t s)
+                    double* out, const AdjPad& P, hipStream_t s)
 {
    const AdjCtx A{b, m, d_sbp, mu, la, strx, stry, strz};
    const double cof = 1.0 / (h * h);
-   // interior rows: S^{-1} L(S zI), the interior formula on every interior
-   // point (the forward kernel, no SBP closure) and on the ghost shell
-   k_adj_weight<<<cdiv(b.npts, 256), 256, 0, s>>>(A, z, work);
-   const int plain[6] = {0, 0, 0, 0, 0, 0};
-   int rc = launch_rhs4sg(1, b, m.nz, plain, d_sbp, out, work, mu, la, h, strx, stry, strz, s);
+   const dim3 blk(64, 4, 1);
+   const unsigned gi = cdiv(b.ni, 64), gj = cdiv(b.je - b.jb + 1, 4);
+   // interior rows: S^{-1} L(S zI) on every point, the interior points by
+   // the LDS predictor sweep (2W - 2W + cof L(W)/S), the ghost shell apart
+   k_adj_weight<false><<<cdiv(P.pb.npts, 256), 256, 0, s>>>(A, P, z);
+   SgCoef sg{};
+   sg.strx = strx;
+   sg.stry = stry;
+   sg.strz = strz;
+   int rc = launch_sweep_lds(false, 6, b, b.kb + 2, b.ke - 2, 64, nullptr, P.wb, P.w2b, mu, la, P.sb, sg, out, h, 1.0,
+                             0.0, s);
    if (rc) return rc;
-   k_adj_unweight<<<cdiv(b.npts, 256), 256, 0, s>>>(A, out);
    const long ns = shell_count(b);
-   k_rhs4sg_T_shell<<<cdiv(ns, 256), 256, 0, s>>>(A, work, out, cof);
-   // SBP rows
-   if (m.top) k_rhs4sg_T_sbp<<<cdiv(9 * b.nij, 256), 256, 0, s>>>(A, 1, 1, z, out, cof);
-   if (m.bot) k_rhs4sg_T_sbp<<<cdiv(9 * b.nij, 256), 256, 0, s>>>(A, m.nz, -1, z, out, cof);
+   k_rhs4sg_T_shell<<<cdiv(ns, 256), 256, 0, s>>>(A, P, out, cof);
+   // SBP rows: in-plane part from (sx sy) zB, then the k-direction terms
+   if (m.top || m.bot) {
+      k_adj_weight<true><<<cdiv(P.pb.npts, 256), 256, 0, s>>>(A, P, z);
+      if (m.top) k_rhs4sg_T_xy<<<dim3(gi, gj, 6), blk, 0, s>>>(A, P, 1, 1, out, cof);
+      if (m.bot) k_rhs4sg_T_xy<<<dim3(gi, gj, 6), blk, 0, s>>>(A, P, m.nz, -1, out, cof);
+      if (m.top) k_rhs4sg_T_col<<<dim3(gi, gj, 9), blk, 0, s>>>(A, 1, 1, z, out, cof);
+      if (m.bot) k_rhs4sg_T_col<<<dim3(gi, gj, 9), blk, 0, s>>>(A, m.nz, -1, z, out, cof);
+   }
    return check_hip(hipGetLastError(), "rhs4sg transpose launch");
 }
 
 int launch_rhs4sg_matgrad(const Box& b, const RowMap& m, const SbpTables* d_sbp, const double* z, const double* u,
                           const double* mu, const double* la, double h, const double* strx, const double* stry,
-                          const double* strz, double* gmu, double* gla, hipStream_t s)
+                          const double* strz, double* gmu, double* gla, const AdjPad& P, hipStream_t s)
 {
    const AdjCtx A{b, m, d_sbp, mu, la, strx, stry, strz};
-   k_rhs4sg_matgrad<<<cdiv(b.npts, 256), 256, 0, s>>>(A, z, u, gmu, gla, 1.0 / (h * h));
+   const double cof = 1.0 / (h * h);
+   const dim3 blk(64, 4, 1);
+   const unsigned gi = cdiv(b.ni, 64), gj = cdiv(b.je - b.jb + 1, 4);
+   k_adj_pad4<<<cdiv(P.pb4.npts, 256), 256, 0, s>>>(A, P, z, u);
+   {
+      const int ntx = (int)cdiv(b.ni, MTX), nty = (int)cdiv(b.je - b.jb + 1, MTY), kc = 32;
+      const int nkc = (int)cdiv(b.ke - b.kb + 1, kc);
+      k_rhs4sg_matgrad_lds<<<ntx * nty * nkc, MNT, 0, s>>>(A, P, ntx, nty, kc, gmu, gla, cof);
+   }
+   if (m.top) k_rhs4sg_matgrad_sbp<<<dim3(gi, gj, 9), blk, 0, s>>>(A, 1, 1, z, u, gmu, gla, cof);
+   if (m.bot) k_rhs4sg_matgrad_sbp<<<dim3(gi, gj, 9), blk, 0, s>>>(A, m.nz, -1, z, u, gmu, gla, cof);
    return check_hip(hipGetLastError(), "rhs4sg material-derivative launch");
 }
 
 int launch_addsgd4_T(const Box& b, const double* y, const double* u, const double* um, const double* rho,
-                     const SgCoef& sg, double beta, double* dbar, double* grho, hipStream_t s)
+                     const SgCoef& sg, double beta, double* dbar, double* grho, hipStream_t s, bool inv_rho)
 {
    const DampCtx D{b, b.ie - b.ib - 3, b.je - b.jb - 3, b.ke - b.kb - 3, sg, rho, beta};
-   k_addsgd4_T<<<cdiv(b.npts, 256), 256, 0, s>>>(D, y, u, um, dbar, grho);
+   k_addsgd4_T<<<cdiv(b.npts, 256), 256, 0, s>>>(D, y, u, um, dbar, grho, inv_rho ? 1 : 0);
+   return check_hip(hipGetLastError(), "addsgd4 transpose launch");
+}
+
+int launch_addsgd4_T_pad(const Box& b, const double* y, const double* u, const double* um, const double* rho,
+                         const SgCoef& sg, double beta, double* dbar, double* grho, const AdjPad& P, hipStream_t s,
+                         bool inv_rho)
+{
+   const DampCtx D{b, b.ie - b.ib - 3, b.je - b.jb - 3, b.ke - b.kb - 3, sg, rho, beta};
+   k_damp_prep<<<cdiv(P.pb.npts, 256), 256, 0, s>>>(D, P, y, u, um);
+   const dim3 blk(64, 4, 1);
+   k_addsgd4_T_pad<<<dim3(cdiv(b.ni, 64), cdiv(b.je - b.jb + 1, 4), b.ke - b.kb + 1), blk, 0, s>>>(D, P, dbar, grho);
+   if (inv_rho)
+      k_damp_invrho<<<dim3(cdiv(D.nx, 64), cdiv(D.ny, 4), D.nz), blk, 0, s>>>(D, y, u, um, grho);
    return check_hip(hipGetLastError(), "addsgd4 transpose launch");
 }
 
@@ -776,6 +1187,13 @@ int launch_rho_grad(const Box& b, double coef, const double* y, const double* lu
                     const double* rho, double* grho, hipStream_t s)
 {
    k_rho_grad<<<cdiv(b.npts, 256), 256, 0, s>>>(b, coef, y, lu, f, rho, grho);
+   return check_hip(hipGetLastError(), "density derivative launch");
+}
+
+int launch_rho_diff(const Box& b, double coef, const double* y, const double* a, const double* bsub, const double* rho,
+                    double* grho, hipStream_t s)
+{
+   k_rho_diff<<<cdiv(b.npts, 256), 256, 0, s>>>(b, coef, y, a, bsub, rho, grho);
    return check_hip(hipGetLastError(), "density derivative launch");
 }
 
