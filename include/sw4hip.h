@@ -312,6 +312,26 @@ int sw4hip_ew_get_receivers(sw4hip_ew* ew, int* nsamples, double* out);
 /* Device pointers of the state: which = 0 U, 1 Um, 2 rho, 3 mu, 4 lambda,
  * 5 metric, 6 Jacobian (curvilinear grid). */
 int sw4hip_ew_device_array(sw4hip_ew* ew, int which, double** ptr);
+
+/* ---- Batched shots (no counterpart in the reference, which runs one grid
+ * per process): several set-up EW objects of any grid size advanced together
+ * with the fused FP64 step.  The three interior sweeps of all shots run as one
+ * launch each; the boundary work of every grid follows on the batch's stream;
+ * sources and receivers read per-shot device clocks, so a captured period of
+ * three steps (the buffer rotation) replays as one hipGraph.  The shots keep
+ * their own state: after a batch step each EW reports its new time, samples
+ * and solution as if stepped alone (same kernels, same results). ---- */
+typedef struct sw4hip_batch sw4hip_batch;
+int sw4hip_batch_create(sw4hip_ew* const* shots, int nshots, sw4hip_batch** batch);
+int sw4hip_batch_destroy(sw4hip_batch* batch);
+/* Planes per workgroup of the batched sweeps (default 64). */
+int sw4hip_batch_set_kchunk(sw4hip_batch* batch, int kchunk);
+/* Autotuner: time one batch step for each candidate k-block (cand NULL:
+ * 16, 32, 64, 128), keep the fastest (*best, per-candidate ms in ms[] when
+ * not NULL), then reset every shot to zero data at t = 0. */
+int sw4hip_batch_tune(sw4hip_batch* batch, int ncand, const int* cand, int* best, double* ms);
+/* Advance every shot by nsteps; use_graph: replay the captured period. */
+int sw4hip_batch_step(sw4hip_batch* batch, int nsteps, int use_graph);
 int sw4hip_ew_sync(sw4hip_ew* ew);
 
 #ifdef __cplusplus

@@ -25,6 +25,7 @@ SOURCES = [
     "setup.cpp",
     "ew.cpp",
     "adjoint.cpp",
+    "batch.cpp",
     "abi.cpp",
 ]
 

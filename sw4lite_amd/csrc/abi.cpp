@@ -1,9 +1,12 @@
 // abi.cpp -- extern "C" entry points declared in include/sw4hip.h.
 #include <cstring>
+#include <new>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/sw4hip.h"
+#include "batch.h"
 #include "ew.h"
 #include "kernels_adj.h"
 #include "kernels_curv.h"
@@ -764,6 +767,59 @@ int sw4hip_ew_device_array(sw4hip_ew* h, int which, double** ptr)
    default: return set_error(E_INVAL, "which must be 0..6");
    }
    return OK;
+}
+
+struct sw4hip_batch {
+   Batch* b;
+};
+
+int sw4hip_batch_create(sw4hip_ew* const* shots, int nshots, sw4hip_batch** out)
+{
+   if (!shots || nshots < 1 || !out) return set_error(E_INVAL, "batch: need shots and an output handle");
+   std::vector<EW*> v;
+   for (int q = 0; q < nshots; q++) {
+      if (!shots[q] || !shots[q]->ew) return set_error(E_INVAL, "batch: null shot");
+      v.push_back(shots[q]->ew);
+   }
+   Batch* b = new (std::nothrow) Batch(v);
+   if (!b) return set_error(E_NOMEM, "batch: allocation failed");
+   const int rc = b->init();
+   if (rc) {
+      delete b;
+      return rc;
+   }
+   *out = new (std::nothrow) sw4hip_batch{b};
+   if (!*out) {
+      delete b;
+      return set_error(E_NOMEM, "batch: allocation failed");
+   }
+   return OK;
+}
+
+int sw4hip_batch_destroy(sw4hip_batch* h)
+{
+   if (!h) return OK;
+   delete h->b;
+   delete h;
+   return OK;
+}
+
+int sw4hip_batch_set_kchunk(sw4hip_batch* h, int kchunk)
+{
+   if (!h || !h->b) return set_error(E_INVAL, "null batch");
+   return h->b->set_kchunk(kchunk);
+}
+
+int sw4hip_batch_tune(sw4hip_batch* h, int ncand, const int* cand, int* best, double* ms)
+{
+   if (!h || !h->b) return set_error(E_INVAL, "null batch");
+   return h->b->tune(best, ms, ncand, cand);
+}
+
+int sw4hip_batch_step(sw4hip_batch* h, int nsteps, int use_graph)
+{
+   if (!h || !h->b) return set_error(E_INVAL, "null batch");
+   return h->b->step(nsteps, use_graph != 0);
 }
 
 int sw4hip_ew_sync(sw4hip_ew* h)

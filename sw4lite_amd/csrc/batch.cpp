@@ -1,0 +1,291 @@
+// batch.cpp -- several shots advanced together (declarations in batch.h).
+//
+// The fused FP64 step of every shot, in the stage order of EW::step_fused:
+// the three interior sweeps (predictor, corrector, damping) of all shots are
+// one launch each; the SBP rows, ghost shell, sources and ghost-point
+// boundary conditions run per grid on the same stream.  Sources and
+// receivers read the time and the sample index from per-shot device clocks,
+// so a captured three-step period (U, Um, Up' rotate with period 3) replays
+// without changes.
+#include "batch.h"
+
+#include <algorithm>
+
+namespace sw4amd {
+
+Batch::~Batch()
+{
+   if (m_exec) (void)hipGraphExecDestroy(m_exec);
+   for (void* p : m_allocs) (void)hipFree(p);
+   if (m_s) (void)hipStreamDestroy(m_s);
+}
+
+int Batch::init()
+{
+   const int n = (int)m_shots.size();
+   if (n < 1) return set_error(E_INVAL, "batch: no shots");
+   for (EW* e : m_shots) {
+      if (!e) return set_error(E_INVAL, "batch: null shot");
+      if (!e->m_setup_done) return set_error(E_STATE, "batch: every shot must be set up");
+      if (!e->m_opt.corder || e->m_curv) return set_error(E_INVAL, "batch: shots must be Cartesian with corder = 1");
+      if (e->m_comp) return set_error(E_STATE, "batch: a shot holds an FP32 state; write it back first");
+   }
+   for (size_t a = 0; a < m_shots.size(); a++)
+      for (size_t b = a + 1; b < m_shots.size(); b++)
+         if (m_shots[a] == m_shots[b]) return set_error(E_INVAL, "batch: a shot is listed twice");
+   int rc;
+   if ((rc = check_hip(hipStreamCreate(&m_s), "batch stream"))) return rc;
+   void* p = nullptr;
+   const size_t tab = sizeof(SweepShot) * 9 * n, clk = (sizeof(double) * 2 + sizeof(int) * 2) * n;
+   if (hipMalloc(&p, tab + clk) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(E_NOMEM, "batch: table allocation failed");
+   }
+   m_allocs.push_back(p);
+   m_dtab = static_cast<SweepShot*>(p);
+   m_clk_t = reinterpret_cast<double*>(static_cast<char*>(p) + tab);
+   m_clk_dt = m_clk_t + n;
+   m_clk_s = reinterpret_cast<int*>(m_clk_dt + n);
+   m_clk_max = m_clk_s + n;
+   return refresh();
+}
+
+int Batch::refresh()
+{
+   const int n = (int)m_shots.size();
+   m_base.assign(3 * n, nullptr);
+   std::vector<double> t(n), dt(n);
+   std::vector<int> smp(n), mx(n);
+   for (int q = 0; q < n; q++) {
+      EW* e = m_shots[q];
+      m_base[3 * q] = e->mUm.dev;
+      m_base[3 * q + 1] = e->mU.dev;
+      m_base[3 * q + 2] = e->mUp2.dev;
+      t[q] = e->mT;
+      dt[q] = e->mDt;
+      smp[q] = e->m_nsamples;
+      mx[q] = (e->m_rec.empty() || !e->m_drecdata) ? 0 : e->m_maxsamples;
+   }
+   m_phase = 0;
+   int rc;
+   if ((rc = check_hip(hipMemcpy(m_clk_t, t.data(), sizeof(double) * n, hipMemcpyHostToDevice), "clock upload")) ||
+       (rc = check_hip(hipMemcpy(m_clk_dt, dt.data(), sizeof(double) * n, hipMemcpyHostToDevice), "clock upload")) ||
+       (rc = check_hip(hipMemcpy(m_clk_s, smp.data(), sizeof(int) * n, hipMemcpyHostToDevice), "clock upload")) ||
+       (rc = check_hip(hipMemcpy(m_clk_max, mx.data(), sizeof(int) * n, hipMemcpyHostToDevice), "clock upload")))
+      return rc;
+   return build_tables();
+}
+
+int Batch::set_kchunk(int kc)
+{
+   if (kc < 1) return set_error(E_INVAL, "batch: k-block must be positive");
+   m_kchunk = kc;
+   return build_tables();
+}
+
+int Batch::build_tables()
+{
+   const int n = (int)m_shots.size();
+   std::vector<SweepShot> h(9 * n);
+   for (int ph = 0; ph < 3; ph++) {
+      int g0[3] = {0, 0, 0};
+      for (int q = 0; q < n; q++) {
+         EW* e = m_shots[q];
+         const GridSetup& g = e->m_opt.grid;
+         const Box& b = e->m_box;
+         double* um = m_base[3 * q + ph % 3];  // phase ph: (Um, U, Up2) = base rotated by ph
+         double* u = m_base[3 * q + (ph + 1) % 3];
+         double* up2 = m_base[3 * q + (ph + 2) % 3];
+         const SgCoef c = e->sg();
+         const int k1 = e->m_onesided[4] == 1 ? 7 : b.kb + 2;
+         const int k2 = e->m_onesided[5] == 1 ? g.nz - 6 : b.ke - 2;
+         SweepShot* t = &h[(ph * 3) * n];
+         g0[0] = fill_sweep_shot(false, b, k1, k2, m_kchunk, nullptr, u, um, e->mMu.dev, e->mLambda.dev, e->mRho.dev, c,
+                                 e->mUp.dev, g.h, e->mDt, 0.0, false, g0[0], &t[q]);
+         g0[1] = fill_sweep_shot(false, b, k1, k2, m_kchunk, e->mUp.dev, u, um, e->mMu.dev, e->mLambda.dev, e->mRho.dev,
+                                 c, up2, g.h, e->mDt, e->m_opt.beta, true, g0[1], &t[n + q]);
+         const bool damp = e->m_opt.beta != 0;
+         g0[2] = fill_sweep_shot(true, b, k1, damp ? k2 : k1 - 1, m_kchunk, nullptr, u, um, nullptr, nullptr, e->mRho.dev,
+                                 c, up2, g.h, e->mDt, e->m_opt.beta, true, g0[2], &t[2 * n + q]);
+      }
+      for (int k = 0; k < 3; k++) m_nblk[ph][k] = g0[k];
+   }
+   if (m_exec) {
+      (void)hipGraphExecDestroy(m_exec);
+      m_exec = nullptr;
+   }
+   return check_hip(hipMemcpy(m_dtab, h.data(), sizeof(SweepShot) * h.size(), hipMemcpyHostToDevice), "table upload");
+}
+
+int Batch::one_step(int ph)
+{
+   const int n = (int)m_shots.size();
+   const SweepShot* tab = m_dtab + (ph * 3) * n;
+   int rc;
+   if ((rc = launch_sweep_batch(0, tab, n, m_nblk[ph][0], m_s))) return rc;
+   for (int q = 0; q < n; q++) {
+      EW* e = m_shots[q];
+      const GridSetup& g = e->m_opt.grid;
+      const SgCoef c = e->sg();
+      double* um = m_base[3 * q + ph % 3];
+      double* u = m_base[3 * q + (ph + 1) % 3];
+      const double dt = e->mDt;
+      if ((rc = launch_pred_tail(e->m_box, g.nz, e->m_onesided, e->m_dsbp, u, um, e->mMu.dev, e->mLambda.dev,
+                                 e->mRho.dev, c, e->mUp.dev, g.h, dt, m_s)) ||
+          (rc = launch_src_apply(e->m_box, e->sources(), 0.0, 0, dt * dt, e->mRho.dev, e->mUp.dev, m_s, m_clk_t + q)) ||
+          (rc = e->enforceBC(e->mUp, e->mMu, e->mLambda, 0.0)))
+         return rc;
+   }
+   if ((rc = launch_sweep_batch(1, tab + n, n, m_nblk[ph][1], m_s)) ||
+       (rc = launch_sweep_batch(2, tab + 2 * n, n, m_nblk[ph][2], m_s)))
+      return rc;
+   for (int q = 0; q < n; q++) {
+      EW* e = m_shots[q];
+      const GridSetup& g = e->m_opt.grid;
+      const SgCoef c = e->sg();
+      double* um = m_base[3 * q + ph % 3];
+      double* u = m_base[3 * q + (ph + 1) % 3];
+      double* up2 = m_base[3 * q + (ph + 2) % 3];
+      const double dt = e->mDt;
+      Sarray v = e->mU;
+      v.dev = up2;
+      if ((rc = launch_corr_tail(e->m_box, g.nz, e->m_onesided, e->m_dsbp, e->mUp.dev, u, um, e->mMu.dev,
+                                 e->mLambda.dev, e->mRho.dev, c, e->m_opt.beta, up2, g.h, dt, m_s)) ||
+          (rc = launch_src_apply(e->m_box, e->sources(), 0.0, 1, dt * dt * dt * dt / 12, e->mRho.dev, up2, m_s,
+                                 m_clk_t + q)) ||
+          (rc = e->enforceBC(v, e->mMu, e->mLambda, 0.0)))
+         return rc;
+      const int nrec = (int)e->m_rec.size() / 4;
+      if (nrec > 0 && e->m_drecdata &&
+          (rc = launch_record(e->m_box, nrec, e->m_drec, up2, um, dt, e->m_drecdata, m_s, m_clk_s + q, e->m_maxsamples)))
+         return rc;
+   }
+   return launch_clock_tick(n, m_clk_t, m_clk_s, m_clk_dt, m_clk_max, m_s);
+}
+
+int Batch::run_period_graph()
+{
+   int rc;
+   if (!m_exec) {
+      hipGraph_t graph = nullptr;
+      if ((rc = check_hip(hipStreamBeginCapture(m_s, hipStreamCaptureModeGlobal), "graph capture"))) return rc;
+      for (int ph = 0; ph < 3 && rc == OK; ph++) rc = one_step(ph);
+      const int re = check_hip(hipStreamEndCapture(m_s, &graph), "graph capture");
+      if (rc || re) {
+         if (graph) (void)hipGraphDestroy(graph);
+         return rc ? rc : re;
+      }
+      rc = check_hip(hipGraphInstantiate(&m_exec, graph, nullptr, nullptr, 0), "graph instantiate");
+      (void)hipGraphDestroy(graph);
+      if (rc) return rc;
+   }
+   return check_hip(hipGraphLaunch(m_exec, m_s), "graph launch");
+}
+
+int Batch::sync_clocks(int nsteps)
+{
+   const int n = (int)m_shots.size();
+   std::vector<double> t(n);
+   std::vector<int> smp(n);
+   int rc;
+   if ((rc = check_hip(hipStreamSynchronize(m_s), "batch synchronize")) ||
+       (rc = check_hip(hipMemcpy(t.data(), m_clk_t, sizeof(double) * n, hipMemcpyDeviceToHost), "clock readback")) ||
+       (rc = check_hip(hipMemcpy(smp.data(), m_clk_s, sizeof(int) * n, hipMemcpyDeviceToHost), "clock readback")))
+      return rc;
+   for (int q = 0; q < n; q++) {
+      EW* e = m_shots[q];
+      e->mT = t[q];
+      if (!e->m_rec.empty() && e->m_drecdata) e->m_nsamples = smp[q];
+      e->mCycle += nsteps;
+      e->mUm.dev = m_base[3 * q + m_phase % 3];
+      e->mU.dev = m_base[3 * q + (m_phase + 1) % 3];
+      e->mUp2.dev = m_base[3 * q + (m_phase + 2) % 3];
+   }
+   return OK;
+}
+
+int Batch::step(int nsteps, bool graph)
+{
+   if (nsteps < 0) return set_error(E_INVAL, "batch: negative step count");
+   const int n = (int)m_shots.size();
+   int rc = OK;
+   // shots advanced outside the batch since the last call: start afresh
+   for (int q = 0; q < n && rc == OK; q++) {
+      EW* e = m_shots[q];
+      if (e->m_comp) return set_error(E_STATE, "batch: a shot holds an FP32 state; write it back first");
+      if (e->mUm.dev != m_base[3 * q + m_phase % 3] || e->mU.dev != m_base[3 * q + (m_phase + 1) % 3] ||
+          e->mUp2.dev != m_base[3 * q + (m_phase + 2) % 3])
+         rc = refresh();
+   }
+   if (rc) return rc;
+   {  // the clocks follow the shots' host state (t, samples) at entry
+      std::vector<double> t(n);
+      std::vector<int> smp(n);
+      for (int q = 0; q < n; q++) {
+         t[q] = m_shots[q]->mT;
+         smp[q] = m_shots[q]->m_nsamples;
+      }
+      if ((rc = check_hip(hipMemcpyAsync(m_clk_t, t.data(), sizeof(double) * n, hipMemcpyHostToDevice, m_s), "clock")) ||
+          (rc = check_hip(hipMemcpyAsync(m_clk_s, smp.data(), sizeof(int) * n, hipMemcpyHostToDevice, m_s), "clock")) ||
+          (rc = check_hip(hipStreamSynchronize(m_s), "clock")))
+         return rc;
+   }
+   m_saved.clear();
+   for (EW* e : m_shots) {
+      m_saved.push_back(e->m_stream);
+      e->m_stream = m_s;
+   }
+   int left = nsteps;
+   while (rc == OK && left > 0) {
+      if (graph && m_phase == 0 && left >= 3) {
+         rc = run_period_graph();
+         left -= 3;
+         continue;
+      }
+      rc = one_step(m_phase);
+      m_phase = (m_phase + 1) % 3;
+      left--;
+   }
+   const int rs = sync_clocks(nsteps - left);
+   for (int q = 0; q < n; q++) m_shots[q]->m_stream = m_saved[q];
+   return rc ? rc : rs;
+}
+
+int Batch::tune(int* best, double* ms, int ncand, const int* cand)
+{
+   static const int defaults[4] = {16, 32, 64, 128};
+   if (ncand <= 0 || !cand) {
+      ncand = 4;
+      cand = defaults;
+   }
+   hipEvent_t e0, e1;
+   int rc;
+   if ((rc = check_hip(hipEventCreate(&e0), "event")) || (rc = check_hip(hipEventCreate(&e1), "event"))) return rc;
+   double tbest = 1e300;
+   int kbest = m_kchunk;
+   for (int c = 0; c < ncand && rc == OK; c++) {
+      if ((rc = set_kchunk(cand[c])) || (rc = step(1, false))) break;  // warm-up
+      if ((rc = check_hip(hipEventRecord(e0, m_s), "event"))) break;
+      rc = step(2, false);
+      if (rc || (rc = check_hip(hipEventRecord(e1, m_s), "event")) ||
+          (rc = check_hip(hipEventSynchronize(e1), "event")))
+         break;
+      float t = 0;
+      if ((rc = check_hip(hipEventElapsedTime(&t, e0, e1), "event"))) break;
+      if (ms) ms[c] = t / 2;
+      if (t < tbest) {
+         tbest = t;
+         kbest = cand[c];
+      }
+   }
+   (void)hipEventDestroy(e0);
+   (void)hipEventDestroy(e1);
+   if (rc) return rc;
+   for (EW* e : m_shots)
+      if ((rc = e->reset())) return rc;
+   if (best) *best = kbest;
+   m_kchunk = kbest;
+   return refresh();
+}
+
+}  // namespace sw4amd

@@ -1,0 +1,54 @@
+// batch.h -- several shots (EW objects, grids of any size) advanced together:
+// the interior sweeps of all shots run as one launch each (the batched sweep
+// kernels of kernels_fast.hip), the per-grid boundary work follows on the
+// same stream, and the time loop can be captured once as a hipGraph and
+// replayed.
+#pragma once
+#include <vector>
+
+#include "ew.h"
+#include "kernels_fast.h"
+
+namespace sw4amd {
+
+class Batch {
+public:
+   explicit Batch(const std::vector<EW*>& shots) : m_shots(shots) {}
+   ~Batch();
+   Batch(const Batch&) = delete;
+   Batch& operator=(const Batch&) = delete;
+
+   int init();
+   int set_kchunk(int kc);
+   // Time one step per candidate k-block (planes per workgroup of the
+   // sweeps) and keep the fastest; the shots are reset to t = 0 afterwards.
+   int tune(int* best, double* ms_per_candidate, int ncand, const int* cand);
+   // Advance every shot by n steps; graph: replay a captured three-step
+   // period (the buffer rotation) where it fits.
+   int step(int n, bool graph);
+   hipStream_t stream() const { return m_s; }
+
+private:
+   int refresh();  // base pointers and clocks from the shots, tables
+   int build_tables();
+   int one_step(int phase);
+   int sync_clocks(int nsteps);
+   int run_period_graph();
+
+   std::vector<EW*> m_shots;
+   hipStream_t m_s = nullptr;
+   SweepShot* m_dtab = nullptr;     // [phase 0..2][kind 0..2][shot]
+   int m_nblk[3][3] = {{0}};
+   double* m_clk_t = nullptr;       // per shot: time, dt
+   double* m_clk_dt = nullptr;
+   int* m_clk_s = nullptr;          // per shot: sample count, maximum
+   int* m_clk_max = nullptr;
+   std::vector<double*> m_base;     // per shot: Um, U, Up2 of phase 0
+   int m_phase = 0;
+   int m_kchunk = 64;
+   hipGraphExec_t m_exec = nullptr;
+   std::vector<void*> m_allocs;
+   std::vector<hipStream_t> m_saved;
+};
+
+}  // namespace sw4amd

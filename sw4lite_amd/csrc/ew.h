@@ -38,7 +38,11 @@ struct EWOptions {
    double cfl = 1.3;    // mCFL (EW.C:163)
 };
 
+class Batch;
+
 class EW {
+   friend class Batch;
+
 public:
    explicit EW(const EWOptions& opt);
    ~EW();

@@ -4,6 +4,36 @@
 
 namespace sw4amd {
 
+struct TileMap {
+   int ntx, nty, ntiles;  // tiles along i and j, tiles in all
+};
+
+// One grid of a batched sweep: the arguments of its single-grid launch and
+// its first block id in the batched launch.
+struct SweepShot {
+   Box b;
+   TileMap map;
+   int kbeg, kend, kc, grid0;
+   const double* upin;
+   const double* u;
+   const double* um;
+   const double* mu;
+   const double* la;
+   const double* rho;
+   SgCoef sg;
+   double* out;
+   double cof, dt2i, cupd, beta;
+};
+// Fill sh for planes k1..k2 of one grid (damping: the 32 x 8 damping sweep,
+// out = the array it updates; else the predictor (corr false) or corrector
+// sweep of variant 6); returns the next free block id.
+int fill_sweep_shot(bool damping, const Box& b, int k1, int k2, int kc, const double* upin, const double* u,
+                    const double* um, const double* mu, const double* la, const double* rho, const SgCoef& sg,
+                    double* out, double h, double dt, double beta, bool corr, int grid0, SweepShot* sh);
+// kind 0 predictor, 1 corrector, 2 damping sweeps of nshots grids in one
+// launch of nblocks workgroups (d_shots in device memory).
+int launch_sweep_batch(int kind, const SweepShot* d_shots, int nshots, int nblocks, hipStream_t s);
+
 // Interior planes k1..k2 of the predictor (corr = false: out = Up from u, um)
 // or corrector sweep (corr = true: out = Up' from upin = Up, u, um), chunks of
 // kc planes per workgroup.

@@ -22,9 +22,6 @@ constexpr int TX = 32, TY = 8, HX = TX + 4, HY = TY + 4, NT = TX * TY, PL = HX *
 inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 }  // namespace
 
-struct TileMap {
-   int ntx, nty, ntiles;  // tiles along i and j, tiles in all
-};
 
 struct LdsSten {
    const double* p0;  // ring slot of plane k-2, offset to this thread's point
@@ -62,12 +59,13 @@ struct LdsSten {
 // the next iteration.  The compute phase itself reads only LDS and registers:
 // the load counter is in order, so a global load issued after the prefetch
 // and waited on in the same iteration would wait for the prefetch as well.
-template <bool CORR, bool PIPE, int MINW, bool DAMP = false>
-__global__ void __launch_bounds__(NT, MINW)
-k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __restrict__ upin,
-            const double* __restrict__ u, const double* __restrict__ um, const double* __restrict__ mu,
-            const double* __restrict__ la, const double* __restrict__ rho, SgCoef sg, double* __restrict__ out,
-            double cof, double dt2i, double cupd, double beta)
+template <bool CORR, bool PIPE, bool DAMP>
+__device__ __forceinline__ void sweep_lds_body(unsigned bid, Box b, TileMap map, int kbeg, int kend, int kc,
+                                               const double* __restrict__ upin, const double* __restrict__ u,
+                                               const double* __restrict__ um, const double* __restrict__ mu,
+                                               const double* __restrict__ la, const double* __restrict__ rho,
+                                               SgCoef sg, double* __restrict__ out, double cof, double dt2i,
+                                               double cupd, double beta)
 {
    static_assert(!DAMP || CORR, "damping belongs to the corrector");
    constexpr int NU = (3 * PL + NT - 1) / NT;  // staged displacement values per thread
@@ -84,7 +82,7 @@ k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __rest
    // contiguous run of tiles (i fastest, then j, then k-chunk).  Tiles that
    // share halos then run on the same XCD at the same time and meet in its L2.
    const int per = (int)((map.ntiles + 7) / 8);
-   const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+   const int t = (int)(bid % 8) * per + (int)(bid / 8);
    if (t >= map.ntiles) return;
    const int tix = t % map.ntx, tr = t / map.ntx, tjy = tr % map.nty, tkz = tr / map.nty;
    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
@@ -351,6 +349,17 @@ k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __rest
    }
 }
 
+template <bool CORR, bool PIPE, int MINW, bool DAMP = false>
+__global__ void __launch_bounds__(NT, MINW)
+k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __restrict__ upin,
+            const double* __restrict__ u, const double* __restrict__ um, const double* __restrict__ mu,
+            const double* __restrict__ la, const double* __restrict__ rho, SgCoef sg, double* __restrict__ out,
+            double cof, double dt2i, double cupd, double beta)
+{
+   sweep_lds_body<CORR, PIPE, DAMP>(blockIdx.x, b, map, kbeg, kend, kc, upin, u, um, mu, la, rho, sg, out, cof, dt2i,
+                                    cupd, beta);
+}
+
 // Supergrid damping of the corrector on planes kbeg..kend as a k-marching LDS
 // sweep (ew-cfromfort.C:985-1041): up -= damping(u - um).  d = u - um of plane
 // k and rho of plane k are staged with their (i,j) halo in LDS; the thread's
@@ -358,9 +367,10 @@ k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __rest
 // and register footprint, so several workgroups share a CU and hide latency.
 // Tiles of 32 x DTY columns (DTY * 32 threads); taller tiles re-read less halo.
 template <int DTY>
-__global__ void __launch_bounds__(32 * DTY)
-k_damp_lds(Box b, TileMap map, int kbeg, int kend, int kc, double* __restrict__ up, const double* __restrict__ u,
-           const double* __restrict__ um, const double* __restrict__ rho, SgCoef sg, double beta)
+__device__ __forceinline__ void damp_lds_body(unsigned bid, Box b, TileMap map, int kbeg, int kend, int kc,
+                                              double* __restrict__ up, const double* __restrict__ u,
+                                              const double* __restrict__ um, const double* __restrict__ rho,
+                                              SgCoef sg, double beta)
 {
    constexpr int TY = DTY, HY = TY + 4, NT = TX * TY, PL = HX * HY;
    constexpr int NU = (3 * PL + NT - 1) / NT;
@@ -368,7 +378,7 @@ k_damp_lds(Box b, TileMap map, int kbeg, int kend, int kc, double* __restrict__ 
    __shared__ double sdp[3 * PL];
    __shared__ double srp[PL];
    const int per = (int)((map.ntiles + 7) / 8);
-   const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+   const int t = (int)(bid % 8) * per + (int)(bid / 8);
    if (t >= map.ntiles) return;
    const int tix = t % map.ntx, tr = t / map.ntx, tjy = tr % map.nty, tkz = tr / map.nty;
    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
@@ -462,6 +472,81 @@ k_damp_lds(Box b, TileMap map, int kbeg, int kend, int kc, double* __restrict__ 
          for (int c = 0; c < 3; c++) dq[q][c] = dq[q + 1][c];
       rkm = r0;
    }
+}
+
+template <int DTY>
+__global__ void __launch_bounds__(32 * DTY)
+k_damp_lds(Box b, TileMap map, int kbeg, int kend, int kc, double* __restrict__ up, const double* __restrict__ u,
+           const double* __restrict__ um, const double* __restrict__ rho, SgCoef sg, double beta)
+{
+   damp_lds_body<DTY>(blockIdx.x, b, map, kbeg, kend, kc, up, u, um, rho, sg, beta);
+}
+
+// ---------------------------------------------------------------------------
+// Batched sweeps: the tiles of several grids in one launch.  Shot s owns the
+// block ids [grid0, grid0 + 8 ceil(ntiles/8)), so the XCD of a block (its id
+// mod 8) is the same as in a launch of that shot alone.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int find_shot(const SweepShot* __restrict__ shots, int nshots, unsigned bid)
+{
+   int s = 0;
+   while (s + 1 < nshots && bid >= (unsigned)shots[s + 1].grid0) s++;
+   return s;
+}
+
+template <bool CORR>
+__global__ void __launch_bounds__(NT, 2) k_sweep_batch(const SweepShot* __restrict__ shots, int nshots)
+{
+   const SweepShot& h = shots[find_shot(shots, nshots, blockIdx.x)];
+   sweep_lds_body<CORR, false, false>(blockIdx.x - h.grid0, h.b, h.map, h.kbeg, h.kend, h.kc, h.upin, h.u, h.um, h.mu,
+                                      h.la, h.rho, h.sg, h.out, h.cof, h.dt2i, h.cupd, h.beta);
+}
+
+__global__ void __launch_bounds__(32 * 8) k_damp_batch(const SweepShot* __restrict__ shots, int nshots)
+{
+   const SweepShot& h = shots[find_shot(shots, nshots, blockIdx.x)];
+   damp_lds_body<8>(blockIdx.x - h.grid0, h.b, h.map, h.kbeg, h.kend, h.kc, h.out, h.u, h.um, h.rho, h.sg, h.beta);
+}
+
+int fill_sweep_shot(bool damping, const Box& b, int k1, int k2, int kc, const double* upin, const double* u,
+                    const double* um, const double* mu, const double* la, const double* rho, const SgCoef& sg,
+                    double* out, double h, double dt, double beta, bool corr, int grid0, SweepShot* sh)
+{
+   const long nx = b.ie - b.ib - 3, ny = b.je - b.jb - 3;
+   if (kc <= 0) kc = 64;
+   sh->b = b;
+   sh->map.ntx = (int)cdiv(nx, TX);
+   sh->map.nty = (int)cdiv(ny, damping ? 8 : TY);
+   sh->map.ntiles = k2 >= k1 ? sh->map.ntx * sh->map.nty * (int)cdiv(k2 - k1 + 1, kc) : 0;
+   sh->kbeg = k1;
+   sh->kend = k2;
+   sh->kc = kc;
+   sh->upin = upin;
+   sh->u = u;
+   sh->um = um;
+   sh->mu = mu;
+   sh->la = la;
+   sh->rho = rho;
+   sh->sg = sg;
+   sh->out = out;
+   sh->cof = 1.0 / (h * h);
+   sh->dt2i = 1.0 / (dt * dt);
+   sh->cupd = corr ? dt * dt * dt * dt / 12 : dt * dt;
+   sh->beta = beta;
+   sh->grid0 = grid0;
+   return grid0 + 8 * (int)cdiv(sh->map.ntiles, 8);
+}
+
+int launch_sweep_batch(int kind, const SweepShot* d_shots, int nshots, int nblocks, hipStream_t s)
+{
+   if (nblocks <= 0) return OK;
+   if (kind == 0)
+      k_sweep_batch<false><<<nblocks, NT, 0, s>>>(d_shots, nshots);
+   else if (kind == 1)
+      k_sweep_batch<true><<<nblocks, NT, 0, s>>>(d_shots, nshots);
+   else
+      k_damp_batch<<<nblocks, 32 * 8, 0, s>>>(d_shots, nshots);
+   return check_hip(hipGetLastError(), "batched sweep launch");
 }
 
 int launch_damp_lds(const Box& b, int k1, int k2, int kc, double* up, const double* u, const double* um,

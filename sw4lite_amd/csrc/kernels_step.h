@@ -32,15 +32,28 @@ int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables
                       double* upo, double* acc, double h, double dt, hipStream_t s,
                       const std::function<int()>& after_interior, const std::function<int()>& after_damping,
                       int variant = 1, int kchunk = 32);
+// Rows next to a free surface and the ghost shell of the fused sweeps
+// (variant >= 1), after the interior sweep.
+int launch_pred_tail(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp, const double* u,
+                     const double* um, const double* mu, const double* la, const double* rho, const SgCoef& sg,
+                     double* up, double h, double dt, hipStream_t s);
+int launch_corr_tail(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp, const double* up,
+                     const double* u, const double* um, const double* mu, const double* la, const double* rho,
+                     const SgCoef& sg, double beta, double* upo, double h, double dt, hipStream_t s);
+// tdev: when not null the time is read from device memory (graph replay)
 int launch_src_apply(const Box& b, const SourceTable& st, double t, int deriv2, double coef,
-                     const double* rho, double* up, hipStream_t s);
+                     const double* rho, double* up, hipStream_t s, const double* tdev = nullptr);
 int launch_maxeig(long n, const double* mu, const double* la, const double* rho, double* partial,
                   int npartial, double* out, hipStream_t s);
 int launch_exact(const Box& b, const ExactParams& P, double* out, hipStream_t s);
 int launch_solerr(const Box& b, const NormWindow& w, const double* uex, const double* u,
                   double* partial, int npartial, double* out, hipStream_t s);
 long shell_count(const Box& b);
+// sdev: when not null, out is the start of the sample buffer and the sample
+// index is read from device memory (samples >= maxs are not recorded)
 int launch_record(const Box& b, int nrec, const int* rec, const double* up, const double* um, double dt,
-                  double* out, hipStream_t s);
+                  double* out, hipStream_t s, const int* sdev = nullptr, int maxs = 0);
+// t[q] += dt[q]; smp[q] += 1 while below maxs[q]
+int launch_clock_tick(int n, double* t, int* smp, const double* dt, const int* maxs, hipStream_t s);
 
 }  // namespace sw4amd

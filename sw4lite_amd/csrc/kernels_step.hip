@@ -177,10 +177,11 @@ long shell_count(const Box& b)
 // Sparse source update: up(:,p) += coef/rho(p) * sum_s f_s g_s(t)
 // ---------------------------------------------------------------------------
 __global__ void k_src_apply(Box b, SourceTable st, double t, int deriv2, double coef,
-                            const double* __restrict__ rho, double* __restrict__ up)
+                            const double* __restrict__ rho, double* __restrict__ up, const double* __restrict__ tdev)
 {
    const int r = blockIdx.x * blockDim.x + threadIdx.x;
    if (r >= st.nunique) return;
+   if (tdev) t = *tdev;
    const int s0 = st.first[r], s1 = st.first[r + 1];
    const int i = st.ijk[3 * s0], j = st.ijk[3 * s0 + 1], k = st.ijk[3 * s0 + 2];
    double f0 = 0, f1 = 0, f2 = 0;
@@ -428,10 +429,16 @@ __global__ void __launch_bounds__(RED_BLOCK) k_final_norm(int n, const double* _
 // rec[4r..4r+3] = i, j, k, mode; out[3r+c] for this sample.
 // ---------------------------------------------------------------------------
 __global__ void k_record(Box b, int nrec, const int* __restrict__ rec, const double* __restrict__ up,
-                         const double* __restrict__ um, double dt, double* __restrict__ out)
+                         const double* __restrict__ um, double dt, double* __restrict__ out,
+                         const int* __restrict__ sdev, int maxs)
 {
    const int r = blockIdx.x * blockDim.x + threadIdx.x;
    if (r >= nrec) return;
+   if (sdev) {  // sample index from the device clock
+      const int smp = *sdev;
+      if (smp >= maxs) return;
+      out += (size_t)3 * nrec * smp;
+   }
    const long p = sidx(b, rec[4 * r], rec[4 * r + 1], rec[4 * r + 2]);
    const int mode = rec[4 * r + 3];
 #pragma unroll
@@ -445,11 +452,26 @@ __global__ void k_record(Box b, int nrec, const int* __restrict__ rec, const dou
 // launchers
 // ===========================================================================
 int launch_record(const Box& b, int nrec, const int* rec, const double* up, const double* um, double dt,
-                  double* out, hipStream_t s)
+                  double* out, hipStream_t s, const int* sdev, int maxs)
 {
    if (nrec <= 0) return OK;
-   k_record<<<cdiv(nrec, 64), 64, 0, s>>>(b, nrec, rec, up, um, dt, out);
+   k_record<<<cdiv(nrec, 64), 64, 0, s>>>(b, nrec, rec, up, um, dt, out, sdev, maxs);
    return check_hip(hipGetLastError(), "receiver launch");
+}
+
+__global__ void k_clock_tick(int n, double* __restrict__ t, int* __restrict__ smp, const double* __restrict__ dt,
+                             const int* __restrict__ maxs)
+{
+   const int q = blockIdx.x * blockDim.x + threadIdx.x;
+   if (q >= n) return;
+   t[q] += dt[q];
+   if (smp[q] < maxs[q]) smp[q] += 1;
+}
+
+int launch_clock_tick(int n, double* t, int* smp, const double* dt, const int* maxs, hipStream_t s)
+{
+   if (n > 0) k_clock_tick<<<cdiv(n, 64), 64, 0, s>>>(n, t, smp, dt, maxs);
+   return check_hip(hipGetLastError(), "clock launch");
 }
 
 int launch_pred_fused(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp,
@@ -474,13 +496,9 @@ int launch_pred_fused(const Box& b, int nk, const int* onesided, const SbpTables
       const int rc = after_interior();
       if (rc) return rc;
    }
+   if (variant >= 1) return launch_pred_tail(b, nk, onesided, d_sbp, u, um, mu, la, rho, sg, up, h, dt, s);
    for (int side = 4; side <= 5; side++) {
       if (onesided[side] != 1) continue;
-      if (variant >= 1) {  // LDS slab of the six rows
-         const int rc = launch_sbp_rows(false, b, side, nk, d_sbp, nullptr, u, um, mu, la, rho, sg, up, h, dt, 0.0, s);
-         if (rc) return rc;
-         continue;
-      }
       dim3 grd(cdiv(nii, 64), cdiv(njj, 4), 6);
       k_pred_sbp<<<grd, blk, 0, s>>>(b, side == 4, nk, d_sbp, u, um, mu, la, rho, sg.strx, sg.stry, up, cof, dt2);
    }
@@ -490,6 +508,34 @@ int launch_pred_fused(const Box& b, int nk, const int* onesided, const SbpTables
 }
 
 long accel_slab_size(const Box& b) { return 3 * b.nij * SLAB_PLANES; }
+
+int launch_pred_tail(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp, const double* u,
+                     const double* um, const double* mu, const double* la, const double* rho, const SgCoef& sg,
+                     double* up, double h, double dt, hipStream_t s)
+{
+   for (int side = 4; side <= 5; side++) {  // LDS slab of the six rows
+      if (onesided[side] != 1) continue;
+      const int rc = launch_sbp_rows(false, b, side, nk, d_sbp, nullptr, u, um, mu, la, rho, sg, up, h, dt, 0.0, s);
+      if (rc) return rc;
+   }
+   const long ns = shell_count(b);
+   k_shell<<<cdiv(ns, 256), 256, 0, s>>>(b, 0, nullptr, u, um, up);
+   return check_hip(hipGetLastError(), "predictor rows and shell launch");
+}
+
+int launch_corr_tail(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp, const double* up,
+                     const double* u, const double* um, const double* mu, const double* la, const double* rho,
+                     const SgCoef& sg, double beta, double* upo, double h, double dt, hipStream_t s)
+{
+   for (int side = 4; side <= 5; side++) {  // LDS slab, acceleration formed while staging
+      if (onesided[side] != 1) continue;
+      const int rc = launch_sbp_rows(true, b, side, nk, d_sbp, up, u, um, mu, la, rho, sg, upo, h, dt, beta, s);
+      if (rc) return rc;
+   }
+   const long ns = shell_count(b);
+   k_shell<<<cdiv(ns, 256), 256, 0, s>>>(b, 1, up, nullptr, nullptr, upo);
+   return check_hip(hipGetLastError(), "corrector rows and shell launch");
+}
 
 int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp,
                       const double* up, const double* u, const double* um, const double* mu,
@@ -525,13 +571,9 @@ int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables
       const int rc = after_damping();
       if (rc) return rc;
    }
+   if (variant >= 1) return launch_corr_tail(b, nk, onesided, d_sbp, up, u, um, mu, la, rho, sg, beta, upo, h, dt, s);
    for (int side = 4; side <= 5; side++) {
       if (onesided[side] != 1) continue;
-      if (variant >= 1) {  // LDS slab of the six rows, acceleration formed while staging
-         const int rc = launch_sbp_rows(true, b, side, nk, d_sbp, up, u, um, mu, la, rho, sg, upo, h, dt, beta, s);
-         if (rc) return rc;
-         continue;
-      }
       const int ks = side == 4 ? 0 : nk - SLAB_PLANES + 2;  // planes KK(0)..KK(8) of the rows
       k_accel_slab<<<cdiv(accel_slab_size(b), 256), 256, 0, s>>>(b, ks, up, u, um, dt2i, acc);
       dim3 grd(cdiv(nii, 64), cdiv(njj, 4), 6);
@@ -544,10 +586,10 @@ int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables
 }
 
 int launch_src_apply(const Box& b, const SourceTable& st, double t, int deriv2, double coef,
-                     const double* rho, double* up, hipStream_t s)
+                     const double* rho, double* up, hipStream_t s, const double* tdev)
 {
    if (st.nunique <= 0) return OK;
-   k_src_apply<<<cdiv(st.nunique, 128), 128, 0, s>>>(b, st, t, deriv2, coef, rho, up);
+   k_src_apply<<<cdiv(st.nunique, 128), 128, 0, s>>>(b, st, t, deriv2, coef, rho, up, tdev);
    return check_hip(hipGetLastError(), "source update launch");
 }
 

@@ -370,3 +370,43 @@ class EW:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Several set-up EW shots of any grid size advanced together (batched
+    sweeps, optional hipGraph replay of the time loop)."""
+
+    def __init__(self, shots):
+        self.lib = L.load()
+        self.shots = list(shots)
+        arr = (C.c_void_p * len(self.shots))(*[s.h for s in self.shots])
+        h = C.c_void_p()
+        L.check(self.lib.sw4hip_batch_create(C.cast(arr, C.c_void_p), len(self.shots), C.byref(h)))
+        self.h = h
+
+    def set_kchunk(self, kc):
+        L.check(self.lib.sw4hip_batch_set_kchunk(self.h, int(kc)))
+
+    def tune(self, candidates=(16, 32, 64, 128)):
+        """Autotune the k-block; returns (best, {candidate: ms per step}).
+        Resets every shot to zero data at t = 0."""
+        cand = np.asarray(candidates, dtype=np.int32)
+        ms = np.zeros(len(cand))
+        best = C.c_int(0)
+        L.check(self.lib.sw4hip_batch_tune(self.h, len(cand), _ptr(cand), C.byref(best), _ptr(ms)))
+        return best.value, dict(zip(cand.tolist(), ms.tolist()))
+
+    def step(self, nsteps, graph=True):
+        L.check(self.lib.sw4hip_batch_step(self.h, int(nsteps), int(bool(graph))))
+
+    def close(self):
+        if self.h:
+            self.lib.sw4hip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

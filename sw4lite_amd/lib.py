@@ -48,6 +48,11 @@ _SIGS = {
     "sw4hip_addsgd4_adjoint": (I, [I] * 6 + [P] * 13 + [D, P, P, P]),
     "sw4hip_bcfortsg_adjoint": (I, [I] * 6 + [P, I, I, I, P, P, D, P, P, P, P, P, P, P, P, P]),
     "sw4hip_ew_reset": (I, [P]),
+    "sw4hip_batch_create": (I, [P, I, C.POINTER(P)]),
+    "sw4hip_batch_destroy": (I, [P]),
+    "sw4hip_batch_set_kchunk": (I, [P, I]),
+    "sw4hip_batch_tune": (I, [P, I, P, IP, P]),
+    "sw4hip_batch_step": (I, [P, I, I]),
     "sw4hip_ew_gradient": (I, [P, I, I, P, P, P, P, I]),
     "sw4hip_host_moment_source_curv": (I, [C.POINTER(GridDesc), P, D, D, D, P, I, IP, P, P]),
     "sw4hip_ew_set_topography": (I, [P, P, I]),

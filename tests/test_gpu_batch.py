@@ -1,0 +1,103 @@
+"""GPU: batched shots (sw4hip_batch_*).  Grids of different sizes advanced
+together must give, shot by shot, the solution and receiver traces of the
+same shot stepped alone with the fused step (same kernels, same order per
+grid: the bar is 1e-13 relative).  The captured-graph path is checked after
+every replay, with eager work (solution read-back, a partial period) between
+replays."""
+import numpy as np
+import pytest
+
+from common import relerr
+from sw4lite_amd import lib as L
+from test_gpu_parity import SOURCES, layered_material
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-13
+
+SHAPES = [(30, 28, 26, 0.08, [2, 2, 2, 2, 0, 2]), (24, 22, 20, 0.1, [2, 2, 2, 2, 2, 2]),
+          (36, 20, 24, 0.09, [1, 2, 2, 1, 0, 0])]
+RECS = [(1.0, 1.05, 0.0, 0), (1.53, 0.77, 0.41, 1), (0.7, 0.6, 0.9, 0)]
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    assert _t.cuda.is_available(), "GPU tests need a visible GPU"
+    L.load()
+    return _t
+
+
+def make_shots(nsteps):
+    from sw4lite_amd import EW
+    shots = []
+    for q, (nx, ny, nz, h, bc) in enumerate(SHAPES):
+        rho, mu, la = layered_material(nx, ny, nz, h, 20 + q)
+        e = EW(nx, ny, nz, h, bc, corder=1, sg_gp_thickness=5)
+        e.set_material(rho, mu, la)
+        for s in SOURCES[: 1 + q % 3]:
+            e.add_moment_source(s["x"], s["y"], s["z"], s["m"], s["t0"], s["freq"], s["tfkind"])
+        for x, y, z, m in RECS:
+            e.add_receiver(x, y, z, m)
+        e.setup(nsteps=nsteps)
+        shots.append(e)
+    return shots
+
+
+def reference(nsteps, checkpoints):
+    """Solutions after each checkpoint step count and the traces, shots alone."""
+    shots = make_shots(nsteps)
+    out = {}
+    done = 0
+    for c in checkpoints:
+        for e in shots:
+            e.timesteploop(c - done, mode=0)
+        done = c
+        out[c] = [e.get_solution()[0] for e in shots]
+    tr = [e.receivers() for e in shots]
+    for e in shots:
+        e.close()
+    return out, tr
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_matches_single_shots(torch, graph):
+    from sw4lite_amd.ew import Batch
+    plan = [4, 3, 3, 1, 3]  # crosses the three-step period in every way
+    marks = list(np.cumsum(plan))
+    want, want_tr = reference(int(marks[-1]), marks)
+    shots = make_shots(int(marks[-1]))
+    b = Batch(shots)
+    for n, m in zip(plan, marks):
+        b.step(n, graph=graph)
+        for q, e in enumerate(shots):
+            u = e.get_solution()[0]  # eager work between replays
+            err = relerr(u, want[m][q])
+            print("batch graph", graph, "after", m, "steps, shot", q, "relerr", err, flush=True)
+            assert e.info().cycle == m
+            assert err < TOL
+    for q, e in enumerate(shots):
+        got = e.receivers()
+        assert got.shape == want_tr[q].shape
+        assert relerr(got, want_tr[q]) < TOL
+    b.close()
+    for e in shots:
+        e.close()
+
+
+def test_batch_autotuner(torch):
+    from sw4lite_amd.ew import Batch
+    shots = make_shots(6)
+    b = Batch(shots)
+    best, ms = b.tune((16, 32, 64))
+    print("batch autotuner", best, ms, flush=True)
+    assert best in (16, 32, 64) and all(v > 0 for v in ms.values())
+    for e in shots:  # reset to zero data at t = 0
+        u, um = e.get_solution()
+        assert e.info().cycle == 0 and np.max(np.abs(u)) == 0 and np.max(np.abs(um)) == 0
+    b.step(6)
+    want, _ = reference(6, [6])
+    for q, e in enumerate(shots):
+        assert relerr(e.get_solution()[0], want[6][q]) < TOL
+    b.close()
+    for e in shots:
+        e.close()
