@@ -67,18 +67,21 @@ def parse():
     p.add_argument("--kchunk", type=int, default=64, help="planes per workgroup of the LDS sweeps")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also launch a plain device copy of 1 GiB (PMC scaling)")
-    p.add_argument("--workload", choices=["cartesian", "curvilinear", "precision", "adjoint"], default="cartesian",
+    p.add_argument("--workload", choices=["cartesian", "curvilinear", "precision", "adjoint", "batch"],
+                   default="cartesian",
                    help="cartesian: the full time step at N^3 (default); curvilinear: the curvilinear-grid "
                         "kernels (rhs4sgcurv, addsgd4c, free-surface closure) on a 384x384x256 grid under "
                         "Gaussian hills; precision: FP32 compensated against FP64 on 64 surface receivers "
                         "(1024^3, h=10 m, 800 steps); adjoint: gradient of a receiver misfit against the forward run "
-                        "(256^3, h=25 m, 600 steps, 128 surface receivers)")
+                        "(256^3, h=25 m, 600 steps, 128 surface receivers); batch: 48 shots of N^3, N uniform in "
+                        "128..288, h=25 m, stepped alone, batched, batched with graph replay")
     p.add_argument("--ckpt", type=int, default=50, help="adjoint workload: steps between checkpoints")
+    p.add_argument("--shots", type=int, default=48, help="batch workload: number of shots")
     a = p.parse_args()
     if a.n is None:
         a.n = {"precision": 1024, "adjoint": 256}.get(a.workload, 512)
     if a.steps is None:
-        a.steps = {"precision": 800, "adjoint": 600}.get(a.workload, 100)
+        a.steps = {"precision": 800, "adjoint": 600, "batch": 30}.get(a.workload, 100)
     return a
 
 
@@ -521,6 +524,77 @@ def main_adjoint(a, torch):
     print(json.dumps(line), flush=True)
 
 
+def main_batch(a, torch):
+    """Batched shots: --shots grids of N^3 points, N drawn uniformly from
+    128..288 (seeded), h = 25 m, workload material, one shallow double couple
+    and 128 surface receivers each.  The same number of steps three ways:
+    every shot stepped alone (fused step per shot), batched (one launch per
+    sweep for all shots), batched with the three-step period replayed as a
+    hipGraph; the k-block is autotuned first.  value = batched + graph
+    grid-point updates per second over all shots."""
+    from sw4lite_amd import EW
+    from sw4lite_amd import lib as L
+    from sw4lite_amd.ew import Batch
+    rng = np.random.default_rng(8)
+    h, steps = 25.0, a.steps
+    sizes = [int(v) for v in rng.integers(128, 289, size=a.shots)]
+    shots = []
+    for q, n in enumerate(sizes):
+        rho, mu, la = make_material(torch, n, h, seed=100 + q)
+        e = EW(n, n, n, h, [L.BC_SUPERGRID] * 4 + [L.BC_FREE, L.BC_SUPERGRID], corder=1, sg_gp_thickness=30)
+        e.set_material(rho, mu, la)
+        e.sync()
+        del rho, mu, la
+        span = (n - 1) * h
+        e.add_moment_source(rng.uniform(0.3, 0.7) * span, rng.uniform(0.3, 0.7) * span, rng.uniform(200, 1000),
+                            double_couple(rng, 1e15), 0.24, 5.0, 0)
+        for _ in range(128):
+            e.add_receiver(rng.uniform(0.15, 0.85) * span, rng.uniform(0.15, 0.85) * span, 0.0, 0)
+        e.setup(nsteps=3 * steps + 10)
+        shots.append(e)
+    torch.cuda.empty_cache()
+    pts = float(sum(n ** 3 for n in sizes))
+    b = Batch(shots)
+    best, tune_ms = b.tune((16, 32, 64, 128))
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    def alone():
+        for e in shots:
+            e.timesteploop(steps, mode=0)
+        for e in shots:
+            e.sync()
+
+    b.step(3, graph=True)  # warm-up and graph capture
+    t_alone = timed(alone)
+    t_eager = timed(lambda: b.step(steps, graph=False))
+    t_graph = timed(lambda: b.step(steps, graph=True))
+    b.close()
+    for e in shots:
+        e.close()
+    line = {
+        "metric": "batched shots: grid-pt updates/s over all shots (FP64 fused step)",
+        "value": pts * steps / t_graph, "unit": "grid-pt updates/s", "n_gpus": 1, "steps": steps, "warmup": 3,
+        "ms_per_step": 1e3 * t_graph / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {
+            "workload": "%d shots of N^3, N uniform in 128..288 (seeded), h=25 m, layered+lognormal material, free "
+                        "top, supergrid 30, one double couple, 128 surface receivers each" % a.shots,
+            "sizes": sizes, "total_points": pts, "autotuned_kchunk": best, "tune_ms_per_step": tune_ms,
+            "alone_updates_per_s": pts * steps / t_alone, "batched_updates_per_s": pts * steps / t_eager,
+            "graph_updates_per_s": pts * steps / t_graph,
+            "ms_per_step": {"alone": 1e3 * t_alone / steps, "batched": 1e3 * t_eager / steps,
+                            "batched_graph": 1e3 * t_graph / steps},
+        },
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -538,6 +612,8 @@ def main():
         return main_precision(a, torch)
     if a.workload == "adjoint":
         return main_adjoint(a, torch)
+    if a.workload == "batch":
+        return main_batch(a, torch)
     n, h = a.n, 20.0
     srcs = sources(n, h, seed=2)
     rho, mu, la = make_material(torch, n, h, seed=2)

@@ -345,16 +345,9 @@ int sw4hip_rhs4sg_matgrad(int ifirst, int ilast, int jfirst, int jlast, int kfir
    const SbpTables* d = nullptr;
    if ((rc = sbp_from(acof, bope, ghcof, &d))) return rc;
    const Box b = make_box(ifirst, ilast, jfirst, jlast, kfirst, klast);
-   void* work = nullptr;
-   if (hipMalloc(&work, sizeof(double) * adj_pad_doubles(b)) != hipSuccess) {
-      (void)hipGetLastError();
-      return set_error(E_NOMEM, "rhs4sg_matgrad: scratch allocation failed");
-   }
-   const AdjPad pad = make_adj_pad(b, static_cast<double*>(work));
    rc = launch_rhs4sg_matgrad(b, make_rowmap(b, nk, onesided), d, z, u, mu, lambda, h, strx, stry, strz, gmu, glambda,
-                              pad, (hipStream_t)stream);
+                              (hipStream_t)stream);
    const int rs = check_hip(hipStreamSynchronize((hipStream_t)stream), "rhs4sg_matgrad");
-   (void)hipFree(work);
    return rc ? rc : rs;
 }
 

@@ -12,6 +12,7 @@
 // factors with respect to the material.  The reverse sweep recomputes the
 // forward states of one checkpoint segment at a time.
 #include <cmath>
+#include <map>
 #include <vector>
 
 #include "ew.h"
@@ -56,14 +57,14 @@ int EW::fused_once(double t)
 }
 
 // Reverse of step n: U = U^n, Um = U^{n-1}, U1 = U^{n+1}; lam_a (adjoint of
-// U^{n+1}) is consumed; lam_b += dJ/dU^n and lam_c += dJ/dU^{n-1} through
-// this step.  Of the forward step only Up is recomputed (fused predictor,
-// sources, BC).  The 1/rho derivatives use state differences on the
-// interior, where every grid-point source lies:
+// U^{n+1}) is consumed and used as work space; lam_b += dJ/dU^n and
+// lam_c = dJ/dU^{n-1} through this step (overwritten).  Of the forward step
+// only Up is recomputed (fused predictor, sources, BC).  The 1/rho derivatives
+// use state differences on the interior, where every grid-point source lies:
 //   dt^2/rho (L(U) + F)        = Up - 2U + Um = dt^2 A,
 //   c4/rho (L(A) + F_tt) - D   = U^{n+1} - Up,
 // so d/drho [y . c4/rho (...)] + d/drho [-y . D]|_(1/rho) = -y . (U^{n+1} - Up)/rho.
-// tmp: 7 work fields of 3 components; pad: padded material (kernels_adj.h).
+// tmp: 6 work fields of 3 components; pad: padded material (kernels_adj.h).
 int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double* lam_b, double* lam_c,
                  double* grho, double* gmu, double* gla, double* const* tmp, double* plane, const AdjPad& pad)
 {
@@ -73,7 +74,9 @@ int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double
    const double dt = mDt, t = n * dt, dt2 = dt * dt, c4 = dt2 * dt2 / 12;
    const long n3 = 3 * m_box.npts;
    const RowMap rm = make_rowmap(m_box, g.nz, m_onesided);
-   double *Up = tmp[0], *Acc = tmp[1], *Y = tmp[2], *Db = tmp[3], *Yp = tmp[4], *Z = tmp[5], *LT = tmp[6];
+   double *Up = tmp[0], *Acc = tmp[1], *Db = tmp[2], *Z = tmp[3], *LT1 = tmp[4], *LT2 = tmp[5];
+   // Y: the adjoint of the corrected field, then (in place) of Up
+   double* Y = lam_a;
    Sarray UpA = mU;
    UpA.dev = Up;
    int rc;
@@ -84,39 +87,28 @@ int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double
        (rc = enforceBC(UpA, mMu, mLambda, t + dt)) || (rc = launch_dpdmt(m_box, Up, u, um, Acc, 1.0 / dt2, m_stream)))
       return rc;
    // corrector BC, transposed (acting on U^{n+1})
-   if ((rc = check_hip(hipMemcpyAsync(Y, lam_a, sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream), "adjoint copy")) ||
-       (rc = launch_bcfortsg_T(m_box, m_wind, g.nx, g.ny, g.nz, g.bc, Y, u1, g.h, m_dsbp, mMu.dev, mLambda.dev, c.strx,
+   if ((rc = launch_bcfortsg_T(m_box, m_wind, g.nx, g.ny, g.nz, g.bc, Y, u1, g.h, m_dsbp, mMu.dev, mLambda.dev, c.strx,
                                c.stry, gmu, gla, plane, m_stream)))
       return rc;
-   // damping: U^{n+1} -= D(U - Um)
+   // damping: U^{n+1} -= D(U - Um); corrector: Up + c4/rho (L(A) + F_tt)
    if ((rc = launch_addsgd4_T_pad(m_box, Y, u, um, mRho.dev, c, m_opt.beta, Db, grho, pad, m_stream, false)) ||
-       (rc = launch_axpy(n3, 1.0, Db, lam_b, m_stream)) || (rc = launch_axpy(n3, -1.0, Db, lam_c, m_stream)))
-      return rc;
-   // corrector: Up + c4/rho (L(A) + F_tt)
-   if ((rc = check_hip(hipMemcpyAsync(Yp, Y, sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream), "adjoint copy")) ||
-       (rc = launch_rows_scale(m_box, c4, mRho.dev, Y, Z, m_stream)) ||
-       (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, LT, pad, m_stream)) ||
+       (rc = launch_vjp_corr_pw(m_box, c4, mRho.dev, Y, u1, Up, Z, grho, m_stream)) ||
+       (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, LT1, pad, m_stream)) ||
        (rc = launch_rhs4sg_matgrad(m_box, rm, m_dsbp, Z, Acc, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, gmu,
-                                   gla, pad, m_stream)) ||
-       (rc = launch_rho_diff(m_box, 1.0, Y, u1, Up, mRho.dev, grho, m_stream)))
+                                   gla, m_stream)))
       return rc;
-   // A = (Up - 2U + Um)/dt^2
-   const double i2 = 1.0 / dt2;
-   if ((rc = launch_axpy(n3, i2, LT, Yp, m_stream)) || (rc = launch_axpy(n3, -2 * i2, LT, lam_b, m_stream)) ||
-       (rc = launch_axpy(n3, i2, LT, lam_c, m_stream)))
-      return rc;
+   // A = (Up - 2U + Um)/dt^2: the Up part
+   if ((rc = launch_axpy(n3, 1.0 / dt2, LT1, Y, m_stream))) return rc;
    // predictor BC, transposed (acting on Up)
-   if ((rc = launch_bcfortsg_T(m_box, m_wind, g.nx, g.ny, g.nz, g.bc, Yp, Up, g.h, m_dsbp, mMu.dev, mLambda.dev, c.strx,
+   if ((rc = launch_bcfortsg_T(m_box, m_wind, g.nx, g.ny, g.nz, g.bc, Y, Up, g.h, m_dsbp, mMu.dev, mLambda.dev, c.strx,
                                c.stry, gmu, gla, plane, m_stream)))
       return rc;
-   // predictor: 2U - Um + dt^2/rho (L(U) + F)
-   if ((rc = launch_axpy(n3, 2.0, Yp, lam_b, m_stream)) || (rc = launch_axpy(n3, -1.0, Yp, lam_c, m_stream)) ||
-       (rc = launch_rows_scale(m_box, dt2, mRho.dev, Yp, Z, m_stream)) ||
-       (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, LT, pad, m_stream)) ||
-       (rc = launch_axpy(n3, 1.0, LT, lam_b, m_stream)) ||
+   // predictor: 2U - Um + dt^2/rho (L(U) + F); then the U, Um parts of every stage
+   if ((rc = launch_vjp_pred_pw(m_box, dt2, mRho.dev, Y, Acc, Z, grho, m_stream)) ||
+       (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, LT2, pad, m_stream)) ||
        (rc = launch_rhs4sg_matgrad(m_box, rm, m_dsbp, Z, u, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, gmu,
-                                   gla, pad, m_stream)) ||
-       (rc = launch_rho_diff(m_box, dt2, Yp, Acc, nullptr, mRho.dev, grho, m_stream)))
+                                   gla, m_stream)) ||
+       (rc = launch_vjp_lam(n3, 1.0 / dt2, Db, LT1, Y, LT2, lam_b, lam_c, m_stream)))
       return rc;
    return OK;
 }
@@ -136,7 +128,7 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
    if (ckpt < 1) ckpt = std::max(1, (int)std::lround(std::sqrt((double)nsteps)));
    const long n1 = m_box.npts, n3 = 3 * n1;
    const int ncp = (nsteps + ckpt - 1) / ckpt;
-   // work memory: 7 fields, 3 adjoint fields, ckpt+2 segment states, 2 per checkpoint, 3 gradients, pad
+   // work memory: 6 fields, 3 adjoint fields, ckpt+2 segment states, 2 per checkpoint, 3 gradients, pad, traces
    std::vector<void*> mine;
    auto release = [&]() {
       for (void* p : mine) (void)hipFree(p);
@@ -153,10 +145,10 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
       return (int)OK;
    };
    int rc = OK;
-   double* tmp[7];
+   double* tmp[6];
    double *lam[3], *gr[3], *plane = nullptr;
    std::vector<double*> seg(ckpt + 2), cp(2 * ncp);
-   for (int q = 0; q < 7 && rc == OK; q++) rc = grab(n3, &tmp[q]);
+   for (int q = 0; q < 6 && rc == OK; q++) rc = grab(n3, &tmp[q]);
    double* padmem = nullptr;
    if (rc == OK) rc = grab(adj_pad_doubles(m_box), &padmem);
    for (int q = 0; q < 3 && rc == OK; q++) rc = grab(n3, &lam[q]);
@@ -164,56 +156,57 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
    for (int q = 0; q <= ckpt + 1 && rc == OK; q++) rc = grab(n3, &seg[q]);
    for (int q = 0; q < 2 * ncp && rc == OK; q++) rc = grab(n3, &cp[q]);
    if (rc == OK) rc = grab(3 * m_box.nij, &plane);
-   int* didx = nullptr;
+   // receiver traces as adjoint sources, grouped by the grid value they act
+   // on (component offsets fit an int for the grids this path is meant for)
+   if (rc == OK && n3 > 0x7fffffff) rc = set_error(E_RANGE, "EW: adjoint source index beyond 32 bits");
+   std::map<int, std::vector<int>> groups;
+   std::vector<int> hmode(nrec);
+   for (int r = 0; r < nrec; r++) {
+      const long p = sidx(m_box, m_rec[4 * r], m_rec[4 * r + 1], m_rec[4 * r + 2]);
+      hmode[r] = m_rec[4 * r + 3];
+      for (int c = 0; c < 3; c++) groups[(int)(p + n1 * c)].push_back(3 * r + c);
+   }
+   std::vector<int> hint;  // uidx | off | ent | mode
+   for (const auto& gq : groups) hint.push_back(gq.first);
+   hint.push_back(0);
+   for (const auto& gq : groups) hint.push_back(hint.back() + (int)gq.second.size());
+   for (const auto& gq : groups) hint.insert(hint.end(), gq.second.begin(), gq.second.end());
+   hint.insert(hint.end(), hmode.begin(), hmode.end());
+   const size_t ntr = (size_t)(nsteps + 1) * 3 * nrec;
+   double* dtr = nullptr;
+   int* dint = nullptr;
+   if (rc == OK) rc = grab(ntr, &dtr);
    if (rc == OK) {
       void* p = nullptr;
-      if (hipMalloc(&p, sizeof(int) * 6 * nrec + sizeof(double) * 6 * nrec) != hipSuccess) {
+      if (hipMalloc(&p, sizeof(int) * hint.size()) != hipSuccess) {
          (void)hipGetLastError();
          rc = set_error(E_NOMEM, "EW: adjoint source allocation failed");
       } else {
          mine.push_back(p);
-         didx = static_cast<int*>(p);
+         dint = static_cast<int*>(p);
       }
    }
+   if (rc == OK)
+      rc = check_hip(hipMemcpyAsync(dint, hint.data(), sizeof(int) * hint.size(), hipMemcpyHostToDevice, m_stream),
+                     "adjoint source upload");
+   if (rc == OK)
+      rc = check_hip(hipMemcpyAsync(dtr, gtr, sizeof(double) * ntr, hipMemcpyHostToDevice, m_stream),
+                     "adjoint source upload");
+   if (rc == OK) rc = check_hip(hipStreamSynchronize(m_stream), "adjoint source upload");
    if (rc) {
       release();
       return rc;
    }
-   double* dval = reinterpret_cast<double*>(didx + 6 * nrec);
-   // direct adjoint source of state s: displacement receivers g[s], velocity
-   // receivers (U^s - U^{s-2}) / (2 dt): +g[s]/(2dt) on U^s, -g[s+2]/(2dt) on U^s
-   std::vector<int> hidx;
-   std::vector<double> hval;
+   ReceiverAdj R;
+   R.nu = (int)groups.size();
+   R.nrec = nrec;
+   R.uidx = dint;
+   R.off = dint + R.nu;
+   R.ent = R.off + R.nu + 1;
+   R.mode = R.ent + 3 * nrec;
+   R.g = dtr;
    auto add_direct = [&](int s, double* lamp) {
-      if (s < 1 || s > nsteps) return (int)OK;
-      hidx.clear();
-      hval.clear();
-      for (int r = 0; r < nrec; r++) {
-         const int i = m_rec[4 * r], j = m_rec[4 * r + 1], k = m_rec[4 * r + 2], mode = m_rec[4 * r + 3];
-         const long p = sidx(m_box, i, j, k);
-         for (int c = 0; c < 3; c++) {
-            double v;
-            if (mode == 0)
-               v = gtr[((size_t)s * nrec + r) * 3 + c];
-            else {
-               v = gtr[((size_t)s * nrec + r) * 3 + c] / (2 * mDt);
-               if (s + 2 <= nsteps) v -= gtr[((size_t)(s + 2) * nrec + r) * 3 + c] / (2 * mDt);
-            }
-            hidx.push_back((int)(p + n1 * c));
-            hval.push_back(v);
-         }
-      }
-      // component offsets fit an int for the grids this path is meant for
-      if (n3 > 0x7fffffff) return set_error(E_RANGE, "EW: adjoint source index beyond 32 bits");
-      int e;
-      if ((e = check_hip(hipMemcpyAsync(didx, hidx.data(), sizeof(int) * hidx.size(), hipMemcpyHostToDevice, m_stream),
-                         "adjoint source upload")) ||
-          (e = check_hip(hipMemcpyAsync(dval, hval.data(), sizeof(double) * hval.size(), hipMemcpyHostToDevice, m_stream),
-                         "adjoint source upload")) ||
-          (e = launch_scatter_add((int)hidx.size(), didx, dval, lamp, m_stream)) ||
-          (e = check_hip(hipStreamSynchronize(m_stream), "adjoint source upload")))
-         return e;
-      return (int)OK;
+      return launch_receiver_sources_T(R, s, nsteps, 2 * mDt, lamp, m_stream);
    };
    // forward pass: segment by segment into seg (seg[m] = U^{s0-1+m}), saving
    // the checkpoint (U^{s0}, U^{s0-1}) of each; seg ends up holding the last
@@ -263,24 +256,24 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
    const AdjPad pad = make_adj_pad(m_box, padmem);
    if (rc == OK) rc = launch_adj_pad_material(m_box, mMu.dev, mLambda.dev, sgc.strx, sgc.stry, sgc.strz, pad, m_stream);
    for (int q = 0; q < 3 && rc == OK; q++) rc = launch_zero(gr[q], n1, m_stream);
-   for (int q = 0; q < 3 && rc == OK; q++) rc = launch_zero(lam[q], n3, m_stream);
+   for (int q = 0; q < 2 && rc == OK; q++) rc = launch_zero(lam[q], n3, m_stream);
    double *la = lam[0], *lb = lam[1], *lc = lam[2];
    if (rc == OK) rc = add_direct(nsteps, la);
    if (rc == OK) rc = add_direct(nsteps - 1, lb);
-   if (rc == OK) rc = add_direct(nsteps - 2, lc);
    for (int cq = ncp - 1; cq >= 0 && rc == OK; cq--) {
       const int s0 = cq * ckpt, s1 = std::min(nsteps, s0 + ckpt);
       if (cq < ncp - 1 && (rc = fill_segment(cq))) break;  // the last one is still in seg
       for (int n = s1 - 1; n >= s0 && rc == OK; n--) {
-         rc = step_vjp(n, seg[n - s0 + 1], seg[n - s0], seg[n - s0 + 2], la, lb, lc, gr[0], gr[1], gr[2], tmp, plane, pad);
-         if (rc) break;
-         // rotate: U^{n} becomes the complete adjoint, U^{n-2} starts
+         // lc: the adjoint of U^{n-1} starts here, then takes its direct source
+         if ((rc = step_vjp(n, seg[n - s0 + 1], seg[n - s0], seg[n - s0 + 2], la, lb, lc, gr[0], gr[1], gr[2], tmp,
+                            plane, pad)) ||
+             (rc = add_direct(n - 1, lc)))
+            break;
+         // rotate: U^{n} becomes the complete adjoint; la was consumed
          double* old = la;
          la = lb;
          lb = lc;
          lc = old;
-         if ((rc = launch_zero(lc, n3, m_stream))) break;
-         rc = add_direct(n - 2, lc);
       }
    }
    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;

@@ -36,9 +36,6 @@ struct AdjPad {
    double* wb;  // W on the box (3 components), 2W, and S = strx stry strz:
    double* w2b; // the inputs of the LDS predictor sweep that evaluates
    double* sb;  // cof L(W)/S = 2W - 2W + cof L(W)/S with rho := S, dt = 1
-   Box pb4;     // the box grown by four points per side
-   double* u4;  // u on pb4 (3 components, 0 outside the box)
-   double* z4;  // z on the interior-formula rows (0 elsewhere), on pb4
    double* gy;  // damping transpose on pb: beta y / rho on the interior (3 components),
    double* dd;  // u - um in the box (3 components),
    double* rh;  // rho in the box (0 in the pad)
@@ -54,10 +51,10 @@ int launch_adj_pad_material(const Box& b, const double* mu, const double* la, co
 int launch_rhs4sg_T(const Box& b, const RowMap& m, const SbpTables* d_sbp, const double* z, const double* mu,
                     const double* la, double h, const double* strx, const double* stry, const double* strz,
                     double* out, const AdjPad& pad, hipStream_t s);
-// gmu += G_mu(z, u), gla += G_la(z, u) on every point (pad as above).
+// gmu += G_mu(z, u), gla += G_la(z, u) on every point.
 int launch_rhs4sg_matgrad(const Box& b, const RowMap& m, const SbpTables* d_sbp, const double* z, const double* u,
                           const double* mu, const double* la, double h, const double* strx, const double* stry,
-                          const double* strz, double* gmu, double* gla, const AdjPad& pad, hipStream_t s);
+                          const double* strz, double* gmu, double* gla, hipStream_t s);
 // Supergrid damping up -= D(u - um; rho) at the interior points
 // (addsgd4fort_indrev, ew-cfromfort.C:985-1041).  For the adjoint y of up:
 // dbar = -D^T y (overwritten on every point), grho += d/drho [-y . D].
@@ -79,15 +76,38 @@ int launch_bcfortsg_T(const Box& b, const int* wind, int nx, int ny, int nz, con
                       const double* strx, const double* stry, double* gmu, double* gla, double* plane,
                       hipStream_t s);
 
-// pointwise helpers over n values
+// pointwise helper over n values
 int launch_axpy(long n, double a, const double* x, double* y, hipStream_t s);              // y += a x
-int launch_rows_scale(const Box& b, double coef, const double* rho, const double* y, double* z,
-                      hipStream_t s);                                                        // z = coef/rho y on the interior, 0 elsewhere
-int launch_rho_grad(const Box& b, double coef, const double* y, const double* lu, const double* f,
-                    const double* rho, double* grho, hipStream_t s);                        // grho -= coef y.(lu+f)/rho^2
-// grho -= coef y.(a - b)/rho on the interior (b may be null)
-int launch_rho_diff(const Box& b, double coef, const double* y, const double* a, const double* bsub, const double* rho,
-                    double* grho, hipStream_t s);
-int launch_scatter_add(int n, const int* idx, const double* val, double* y, hipStream_t s);  // y[idx] += val
+
+// Fused pointwise stages of the reverse step (every point unless said):
+// corrector: z = c4/rho y on the interior (0 elsewhere), grho -= y.(u1 - up)/rho
+// on the interior;
+int launch_vjp_corr_pw(const Box& b, double c4, const double* rho, const double* y, const double* u1,
+                       const double* up, double* z, double* grho, hipStream_t s);
+// predictor: z = dt2/rho yp on the interior (0 elsewhere), grho -= dt2 yp.acc/rho
+// on the interior;
+int launch_vjp_pred_pw(const Box& b, double dt2, const double* rho, const double* yp, const double* acc, double* z,
+                       double* grho, hipStream_t s);
+// adjoints of U^n and U^{n-1} through one step, with a = lt1/dt2:
+// lb += db - 2a + 2yp + lt2, lc = a - db - yp (lc is overwritten).
+int launch_vjp_lam(long n3, double dt2i, const double* db, const double* lt1, const double* yp, const double* lt2,
+                   double* lb, double* lc, hipStream_t s);
+// Receiver traces g[s][r][c] (s = 0..nsteps) as adjoint sources, grouped by
+// the grid value they act on: group q adds to lam[uidx[q]] the entries
+// ent[off[q] .. off[q+1]-1] (e = 3r + c) in that order, so receivers that
+// share a grid point sum deterministically.
+struct ReceiverAdj {
+   int nu;            // groups
+   int nrec;
+   const int* uidx;   // [nu] component offsets into the field
+   const int* off;    // [nu + 1]
+   const int* ent;    // [3 nrec]
+   const int* mode;   // [nrec] 0 displacement, 1 velocity
+   const double* g;   // [(nsteps + 1) 3 nrec]
+};
+// Adjoint source of state s: displacement receivers add g[s], velocity
+// receivers (U^s - U^{s-2}) / (2 dt) add g[s]/(2 dt) - g[s+2]/(2 dt)
+// (g[s+2] only for s + 2 <= nsteps); nothing for s outside 1..nsteps.
+int launch_receiver_sources_T(const ReceiverAdj& R, int s, int nsteps, double twodt, double* lam, hipStream_t st);
 
 }  // namespace sw4amd

@@ -60,7 +60,6 @@ struct AdjCtx {
    const double* strx;
    const double* stry;
    const double* strz;
-   Box fb;  // index box of the padded fields read by the unchecked gathers
    __device__ __forceinline__ double vmu(int i, int j, int k) const { return inbox(b, i, j, k) ? mu[sidx(b, i, j, k)] : 0.0; }
    __device__ __forceinline__ double vla(int i, int j, int k) const { return inbox(b, i, j, k) ? la[sidx(b, i, j, k)] : 0.0; }
    __device__ __forceinline__ double sx(int i) const { return i >= b.ib && i <= b.ie ? strx[i - b.ib] : 1.0; }
@@ -189,14 +188,10 @@ __device__ void sbp_col_T(const AdjCtx& A, const double* z, int i, int j, int k,
 
 Box pad_box(const Box& b) { return make_box(b.ib - 2, b.ie + 2, b.jb - 2, b.je + 2, b.kb - 2, b.ke + 2); }
 
-namespace {
-Box pad4_box(const Box& b) { return make_box(b.ib - 4, b.ie + 4, b.jb - 4, b.je + 4, b.kb - 4, b.ke + 4); }
-}  // namespace
-
 long adj_pad_doubles(const Box& b)
 {
    const Box pb = pad_box(b);
-   return 12 * pb.npts + pb.ni + (pb.je - pb.jb + 1) + (pb.ke - pb.kb + 1) + 7 * b.npts + 6 * pad4_box(b).npts;
+   return 12 * pb.npts + pb.ni + (pb.je - pb.jb + 1) + (pb.ke - pb.kb + 1) + 7 * b.npts;
 }
 
 AdjPad make_adj_pad(const Box& b, double* mem)
@@ -212,10 +207,7 @@ AdjPad make_adj_pad(const Box& b, double* mem)
    p.wb = p.sz + (p.pb.ke - p.pb.kb + 1);
    p.w2b = p.wb + 3 * b.npts;
    p.sb = p.w2b + 3 * b.npts;
-   p.pb4 = pad4_box(b);
-   p.u4 = p.sb + b.npts;
-   p.z4 = p.u4 + 3 * p.pb4.npts;
-   p.gy = p.z4 + 3 * p.pb4.npts;
+   p.gy = p.sb + b.npts;
    p.dd = p.gy + 3 * p.pb.npts;
    p.rh = p.dd + 3 * p.pb.npts;
    return p;
@@ -358,19 +350,16 @@ __device__ __forceinline__ double psi(int t, const double f[5])
 
 // Second-difference terms along axis AX of the rows selected by ROWS (0
 // interior formula, 1 SBP rows): contribution to (gmu, gla) at p.
-// FAST: every row and every read of the neighbourhood is an interior-formula
-// row / inside the box, so nothing is checked.
-template <int AX, int ROWS, bool FAST = false>
+template <int AX, int ROWS>
 __device__ __forceinline__ void grad_second(const AdjCtx& A, const double* z, const double* u, int i, int j, int k,
                                             double& gm, double& gl)
 {
-   const Box& b = A.fb;
    const double sp = AX == 0 ? A.sx(i) : AX == 1 ? A.sy(j) : A.sz(k);
 #pragma unroll 1
    for (int t = 0; t < 5; t++) {
       const int o = 2 - t;  // row = p + o e_AX
       const int ir = i + (AX == 0) * o, jr = j + (AX == 1) * o, kr = k + (AX == 2) * o;
-      const bool ok = FAST || (ROWS == 0 ? int_row(A.m, ir, jr, kr) : sbp_row(A.m, ir, jr, kr));
+      const bool ok = ROWS == 0 ? int_row(A.m, ir, jr, kr) : sbp_row(A.m, ir, jr, kr);
       if (!ok) continue;
       const double srow = AX == 0 ? A.sx(ir) : AX == 1 ? A.sy(jr) : A.sz(kr);
       const double w = srow * sp * (1.0 / 6);
@@ -380,10 +369,9 @@ __device__ __forceinline__ void grad_second(const AdjCtx& A, const double* z, co
 #pragma unroll
          for (int s = 0; s < 5; s++) {
             const int ii = ir + (AX == 0) * (s - 2), jj = jr + (AX == 1) * (s - 2), kk = kr + (AX == 2) * (s - 2);
-            f[s] = FAST ? u[sidx(b, ii, jj, kk) + b.npts * c] : A.f(u, c, ii, jj, kk);
+            f[s] = A.f(u, c, ii, jj, kk);
          }
-         const double val = (FAST ? z[sidx(b, ir, jr, kr) + b.npts * c] : z[sidx(A.b, ir, jr, kr) + A.b.npts * c]) * w *
-                            psi(t, f);
+         const double val = z[sidx(A.b, ir, jr, kr) + A.b.npts * c] * w * psi(t, f);
          gm += (c == AX ? 2.0 : 1.0) * val;
          if (c == AX) gl += val;
       }
@@ -391,19 +379,17 @@ __device__ __forceinline__ void grad_second(const AdjCtx& A, const double* z, co
 }
 
 // Mixed term (coef D_DB u_C)_DA of row component RC: coef at p.
-template <int RC, int DA, int DB, int C, int ROWS, bool FAST = false>
+template <int RC, int DA, int DB, int C, int ROWS>
 __device__ __forceinline__ double grad_mixed(const AdjCtx& A, const double* z, const double* u, int i, int j, int k)
 {
-   const Box& b = A.fb;
    double D = 0, W = 0;
 #pragma unroll
    for (int t = 0; t < 4; t++) {
       const int a = t < 2 ? t - 2 : t - 1;
       const int iu = i + (DB == 0) * a, ju = j + (DB == 1) * a, ku = k + (DB == 2) * a;
-      D += cd4(a) * (FAST ? u[sidx(b, iu, ju, ku) + b.npts * C] : A.f(u, C, iu, ju, ku));
+      D += cd4(a) * A.f(u, C, iu, ju, ku);
       const int ir = i - (DA == 0) * a, jr = j - (DA == 1) * a, kr = k - (DA == 2) * a;
-      const double zr = FAST ? z[sidx(b, ir, jr, kr) + b.npts * RC]
-                             : (ROWS == 0 ? A.zi(z, RC, ir, jr, kr) : A.zb(z, RC, ir, jr, kr));
+      const double zr = ROWS == 0 ? A.zi(z, RC, ir, jr, kr) : A.zb(z, RC, ir, jr, kr);
       const double sa = DA == 0 ? A.sx(ir) : DA == 1 ? A.sy(jr) : A.sz(kr);
       const double sb = DB == 0 ? A.sx(ir) : DB == 1 ? A.sy(jr) : A.sz(kr);
       W += cd4(a) * sa * sb * zr;
@@ -488,201 +474,191 @@ __device__ void sbp_col_grad(const AdjCtx& A, const double* z, const double* u, 
    }
 }
 
-// u and the interior-row part of z on the box grown by 4 (zeros outside)
-__global__ void k_adj_pad4(AdjCtx A, AdjPad P, const double* __restrict__ z, const double* __restrict__ u)
-{
-   const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   const Box& p4 = P.pb4;
-   if (q >= p4.npts) return;
-   const int i = p4.ib + (int)(q % p4.ni);
-   const int j = p4.jb + (int)((q / p4.ni) % (p4.je - p4.jb + 1));
-   const int k = p4.kb + (int)(q / p4.nij);
-   const bool in = inbox(A.b, i, j, k), row = int_row(A.m, i, j, k);
-   const long p = in ? sidx(A.b, i, j, k) : 0;
-   for (int c = 0; c < 3; c++) {
-      P.u4[q + p4.npts * c] = in ? u[p + A.b.npts * c] : 0.0;
-      P.z4[q + p4.npts * c] = row ? z[p + A.b.npts * c] : 0.0;
-   }
-}
-
-// Interior-row terms of G_mu, G_la at every point of the box from the padded
-// fields (no checks: z4 vanishes off the rows, u4 off the box).
-__global__ void __launch_bounds__(256) k_rhs4sg_matgrad_int(AdjCtx A, AdjPad P, double* __restrict__ gmu,
-                                                            double* __restrict__ gla, double cof)
-{
-   const Box& b = A.b;
-   const int i = b.ib + blockIdx.x * blockDim.x + threadIdx.x;
-   const int j = b.jb + blockIdx.y * blockDim.y + threadIdx.y;
-   const int k = b.kb + blockIdx.z;
-   if (i > b.ie || j > b.je) return;
-   AdjCtx Q = A;
-   Q.fb = P.pb4;  // the fields are read on the padded box
-   double gm = 0, gl = 0;
-   grad_second<0, 0, true>(Q, P.z4, P.u4, i, j, k, gm, gl);
-   grad_second<1, 0, true>(Q, P.z4, P.u4, i, j, k, gm, gl);
-   grad_second<2, 0, true>(Q, P.z4, P.u4, i, j, k, gm, gl);
-   gl += grad_mixed<0, 0, 1, 1, 0, true>(Q, P.z4, P.u4, i, j, k) + grad_mixed<0, 0, 2, 2, 0, true>(Q, P.z4, P.u4, i, j, k) +
-         grad_mixed<1, 1, 0, 0, 0, true>(Q, P.z4, P.u4, i, j, k) + grad_mixed<1, 1, 2, 2, 0, true>(Q, P.z4, P.u4, i, j, k) +
-         grad_mixed<2, 2, 0, 0, 0, true>(Q, P.z4, P.u4, i, j, k) + grad_mixed<2, 2, 1, 1, 0, true>(Q, P.z4, P.u4, i, j, k);
-   gm += grad_mixed<0, 1, 0, 1, 0, true>(Q, P.z4, P.u4, i, j, k) + grad_mixed<0, 2, 0, 2, 0, true>(Q, P.z4, P.u4, i, j, k) +
-         grad_mixed<1, 0, 1, 0, 0, true>(Q, P.z4, P.u4, i, j, k) + grad_mixed<1, 2, 1, 2, 0, true>(Q, P.z4, P.u4, i, j, k) +
-         grad_mixed<2, 0, 2, 0, 0, true>(Q, P.z4, P.u4, i, j, k) + grad_mixed<2, 1, 2, 1, 0, true>(Q, P.z4, P.u4, i, j, k);
-   const long p = sidx(b, i, j, k);
-   gmu[p] += cof * gm;
-   gla[p] += cof * gl;
-}
-
-// LDS k-march version of k_rhs4sg_matgrad_int.  A workgroup owns a 32 x 8
-// tile of (i,j) columns of the box (ghosts included) and marches along k.
-// Plane k of u (halo 4) and of z (halo 2) sit in LDS for the x and y terms;
-// the thread's own column of u (k-4..k+4) and z (k-2..k+2) rotates in
-// registers for the z terms.  Reads the padded fields (kernels_adj.h).
+// Interior-row terms of G_mu, G_la at every point p of the box (ghosts
+// included), marching along k.  A workgroup owns a 32 x 8 tile of (i,j)
+// columns.  Plane k of u (0 off the box) and of z (0 off the interior-formula
+// rows) sit in LDS with a halo of 2, double-buffered so that one barrier per
+// plane suffices; the next plane is fetched into registers before the current
+// one is evaluated.  The thread's own columns of u and z (k-2..k+2) rotate in
+// registers.
+//
+// Along axis d, for component c, with w(e) = u_c(p + e e_d) and
+// y(e) = s_d(p + e e_d) z_c(p + e e_d), e = -2..2: the five rows p + e e_d
+// whose varcoef4 coefficients contain the material at p contribute
+// s_d(p)/6 sum_e w(e) K_e(y) (second_form below: the psi factors of varcoef4
+// summed over the rows, which only reach u within 2 points of p).  The mixed
+// terms (coef D_a u_b)_d are D[a][b] = d4 of u_b along a at p times
+// V[r][d] = d4 of s_d z_r along d, taken with its sign reversed, at p.
 namespace {
-constexpr int MTX = 32, MTY = 8, MNT = MTX * MTY;
-constexpr int MHX4 = MTX + 8, MHY4 = MTY + 8, MPL4 = MHX4 * MHY4;  // u plane, halo 4
-constexpr int MHX2 = MTX + 4, MHY2 = MTY + 4, MPL2 = MHX2 * MHY2;  // z plane, halo 2
+constexpr int GTX = 32, GTY = 8, GNT = GTX * GTY;
+constexpr int GHX = GTX + 4, GHY = GTY + 4, GPL = GHX * GHY;  // plane with a halo of 2
+constexpr int GSL = (GPL + GNT - 1) / GNT;                      // staging slots per thread
 }  // namespace
 
-__global__ void __launch_bounds__(MNT, 2) k_rhs4sg_matgrad_lds(AdjCtx A, AdjPad P, int ntx, int nty, int kc,
-                                                               double* __restrict__ gmu, double* __restrict__ gla,
-                                                               double cof)
+// sum_e w(e) K_e(y), arrays indexed e + 2
+__device__ __forceinline__ double second_form(const double w[5], const double y[5])
 {
-   __shared__ double su[3 * MPL4];
-   __shared__ double sz[3 * MPL2];
+   const double k0 = y[1] - 0.75 * y[2] - 0.25 * y[0];
+   const double k1 = y[3] + 3 * y[2] - 5 * y[1] + y[0];
+   const double k2 = 3 * (y[3] + y[1]) - 0.75 * (y[4] + y[0]) - 4.5 * y[2];
+   const double k3 = y[4] - 5 * y[3] + 3 * y[2] + y[1];
+   const double k4 = y[3] - 0.75 * y[2] - 0.25 * y[4];
+   return w[0] * k0 + w[1] * k1 + w[2] * k2 + w[3] * k3 + w[4] * k4;
+}
+// d4 at e = 0: sum_a cd4(a) f(a), and the same of f(-a)
+__device__ __forceinline__ double d4w(const double f[5]) { return f[0] - f[4] + 8 * (f[3] - f[1]); }
+__device__ __forceinline__ double d4r(const double f[5]) { return f[4] - f[0] + 8 * (f[1] - f[3]); }
+
+__global__ void __launch_bounds__(GNT, 2) k_rhs4sg_matgrad_march(AdjCtx A, int ntx, int nty, int kc,
+                                                                 const double* __restrict__ z,
+                                                                 const double* __restrict__ u,
+                                                                 double* __restrict__ gmu, double* __restrict__ gla,
+                                                                 double cof)
+{
+   __shared__ double su[2][3 * GPL];
+   __shared__ double sz[2][3 * GPL];
    const Box& b = A.b;
-   const Box& q4 = P.pb4;
+   const RowMap& m = A.m;
    const int t = blockIdx.x;
    const int tix = t % ntx, tr = t / ntx, tjy = tr % nty, tkz = tr / nty;
-   const int tid = threadIdx.x, tx = tid % MTX, ty = tid / MTX;
-   const int ia = b.ib + tix * MTX, ja = b.jb + tjy * MTY;
+   const int tid = threadIdx.x, tx = tid % GTX, ty = tid / GTX;
+   const int ia = b.ib + tix * GTX, ja = b.jb + tjy * GTY;
    const int i = ia + tx, j = ja + ty;
    const bool active = i <= b.ie && j <= b.je;
    const int k0 = b.kb + tkz * kc, k1 = min(b.ke, k0 + kc - 1);
-   const double* u4 = P.u4;
-   const double* z4 = P.z4;
-   const long n4 = q4.npts;
-   // own column (inactive threads read column (ib, jb): inside the pad box)
-   const long col = sidx(q4, active ? i : b.ib, active ? j : b.jb, 0) - sidx(q4, 0, 0, 0);
-   auto at4 = [&](int kk) { return col + (long)q4.nij * (kk - q4.kb) + (sidx(q4, 0, 0, q4.kb) - sidx(q4, 0, 0, q4.kb)); };
-   double ucol[9][3], zcol[5][3];
+   const long n = b.npts;
+   // staging slots: in-plane offset of the point (-1 off the box) and whether
+   // its column carries interior-formula rows
+   int soff[GSL];
+   bool srow[GSL];
 #pragma unroll
-   for (int q = 0; q < 8; q++)
-#pragma unroll
-      for (int c = 0; c < 3; c++) ucol[q][c] = u4[sidx(q4, 0, 0, k0 - 4 + q) + col + n4 * c];
-#pragma unroll
-   for (int q = 0; q < 4; q++)
-#pragma unroll
-      for (int c = 0; c < 3; c++) zcol[q][c] = z4[sidx(q4, 0, 0, k0 - 2 + q) + col + n4 * c];
-   (void)at4;
-   const double sxi = A.sx(i), syj = A.sy(j);
-   double sxo[5], syo[5];
-#pragma unroll
-   for (int o = 0; o < 5; o++) {
-      sxo[o] = A.sx(i + o - 2);
-      syo[o] = A.sy(j + o - 2);
+   for (int q = 0; q < GSL; q++) {
+      const int e = tid + GNT * q, yy = e / GHX, xx = e - yy * GHX;
+      const int gi = ia - 2 + xx, gj = ja - 2 + yy;
+      const bool in = e < GPL && gi >= b.ib && gi <= b.ie && gj >= b.jb && gj <= b.je;
+      soff[q] = in ? (gj - b.jb) * b.ni + (gi - b.ib) : -1;
+      srow[q] = in && gi >= 1 && gi <= m.nx && gj >= 1 && gj <= m.ny;
    }
-   const double i6 = 1.0 / 6, i144 = 1.0 / 144;
-   for (int k = k0; k <= k1; k++) {
+   // reads go to a point of the box and are masked afterwards (no branches)
+   double pu[GSL][3], pz[GSL][3];
+   auto fetch = [&](int kk) {  // plane kk (inside the box) into pu, pz
+      const long pk = (long)(kk - b.kb) * b.nij;
+      const bool krow = kk >= m.k1 && kk <= m.k2;
+#pragma unroll
+      for (int q = 0; q < GSL; q++) {
+         const long a = pk + max(soff[q], 0);
+         const bool zin = krow && srow[q];
+#pragma unroll
+         for (int c = 0; c < 3; c++) {
+            const double uv = u[a + n * c], zv = z[a + n * c];
+            pu[q][c] = soff[q] >= 0 ? uv : 0.0;
+            pz[q][c] = zin ? zv : 0.0;
+         }
+      }
+   };
+   auto store = [&](int buf) {
+#pragma unroll
+      for (int q = 0; q < GSL; q++) {
+         const int e = tid + GNT * q;
+         if (e < GPL) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+               su[buf][c * GPL + e] = pu[q][c];
+               sz[buf][c * GPL + e] = pz[q][c];
+            }
+         }
+      }
+   };
+   // own column (inactive threads keep zeros)
+   const int ic = active ? i : b.ib, jc = active ? j : b.jb;
+   const long col = (long)(jc - b.jb) * b.ni + (ic - b.ib);
+   const bool crow = active && i >= 1 && i <= m.nx && j >= 1 && j <= m.ny;
+   double uc[5][3], zc[5][3], szr[5];
+   auto column = [&](int q, int kk) {
+      const long pk = (long)(min(max(kk, b.kb), b.ke) - b.kb) * b.nij + col;
+      const bool uin = active && kk >= b.kb && kk <= b.ke, zin = crow && kk >= m.k1 && kk <= m.k2;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-         ucol[8][c] = u4[sidx(q4, 0, 0, k + 4) + col + n4 * c];
-         zcol[4][c] = z4[sidx(q4, 0, 0, k + 2) + col + n4 * c];
+         const double uv = u[pk + n * c], zv = z[pk + n * c];
+         uc[q][c] = uin ? uv : 0.0;
+         zc[q][c] = zin ? zv : 0.0;
       }
-      // stage plane k: u with halo 4, z with halo 2 (0 beyond the pad box)
-      for (int e = tid; e < 3 * MPL4; e += MNT) {
-         const int c = e / MPL4, r = e - c * MPL4, yy = r / MHX4, xx = r - yy * MHX4;
-         const int gi = ia - 4 + xx, gj = ja - 4 + yy;
-         const bool ok = gi <= q4.ie && gj <= q4.je;
-         su[e] = ok ? u4[sidx(q4, gi, gj, k) + n4 * c] : 0.0;
-      }
-      for (int e = tid; e < 3 * MPL2; e += MNT) {
-         const int c = e / MPL2, r = e - c * MPL2, yy = r / MHX2, xx = r - yy * MHX2;
-         const int gi = ia - 2 + xx, gj = ja - 2 + yy;
-         const bool ok = gi <= q4.ie && gj <= q4.je;
-         sz[e] = ok ? z4[sidx(q4, gi, gj, k) + n4 * c] : 0.0;
-      }
-      __syncthreads();
+      szr[q] = A.sz(kk);
+   };
+#pragma unroll
+   for (int q = 0; q < 4; q++) column(q, k0 - 2 + q);
+   double sxo[5], syo[5];
+#pragma unroll
+   for (int e = 0; e < 5; e++) {
+      sxo[e] = A.sx(ic + e - 2);
+      syo[e] = A.sy(jc + e - 2);
+   }
+   fetch(k0);
+   store(0);
+   __syncthreads();
+   const int ctr = (ty + 2) * GHX + tx + 2;
+   for (int k = k0; k <= k1; k++) {
+      const int cur = (k - k0) & 1;
+      column(4, k + 2);
+      if (k < k1) fetch(k + 1);
       if (active) {
-         auto U = [&](int c, int dx, int dy) { return su[c * MPL4 + (ty + 4 + dy) * MHX4 + tx + 4 + dx]; };
-         auto Z = [&](int c, int dx, int dy) { return sz[c * MPL2 + (ty + 2 + dy) * MHX2 + tx + 2 + dx]; };
-         const double szk = A.sz(k);
-         double gm = 0, gl = 0;
-         // second differences along x and y (plane k, LDS)
-#pragma unroll 1
-         for (int tt = 0; tt < 5; tt++) {
-            const int o = 2 - tt;
-            const double wx = A.sx(i + o) * sxi * i6, wy = A.sy(j + o) * syj * i6;
+         const double* U = su[cur];
+         const double* Zs = sz[cur];
+         double D[3][3], V[3][3], gs = 0, gd = 0;
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-               double f[5];
+         for (int c = 0; c < 3; c++) {
+            double w[5], y[5], S;
 #pragma unroll
-               for (int q = 0; q < 5; q++) f[q] = U(c, o + q - 2, 0);
-               const double vx = Z(c, o, 0) * wx * psi(tt, f);
-#pragma unroll
-               for (int q = 0; q < 5; q++) f[q] = U(c, 0, o + q - 2);
-               const double vy = Z(c, 0, o) * wy * psi(tt, f);
-               gm += (c == 0 ? 2.0 : 1.0) * vx + (c == 1 ? 2.0 : 1.0) * vy;
-               gl += (c == 0 ? vx : 0.0) + (c == 1 ? vy : 0.0);
+            for (int e = 0; e < 5; e++) {
+               w[e] = U[c * GPL + ctr + e - 2];
+               y[e] = sxo[e] * Zs[c * GPL + ctr + e - 2];
             }
+            S = sxo[2] * second_form(w, y);
+            D[0][c] = d4w(w);
+            V[c][0] = d4r(y);
+            gs += S;
+            if (c == 0) gd += S;
+#pragma unroll
+            for (int e = 0; e < 5; e++) {
+               w[e] = U[c * GPL + ctr + (e - 2) * GHX];
+               y[e] = syo[e] * Zs[c * GPL + ctr + (e - 2) * GHX];
+            }
+            S = syo[2] * second_form(w, y);
+            D[1][c] = d4w(w);
+            V[c][1] = d4r(y);
+            gs += S;
+            if (c == 1) gd += S;
+#pragma unroll
+            for (int e = 0; e < 5; e++) {
+               w[e] = uc[e][c];
+               y[e] = szr[e] * zc[e][c];
+            }
+            S = szr[2] * second_form(w, y);
+            D[2][c] = d4w(w);
+            V[c][2] = d4r(y);
+            gs += S;
+            if (c == 2) gd += S;
          }
-         // along z (own column, registers)
-#pragma unroll
-         for (int tt = 0; tt < 5; tt++) {
-            const int o = 2 - tt;
-            const double wz = A.sz(k + o) * szk * i6;
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-               double f[5];
-#pragma unroll
-               for (int q = 0; q < 5; q++) f[q] = ucol[o + q + 2][c];
-               const double vz = zcol[o + 2][c] * wz * psi(tt, f);
-               gm += (c == 2 ? 2.0 : 1.0) * vz;
-               gl += c == 2 ? vz : 0.0;
-            }
-         }
-         // mixed terms (coef D_DB u_C)_DA of row component RC, coef at this point
-         auto mixed = [&](int RC, int DA, int DB, int C) {
-            double D = 0, W = 0;
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-               const int a = q < 2 ? q - 2 : q - 1;
-               const double ca = cd4(a);
-               D += ca * (DB == 0 ? U(C, a, 0) : DB == 1 ? U(C, 0, a) : ucol[a + 4][C]);
-               double zr, sa, sb;
-               if (DA == 0) {
-                  zr = Z(RC, -a, 0);
-                  sa = sxo[2 - a];
-                  sb = DB == 1 ? syj : szk;
-               } else if (DA == 1) {
-                  zr = Z(RC, 0, -a);
-                  sa = syo[2 - a];
-                  sb = DB == 0 ? sxi : szk;
-               } else {
-                  zr = zcol[2 - a][RC];
-                  sa = A.sz(k - a);
-                  sb = DB == 0 ? sxi : syj;
-               }
-               W += ca * sa * sb * zr;
-            }
-            return D * W * i144;
-         };
-         gl += mixed(0, 0, 1, 1) + mixed(0, 0, 2, 2) + mixed(1, 1, 0, 0) + mixed(1, 1, 2, 2) + mixed(2, 2, 0, 0) +
-               mixed(2, 2, 1, 1);
-         gm += mixed(0, 1, 0, 1) + mixed(0, 2, 0, 2) + mixed(1, 0, 1, 0) + mixed(1, 2, 1, 2) + mixed(2, 0, 2, 0) +
-               mixed(2, 1, 2, 1);
-         const long p = sidx(b, i, j, k);
-         gmu[p] += cof * gm;
-         gla[p] += cof * gl;
+         const double s0 = sxo[2], s1 = syo[2], s2 = szr[2];
+         const double ml = V[0][0] * (s1 * D[1][1] + s2 * D[2][2]) + V[1][1] * (s0 * D[0][0] + s2 * D[2][2]) +
+                           V[2][2] * (s0 * D[0][0] + s1 * D[1][1]);
+         const double mm = s0 * (D[0][1] * V[0][1] + D[0][2] * V[0][2]) + s1 * (D[1][0] * V[1][0] + D[1][2] * V[1][2]) +
+                           s2 * (D[2][0] * V[2][0] + D[2][1] * V[2][1]);
+         const long p = (long)(k - b.kb) * b.nij + col;
+         gmu[p] += cof * ((gs + gd) * (1.0 / 6) + mm * (1.0 / 144));
+         gla[p] += cof * (gd * (1.0 / 6) + ml * (1.0 / 144));
       }
+      if (k < k1) store(cur ^ 1);
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < 8; q++)
+      for (int q = 0; q < 4; q++) {
 #pragma unroll
-         for (int c = 0; c < 3; c++) ucol[q][c] = ucol[q + 1][c];
-#pragma unroll
-      for (int q = 0; q < 4; q++)
-#pragma unroll
-         for (int c = 0; c < 3; c++) zcol[q][c] = zcol[q + 1][c];
+         for (int c = 0; c < 3; c++) {
+            uc[q][c] = uc[q + 1][c];
+            zc[q][c] = zc[q + 1][c];
+         }
+         szr[q] = szr[q + 1];
+      }
    }
 }
 
@@ -1013,53 +989,78 @@ __global__ void k_axpy(long n, double a, const double* __restrict__ x, double* _
    if (q < n) y[q] += a * x[q];
 }
 
-__global__ void k_rows_scale(Box b, double coef, const double* __restrict__ rho, const double* __restrict__ y,
-                             double* __restrict__ z)
+__device__ __forceinline__ bool box_interior(const Box& b, long p)
 {
-   const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   if (p >= b.npts) return;
    const int i = b.ib + (int)(p % b.ni);
    const int j = b.jb + (int)((p / b.ni) % (b.je - b.jb + 1));
    const int k = b.kb + (int)(p / b.nij);
-   const bool in = i >= b.ib + 2 && i <= b.ie - 2 && j >= b.jb + 2 && j <= b.je - 2 && k >= b.kb + 2 && k <= b.ke - 2;
-   const double s = in ? coef / rho[p] : 0.0;
-   for (int c = 0; c < 3; c++) z[p + b.npts * c] = s * y[p + b.npts * c];
+   return i >= b.ib + 2 && i <= b.ie - 2 && j >= b.jb + 2 && j <= b.je - 2 && k >= b.kb + 2 && k <= b.ke - 2;
 }
 
-__global__ void k_rho_grad(Box b, double coef, const double* __restrict__ y, const double* __restrict__ lu,
-                           const double* __restrict__ f, const double* __restrict__ rho, double* __restrict__ grho)
+__global__ void k_vjp_corr_pw(Box b, double c4, const double* __restrict__ rho, const double* __restrict__ y,
+                              const double* __restrict__ u1, const double* __restrict__ up, double* __restrict__ z,
+                              double* __restrict__ grho)
 {
    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (p >= b.npts) return;
-   double s = 0;
+   const bool in = box_interior(b, p);
+   const double r = rho[p], s = in ? c4 / r : 0.0;
+   double g = 0;
+#pragma unroll
    for (int c = 0; c < 3; c++) {
       const long q = p + b.npts * c;
-      s += y[q] * (lu[q] + f[q]);
+      const double yv = y[q];
+      z[q] = s * yv;
+      g += yv * (u1[q] - up[q]);
    }
-   grho[p] -= coef * s / (rho[p] * rho[p]);
+   if (in) grho[p] -= g / r;
 }
 
-__global__ void k_rho_diff(Box b, double coef, const double* __restrict__ y, const double* __restrict__ a,
-                           const double* __restrict__ bs, const double* __restrict__ rho, double* __restrict__ grho)
+__global__ void k_vjp_pred_pw(Box b, double dt2, const double* __restrict__ rho, const double* __restrict__ yp,
+                              const double* __restrict__ acc, double* __restrict__ z, double* __restrict__ grho)
 {
    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (p >= b.npts) return;
-   const int i = b.ib + (int)(p % b.ni);
-   const int j = b.jb + (int)((p / b.ni) % (b.je - b.jb + 1));
-   const int k = b.kb + (int)(p / b.nij);
-   if (i < b.ib + 2 || i > b.ie - 2 || j < b.jb + 2 || j > b.je - 2 || k < b.kb + 2 || k > b.ke - 2) return;
-   double s = 0;
+   const bool in = box_interior(b, p);
+   const double r = rho[p], s = in ? dt2 / r : 0.0;
+   double g = 0;
+#pragma unroll
    for (int c = 0; c < 3; c++) {
       const long q = p + b.npts * c;
-      s += y[q] * (a[q] - (bs ? bs[q] : 0.0));
+      const double yv = yp[q];
+      z[q] = s * yv;
+      g += yv * acc[q];
    }
-   grho[p] -= coef * s / rho[p];
+   if (in) grho[p] -= dt2 * g / r;
 }
 
-__global__ void k_scatter_add(int n, const int* __restrict__ idx, const double* __restrict__ val, double* __restrict__ y)
+__global__ void k_vjp_lam(long n3, double dt2i, const double* __restrict__ db, const double* __restrict__ lt1,
+                          const double* __restrict__ yp, const double* __restrict__ lt2, double* __restrict__ lb,
+                          double* __restrict__ lc)
+{
+   const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (q >= n3) return;
+   const double a = dt2i * lt1[q], d = db[q], y = yp[q];
+   lb[q] += d - 2 * a + 2 * y + lt2[q];
+   lc[q] = a - d - y;
+}
+
+__global__ void k_receiver_sources_T(ReceiverAdj R, int s, int nsteps, double twodt, double* __restrict__ lam)
 {
    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-   if (q < n) y[idx[q]] += val[q];
+   if (q >= R.nu) return;
+   const long n = 3L * R.nrec;
+   double v = 0;
+   for (int t = R.off[q]; t < R.off[q + 1]; t++) {
+      const int e = R.ent[t];
+      double w = R.g[s * n + e];
+      if (R.mode[e / 3] == 1) {
+         w /= twodt;
+         if (s + 2 <= nsteps) w -= R.g[(s + 2) * n + e] / twodt;
+      }
+      v += w;
+   }
+   lam[R.uidx[q]] += v;
 }
 
 // ===========================================================================
@@ -1107,17 +1108,16 @@ int launch_rhs4sg_T(const Box& b, const RowMap& m, const SbpTables* d_sbp, const
 
 int launch_rhs4sg_matgrad(const Box& b, const RowMap& m, const SbpTables* d_sbp, const double* z, const double* u,
                           const double* mu, const double* la, double h, const double* strx, const double* stry,
-                          const double* strz, double* gmu, double* gla, const AdjPad& P, hipStream_t s)
+                          const double* strz, double* gmu, double* gla, hipStream_t s)
 {
    const AdjCtx A{b, m, d_sbp, mu, la, strx, stry, strz};
    const double cof = 1.0 / (h * h);
    const dim3 blk(64, 4, 1);
    const unsigned gi = cdiv(b.ni, 64), gj = cdiv(b.je - b.jb + 1, 4);
-   k_adj_pad4<<<cdiv(P.pb4.npts, 256), 256, 0, s>>>(A, P, z, u);
    {
-      const int ntx = (int)cdiv(b.ni, MTX), nty = (int)cdiv(b.je - b.jb + 1, MTY), kc = 32;
+      const int ntx = (int)cdiv(b.ni, GTX), nty = (int)cdiv(b.je - b.jb + 1, GTY), kc = 32;
       const int nkc = (int)cdiv(b.ke - b.kb + 1, kc);
-      k_rhs4sg_matgrad_lds<<<ntx * nty * nkc, MNT, 0, s>>>(A, P, ntx, nty, kc, gmu, gla, cof);
+      k_rhs4sg_matgrad_march<<<ntx * nty * nkc, GNT, 0, s>>>(A, ntx, nty, kc, z, u, gmu, gla, cof);
    }
    if (m.top) k_rhs4sg_matgrad_sbp<<<dim3(gi, gj, 9), blk, 0, s>>>(A, 1, 1, z, u, gmu, gla, cof);
    if (m.bot) k_rhs4sg_matgrad_sbp<<<dim3(gi, gj, 9), blk, 0, s>>>(A, m.nz, -1, z, u, gmu, gla, cof);
@@ -1177,30 +1177,32 @@ int launch_axpy(long n, double a, const double* x, double* y, hipStream_t s)
    return check_hip(hipGetLastError(), "axpy launch");
 }
 
-int launch_rows_scale(const Box& b, double coef, const double* rho, const double* y, double* z, hipStream_t s)
+int launch_vjp_corr_pw(const Box& b, double c4, const double* rho, const double* y, const double* u1,
+                       const double* up, double* z, double* grho, hipStream_t s)
 {
-   k_rows_scale<<<cdiv(b.npts, 256), 256, 0, s>>>(b, coef, rho, y, z);
-   return check_hip(hipGetLastError(), "row scaling launch");
+   k_vjp_corr_pw<<<cdiv(b.npts, 256), 256, 0, s>>>(b, c4, rho, y, u1, up, z, grho);
+   return check_hip(hipGetLastError(), "reverse corrector launch");
 }
 
-int launch_rho_grad(const Box& b, double coef, const double* y, const double* lu, const double* f,
-                    const double* rho, double* grho, hipStream_t s)
+int launch_vjp_pred_pw(const Box& b, double dt2, const double* rho, const double* yp, const double* acc, double* z,
+                       double* grho, hipStream_t s)
 {
-   k_rho_grad<<<cdiv(b.npts, 256), 256, 0, s>>>(b, coef, y, lu, f, rho, grho);
-   return check_hip(hipGetLastError(), "density derivative launch");
+   k_vjp_pred_pw<<<cdiv(b.npts, 256), 256, 0, s>>>(b, dt2, rho, yp, acc, z, grho);
+   return check_hip(hipGetLastError(), "reverse predictor launch");
 }
 
-int launch_rho_diff(const Box& b, double coef, const double* y, const double* a, const double* bsub, const double* rho,
-                    double* grho, hipStream_t s)
+int launch_vjp_lam(long n3, double dt2i, const double* db, const double* lt1, const double* yp, const double* lt2,
+                   double* lb, double* lc, hipStream_t s)
 {
-   k_rho_diff<<<cdiv(b.npts, 256), 256, 0, s>>>(b, coef, y, a, bsub, rho, grho);
-   return check_hip(hipGetLastError(), "density derivative launch");
+   k_vjp_lam<<<cdiv(n3, 256), 256, 0, s>>>(n3, dt2i, db, lt1, yp, lt2, lb, lc);
+   return check_hip(hipGetLastError(), "reverse step adjoint launch");
 }
 
-int launch_scatter_add(int n, const int* idx, const double* val, double* y, hipStream_t s)
+int launch_receiver_sources_T(const ReceiverAdj& R, int s, int nsteps, double twodt, double* lam, hipStream_t st)
 {
-   if (n > 0) k_scatter_add<<<cdiv(n, 128), 128, 0, s>>>(n, idx, val, y);
-   return check_hip(hipGetLastError(), "scatter launch");
+   if (R.nu <= 0 || s < 1 || s > nsteps) return OK;
+   k_receiver_sources_T<<<cdiv(R.nu, 256), 256, 0, st>>>(R, s, nsteps, twodt, lam);
+   return check_hip(hipGetLastError(), "receiver adjoint source launch");
 }
 
 }  // namespace sw4amd
