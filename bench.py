@@ -75,7 +75,8 @@ def parse():
                         "(1024^3, h=10 m, 800 steps); adjoint: gradient of a receiver misfit against the forward run "
                         "(256^3, h=25 m, 600 steps, 128 surface receivers); batch: 48 shots of N^3, N uniform in "
                         "128..288, h=25 m, stepped alone, batched, batched with graph replay")
-    p.add_argument("--ckpt", type=int, default=50, help="adjoint workload: steps between checkpoints")
+    p.add_argument("--ckpt", type=int, default=0,
+                   help="adjoint workload: steps between checkpoints (0: from free HBM, the whole history when it fits)")
     p.add_argument("--shots", type=int, default=48, help="batch workload: number of shots")
     a = p.parse_args()
     if a.n is None:
@@ -472,7 +473,12 @@ def main_adjoint(a, torch):
     steps recording the receivers) and the gradient of J = 1/2 |d - d_obs|^2
     with respect to rho, mu, lambda (sw4hip_ew_gradient: forward with
     checkpoints every --ckpt steps, reverse sweep recomputing one segment at
-    a time).  value = gradient seconds / forward seconds."""
+    a time; --ckpt 0 lets the library choose the interval from free HBM,
+    which at 256^3 x 600 steps holds the whole history, and the run is then
+    repeated with checkpoints every 50 steps for comparison).  The library
+    keeps its gradient workspace between calls: each setting is run twice
+    and the second call timed (the first, which allocates, is reported).
+    value = gradient seconds / forward seconds."""
     from sw4lite_amd import EW
     from sw4lite_amd import lib as L
     n, h, steps = a.n, 25.0, a.steps
@@ -499,12 +505,30 @@ def main_adjoint(a, torch):
     tr = e.receivers()
     dobs = 0.9 * tr
     out = [torch.empty_like(rho), torch.empty_like(mu), torch.empty_like(la)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e.gradient(tr - dobs, steps, ckpt=a.ckpt, out=out)
-    torch.cuda.synchronize()
-    t_grad = time.perf_counter() - t0
+    used, nbytes = e.gradient_plan(steps, a.ckpt)
+
+    def timed_gradient(ck):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e.gradient(tr - dobs, steps, ckpt=ck, out=out)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    # the first call allocates the workspace the library keeps for later
+    # gradients (an inversion computes many); the second one is timed
+    t_first = timed_gradient(a.ckpt)
+    t_grad = timed_gradient(a.ckpt)
     gn = [float(o.abs().max()) for o in out]
+    fixed = {}
+    if a.ckpt <= 0 and used != min(50, steps):
+        g_auto = [o.clone() for o in out]
+        c50 = min(50, steps)
+        e.release_workspace()
+        timed_gradient(c50)
+        t50 = timed_gradient(c50)
+        fixed = {"checkpoint_every": c50, "device_bytes": e.gradient_plan(steps, c50)[1], "gradient_s": t50,
+                 "ratio": t50 / t_fwd,
+                 "max_rel_diff_vs_first": max(float((x - y).abs().max() / y.abs().max()) for x, y in zip(out, g_auto))}
     e.close()
     pts = float(n) ** 3
     line = {
@@ -515,8 +539,10 @@ def main_adjoint(a, torch):
         "config": {
             "workload": "%d^3 Cartesian FP64 step, h=%g m, layered+lognormal material, free top, supergrid 30, "
                         "one double couple at 500 m (Ricker 5 Hz), 128 surface receivers, one shot" % (n, h),
-            "grid": [n, n, n], "h": h, "checkpoint_every": a.ckpt,
-            "forward_s": t_fwd, "gradient_s": t_grad,
+            "grid": [n, n, n], "h": h, "checkpoint_every": used, "checkpoint_arg": a.ckpt,
+            "gradient_device_bytes": nbytes, "forward_s": t_fwd, "gradient_s": t_grad,
+            "gradient_first_call_s": t_first,
+            "fixed_checkpoints": fixed,
             "forward_grid_pt_updates_per_s": pts * steps / t_fwd,
             "max_abs_gradient": {"rho": gn[0], "mu": gn[1], "lambda": gn[2]},
         },

@@ -293,14 +293,24 @@ int sw4hip_ew_point_source_error(sw4hip_ew* ew, double* linf, double* l2, double
 int sw4hip_ew_reset(sw4hip_ew* ew);
 /* Gradient of a receiver misfit J with respect to rho, mu and lambda on
  * every point of the box (ghost points included).  Runs nsteps fused steps
- * from zero initial data, saving (U, Um) every ckpt steps (ckpt <= 0:
- * sqrt(nsteps)), then the reverse sweep, recomputing one checkpoint segment
- * at a time.  grad_traces: HOST array [nsteps+1][nreceivers][3] of dJ/d(sample)
+ * from zero initial data, saving (U, Um) every ckpt steps, then the reverse
+ * sweep, recomputing one checkpoint segment at a time (all but the last:
+ * nsteps - ckpt forward steps when ckpt divides nsteps).  ckpt <= 0: the
+ * largest interval whose states fit in free device memory, nsteps (nothing
+ * recomputed) when the whole history fits; sqrt(nsteps) when not even that
+ * bound can be met.  grad_traces: HOST array [nsteps+1][nreceivers][3] of dJ/d(sample)
  * in the order of sw4hip_ew_get_receivers.  dt stays the one of setup (set
  * a new material with sw4hip_ew_set_material after setup to keep it).
  * Outputs overwritten; device pointers when on_device. */
 int sw4hip_ew_gradient(sw4hip_ew* ew, int nsteps, int ckpt, const double* grad_traces, double* grho,
                        double* gmu, double* glambda, int on_device);
+/* The checkpoint interval sw4hip_ew_gradient would use for (nsteps, ckpt)
+ * and the device bytes it needs (device_bytes may be NULL). */
+int sw4hip_ew_gradient_plan(sw4hip_ew* ew, int nsteps, int ckpt, int* used_ckpt, double* device_bytes);
+/* sw4hip_ew_gradient keeps its device buffers (states, work fields) between
+ * calls and reuses them; this frees them (sw4hip_ew_destroy does too).  No
+ * counterpart in the reference. */
+int sw4hip_ew_release_workspace(sw4hip_ew* ew);
 /* A receiver at the grid point nearest to (x,y,z) (EW::computeNearestGridPoint,
  * EW.C:5149-5180); mode 0 records displacement, 1 velocity (up-um)/(2dt)
  * (EW::extractRecordData, EW.C:5854-5875).  One sample at t=0 and one per

@@ -362,7 +362,7 @@ int sw4hip_addsgd4_adjoint(int ifirst, int ilast, int jfirst, int jlast, int kfi
        !grho)
       return set_error(E_INVAL, "addsgd4_adjoint: null argument");
    SgCoef c{dcx, dcy, dcz, strx, stry, strz, cox, coy, coz};
-   return launch_addsgd4_T(make_box(ifirst, ilast, jfirst, jlast, kfirst, klast), y, u, um, rho, c, beta, dbar, grho,
+   return launch_addsgd4_T_march(make_box(ifirst, ilast, jfirst, jlast, kfirst, klast), y, u, um, rho, c, beta, dbar, grho,
                            (hipStream_t)stream);
 }
 
@@ -716,6 +716,19 @@ int sw4hip_ew_gradient(sw4hip_ew* h, int nsteps, int ckpt, const double* grad_tr
 {
    EWCHK(h);
    return h->ew->gradient(nsteps, ckpt, grad_traces, grho, gmu, glambda, on_device != 0);
+}
+
+int sw4hip_ew_gradient_plan(sw4hip_ew* h, int nsteps, int ckpt, int* used_ckpt, double* device_bytes)
+{
+   EWCHK(h);
+   if (!used_ckpt) return set_error(E_INVAL, "null output");
+   return h->ew->gradient_plan(nsteps, ckpt, used_ckpt, device_bytes);
+}
+
+int sw4hip_ew_release_workspace(sw4hip_ew* h)
+{
+   EWCHK(h);
+   return h->ew->release_workspace();
 }
 
 int sw4hip_ew_point_source_error(sw4hip_ew* h, double* linf, double* l2, double* solinf)

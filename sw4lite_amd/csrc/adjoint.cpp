@@ -91,7 +91,7 @@ int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double
                                c.stry, gmu, gla, plane, m_stream)))
       return rc;
    // damping: U^{n+1} -= D(U - Um); corrector: Up + c4/rho (L(A) + F_tt)
-   if ((rc = launch_addsgd4_T_pad(m_box, Y, u, um, mRho.dev, c, m_opt.beta, Db, grho, pad, m_stream, false)) ||
+   if ((rc = launch_addsgd4_T_march(m_box, Y, u, um, mRho.dev, c, m_opt.beta, Db, grho, m_stream, false)) ||
        (rc = launch_vjp_corr_pw(m_box, c4, mRho.dev, Y, u1, Up, Z, grho, m_stream)) ||
        (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, LT1, pad, m_stream)) ||
        (rc = launch_rhs4sg_matgrad(m_box, rm, m_dsbp, Z, Acc, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, gmu,
@@ -113,6 +113,97 @@ int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double
    return OK;
 }
 
+void* EW::pool_take(size_t bytes)
+{
+   for (PoolBuf& b : m_pool)
+      if (!b.used && b.bytes == bytes) {
+         b.used = true;
+         return b.p;
+      }
+   void* p = nullptr;
+   if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      // make room: drop the idle buffers of other sizes, then try once more
+      std::vector<PoolBuf> keep;
+      for (const PoolBuf& b : m_pool) {
+         if (b.used)
+            keep.push_back(b);
+         else
+            (void)hipFree(b.p);
+      }
+      m_pool.swap(keep);
+      if (hipMalloc(&p, bytes) != hipSuccess) {
+         (void)hipGetLastError();
+         return nullptr;
+      }
+   }
+   m_pool.push_back(PoolBuf{bytes, p, true});
+   return p;
+}
+
+void EW::pool_return()
+{
+   for (PoolBuf& b : m_pool) b.used = false;
+}
+
+int EW::release_workspace()
+{
+   int rc = check_hip(hipStreamSynchronize(m_stream), "EW: workspace release");
+   for (const PoolBuf& b : m_pool) (void)hipFree(b.p);
+   m_pool.clear();
+   return rc;
+}
+
+double EW::workspace_bytes() const
+{
+   double t = 0;
+   for (const PoolBuf& b : m_pool) t += (double)b.bytes;
+   return t;
+}
+
+// Device doubles of the gradient's work memory besides the stored states:
+// 6 work fields, 3 adjoint fields, 3 gradients, the padded material, one
+// plane of the boundary transposes and the receiver traces.
+static double adjoint_fixed_doubles(const Box& b, int nsteps, int nrec)
+{
+   return 9.0 * 3 * b.npts + 3.0 * b.npts + (double)adj_pad_doubles(b) + 3.0 * b.nij + 3.0 * (nsteps + 1) * nrec +
+          2.0 * (8 * nrec + 1);
+}
+
+// States held by a checkpoint interval c: the segment buffer (c + 2) and two
+// per checkpoint.  The reverse sweep recomputes every segment but the last,
+// nsteps - (length of the last segment) forward steps in all.
+static double adjoint_states(int nsteps, int c) { return (c + 2) + 2.0 * ((nsteps + c - 1) / c); }
+
+int EW::gradient_plan(int nsteps, int ckpt, int* used, double* bytes)
+{
+   if (nsteps < 1) return set_error(E_INVAL, "EW: gradient plan needs nsteps >= 1");
+   const int nrec = (int)m_rec.size() / 4;
+   const double n3 = 3.0 * m_box.npts, fixed = adjoint_fixed_doubles(m_box, nsteps, nrec);
+   int c = std::min(ckpt, nsteps);
+   if (ckpt < 1) {
+      // the largest interval whose states fit in free device memory (all
+      // of them when they fit: no recomputation), keeping a reserve
+      size_t freeb = 0, totalb = 0;
+      if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
+         (void)hipGetLastError();
+         freeb = 0;
+      }
+      const double reserve = 6.0 * (1 << 30) + 0.02 * (double)totalb;
+      const double room = ((double)freeb + workspace_bytes() - reserve) / 8.0 - fixed;
+      c = 0;
+      for (int q = nsteps; q >= 1 && room > 0; q--)
+         if (adjoint_states(nsteps, q) * n3 <= room) {
+            c = q;
+            break;
+         }
+      if (c < 1) c = std::max(1, (int)std::lround(std::sqrt((double)nsteps)));
+   }
+   if (used) *used = c;
+   if (bytes) *bytes = 8.0 * (fixed + adjoint_states(nsteps, c) * n3);
+   return OK;
+}
+
 int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* gmu, double* gla, bool on_device)
 {
    if (!m_setup_done) return set_error(E_STATE, "EW: gradient before setup");
@@ -125,22 +216,18 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
    if (nsteps < 1 || nsteps + 1 > m_maxsamples || nrec == 0)
       return set_error(E_INVAL, "EW: gradient needs receivers and 1 <= nsteps <= the set-up step count");
    if (!gtr || !grho || !gmu || !gla) return set_error(E_INVAL, "EW: null gradient argument");
-   if (ckpt < 1) ckpt = std::max(1, (int)std::lround(std::sqrt((double)nsteps)));
+   {
+      int rc0 = gradient_plan(nsteps, ckpt, &ckpt, nullptr);
+      if (rc0) return rc0;
+   }
    const long n1 = m_box.npts, n3 = 3 * n1;
    const int ncp = (nsteps + ckpt - 1) / ckpt;
    // work memory: 6 fields, 3 adjoint fields, ckpt+2 segment states, 2 per checkpoint, 3 gradients, pad, traces
-   std::vector<void*> mine;
-   auto release = [&]() {
-      for (void* p : mine) (void)hipFree(p);
-      mine.clear();
-   };
+   // buffers come from the workspace pool and go back to it at the end
+   auto release = [&]() { pool_return(); };
    auto grab = [&](size_t n, double** out) {
-      void* p = nullptr;
-      if (hipMalloc(&p, sizeof(double) * n) != hipSuccess) {
-         (void)hipGetLastError();
-         return set_error(E_NOMEM, "EW: adjoint work allocation failed");
-      }
-      mine.push_back(p);
+      void* p = pool_take(sizeof(double) * n);
+      if (!p) return set_error(E_NOMEM, "EW: adjoint work allocation failed");
       *out = static_cast<double*>(p);
       return (int)OK;
    };
@@ -177,14 +264,8 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
    int* dint = nullptr;
    if (rc == OK) rc = grab(ntr, &dtr);
    if (rc == OK) {
-      void* p = nullptr;
-      if (hipMalloc(&p, sizeof(int) * hint.size()) != hipSuccess) {
-         (void)hipGetLastError();
-         rc = set_error(E_NOMEM, "EW: adjoint source allocation failed");
-      } else {
-         mine.push_back(p);
-         dint = static_cast<int*>(p);
-      }
+      dint = static_cast<int*>(pool_take(sizeof(int) * hint.size()));
+      if (!dint) rc = set_error(E_NOMEM, "EW: adjoint source allocation failed");
    }
    if (rc == OK)
       rc = check_hip(hipMemcpyAsync(dint, hint.data(), sizeof(int) * hint.size(), hipMemcpyHostToDevice, m_stream),

@@ -21,6 +21,7 @@ EW::EW(const EWOptions& opt) : m_opt(opt)
 EW::~EW()
 {
    for (void* p : m_allocs) (void)hipFree(p);
+   for (const PoolBuf& b : m_pool) (void)hipFree(b.p);
 }
 
 int EW::alloc(Sarray& a, int nc)

@@ -99,6 +99,14 @@ public:
    // [nsteps+1][nreceivers][3], the misfit's derivative with respect to each
    // recorded sample.  Outputs are overwritten (device pointers if on_device).
    int gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* gmu, double* gla, bool on_device);
+   // checkpoint interval the gradient uses for ckpt (< 1: from free device
+   // memory and the workspace already held) and the device bytes it needs
+   int gradient_plan(int nsteps, int ckpt, int* used, double* bytes);
+   // The gradient's device buffers stay allocated between calls (a pool of
+   // exact-size buffers, grown as needed) so that repeated gradients skip
+   // allocating hundreds of GB; this frees them.
+   int release_workspace();
+   double workspace_bytes() const;
 
    // receivers (TimeSeries, EW.C:5854-5875): mode 0 displacement, 1 velocity
    int add_receiver(double x, double y, double z, int mode, int* index);
@@ -129,6 +137,14 @@ public:
    int m_kchunk = 64;   // planes per workgroup of the LDS sweeps
 
 private:
+   struct PoolBuf {
+      size_t bytes;
+      void* p;
+      bool used;
+   };
+   std::vector<PoolBuf> m_pool;  // gradient workspace
+   void* pool_take(size_t bytes);
+   void pool_return();
    int alloc(Sarray& a, int nc);
    int upload_sources();
    int ensure_reference_arrays();

@@ -33,12 +33,10 @@ struct AdjPad {
    double* sx;  // pb.ie - pb.ib + 1 values, and so on
    double* sy;
    double* sz;
-   double* wb;  // W on the box (3 components), 2W, and S = strx stry strz:
-   double* w2b; // the inputs of the LDS predictor sweep that evaluates
-   double* sb;  // cof L(W)/S = 2W - 2W + cof L(W)/S with rho := S, dt = 1
-   double* gy;  // damping transpose on pb: beta y / rho on the interior (3 components),
-   double* dd;  // u - um in the box (3 components),
-   double* rh;  // rho in the box (0 in the pad)
+   double* w2;  // 2W (3 components), S = sx sy sz and the sweep's output
+   double* s;   // (3 components), all on pb: the LDS predictor sweep over pb
+   double* o;   // evaluates cof L(W)/S = 2W - 2W + cof L(W)/S (rho := S, dt = 1)
+                // on every point of the box
 };
 Box pad_box(const Box& b);
 long adj_pad_doubles(const Box& b);                 // doubles AdjPad needs
@@ -60,12 +58,10 @@ int launch_rhs4sg_matgrad(const Box& b, const RowMap& m, const SbpTables* d_sbp,
 // dbar = -D^T y (overwritten on every point), grho += d/drho [-y . D].
 // inv_rho = false leaves out the derivative through the 1/rho factor at the
 // damped point (y . D / rho), for callers that form it from state differences.
-int launch_addsgd4_T(const Box& b, const double* y, const double* u, const double* um, const double* rho,
-                     const SgCoef& sg, double beta, double* dbar, double* grho, hipStream_t s, bool inv_rho = true);
-// As launch_addsgd4_T with unchecked reads of padded fields (pad as above).
-int launch_addsgd4_T_pad(const Box& b, const double* y, const double* u, const double* um, const double* rho,
-                         const SgCoef& sg, double beta, double* dbar, double* grho, const AdjPad& pad, hipStream_t s,
-                         bool inv_rho = true);
+// LDS k-march over column tiles.
+int launch_addsgd4_T_march(const Box& b, const double* y, const double* u, const double* um, const double* rho,
+                           const SgCoef& sg, double beta, double* dbar, double* grho, hipStream_t s,
+                           bool inv_rho = true);
 // Transpose of bcfortsg with zero boundary data on the adjoint y of its
 // output (sides in reverse order): Dirichlet/supergrid ghosts are dropped, a
 // free-surface ghost plane is pushed into the points its closure reads.  The

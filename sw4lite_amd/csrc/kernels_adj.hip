@@ -10,6 +10,8 @@
 // in-plane part of the SBP rows is the same with S = strx stry.  The k
 // direction of the SBP rows (acof, ghcof, bope terms, rhs4sg_rev.C:353-629)
 // is transposed term by term.
+#include <algorithm>
+
 #include "kernels_adj.h"
 #include "kernels_fast.h"
 #include "kernels_step.h"
@@ -49,6 +51,11 @@ __device__ __forceinline__ bool sbp_row(const RowMap& m, int i, int j, int k)
 {
    return i >= 1 && i <= m.nx && j >= 1 && j <= m.ny &&
           ((m.top && k >= 1 && k <= 6) || (m.bot && k >= m.nz - 5 && k <= m.nz));
+}
+// plane k holds rows (of either kind) on its columns 1..nx x 1..ny
+__device__ __forceinline__ bool plane_rows(const RowMap& m, int k)
+{
+   return (k >= m.k1 && k <= m.k2) || (m.top && k >= 1 && k <= 6) || (m.bot && k >= m.nz - 5 && k <= m.nz);
 }
 
 struct AdjCtx {
@@ -134,27 +141,45 @@ __device__ void sbp_col_T(const AdjCtx& A, const double* z, int i, int j, int k,
    const SbpTables* sbp = A.sbp;
    const double i12 = 1.0 / 12;
    const bool col = i >= 1 && i <= A.m.nx && j >= 1 && j <= A.m.ny;
-#pragma unroll 1
+   // mu, lambda on the side's planes m = 1..8 of this column
+   double muc[8], lac[8];
+#pragma unroll
+   for (int mm = 0; mm < 8; mm++) {
+      const int kmg = kbase + sgn * mm;
+      muc[mm] = A.vmu(i, j, kmg);
+      lac[mm] = A.vla(i, j, kmg);
+   }
+   // mu, lambda around this point in its plane (x and y taps)
+   double mxa[4], mya[4], lxa[4], lya[4];
+   const double sxa[4] = {A.sx(i + 2), A.sx(i + 1), A.sx(i - 1), A.sx(i - 2)};
+   const double sya[4] = {A.sy(j + 2), A.sy(j + 1), A.sy(j - 1), A.sy(j - 2)};
+#pragma unroll
+   for (int t = 0; t < 4; t++) {
+      const int a = t < 2 ? t - 2 : t - 1;
+      mxa[t] = A.vmu(i - a, j, k);
+      mya[t] = A.vmu(i, j - a, k);
+      lxa[t] = A.vla(i - a, j, k);
+      lya[t] = A.vla(i, j - a, k);
+   }
+#pragma unroll
    for (int kr = 1; kr <= 6; kr++) {
       const int krg = kbase + sgn * (kr - 1);
-      if (col) {
-         // acof / ghcof: u_c(q) enters row kr of the same column
-         const double z0 = A.zb(z, 0, i, j, krg), z1 = A.zb(z, 1, i, j, krg), z2 = A.zb(z, 2, i, j, krg);
-         double amu = 0, a2ml = 0;
-         if (q == 0) {
-            const int k1g = kbase;  // plane m = 1
-            const double gh = sbp->ghcof[kr - 1];
-            amu = gh * A.vmu(i, j, k1g);
-            a2ml = gh * (2 * A.vmu(i, j, k1g) + A.vla(i, j, k1g));
-         } else {
-#pragma unroll 1
-            for (int mm = 1; mm <= 8; mm++) {
-               const double a = sbp->acof[(kr - 1) + 6 * (q - 1) + 48 * (mm - 1)];
-               const int kmg = kbase + sgn * (mm - 1);
-               amu += a * A.vmu(i, j, kmg);
-               a2ml += a * (2 * A.vmu(i, j, kmg) + A.vla(i, j, kmg));
-            }
+      const double z0 = A.zb(z, 0, i, j, krg), z1 = A.zb(z, 1, i, j, krg), z2 = A.zb(z, 2, i, j, krg);
+      // acof / ghcof: u_c(q) enters row kr of the same column
+      double amu = 0, a2ml = 0;
+      if (q == 0) {
+         const double gh = sbp->ghcof[kr - 1];
+         amu = gh * muc[0];
+         a2ml = gh * (2 * muc[0] + lac[0]);
+      } else {
+#pragma unroll
+         for (int mm = 0; mm < 8; mm++) {
+            const double a = sbp->acof[(kr - 1) + 6 * (q - 1) + 48 * mm];
+            amu += a * muc[mm];
+            a2ml += a * (2 * muc[mm] + lac[mm]);
          }
+      }
+      if (col) {
          r[0] += z0 * amu;
          r[1] += z1 * amu;
          r[2] += z2 * a2ml;
@@ -162,23 +187,23 @@ __device__ void sbp_col_T(const AdjCtx& A, const double* z, int i, int j, int k,
       if (q < 1) continue;
       const double bq = sgn * sbp->bope[(kr - 1) + 6 * (q - 1)];
       double s2 = 0, s0 = 0, s1 = 0;
-      const double lar = A.vla(i, j, krg), mur = A.vmu(i, j, krg);
+      const double lar = lac[kr - 1], mur = muc[kr - 1];
 #pragma unroll
       for (int t = 0; t < 4; t++) {
          const int a = t < 2 ? t - 2 : t - 1;
          const double ca = cd4(a);
-         const double wx0 = A.sx(i - a) * A.zb(z, 0, i - a, j, krg);
-         const double wy1 = A.sy(j - a) * A.zb(z, 1, i, j - a, krg);
-         const double wx2 = A.sx(i - a) * A.zb(z, 2, i - a, j, krg);
-         const double wy2 = A.sy(j - a) * A.zb(z, 2, i, j - a, krg);
+         const double wx0 = sxa[t] * A.zb(z, 0, i - a, j, krg);
+         const double wy1 = sya[t] * A.zb(z, 1, i, j - a, krg);
+         const double wx2 = sxa[t] * A.zb(z, 2, i - a, j, krg);
+         const double wy2 = sya[t] * A.zb(z, 2, i, j - a, krg);
          // (la w_z)_x, (la w_z)_y: coefficient la at this column, row plane
          s2 += ca * lar * (wx0 + wy1);
          // (mu w_x)_z, (mu w_y)_z: coefficient mu at the row's column, plane q
-         s2 += ca * (A.vmu(i - a, j, k) * wx0 + A.vmu(i, j - a, k) * wy1);
+         s2 += ca * (mxa[t] * wx0 + mya[t] * wy1);
          // (mu u_z)_x and (la u_x)_z
-         s0 += ca * (mur * wx2 + A.vla(i - a, j, k) * wx2);
+         s0 += ca * (mur * wx2 + lxa[t] * wx2);
          // (mu v_z)_y and (la v_y)_z
-         s1 += ca * (mur * wy2 + A.vla(i, j - a, k) * wy2);
+         s1 += ca * (mur * wy2 + lya[t] * wy2);
       }
       r[0] += bq * i12 * s0;
       r[1] += bq * i12 * s1;
@@ -186,12 +211,43 @@ __device__ void sbp_col_T(const AdjCtx& A, const double* z, int i, int j, int k,
    }
 }
 
+// Planes per workgroup for an LDS k-march of `tiles` column tiles over `npl`
+// planes when `per_cu` workgroups fit on a CU: the k-block count near npl/64
+// that minimises (rounds of resident workgroups) x (planes per workgroup +
+// the 4 planes each one primes).  Workgroups run in rounds, so a count just
+// above a multiple of the resident slots costs a whole extra round.
+int march_kchunk(long tiles, int npl, int per_cu)
+{
+   static int cus = 0;
+   if (!cus) {
+      int dev = 0, v = 0;
+      if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) {
+         (void)hipGetLastError();
+         v = 256;
+      }
+      cus = v;
+   }
+   const long slots = (long)cus * per_cu;
+   const int nc0 = std::max(1, (npl + 32) / 64);
+   int best = nc0;
+   long bcost = -1;
+   for (int nc = std::max(1, nc0 / 2); nc <= 2 * nc0 + 1; nc++) {
+      const long rounds = (tiles * nc + slots - 1) / slots;
+      const long cost = rounds * ((npl + nc - 1) / nc + 4);
+      if (bcost < 0 || cost < bcost) {
+         bcost = cost;
+         best = nc;
+      }
+   }
+   return (npl + best - 1) / best;
+}
+
 Box pad_box(const Box& b) { return make_box(b.ib - 2, b.ie + 2, b.jb - 2, b.je + 2, b.kb - 2, b.ke + 2); }
 
 long adj_pad_doubles(const Box& b)
 {
    const Box pb = pad_box(b);
-   return 12 * pb.npts + pb.ni + (pb.je - pb.jb + 1) + (pb.ke - pb.kb + 1) + 7 * b.npts;
+   return 12 * pb.npts + pb.ni + (pb.je - pb.jb + 1) + (pb.ke - pb.kb + 1);
 }
 
 AdjPad make_adj_pad(const Box& b, double* mem)
@@ -204,12 +260,10 @@ AdjPad make_adj_pad(const Box& b, double* mem)
    p.sx = p.la + p.pb.npts;
    p.sy = p.sx + p.pb.ni;
    p.sz = p.sy + (p.pb.je - p.pb.jb + 1);
-   p.wb = p.sz + (p.pb.ke - p.pb.kb + 1);
-   p.w2b = p.wb + 3 * b.npts;
-   p.sb = p.w2b + 3 * b.npts;
-   p.gy = p.sb + b.npts;
-   p.dd = p.gy + 3 * p.pb.npts;
-   p.rh = p.dd + 3 * p.pb.npts;
+   p.w2 = p.sz + (p.pb.ke - p.pb.kb + 1);
+   p.s = p.w2 + 3 * p.pb.npts;
+   p.o = p.s + p.pb.npts;
+
    return p;
 }
 
@@ -232,13 +286,14 @@ __global__ void k_adj_pad_material(Box b, Box pb, const double* __restrict__ mu,
    if (q < pb.ke - pb.kb + 1) szp[q] = (pb.kb + q >= b.kb && pb.kb + q <= b.ke) ? strz[pb.kb + q - b.kb] : 1.0;
 }
 
-__global__ void k_adj_sfield(Box b, const double* __restrict__ strx, const double* __restrict__ stry,
-                             const double* __restrict__ strz, double* __restrict__ sb)
+// S = sx sy sz on the padded box, from the padded stretching
+__global__ void k_adj_sfield(Box pb, const double* __restrict__ sx, const double* __restrict__ sy,
+                             const double* __restrict__ sz, double* __restrict__ s)
 {
    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   if (p >= b.npts) return;
-   const long i = p % b.ni, j = (p / b.ni) % (b.je - b.jb + 1), k = p / b.nij;
-   sb[p] = strx[i] * stry[j] * strz[k];
+   if (p >= pb.npts) return;
+   const long i = p % pb.ni, j = (p / pb.ni) % (pb.je - pb.jb + 1), k = p / pb.nij;
+   s[p] = sx[i] * sy[j] * sz[k];
 }
 
 // padded W: INPLANE = false: S zI (interior-formula rows, S = sx sy sz);
@@ -255,32 +310,24 @@ __global__ void k_adj_weight(AdjCtx A, AdjPad P, const double* __restrict__ z)
    const bool row = INPLANE ? sbp_row(A.m, i, j, k) : int_row(A.m, i, j, k);
    const double s = row ? A.sx(i) * A.sy(j) * (INPLANE ? 1.0 : A.sz(k)) : 0.0;
    const long p = row ? sidx(A.b, i, j, k) : 0;
-   const bool in = inbox(A.b, i, j, k);
-   const long pbx = in ? sidx(A.b, i, j, k) : 0;
    for (int c = 0; c < 3; c++) {
       const double v = row ? s * z[p + A.b.npts * c] : 0.0;
       P.w[q + pb.npts * c] = v;
-      if (!INPLANE && in) {
-         P.wb[pbx + A.b.npts * c] = v;
-         P.w2b[pbx + A.b.npts * c] = 2 * v;
-      }
+      if (!INPLANE) P.w2[q + pb.npts * c] = 2 * v;
    }
 }
 
-// interior rows on the ghost shell: out = cof S^{-1} L(W) (the forward
-// interior formula on the padded arrays); the interior points are done by
-// the LDS predictor sweep
-__global__ void __launch_bounds__(256) k_rhs4sg_T_shell(AdjCtx A, AdjPad P, double* __restrict__ out, double cof)
+// out = o on the box (o lives on the padded box)
+__global__ void k_adj_crop(Box b, Box pb, const double* __restrict__ o, double* __restrict__ out)
 {
-   const Box& b = A.b;
-   const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   int i, j, k;
-   if (!shell_point(b, q, i, j, k)) return;
-   double r[3];
-   rhs_interior_point<1>(P.pb, i, j, k, Plain{P.w}, P.mu, P.la, P.sx, P.sy, P.sz, r);
-   const double sc = cof / (A.sx(i) * A.sy(j) * A.sz(k));
-   const long p = sidx(b, i, j, k);
-   for (int c = 0; c < 3; c++) out[p + b.npts * c] = sc * r[c];
+   const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (p >= b.npts) return;
+   const int i = b.ib + (int)(p % b.ni);
+   const int j = b.jb + (int)((p / b.ni) % (b.je - b.jb + 1));
+   const int k = b.kb + (int)(p / b.nij);
+   const long q = sidx(pb, i, j, k);
+#pragma unroll
+   for (int c = 0; c < 3; c++) out[p + b.npts * c] = o[q + pb.npts * c];
 }
 
 // Accessor of rhs4_xy_body over the padded arrays (no checks).
@@ -334,69 +381,6 @@ __global__ void __launch_bounds__(256) k_rhs4sg_T_col(AdjCtx A, int kbase, int s
 // ---------------------------------------------------------------------------
 // Material derivatives
 // ---------------------------------------------------------------------------
-// Psi_t(f) = sum_q C[q][t] (f[o_q] - f[0]): the factor of a[t] in varsum4 with
-// the coefficients varcoef4 builds from a[0..4] (offsets -2..2).
-__device__ __forceinline__ double psi(int t, const double f[5])
-{
-   const double d0 = f[0] - f[2], d1 = f[1] - f[2], d3 = f[3] - f[2], d4v = f[4] - f[2];
-   switch (t) {
-   case 0: return -0.75 * d0 + d1;
-   case 1: return d0 + 3 * d1 + d3;
-   case 2: return -0.75 * d0 + 3 * d1 + 3 * d3 - 0.75 * d4v;
-   case 3: return d1 + 3 * d3 + d4v;
-   default: return d3 - 0.75 * d4v;
-   }
-}
-
-// Second-difference terms along axis AX of the rows selected by ROWS (0
-// interior formula, 1 SBP rows): contribution to (gmu, gla) at p.
-template <int AX, int ROWS>
-__device__ __forceinline__ void grad_second(const AdjCtx& A, const double* z, const double* u, int i, int j, int k,
-                                            double& gm, double& gl)
-{
-   const double sp = AX == 0 ? A.sx(i) : AX == 1 ? A.sy(j) : A.sz(k);
-#pragma unroll 1
-   for (int t = 0; t < 5; t++) {
-      const int o = 2 - t;  // row = p + o e_AX
-      const int ir = i + (AX == 0) * o, jr = j + (AX == 1) * o, kr = k + (AX == 2) * o;
-      const bool ok = ROWS == 0 ? int_row(A.m, ir, jr, kr) : sbp_row(A.m, ir, jr, kr);
-      if (!ok) continue;
-      const double srow = AX == 0 ? A.sx(ir) : AX == 1 ? A.sy(jr) : A.sz(kr);
-      const double w = srow * sp * (1.0 / 6);
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-         double f[5];
-#pragma unroll
-         for (int s = 0; s < 5; s++) {
-            const int ii = ir + (AX == 0) * (s - 2), jj = jr + (AX == 1) * (s - 2), kk = kr + (AX == 2) * (s - 2);
-            f[s] = A.f(u, c, ii, jj, kk);
-         }
-         const double val = z[sidx(A.b, ir, jr, kr) + A.b.npts * c] * w * psi(t, f);
-         gm += (c == AX ? 2.0 : 1.0) * val;
-         if (c == AX) gl += val;
-      }
-   }
-}
-
-// Mixed term (coef D_DB u_C)_DA of row component RC: coef at p.
-template <int RC, int DA, int DB, int C, int ROWS>
-__device__ __forceinline__ double grad_mixed(const AdjCtx& A, const double* z, const double* u, int i, int j, int k)
-{
-   double D = 0, W = 0;
-#pragma unroll
-   for (int t = 0; t < 4; t++) {
-      const int a = t < 2 ? t - 2 : t - 1;
-      const int iu = i + (DB == 0) * a, ju = j + (DB == 1) * a, ku = k + (DB == 2) * a;
-      D += cd4(a) * A.f(u, C, iu, ju, ku);
-      const int ir = i - (DA == 0) * a, jr = j - (DA == 1) * a, kr = k - (DA == 2) * a;
-      const double zr = ROWS == 0 ? A.zi(z, RC, ir, jr, kr) : A.zb(z, RC, ir, jr, kr);
-      const double sa = DA == 0 ? A.sx(ir) : DA == 1 ? A.sy(jr) : A.sz(kr);
-      const double sb = DB == 0 ? A.sx(ir) : DB == 1 ? A.sy(jr) : A.sz(kr);
-      W += cd4(a) * sa * sb * zr;
-   }
-   return D * W * (1.0 / 144);
-}
-
 // acof / ghcof / bope terms of the SBP rows of one side at p
 __device__ void sbp_col_grad(const AdjCtx& A, const double* z, const double* u, int i, int j, int k, int kbase,
                              int sgn, double& gm, double& gl)
@@ -406,59 +390,62 @@ __device__ void sbp_col_grad(const AdjCtx& A, const double* z, const double* u, 
    const SbpTables* sbp = A.sbp;
    const double i12 = 1.0 / 12;
    const bool col = i >= 1 && i <= A.m.nx && j >= 1 && j <= A.m.ny;
-#pragma unroll 1
+   // u on the side's planes 1..8 of this column, and on its ghost plane
+   double uk[8][3], ug[3];
+#pragma unroll
+   for (int qq = 0; qq < 8; qq++) {
+      const int kq = kbase + sgn * qq;
+#pragma unroll
+      for (int c = 0; c < 3; c++) uk[qq][c] = A.f(u, c, i, j, kq);
+   }
+#pragma unroll
+   for (int c = 0; c < 3; c++) ug[c] = A.f(u, c, i, j, kbase - sgn);
+   // x and y differences of u in this point's plane
+   double dx2 = 0, dy2 = 0, dx0 = 0, dy1 = 0;
+#pragma unroll
+   for (int t = 0; t < 4; t++) {
+      const int a = t < 2 ? t - 2 : t - 1;
+      dx2 += cd4(a) * A.f(u, 2, i + a, j, k);
+      dy2 += cd4(a) * A.f(u, 2, i, j + a, k);
+      dx0 += cd4(a) * A.f(u, 0, i + a, j, k);
+      dy1 += cd4(a) * A.f(u, 1, i, j + a, k);
+   }
+   const double sxi = A.sx(i), syj = A.sy(j);
+#pragma unroll
    for (int kr = 1; kr <= 6; kr++) {
       const int krg = kbase + sgn * (kr - 1);
+      const double z0 = A.zb(z, 0, i, j, krg), z1 = A.zb(z, 1, i, j, krg), z2 = A.zb(z, 2, i, j, krg);
       if (col) {
-         const double z0 = A.zb(z, 0, i, j, krg), z1 = A.zb(z, 1, i, j, krg), z2 = A.zb(z, 2, i, j, krg);
          // mu, la at plane m = q enter row kr through acof(kr, q', q) u(q')
          double sm = 0, sl = 0;
-#pragma unroll 1
-         for (int qq = 1; qq <= 8; qq++) {
-            const double a = sbp->acof[(kr - 1) + 6 * (qq - 1) + 48 * (q - 1)];
-            const int kq = kbase + sgn * (qq - 1);
-            const double u0 = A.f(u, 0, i, j, kq), u1 = A.f(u, 1, i, j, kq), u2 = A.f(u, 2, i, j, kq);
-            sm += a * (z0 * u0 + z1 * u1 + 2 * z2 * u2);
-            sl += a * z2 * u2;
+#pragma unroll
+         for (int qq = 0; qq < 8; qq++) {
+            const double a = sbp->acof[(kr - 1) + 6 * qq + 48 * (q - 1)];
+            sm += a * (z0 * uk[qq][0] + z1 * uk[qq][1] + 2 * z2 * uk[qq][2]);
+            sl += a * z2 * uk[qq][2];
          }
          if (q == 1) {  // ghost term gh mu(1) u(0)
             const double gh = sbp->ghcof[kr - 1];
-            const int k0g = kbase - sgn;
-            const double u0 = A.f(u, 0, i, j, k0g), u1 = A.f(u, 1, i, j, k0g), u2 = A.f(u, 2, i, j, k0g);
-            sm += gh * (z0 * u0 + z1 * u1 + 2 * z2 * u2);
-            sl += gh * z2 * u2;
+            sm += gh * (z0 * ug[0] + z1 * ug[1] + 2 * z2 * ug[2]);
+            sl += gh * z2 * ug[2];
          }
          gm += sm;
          gl += sl;
       }
       // (mu w_x)_z, (mu w_y)_z, (la u_x)_z, (la v_y)_z: coefficient at (i,j) plane q, row (i,j,kr)
-      {
-         const double bq = sgn * sbp->bope[(kr - 1) + 6 * (q - 1)];
-         double dx2 = 0, dy2 = 0, dx0 = 0, dy1 = 0;
-#pragma unroll
-         for (int t = 0; t < 4; t++) {
-            const int a = t < 2 ? t - 2 : t - 1;
-            dx2 += cd4(a) * A.f(u, 2, i + a, j, k);
-            dy2 += cd4(a) * A.f(u, 2, i, j + a, k);
-            dx0 += cd4(a) * A.f(u, 0, i + a, j, k);
-            dy1 += cd4(a) * A.f(u, 1, i, j + a, k);
-         }
-         const double z0 = A.zb(z, 0, i, j, krg), z1 = A.zb(z, 1, i, j, krg), z2 = A.zb(z, 2, i, j, krg);
-         gm += bq * i12 * (A.sx(i) * dx2 * z0 + A.sy(j) * dy2 * z1);
-         gl += bq * i12 * (A.sx(i) * dx0 + A.sy(j) * dy1) * z2;
-      }
+      const double bq = sgn * sbp->bope[(kr - 1) + 6 * (q - 1)];
+      gm += bq * i12 * (sxi * dx2 * z0 + syj * dy2 * z1);
+      gl += bq * i12 * (sxi * dx0 + syj * dy1) * z2;
    }
    // (la w_z)_x, (la w_z)_y, (mu u_z)_x, (mu v_z)_y: coefficient at p on row plane kr = q
    if (q <= 6) {
-      const int kr = q;
       double bu0 = 0, bu1 = 0, bu2 = 0;
-#pragma unroll 1
-      for (int qq = 1; qq <= 8; qq++) {
-         const double bq = sgn * sbp->bope[(kr - 1) + 6 * (qq - 1)];
-         const int kq = kbase + sgn * (qq - 1);
-         bu0 += bq * A.f(u, 0, i, j, kq);
-         bu1 += bq * A.f(u, 1, i, j, kq);
-         bu2 += bq * A.f(u, 2, i, j, kq);
+#pragma unroll
+      for (int qq = 0; qq < 8; qq++) {
+         const double bq = sgn * sbp->bope[(q - 1) + 6 * qq];
+         bu0 += bq * uk[qq][0];
+         bu1 += bq * uk[qq][1];
+         bu2 += bq * uk[qq][2];
       }
       double wx0 = 0, wy1 = 0, wx2 = 0, wy2 = 0;
 #pragma unroll
@@ -528,7 +515,7 @@ __global__ void __launch_bounds__(GNT, 2) k_rhs4sg_matgrad_march(AdjCtx A, int n
    const int k0 = b.kb + tkz * kc, k1 = min(b.ke, k0 + kc - 1);
    const long n = b.npts;
    // staging slots: in-plane offset of the point (-1 off the box) and whether
-   // its column carries interior-formula rows
+   // its column carries rows (interior-formula or SBP rows: same columns)
    int soff[GSL];
    bool srow[GSL];
 #pragma unroll
@@ -543,7 +530,7 @@ __global__ void __launch_bounds__(GNT, 2) k_rhs4sg_matgrad_march(AdjCtx A, int n
    double pu[GSL][3], pz[GSL][3];
    auto fetch = [&](int kk) {  // plane kk (inside the box) into pu, pz
       const long pk = (long)(kk - b.kb) * b.nij;
-      const bool krow = kk >= m.k1 && kk <= m.k2;
+      const bool krow = plane_rows(m, kk);
 #pragma unroll
       for (int q = 0; q < GSL; q++) {
          const long a = pk + max(soff[q], 0);
@@ -639,11 +626,14 @@ __global__ void __launch_bounds__(GNT, 2) k_rhs4sg_matgrad_march(AdjCtx A, int n
             gs += S;
             if (c == 2) gd += S;
          }
-         const double s0 = sxo[2], s1 = syo[2], s2 = szr[2];
-         const double ml = V[0][0] * (s1 * D[1][1] + s2 * D[2][2]) + V[1][1] * (s0 * D[0][0] + s2 * D[2][2]) +
+         // on an SBP row plane only the in-plane terms belong to the rows
+         // (their k terms are the acof/bope stencils of k_rhs4sg_matgrad_sbp)
+         const double fi = k >= m.k1 && k <= m.k2 ? 1.0 : 0.0;
+         const double s0 = sxo[2], s1 = syo[2], s2 = szr[2], s2i = fi * s2;
+         const double ml = V[0][0] * (s1 * D[1][1] + s2i * D[2][2]) + V[1][1] * (s0 * D[0][0] + s2i * D[2][2]) +
                            V[2][2] * (s0 * D[0][0] + s1 * D[1][1]);
          const double mm = s0 * (D[0][1] * V[0][1] + D[0][2] * V[0][2]) + s1 * (D[1][0] * V[1][0] + D[1][2] * V[1][2]) +
-                           s2 * (D[2][0] * V[2][0] + D[2][1] * V[2][1]);
+                           s2i * (D[2][0] * V[2][0] + D[2][1] * V[2][1]);
          const long p = (long)(k - b.kb) * b.nij + col;
          gmu[p] += cof * ((gs + gd) * (1.0 / 6) + mm * (1.0 / 144));
          gla[p] += cof * (gd * (1.0 / 6) + ml * (1.0 / 144));
@@ -671,16 +661,9 @@ __global__ void __launch_bounds__(256) k_rhs4sg_matgrad_sbp(AdjCtx A, int kbase,
    const Box& b = A.b;
    const int i = b.ib + blockIdx.x * blockDim.x + threadIdx.x;
    const int j = b.jb + blockIdx.y * blockDim.y + threadIdx.y;
-   const int q = blockIdx.z;  // plane 0..8
-   const int k = kbase + sgn * (q - 1);
+   const int k = kbase + sgn * (int)blockIdx.z;  // planes 1..8 of the side
    if (i > b.ie || j > b.je || k < b.kb || k > b.ke) return;
    double gm = 0, gl = 0;
-   if (q >= 1 && q <= 6) {
-      grad_second<0, 1>(A, z, u, i, j, k, gm, gl);
-      grad_second<1, 1>(A, z, u, i, j, k, gm, gl);
-      gl += grad_mixed<0, 0, 1, 1, 1>(A, z, u, i, j, k) + grad_mixed<1, 1, 0, 0, 1>(A, z, u, i, j, k);
-      gm += grad_mixed<0, 1, 0, 1, 1>(A, z, u, i, j, k) + grad_mixed<1, 0, 1, 0, 1>(A, z, u, i, j, k);
-   }
    sbp_col_grad(A, z, u, i, j, k, kbase, sgn, gm, gl);
    const long p = sidx(b, i, j, k);
    gmu[p] += cof * gm;
@@ -700,176 +683,185 @@ struct DampCtx {
    {
       return i >= 1 && i <= nx && j >= 1 && j <= ny && k >= 1 && k <= nz;
    }
-   // g^ax = beta w_ax / rho y at interior points (w_x = strx coy coz, ...)
-   template <bool FAST = false>
-   __device__ __forceinline__ double g(int ax, const double* y, int c, int i, int j, int k) const
-   {
-      if (!FAST && !interior(i, j, k)) return 0.0;
-      const int ii = i - b.ib, jj = j - b.jb, kk = k - b.kb;
-      const double w = ax == 0 ? sg.strx[ii] * sg.coy[jj] * sg.coz[kk]
-                     : ax == 1 ? sg.stry[jj] * sg.cox[ii] * sg.coz[kk]
-                               : sg.strz[kk] * sg.cox[ii] * sg.coy[jj];
-      const long p = sidx(b, i, j, k);
-      return beta * w / rho[p] * y[p + b.npts * c];
-   }
-   // r^ax = rho dc_ax, 0 outside the box
-   template <bool FAST = false>
-   __device__ __forceinline__ double r(int ax, int i, int j, int k) const
-   {
-      if (!FAST && !inbox(b, i, j, k)) return 0.0;
-      const double dc = ax == 0 ? sg.dcx[i - b.ib] : ax == 1 ? sg.dcy[j - b.jb] : sg.dcz[k - b.kb];
-      return rho[sidx(b, i, j, k)] * dc;
-   }
 };
 
-template <int AX>
-__device__ __forceinline__ void step3(int& i, int& j, int& k, int d)
+// Damping transpose as an LDS k-march over 32 x 8 column tiles (the tile
+// geometry of k_rhs4sg_matgrad_march).  Planes of g = (beta/rho) y (interior
+// points), d = u - um and rho (box points), 0 elsewhere, with a halo of 2,
+// double-buffered with one barrier per plane; the own columns of g
+// (k-2..k+2), d and rho (k-1..k+1) rotate in registers.  Per point:
+// dbar = -sum_ax w_ax D2(r D2(s_ax g)) and grho += -sum_ax dc_ax D2(d) w_ax
+// D2(s_ax g) at the point (r = rho dc_ax; D2 the centred second difference;
+// w_x = coy coz, w_y = cox coz, w_z = cox coy).
+__global__ void __launch_bounds__(GNT, 2) k_addsgd4_T_march(DampCtx D, int ntx, int nty, int kc,
+                                                            const double* __restrict__ y, const double* __restrict__ u,
+                                                            const double* __restrict__ um, double* __restrict__ dbar,
+                                                            double* __restrict__ grho)
 {
-   if (AX == 0) i += d;
-   if (AX == 1) j += d;
-   if (AX == 2) k += d;
-}
-
-// (D2 g)(l) along AX
-template <int AX, bool FAST = false>
-__device__ __forceinline__ double d2g(const DampCtx& D, const double* y, int c, int i, int j, int k)
-{
-   int im = i, jm = j, km = k, ip = i, jp = j, kp = k;
-   step3<AX>(im, jm, km, -1);
-   step3<AX>(ip, jp, kp, 1);
-   return D.g<FAST>(AX, y, c, ip, jp, kp) - 2 * D.g<FAST>(AX, y, c, i, j, k) + D.g<FAST>(AX, y, c, im, jm, km);
-}
-
-// D^T y at p (dbar = -D^T y) and the derivative through r = rho dc at p;
-// FAST: p and its neighbours within 2 are interior points (no checks)
-template <bool FAST>
-__device__ __forceinline__ double damp_T_point(const DampCtx& D, const double* __restrict__ y,
-                                               const double* __restrict__ u, const double* __restrict__ um,
-                                               double* __restrict__ dbar, long p, int i, int j, int k)
-{
+   __shared__ double sgl[2][3 * GPL];
+   __shared__ double sdl[2][3 * GPL];
+   __shared__ double srl[2][GPL];
    const Box& b = D.b;
-   auto dd = [&](int c, int ii, int jj, int kk) {
-      if (!FAST && !inbox(b, ii, jj, kk)) return 0.0;
-      const long q = sidx(b, ii, jj, kk) + b.npts * c;
-      return u[q] - um[q];
-   };
-   double gr = 0;
-#pragma unroll
-   for (int c = 0; c < 3; c++) {
-      // D^T y = sum_ax D2 (r D2 g) at p
-      const double gx = d2g<0, FAST>(D, y, c, i, j, k), gy = d2g<1, FAST>(D, y, c, i, j, k),
-                   gz = d2g<2, FAST>(D, y, c, i, j, k);
-      double s = 0;
-      s += D.r<FAST>(0, i + 1, j, k) * d2g<0, FAST>(D, y, c, i + 1, j, k) - 2 * D.r<FAST>(0, i, j, k) * gx +
-           D.r<FAST>(0, i - 1, j, k) * d2g<0, FAST>(D, y, c, i - 1, j, k);
-      s += D.r<FAST>(1, i, j + 1, k) * d2g<1, FAST>(D, y, c, i, j + 1, k) - 2 * D.r<FAST>(1, i, j, k) * gy +
-           D.r<FAST>(1, i, j - 1, k) * d2g<1, FAST>(D, y, c, i, j - 1, k);
-      s += D.r<FAST>(2, i, j, k + 1) * d2g<2, FAST>(D, y, c, i, j, k + 1) - 2 * D.r<FAST>(2, i, j, k) * gz +
-           D.r<FAST>(2, i, j, k - 1) * d2g<2, FAST>(D, y, c, i, j, k - 1);
-      dbar[p + b.npts * c] = -s;
-      const double d0 = dd(c, i, j, k);
-      const double dxx = dd(c, i + 1, j, k) - 2 * d0 + dd(c, i - 1, j, k);
-      const double dyy = dd(c, i, j + 1, k) - 2 * d0 + dd(c, i, j - 1, k);
-      const double dzz = dd(c, i, j, k + 1) - 2 * d0 + dd(c, i, j, k - 1);
-      gr -= D.sg.dcx[i - b.ib] * dxx * gx + D.sg.dcy[j - b.jb] * dyy * gy + D.sg.dcz[k - b.kb] * dzz * gz;
-   }
-   return gr;
-}
-
-__global__ void __launch_bounds__(256) k_addsgd4_T(DampCtx D, const double* __restrict__ y,
-                                                   const double* __restrict__ u, const double* __restrict__ um,
-                                                   double* __restrict__ dbar, double* __restrict__ grho, int inv_rho)
-{
-   const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   const Box& b = D.b;
-   if (p >= b.npts) return;
-   const int i = b.ib + (int)(p % b.ni);
-   const int j = b.jb + (int)((p / b.ni) % (b.je - b.jb + 1));
-   const int k = b.kb + (int)(p / b.nij);
-   const bool fast = i >= 3 && i <= D.nx - 2 && j >= 3 && j <= D.ny - 2 && k >= 3 && k <= D.nz - 2;
-   double gr = fast ? damp_T_point<true>(D, y, u, um, dbar, p, i, j, k) : damp_T_point<false>(D, y, u, um, dbar, p, i, j, k);
-   if (inv_rho && D.interior(i, j, k)) {
-      double dm[3];
-      addsgd4_point<1>(b, i, j, k, u, um, D.rho, D.sg, D.beta, dm);
-      const double rp = D.rho[p];
-      gr += (y[p] * dm[0] + y[p + b.npts] * dm[1] + y[p + 2 * b.npts] * dm[2]) / rp;
-   }
-   grho[p] += gr;
-}
-
-// padded inputs of the damping transpose
-__global__ void k_damp_prep(DampCtx D, AdjPad P, const double* __restrict__ y, const double* __restrict__ u,
-                            const double* __restrict__ um)
-{
-   const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   const Box& pb = P.pb;
-   if (q >= pb.npts) return;
-   const int i = pb.ib + (int)(q % pb.ni);
-   const int j = pb.jb + (int)((q / pb.ni) % (pb.je - pb.jb + 1));
-   const int k = pb.kb + (int)(q / pb.nij);
-   const Box& b = D.b;
-   const bool in = inbox(b, i, j, k), itr = D.interior(i, j, k);
-   const long p = in ? sidx(b, i, j, k) : 0;
-   const double rp = D.rho[p];
-   P.rh[q] = in ? rp : 0.0;
-   for (int c = 0; c < 3; c++) {
-      const long v = p + b.npts * c;
-      P.gy[q + pb.npts * c] = itr ? D.beta * y[v] / rp : 0.0;
-      P.dd[q + pb.npts * c] = in ? u[v] - um[v] : 0.0;
-   }
-}
-
-// D^T y and the r = rho dc derivative at every box point from the padded
-// fields; along each axis the weight w_ax = str_ax co_b co_c factors into
-// co_b co_c (constant along the axis) and str_ax (inside the differences).
-__global__ void __launch_bounds__(256) k_addsgd4_T_pad(DampCtx D, AdjPad P, double* __restrict__ dbar,
-                                                       double* __restrict__ grho)
-{
-   const Box& b = D.b;
-   const Box& pb = P.pb;
-   const int i = b.ib + blockIdx.x * blockDim.x + threadIdx.x;
-   const int j = b.jb + blockIdx.y * blockDim.y + threadIdx.y;
-   const int k = b.kb + blockIdx.z;
-   if (i > b.ie || j > b.je) return;
-   const long q0 = sidx(pb, i, j, k);
-   const long sxs = 1, sys = pb.ni, szs = pb.nij;
-   const int ii = i - b.ib, jj = j - b.jb, kk = k - b.kb;
    const SgCoef& sg = D.sg;
-   const double* sx = P.sx + (i - pb.ib);  // stretching at i (padded tables, 1 in the pad)
-   const double* sy = P.sy + (j - pb.jb);
-   const double* sz = P.sz + (k - pb.kb);
-   auto cl = [](int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; };
-   const int ni = b.ie - b.ib, nj = b.je - b.jb, nk = b.ke - b.kb;
-   // r = rho dc at -1, 0, +1 along each axis (rho is 0 in the pad)
-   const double rx[3] = {P.rh[q0 - sxs] * sg.dcx[cl(ii - 1, 0, ni)], P.rh[q0] * sg.dcx[ii], P.rh[q0 + sxs] * sg.dcx[cl(ii + 1, 0, ni)]};
-   const double ry[3] = {P.rh[q0 - sys] * sg.dcy[cl(jj - 1, 0, nj)], P.rh[q0] * sg.dcy[jj], P.rh[q0 + sys] * sg.dcy[cl(jj + 1, 0, nj)]};
-   const double rz[3] = {P.rh[q0 - szs] * sg.dcz[cl(kk - 1, 0, nk)], P.rh[q0] * sg.dcz[kk], P.rh[q0 + szs] * sg.dcz[cl(kk + 1, 0, nk)]};
-   const double wx = sg.coy[jj] * sg.coz[kk], wy = sg.cox[ii] * sg.coz[kk], wz = sg.cox[ii] * sg.coy[jj];
-   double gr = 0;
+   const double* rho = D.rho;
+   const double beta = D.beta;
+   const int t = blockIdx.x;
+   const int tix = t % ntx, tr = t / ntx, tjy = tr % nty, tkz = tr / nty;
+   const int tid = threadIdx.x, tx = tid % GTX, ty = tid / GTX;
+   const int ia = b.ib + tix * GTX, ja = b.jb + tjy * GTY;
+   const int i = ia + tx, j = ja + ty;
+   const bool active = i <= b.ie && j <= b.je;
+   const int k0 = b.kb + tkz * kc, k1 = min(b.ke, k0 + kc - 1);
+   const long n = b.npts;
+   // staging slots: in-plane offset (-1 off the box), interior column
+   int soff[GSL];
+   bool sint[GSL];
 #pragma unroll
-   for (int c = 0; c < 3; c++) {
-      const double* g = P.gy + pb.npts * c + q0;
-      const double* d = P.dd + pb.npts * c + q0;
-      double hx[5], hy[5], hz[5];
-#pragma unroll
-      for (int o = 0; o < 5; o++) {
-         hx[o] = sx[o - 2] * g[(o - 2) * sxs];
-         hy[o] = sy[o - 2] * g[(o - 2) * sys];
-         hz[o] = sz[o - 2] * g[(o - 2) * szs];
-      }
-      const double ax[3] = {hx[2] - 2 * hx[1] + hx[0], hx[3] - 2 * hx[2] + hx[1], hx[4] - 2 * hx[3] + hx[2]};
-      const double ay[3] = {hy[2] - 2 * hy[1] + hy[0], hy[3] - 2 * hy[2] + hy[1], hy[4] - 2 * hy[3] + hy[2]};
-      const double az[3] = {hz[2] - 2 * hz[1] + hz[0], hz[3] - 2 * hz[2] + hz[1], hz[4] - 2 * hz[3] + hz[2]};
-      const double tx = wx * (rx[2] * ax[2] - 2 * rx[1] * ax[1] + rx[0] * ax[0]);
-      const double ty = wy * (ry[2] * ay[2] - 2 * ry[1] * ay[1] + ry[0] * ay[0]);
-      const double tz = wz * (rz[2] * az[2] - 2 * rz[1] * az[1] + rz[0] * az[0]);
-      dbar[sidx(b, i, j, k) + b.npts * c] = -(tx + ty + tz);
-      const double dxx = d[sxs] - 2 * d[0] + d[-sxs];
-      const double dyy = d[sys] - 2 * d[0] + d[-sys];
-      const double dzz = d[szs] - 2 * d[0] + d[-szs];
-      gr -= sg.dcx[ii] * dxx * wx * ax[1] + sg.dcy[jj] * dyy * wy * ay[1] + sg.dcz[kk] * dzz * wz * az[1];
+   for (int q = 0; q < GSL; q++) {
+      const int e = tid + GNT * q, yy = e / GHX, xx = e - yy * GHX;
+      const int gi = ia - 2 + xx, gj = ja - 2 + yy;
+      const bool in = e < GPL && gi >= b.ib && gi <= b.ie && gj >= b.jb && gj <= b.je;
+      soff[q] = in ? (gj - b.jb) * b.ni + (gi - b.ib) : -1;
+      sint[q] = in && gi >= 1 && gi <= D.nx && gj >= 1 && gj <= D.ny;
    }
-   grho[sidx(b, i, j, k)] += gr;
+   // reads go to a point of the box and are masked afterwards (no branches)
+   double pg[GSL][3], pd[GSL][3], pr[GSL];
+   auto fetch = [&](int kk) {  // plane kk (inside the box)
+      const long pk = (long)(kk - b.kb) * b.nij;
+      const bool kint = kk >= 1 && kk <= D.nz;
+#pragma unroll
+      for (int q = 0; q < GSL; q++) {
+         const long a = pk + max(soff[q], 0);
+         const bool in = soff[q] >= 0, gin = kint && sint[q];
+         const double r = rho[a], br = beta / r;
+         pr[q] = in ? r : 0.0;
+#pragma unroll
+         for (int c = 0; c < 3; c++) {
+            const double yv = y[a + n * c], uv = u[a + n * c], mv = um[a + n * c];
+            pg[q][c] = gin ? br * yv : 0.0;
+            pd[q][c] = in ? uv - mv : 0.0;
+         }
+      }
+   };
+   auto store = [&](int buf) {
+#pragma unroll
+      for (int q = 0; q < GSL; q++) {
+         const int e = tid + GNT * q;
+         if (e < GPL) {
+            srl[buf][e] = pr[q];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+               sgl[buf][c * GPL + e] = pg[q][c];
+               sdl[buf][c * GPL + e] = pd[q][c];
+            }
+         }
+      }
+   };
+   // own column (inactive threads read column (ib, jb) and keep zeros)
+   const int ic = active ? i : b.ib, jc = active ? j : b.jb;
+   const long col = (long)(jc - b.jb) * b.ni + (ic - b.ib);
+   const bool cint = active && i >= 1 && i <= D.nx && j >= 1 && j <= D.ny;
+   double gc[5][3], dc[3][3], rc[3], szr[5], dczr[3];
+   auto gcolumn = [&](int q, int kk) {
+      const long a = (long)(min(max(kk, b.kb), b.ke) - b.kb) * b.nij + col;
+      const bool gin = cint && kk >= 1 && kk <= D.nz;
+      const double br = beta / rho[a];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+         const double yv = y[a + n * c];
+         gc[q][c] = gin ? br * yv : 0.0;
+      }
+      szr[q] = kk >= b.kb && kk <= b.ke ? sg.strz[kk - b.kb] : 1.0;
+   };
+   auto dcolumn = [&](int q, int kk) {
+      const long a = (long)(min(max(kk, b.kb), b.ke) - b.kb) * b.nij + col;
+      const bool in = active && kk >= b.kb && kk <= b.ke;
+      const double r = rho[a];
+      rc[q] = in ? r : 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+         const double uv = u[a + n * c], mv = um[a + n * c];
+         dc[q][c] = in ? uv - mv : 0.0;
+      }
+      dczr[q] = sg.dcz[min(max(kk, b.kb), b.ke) - b.kb];
+   };
+#pragma unroll
+   for (int q = 0; q < 4; q++) gcolumn(q, k0 - 2 + q);
+#pragma unroll
+   for (int q = 0; q < 2; q++) dcolumn(q, k0 - 1 + q);
+   const int ii = ic - b.ib, jj = jc - b.jb, ni = b.ie - b.ib, nj = b.je - b.jb;
+   double sxo[5], syo[5];
+#pragma unroll
+   for (int e = 0; e < 5; e++) {
+      const int iq = ic + e - 2, jq = jc + e - 2;
+      sxo[e] = iq >= b.ib && iq <= b.ie ? sg.strx[iq - b.ib] : 1.0;
+      syo[e] = jq >= b.jb && jq <= b.je ? sg.stry[jq - b.jb] : 1.0;
+   }
+   const double dcx[3] = {sg.dcx[max(ii - 1, 0)], sg.dcx[ii], sg.dcx[min(ii + 1, ni)]};
+   const double dcy[3] = {sg.dcy[max(jj - 1, 0)], sg.dcy[jj], sg.dcy[min(jj + 1, nj)]};
+   const double cxi = sg.cox[ii], cyj = sg.coy[jj];
+   fetch(k0);
+   store(0);
+   __syncthreads();
+   const int ctr = (ty + 2) * GHX + tx + 2;
+   for (int k = k0; k <= k1; k++) {
+      const int cur = (k - k0) & 1;
+      gcolumn(4, k + 2);
+      dcolumn(2, k + 1);
+      if (k < k1) fetch(k + 1);
+      if (active) {
+         const double* G = sgl[cur];
+         const double* Dd = sdl[cur];
+         const double* R = srl[cur];
+         const int kk = k - b.kb;
+         const double czk = sg.coz[kk];
+         const double rx[3] = {R[ctr - 1] * dcx[0], R[ctr] * dcx[1], R[ctr + 1] * dcx[2]};
+         const double ry[3] = {R[ctr - GHX] * dcy[0], R[ctr] * dcy[1], R[ctr + GHX] * dcy[2]};
+         const double rz[3] = {rc[0] * dczr[0], rc[1] * dczr[1], rc[2] * dczr[2]};
+         const double wx = cyj * czk, wy = cxi * czk, wz = cxi * cyj;
+         double gr = 0;
+         const long p = (long)kk * b.nij + col;
+#pragma unroll
+         for (int c = 0; c < 3; c++) {
+            const double* g = G + c * GPL + ctr;
+            const double* d = Dd + c * GPL + ctr;
+            double hx[5], hy[5], hz[5];
+#pragma unroll
+            for (int o = 0; o < 5; o++) {
+               hx[o] = sxo[o] * g[o - 2];
+               hy[o] = syo[o] * g[(o - 2) * GHX];
+               hz[o] = szr[o] * gc[o][c];
+            }
+            const double ax[3] = {hx[2] - 2 * hx[1] + hx[0], hx[3] - 2 * hx[2] + hx[1], hx[4] - 2 * hx[3] + hx[2]};
+            const double ay[3] = {hy[2] - 2 * hy[1] + hy[0], hy[3] - 2 * hy[2] + hy[1], hy[4] - 2 * hy[3] + hy[2]};
+            const double az[3] = {hz[2] - 2 * hz[1] + hz[0], hz[3] - 2 * hz[2] + hz[1], hz[4] - 2 * hz[3] + hz[2]};
+            const double tx_ = wx * (rx[2] * ax[2] - 2 * rx[1] * ax[1] + rx[0] * ax[0]);
+            const double ty_ = wy * (ry[2] * ay[2] - 2 * ry[1] * ay[1] + ry[0] * ay[0]);
+            const double tz_ = wz * (rz[2] * az[2] - 2 * rz[1] * az[1] + rz[0] * az[0]);
+            dbar[p + n * c] = -(tx_ + ty_ + tz_);
+            const double dxx = d[1] - 2 * d[0] + d[-1];
+            const double dyy = d[GHX] - 2 * d[0] + d[-GHX];
+            const double dzz = dc[2][c] - 2 * dc[1][c] + dc[0][c];
+            gr -= dcx[1] * dxx * wx * ax[1] + dcy[1] * dyy * wy * ay[1] + dczr[1] * dzz * wz * az[1];
+         }
+         grho[p] += gr;
+      }
+      if (k < k1) store(cur ^ 1);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+#pragma unroll
+         for (int c = 0; c < 3; c++) gc[q][c] = gc[q + 1][c];
+         szr[q] = szr[q + 1];
+      }
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+#pragma unroll
+         for (int c = 0; c < 3; c++) dc[q][c] = dc[q + 1][c];
+         rc[q] = rc[q + 1];
+         dczr[q] = dczr[q + 1];
+      }
+   }
 }
 
 __global__ void k_damp_invrho(DampCtx D, const double* __restrict__ y, const double* __restrict__ u,
@@ -1071,7 +1063,7 @@ int launch_adj_pad_material(const Box& b, const double* mu, const double* la, co
 {
    k_adj_pad_material<<<cdiv(P.pb.npts, 256), 256, 0, s>>>(b, P.pb, mu, la, strx, stry, strz, P.mu, P.la, P.sx, P.sy,
                                                            P.sz);
-   k_adj_sfield<<<cdiv(b.npts, 256), 256, 0, s>>>(b, strx, stry, strz, P.sb);
+   k_adj_sfield<<<cdiv(P.pb.npts, 256), 256, 0, s>>>(P.pb, P.sx, P.sy, P.sz, P.s);
    return check_hip(hipGetLastError(), "adjoint pad launch");
 }
 
@@ -1083,18 +1075,21 @@ int launch_rhs4sg_T(const Box& b, const RowMap& m, const SbpTables* d_sbp, const
    const double cof = 1.0 / (h * h);
    const dim3 blk(64, 4, 1);
    const unsigned gi = cdiv(b.ni, 64), gj = cdiv(b.je - b.jb + 1, 4);
-   // interior rows: S^{-1} L(S zI) on every point, the interior points by
-   // the LDS predictor sweep (2W - 2W + cof L(W)/S), the ghost shell apart
+   // interior rows: S^{-1} L(S zI) on every point of the box, by the LDS
+   // predictor sweep over the padded box (2W - 2W + cof L(W)/S; W, mu and
+   // lambda vanish in the pad, the stretching is 1 there)
    k_adj_weight<false><<<cdiv(P.pb.npts, 256), 256, 0, s>>>(A, P, z);
    SgCoef sg{};
-   sg.strx = strx;
-   sg.stry = stry;
-   sg.strz = strz;
-   int rc = launch_sweep_lds(false, 6, b, b.kb + 2, b.ke - 2, 64, nullptr, P.wb, P.w2b, mu, la, P.sb, sg, out, h, 1.0,
-                             0.0, s);
+   sg.strx = P.sx;
+   sg.stry = P.sy;
+   sg.strz = P.sz;
+   const Box& pb = P.pb;
+   const int npl = pb.ke - pb.kb - 3;  // the sweep's 32 x 8 tiles over the box's columns
+   const long tiles = cdiv(pb.ni - 4, 32) * cdiv(pb.je - pb.jb - 3, 8);
+   int rc = launch_sweep_lds(false, 6, pb, pb.kb + 2, pb.ke - 2, march_kchunk(tiles, npl, 2), nullptr, P.w, P.w2,
+                             P.mu, P.la, P.s, sg, P.o, h, 1.0, 0.0, s);
    if (rc) return rc;
-   const long ns = shell_count(b);
-   k_rhs4sg_T_shell<<<cdiv(ns, 256), 256, 0, s>>>(A, P, out, cof);
+   k_adj_crop<<<cdiv(b.npts, 256), 256, 0, s>>>(b, pb, P.o, out);
    // SBP rows: in-plane part from (sx sy) zB, then the k-direction terms
    if (m.top || m.bot) {
       k_adj_weight<true><<<cdiv(P.pb.npts, 256), 256, 0, s>>>(A, P, z);
@@ -1115,33 +1110,26 @@ int launch_rhs4sg_matgrad(const Box& b, const RowMap& m, const SbpTables* d_sbp,
    const dim3 blk(64, 4, 1);
    const unsigned gi = cdiv(b.ni, 64), gj = cdiv(b.je - b.jb + 1, 4);
    {
-      const int ntx = (int)cdiv(b.ni, GTX), nty = (int)cdiv(b.je - b.jb + 1, GTY), kc = 32;
-      const int nkc = (int)cdiv(b.ke - b.kb + 1, kc);
+      const int ntx = (int)cdiv(b.ni, GTX), nty = (int)cdiv(b.je - b.jb + 1, GTY);
+      const int kc = march_kchunk((long)ntx * nty, b.ke - b.kb + 1, 2), nkc = (int)cdiv(b.ke - b.kb + 1, kc);
       k_rhs4sg_matgrad_march<<<ntx * nty * nkc, GNT, 0, s>>>(A, ntx, nty, kc, z, u, gmu, gla, cof);
    }
-   if (m.top) k_rhs4sg_matgrad_sbp<<<dim3(gi, gj, 9), blk, 0, s>>>(A, 1, 1, z, u, gmu, gla, cof);
-   if (m.bot) k_rhs4sg_matgrad_sbp<<<dim3(gi, gj, 9), blk, 0, s>>>(A, m.nz, -1, z, u, gmu, gla, cof);
+   if (m.top) k_rhs4sg_matgrad_sbp<<<dim3(gi, gj, 8), blk, 0, s>>>(A, 1, 1, z, u, gmu, gla, cof);
+   if (m.bot) k_rhs4sg_matgrad_sbp<<<dim3(gi, gj, 8), blk, 0, s>>>(A, m.nz, -1, z, u, gmu, gla, cof);
    return check_hip(hipGetLastError(), "rhs4sg material-derivative launch");
 }
 
-int launch_addsgd4_T(const Box& b, const double* y, const double* u, const double* um, const double* rho,
-                     const SgCoef& sg, double beta, double* dbar, double* grho, hipStream_t s, bool inv_rho)
+int launch_addsgd4_T_march(const Box& b, const double* y, const double* u, const double* um, const double* rho,
+                           const SgCoef& sg, double beta, double* dbar, double* grho, hipStream_t s, bool inv_rho)
 {
    const DampCtx D{b, b.ie - b.ib - 3, b.je - b.jb - 3, b.ke - b.kb - 3, sg, rho, beta};
-   k_addsgd4_T<<<cdiv(b.npts, 256), 256, 0, s>>>(D, y, u, um, dbar, grho, inv_rho ? 1 : 0);
-   return check_hip(hipGetLastError(), "addsgd4 transpose launch");
-}
-
-int launch_addsgd4_T_pad(const Box& b, const double* y, const double* u, const double* um, const double* rho,
-                         const SgCoef& sg, double beta, double* dbar, double* grho, const AdjPad& P, hipStream_t s,
-                         bool inv_rho)
-{
-   const DampCtx D{b, b.ie - b.ib - 3, b.je - b.jb - 3, b.ke - b.kb - 3, sg, rho, beta};
-   k_damp_prep<<<cdiv(P.pb.npts, 256), 256, 0, s>>>(D, P, y, u, um);
-   const dim3 blk(64, 4, 1);
-   k_addsgd4_T_pad<<<dim3(cdiv(b.ni, 64), cdiv(b.je - b.jb + 1, 4), b.ke - b.kb + 1), blk, 0, s>>>(D, P, dbar, grho);
-   if (inv_rho)
+   const int ntx = (int)cdiv(b.ni, GTX), nty = (int)cdiv(b.je - b.jb + 1, GTY);
+   const int kc = march_kchunk((long)ntx * nty, b.ke - b.kb + 1, 2), nkc = (int)cdiv(b.ke - b.kb + 1, kc);
+   k_addsgd4_T_march<<<ntx * nty * nkc, GNT, 0, s>>>(D, ntx, nty, kc, y, u, um, dbar, grho);
+   if (inv_rho) {
+      const dim3 blk(64, 4, 1);
       k_damp_invrho<<<dim3(cdiv(D.nx, 64), cdiv(D.ny, 4), D.nz), blk, 0, s>>>(D, y, u, um, grho);
+   }
    return check_hip(hipGetLastError(), "addsgd4 transpose launch");
 }
 

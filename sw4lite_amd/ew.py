@@ -355,6 +355,17 @@ class EW:
                                             _ptr(res[2]), on_dev))
         return tuple(res)
 
+    def gradient_plan(self, nsteps, ckpt=0):
+        """(checkpoint interval, device bytes) that gradient(nsteps, ckpt)
+        would use; ckpt <= 0 chooses the interval from free device memory."""
+        used, nbytes = C.c_int(), C.c_double()
+        L.check(self.lib.sw4hip_ew_gradient_plan(self.h, int(nsteps), int(ckpt), C.byref(used), C.byref(nbytes)))
+        return used.value, nbytes.value
+
+    def release_workspace(self):
+        """Free the device buffers that gradient() keeps between calls."""
+        L.check(self.lib.sw4hip_ew_release_workspace(self.h))
+
     def point_source_error(self):
         li, l2, xi = C.c_double(), C.c_double(), C.c_double()
         L.check(self.lib.sw4hip_ew_point_source_error(self.h, C.byref(li), C.byref(l2), C.byref(xi)))

@@ -54,6 +54,8 @@ _SIGS = {
     "sw4hip_batch_tune": (I, [P, I, P, IP, P]),
     "sw4hip_batch_step": (I, [P, I, I]),
     "sw4hip_ew_gradient": (I, [P, I, I, P, P, P, P, I]),
+    "sw4hip_ew_gradient_plan": (I, [P, I, I, P, P]),
+    "sw4hip_ew_release_workspace": (I, [P]),
     "sw4hip_host_moment_source_curv": (I, [C.POINTER(GridDesc), P, D, D, D, P, I, IP, P, P]),
     "sw4hip_ew_set_topography": (I, [P, P, I]),
     "sw4hip_metric": (I, [I] * 7 + [P] * 5 + [P]),

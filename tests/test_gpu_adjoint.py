@@ -59,11 +59,14 @@ def dot_rel(a, b):
     return abs(a - b) / max(abs(a), abs(b), 1e-300)
 
 
-@pytest.mark.parametrize("os45", [(0, 0), (1, 0), (0, 1), (1, 1)])
-def test_rhs4sg_transpose(torch, coefs, os45):
+# the tall shapes span several k-blocks of the LDS k-march kernels (32 or 64
+# planes per workgroup) with a partial last block
+@pytest.mark.parametrize("os45,shape", [((0, 0), (21, 18, 17)), ((1, 0), (21, 18, 17)), ((0, 1), (21, 18, 17)),
+                                        ((1, 1), (21, 18, 17)), ((1, 0), (37, 11, 75))])
+def test_rhs4sg_transpose(torch, coefs, os45, shape):
     from sw4lite_amd import ew
     acof, ghcof, bope, _ = coefs
-    nx, ny, nz = 21, 18, 17
+    nx, ny, nz = shape
     box, f = random_inputs(300, nx, ny, nz)
     rng = np.random.default_rng(301)
     z = rng.uniform(-1, 1, f["u"].size)
@@ -73,7 +76,7 @@ def test_rhs4sg_transpose(torch, coefs, os45):
     lu = dev(torch, np.zeros_like(f["u"]))
     ew.rhs4sg(1, box, nz, os_, acof, bope, ghcof, lu, dev(torch, f["u"]), d["mu"], d["la"], 0.3, d["strx"],
               d["stry"], d["strz"])
-    lt = dev(torch, np.zeros_like(f["u"]))
+    lt = dev(torch, np.full_like(f["u"], 7.0))  # overwritten on every point
     ew.rhs4sg_adjoint(box, nz, os_, acof, bope, ghcof, lt, dev(torch, z), d["mu"], d["la"], 0.3, d["strx"], d["stry"],
                       d["strz"])
     a = float(np.dot(zr, host(lu)))
@@ -84,11 +87,12 @@ def test_rhs4sg_transpose(torch, coefs, os45):
     assert np.all(np.isfinite(host(lt)))
 
 
-@pytest.mark.parametrize("os45", [(0, 0), (1, 0), (1, 1)])
-def test_rhs4sg_material_derivative(torch, coefs, os45):
+@pytest.mark.parametrize("os45,shape", [((0, 0), (19, 17, 16)), ((1, 0), (19, 17, 16)), ((1, 1), (19, 17, 16)),
+                                        ((1, 1), (35, 9, 69))])
+def test_rhs4sg_material_derivative(torch, coefs, os45, shape):
     from sw4lite_amd import ew
     acof, ghcof, bope, _ = coefs
-    nx, ny, nz = 19, 17, 16
+    nx, ny, nz = shape
     box, f = random_inputs(310, nx, ny, nz)
     rng = np.random.default_rng(311)
     z = rng.uniform(-1, 1, f["u"].size)
@@ -115,9 +119,10 @@ def test_rhs4sg_material_derivative(torch, coefs, os45):
     assert dot_rel(a_m, b_m) < DOT_TOL and dot_rel(a_l, b_l) < DOT_TOL
 
 
-def test_addsgd4_transpose(torch):
+@pytest.mark.parametrize("shape", [(23, 17, 19), (37, 11, 70)])
+def test_addsgd4_transpose(torch, shape):
     from sw4lite_amd import ew
-    nx, ny, nz = 23, 17, 19
+    nx, ny, nz = shape
     box, f = random_inputs(320, nx, ny, nz)
     rng = np.random.default_rng(321)
     y = rng.uniform(-1, 1, f["u"].size)
@@ -130,7 +135,7 @@ def test_addsgd4_transpose(torch):
         ew.addsgd4(1, box, up, u, um, dev(torch, rho), *done, 0.02)
         return -host(up)  # D(u - um)
 
-    dbar = dev(torch, np.zeros_like(f["u"]))
+    dbar = dev(torch, np.full_like(f["u"], 7.0))  # overwritten on every point
     grho = dev(torch, np.zeros_like(f["rho"]))
     ew.addsgd4_adjoint(box, dev(torch, y), u, um, dev(torch, f["rho"]), *done, 0.02, dbar, grho)
     d = f["u"] - f["um"]
@@ -244,11 +249,20 @@ def test_gradient_independent_of_checkpoints(torch):
     e, mat = make_ew(nx, ny, nz, h, [2, 2, 2, 2, 0, 2], nsteps)
     rng = np.random.default_rng(43)
     gtr = rng.standard_normal((nsteps + 1, len(RECS), 3))
-    res = [e.gradient(gtr, nsteps, ckpt=c) for c in (1, 5, 17)]
+    # 0: chosen from free device memory, the whole history for this grid
+    assert e.gradient_plan(nsteps, 0)[0] == nsteps
+    assert e.gradient_plan(nsteps, 5)[0] == 5 and e.gradient_plan(nsteps, 99)[0] == nsteps
+    assert e.gradient_plan(nsteps, 5)[1] < e.gradient_plan(nsteps, 0)[1]
+    res = [e.gradient(gtr, nsteps, ckpt=c) for c in (1, 5, 17, 0)]
+    again = e.gradient(gtr, nsteps, ckpt=0)  # reuses the kept workspace
+    e.release_workspace()
+    after = e.gradient(gtr, nsteps, ckpt=0)  # allocates it again
     e.close()
     for r in res[1:]:
         for a, b in zip(r, res[0]):
             assert relerr(a, b) < 1e-12
+    for a, b, c in zip(again, after, res[3]):
+        assert np.array_equal(a, c) and np.array_equal(b, c)
 
 
 def test_autograd_function(torch):
