@@ -2,8 +2,9 @@
 //
 // The fused FP64 step of every shot, in the stage order of EW::step_fused:
 // the three interior sweeps (predictor, corrector, damping) of all shots are
-// one launch each; the SBP rows, ghost shell, sources and ghost-point
-// boundary conditions run per grid on the same stream.  Sources and
+// one launch each on the batch's stream; the SBP rows, ghost shell, sources,
+// ghost-point boundary conditions and receivers run per grid, shot q on side
+// stream q mod NSIDE between a fork and a join of the batch's stream.  Sources and
 // receivers read the time and the sample index from per-shot device clocks,
 // so a captured three-step period (U, Um, Up' rotate with period 3) replays
 // without changes.
@@ -17,7 +18,28 @@ Batch::~Batch()
 {
    if (m_exec) (void)hipGraphExecDestroy(m_exec);
    for (void* p : m_allocs) (void)hipFree(p);
+   for (int i = 0; i < NSIDE; i++) {
+      if (m_join[i]) (void)hipEventDestroy(m_join[i]);
+      if (m_side[i]) (void)hipStreamDestroy(m_side[i]);
+   }
+   if (m_fork) (void)hipEventDestroy(m_fork);
    if (m_s) (void)hipStreamDestroy(m_s);
+}
+
+int Batch::fork()
+{
+   int rc = check_hip(hipEventRecord(m_fork, m_s), "batch fork");
+   for (int i = 0; i < NSIDE && rc == OK; i++) rc = check_hip(hipStreamWaitEvent(m_side[i], m_fork, 0), "batch fork");
+   return rc;
+}
+
+int Batch::join()
+{
+   int rc = OK;
+   for (int i = 0; i < NSIDE && rc == OK; i++)
+      if ((rc = check_hip(hipEventRecord(m_join[i], m_side[i]), "batch join")) == OK)
+         rc = check_hip(hipStreamWaitEvent(m_s, m_join[i], 0), "batch join");
+   return rc;
 }
 
 int Batch::init()
@@ -34,7 +56,13 @@ int Batch::init()
       for (size_t b = a + 1; b < m_shots.size(); b++)
          if (m_shots[a] == m_shots[b]) return set_error(E_INVAL, "batch: a shot is listed twice");
    int rc;
-   if ((rc = check_hip(hipStreamCreate(&m_s), "batch stream"))) return rc;
+   if ((rc = check_hip(hipStreamCreate(&m_s), "batch stream")) ||
+       (rc = check_hip(hipEventCreateWithFlags(&m_fork, hipEventDisableTiming), "batch event")))
+      return rc;
+   for (int i = 0; i < NSIDE; i++)
+      if ((rc = check_hip(hipStreamCreate(&m_side[i]), "batch stream")) ||
+          (rc = check_hip(hipEventCreateWithFlags(&m_join[i], hipEventDisableTiming), "batch event")))
+         return rc;
    void* p = nullptr;
    const size_t tab = sizeof(SweepShot) * 9 * n, clk = (sizeof(double) * 2 + sizeof(int) * 2) * n;
    if (hipMalloc(&p, tab + clk) != hipSuccess) {
@@ -122,25 +150,29 @@ int Batch::one_step(int ph)
    const int n = (int)m_shots.size();
    const SweepShot* tab = m_dtab + (ph * 3) * n;
    int rc;
-   if ((rc = launch_sweep_batch(0, tab, n, m_nblk[ph][0], m_s))) return rc;
+   if ((rc = launch_sweep_batch(0, tab, n, m_nblk[ph][0], m_s)) || (rc = fork())) return rc;
    for (int q = 0; q < n; q++) {
       EW* e = m_shots[q];
+      e->m_stream = m_side[q % NSIDE];
       const GridSetup& g = e->m_opt.grid;
       const SgCoef c = e->sg();
       double* um = m_base[3 * q + ph % 3];
       double* u = m_base[3 * q + (ph + 1) % 3];
       const double dt = e->mDt;
+      hipStream_t st = e->m_stream;
       if ((rc = launch_pred_tail(e->m_box, g.nz, e->m_onesided, e->m_dsbp, u, um, e->mMu.dev, e->mLambda.dev,
-                                 e->mRho.dev, c, e->mUp.dev, g.h, dt, m_s)) ||
-          (rc = launch_src_apply(e->m_box, e->sources(), 0.0, 0, dt * dt, e->mRho.dev, e->mUp.dev, m_s, m_clk_t + q)) ||
+                                 e->mRho.dev, c, e->mUp.dev, g.h, dt, st)) ||
+          (rc = launch_src_apply(e->m_box, e->sources(), 0.0, 0, dt * dt, e->mRho.dev, e->mUp.dev, st, m_clk_t + q)) ||
           (rc = e->enforceBC(e->mUp, e->mMu, e->mLambda, 0.0)))
          return rc;
    }
-   if ((rc = launch_sweep_batch(1, tab + n, n, m_nblk[ph][1], m_s)) ||
-       (rc = launch_sweep_batch(2, tab + 2 * n, n, m_nblk[ph][2], m_s)))
+   if ((rc = join()) || (rc = launch_sweep_batch(1, tab + n, n, m_nblk[ph][1], m_s)) ||
+       (rc = launch_sweep_batch(2, tab + 2 * n, n, m_nblk[ph][2], m_s)) || (rc = fork()))
       return rc;
    for (int q = 0; q < n; q++) {
       EW* e = m_shots[q];
+      hipStream_t st = m_side[q % NSIDE];
+      e->m_stream = st;
       const GridSetup& g = e->m_opt.grid;
       const SgCoef c = e->sg();
       double* um = m_base[3 * q + ph % 3];
@@ -150,16 +182,17 @@ int Batch::one_step(int ph)
       Sarray v = e->mU;
       v.dev = up2;
       if ((rc = launch_corr_tail(e->m_box, g.nz, e->m_onesided, e->m_dsbp, e->mUp.dev, u, um, e->mMu.dev,
-                                 e->mLambda.dev, e->mRho.dev, c, e->m_opt.beta, up2, g.h, dt, m_s)) ||
-          (rc = launch_src_apply(e->m_box, e->sources(), 0.0, 1, dt * dt * dt * dt / 12, e->mRho.dev, up2, m_s,
+                                 e->mLambda.dev, e->mRho.dev, c, e->m_opt.beta, up2, g.h, dt, st)) ||
+          (rc = launch_src_apply(e->m_box, e->sources(), 0.0, 1, dt * dt * dt * dt / 12, e->mRho.dev, up2, st,
                                  m_clk_t + q)) ||
           (rc = e->enforceBC(v, e->mMu, e->mLambda, 0.0)))
          return rc;
       const int nrec = (int)e->m_rec.size() / 4;
       if (nrec > 0 && e->m_drecdata &&
-          (rc = launch_record(e->m_box, nrec, e->m_drec, up2, um, dt, e->m_drecdata, m_s, m_clk_s + q, e->m_maxsamples)))
+          (rc = launch_record(e->m_box, nrec, e->m_drec, up2, um, dt, e->m_drecdata, st, m_clk_s + q, e->m_maxsamples)))
          return rc;
    }
+   if ((rc = join())) return rc;
    return launch_clock_tick(n, m_clk_t, m_clk_s, m_clk_dt, m_clk_max, m_s);
 }
 

@@ -1,8 +1,8 @@
 // batch.h -- several shots (EW objects, grids of any size) advanced together:
 // the interior sweeps of all shots run as one launch each (the batched sweep
-// kernels of kernels_fast.hip), the per-grid boundary work follows on the
-// same stream, and the time loop can be captured once as a hipGraph and
-// replayed.
+// kernels of kernels_fast.hip), the per-grid boundary work follows on side
+// streams forked from the batch's stream, and the time loop can be captured
+// once as a hipGraph and replayed.
 #pragma once
 #include <vector>
 
@@ -49,6 +49,15 @@ private:
    hipGraphExec_t m_exec = nullptr;
    std::vector<void*> m_allocs;
    std::vector<hipStream_t> m_saved;
+   // the per-grid boundary work of the shots runs on NSIDE streams (default
+   // priority) forked from and joined back into m_s, so the small kernels of
+   // different grids overlap; the fork/join is captured into the graph too
+   static constexpr int NSIDE = 4;
+   hipStream_t m_side[NSIDE] = {nullptr, nullptr, nullptr, nullptr};
+   hipEvent_t m_fork = nullptr;
+   hipEvent_t m_join[NSIDE] = {nullptr, nullptr, nullptr, nullptr};
+   int fork();
+   int join();
 };
 
 }  // namespace sw4amd
