@@ -316,8 +316,10 @@ __device__ __forceinline__ bool shell_point(const Box& b, long q, int& i, int& j
    }
    q -= nj_planes;
    if (q < ni_planes) {
-      const long pl = q / (nyi * nzi), r = q % (nyi * nzi);
-      i = pl < 2 ? b.ib + (int)pl : b.ie - 3 + (int)pl;
+      // four consecutive threads take the four ghost columns of one row, so a
+      // wave touches 16 rows (two cache lines each) rather than 64
+      const long s = q % 4, r = q / 4;
+      i = s < 2 ? b.ib + (int)s : b.ie - 3 + (int)s;
       k = b.kb + 2 + (int)(r / nyi);
       j = b.jb + 2 + (int)(r % nyi);
       return true;
