@@ -581,7 +581,7 @@ def main_batch(a, torch):
     torch.cuda.empty_cache()
     pts = float(sum(n ** 3 for n in sizes))
     b = Batch(shots)
-    best, tune_ms = b.tune((16, 32, 64, 128))
+    best, tune_ms = b.tune()
 
     def timed(fn):
         torch.cuda.synchronize()

@@ -337,7 +337,7 @@ int sw4hip_batch_destroy(sw4hip_batch* batch);
 /* Planes per workgroup of the batched sweeps (default 64). */
 int sw4hip_batch_set_kchunk(sw4hip_batch* batch, int kchunk);
 /* Autotuner: time one batch step for each candidate k-block (cand NULL:
- * 16, 32, 64, 128), keep the fastest (*best, per-candidate ms in ms[] when
+ * 16, 24, 32, 40, 48, 64, 96), keep the fastest (*best, per-candidate ms in ms[] when
  * not NULL), then reset every shot to zero data at t = 0. */
 int sw4hip_batch_tune(sw4hip_batch* batch, int ncand, const int* cand, int* best, double* ms);
 /* Advance every shot by nsteps; use_graph: replay the captured period. */

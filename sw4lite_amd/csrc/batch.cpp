@@ -286,9 +286,9 @@ int Batch::step(int nsteps, bool graph)
 
 int Batch::tune(int* best, double* ms, int ncand, const int* cand)
 {
-   static const int defaults[4] = {16, 32, 64, 128};
+   static const int defaults[7] = {16, 24, 32, 40, 48, 64, 96};
    if (ncand <= 0 || !cand) {
-      ncand = 4;
+      ncand = 7;
       cand = defaults;
    }
    hipEvent_t e0, e1;

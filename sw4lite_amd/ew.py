@@ -398,7 +398,7 @@ class Batch:
     def set_kchunk(self, kc):
         L.check(self.lib.sw4hip_batch_set_kchunk(self.h, int(kc)))
 
-    def tune(self, candidates=(16, 32, 64, 128)):
+    def tune(self, candidates=(16, 24, 32, 40, 48, 64, 96)):
         """Autotune the k-block; returns (best, {candidate: ms per step}).
         Resets every shot to zero data at t = 0."""
         cand = np.asarray(candidates, dtype=np.int32)
