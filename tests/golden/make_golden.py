@@ -16,13 +16,17 @@ Run in a container that holds the reference source tree (default
     rows (curv.npz / curv.json);
   * copies the point-source accuracy fixture
     (pytest/reference/pointsource/pointsource-h0p04/PointSourceErr.txt) and
-    the parameters of its deck (pytest/reference/pointsource/pointsource.in).
+    the parameters of its deck (pytest/reference/pointsource/pointsource.in);
+  * copies the LOH.1 and LOH.2 station records at h = 100 m
+    (pytest/reference/loh{1,2}/LOH{1,2}-h100/sta10.txt) and writes LOH.2's
+    point-source list from its deck (pytest/reference/loh2/LOH.2-h100.in).
 
-Usage: python tests/golden/make_golden.py [/path/to/reference]
+Usage: python tests/golden/make_golden.py [/path/to/reference] [--loh-only]
 """
 import json
 import os
 import re
+import shutil
 import sys
 from fractions import Fraction
 
@@ -155,5 +159,30 @@ def main(ref):
     print("golden vectors written to", HERE)
 
 
+def loh_fixtures(ref):
+    """Station records of LOH.1/LOH.2 (USGS text, copied as they are) and the
+    3200 point sources of the LOH.2 deck as [x, y, z, Mxy, t0]."""
+    base = os.path.join(ref, "pytest", "reference")
+    for case in ("loh1", "loh2"):
+        shutil.copyfile(os.path.join(base, case, case.upper() + "-h100", "sta10.txt"),
+                        os.path.join(HERE, case + "_h100_sta10.txt"))
+    src = []
+    for line in open(os.path.join(base, "loh2", "LOH.2-h100.in")):
+        if not line.startswith("source "):
+            continue
+        kv = dict(t.split("=", 1) for t in line.split()[1:])
+        assert kv["type"] == "Gaussian" and float(kv["freq"]) == 16.6667 and float(kv["m0"]) == 1.0
+        assert set(kv) == {"x", "y", "z", "Mxy", "t0", "freq", "type", "m0"}
+        src.append([float(kv["x"]), float(kv["y"]), float(kv["z"]), float(kv["Mxy"]), float(kv["t0"])])
+    with open(os.path.join(HERE, "loh2_h100_sources.json"), "w") as fh:
+        json.dump({"deck": "pytest/reference/loh2/LOH.2-h100.in", "type": "Gaussian", "freq": 16.6667, "m0": 1.0,
+                   "columns": ["x", "y", "z", "Mxy", "t0"], "sources": src}, fh)
+    print("LOH fixtures written:", len(src), "LOH.2 sources")
+
+
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    ref_dir = args[0] if args else "/root/reference"
+    if "--loh-only" not in sys.argv:
+        main(ref_dir)
+    loh_fixtures(ref_dir)

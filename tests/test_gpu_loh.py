@@ -1,0 +1,93 @@
+"""GPU: the reference's LOH fixtures (SCEC layer over half-space): the decks
+pytest/reference/loh1/LOH.1-h100.in and pytest/reference/loh2/LOH.2-h100.in
+with their recorded station pytest/reference/loh{1,2}/LOH{1,2}-h100/sta10.txt
+(kept as tests/golden/loh{1,2}_h100_sta10.txt; LOH.2's source list as
+tests/golden/loh2_h100_sources.json, both written by make_golden.py).
+
+Both decks: 30 x 30 x 17 km at h = 100 m (301 x 301 x 171 points), t = 9 s,
+supergrid 30 points on the four sides and the bottom, free top; material
+blocks (MaterialBlock.C:116-190): half-space vp 6000, vs 3464, rho 2700 and
+the top layer z <= 1000 m vp 4000, vs 2000, rho 2600; LOH.1 adds the
+interface plane 999 <= z <= 1001 with vp 4630.76, vs 2437.56, rho 2650.  A
+block sets the interior columns at every k with a tolerance of 1e-5 m; the
+lateral ghost columns take the value of the nearest interior column
+(EW::extrapolateInXY, EW.C:5493-5532), and mu = rho vs^2, lambda =
+rho vp^2 - 2 mu (EW.C:5471-5490): the material is a function of z = (k-1) h.
+Sources, Gaussian time function with freq 16.6667: LOH.1 one moment tensor
+mxy = 1e18 at (15000, 15000, 2000), t0 = 0.36, on a grid point; LOH.2 a
+fault of 3200 point sources mxy = 3.24e14 with rupture delays, half-way
+between grid planes (each spread over a 6 x 6 x 6 stencil).  Station sta10
+at (21000, 23000, 0) records displacement (USGS format, 17 digits).
+
+The whole FP64 step (fused sweeps, and the reference stage order) must
+reproduce the 537 recorded samples of all three components to 1e-12 of the
+largest recorded value: the same algorithm in a different rounding order."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from common import ROOT
+from sw4lite_amd import lib as L
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def loh_material(nx, ny, nz, h, interface):
+    """rho, mu, lambda over the box (ghosts included), corder 1."""
+    k = np.arange(-1, nz + 3)
+    z = (k - 1) * h
+    tol = 1e-5
+    rho = np.full(k.size, 2700.0)
+    vs = np.full(k.size, 3464.0)
+    vp = np.full(k.size, 6000.0)
+    top = z <= 1000.0 + tol
+    rho[top], vs[top], vp[top] = 2600.0, 2000.0, 4000.0
+    if interface:
+        face = (z >= 999.0 - tol) & (z <= 1001.0 + tol)
+        rho[face], vs[face], vp[face] = 2650.0, 2437.56, 4630.76
+    mu = rho * vs * vs
+    la = rho * vp * vp - 2 * mu
+    shape = (nz + 4, ny + 4, nx + 4)
+    return [np.ascontiguousarray(np.broadcast_to(a[:, None, None], shape)).ravel() for a in (rho, mu, la)]
+
+
+def sources(case):
+    if case == "loh1":
+        return [(15000.0, 15000.0, 2000.0, 1e18, 0.36)]
+    d = json.load(open(os.path.join(GOLDEN, "loh2_h100_sources.json")))
+    assert d["columns"] == ["x", "y", "z", "Mxy", "t0"] and d["m0"] == 1.0 and d["type"] == "Gaussian"
+    assert len(d["sources"]) == 3200
+    return [tuple(s) for s in d["sources"]]
+
+
+@pytest.mark.parametrize("case,mode", [("loh1", 0), ("loh1", 1), ("loh2", 0), ("loh2", 1)])
+def test_loh_station_matches_fixture(case, mode):
+    import torch
+    from sw4lite_amd import EW
+    assert torch.cuda.is_available(), "GPU tests need a visible GPU"
+    want_all = np.loadtxt(os.path.join(GOLDEN, case + "_h100_sta10.txt"), comments="#")
+    assert want_all.shape == (537, 4)
+    nx, ny, nz, h = 301, 301, 171, 100.0
+    e = EW(nx, ny, nz, h, [L.BC_SUPERGRID] * 4 + [L.BC_FREE, L.BC_SUPERGRID], corder=1, sg_gp_thickness=30)
+    rho, mu, la = loh_material(nx, ny, nz, h, interface=(case == "loh1"))
+    e.set_material(rho, mu, la)
+    for x, y, z, mxy, t0 in sources(case):
+        e.add_moment_source(x, y, z, [0.0, mxy, 0.0, 0.0, 0.0, 0.0], t0, 16.6667, L.TF_GAUSSIAN)
+    e.add_receiver(21000.0, 23000.0, 0.0, 0)
+    e.setup(tmax=9.0)
+    inf = e.info()
+    assert inf.nsteps == 536
+    assert abs(inf.dt - want_all[1, 0]) < 1e-7  # the fixture prints the time to 7 digits
+    e.timesteploop(inf.nsteps, mode=mode)
+    got = e.receivers()[:, 0, :]
+    e.close()
+    want = want_all[:, 1:]
+    scale = np.max(np.abs(want))
+    err = np.max(np.abs(got - want)) / scale
+    comp = [float(np.max(np.abs(got[:, c] - want[:, c])) / scale) for c in range(3)]
+    print(case, "sta10 mode", mode, "max |u| %.6g" % scale, "relerr", err, "per component", comp, flush=True)
+    assert err < TOL
