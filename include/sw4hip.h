@@ -197,6 +197,15 @@ int sw4hip_freesurfcurvisg(int corder, int ib, int ie, int jb, int je, int kb, i
                            int side, double* u, const double* mu, const double* lambda,
                            const double* met, const double* sbop, const double* forcing,
                            const double* strx, const double* stry, void* stream);
+/* Replaces EW::enforceCartTopo (EW.C:3503-3531): the interface of a
+ * curvilinear grid stacked on the top Cartesian grid, both over the same
+ * (i, j) box [ib, ie] x [jb, je] (ghosts included).  The Cartesian ghost
+ * planes kb_cart .. kb_cart+g-1 take the curvilinear planes
+ * ke_curv-2g .. ke_curv-g-1; then the curvilinear planes ke_curv-g .. ke_curv
+ * (the common plane and the ghosts below it) take the Cartesian planes
+ * kb_cart+2g .. kb_cart+g, g = ghost_points (2 in the reference). */
+int sw4hip_enforce_cart_topo(int corder, int ib, int ie, int jb, int je, int kb_cart, int ke_cart, double* ucart,
+                             int kb_curv, int ke_curv, double* ucurv, int ghost_points, void* stream);
 /* ----------------------------------------------------------------------- */
 /* EW-level object                                                           */
 /* ----------------------------------------------------------------------- */

@@ -507,6 +507,19 @@ int sw4hip_freesurfcurvisg(int corder, int ib, int ie, int jb, int je, int kb, i
                                strx, stry, (hipStream_t)stream);
 }
 
+int sw4hip_enforce_cart_topo(int corder, int ib, int ie, int jb, int je, int kb_cart, int ke_cart, double* ucart,
+                             int kb_curv, int ke_curv, double* ucurv, int ghost_points, void* stream)
+{
+   if (corder != 0 && corder != 1) return set_error(E_INVAL, "corder must be 0 or 1");
+   if (!ucart || !ucurv) return set_error(E_INVAL, "enforce_cart_topo: null argument");
+   if (ib > ie || jb > je) return set_error(E_INVAL, "enforce_cart_topo: empty (i, j) range");
+   if (ghost_points < 1) return set_error(E_INVAL, "enforce_cart_topo: ghost_points must be at least 1");
+   if (kb_cart + 2 * ghost_points > ke_cart || ke_curv - 2 * ghost_points < kb_curv)
+      return set_error(E_INVAL, "enforce_cart_topo: a grid is too thin for its ghost planes");
+   return launch_enforce_cart_topo(corder, make_box(ib, ie, jb, je, kb_cart, ke_cart), ucart,
+                                   make_box(ib, ie, jb, je, kb_curv, ke_curv), ucurv, ghost_points, (hipStream_t)stream);
+}
+
 // ---------------------------------------------------------------------------
 // EW level
 // ---------------------------------------------------------------------------

@@ -19,6 +19,15 @@ int launch_addsgd4c(int co, const Box& b, double* up, const double* u, const dou
 // Ghost points of side 4 (k = 0 from k = 1..4) or 5 (k = nz+1 from nz..nz-3)
 // for zero traction minus forcing (3 interleaved values per (i,j) of the box
 // plane, or NULL for none).
+// Interface of a curvilinear grid stacked on a Cartesian one
+// (EW::enforceCartTopo, EW.C:3503-3531), over the common (i, j) range
+// [ib, ie] x [jb, je]: the Cartesian planes kbc .. kbc+g-1 (its ghosts above
+// the interface) take the curvilinear planes kec-2g .. kec-g-1, then the
+// curvilinear planes kec-g .. kec (the common plane and the ghosts below it)
+// take the Cartesian planes kbc+2g .. kbc+g, with g ghost points.  cart and
+// curv are the two grids' boxes (same i, j bounds).
+int launch_enforce_cart_topo(int co, const Box& cart, double* ucart, const Box& curv, double* ucurv, int g,
+                             hipStream_t s);
 int launch_freesurf_curv(int co, const Box& b, int nz, int side, double* u, const double* mu, const double* la,
                          const double* met, const SbpTables* d_sbp, const double* forcing, const double* strx,
                          const double* stry, hipStream_t s);

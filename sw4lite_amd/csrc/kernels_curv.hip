@@ -545,6 +545,30 @@ int launch_freesurf_curv(int co, const Box& b, int nz, int side, double* u, cons
    return check_hip(hipGetLastError(), "curvilinear free-surface launch");
 }
 
+template <int CO>
+__global__ void k_enforce_cart_topo(Box cart, double* __restrict__ ucart, Box curv, double* __restrict__ ucurv, int g)
+{
+   const int i = cart.ib + blockIdx.x * blockDim.x + threadIdx.x;
+   const int j = cart.jb + blockIdx.y * blockDim.y + threadIdx.y;
+   if (i > cart.ie || j > cart.je) return;
+   const int kbc = cart.kb, kec = curv.ke;
+   for (int c = 0; c < 3; c++) {
+      for (int q = 0; q < g; q++) ucart[vidx<CO>(cart, c, i, j, kbc + q)] = ucurv[vidx<CO>(curv, c, i, j, kec - 2 * g + q)];
+      for (int q = 0; q <= g; q++) ucurv[vidx<CO>(curv, c, i, j, kec - q)] = ucart[vidx<CO>(cart, c, i, j, kbc + 2 * g - q)];
+   }
+}
+
+int launch_enforce_cart_topo(int co, const Box& cart, double* ucart, const Box& curv, double* ucurv, int g,
+                             hipStream_t s)
+{
+   const dim3 blk(64, 4, 1), grd(cdiv(cart.ni, 64), cdiv(cart.je - cart.jb + 1, 4), 1);
+   if (co)
+      k_enforce_cart_topo<1><<<grd, blk, 0, s>>>(cart, ucart, curv, ucurv, g);
+   else
+      k_enforce_cart_topo<0><<<grd, blk, 0, s>>>(cart, ucart, curv, ucurv, g);
+   return check_hip(hipGetLastError(), "grid interface launch");
+}
+
 int launch_maxeig_curv(int co, const Box& b, const double* met, const double* jac, const double* mu, const double* la,
                        const double* rho, double* partial, int npartial, double* out, hipStream_t s)
 {

@@ -154,6 +154,15 @@ def freesurfcurvisg(corder, box, nz, side, u, mu, la, met, sbop, forcing, strx, 
                                             _ptr(forcing), _ptr(strx), _ptr(stry), _stream(stream)))
 
 
+def enforce_cart_topo(corder, ij, cart_k, ucart, curv_k, ucurv, ghost_points=2, stream=None):
+    """Grid interface of a curvilinear grid stacked on a Cartesian one
+    (EW::enforceCartTopo): ij = (ib, ie, jb, je) common to both boxes,
+    cart_k = (kb, ke) of the Cartesian box, curv_k = (kb, ke) of the
+    curvilinear box."""
+    L.check(L.load().sw4hip_enforce_cart_topo(corder, *ij, *cart_k, _ptr(ucart), *curv_k, _ptr(ucurv), ghost_points,
+                                              _stream(stream)))
+
+
 def bcfortsg(corder, box, wind, nx, ny, nz, u, h, bccnd, sbop, mu, la, bforce, strx, stry, stream=None):
     w = np.ascontiguousarray(wind, dtype=np.int32)
     bc = np.ascontiguousarray(bccnd, dtype=np.int32)

@@ -62,6 +62,7 @@ _SIGS = {
     "sw4hip_rhs4sgcurv": (I, [I] * 7 + [P] * 12 + [P]),
     "sw4hip_addsgd4c": (I, [I] * 7 + [P] * 11 + [D, P]),
     "sw4hip_freesurfcurvisg": (I, [I] * 9 + [P] * 8 + [P]),
+    "sw4hip_enforce_cart_topo": (I, [I] * 7 + [P, I, I, P, I, P]),
     "sw4hip_ew_create": (I, [C.POINTER(GridDesc), C.POINTER(P)]),
     "sw4hip_ew_destroy": (I, [P]),
     "sw4hip_ew_set_stream": (I, [P, P]),

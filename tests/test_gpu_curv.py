@@ -176,3 +176,23 @@ def test_curvilinear_time_loop(torch, corder, bc):
     assert np.max(np.abs(ref.U)) > 0
     assert err < TOL and relerr(um, ref.Um) < TOL
     assert rec.shape == (9, 1, 3) and np.max(np.abs(rec[-1])) > 0
+
+
+@pytest.mark.parametrize("corder", [0, 1])
+def test_enforce_cart_topo_matches_oracle(corder):
+    """sw4hip_enforce_cart_topo against the numpy restatement of
+    EW::enforceCartTopo on random fields: a copy, so bit for bit."""
+    import torch
+    from oracle.coupling import enforce_cart_topo
+    from sw4lite_amd import ew
+    ni, nj, nzc, nzk = 37, 21, 15, 12
+    ij = (-1, ni - 2, -1, nj - 2)
+    rng = np.random.default_rng(17 + corder)
+    uc = rng.standard_normal(3 * ni * nj * (nzc + 4))
+    uk = rng.standard_normal(3 * ni * nj * (nzk + 4))
+    want_c, want_k = uc.copy(), uk.copy()
+    enforce_cart_topo(corder, ij, (-1, nzk + 2), want_k, (-1, nzc + 2), want_c, 2)
+    dc, dk = torch.from_numpy(uc).cuda(), torch.from_numpy(uk).cuda()
+    ew.enforce_cart_topo(corder, ij, (-1, nzk + 2), dk, (-1, nzc + 2), dc, 2)
+    torch.cuda.synchronize()
+    assert np.array_equal(dc.cpu().numpy(), want_c) and np.array_equal(dk.cpu().numpy(), want_k)

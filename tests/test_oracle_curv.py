@@ -245,3 +245,38 @@ def test_freesurf_curved_leaves_given_traction(coefs, side):
     res = [m0 * ac * du[c] + (m0 + l0) * v[c] * vdu + g[c] for c in range(3)]
     scale = max(np.max(np.abs(gc)) for gc in g)
     assert max(np.max(np.abs(r)) for r in res) < 1e-12 * scale
+
+
+@pytest.mark.parametrize("corder", [0, 1])
+def test_cart_topo_interface_places_physical_depths(corder):
+    """enforce_cart_topo (EW.C:3503-3531) on a field that is one linear
+    function of physical depth on both grids: the curvilinear grid's last
+    interior plane k = nzc and the Cartesian plane k = 1 are the same depth,
+    and the spacing is h across the interface.  After the coupling every
+    ghost plane of both grids holds the field at its own depth, whatever it
+    held before: the plane offsets connect points at the same depth."""
+    from oracle.coupling import enforce_cart_topo
+    ni, nj, nzc, nzk, h, zint = 7, 6, 9, 8, 0.25, 3.0
+    ib, ie, jb, je = -1, ni - 2, -1, nj - 2
+    kc = np.arange(-1, nzc + 3)           # curvilinear planes, nzc at zint
+    kk = np.arange(-1, nzk + 3)           # Cartesian planes, 1 at zint
+    zc = zint + (kc - nzc) * h
+    zk = zint + (kk - 1) * h
+    rng = np.random.default_rng(5)
+    a, b = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
+
+    def field(z):
+        f = a[:, None] * z[None, :] + b[:, None]  # [c, k]
+        return np.broadcast_to(f[:, :, None, None], (3, z.size, nj, ni)).copy()
+
+    def pack(f):
+        return f.ravel().copy() if corder == 1 else f.transpose(1, 2, 3, 0).ravel().copy()
+
+    fc, fk = field(zc), field(zk)
+    gc, gk = fc.copy(), fk.copy()
+    gc[:, -3:] = rng.uniform(-9, 9, gc[:, -3:].shape)   # curvilinear ghosts and common plane: garbage
+    gk[:, :2] = rng.uniform(-9, 9, gk[:, :2].shape)     # Cartesian ghosts above the interface: garbage
+    uc, uk = pack(gc), pack(gk)
+    enforce_cart_topo(corder, (ib, ie, jb, je), (-1, nzk + 2), uk, (-1, nzc + 2), uc, 2)
+    assert np.allclose(uc, pack(fc), rtol=0, atol=1e-13)
+    assert np.allclose(uk, pack(fk), rtol=0, atol=1e-13)
