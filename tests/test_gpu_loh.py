@@ -21,7 +21,17 @@ at (21000, 23000, 0) records displacement (USGS format, 17 digits).
 
 The whole FP64 step (fused sweeps, and the reference stage order) must
 reproduce the 537 recorded samples of all three components to 1e-12 of the
-largest recorded value: the same algorithm in a different rounding order."""
+largest recorded value: the same algorithm in a different rounding order.
+The FP32 form with the compensated update (step mode 2) is checked against
+the same records by relative L2 per component.  Its error grows with the
+number of periods a run spans: each step's increment carries the float
+rounding of its stencils (~1e-7 of the increment), and these add up over
+the run, about 1e-7 |u| per radian of phase.  The 1024^3 precision workload
+(800 steps of 1.5 ms) stays below 1e-6; these runs cover 9 s with a
+16.7 Hz source, and measure 8.7e-6 / 1.05e-5 / 2.26e-5 (LOH.1, x / y / z)
+and 9.7e-6 / 1.12e-5 / 2.75e-5 (LOH.2).  The float rounding of the
+material (a velocity shift of up to 3e-8 in each layer, i.e. a phase error
+of 3e-8 per radian) contributes as much.  The bar here is 5e-5."""
 import json
 import os
 
@@ -33,6 +43,7 @@ from sw4lite_amd import lib as L
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
+FP32_L2 = 5e-5
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
@@ -64,7 +75,7 @@ def sources(case):
     return [tuple(s) for s in d["sources"]]
 
 
-@pytest.mark.parametrize("case,mode", [("loh1", 0), ("loh1", 1), ("loh2", 0), ("loh2", 1)])
+@pytest.mark.parametrize("case,mode", [("loh1", 0), ("loh1", 1), ("loh1", 2), ("loh2", 0), ("loh2", 1), ("loh2", 2)])
 def test_loh_station_matches_fixture(case, mode):
     import torch
     from sw4lite_amd import EW
@@ -90,4 +101,9 @@ def test_loh_station_matches_fixture(case, mode):
     err = np.max(np.abs(got - want)) / scale
     comp = [float(np.max(np.abs(got[:, c] - want[:, c])) / scale) for c in range(3)]
     print(case, "sta10 mode", mode, "max |u| %.6g" % scale, "relerr", err, "per component", comp, flush=True)
-    assert err < TOL
+    if mode == 2:
+        l2 = [float(np.linalg.norm(got[:, c] - want[:, c]) / np.linalg.norm(want[:, c])) for c in range(3)]
+        print(case, "FP32 form relative L2 per component", l2, flush=True)
+        assert max(l2) <= FP32_L2
+    else:
+        assert err < TOL
