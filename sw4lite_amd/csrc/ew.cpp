@@ -592,23 +592,33 @@ int EW::step_comp(int n, double* stage_ms)
    auto mark = [&](int s, int q) {
       return stage_ms ? check_hip(hipEventRecord(ev[(size_t)s * (kStages + 1) + q], m_stream), "hipEventRecord") : OK;
    };
+   // zero ghost values of Dirichlet/supergrid sides set in the shell's pass
+   // (the shell is exactly the union of those sides' two ghost layers; with a
+   // periodic side the copies have to see the sides in their order)
+   int zfill = 0;
+   bool periodic = false;
+   for (int q = 0; q < 6; q++) {
+      if (g.bc[q] == BC_DIRICHLET || g.bc[q] == BC_SUPERGRID) zfill |= 1 << q;
+      periodic = periodic || g.bc[q] == BC_PERIODIC;
+   }
+   if (periodic) zfill = 0;
    for (int s = 0; s < n && rc == OK; s++) {
       const double t = mT, dt = mDt;
       const CompState cs = comp_state();
       // the damping is part of the corrector sweep: stages 4 and 5 coincide
       if ((rc = mark(s, 0)) ||
           (rc = launch_comp_pred(m_box, g.nz, m_onesided, m_dsbpf, cs, cf, g.h, m_stream,
-                                 [&]() { return mark(s, 1); }, m_kchunk)) ||
+                                 [&]() { return mark(s, 1); }, m_kchunk, dt, zfill)) ||
           (rc = mark(s, 2)) || (rc = launch_comp_src(false, m_box, st, t, 1.0, cs, m_stream)) ||
           (rc = launch_comp_bc(false, m_box, m_wind, g.nx, g.ny, g.nz, g.bc, cs, g.h, dt, m_dsbp, mMu.dev,
-                               mLambda.dev, c.strx, c.stry, m_stream)) ||
+                               mLambda.dev, c.strx, c.stry, m_stream, zfill)) ||
           (rc = mark(s, 3)) ||
           (rc = launch_comp_corr(m_box, g.nz, m_onesided, m_dsbpf, cs, cf, m_opt.beta, g.h, dt, m_stream,
-                                 [&]() { const int r = mark(s, 4); return r ? r : mark(s, 5); }, m_kchunk)) ||
+                                 [&]() { const int r = mark(s, 4); return r ? r : mark(s, 5); }, m_kchunk, zfill)) ||
           (rc = mark(s, 6)) ||
           (rc = launch_comp_src(true, m_box, st, t, dt * dt * dt * dt / 12, cs, m_stream)) ||
           (rc = launch_comp_bc(true, m_box, m_wind, g.nx, g.ny, g.nz, g.bc, cs, g.h, dt, m_dsbp, mMu.dev,
-                               mLambda.dev, c.strx, c.stry, m_stream)) ||
+                               mLambda.dev, c.strx, c.stry, m_stream, zfill)) ||
           (rc = mark(s, 7)))
          break;
       if (nrec > 0 && m_drecdata && m_nsamples < m_maxsamples) {

@@ -45,13 +45,16 @@ int launch_to_float(long n, const double* src, float* dst, hipStream_t s);
 // Predictor: A on every point (interior sweep, SBP rows, zero on the ghost
 // shell), then the sources A += F/rho and the ghost-point boundary conditions
 // of Up.  after_interior runs after the interior sweep is queued.
+// zfill: sides (bit s) whose zero ghost values (Dirichlet or supergrid, no
+// data) are set in the shell's pass; launch_comp_bc then skips them.  Valid
+// when no side is periodic.
 int launch_comp_pred(const Box& b, int nk, const int* onesided, const SbpTablesT<float>* d_sbpf, const CompState& st,
                      const SgCoefF& sgf, double h, hipStream_t s, const std::function<int()>& after_interior,
-                     int kchunk);
+                     int kchunk, double dt = 0.0, int zfill = 0);
 // Corrector: Dn and the compensated (U, C) += Dn on every point.
 int launch_comp_corr(const Box& b, int nk, const int* onesided, const SbpTablesT<float>* d_sbpf, const CompState& st,
                      const SgCoefF& sgf, double beta, double h, double dt, hipStream_t s,
-                     const std::function<int()>& after_interior, int kchunk);
+                     const std::function<int()>& after_interior, int kchunk, int zfill = 0);
 // Sources: predictor A += F(t)/rho; corrector Dn += coef/rho F_tt(t) and
 // (U, C) += the same increment.
 int launch_comp_src(bool corr, const Box& b, const SourceTable& st, double t, double coef, const CompState& cs,
@@ -61,7 +64,7 @@ int launch_comp_src(bool corr, const Box& b, const SourceTable& st, double t, do
 // on u^{n+1} = U + C (corrector); mu, la, strx, stry are the FP64 arrays.
 int launch_comp_bc(bool corr, const Box& b, const int* wind, int nx, int ny, int nz, const int* bccnd,
                    const CompState& cs, double h, double dt, const SbpTables* d_sbp, const double* mu,
-                   const double* la, const double* strx, const double* stry, hipStream_t s);
+                   const double* la, const double* strx, const double* stry, hipStream_t s, int skip = 0);
 // Receivers: displacement U + C of u^{n+1}, or velocity (Dn + D)/(2 dt).
 int launch_comp_record(const Box& b, int nrec, const int* rec, const CompState& cs, double dt, double* out,
                        hipStream_t s);

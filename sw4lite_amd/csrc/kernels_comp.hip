@@ -556,6 +556,46 @@ __global__ void k_cbc_fill(Box b, CompWindow w, V v, const double* __restrict__ 
    for (int c = 0; c < 3; c++) v.set(vidx<1>(b, c, i, j, k), bf ? bf[3 * q + c] : 0.0);
 }
 
+// Ghost shell and, in the same pass, the zero ghost values of the sides in
+// zfill (bit s: side s is Dirichlet or supergrid with zero data, its two
+// ghost layers): at such a point the shell's update is followed by the
+// view's set(0), leaving exactly what k_comp_shell and then k_cbc_fill
+// would.  Grid-point sources never lie in the shell, so the sources that
+// run in between do not see the difference.
+template <bool CORR>
+__global__ void k_comp_shell_fill(Box b, CompState st, float dt2, double dt2d, int zfill)
+{
+   const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   int i, j, k;
+   if (!shell_point(b, q, i, j, k)) return;
+   const bool zero = ((zfill & 1) && i <= b.ib + 1) || ((zfill & 2) && i >= b.ie - 1) ||
+                     ((zfill & 4) && j <= b.jb + 1) || ((zfill & 8) && j >= b.je - 1) ||
+                     ((zfill & 16) && k <= b.kb + 1) || ((zfill & 32) && k >= b.ke - 1);
+   const long p = sidx(b, i, j, k);
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      const long v = p + b.npts * c;
+      if (!CORR) {
+         st.A[v] = 0.0f;
+         if (zero) PredView{st.U, st.C, st.D, st.A, dt2d}.set(v, 0.0);
+      } else {
+         const float dn = st.D[v] + dt2 * st.A[v];
+         float U = st.U[v], C = st.C[v];
+         comp_add(U, C, dn);
+         if (zero) {
+            const double u = (double)U + (double)C;
+            st.Dn[v] = (float)((double)dn + (0.0 - u));
+            st.U[v] = 0.0f;
+            st.C[v] = 0.0f;
+         } else {
+            st.Dn[v] = dn;
+            st.U[v] = U;
+            st.C[v] = C;
+         }
+      }
+   }
+}
+
 template <class V>
 __global__ void k_cbc_periodic(Box b, CompWindow w, int pi, int pj, int pk, V v)
 {
@@ -685,7 +725,7 @@ int comp_rows(bool corr, const Box& b, int side, int nk, const SbpTablesT<float>
 
 int launch_comp_pred(const Box& b, int nk, const int* onesided, const SbpTablesT<float>* d_sbpf, const CompState& st,
                      const SgCoefF& sgf, double h, hipStream_t s, const std::function<int()>& after_interior,
-                     int kchunk)
+                     int kchunk, double dt, int zfill)
 {
    const float cof = (float)(1.0 / (h * h));
    const int k1 = onesided[4] == 1 ? 7 : b.kb + 2;
@@ -696,13 +736,16 @@ int launch_comp_pred(const Box& b, int nk, const int* onesided, const SbpTablesT
       if (onesided[side] == 1 && (rc = comp_rows(false, b, side, nk, d_sbpf, st, sgf, cof, 0.0f, 0.0f, 0.0f, s)))
          return rc;
    const long ns = shell_count(b);
-   k_comp_shell<false><<<cdiv(ns, 256), 256, 0, s>>>(b, st, 0.0f);
+   if (zfill)
+      k_comp_shell_fill<false><<<cdiv(ns, 256), 256, 0, s>>>(b, st, 0.0f, dt * dt, zfill);
+   else
+      k_comp_shell<false><<<cdiv(ns, 256), 256, 0, s>>>(b, st, 0.0f);
    return check_hip(hipGetLastError(), "FP32 predictor shell launch");
 }
 
 int launch_comp_corr(const Box& b, int nk, const int* onesided, const SbpTablesT<float>* d_sbpf, const CompState& st,
                      const SgCoefF& sgf, double beta, double h, double dt, hipStream_t s,
-                     const std::function<int()>& after_interior, int kchunk)
+                     const std::function<int()>& after_interior, int kchunk, int zfill)
 {
    const float cof = (float)(1.0 / (h * h)), dt2 = (float)(dt * dt), c4 = (float)(dt * dt * dt * dt / 12);
    const int k1 = onesided[4] == 1 ? 7 : b.kb + 2;
@@ -713,7 +756,10 @@ int launch_comp_corr(const Box& b, int nk, const int* onesided, const SbpTablesT
       if (onesided[side] == 1 && (rc = comp_rows(true, b, side, nk, d_sbpf, st, sgf, cof, dt2, c4, (float)beta, s)))
          return rc;
    const long ns = shell_count(b);
-   k_comp_shell<true><<<cdiv(ns, 256), 256, 0, s>>>(b, st, dt2);
+   if (zfill)
+      k_comp_shell_fill<true><<<cdiv(ns, 256), 256, 0, s>>>(b, st, dt2, dt * dt, zfill);
+   else
+      k_comp_shell<true><<<cdiv(ns, 256), 256, 0, s>>>(b, st, dt2);
    return check_hip(hipGetLastError(), "FP32 corrector shell launch");
 }
 
@@ -732,9 +778,10 @@ namespace {
 template <class V>
 int comp_bc_view(const Box& b, const int* wind, int nx, int ny, int nz, const int* bccnd, const V& v, double h,
                  const SbpTables* d_sbp, const double* mu, const double* la, const double* strx, const double* stry,
-                 hipStream_t s)
+                 hipStream_t s, int skip)
 {
    for (int side = 0; side < 6; side++) {
+      if (skip & (1 << side)) continue;  // zero ghost values already set with the shell
       const int* w = wind + 6 * side;
       const CompWindow win{w[0], w[1], w[2], w[3], w[4], w[5]};
       const long n = (long)(w[1] - w[0] + 1) * (w[3] - w[2] + 1) * (w[5] - w[4] + 1);
@@ -761,14 +808,14 @@ int comp_bc_view(const Box& b, const int* wind, int nx, int ny, int nz, const in
 
 int launch_comp_bc(bool corr, const Box& b, const int* wind, int nx, int ny, int nz, const int* bccnd,
                    const CompState& cs, double h, double dt, const SbpTables* d_sbp, const double* mu,
-                   const double* la, const double* strx, const double* stry, hipStream_t s)
+                   const double* la, const double* strx, const double* stry, hipStream_t s, int skip)
 {
    if (corr) {
       const CorrView v{cs.U, cs.C, cs.Dn};
-      return comp_bc_view(b, wind, nx, ny, nz, bccnd, v, h, d_sbp, mu, la, strx, stry, s);
+      return comp_bc_view(b, wind, nx, ny, nz, bccnd, v, h, d_sbp, mu, la, strx, stry, s, skip);
    }
    const PredView v{cs.U, cs.C, cs.D, cs.A, dt * dt};
-   return comp_bc_view(b, wind, nx, ny, nz, bccnd, v, h, d_sbp, mu, la, strx, stry, s);
+   return comp_bc_view(b, wind, nx, ny, nz, bccnd, v, h, d_sbp, mu, la, strx, stry, s, skip);
 }
 
 int launch_comp_record(const Box& b, int nrec, const int* rec, const CompState& cs, double dt, double* out,
