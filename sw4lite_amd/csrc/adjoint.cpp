@@ -64,7 +64,9 @@ int EW::fused_once(double t)
 //   dt^2/rho (L(U) + F)        = Up - 2U + Um = dt^2 A,
 //   c4/rho (L(A) + F_tt) - D   = U^{n+1} - Up,
 // so d/drho [y . c4/rho (...)] + d/drho [-y . D]|_(1/rho) = -y . (U^{n+1} - Up)/rho.
-// tmp: 6 work fields of 3 components; pad: padded material (kernels_adj.h).
+// tmp: 5 work fields of 3 components; pad: padded material and the
+// transposed operator's work fields (kernels_adj.h), pad.w and pad.w2 zero
+// in the pad.
 int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double* lam_b, double* lam_c,
                  double* grho, double* gmu, double* gla, double* const* tmp, double* plane, const AdjPad& pad)
 {
@@ -74,7 +76,7 @@ int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double
    const double dt = mDt, t = n * dt, dt2 = dt * dt, c4 = dt2 * dt2 / 12;
    const long n3 = 3 * m_box.npts;
    const RowMap rm = make_rowmap(m_box, g.nz, m_onesided);
-   double *Up = tmp[0], *Acc = tmp[1], *Db = tmp[2], *Z = tmp[3], *LT1 = tmp[4], *LT2 = tmp[5];
+   double *Up = tmp[0], *Acc = tmp[1], *Db = tmp[2], *Z = tmp[3], *LT1 = tmp[4];
    // Y: the adjoint of the corrected field, then (in place) of Up
    double* Y = lam_a;
    Sarray UpA = mU;
@@ -92,23 +94,25 @@ int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double
       return rc;
    // damping: U^{n+1} -= D(U - Um); corrector: Up + c4/rho (L(A) + F_tt)
    if ((rc = launch_addsgd4_T_march(m_box, Y, u, um, mRho.dev, c, m_opt.beta, Db, grho, m_stream, false)) ||
-       (rc = launch_vjp_corr_pw(m_box, c4, mRho.dev, Y, u1, Up, Z, grho, m_stream)) ||
-       (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, LT1, pad, m_stream)) ||
+       (rc = launch_vjp_corr_pw(m_box, rm, c.strx, c.stry, c.strz, c4, mRho.dev, Y, u1, Up, Z, grho, pad, m_stream)) ||
+       (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, nullptr, pad,
+                             m_stream, true)) ||
        (rc = launch_rhs4sg_matgrad(m_box, rm, m_dsbp, Z, Acc, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, gmu,
                                    gla, m_stream)))
       return rc;
-   // A = (Up - 2U + Um)/dt^2: the Up part
-   if ((rc = launch_axpy(n3, 1.0 / dt2, LT1, Y, m_stream))) return rc;
+   // A = (Up - 2U + Um)/dt^2: the Up part (and L^T of the corrector kept for the U, Um parts)
+   if ((rc = launch_vjp_crop_axpy(m_box, 1.0 / dt2, pad, LT1, Y, m_stream))) return rc;
    // predictor BC, transposed (acting on Up)
    if ((rc = launch_bcfortsg_T(m_box, m_wind, g.nx, g.ny, g.nz, g.bc, Y, Up, g.h, m_dsbp, mMu.dev, mLambda.dev, c.strx,
                                c.stry, gmu, gla, plane, m_stream)))
       return rc;
    // predictor: 2U - Um + dt^2/rho (L(U) + F); then the U, Um parts of every stage
-   if ((rc = launch_vjp_pred_pw(m_box, dt2, mRho.dev, Y, Acc, Z, grho, m_stream)) ||
-       (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, LT2, pad, m_stream)) ||
+   if ((rc = launch_vjp_pred_pw(m_box, rm, c.strx, c.stry, c.strz, dt2, mRho.dev, Y, Acc, Z, grho, pad, m_stream)) ||
+       (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, nullptr, pad,
+                             m_stream, true)) ||
        (rc = launch_rhs4sg_matgrad(m_box, rm, m_dsbp, Z, u, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, gmu,
                                    gla, m_stream)) ||
-       (rc = launch_vjp_lam(n3, 1.0 / dt2, Db, LT1, Y, LT2, lam_b, lam_c, m_stream)))
+       (rc = launch_vjp_lam(m_box, 1.0 / dt2, Db, LT1, Y, pad, lam_b, lam_c, m_stream)))
       return rc;
    return OK;
 }
@@ -162,11 +166,11 @@ double EW::workspace_bytes() const
 }
 
 // Device doubles of the gradient's work memory besides the stored states:
-// 6 work fields, 3 adjoint fields, 3 gradients, the padded material, one
+// 5 work fields, 3 adjoint fields, 3 gradients, the padded material, one
 // plane of the boundary transposes and the receiver traces.
 static double adjoint_fixed_doubles(const Box& b, int nsteps, int nrec)
 {
-   return 9.0 * 3 * b.npts + 3.0 * b.npts + (double)adj_pad_doubles(b) + 3.0 * b.nij + 3.0 * (nsteps + 1) * nrec +
+   return 8.0 * 3 * b.npts + 3.0 * b.npts + (double)adj_pad_doubles(b) + 3.0 * b.nij + 3.0 * (nsteps + 1) * nrec +
           2.0 * (8 * nrec + 1);
 }
 
@@ -222,7 +226,7 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
    }
    const long n1 = m_box.npts, n3 = 3 * n1;
    const int ncp = (nsteps + ckpt - 1) / ckpt;
-   // work memory: 6 fields, 3 adjoint fields, ckpt+2 segment states, 2 per checkpoint, 3 gradients, pad, traces
+   // work memory: 5 fields, 3 adjoint fields, ckpt+2 segment states, 2 per checkpoint, 3 gradients, pad, traces
    // buffers come from the workspace pool and go back to it at the end
    auto release = [&]() { pool_return(); };
    auto grab = [&](size_t n, double** out) {
@@ -232,10 +236,10 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
       return (int)OK;
    };
    int rc = OK;
-   double* tmp[6];
+   double* tmp[5];
    double *lam[3], *gr[3], *plane = nullptr;
    std::vector<double*> seg(ckpt + 2), cp(2 * ncp);
-   for (int q = 0; q < 6 && rc == OK; q++) rc = grab(n3, &tmp[q]);
+   for (int q = 0; q < 5 && rc == OK; q++) rc = grab(n3, &tmp[q]);
    double* padmem = nullptr;
    if (rc == OK) rc = grab(adj_pad_doubles(m_box), &padmem);
    for (int q = 0; q < 3 && rc == OK; q++) rc = grab(n3, &lam[q]);
@@ -336,6 +340,9 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
    const SgCoef sgc = sg();
    const AdjPad pad = make_adj_pad(m_box, padmem);
    if (rc == OK) rc = launch_adj_pad_material(m_box, mMu.dev, mLambda.dev, sgc.strx, sgc.stry, sgc.strz, pad, m_stream);
+   // the reverse step writes W, 2W on the box points only: their pad stays zero
+   if (rc == OK) rc = launch_zero(pad.w, 3 * pad.pb.npts, m_stream);
+   if (rc == OK) rc = launch_zero(pad.w2, 3 * pad.pb.npts, m_stream);
    for (int q = 0; q < 3 && rc == OK; q++) rc = launch_zero(gr[q], n1, m_stream);
    for (int q = 0; q < 2 && rc == OK; q++) rc = launch_zero(lam[q], n3, m_stream);
    double *la = lam[0], *lb = lam[1], *lc = lam[2];

@@ -45,10 +45,13 @@ int launch_adj_pad_material(const Box& b, const double* mu, const double* la, co
                             const double* strz, const AdjPad& pad, hipStream_t s);
 
 // out = L^T z (overwritten on every point); z is read on the rows only.
-// pad: filled by launch_adj_pad_material for the same material.
+// pad: filled by launch_adj_pad_material for the same material.  weighted:
+// pad.w and pad.w2 already hold W = S z on the interior-formula rows and 2W
+// (launch_vjp_corr_pw / launch_vjp_pred_pw).  out null: the result stays in
+// pad.o on the padded box.
 int launch_rhs4sg_T(const Box& b, const RowMap& m, const SbpTables* d_sbp, const double* z, const double* mu,
                     const double* la, double h, const double* strx, const double* stry, const double* strz,
-                    double* out, const AdjPad& pad, hipStream_t s);
+                    double* out, const AdjPad& pad, hipStream_t s, bool weighted = false);
 // gmu += G_mu(z, u), gla += G_la(z, u) on every point.
 int launch_rhs4sg_matgrad(const Box& b, const RowMap& m, const SbpTables* d_sbp, const double* z, const double* u,
                           const double* mu, const double* la, double h, const double* strx, const double* stry,
@@ -75,19 +78,24 @@ int launch_bcfortsg_T(const Box& b, const int* wind, int nx, int ny, int nz, con
 // pointwise helper over n values
 int launch_axpy(long n, double a, const double* x, double* y, hipStream_t s);              // y += a x
 
-// Fused pointwise stages of the reverse step (every point unless said):
-// corrector: z = c4/rho y on the interior (0 elsewhere), grho -= y.(u1 - up)/rho
-// on the interior;
-int launch_vjp_corr_pw(const Box& b, double c4, const double* rho, const double* y, const double* u1,
-                       const double* up, double* z, double* grho, hipStream_t s);
-// predictor: z = dt2/rho yp on the interior (0 elsewhere), grho -= dt2 yp.acc/rho
-// on the interior;
-int launch_vjp_pred_pw(const Box& b, double dt2, const double* rho, const double* yp, const double* acc, double* z,
-                       double* grho, hipStream_t s);
-// adjoints of U^n and U^{n-1} through one step, with a = lt1/dt2:
-// lb += db - 2a + 2yp + lt2, lc = a - db - yp (lc is overwritten).
-int launch_vjp_lam(long n3, double dt2i, const double* db, const double* lt1, const double* yp, const double* lt2,
-                   double* lb, double* lc, hipStream_t s);
+// Fused pointwise stages of the reverse step (every point of the box unless
+// said).  corrector: z = c4/rho y on the interior (0 elsewhere), the
+// transposed operator's inputs W = S z on the interior-formula rows and 2W
+// into pad.w, pad.w2 (their pad must hold zeros), grho -= y.(u1 - up)/rho on
+// the interior;
+int launch_vjp_corr_pw(const Box& b, const RowMap& m, const double* strx, const double* stry, const double* strz,
+                       double c4, const double* rho, const double* y, const double* u1, const double* up, double* z,
+                       double* grho, const AdjPad& pad, hipStream_t s);
+// predictor: z = dt2/rho yp, W and 2W likewise, grho -= dt2 yp.acc/rho;
+int launch_vjp_pred_pw(const Box& b, const RowMap& m, const double* strx, const double* stry, const double* strz,
+                       double dt2, const double* rho, const double* yp, const double* acc, double* z, double* grho,
+                       const AdjPad& pad, hipStream_t s);
+// lt1 = pad.o (the transposed operator's result) on the box, y += lt1/dt2;
+int launch_vjp_crop_axpy(const Box& b, double dt2i, const AdjPad& pad, double* lt1, double* y, hipStream_t s);
+// adjoints of U^n and U^{n-1} through one step, with a = lt1/dt2 and lt2 =
+// pad.o: lb += db - 2a + 2yp + lt2, lc = a - db - yp (lc is overwritten).
+int launch_vjp_lam(const Box& b, double dt2i, const double* db, const double* lt1, const double* yp,
+                   const AdjPad& pad, double* lb, double* lc, hipStream_t s);
 // Receiver traces g[s][r][c] (s = 0..nsteps) as adjoint sources, grouped by
 // the grid value they act on: group q adds to lam[uidx[q]] the entries
 // ent[off[q] .. off[q+1]-1] (e = 3r + c) in that order, so receivers that

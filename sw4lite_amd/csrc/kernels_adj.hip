@@ -347,8 +347,8 @@ struct PadSten {
 
 // SBP rows, in-plane part on the row planes of one side: out += cof
 // (sx sy)^{-1} Lxy(W), W = sx sy zB
-__global__ void __launch_bounds__(256) k_rhs4sg_T_xy(AdjCtx A, AdjPad P, int kbase, int sgn, double* __restrict__ out,
-                                                     double cof)
+__global__ void __launch_bounds__(256) k_rhs4sg_T_xy(AdjCtx A, AdjPad P, int kbase, int sgn, Box ob,
+                                                     double* __restrict__ out, double cof)
 {
    const Box& b = A.b;
    const int i = b.ib + blockIdx.x * blockDim.x + threadIdx.x;
@@ -359,13 +359,13 @@ __global__ void __launch_bounds__(256) k_rhs4sg_T_xy(AdjCtx A, AdjPad P, int kba
    double r[3];
    rhs4_xy_body(s, r);
    const double sc = cof / (A.sx(i) * A.sy(j));
-   const long p = sidx(b, i, j, k);
-   for (int c = 0; c < 3; c++) out[p + b.npts * c] += sc * r[c];
+   const long p = sidx(ob, i, j, k);
+   for (int c = 0; c < 3; c++) out[p + ob.npts * c] += sc * r[c];
 }
 
 // SBP rows, k-direction terms on the planes 0..8 of one side
 __global__ void __launch_bounds__(256) k_rhs4sg_T_col(AdjCtx A, int kbase, int sgn, const double* __restrict__ z,
-                                                      double* __restrict__ out, double cof)
+                                                      Box ob, double* __restrict__ out, double cof)
 {
    const Box& b = A.b;
    const int i = b.ib + blockIdx.x * blockDim.x + threadIdx.x;
@@ -374,8 +374,8 @@ __global__ void __launch_bounds__(256) k_rhs4sg_T_col(AdjCtx A, int kbase, int s
    if (i > b.ie || j > b.je || k < b.kb || k > b.ke) return;
    double r[3] = {0, 0, 0};
    sbp_col_T(A, z, i, j, k, kbase, sgn, r);
-   const long p = sidx(b, i, j, k);
-   for (int c = 0; c < 3; c++) out[p + b.npts * c] += cof * r[c];
+   const long p = sidx(ob, i, j, k);
+   for (int c = 0; c < 3; c++) out[p + ob.npts * c] += cof * r[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -981,60 +981,95 @@ __global__ void k_axpy(long n, double a, const double* __restrict__ x, double* _
    if (q < n) y[q] += a * x[q];
 }
 
-__device__ __forceinline__ bool box_interior(const Box& b, long p)
+// z = coef/rho y on the interior (0 elsewhere) and, for the transposed
+// operator, W = S z on the interior-formula rows (0 elsewhere in the box;
+// S = strx stry strz) with 2W, on the padded box; grho -= gs y.(a - bs)/rho
+// on the interior (bs may be null).
+__device__ __forceinline__ void vjp_scaled_rows(const AdjCtx& A, const AdjPad& P, long p, double coef, double gs,
+                                                const double* __restrict__ rho, const double* __restrict__ y,
+                                                const double* __restrict__ a, const double* __restrict__ bs,
+                                                double* __restrict__ z, double* __restrict__ grho)
 {
+   const Box& b = A.b;
    const int i = b.ib + (int)(p % b.ni);
    const int j = b.jb + (int)((p / b.ni) % (b.je - b.jb + 1));
    const int k = b.kb + (int)(p / b.nij);
-   return i >= b.ib + 2 && i <= b.ie - 2 && j >= b.jb + 2 && j <= b.je - 2 && k >= b.kb + 2 && k <= b.ke - 2;
+   const bool in = i >= b.ib + 2 && i <= b.ie - 2 && j >= b.jb + 2 && j <= b.je - 2 && k >= b.kb + 2 && k <= b.ke - 2;
+   const bool row = int_row(A.m, i, j, k);
+   const double r = rho[p], sc = in ? coef / r : 0.0;
+   const double sw = row ? A.sx(i) * A.sy(j) * A.sz(k) : 0.0;
+   const long q = sidx(P.pb, i, j, k);
+   double g = 0;
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      const long v = p + b.npts * c;
+      const double yv = y[v], zv = sc * yv;
+      z[v] = zv;
+      const double w = row ? sw * zv : 0.0;
+      P.w[q + P.pb.npts * c] = w;
+      P.w2[q + P.pb.npts * c] = 2 * w;
+      g += yv * (a[v] - (bs ? bs[v] : 0.0));
+   }
+   if (in) grho[p] -= gs * g / r;
 }
 
-__global__ void k_vjp_corr_pw(Box b, double c4, const double* __restrict__ rho, const double* __restrict__ y,
-                              const double* __restrict__ u1, const double* __restrict__ up, double* __restrict__ z,
+// corrector: z = c4/rho y, W, 2W; grho -= y.(u1 - up)/rho
+__global__ void k_vjp_corr_pw(AdjCtx A, AdjPad P, double c4, const double* __restrict__ rho,
+                              const double* __restrict__ y, const double* __restrict__ u1,
+                              const double* __restrict__ up, double* __restrict__ z, double* __restrict__ grho)
+{
+   const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (p >= A.b.npts) return;
+   vjp_scaled_rows(A, P, p, c4, 1.0, rho, y, u1, up, z, grho);
+}
+
+// predictor: z = dt2/rho yp, W, 2W; grho -= dt2 yp.acc/rho
+__global__ void k_vjp_pred_pw(AdjCtx A, AdjPad P, double dt2, const double* __restrict__ rho,
+                              const double* __restrict__ yp, const double* __restrict__ acc, double* __restrict__ z,
                               double* __restrict__ grho)
 {
    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   if (p >= b.npts) return;
-   const bool in = box_interior(b, p);
-   const double r = rho[p], s = in ? c4 / r : 0.0;
-   double g = 0;
-#pragma unroll
-   for (int c = 0; c < 3; c++) {
-      const long q = p + b.npts * c;
-      const double yv = y[q];
-      z[q] = s * yv;
-      g += yv * (u1[q] - up[q]);
-   }
-   if (in) grho[p] -= g / r;
+   if (p >= A.b.npts) return;
+   vjp_scaled_rows(A, P, p, dt2, dt2, rho, yp, acc, nullptr, z, grho);
 }
 
-__global__ void k_vjp_pred_pw(Box b, double dt2, const double* __restrict__ rho, const double* __restrict__ yp,
-                              const double* __restrict__ acc, double* __restrict__ z, double* __restrict__ grho)
+// lt1 = the transposed operator's result (padded box) on the box; y += lt1/dt2
+__global__ void k_vjp_crop_axpy(Box b, Box pb, double dt2i, const double* __restrict__ o, double* __restrict__ lt1,
+                                double* __restrict__ y)
 {
    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (p >= b.npts) return;
-   const bool in = box_interior(b, p);
-   const double r = rho[p], s = in ? dt2 / r : 0.0;
-   double g = 0;
+   const int i = b.ib + (int)(p % b.ni);
+   const int j = b.jb + (int)((p / b.ni) % (b.je - b.jb + 1));
+   const int k = b.kb + (int)(p / b.nij);
+   const long q = sidx(pb, i, j, k);
 #pragma unroll
    for (int c = 0; c < 3; c++) {
-      const long q = p + b.npts * c;
-      const double yv = yp[q];
-      z[q] = s * yv;
-      g += yv * acc[q];
+      const double v = o[q + pb.npts * c];
+      lt1[p + b.npts * c] = v;
+      y[p + b.npts * c] += dt2i * v;
    }
-   if (in) grho[p] -= dt2 * g / r;
 }
 
-__global__ void k_vjp_lam(long n3, double dt2i, const double* __restrict__ db, const double* __restrict__ lt1,
-                          const double* __restrict__ yp, const double* __restrict__ lt2, double* __restrict__ lb,
+// with a = lt1/dt2 and lt2 read on the padded box:
+// lb += db - 2a + 2yp + lt2, lc = a - db - yp
+__global__ void k_vjp_lam(Box b, Box pb, double dt2i, const double* __restrict__ db, const double* __restrict__ lt1,
+                          const double* __restrict__ yp, const double* __restrict__ o2, double* __restrict__ lb,
                           double* __restrict__ lc)
 {
-   const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   if (q >= n3) return;
-   const double a = dt2i * lt1[q], d = db[q], y = yp[q];
-   lb[q] += d - 2 * a + 2 * y + lt2[q];
-   lc[q] = a - d - y;
+   const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (p >= b.npts) return;
+   const int i = b.ib + (int)(p % b.ni);
+   const int j = b.jb + (int)((p / b.ni) % (b.je - b.jb + 1));
+   const int k = b.kb + (int)(p / b.nij);
+   const long q = sidx(pb, i, j, k);
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      const long v = p + b.npts * c;
+      const double a = dt2i * lt1[v], d = db[v], y = yp[v];
+      lb[v] += d - 2 * a + 2 * y + o2[q + pb.npts * c];
+      lc[v] = a - d - y;
+   }
 }
 
 __global__ void k_receiver_sources_T(ReceiverAdj R, int s, int nsteps, double twodt, double* __restrict__ lam)
@@ -1069,7 +1104,7 @@ int launch_adj_pad_material(const Box& b, const double* mu, const double* la, co
 
 int launch_rhs4sg_T(const Box& b, const RowMap& m, const SbpTables* d_sbp, const double* z, const double* mu,
                     const double* la, double h, const double* strx, const double* stry, const double* strz,
-                    double* out, const AdjPad& P, hipStream_t s)
+                    double* out, const AdjPad& P, hipStream_t s, bool weighted)
 {
    const AdjCtx A{b, m, d_sbp, mu, la, strx, stry, strz};
    const double cof = 1.0 / (h * h);
@@ -1078,7 +1113,7 @@ int launch_rhs4sg_T(const Box& b, const RowMap& m, const SbpTables* d_sbp, const
    // interior rows: S^{-1} L(S zI) on every point of the box, by the LDS
    // predictor sweep over the padded box (2W - 2W + cof L(W)/S; W, mu and
    // lambda vanish in the pad, the stretching is 1 there)
-   k_adj_weight<false><<<cdiv(P.pb.npts, 256), 256, 0, s>>>(A, P, z);
+   if (!weighted) k_adj_weight<false><<<cdiv(P.pb.npts, 256), 256, 0, s>>>(A, P, z);
    SgCoef sg{};
    sg.strx = P.sx;
    sg.stry = P.sy;
@@ -1089,14 +1124,17 @@ int launch_rhs4sg_T(const Box& b, const RowMap& m, const SbpTables* d_sbp, const
    int rc = launch_sweep_lds(false, 6, pb, pb.kb + 2, pb.ke - 2, march_kchunk(tiles, npl, 2), nullptr, P.w, P.w2,
                              P.mu, P.la, P.s, sg, P.o, h, 1.0, 0.0, s);
    if (rc) return rc;
-   k_adj_crop<<<cdiv(b.npts, 256), 256, 0, s>>>(b, pb, P.o, out);
+   // the result on the box, or left on the padded box (out null)
+   const Box ob = out ? b : pb;
+   double* const dst = out ? out : P.o;
+   if (out) k_adj_crop<<<cdiv(b.npts, 256), 256, 0, s>>>(b, pb, P.o, out);
    // SBP rows: in-plane part from (sx sy) zB, then the k-direction terms
    if (m.top || m.bot) {
       k_adj_weight<true><<<cdiv(P.pb.npts, 256), 256, 0, s>>>(A, P, z);
-      if (m.top) k_rhs4sg_T_xy<<<dim3(gi, gj, 6), blk, 0, s>>>(A, P, 1, 1, out, cof);
-      if (m.bot) k_rhs4sg_T_xy<<<dim3(gi, gj, 6), blk, 0, s>>>(A, P, m.nz, -1, out, cof);
-      if (m.top) k_rhs4sg_T_col<<<dim3(gi, gj, 9), blk, 0, s>>>(A, 1, 1, z, out, cof);
-      if (m.bot) k_rhs4sg_T_col<<<dim3(gi, gj, 9), blk, 0, s>>>(A, m.nz, -1, z, out, cof);
+      if (m.top) k_rhs4sg_T_xy<<<dim3(gi, gj, 6), blk, 0, s>>>(A, P, 1, 1, ob, dst, cof);
+      if (m.bot) k_rhs4sg_T_xy<<<dim3(gi, gj, 6), blk, 0, s>>>(A, P, m.nz, -1, ob, dst, cof);
+      if (m.top) k_rhs4sg_T_col<<<dim3(gi, gj, 9), blk, 0, s>>>(A, 1, 1, z, ob, dst, cof);
+      if (m.bot) k_rhs4sg_T_col<<<dim3(gi, gj, 9), blk, 0, s>>>(A, m.nz, -1, z, ob, dst, cof);
    }
    return check_hip(hipGetLastError(), "rhs4sg transpose launch");
 }
@@ -1165,24 +1203,34 @@ int launch_axpy(long n, double a, const double* x, double* y, hipStream_t s)
    return check_hip(hipGetLastError(), "axpy launch");
 }
 
-int launch_vjp_corr_pw(const Box& b, double c4, const double* rho, const double* y, const double* u1,
-                       const double* up, double* z, double* grho, hipStream_t s)
+int launch_vjp_corr_pw(const Box& b, const RowMap& m, const double* strx, const double* stry, const double* strz,
+                       double c4, const double* rho, const double* y, const double* u1, const double* up, double* z,
+                       double* grho, const AdjPad& P, hipStream_t s)
 {
-   k_vjp_corr_pw<<<cdiv(b.npts, 256), 256, 0, s>>>(b, c4, rho, y, u1, up, z, grho);
+   const AdjCtx A{b, m, nullptr, nullptr, nullptr, strx, stry, strz};
+   k_vjp_corr_pw<<<cdiv(b.npts, 256), 256, 0, s>>>(A, P, c4, rho, y, u1, up, z, grho);
    return check_hip(hipGetLastError(), "reverse corrector launch");
 }
 
-int launch_vjp_pred_pw(const Box& b, double dt2, const double* rho, const double* yp, const double* acc, double* z,
-                       double* grho, hipStream_t s)
+int launch_vjp_pred_pw(const Box& b, const RowMap& m, const double* strx, const double* stry, const double* strz,
+                       double dt2, const double* rho, const double* yp, const double* acc, double* z, double* grho,
+                       const AdjPad& P, hipStream_t s)
 {
-   k_vjp_pred_pw<<<cdiv(b.npts, 256), 256, 0, s>>>(b, dt2, rho, yp, acc, z, grho);
+   const AdjCtx A{b, m, nullptr, nullptr, nullptr, strx, stry, strz};
+   k_vjp_pred_pw<<<cdiv(b.npts, 256), 256, 0, s>>>(A, P, dt2, rho, yp, acc, z, grho);
    return check_hip(hipGetLastError(), "reverse predictor launch");
 }
 
-int launch_vjp_lam(long n3, double dt2i, const double* db, const double* lt1, const double* yp, const double* lt2,
+int launch_vjp_crop_axpy(const Box& b, double dt2i, const AdjPad& P, double* lt1, double* y, hipStream_t s)
+{
+   k_vjp_crop_axpy<<<cdiv(b.npts, 256), 256, 0, s>>>(b, P.pb, dt2i, P.o, lt1, y);
+   return check_hip(hipGetLastError(), "reverse step copy launch");
+}
+
+int launch_vjp_lam(const Box& b, double dt2i, const double* db, const double* lt1, const double* yp, const AdjPad& P,
                    double* lb, double* lc, hipStream_t s)
 {
-   k_vjp_lam<<<cdiv(n3, 256), 256, 0, s>>>(n3, dt2i, db, lt1, yp, lt2, lb, lc);
+   k_vjp_lam<<<cdiv(b.npts, 256), 256, 0, s>>>(b, P.pb, dt2i, db, lt1, yp, P.o, lb, lc);
    return check_hip(hipGetLastError(), "reverse step adjoint launch");
 }
 
