@@ -42,16 +42,17 @@ int EW::fused_once(double t)
    const SgCoef c = sg();
    const SourceTable st = sources();
    const double dt = mDt;
+   const int zf = zfill();
    int rc;
    if ((rc = launch_pred_fused(m_box, g.nz, m_onesided, m_dsbp, mU.dev, mUm.dev, mMu.dev, mLambda.dev, mRho.dev, c,
-                               mUp.dev, g.h, dt, m_stream, nullptr, m_variant, m_kchunk)) ||
+                               mUp.dev, g.h, dt, m_stream, nullptr, m_variant, m_kchunk, zf)) ||
        (rc = launch_src_apply(m_box, st, t, 0, dt * dt, mRho.dev, mUp.dev, m_stream)) ||
-       (rc = enforceBC(mUp, mMu, mLambda, t + dt)) ||
+       (rc = enforceBC(mUp, mMu, mLambda, t + dt, zf)) ||
        (rc = launch_corr_fused(m_box, g.nz, m_onesided, m_dsbp, mUp.dev, mU.dev, mUm.dev, mMu.dev, mLambda.dev,
                                mRho.dev, c, m_opt.beta, mUp2.dev, m_dacc, g.h, dt, m_stream, nullptr, nullptr, m_variant,
-                               m_kchunk)) ||
+                               m_kchunk, zf)) ||
        (rc = launch_src_apply(m_box, st, t, 1, dt * dt * dt * dt / 12, mRho.dev, mUp2.dev, m_stream)) ||
-       (rc = enforceBC(mUp2, mMu, mLambda, t + dt)))
+       (rc = enforceBC(mUp2, mMu, mLambda, t + dt, zf)))
       return rc;
    return OK;
 }
@@ -83,10 +84,12 @@ int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double
    UpA.dev = Up;
    int rc;
    // forward: Up = BC(2U - Um + dt^2/rho (L(U) + F)), A = (Up - 2U + Um)/dt^2
+   const int zf = zfill();
    if ((rc = launch_pred_fused(m_box, g.nz, m_onesided, m_dsbp, u, um, mMu.dev, mLambda.dev, mRho.dev, c, Up, g.h, dt,
-                               m_stream, nullptr, m_variant, m_kchunk)) ||
+                               m_stream, nullptr, m_variant, m_kchunk, zf)) ||
        (rc = launch_src_apply(m_box, st, t, 0, dt2, mRho.dev, Up, m_stream)) ||
-       (rc = enforceBC(UpA, mMu, mLambda, t + dt)) || (rc = launch_dpdmt(m_box, Up, u, um, Acc, 1.0 / dt2, m_stream)))
+       (rc = enforceBC(UpA, mMu, mLambda, t + dt, zf)) ||
+       (rc = launch_dpdmt(m_box, Up, u, um, Acc, 1.0 / dt2, m_stream)))
       return rc;
    // corrector BC, transposed (acting on U^{n+1})
    if ((rc = launch_bcfortsg_T(m_box, m_wind, g.nx, g.ny, g.nz, g.bc, Y, u1, g.h, m_dsbp, mMu.dev, mLambda.dev, c.strx,

@@ -160,10 +160,11 @@ int Batch::one_step(int ph)
       double* u = m_base[3 * q + (ph + 1) % 3];
       const double dt = e->mDt;
       hipStream_t st = e->m_stream;
+      const int zf = e->zfill();
       if ((rc = launch_pred_tail(e->m_box, g.nz, e->m_onesided, e->m_dsbp, u, um, e->mMu.dev, e->mLambda.dev,
-                                 e->mRho.dev, c, e->mUp.dev, g.h, dt, st)) ||
+                                 e->mRho.dev, c, e->mUp.dev, g.h, dt, st, zf)) ||
           (rc = launch_src_apply(e->m_box, e->sources(), 0.0, 0, dt * dt, e->mRho.dev, e->mUp.dev, st, m_clk_t + q)) ||
-          (rc = e->enforceBC(e->mUp, e->mMu, e->mLambda, 0.0)))
+          (rc = e->enforceBC(e->mUp, e->mMu, e->mLambda, 0.0, zf)))
          return rc;
    }
    if ((rc = join()) || (rc = launch_sweep_batch(1, tab + n, n, m_nblk[ph][1], m_s)) ||
@@ -181,11 +182,12 @@ int Batch::one_step(int ph)
       const double dt = e->mDt;
       Sarray v = e->mU;
       v.dev = up2;
+      const int zf = e->zfill();
       if ((rc = launch_corr_tail(e->m_box, g.nz, e->m_onesided, e->m_dsbp, e->mUp.dev, u, um, e->mMu.dev,
-                                 e->mLambda.dev, e->mRho.dev, c, e->m_opt.beta, up2, g.h, dt, st)) ||
+                                 e->mLambda.dev, e->mRho.dev, c, e->m_opt.beta, up2, g.h, dt, st, zf)) ||
           (rc = launch_src_apply(e->m_box, e->sources(), 0.0, 1, dt * dt * dt * dt / 12, e->mRho.dev, up2, st,
                                  m_clk_t + q)) ||
-          (rc = e->enforceBC(v, e->mMu, e->mLambda, 0.0)))
+          (rc = e->enforceBC(v, e->mMu, e->mLambda, 0.0, zf)))
          return rc;
       const int nrec = (int)e->m_rec.size() / 4;
       if (nrec > 0 && e->m_drecdata &&

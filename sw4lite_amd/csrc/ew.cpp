@@ -372,15 +372,30 @@ int EW::addSuperGridDamping(Sarray& Up, Sarray& U, Sarray& Um, Sarray& Rho)
 
 // EW::enforceBC with cartesian_bc_forcing giving zero boundary data
 // (EW.C:3320-3344, 3446-3501)
-int EW::enforceBC(Sarray& U, Sarray& Mu, Sarray& Lambda, double t)
+int EW::zfill() const
+{
+   if (m_curv) return 0;
+   const GridSetup& g = m_opt.grid;
+   int z = 0;
+   for (int q = 0; q < 6; q++) {
+      if (g.bc[q] == BC_PERIODIC) return 0;
+      if (g.bc[q] == BC_DIRICHLET || g.bc[q] == BC_SUPERGRID) z |= 1 << q;
+   }
+   return z;
+}
+
+int EW::enforceBC(Sarray& U, Sarray& Mu, Sarray& Lambda, double t, int skip)
 {
    (void)t;
    const GridSetup& g = m_opt.grid;
    const double* bforce[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
    const SgCoef c = sg();
-   if (!m_curv)
-      return launch_bcfortsg(m_opt.corder, m_box, m_wind, g.nx, g.ny, g.nz, U.dev, g.h, g.bc, m_dsbp, Mu.dev,
+   if (!m_curv) {
+      int bc[6];
+      for (int q = 0; q < 6; q++) bc[q] = (skip >> q) & 1 ? BC_NONE : g.bc[q];
+      return launch_bcfortsg(m_opt.corder, m_box, m_wind, g.nx, g.ny, g.nz, U.dev, g.h, bc, m_dsbp, Mu.dev,
                              Lambda.dev, bforce, c.strx, c.stry, m_stream);
+   }
    // curvilinear top: the Cartesian closure of bcfortsg is overwritten by
    // freesurfcurvisg at the same points (EW.C:3479-3497), so it is skipped
    int bc[6];
@@ -452,19 +467,21 @@ int EW::step_fused(int n, double* stage_ms)
    auto mark = [&](int s, int q) {
       return stage_ms ? check_hip(hipEventRecord(ev[(size_t)s * (kStages + 1) + q], m_stream), "hipEventRecord") : OK;
    };
+   const int zf = zfill();
    for (int s = 0; s < n && rc == OK; s++) {
       const double t = mT, dt = mDt;
       if ((rc = mark(s, 0)) ||
           (rc = launch_pred_fused(m_box, g.nz, m_onesided, m_dsbp, mU.dev, mUm.dev, mMu.dev, mLambda.dev, mRho.dev,
-                                  c, mUp.dev, g.h, dt, m_stream, [&]() { return mark(s, 1); }, m_variant, m_kchunk)) ||
+                                  c, mUp.dev, g.h, dt, m_stream, [&]() { return mark(s, 1); }, m_variant, m_kchunk,
+                                  zf)) ||
           (rc = mark(s, 2)) || (rc = launch_src_apply(m_box, st, t, 0, dt * dt, mRho.dev, mUp.dev, m_stream)) ||
-          (rc = enforceBC(mUp, mMu, mLambda, t + dt)) || (rc = mark(s, 3)) ||
+          (rc = enforceBC(mUp, mMu, mLambda, t + dt, zf)) || (rc = mark(s, 3)) ||
           (rc = launch_corr_fused(m_box, g.nz, m_onesided, m_dsbp, mUp.dev, mU.dev, mUm.dev, mMu.dev, mLambda.dev,
                                   mRho.dev, c, m_opt.beta, mUp2.dev, m_dacc, g.h, dt, m_stream, [&]() { return mark(s, 4); },
-                                  [&]() { return mark(s, 5); }, m_variant, m_kchunk)) ||
+                                  [&]() { return mark(s, 5); }, m_variant, m_kchunk, zf)) ||
           (rc = mark(s, 6)) ||
           (rc = launch_src_apply(m_box, st, t, 1, dt * dt * dt * dt / 12, mRho.dev, mUp2.dev, m_stream)) ||
-          (rc = enforceBC(mUp2, mMu, mLambda, t + dt)) || (rc = mark(s, 7)) || (rc = record(mUp2, mUm)))
+          (rc = enforceBC(mUp2, mMu, mLambda, t + dt, zf)) || (rc = mark(s, 7)) || (rc = record(mUp2, mUm)))
          break;
       mT += dt;
       mCycle++;

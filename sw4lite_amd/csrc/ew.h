@@ -67,7 +67,11 @@ public:
    int evalCorrector(Sarray& Up, Sarray& Rho, Sarray& Lu, Sarray& F);
    int evalDpDmInTime(Sarray& Up, Sarray& U, Sarray& Um, Sarray& Uacc);
    int addSuperGridDamping(Sarray& Up, Sarray& U, Sarray& Um, Sarray& Rho);
-   int enforceBC(Sarray& U, Sarray& Mu, Sarray& Lambda, double t);
+   // skip (bit s): side s already holds its zero ghost values (zfill())
+   int enforceBC(Sarray& U, Sarray& Mu, Sarray& Lambda, double t, int skip = 0);
+   // Sides whose zero ghost values the fused step's shell pass sets
+   // (Dirichlet or supergrid; none with a periodic side or on a curvilinear grid)
+   int zfill() const;
    void cycleSolutionArrays();
 
    bool curvilinear() const { return m_curv; }

@@ -154,16 +154,21 @@ k_addsgd4_planes(Box b, int klo, int khi, double* __restrict__ up, const double*
 // predictor where L(u) and F vanish); mode 1: out = in.
 // ---------------------------------------------------------------------------
 __global__ void k_shell(Box b, int mode, const double* __restrict__ a, const double* __restrict__ u,
-                        const double* __restrict__ um, double* __restrict__ out)
+                        const double* __restrict__ um, double* __restrict__ out, int zfill)
 {
    const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
    int i, j, k;
    if (!shell_point(b, q, i, j, k)) return;
+   // zfill (bit s): side s takes zero ghost values (Dirichlet or supergrid,
+   // no data) in this pass instead of in the boundary conditions
+   const bool zero = ((zfill & 1) && i <= b.ib + 1) || ((zfill & 2) && i >= b.ie - 1) ||
+                     ((zfill & 4) && j <= b.jb + 1) || ((zfill & 8) && j >= b.je - 1) ||
+                     ((zfill & 16) && k <= b.kb + 1) || ((zfill & 32) && k >= b.ke - 1);
    const long p = sidx(b, i, j, k);
 #pragma unroll
    for (int c = 0; c < 3; c++) {
       const long v = p + b.npts * c;
-      out[v] = mode == 0 ? 2 * u[v] - um[v] : a[v];
+      out[v] = zero ? 0.0 : mode == 0 ? 2 * u[v] - um[v] : a[v];
    }
 }
 
@@ -477,7 +482,8 @@ int launch_clock_tick(int n, double* t, int* smp, const double* dt, const int* m
 int launch_pred_fused(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp,
                       const double* u, const double* um, const double* mu, const double* la,
                       const double* rho, const SgCoef& sg, double* up, double h, double dt,
-                      hipStream_t s, const std::function<int()>& after_interior, int variant, int kchunk)
+                      hipStream_t s, const std::function<int()>& after_interior, int variant, int kchunk,
+                      int zfill)
 {
    const double cof = 1.0 / (h * h), dt2 = dt * dt;
    const int k1 = onesided[4] == 1 ? 7 : b.kb + 2;
@@ -496,14 +502,14 @@ int launch_pred_fused(const Box& b, int nk, const int* onesided, const SbpTables
       const int rc = after_interior();
       if (rc) return rc;
    }
-   if (variant >= 1) return launch_pred_tail(b, nk, onesided, d_sbp, u, um, mu, la, rho, sg, up, h, dt, s);
+   if (variant >= 1) return launch_pred_tail(b, nk, onesided, d_sbp, u, um, mu, la, rho, sg, up, h, dt, s, zfill);
    for (int side = 4; side <= 5; side++) {
       if (onesided[side] != 1) continue;
       dim3 grd(cdiv(nii, 64), cdiv(njj, 4), 6);
       k_pred_sbp<<<grd, blk, 0, s>>>(b, side == 4, nk, d_sbp, u, um, mu, la, rho, sg.strx, sg.stry, up, cof, dt2);
    }
    const long ns = shell_count(b);
-   k_shell<<<cdiv(ns, 256), 256, 0, s>>>(b, 0, nullptr, u, um, up);
+   k_shell<<<cdiv(ns, 256), 256, 0, s>>>(b, 0, nullptr, u, um, up, zfill);
    return check_hip(hipGetLastError(), "fused predictor launch");
 }
 
@@ -511,7 +517,7 @@ long accel_slab_size(const Box& b) { return 3 * b.nij * SLAB_PLANES; }
 
 int launch_pred_tail(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp, const double* u,
                      const double* um, const double* mu, const double* la, const double* rho, const SgCoef& sg,
-                     double* up, double h, double dt, hipStream_t s)
+                     double* up, double h, double dt, hipStream_t s, int zfill)
 {
    for (int side = 4; side <= 5; side++) {  // LDS slab of the six rows
       if (onesided[side] != 1) continue;
@@ -519,13 +525,14 @@ int launch_pred_tail(const Box& b, int nk, const int* onesided, const SbpTables*
       if (rc) return rc;
    }
    const long ns = shell_count(b);
-   k_shell<<<cdiv(ns, 256), 256, 0, s>>>(b, 0, nullptr, u, um, up);
+   k_shell<<<cdiv(ns, 256), 256, 0, s>>>(b, 0, nullptr, u, um, up, zfill);
    return check_hip(hipGetLastError(), "predictor rows and shell launch");
 }
 
 int launch_corr_tail(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp, const double* up,
                      const double* u, const double* um, const double* mu, const double* la, const double* rho,
-                     const SgCoef& sg, double beta, double* upo, double h, double dt, hipStream_t s)
+                     const SgCoef& sg, double beta, double* upo, double h, double dt, hipStream_t s,
+                     int zfill)
 {
    for (int side = 4; side <= 5; side++) {  // LDS slab, acceleration formed while staging
       if (onesided[side] != 1) continue;
@@ -533,7 +540,7 @@ int launch_corr_tail(const Box& b, int nk, const int* onesided, const SbpTables*
       if (rc) return rc;
    }
    const long ns = shell_count(b);
-   k_shell<<<cdiv(ns, 256), 256, 0, s>>>(b, 1, up, nullptr, nullptr, upo);
+   k_shell<<<cdiv(ns, 256), 256, 0, s>>>(b, 1, up, nullptr, nullptr, upo, zfill);
    return check_hip(hipGetLastError(), "corrector rows and shell launch");
 }
 
@@ -542,7 +549,7 @@ int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables
                       const double* la, const double* rho, const SgCoef& sg, double beta,
                       double* upo, double* acc, double h, double dt, hipStream_t s,
                       const std::function<int()>& after_interior, const std::function<int()>& after_damping,
-                      int variant, int kchunk)
+                      int variant, int kchunk, int zfill)
 {
    const double cof = 1.0 / (h * h), dt2i = 1.0 / (dt * dt), dt4i12 = dt * dt * dt * dt / 12;
    const int k1 = onesided[4] == 1 ? 7 : b.kb + 2;
@@ -571,7 +578,8 @@ int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables
       const int rc = after_damping();
       if (rc) return rc;
    }
-   if (variant >= 1) return launch_corr_tail(b, nk, onesided, d_sbp, up, u, um, mu, la, rho, sg, beta, upo, h, dt, s);
+   if (variant >= 1)
+      return launch_corr_tail(b, nk, onesided, d_sbp, up, u, um, mu, la, rho, sg, beta, upo, h, dt, s, zfill);
    for (int side = 4; side <= 5; side++) {
       if (onesided[side] != 1) continue;
       const int ks = side == 4 ? 0 : nk - SLAB_PLANES + 2;  // planes KK(0)..KK(8) of the rows
@@ -581,7 +589,7 @@ int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables
                                      dt4i12, beta);
    }
    const long ns = shell_count(b);
-   k_shell<<<cdiv(ns, 256), 256, 0, s>>>(b, 1, up, nullptr, nullptr, upo);
+   k_shell<<<cdiv(ns, 256), 256, 0, s>>>(b, 1, up, nullptr, nullptr, upo, zfill);
    return check_hip(hipGetLastError(), "fused corrector launch");
 }
 
