@@ -29,9 +29,9 @@ rounding of its stencils (~1e-7 of the increment), and these add up over
 the run, about 1e-7 |u| per radian of phase.  The 1024^3 precision workload
 (800 steps of 1.5 ms) stays below 1e-6; these runs cover 9 s with a
 16.7 Hz source, and measure 8.7e-6 / 1.05e-5 / 2.26e-5 (LOH.1, x / y / z)
-and 9.7e-6 / 1.12e-5 / 2.75e-5 (LOH.2).  The float rounding of the
-material (a velocity shift of up to 3e-8 in each layer, i.e. a phase error
-of 3e-8 per radian) contributes as much.  The bar here is 5e-5."""
+and 9.7e-6 / 1.12e-5 / 2.75e-5 (LOH.2).  The float material alone is
+not the cause: the FP64 step with rho, mu, lambda rounded to float gives
+1.5e-7 .. 2.8e-7 on LOH.1.  The bar here is 5e-5."""
 import json
 import os
 
