@@ -66,8 +66,8 @@ int EW::fused_once(double t)
 //   c4/rho (L(A) + F_tt) - D   = U^{n+1} - Up,
 // so d/drho [y . c4/rho (...)] + d/drho [-y . D]|_(1/rho) = -y . (U^{n+1} - Up)/rho.
 // tmp: 5 work fields of 3 components; pad: padded material and the
-// transposed operator's work fields (kernels_adj.h), pad.w and pad.w2 zero
-// in the pad.
+// transposed operator's work fields (kernels_adj.h), pad.w zero in the
+// pad.
 int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double* lam_b, double* lam_c,
                  double* grho, double* gmu, double* gla, double* const* tmp, double* plane, const AdjPad& pad)
 {
@@ -343,9 +343,8 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
    const SgCoef sgc = sg();
    const AdjPad pad = make_adj_pad(m_box, padmem);
    if (rc == OK) rc = launch_adj_pad_material(m_box, mMu.dev, mLambda.dev, sgc.strx, sgc.stry, sgc.strz, pad, m_stream);
-   // the reverse step writes W, 2W on the box points only: their pad stays zero
+   // the reverse step writes W on the box points only: its pad stays zero
    if (rc == OK) rc = launch_zero(pad.w, 3 * pad.pb.npts, m_stream);
-   if (rc == OK) rc = launch_zero(pad.w2, 3 * pad.pb.npts, m_stream);
    for (int q = 0; q < 3 && rc == OK; q++) rc = launch_zero(gr[q], n1, m_stream);
    for (int q = 0; q < 2 && rc == OK; q++) rc = launch_zero(lam[q], n3, m_stream);
    double *la = lam[0], *lb = lam[1], *lc = lam[2];

@@ -33,10 +33,8 @@ struct AdjPad {
    double* sx;  // pb.ie - pb.ib + 1 values, and so on
    double* sy;
    double* sz;
-   double* w2;  // 2W (3 components), S = sx sy sz and the sweep's output
-   double* s;   // (3 components), all on pb: the LDS predictor sweep over pb
-   double* o;   // evaluates cof L(W)/S = 2W - 2W + cof L(W)/S (rho := S, dt = 1)
-                // on every point of the box
+   double* o;   // the transposed sweep's output cof L(W)/S (3 components on
+                // pb), S = sx sy sz, on every point of the box
 };
 Box pad_box(const Box& b);
 long adj_pad_doubles(const Box& b);                 // doubles AdjPad needs
@@ -46,7 +44,7 @@ int launch_adj_pad_material(const Box& b, const double* mu, const double* la, co
 
 // out = L^T z (overwritten on every point); z is read on the rows only.
 // pad: filled by launch_adj_pad_material for the same material.  weighted:
-// pad.w and pad.w2 already hold W = S z on the interior-formula rows and 2W
+// pad.w already holds W = S z on the interior-formula rows
 // (launch_vjp_corr_pw / launch_vjp_pred_pw).  out null: the result stays in
 // pad.o on the padded box.
 int launch_rhs4sg_T(const Box& b, const RowMap& m, const SbpTables* d_sbp, const double* z, const double* mu,
@@ -80,13 +78,12 @@ int launch_axpy(long n, double a, const double* x, double* y, hipStream_t s);   
 
 // Fused pointwise stages of the reverse step (every point of the box unless
 // said).  corrector: z = c4/rho y on the interior (0 elsewhere), the
-// transposed operator's inputs W = S z on the interior-formula rows and 2W
-// into pad.w, pad.w2 (their pad must hold zeros), grho -= y.(u1 - up)/rho on
-// the interior;
+// transposed operator's input W = S z on the interior-formula rows into
+// pad.w (its pad must hold zeros), grho -= y.(u1 - up)/rho on the interior;
 int launch_vjp_corr_pw(const Box& b, const RowMap& m, const double* strx, const double* stry, const double* strz,
                        double c4, const double* rho, const double* y, const double* u1, const double* up, double* z,
                        double* grho, const AdjPad& pad, hipStream_t s);
-// predictor: z = dt2/rho yp, W and 2W likewise, grho -= dt2 yp.acc/rho;
+// predictor: z = dt2/rho yp, W likewise, grho -= dt2 yp.acc/rho;
 int launch_vjp_pred_pw(const Box& b, const RowMap& m, const double* strx, const double* stry, const double* strz,
                        double dt2, const double* rho, const double* yp, const double* acc, double* z, double* grho,
                        const AdjPad& pad, hipStream_t s);

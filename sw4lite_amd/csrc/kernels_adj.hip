@@ -247,7 +247,7 @@ Box pad_box(const Box& b) { return make_box(b.ib - 2, b.ie + 2, b.jb - 2, b.je +
 long adj_pad_doubles(const Box& b)
 {
    const Box pb = pad_box(b);
-   return 12 * pb.npts + pb.ni + (pb.je - pb.jb + 1) + (pb.ke - pb.kb + 1);
+   return 8 * pb.npts + pb.ni + (pb.je - pb.jb + 1) + (pb.ke - pb.kb + 1);
 }
 
 AdjPad make_adj_pad(const Box& b, double* mem)
@@ -260,9 +260,7 @@ AdjPad make_adj_pad(const Box& b, double* mem)
    p.sx = p.la + p.pb.npts;
    p.sy = p.sx + p.pb.ni;
    p.sz = p.sy + (p.pb.je - p.pb.jb + 1);
-   p.w2 = p.sz + (p.pb.ke - p.pb.kb + 1);
-   p.s = p.w2 + 3 * p.pb.npts;
-   p.o = p.s + p.pb.npts;
+   p.o = p.sz + (p.pb.ke - p.pb.kb + 1);
 
    return p;
 }
@@ -286,16 +284,6 @@ __global__ void k_adj_pad_material(Box b, Box pb, const double* __restrict__ mu,
    if (q < pb.ke - pb.kb + 1) szp[q] = (pb.kb + q >= b.kb && pb.kb + q <= b.ke) ? strz[pb.kb + q - b.kb] : 1.0;
 }
 
-// S = sx sy sz on the padded box, from the padded stretching
-__global__ void k_adj_sfield(Box pb, const double* __restrict__ sx, const double* __restrict__ sy,
-                             const double* __restrict__ sz, double* __restrict__ s)
-{
-   const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   if (p >= pb.npts) return;
-   const long i = p % pb.ni, j = (p / pb.ni) % (pb.je - pb.jb + 1), k = p / pb.nij;
-   s[p] = sx[i] * sy[j] * sz[k];
-}
-
 // padded W: INPLANE = false: S zI (interior-formula rows, S = sx sy sz);
 // true: sx sy zB (SBP rows)
 template <bool INPLANE>
@@ -313,7 +301,6 @@ __global__ void k_adj_weight(AdjCtx A, AdjPad P, const double* __restrict__ z)
    for (int c = 0; c < 3; c++) {
       const double v = row ? s * z[p + A.b.npts * c] : 0.0;
       P.w[q + pb.npts * c] = v;
-      if (!INPLANE) P.w2[q + pb.npts * c] = 2 * v;
    }
 }
 
@@ -983,8 +970,8 @@ __global__ void k_axpy(long n, double a, const double* __restrict__ x, double* _
 
 // z = coef/rho y on the interior (0 elsewhere) and, for the transposed
 // operator, W = S z on the interior-formula rows (0 elsewhere in the box;
-// S = strx stry strz) with 2W, on the padded box; grho -= gs y.(a - bs)/rho
-// on the interior (bs may be null).
+// S = strx stry strz) on the padded box; grho -= gs y.(a - bs)/rho on the
+// interior (bs may be null).
 __device__ __forceinline__ void vjp_scaled_rows(const AdjCtx& A, const AdjPad& P, long p, double coef, double gs,
                                                 const double* __restrict__ rho, const double* __restrict__ y,
                                                 const double* __restrict__ a, const double* __restrict__ bs,
@@ -1005,15 +992,13 @@ __device__ __forceinline__ void vjp_scaled_rows(const AdjCtx& A, const AdjPad& P
       const long v = p + b.npts * c;
       const double yv = y[v], zv = sc * yv;
       z[v] = zv;
-      const double w = row ? sw * zv : 0.0;
-      P.w[q + P.pb.npts * c] = w;
-      P.w2[q + P.pb.npts * c] = 2 * w;
+      P.w[q + P.pb.npts * c] = row ? sw * zv : 0.0;
       g += yv * (a[v] - (bs ? bs[v] : 0.0));
    }
    if (in) grho[p] -= gs * g / r;
 }
 
-// corrector: z = c4/rho y, W, 2W; grho -= y.(u1 - up)/rho
+// corrector: z = c4/rho y, W; grho -= y.(u1 - up)/rho
 __global__ void k_vjp_corr_pw(AdjCtx A, AdjPad P, double c4, const double* __restrict__ rho,
                               const double* __restrict__ y, const double* __restrict__ u1,
                               const double* __restrict__ up, double* __restrict__ z, double* __restrict__ grho)
@@ -1023,7 +1008,7 @@ __global__ void k_vjp_corr_pw(AdjCtx A, AdjPad P, double c4, const double* __res
    vjp_scaled_rows(A, P, p, c4, 1.0, rho, y, u1, up, z, grho);
 }
 
-// predictor: z = dt2/rho yp, W, 2W; grho -= dt2 yp.acc/rho
+// predictor: z = dt2/rho yp, W; grho -= dt2 yp.acc/rho
 __global__ void k_vjp_pred_pw(AdjCtx A, AdjPad P, double dt2, const double* __restrict__ rho,
                               const double* __restrict__ yp, const double* __restrict__ acc, double* __restrict__ z,
                               double* __restrict__ grho)
@@ -1098,7 +1083,6 @@ int launch_adj_pad_material(const Box& b, const double* mu, const double* la, co
 {
    k_adj_pad_material<<<cdiv(P.pb.npts, 256), 256, 0, s>>>(b, P.pb, mu, la, strx, stry, strz, P.mu, P.la, P.sx, P.sy,
                                                            P.sz);
-   k_adj_sfield<<<cdiv(P.pb.npts, 256), 256, 0, s>>>(P.pb, P.sx, P.sy, P.sz, P.s);
    return check_hip(hipGetLastError(), "adjoint pad launch");
 }
 
@@ -1111,8 +1095,8 @@ int launch_rhs4sg_T(const Box& b, const RowMap& m, const SbpTables* d_sbp, const
    const dim3 blk(64, 4, 1);
    const unsigned gi = cdiv(b.ni, 64), gj = cdiv(b.je - b.jb + 1, 4);
    // interior rows: S^{-1} L(S zI) on every point of the box, by the LDS
-   // predictor sweep over the padded box (2W - 2W + cof L(W)/S; W, mu and
-   // lambda vanish in the pad, the stretching is 1 there)
+   // predictor sweep over the padded box in its transposed mode (cof L(W)/S;
+   // W, mu and lambda vanish in the pad, the stretching is 1 there)
    if (!weighted) k_adj_weight<false><<<cdiv(P.pb.npts, 256), 256, 0, s>>>(A, P, z);
    SgCoef sg{};
    sg.strx = P.sx;
@@ -1121,8 +1105,7 @@ int launch_rhs4sg_T(const Box& b, const RowMap& m, const SbpTables* d_sbp, const
    const Box& pb = P.pb;
    const int npl = pb.ke - pb.kb - 3;  // the sweep's 32 x 8 tiles over the box's columns
    const long tiles = cdiv(pb.ni - 4, 32) * cdiv(pb.je - pb.jb - 3, 8);
-   int rc = launch_sweep_lds(false, 6, pb, pb.kb + 2, pb.ke - 2, march_kchunk(tiles, npl, 2), nullptr, P.w, P.w2,
-                             P.mu, P.la, P.s, sg, P.o, h, 1.0, 0.0, s);
+   int rc = launch_sweep_lds_T(pb, pb.kb + 2, pb.ke - 2, march_kchunk(tiles, npl, 2), P.w, P.mu, P.la, sg, P.o, h, s);
    if (rc) return rc;
    // the result on the box, or left on the padded box (out null)
    const Box ob = out ? b : pb;

@@ -52,5 +52,10 @@ int launch_damp_lds(const Box& b, int k1, int k2, int kc, double* up, const doub
 int launch_sweep_lds(bool corr, int variant, const Box& b, int k1, int k2, int kc, const double* upin,
                      const double* u, const double* um, const double* mu, const double* la, const double* rho,
                      const SgCoef& sg, double* out, double h, double dt, double beta, hipStream_t s);
+// out = cof L(w)/S (S = strx stry strz) on planes k1..k2 of the box's
+// interior columns: the predictor sweep without the time update, for the
+// transposed operator.
+int launch_sweep_lds_T(const Box& b, int k1, int k2, int kc, const double* w, const double* mu, const double* la,
+                       const SgCoef& sg, double* out, double h, hipStream_t s);
 
 }  // namespace sw4amd

@@ -59,7 +59,9 @@ struct LdsSten {
 // the next iteration.  The compute phase itself reads only LDS and registers:
 // the load counter is in order, so a global load issued after the prefetch
 // and waited on in the same iteration would wait for the prefetch as well.
-template <bool CORR, bool PIPE, bool DAMP>
+// TR (predictor only): out = cof L(u)/S with S = strx stry strz at the point
+// (no um, no rho): the transposed operator's L^T z = S^{-1} L(S z).
+template <bool CORR, bool PIPE, bool DAMP, bool TR = false>
 __device__ __forceinline__ void sweep_lds_body(unsigned bid, Box b, TileMap map, int kbeg, int kend, int kc,
                                                const double* __restrict__ upin, const double* __restrict__ u,
                                                const double* __restrict__ um, const double* __restrict__ mu,
@@ -172,9 +174,11 @@ __device__ __forceinline__ void sweep_lds_body(unsigned bid, Box b, TileMap map,
    double ctr[3] = {0, 0, 0}, crho = 1.0;
    auto load_centre = [&](int kp, double c3[3], double& r) {
       const long p = col + nij * (kp - b.kb);  // col = 0 where inactive: a valid address
+      if (!TR) {
 #pragma unroll
-      for (int c = 0; c < 3; c++) c3[c] = (CORR ? upin : um)[p + npts * c];
-      if (!DAMP) {
+         for (int c = 0; c < 3; c++) c3[c] = (CORR ? upin : um)[p + npts * c];
+      }
+      if (!DAMP && !TR) {
          const double v = rho[p];
          r = active ? v : 1.0;
       }
@@ -291,7 +295,7 @@ __device__ __forceinline__ void sweep_lds_body(unsigned bid, Box b, TileMap map,
          const long p = col + nij * (k - b.kb);
          const double r0 = DAMP ? sr[o] : crho;
          if (DAMP) rk0 = r0;
-         const double sc = cupd / r0;
+         const double sc = TR ? 1.0 / (s.psx[0] * s.psy[0] * s.szv[2]) : cupd / r0;
          double dmp[3] = {0, 0, 0};
          if (DAMP) {
             const double* dpl = sd + ((k - k0) % 3) * 3 * PL + o;  // d of plane k at this point
@@ -323,7 +327,9 @@ __device__ __forceinline__ void sweep_lds_body(unsigned bid, Box b, TileMap map,
          for (int c = 0; c < 3; c++) {
             const long q = p + npts * c;
             const double cv = sct[c * NT + tid];
-            if (!CORR)
+            if (TR)
+               out[q] = sc * (cof * r[c]);
+            else if (!CORR)
                out[q] = 2 * s.p2[c * PL] - cv + sc * (cof * r[c]);
             else if (DAMP)
                out[q] = (cv + sc * (cof * r[c])) - dmp[c];
@@ -349,14 +355,14 @@ __device__ __forceinline__ void sweep_lds_body(unsigned bid, Box b, TileMap map,
    }
 }
 
-template <bool CORR, bool PIPE, int MINW, bool DAMP = false>
+template <bool CORR, bool PIPE, int MINW, bool DAMP = false, bool TR = false>
 __global__ void __launch_bounds__(NT, MINW)
 k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __restrict__ upin,
             const double* __restrict__ u, const double* __restrict__ um, const double* __restrict__ mu,
             const double* __restrict__ la, const double* __restrict__ rho, SgCoef sg, double* __restrict__ out,
             double cof, double dt2i, double cupd, double beta)
 {
-   sweep_lds_body<CORR, PIPE, DAMP>(blockIdx.x, b, map, kbeg, kend, kc, upin, u, um, mu, la, rho, sg, out, cof, dt2i,
+   sweep_lds_body<CORR, PIPE, DAMP, TR>(blockIdx.x, b, map, kbeg, kend, kc, upin, u, um, mu, la, rho, sg, out, cof, dt2i,
                                     cupd, beta);
 }
 
@@ -728,6 +734,22 @@ int launch_sweep_lds(bool corr, int variant, const Box& b, int k1, int k2, int k
    }
 #undef SW4_SWEEP
    return check_hip(hipGetLastError(), corr ? "LDS corrector launch" : "LDS predictor launch");
+}
+
+int launch_sweep_lds_T(const Box& b, int k1, int k2, int kc, const double* w, const double* mu, const double* la,
+                       const SgCoef& sg, double* out, double h, hipStream_t s)
+{
+   if (k2 < k1) return OK;
+   const long nx = b.ie - b.ib - 3, ny = b.je - b.jb - 3;
+   if (kc <= 0) kc = 32;
+   TileMap map;
+   map.ntx = (int)cdiv(nx, TX);
+   map.nty = (int)cdiv(ny, TY);
+   map.ntiles = map.ntx * map.nty * (int)cdiv(k2 - k1 + 1, kc);
+   const unsigned grd = 8 * cdiv(map.ntiles, 8);
+   k_sweep_lds<false, false, 2, false, true><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, nullptr, w, nullptr, mu, la, nullptr,
+                                                               sg, out, 1.0 / (h * h), 1.0, 1.0, 0.0);
+   return check_hip(hipGetLastError(), "transposed sweep launch");
 }
 
 }  // namespace sw4amd
