@@ -61,9 +61,10 @@ def parse():
     p.add_argument("--no-fp32", action="store_true", help="skip the FP32 compensated form")
     p.add_argument("--cpu-nz", type=int, default=128, help="depth of the CPU-baseline slab")
     p.add_argument("--cpu-steps", type=int, default=3)
-    p.add_argument("--variant", type=int, default=6, help="interior sweep kernels: 0 thread/point, 1 LDS k-march, 2 with prefetch, "
+    p.add_argument("--variant", type=int, default=7, help="interior sweep kernels: 0 thread/point, 1 LDS k-march, 2 with prefetch, "
                         "3 prefetch without register cap, 4 as 3 with the damping in the corrector sweep, "
-                        "5 corrector as 4 and predictor as 1, 6 as 1 with an LDS damping sweep")
+                        "5 corrector as 4 and predictor as 1, 6 as 1 with an LDS damping sweep, "
+                        "7 as 6 with the predictor planes loaded asynchronously into LDS (even row lengths)")
     p.add_argument("--kchunk", type=int, default=64, help="planes per workgroup of the LDS sweeps")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also launch a plain device copy of 1 GiB (PMC scaling)")

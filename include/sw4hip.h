@@ -230,12 +230,14 @@ int sw4hip_ew_destroy(sw4hip_ew* ew);
 /* Stream used by every later call on this object (hipStream_t). */
 int sw4hip_ew_set_stream(sw4hip_ew* ew, void* stream);
 /* Interior kernels of the fused sweeps: variant 0 one thread per point;
- * variants 1-6 LDS-staged k-marching tiles with kchunk planes per workgroup
+ * variants 1-7 LDS-staged k-marching tiles with kchunk planes per workgroup
  * (default 64): 1 plain at 2 waves/SIMD, 2 register prefetch of the next
  * plane, 3 prefetch without a register cap, 4 as 3 with the supergrid
  * damping inside the corrector sweep, 5 corrector as 4 and predictor as 1,
- * 6 (the default) as 1 with the damping as its own LDS sweep.  All give the
- * same results to rounding. */
+ * 6 as 1 with the damping as its own LDS sweep, 7 (the default) as 6 with
+ * the predictor's planes brought to LDS by asynchronous loads one plane
+ * ahead (even row lengths; otherwise as 6).  All give the same results to
+ * rounding. */
 int sw4hip_ew_set_kernel_variant(sw4hip_ew* ew, int variant, int kchunk);
 /* rho, mu, lambda over the whole box incl. ghosts; on_device selects whether
  * the pointers are device (1) or host (0) arrays. */

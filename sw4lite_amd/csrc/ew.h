@@ -137,7 +137,7 @@ public:
    std::vector<GridPointSource> m_gps;
    std::vector<int> m_first;
    hipStream_t m_stream = nullptr;
-   int m_variant = 6;   // interior sweep kernels: 0 thread per point, 1-6 LDS-staged k-march
+   int m_variant = 7;   // interior sweep kernels: 0 thread per point, 1-7 LDS-staged k-march
    int m_kchunk = 64;   // planes per workgroup of the LDS sweeps
 
 private:

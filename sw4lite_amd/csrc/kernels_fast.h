@@ -40,7 +40,8 @@ int launch_sweep_batch(int kind, const SweepShot* d_shots, int nshots, int nbloc
 // variant 1: plain k-march, 2 waves/SIMD; 2: register prefetch of the next
 // plane, 2 waves/SIMD; 3: prefetch, 1 wave/SIMD; 4: as 3 with the corrector's
 // supergrid damping (beta != 0) applied in the same sweep; 5: corrector as 4,
-// predictor as 1; 6: both as 1, the damping by launch_damp_lds.
+// predictor as 1; 6: both as 1, the damping by launch_damp_lds; 7: as 6, the
+// predictor by asynchronous global-to-LDS loads where the rows allow it.
 // The six SBP rows next to side 4 (top) or 5 (bottom) of one sweep from an
 // LDS slab (corr: with the acceleration and the supergrid damping).
 int launch_sbp_rows(bool corr, const Box& b, int side, int nk, const SbpTables* d_sbp, const double* upin,

@@ -366,6 +366,167 @@ k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __rest
                                     cupd, beta);
 }
 
+// Predictor sweep (TR: the transposed mode above) whose staged planes reach
+// LDS by asynchronous global-to-LDS loads (global_load_lds_dwordx4: lane l of
+// a wave moves the two doubles at its address to LDS at M0 + 16 l).  The
+// load of plane k+3 is issued right after the barrier of plane k and lands
+// while plane k is computed, with no registers held for it; the column
+// values of the next plane are prefetched into registers alongside.
+// Six ring slots (planes k-2..k+3) and two material buffers leave one
+// barrier per plane.  Every staged pair of doubles must lie in one row and be
+// 16-byte aligned: an even row length ni and 16-byte aligned arrays (the
+// launcher checks both).  Pairs off the box load from the plane's first
+// point instead (a valid address); only inactive points would read them.
+#define SW4_LDS(p) ((__attribute__((address_space(3))) void*)(p))
+template <bool TR>
+__device__ __forceinline__ void sweep_dma_body(unsigned bid, Box b, TileMap map, int kbeg, int kend, int kc,
+                                               const double* __restrict__ u, const double* __restrict__ um,
+                                               const double* __restrict__ mu, const double* __restrict__ la,
+                                               const double* __restrict__ rho, SgCoef sg, double* __restrict__ out,
+                                               double cof, double cupd)
+{
+   constexpr int PP = 3 * PL / 2;          // pairs of a staged displacement plane
+   constexpr int NP = (PP + NT - 1) / NT;  // asynchronous loads per thread and plane
+   constexpr int MP = PL / 2;              // pairs of a material plane
+   static_assert(HX % 2 == 0 && MP <= NT, "pairs must tile the rows; one material load per thread");
+   __shared__ __align__(16) double su[6 * 3 * PL];  // plane kp in slot (kp - k0 + 2) % 6
+   __shared__ __align__(16) double sm[2 * PL];      // mu of plane kp in half (kp - k0) & 1
+   __shared__ __align__(16) double sl[2 * PL];
+   __shared__ double ssx[HX], ssy[HY];
+   const int per = (int)((map.ntiles + 7) / 8);
+   const int t = (int)(bid % 8) * per + (int)(bid / 8);
+   if (t >= map.ntiles) return;
+   const int tix = t % map.ntx, tr = t / map.ntx, tjy = tr % map.nty, tkz = tr / map.nty;
+   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave of the workgroup
+   const int ia = b.ib + tix * TX, ja = b.jb + tjy * TY;
+   const int i = ia + 2 + tx, j = ja + 2 + ty;
+   const bool active = i <= b.ie - 2 && j <= b.je - 2;
+   const int k0 = kbeg + tkz * kc;
+   const int k1 = min(kend, k0 + kc - 1);
+   const long ni = b.ni, nij = b.nij, npts = b.npts;
+   const long col = active ? (i - b.ib) + ni * (j - b.jb) : 0;
+   const int o = (ty + 2) * HX + tx + 2;
+
+   // pair m = tid + p NT of the staged (c, jj, ii) tile: offset of its first double
+   long goff[NP];
+#pragma unroll
+   for (int p = 0; p < NP; p++) {
+      const int e = 2 * (tid + p * NT);
+      const int c = e / PL, r = e - c * PL, jj = r / HX, ii = r - jj * HX;
+      const int gi = ia + ii, gj = ja + jj;
+      const bool ok = e < 3 * PL && gi <= b.ie && gj <= b.je;
+      goff[p] = ok ? (gi - b.ib) + ni * (gj - b.jb) + npts * c : 0;
+   }
+   long moff;
+   {
+      const int e = 2 * tid, jj = e / HX, ii = e - jj * HX;
+      const int gi = ia + ii, gj = ja + jj;
+      moff = e < PL && gi <= b.ie && gj <= b.je ? (gi - b.ib) + ni * (gj - b.jb) : 0;
+   }
+   auto dma_plane = [&](int kp, int slot) {
+      const double* src = u + nij * (kp - b.kb);
+      double* dst = su + slot * 3 * PL + 2 * 64 * wv;
+#pragma unroll
+      for (int p = 0; p < NP; p++)
+         if (tid + p * NT < PP) __builtin_amdgcn_global_load_lds(src + goff[p], SW4_LDS(dst + 2 * p * NT), 16, 0, 0);
+   };
+   auto dma_mat = [&](int kp, int half) {
+      const long pb = nij * (kp - b.kb);
+      if (tid < MP) {
+         __builtin_amdgcn_global_load_lds(mu + pb + moff, SW4_LDS(sm + half * PL + 2 * 64 * wv), 16, 0, 0);
+         __builtin_amdgcn_global_load_lds(la + pb + moff, SW4_LDS(sl + half * PL + 2 * 64 * wv), 16, 0, 0);
+      }
+   };
+
+   LdsSten s;
+   if (tid >= 128 && tid < 128 + HX) ssx[tid - 128] = ia + tid - 128 <= b.ie ? sg.strx[ia + tid - 128 - b.ib] : 1.0;
+   if (tid >= 192 && tid < 192 + HY) ssy[tid - 192] = ja + tid - 192 <= b.je ? sg.stry[ja + tid - 192 - b.jb] : 1.0;
+#pragma unroll
+   for (int q = 0; q < 5; q++) dma_plane(k0 - 2 + q, q);
+   dma_mat(k0, 0);
+#pragma unroll
+   for (int q = 0; q < 4; q++) {
+      const long pq = nij * (k0 - 2 + q - b.kb);
+      s.muc[q] = active ? mu[col + pq] : 0.0;
+      s.lac[q] = active ? la[col + pq] : 0.0;
+      s.szv[q] = sg.strz[k0 - 2 + q - b.kb];
+   }
+   s.psx = ssx + tx + 2;
+   s.psy = ssy + ty + 2;
+   // column values of plane k+2, loaded one plane ahead
+   double cmu = mu[col + nij * (k0 + 2 - b.kb)], cla = la[col + nij * (k0 + 2 - b.kb)];
+   double csz = sg.strz[k0 + 2 - b.kb];
+
+   for (int k = k0; k <= k1; k++) {
+      const int sl0 = (k - k0) % 6;  // slot of plane k-2
+      s.muc[4] = active ? cmu : 0.0;
+      s.lac[4] = active ? cla : 0.0;
+      s.szv[4] = csz;
+      __syncthreads();  // plane k+2 and the material of plane k have landed; plane k-1 is done
+      // centre values (um, rho) of plane k: issued first, needed only after
+      // the stencil, so the stencil covers their latency
+      double ctr[3] = {0, 0, 0}, crho = 1.0;
+      if (!TR) {
+         const long p = col + nij * (k - b.kb);  // col = 0 where inactive: a valid address
+#pragma unroll
+         for (int c = 0; c < 3; c++) ctr[c] = um[p + npts * c];
+         crho = rho[p];
+      }
+      if (k < k1) {
+         dma_plane(k + 3, (sl0 + 5) % 6);
+         dma_mat(k + 1, (k + 1 - k0) & 1);
+         const long pq = nij * (k + 3 - b.kb);
+         cmu = mu[col + pq];  // value unused where inactive
+         cla = la[col + pq];
+         csz = sg.strz[k + 3 - b.kb];
+      }
+      if (active) {
+         s.p0 = su + (sl0 * 3) * PL + o;
+         s.p1 = su + (((sl0 + 1) % 6) * 3) * PL + o;
+         s.p2 = su + (((sl0 + 2) % 6) * 3) * PL + o;
+         s.p3 = su + (((sl0 + 3) % 6) * 3) * PL + o;
+         s.p4 = su + (((sl0 + 4) % 6) * 3) * PL + o;
+         s.pm = sm + ((k - k0) & 1) * PL + o;
+         s.pl = sl + ((k - k0) & 1) * PL + o;
+         double r[3];
+         rhs4_interior_body(s, r);
+         const long p = col + nij * (k - b.kb);
+         const double sc = TR ? 1.0 / (s.psx[0] * s.psy[0] * s.szv[2]) : cupd / crho;
+#pragma unroll
+         for (int c = 0; c < 3; c++) {
+            const long q = p + npts * c;
+            if (TR)
+               out[q] = sc * (cof * r[c]);
+            else
+               out[q] = 2 * s.p2[c * PL] - ctr[c] + sc * (cof * r[c]);
+         }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+         s.muc[q] = s.muc[q + 1];
+         s.lac[q] = s.lac[q + 1];
+         s.szv[q] = s.szv[q + 1];
+      }
+   }
+}
+
+template <bool TR>
+__global__ void __launch_bounds__(NT, 2)
+k_sweep_dma(Box b, TileMap map, int kbeg, int kend, int kc, const double* __restrict__ u,
+            const double* __restrict__ um, const double* __restrict__ mu, const double* __restrict__ la,
+            const double* __restrict__ rho, SgCoef sg, double* __restrict__ out, double cof, double cupd)
+{
+   sweep_dma_body<TR>(blockIdx.x, b, map, kbeg, kend, kc, u, um, mu, la, rho, sg, out, cof, cupd);
+}
+
+// The asynchronous loads need whole 16-byte pairs: even row length, aligned arrays.
+static bool dma_fits(const Box& b, const double* u, const double* mu, const double* la)
+{
+   auto al = [](const double* p) { return ((uintptr_t)p & 15) == 0; };
+   return b.ni % 2 == 0 && al(u) && al(mu) && al(la);
+}
+
 // Supergrid damping of the corrector on planes kbeg..kend as a k-marching LDS
 // sweep (ew-cfromfort.C:985-1041): up -= damping(u - um).  d = u - um of plane
 // k and rho of plane k are staged with their (i,j) halo in LDS; the thread's
@@ -728,6 +889,14 @@ int launch_sweep_lds(bool corr, int variant, const Box& b, int k1, int k2, int k
    case 6:  // both sweeps without prefetch at 2 waves/SIMD; damping by launch_damp_lds
       if (corr) SW4_SWEEP(true, false, 2, false); else SW4_SWEEP(false, false, 2, false);
       break;
+   case 7:  // as 6, the predictor's planes by asynchronous global-to-LDS loads where they fit
+      if (corr)
+         SW4_SWEEP(true, false, 2, false);
+      else if (dma_fits(b, u, mu, la))
+         k_sweep_dma<false><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, u, um, mu, la, rho, sg, out, cof, c2);
+      else
+         SW4_SWEEP(false, false, 2, false);
+      break;
    default:  // 1: no prefetch, 2 waves/SIMD
       if (corr) SW4_SWEEP(true, false, 2, false); else SW4_SWEEP(false, false, 2, false);
       break;
@@ -747,8 +916,11 @@ int launch_sweep_lds_T(const Box& b, int k1, int k2, int kc, const double* w, co
    map.nty = (int)cdiv(ny, TY);
    map.ntiles = map.ntx * map.nty * (int)cdiv(k2 - k1 + 1, kc);
    const unsigned grd = 8 * cdiv(map.ntiles, 8);
-   k_sweep_lds<false, false, 2, false, true><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, nullptr, w, nullptr, mu, la, nullptr,
-                                                               sg, out, 1.0 / (h * h), 1.0, 1.0, 0.0);
+   if (dma_fits(b, w, mu, la))
+      k_sweep_dma<true><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, w, nullptr, mu, la, nullptr, sg, out, 1.0 / (h * h), 1.0);
+   else
+      k_sweep_lds<false, false, 2, false, true><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, nullptr, w, nullptr, mu, la,
+                                                                  nullptr, sg, out, 1.0 / (h * h), 1.0, 1.0, 0.0);
    return check_hip(hipGetLastError(), "transposed sweep launch");
 }
 

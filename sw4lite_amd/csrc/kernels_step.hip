@@ -560,7 +560,7 @@ int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables
       int rc = launch_sweep_lds(true, variant, b, k1, k2, kchunk, up, u, um, mu, la, rho, sg, upo, h, dt, beta, s);
       if (rc) return rc;
       if (after_interior && (rc = after_interior())) return rc;
-      if (beta != 0 && variant == 6) {
+      if (beta != 0 && variant >= 6) {
          if ((rc = launch_damp_lds(b, k1, k2, kchunk, upo, u, um, rho, sg, beta, s))) return rc;
       } else if (beta != 0 && variant < 4) {  // variants 4, 5 damp inside the sweep
          dim3 grd(cdiv(nii, 64), cdiv(njj, 4), k2 - k1 + 1);

@@ -234,11 +234,12 @@ class EW:
 
     # set-up -------------------------------------------------------------
     def set_kernel_variant(self, variant, kchunk=64):
-        """0: one thread per point; 1-6: LDS-staged k-marching tiles (1 plain
+        """0: one thread per point; 1-7: LDS-staged k-marching tiles (1 plain
         at 2 waves/SIMD, 2 with register prefetch, 3 prefetch without a
         register cap, 4 as 3 with the corrector's damping in the same sweep, 5
         corrector as 4 and predictor as 1, 6 as 1 with an LDS damping sweep,
-        the default)."""
+        7 as 6 with the predictor's planes loaded asynchronously into LDS
+        where the row length is even, the default)."""
         L.check(self.lib.sw4hip_ew_set_kernel_variant(self.h, int(variant), int(kchunk)))
 
     def set_material(self, rho, mu, la):

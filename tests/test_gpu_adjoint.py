@@ -60,9 +60,11 @@ def dot_rel(a, b):
 
 
 # the tall shapes span several k-blocks of the LDS k-march kernels (32 or 64
-# planes per workgroup) with a partial last block
+# planes per workgroup) with a partial last block; even nx (even row length)
+# takes the transposed sweep that loads its planes asynchronously into LDS
 @pytest.mark.parametrize("os45,shape", [((0, 0), (21, 18, 17)), ((1, 0), (21, 18, 17)), ((0, 1), (21, 18, 17)),
-                                        ((1, 1), (21, 18, 17)), ((1, 0), (37, 11, 75))])
+                                        ((1, 1), (21, 18, 17)), ((1, 0), (37, 11, 75)), ((1, 0), (22, 18, 17)),
+                                        ((1, 1), (36, 19, 17)), ((0, 1), (40, 13, 75))])
 def test_rhs4sg_transpose(torch, coefs, os45, shape):
     from sw4lite_amd import ew
     acof, ghcof, bope, _ = coefs
