@@ -458,12 +458,21 @@ __device__ __forceinline__ void sweep_dma_body(unsigned bid, Box b, TileMap map,
    double cmu = mu[col + nij * (k0 + 2 - b.kb)], cla = la[col + nij * (k0 + 2 - b.kb)];
    double csz = sg.strz[k0 + 2 - b.kb];
 
+   // the results of plane k are stored after the barrier of plane k+1, so
+   // that this barrier (which waits for every outstanding memory operation)
+   // does not wait for stores issued just before it
+   double ov[3] = {0, 0, 0};
+   long op = -1;  // offset of the held results (-1: none)
    for (int k = k0; k <= k1; k++) {
       const int sl0 = (k - k0) % 6;  // slot of plane k-2
       s.muc[4] = active ? cmu : 0.0;
       s.lac[4] = active ? cla : 0.0;
       s.szv[4] = csz;
       __syncthreads();  // plane k+2 and the material of plane k have landed; plane k-1 is done
+      if (op >= 0) {
+#pragma unroll
+         for (int c = 0; c < 3; c++) out[op + npts * c] = ov[c];
+      }
       // centre values (um, rho) of plane k: issued first, needed only after
       // the stencil, so the stencil covers their latency
       double ctr[3] = {0, 0, 0}, crho = 1.0;
@@ -491,16 +500,10 @@ __device__ __forceinline__ void sweep_dma_body(unsigned bid, Box b, TileMap map,
          s.pl = sl + ((k - k0) & 1) * PL + o;
          double r[3];
          rhs4_interior_body(s, r);
-         const long p = col + nij * (k - b.kb);
          const double sc = TR ? 1.0 / (s.psx[0] * s.psy[0] * s.szv[2]) : cupd / crho;
 #pragma unroll
-         for (int c = 0; c < 3; c++) {
-            const long q = p + npts * c;
-            if (TR)
-               out[q] = sc * (cof * r[c]);
-            else
-               out[q] = 2 * s.p2[c * PL] - ctr[c] + sc * (cof * r[c]);
-         }
+         for (int c = 0; c < 3; c++) ov[c] = TR ? sc * (cof * r[c]) : 2 * s.p2[c * PL] - ctr[c] + sc * (cof * r[c]);
+         op = col + nij * (k - b.kb);
       }
 #pragma unroll
       for (int q = 0; q < 4; q++) {
@@ -508,6 +511,10 @@ __device__ __forceinline__ void sweep_dma_body(unsigned bid, Box b, TileMap map,
          s.lac[q] = s.lac[q + 1];
          s.szv[q] = s.szv[q + 1];
       }
+   }
+   if (op >= 0) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) out[op + npts * c] = ov[c];
    }
 }
 
@@ -579,6 +586,11 @@ __device__ __forceinline__ void damp_lds_body(unsigned bid, Box b, TileMap map, 
    }
    for (int k = k0; k <= k1; k++) {
       const long pb = nij * (k - b.kb);
+      // this point's values of up, updated below: loaded with the plane so
+      // that their latency overlaps the staging loads'
+      double upc[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) upc[c] = up[col + pb + npts * c];  // col = 0 where inactive: a valid address
 #pragma unroll
       for (int p = 0; p < NU; p++) {
          const int e = tid + p * NT;
@@ -629,7 +641,7 @@ __device__ __forceinline__ void damp_lds_body(unsigned bid, Box b, TileMap map, 
             const double az = rzp * (dq[4][c] - 2 * dq[3][c] + dq[2][c]) -
                               2 * rz0 * (dq[3][c] - 2 * dq[2][c] + dq[1][c]) +
                               rzm * (dq[2][c] - 2 * dq[1][c] + dq[0][c]);
-            up[col + pb + npts * c] -= birho * (wx * ax + wy * ay + wz * az);
+            up[col + pb + npts * c] = upc[c] - birho * (wx * ax + wy * ay + wz * az);
          }
       }
       __syncthreads();
@@ -641,8 +653,172 @@ __device__ __forceinline__ void damp_lds_body(unsigned bid, Box b, TileMap map, 
    }
 }
 
+// The damping sweep with its planes brought to LDS by asynchronous
+// global-to-LDS loads (as sweep_dma_body): raw u, um and rho of plane k+1
+// land while plane k is computed, d = u - um is formed from LDS; the column
+// values (d at k+3, rho at k+2) and this point's up of plane k+1 are loaded
+// one plane ahead into registers, and the results of plane k are stored
+// after the next barrier.  Two slots of 7 staged values per point (48 KB):
+// three workgroups per CU.  Same conditions as sweep_dma_body (even ni,
+// 16-byte aligned arrays).
+__device__ __forceinline__ void damp_dma_body(unsigned bid, Box b, TileMap map, int kbeg, int kend, int kc,
+                                              double* __restrict__ up, const double* __restrict__ u,
+                                              const double* __restrict__ um, const double* __restrict__ rho,
+                                              SgCoef sg, double beta)
+{
+   constexpr int PP = 3 * PL / 2, NP = (PP + NT - 1) / NT, MP = PL / 2;
+   static_assert(HX % 2 == 0 && MP <= NT, "pairs must tile the rows; one rho load per thread");
+   __shared__ __align__(16) double su[2 * 3 * PL];  // u of plane kp in half (kp - k0) & 1
+   __shared__ __align__(16) double sq[2 * 3 * PL];  // um
+   __shared__ __align__(16) double sr[2 * PL];      // rho
+   const int per = (int)((map.ntiles + 7) / 8);
+   const int t = (int)(bid % 8) * per + (int)(bid / 8);
+   if (t >= map.ntiles) return;
+   const int tix = t % map.ntx, tr = t / map.ntx, tjy = tr % map.nty, tkz = tr / map.nty;
+   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+   const int ia = b.ib + tix * TX, ja = b.jb + tjy * TY;
+   const int i = ia + 2 + tx, j = ja + 2 + ty;
+   const bool active = i <= b.ie - 2 && j <= b.je - 2;
+   const int k0 = kbeg + tkz * kc;
+   const int k1 = min(kend, k0 + kc - 1);
+   const long ni = b.ni, nij = b.nij, npts = b.npts;
+   const long col = active ? (i - b.ib) + ni * (j - b.jb) : 0;
+   const int o = (ty + 2) * HX + tx + 2;
+   const int ii = i - b.ib, jj = j - b.jb;
+   long goff[NP];
+#pragma unroll
+   for (int p = 0; p < NP; p++) {
+      const int e = 2 * (tid + p * NT);
+      const int c = e / PL, r = e - c * PL, y = r / HX, x = r - y * HX;
+      const int gi = ia + x, gj = ja + y;
+      const bool ok = e < 3 * PL && gi <= b.ie && gj <= b.je;
+      goff[p] = ok ? (gi - b.ib) + ni * (gj - b.jb) + npts * c : 0;
+   }
+   long roff;
+   {
+      const int e = 2 * tid, y = e / HX, x = e - y * HX;
+      const int gi = ia + x, gj = ja + y;
+      roff = e < PL && gi <= b.ie && gj <= b.je ? (gi - b.ib) + ni * (gj - b.jb) : 0;
+   }
+   auto dma_plane = [&](int kp, int half) {
+      const long pb = nij * (kp - b.kb);
+      double* du = su + half * 3 * PL + 2 * 64 * wv;
+      double* dq = sq + half * 3 * PL + 2 * 64 * wv;
+#pragma unroll
+      for (int p = 0; p < NP; p++)
+         if (tid + p * NT < PP) {
+            __builtin_amdgcn_global_load_lds(u + pb + goff[p], SW4_LDS(du + 2 * p * NT), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(um + pb + goff[p], SW4_LDS(dq + 2 * p * NT), 16, 0, 0);
+         }
+      if (tid < MP) __builtin_amdgcn_global_load_lds(rho + pb + roff, SW4_LDS(sr + half * PL + 2 * 64 * wv), 16, 0, 0);
+   };
+   // own column: d at k-2..k+2, rho at k-1 and k+1 (0 where inactive)
+   double dq[5][3];
+#pragma unroll
+   for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+         const long g = col + nij * (k0 - 2 + q - b.kb) + npts * c;
+         const double a = u[g], e = um[g];
+         dq[q][c] = active ? a - e : 0.0;
+      }
+   double rkm = active ? rho[col + nij * (k0 - 1 - b.kb)] : 0.0;
+   double gx[5] = {0, 0, 0, 0, 0}, gy[5] = {0, 0, 0, 0, 0};
+   if (active) {
+      gx[0] = sg.dcx[ii - 1]; gx[1] = sg.dcx[ii]; gx[2] = sg.dcx[ii + 1]; gx[3] = sg.strx[ii]; gx[4] = sg.cox[ii];
+      gy[0] = sg.dcy[jj - 1]; gy[1] = sg.dcy[jj]; gy[2] = sg.dcy[jj + 1]; gy[3] = sg.stry[jj]; gy[4] = sg.coy[jj];
+   }
+   dma_plane(k0, 0);
+   // one plane ahead: u, um at k+2 (column), rho at k+1, up at k (this point)
+   double nu[3], nm[3], nup[3], nrk;
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      nu[c] = u[col + nij * (k0 + 2 - b.kb) + npts * c];  // col = 0 where inactive: a valid address
+      nm[c] = um[col + nij * (k0 + 2 - b.kb) + npts * c];
+      nup[c] = up[col + nij * (k0 - b.kb) + npts * c];
+   }
+   nrk = rho[col + nij * (k0 + 1 - b.kb)];
+   double ov[3] = {0, 0, 0};
+   long op = -1;  // offset of the held results (-1: none)
+   for (int k = k0; k <= k1; k++) {
+      const int h = (k - k0) & 1;
+      __syncthreads();  // plane k has landed; plane k-1 is done
+      if (op >= 0) {
+#pragma unroll
+         for (int c = 0; c < 3; c++) up[op + npts * c] = ov[c];
+      }
+      double upc[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+         dq[4][c] = active ? nu[c] - nm[c] : 0.0;
+         upc[c] = nup[c];
+      }
+      const double rkp = active ? nrk : 0.0;
+      if (k < k1) {
+         dma_plane(k + 1, h ^ 1);
+#pragma unroll
+         for (int c = 0; c < 3; c++) {
+            nu[c] = u[col + nij * (k + 3 - b.kb) + npts * c];
+            nm[c] = um[col + nij * (k + 3 - b.kb) + npts * c];
+            nup[c] = up[col + nij * (k + 1 - b.kb) + npts * c];
+         }
+         nrk = rho[col + nij * (k + 2 - b.kb)];
+      }
+      const int kz = k - b.kb;
+      const double gz0 = sg.dcz[kz - 1], gz1 = sg.dcz[kz], gz2 = sg.dcz[kz + 1], gz3 = sg.strz[kz], gz4 = sg.coz[kz];
+      const double* pr = sr + h * PL + o;
+      double r0 = 0;
+      if (active) {
+         r0 = pr[0];
+         const double birho = beta / r0;
+         const double rxp = pr[1] * gx[2], rx0 = r0 * gx[1], rxm = pr[-1] * gx[0];
+         const double ryp = pr[HX] * gy[2], ry0 = r0 * gy[1], rym = pr[-HX] * gy[0];
+         const double rzp = rkp * gz2, rz0 = r0 * gz1, rzm = rkm * gz0;
+         const double wx = gx[3] * gy[4] * gz4, wy = gy[3] * gx[4] * gz4, wz = gz3 * gx[4] * gy[4];
+#pragma unroll
+         for (int c = 0; c < 3; c++) {
+            const double* pu = su + (h * 3 + c) * PL + o;
+            const double* pm = sq + (h * 3 + c) * PL + o;
+            double q[9];  // d at x-2..x+2 (q[0..4]), y-2, y-1, y+1, y+2 (q[5..8])
+#pragma unroll
+            for (int d = -2; d <= 2; d++) q[d + 2] = pu[d] - pm[d];
+            q[5] = pu[-2 * HX] - pm[-2 * HX];
+            q[6] = pu[-HX] - pm[-HX];
+            q[7] = pu[HX] - pm[HX];
+            q[8] = pu[2 * HX] - pm[2 * HX];
+            const double ax = rxp * (q[4] - 2 * q[3] + q[2]) - 2 * rx0 * (q[3] - 2 * q[2] + q[1]) +
+                              rxm * (q[2] - 2 * q[1] + q[0]);
+            const double ay = ryp * (q[8] - 2 * q[7] + q[2]) - 2 * ry0 * (q[7] - 2 * q[2] + q[6]) +
+                              rym * (q[2] - 2 * q[6] + q[5]);
+            const double az = rzp * (dq[4][c] - 2 * dq[3][c] + dq[2][c]) -
+                              2 * rz0 * (dq[3][c] - 2 * dq[2][c] + dq[1][c]) +
+                              rzm * (dq[2][c] - 2 * dq[1][c] + dq[0][c]);
+            ov[c] = upc[c] - birho * (wx * ax + wy * ay + wz * az);
+         }
+         op = col + nij * (k - b.kb);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+         for (int c = 0; c < 3; c++) dq[q][c] = dq[q + 1][c];
+      rkm = r0;
+   }
+   if (op >= 0) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) up[op + npts * c] = ov[c];
+   }
+}
+
+__global__ void __launch_bounds__(NT, 3)
+k_damp_dma(Box b, TileMap map, int kbeg, int kend, int kc, double* __restrict__ up, const double* __restrict__ u,
+           const double* __restrict__ um, const double* __restrict__ rho, SgCoef sg, double beta)
+{
+   damp_dma_body(blockIdx.x, b, map, kbeg, kend, kc, up, u, um, rho, sg, beta);
+}
+
 template <int DTY>
-__global__ void __launch_bounds__(32 * DTY)
+__global__ void __launch_bounds__(32 * DTY, 4)
 k_damp_lds(Box b, TileMap map, int kbeg, int kend, int kc, double* __restrict__ up, const double* __restrict__ u,
            const double* __restrict__ um, const double* __restrict__ rho, SgCoef sg, double beta)
 {
@@ -727,7 +903,11 @@ int launch_damp_lds(const Box& b, int k1, int k2, int kc, double* up, const doub
    map.ntx = (int)cdiv(nx, TX);
    map.nty = (int)cdiv(ny, DTY);
    map.ntiles = map.ntx * map.nty * (int)cdiv(k2 - k1 + 1, kc);
-   k_damp_lds<DTY><<<8 * cdiv(map.ntiles, 8), 32 * DTY, 0, s>>>(b, map, k1, k2, kc, up, u, um, rho, sg, beta);
+   static_assert(DTY == TY, "the asynchronous-load damping sweep uses the sweeps' tile");
+   if (dma_fits(b, u, um, rho) && ((uintptr_t)up & 15) == 0)
+      k_damp_dma<<<8 * cdiv(map.ntiles, 8), NT, 0, s>>>(b, map, k1, k2, kc, up, u, um, rho, sg, beta);
+   else
+      k_damp_lds<DTY><<<8 * cdiv(map.ntiles, 8), 32 * DTY, 0, s>>>(b, map, k1, k2, kc, up, u, um, rho, sg, beta);
    return check_hip(hipGetLastError(), "damping sweep launch");
 }
 
