@@ -548,19 +548,24 @@ __global__ void __launch_bounds__(GNT, 2) k_rhs4sg_matgrad_march(AdjCtx A, int n
    const long col = (long)(jc - b.jb) * b.ni + (ic - b.ib);
    const bool crow = active && i >= 1 && i <= m.nx && j >= 1 && j <= m.ny;
    double uc[5][3], zc[5][3], szr[5];
-   auto column = [&](int q, int kk) {
+   auto column = [&](int kk, double* uq, double* zq, double& sq) {
       const long pk = (long)(min(max(kk, b.kb), b.ke) - b.kb) * b.nij + col;
       const bool uin = active && kk >= b.kb && kk <= b.ke, zin = crow && kk >= m.k1 && kk <= m.k2;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
          const double uv = u[pk + n * c], zv = z[pk + n * c];
-         uc[q][c] = uin ? uv : 0.0;
-         zc[q][c] = zin ? zv : 0.0;
+         uq[c] = uin ? uv : 0.0;
+         zq[c] = zin ? zv : 0.0;
       }
-      szr[q] = A.sz(kk);
+      sq = A.sz(kk);
    };
 #pragma unroll
-   for (int q = 0; q < 4; q++) column(q, k0 - 2 + q);
+   for (int q = 0; q < 4; q++) column(k0 - 2 + q, uc[q], zc[q], szr[q]);
+   // one plane ahead: the column values at k+2 and gmu, gla at this point,
+   // so that their latency overlaps the plane before
+   double nu[3], nz[3], nsz;
+   column(k0 + 2, nu, nz, nsz);
+   double ngm = gmu[(long)(k0 - b.kb) * b.nij + col], ngl = gla[(long)(k0 - b.kb) * b.nij + col];
    double sxo[5], syo[5];
 #pragma unroll
    for (int e = 0; e < 5; e++) {
@@ -573,8 +578,20 @@ __global__ void __launch_bounds__(GNT, 2) k_rhs4sg_matgrad_march(AdjCtx A, int n
    const int ctr = (ty + 2) * GHX + tx + 2;
    for (int k = k0; k <= k1; k++) {
       const int cur = (k - k0) & 1;
-      column(4, k + 2);
-      if (k < k1) fetch(k + 1);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+         uc[4][c] = nu[c];
+         zc[4][c] = nz[c];
+      }
+      szr[4] = nsz;
+      const double gmc = ngm, glc = ngl;
+      if (k < k1) {
+         fetch(k + 1);
+         column(k + 3, nu, nz, nsz);
+         const long pn = (long)(k + 1 - b.kb) * b.nij + col;  // col: a point of the box also where inactive
+         ngm = gmu[pn];
+         ngl = gla[pn];
+      }
       if (active) {
          const double* U = su[cur];
          const double* Zs = sz[cur];
@@ -622,8 +639,8 @@ __global__ void __launch_bounds__(GNT, 2) k_rhs4sg_matgrad_march(AdjCtx A, int n
          const double mm = s0 * (D[0][1] * V[0][1] + D[0][2] * V[0][2]) + s1 * (D[1][0] * V[1][0] + D[1][2] * V[1][2]) +
                            s2i * (D[2][0] * V[2][0] + D[2][1] * V[2][1]);
          const long p = (long)(k - b.kb) * b.nij + col;
-         gmu[p] += cof * ((gs + gd) * (1.0 / 6) + mm * (1.0 / 144));
-         gla[p] += cof * (gd * (1.0 / 6) + ml * (1.0 / 144));
+         gmu[p] = gmc + cof * ((gs + gd) * (1.0 / 6) + mm * (1.0 / 144));
+         gla[p] = glc + cof * (gd * (1.0 / 6) + ml * (1.0 / 144));
       }
       if (k < k1) store(cur ^ 1);
       __syncthreads();
@@ -749,33 +766,40 @@ __global__ void __launch_bounds__(GNT, 2) k_addsgd4_T_march(DampCtx D, int ntx, 
    const long col = (long)(jc - b.jb) * b.ni + (ic - b.ib);
    const bool cint = active && i >= 1 && i <= D.nx && j >= 1 && j <= D.ny;
    double gc[5][3], dc[3][3], rc[3], szr[5], dczr[3];
-   auto gcolumn = [&](int q, int kk) {
+   auto gcolumn = [&](int kk, double* g, double& sz) {
       const long a = (long)(min(max(kk, b.kb), b.ke) - b.kb) * b.nij + col;
       const bool gin = cint && kk >= 1 && kk <= D.nz;
       const double br = beta / rho[a];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
          const double yv = y[a + n * c];
-         gc[q][c] = gin ? br * yv : 0.0;
+         g[c] = gin ? br * yv : 0.0;
       }
-      szr[q] = kk >= b.kb && kk <= b.ke ? sg.strz[kk - b.kb] : 1.0;
+      sz = kk >= b.kb && kk <= b.ke ? sg.strz[kk - b.kb] : 1.0;
    };
-   auto dcolumn = [&](int q, int kk) {
+   auto dcolumn = [&](int kk, double* d, double& r, double& dz) {
       const long a = (long)(min(max(kk, b.kb), b.ke) - b.kb) * b.nij + col;
       const bool in = active && kk >= b.kb && kk <= b.ke;
-      const double r = rho[a];
-      rc[q] = in ? r : 0.0;
+      const double rv = rho[a];
+      r = in ? rv : 0.0;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
          const double uv = u[a + n * c], mv = um[a + n * c];
-         dc[q][c] = in ? uv - mv : 0.0;
+         d[c] = in ? uv - mv : 0.0;
       }
-      dczr[q] = sg.dcz[min(max(kk, b.kb), b.ke) - b.kb];
+      dz = sg.dcz[min(max(kk, b.kb), b.ke) - b.kb];
    };
 #pragma unroll
-   for (int q = 0; q < 4; q++) gcolumn(q, k0 - 2 + q);
+   for (int q = 0; q < 4; q++) gcolumn(k0 - 2 + q, gc[q], szr[q]);
 #pragma unroll
-   for (int q = 0; q < 2; q++) dcolumn(q, k0 - 1 + q);
+   for (int q = 0; q < 2; q++) dcolumn(k0 - 1 + q, dc[q], rc[q], dczr[q]);
+   // one plane ahead: the column values the next plane adds (g at k+2, d and
+   // rho at k+1) and grho at this point, so that their latency overlaps
+   // the plane before
+   double ng[3], nsz, nd[3], nr, ndz;
+   gcolumn(k0 + 2, ng, nsz);
+   dcolumn(k0 + 1, nd, nr, ndz);
+   double ngr = grho[(long)(k0 - b.kb) * b.nij + col];  // col: a point of the box also where inactive
    const int ii = ic - b.ib, jj = jc - b.jb, ni = b.ie - b.ib, nj = b.je - b.jb;
    double sxo[5], syo[5];
 #pragma unroll
@@ -793,9 +817,21 @@ __global__ void __launch_bounds__(GNT, 2) k_addsgd4_T_march(DampCtx D, int ntx, 
    const int ctr = (ty + 2) * GHX + tx + 2;
    for (int k = k0; k <= k1; k++) {
       const int cur = (k - k0) & 1;
-      gcolumn(4, k + 2);
-      dcolumn(2, k + 1);
-      if (k < k1) fetch(k + 1);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+         gc[4][c] = ng[c];
+         dc[2][c] = nd[c];
+      }
+      szr[4] = nsz;
+      rc[2] = nr;
+      dczr[2] = ndz;
+      const double grc = ngr;
+      if (k < k1) {
+         fetch(k + 1);
+         gcolumn(k + 3, ng, nsz);
+         dcolumn(k + 2, nd, nr, ndz);
+         ngr = grho[(long)(k + 1 - b.kb) * b.nij + col];
+      }
       if (active) {
          const double* G = sgl[cur];
          const double* Dd = sdl[cur];
@@ -831,7 +867,7 @@ __global__ void __launch_bounds__(GNT, 2) k_addsgd4_T_march(DampCtx D, int ntx, 
             const double dzz = dc[2][c] - 2 * dc[1][c] + dc[0][c];
             gr -= dcx[1] * dxx * wx * ax[1] + dcy[1] * dyy * wy * ay[1] + dczr[1] * dzz * wz * az[1];
          }
-         grho[p] += gr;
+         grho[p] = grc + gr;
       }
       if (k < k1) store(cur ^ 1);
       __syncthreads();

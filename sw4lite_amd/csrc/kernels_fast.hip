@@ -845,7 +845,7 @@ __global__ void __launch_bounds__(NT, 2) k_sweep_batch(const SweepShot* __restri
                                       h.la, h.rho, h.sg, h.out, h.cof, h.dt2i, h.cupd, h.beta);
 }
 
-__global__ void __launch_bounds__(32 * 8) k_damp_batch(const SweepShot* __restrict__ shots, int nshots)
+__global__ void __launch_bounds__(32 * 8, 4) k_damp_batch(const SweepShot* __restrict__ shots, int nshots)
 {
    const SweepShot& h = shots[find_shot(shots, nshots, blockIdx.x)];
    damp_lds_body<8>(blockIdx.x - h.grid0, h.b, h.map, h.kbeg, h.kend, h.kc, h.out, h.u, h.um, h.rho, h.sg, h.beta);
