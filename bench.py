@@ -40,6 +40,10 @@ METRIC = "grid-pt updates/s per RK step (FP64 vs FP32-comp, %HBM roofline); adjo
 HBM_PEAK_GBS = 8000.0
 CORR_BYTES_PER_PT = 15 * 8     # corrector sweep: Up, U, Um, mu, lambda, rho in, Up' out (FP64)
 PRED_BYTES_PER_PT = 12 * 8     # predictor sweep, FP64
+# kernel variant 8 (even row length): the predictor also writes the
+# acceleration A, the corrector reads A, Up, mu, lambda, rho and writes Up'
+PRED_ACC_BYTES_PER_PT = 15 * 8
+CORR_ACC_BYTES_PER_PT = 12 * 8
 STEP_BYTES_PER_PT = 27 * 8     # whole step, compulsory traffic
 # FP32 form with the compensated update (float words): the predictor reads U,
 # mu, lambda, rho and writes A; the corrector (damping included) reads A, D,
@@ -61,10 +65,11 @@ def parse():
     p.add_argument("--no-fp32", action="store_true", help="skip the FP32 compensated form")
     p.add_argument("--cpu-nz", type=int, default=128, help="depth of the CPU-baseline slab")
     p.add_argument("--cpu-steps", type=int, default=3)
-    p.add_argument("--variant", type=int, default=7, help="interior sweep kernels: 0 thread/point, 1 LDS k-march, 2 with prefetch, "
+    p.add_argument("--variant", type=int, default=8, help="interior sweep kernels: 0 thread/point, 1 LDS k-march, 2 with prefetch, "
                         "3 prefetch without register cap, 4 as 3 with the damping in the corrector sweep, "
                         "5 corrector as 4 and predictor as 1, 6 as 1 with an LDS damping sweep, "
-                        "7 as 6 with the predictor planes loaded asynchronously into LDS (even row lengths)")
+                        "7 as 6 with the predictor planes loaded asynchronously into LDS (even row lengths), "
+                        "8 as 7 with the acceleration stored by the predictor and staged by the corrector")
     p.add_argument("--kchunk", type=int, default=64, help="planes per workgroup of the LDS sweeps")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also launch a plain device copy of 1 GiB (PMC scaling)")
@@ -668,15 +673,28 @@ def main():
     inf = e.info()
     pts = float(n) ** 3
     value = pts * a.steps / wall
-    # dominant kernel: corrector sweep over the interior planes k = 7..n
+    # dominant kernel: the slower of the predictor and corrector sweeps over
+    # the interior planes k = 7..n
     corr_pts = float(n) * n * (n - 6)
     corr_ms = stage_ms[3] / a.steps
-    achieved = corr_pts * CORR_BYTES_PER_PT / (corr_ms * 1e-3) / 1e9
     pred_ms = stage_ms[0] / a.steps
+    stored_acc = a.variant == 8 and (n + 4) % 2 == 0
+    if stored_acc:
+        kern = {"pred": ("k_sweep_dma<0>", PRED_ACC_BYTES_PER_PT), "corr": ("k_sweep_dma<2>", CORR_ACC_BYTES_PER_PT)}
+    elif a.variant == 7 and (n + 4) % 2 == 0:
+        kern = {"pred": ("k_sweep_dma<0>", PRED_BYTES_PER_PT), "corr": ("k_sweep_lds<true,...>", CORR_BYTES_PER_PT)}
+    elif a.variant >= 1:
+        kern = {"pred": ("k_sweep_lds<false,...>", PRED_BYTES_PER_PT), "corr": ("k_sweep_lds<true,...>", CORR_BYTES_PER_PT)}
+    else:
+        kern = {"pred": ("k_pred_interior", PRED_BYTES_PER_PT), "corr": ("k_corr_interior", CORR_BYTES_PER_PT)}
+    dom = "pred" if pred_ms > corr_ms else "corr"
+    dom_kernel, dom_bpp = kern[dom]
+    dom_ms = pred_ms if dom == "pred" else corr_ms
+    achieved = corr_pts * dom_bpp / (dom_ms * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_corrector_512.json")
+    tf = os.path.join(ROOT, "profiles", "traffic_sweeps_512.json")
     if os.path.exists(tf) and n == 512:
-        traffic = json.load(open(tf)).get("bytes_per_launch")
+        traffic = json.load(open(tf)).get("bytes_per_launch", {}).get(dom_kernel)
     line = {
         "metric": METRIC,
         "value": value,
@@ -696,13 +714,15 @@ def main():
             "grid": [n, n, n], "h": h, "dt": inf.dt, "kernel_variant": a.variant, "kchunk": a.kchunk, "sources": len(srcs), "grid_point_sources": inf.nsources,
             "step_hbm_fraction": value * STEP_BYTES_PER_PT / (HBM_PEAK_GBS * 1e9),
             "stage_ms_per_step": {k: v / a.steps for k, v in zip(STAGES, stage_ms.tolist())},
-            "pred_interior_GBps": corr_pts * PRED_BYTES_PER_PT / (pred_ms * 1e-3) / 1e9,
+            "pred_interior_GBps": corr_pts * kern["pred"][1] / (pred_ms * 1e-3) / 1e9,
+            "corr_interior_GBps": corr_pts * kern["corr"][1] / (corr_ms * 1e-3) / 1e9,
+            "stored_acceleration": stored_acc,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "k_sweep_lds<true,...>" if a.variant >= 1 else "k_corr_interior", "bytes_per_launch": corr_pts * CORR_BYTES_PER_PT,
-            "avg_launch_ms": corr_ms,
+            "kernel": dom_kernel, "bytes_per_launch": corr_pts * dom_bpp,
+            "avg_launch_ms": dom_ms,
         },
         "cpu_baseline": None,
     }

@@ -234,10 +234,13 @@ int sw4hip_ew_set_stream(sw4hip_ew* ew, void* stream);
  * (default 64): 1 plain at 2 waves/SIMD, 2 register prefetch of the next
  * plane, 3 prefetch without a register cap, 4 as 3 with the supergrid
  * damping inside the corrector sweep, 5 corrector as 4 and predictor as 1,
- * 6 as 1 with the damping as its own LDS sweep, 7 (the default) as 6 with
- * the predictor's planes brought to LDS by asynchronous loads one plane
- * ahead (even row lengths; otherwise as 6).  All give the same results to
- * rounding. */
+ * 6 as 1 with the damping as its own LDS sweep, 7 as 6 with the
+ * predictor's planes brought to LDS by asynchronous loads one plane ahead
+ * (even row lengths; otherwise as 6), 8 (the default) as 7 with the
+ * acceleration (Up - 2U + Um)/dt^2 stored by the predictor (and its ghost
+ * shell) and brought to LDS by the corrector the same way (one more field of
+ * device memory, allocated at the first step; even row lengths, otherwise as
+ * 7; the same bits as 7).  All give the same results to rounding. */
 int sw4hip_ew_set_kernel_variant(sw4hip_ew* ew, int variant, int kchunk);
 /* rho, mu, lambda over the whole box incl. ghosts; on_device selects whether
  * the pointers are device (1) or host (0) arrays. */

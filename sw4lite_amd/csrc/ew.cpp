@@ -2,6 +2,7 @@
 // loop on the GPU.  Stage order and formulas follow EW::timesteploop
 // (EW.C:2339-2928) and the EW step methods (EW.C:3085-3636).
 #include "ew.h"
+#include "kernels_fast.h"
 
 #include <cmath>
 #include <cstring>
@@ -68,6 +69,26 @@ int EW::init()
       return set_error(E_NOMEM, "EW: acceleration slab allocation failed");
    m_allocs.push_back(p);
    m_dacc = static_cast<double*>(p);
+   return OK;
+}
+
+// The whole acceleration for kernel variant 8 (3 npts doubles, allocated at
+// the first step that uses it); null for the other variants and for odd row
+// lengths, whose sweeps cannot stage it.
+int EW::accel_field(double** p)
+{
+   *p = nullptr;
+   if (m_variant != 8 || m_box.ni % 2 != 0) return OK;
+   if (!m_accf) {
+      void* q = nullptr;
+      if (hipMalloc(&q, sizeof(double) * 3 * m_box.npts) != hipSuccess) {
+         (void)hipGetLastError();
+         return set_error(E_NOMEM, "EW: acceleration field allocation failed");
+      }
+      m_allocs.push_back(q);
+      m_accf = static_cast<double*>(q);
+   }
+   *p = m_accf;
    return OK;
 }
 
@@ -468,17 +489,29 @@ int EW::step_fused(int n, double* stage_ms)
       return stage_ms ? check_hip(hipEventRecord(ev[(size_t)s * (kStages + 1) + q], m_stream), "hipEventRecord") : OK;
    };
    const int zf = zfill();
+   double* accf = nullptr;  // variant 8: the acceleration stored by the predictor, staged by the corrector
+   if (rc == OK) rc = accel_field(&accf);
+   int ka = 0, kz = 0;
+   fused_planes(m_box, g.nz, m_onesided, &ka, &kz);
+   // completes the stored acceleration from the final Up (after sources and BC)
+   auto accel_rest = [&](double dt) -> int {
+      if (!accf) return OK;
+      // the shell's values of the side frame are final when every side is zero-filled
+      const bool frame = (zf & 15) != 15;
+      const int r = launch_accel_fix(m_box, ka, kz, frame, mUp.dev, mU.dev, mUm.dev, dt, accf, m_stream);
+      return r ? r : launch_accel_points(m_box, st, mUp.dev, mU.dev, mUm.dev, dt, accf, m_stream);
+   };
    for (int s = 0; s < n && rc == OK; s++) {
       const double t = mT, dt = mDt;
       if ((rc = mark(s, 0)) ||
           (rc = launch_pred_fused(m_box, g.nz, m_onesided, m_dsbp, mU.dev, mUm.dev, mMu.dev, mLambda.dev, mRho.dev,
                                   c, mUp.dev, g.h, dt, m_stream, [&]() { return mark(s, 1); }, m_variant, m_kchunk,
-                                  zf)) ||
+                                  zf, accf)) ||
           (rc = mark(s, 2)) || (rc = launch_src_apply(m_box, st, t, 0, dt * dt, mRho.dev, mUp.dev, m_stream)) ||
-          (rc = enforceBC(mUp, mMu, mLambda, t + dt, zf)) || (rc = mark(s, 3)) ||
+          (rc = enforceBC(mUp, mMu, mLambda, t + dt, zf)) || (rc = accel_rest(dt)) || (rc = mark(s, 3)) ||
           (rc = launch_corr_fused(m_box, g.nz, m_onesided, m_dsbp, mUp.dev, mU.dev, mUm.dev, mMu.dev, mLambda.dev,
                                   mRho.dev, c, m_opt.beta, mUp2.dev, m_dacc, g.h, dt, m_stream, [&]() { return mark(s, 4); },
-                                  [&]() { return mark(s, 5); }, m_variant, m_kchunk, zf)) ||
+                                  [&]() { return mark(s, 5); }, m_variant, m_kchunk, zf, accf)) ||
           (rc = mark(s, 6)) ||
           (rc = launch_src_apply(m_box, st, t, 1, dt * dt * dt * dt / 12, mRho.dev, mUp2.dev, m_stream)) ||
           (rc = enforceBC(mUp2, mMu, mLambda, t + dt, zf)) || (rc = mark(s, 7)) || (rc = record(mUp2, mUm)))

@@ -125,6 +125,8 @@ public:
    Sarray mMetric, mJ;     // curvilinear grid only
    std::vector<double> m_zhost;
    double* m_dacc = nullptr;  // acceleration on the planes the corrector's SBP rows read
+   double* m_accf = nullptr;  // the whole acceleration (kernel variant 8), allocated at its first step
+   int accel_field(double** p);
    double mDt = 0, mT = 0;
    int mNumberOfTimeSteps = 0, mCycle = 0;
    int m_onesided[6] = {0, 0, 0, 0, 0, 0};
@@ -137,7 +139,7 @@ public:
    std::vector<GridPointSource> m_gps;
    std::vector<int> m_first;
    hipStream_t m_stream = nullptr;
-   int m_variant = 7;   // interior sweep kernels: 0 thread per point, 1-7 LDS-staged k-march
+   int m_variant = 8;   // interior sweep kernels: 0 thread per point, 1-8 LDS-staged k-march
    int m_kchunk = 64;   // planes per workgroup of the LDS sweeps
 
 private:

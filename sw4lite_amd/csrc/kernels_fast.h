@@ -41,7 +41,8 @@ int launch_sweep_batch(int kind, const SweepShot* d_shots, int nshots, int nbloc
 // plane, 2 waves/SIMD; 3: prefetch, 1 wave/SIMD; 4: as 3 with the corrector's
 // supergrid damping (beta != 0) applied in the same sweep; 5: corrector as 4,
 // predictor as 1; 6: both as 1, the damping by launch_damp_lds; 7: as 6, the
-// predictor by asynchronous global-to-LDS loads where the rows allow it.
+// predictor by asynchronous global-to-LDS loads where the rows allow it; 8:
+// as 7, with acc the acceleration stored and staged (below).
 // The six SBP rows next to side 4 (top) or 5 (bottom) of one sweep from an
 // LDS slab (corr: with the acceleration and the supergrid damping).
 int launch_sbp_rows(bool corr, const Box& b, int side, int nk, const SbpTables* d_sbp, const double* upin,
@@ -50,9 +51,22 @@ int launch_sbp_rows(bool corr, const Box& b, int side, int nk, const SbpTables* 
 // up -= supergrid damping(u, um) on planes k1..k2, k-marching LDS sweep.
 int launch_damp_lds(const Box& b, int k1, int k2, int kc, double* up, const double* u, const double* um,
                     const double* rho, const SgCoef& sg, double beta, hipStream_t s);
+// Variant 8 with acc (3 npts doubles, even row length, aligned arrays): the
+// predictor also stores the acceleration (up - 2u + um)/dt^2 at the points
+// it computes, the corrector stages acc instead of forming it; between the
+// two, launch_accel_fix and launch_accel_points complete acc from the final
+// Up.  Without acc, variant 8 is variant 7.
 int launch_sweep_lds(bool corr, int variant, const Box& b, int k1, int k2, int kc, const double* upin,
                      const double* u, const double* um, const double* mu, const double* la, const double* rho,
-                     const SgCoef& sg, double* out, double h, double dt, double beta, hipStream_t s);
+                     const SgCoef& sg, double* out, double h, double dt, double beta, hipStream_t s,
+                     double* acc = nullptr);
+// acc = (up - 2u + um)/dt^2 on the planes k1-2, k1-1, k2+1, k2+2 and (frame)
+// the two-point frame of columns of planes k1..k2: where the sweep of planes
+// k1..k2 does not form it.  The predictor's ghost shell stores the frame's
+// values; they stay final, and frame may be false, when the boundary
+// conditions change no ghost value of the four sides (all zero-filled).
+int launch_accel_fix(const Box& b, int k1, int k2, bool frame, const double* up, const double* u, const double* um,
+                     double dt, double* acc, hipStream_t s);
 // out = cof L(w)/S (S = strx stry strz) on planes k1..k2 of the box's
 // interior columns: the predictor sweep without the time update, for the
 // transposed operator.

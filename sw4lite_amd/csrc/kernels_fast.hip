@@ -366,7 +366,22 @@ k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __rest
                                     cupd, beta);
 }
 
-// Predictor sweep (TR: the transposed mode above) whose staged planes reach
+// Modes of the asynchronous-load sweep: the predictor (out = Up from u = U,
+// um = Um; with acc, also the acceleration (Up - 2U + Um)/dt^2 at the points
+// it computes, the expression the corrector stages), the transposed operator
+// (TR above: u = W, no um, no rho), and the corrector from the stored
+// acceleration (u = A, um = Up for the centre values, cupd = dt^4/12).
+enum DmaMode { DMA_PRED = 0, DMA_TR = 1, DMA_CORR = 2 };
+
+// The acceleration the corrector applies its stencil to (dpdmtfort,
+// ew-cfromfort.C:116-141), one expression wherever it is formed so that a
+// stored value equals the one staged on the fly.
+__device__ __forceinline__ double accel_of(double up, double u, double um, double dt2i)
+{
+   return dt2i * (up - 2 * u + um);
+}
+
+// Sweep (mode MODE above) whose staged planes reach
 // LDS by asynchronous global-to-LDS loads (global_load_lds_dwordx4: lane l of
 // a wave moves the two doubles at its address to LDS at M0 + 16 l).  The
 // load of plane k+3 is issued right after the barrier of plane k and lands
@@ -378,13 +393,14 @@ k_sweep_lds(Box b, TileMap map, int kbeg, int kend, int kc, const double* __rest
 // launcher checks both).  Pairs off the box load from the plane's first
 // point instead (a valid address); only inactive points would read them.
 #define SW4_LDS(p) ((__attribute__((address_space(3))) void*)(p))
-template <bool TR>
+template <int MODE>
 __device__ __forceinline__ void sweep_dma_body(unsigned bid, Box b, TileMap map, int kbeg, int kend, int kc,
                                                const double* __restrict__ u, const double* __restrict__ um,
                                                const double* __restrict__ mu, const double* __restrict__ la,
                                                const double* __restrict__ rho, SgCoef sg, double* __restrict__ out,
-                                               double cof, double cupd)
+                                               double* __restrict__ acc, double cof, double cupd, double dt2i)
 {
+   constexpr bool TR = MODE == DMA_TR;
    constexpr int PP = 3 * PL / 2;          // pairs of a staged displacement plane
    constexpr int NP = (PP + NT - 1) / NT;  // asynchronous loads per thread and plane
    constexpr int MP = PL / 2;              // pairs of a material plane
@@ -461,8 +477,9 @@ __device__ __forceinline__ void sweep_dma_body(unsigned bid, Box b, TileMap map,
    // the results of plane k are stored after the barrier of plane k+1, so
    // that this barrier (which waits for every outstanding memory operation)
    // does not wait for stores issued just before it
-   double ov[3] = {0, 0, 0};
+   double ov[3] = {0, 0, 0}, av[3] = {0, 0, 0};
    long op = -1;  // offset of the held results (-1: none)
+   const bool wacc = MODE == DMA_PRED && acc != nullptr;
    for (int k = k0; k <= k1; k++) {
       const int sl0 = (k - k0) % 6;  // slot of plane k-2
       s.muc[4] = active ? cmu : 0.0;
@@ -471,9 +488,12 @@ __device__ __forceinline__ void sweep_dma_body(unsigned bid, Box b, TileMap map,
       __syncthreads();  // plane k+2 and the material of plane k have landed; plane k-1 is done
       if (op >= 0) {
 #pragma unroll
-         for (int c = 0; c < 3; c++) out[op + npts * c] = ov[c];
+         for (int c = 0; c < 3; c++) {
+            out[op + npts * c] = ov[c];
+            if (wacc) acc[op + npts * c] = av[c];
+         }
       }
-      // centre values (um, rho) of plane k: issued first, needed only after
+      // centre values (um, or up for the corrector; rho) of plane k: issued first, needed only after
       // the stencil, so the stencil covers their latency
       double ctr[3] = {0, 0, 0}, crho = 1.0;
       if (!TR) {
@@ -502,7 +522,15 @@ __device__ __forceinline__ void sweep_dma_body(unsigned bid, Box b, TileMap map,
          rhs4_interior_body(s, r);
          const double sc = TR ? 1.0 / (s.psx[0] * s.psy[0] * s.szv[2]) : cupd / crho;
 #pragma unroll
-         for (int c = 0; c < 3; c++) ov[c] = TR ? sc * (cof * r[c]) : 2 * s.p2[c * PL] - ctr[c] + sc * (cof * r[c]);
+         for (int c = 0; c < 3; c++) {
+            if (MODE == DMA_TR)
+               ov[c] = sc * (cof * r[c]);
+            else if (MODE == DMA_CORR)
+               ov[c] = ctr[c] + sc * (cof * r[c]);
+            else
+               ov[c] = 2 * s.p2[c * PL] - ctr[c] + sc * (cof * r[c]);
+            if (MODE == DMA_PRED) av[c] = accel_of(ov[c], s.p2[c * PL], ctr[c], dt2i);
+         }
          op = col + nij * (k - b.kb);
       }
 #pragma unroll
@@ -514,17 +542,21 @@ __device__ __forceinline__ void sweep_dma_body(unsigned bid, Box b, TileMap map,
    }
    if (op >= 0) {
 #pragma unroll
-      for (int c = 0; c < 3; c++) out[op + npts * c] = ov[c];
+      for (int c = 0; c < 3; c++) {
+         out[op + npts * c] = ov[c];
+         if (wacc) acc[op + npts * c] = av[c];
+      }
    }
 }
 
-template <bool TR>
+template <int MODE>
 __global__ void __launch_bounds__(NT, 2)
 k_sweep_dma(Box b, TileMap map, int kbeg, int kend, int kc, const double* __restrict__ u,
             const double* __restrict__ um, const double* __restrict__ mu, const double* __restrict__ la,
-            const double* __restrict__ rho, SgCoef sg, double* __restrict__ out, double cof, double cupd)
+            const double* __restrict__ rho, SgCoef sg, double* __restrict__ out, double* __restrict__ acc, double cof,
+            double cupd, double dt2i)
 {
-   sweep_dma_body<TR>(blockIdx.x, b, map, kbeg, kend, kc, u, um, mu, la, rho, sg, out, cof, cupd);
+   sweep_dma_body<MODE>(blockIdx.x, b, map, kbeg, kend, kc, u, um, mu, la, rho, sg, out, acc, cof, cupd, dt2i);
 }
 
 // The asynchronous loads need whole 16-byte pairs: even row length, aligned arrays.
@@ -892,6 +924,57 @@ int launch_sweep_batch(int kind, const SweepShot* d_shots, int nshots, int nbloc
    return check_hip(hipGetLastError(), "batched sweep launch");
 }
 
+// The acceleration where the predictor sweep of planes k1..k2 did not form
+// it: the four planes k1-2, k1-1, k2+1, k2+2 whole and (frame) on the planes
+// k1..k2 the two-point frame of columns next to the sides.  Thread q runs
+// over the planes' points first, then over the frame's columns (the rows
+// jb, jb+1, je-1, je whole, then the four side columns of each row between,
+// on consecutive threads, so that a row's two cache lines are read once).
+__global__ void k_accel_fix(Box b, int k1, int k2, bool frame, const double* __restrict__ up,
+                            const double* __restrict__ u, const double* __restrict__ um, double dt2i,
+                            double* __restrict__ acc)
+{
+   const long ni = b.ni, nj = b.je - b.jb + 1, nij = b.nij, npts = b.npts;
+   const long nplane = 4 * nij;
+   const long nframe = 4 * (ni + nj - 4);  // columns of the frame in one plane
+   const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   long p;
+   if (q < nplane) {
+      const int s = (int)(q / nij);
+      const int k = s < 2 ? k1 - 2 + s : k2 - 1 + s;
+      p = nij * (k - b.kb) + q % nij;
+   } else {
+      const long r = q - nplane;
+      if (!frame || r >= nframe * (k2 - k1 + 1)) return;
+      const int k = k1 + (int)(r / nframe);
+      const long f = r % nframe;
+      long ii, jj;  // offsets from (ib, jb)
+      if (f < 4 * ni) {  // rows jb, jb+1, je-1, je, every i
+         const long w = f / ni;
+         ii = f % ni;
+         jj = w < 2 ? w : nj - 4 + w;
+      } else {  // columns ib, ib+1, ie-1, ie on the rows between: four consecutive threads per row
+         const long g = f - 4 * ni, w = g % 4;
+         jj = 2 + g / 4;
+         ii = w < 2 ? w : ni - 4 + w;
+      }
+      p = nij * (k - b.kb) + ni * jj + ii;
+   }
+#pragma unroll
+   for (int c = 0; c < 3; c++) acc[p + npts * c] = accel_of(up[p + npts * c], u[p + npts * c], um[p + npts * c], dt2i);
+}
+
+int launch_accel_fix(const Box& b, int k1, int k2, bool frame, const double* up, const double* u, const double* um,
+                     double dt, double* acc, hipStream_t s)
+{
+   if (k2 < k1) return OK;
+   if (k1 - 2 < b.kb || k2 + 2 > b.ke || b.ni < 5 || b.je - b.jb + 1 < 5)
+      return set_error(E_INVAL, "acceleration fix: planes or box out of range");
+   const long n = 4 * b.nij + (frame ? 4 * (b.ni + (b.je - b.jb + 1) - 4) * (long)(k2 - k1 + 1) : 0);
+   k_accel_fix<<<cdiv(n, 256), 256, 0, s>>>(b, k1, k2, frame, up, u, um, 1.0 / (dt * dt), acc);
+   return check_hip(hipGetLastError(), "acceleration fix launch");
+}
+
 int launch_damp_lds(const Box& b, int k1, int k2, int kc, double* up, const double* u, const double* um,
                     const double* rho, const SgCoef& sg, double beta, hipStream_t s)
 {
@@ -1038,7 +1121,7 @@ int launch_sbp_rows(bool corr, const Box& b, int side, int nk, const SbpTables* 
 
 int launch_sweep_lds(bool corr, int variant, const Box& b, int k1, int k2, int kc, const double* upin,
                      const double* u, const double* um, const double* mu, const double* la, const double* rho,
-                     const SgCoef& sg, double* out, double h, double dt, double beta, hipStream_t s)
+                     const SgCoef& sg, double* out, double h, double dt, double beta, hipStream_t s, double* acc)
 {
    if (k2 < k1) return OK;
    const long nx = b.ie - b.ib - 3, ny = b.je - b.jb - 3;
@@ -1070,10 +1153,21 @@ int launch_sweep_lds(bool corr, int variant, const Box& b, int k1, int k2, int k
       if (corr) SW4_SWEEP(true, false, 2, false); else SW4_SWEEP(false, false, 2, false);
       break;
    case 7:  // as 6, the predictor's planes by asynchronous global-to-LDS loads where they fit
-      if (corr)
+   case 8:  // as 7; with acc, the predictor also stores the acceleration and the corrector stages it
+      if (variant == 8 && acc) {
+         if (!dma_fits(b, acc, mu, la) || !dma_fits(b, corr ? upin : u, mu, la))
+            return set_error(E_INVAL, "acceleration sweeps need an even row length and aligned arrays");
+         if (corr)
+            k_sweep_dma<DMA_CORR><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, acc, upin, mu, la, rho, sg, out, nullptr,
+                                                     cof, c4, dt2i);
+         else
+            k_sweep_dma<DMA_PRED><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, u, um, mu, la, rho, sg, out, acc, cof, c2,
+                                                     dt2i);
+      } else if (corr)
          SW4_SWEEP(true, false, 2, false);
       else if (dma_fits(b, u, mu, la))
-         k_sweep_dma<false><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, u, um, mu, la, rho, sg, out, cof, c2);
+         k_sweep_dma<DMA_PRED><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, u, um, mu, la, rho, sg, out, nullptr, cof, c2,
+                                                  dt2i);
       else
          SW4_SWEEP(false, false, 2, false);
       break;
@@ -1097,7 +1191,8 @@ int launch_sweep_lds_T(const Box& b, int k1, int k2, int kc, const double* w, co
    map.ntiles = map.ntx * map.nty * (int)cdiv(k2 - k1 + 1, kc);
    const unsigned grd = 8 * cdiv(map.ntiles, 8);
    if (dma_fits(b, w, mu, la))
-      k_sweep_dma<true><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, w, nullptr, mu, la, nullptr, sg, out, 1.0 / (h * h), 1.0);
+      k_sweep_dma<DMA_TR><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, w, nullptr, mu, la, nullptr, sg, out, nullptr,
+                                             1.0 / (h * h), 1.0, 1.0);
    else
       k_sweep_lds<false, false, 2, false, true><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, nullptr, w, nullptr, mu, la,
                                                                   nullptr, sg, out, 1.0 / (h * h), 1.0, 1.0, 0.0);

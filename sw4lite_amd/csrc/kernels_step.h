@@ -23,7 +23,7 @@ int launch_pred_fused(const Box& b, int nk, const int* onesided, const SbpTables
                       const double* u, const double* um, const double* mu, const double* la,
                       const double* rho, const SgCoef& sg, double* up, double h, double dt,
                       hipStream_t s, const std::function<int()>& after_interior, int variant = 1,
-                      int kchunk = 32, int zfill = 0);
+                      int kchunk = 32, int zfill = 0, double* accf = nullptr);
 // Doubles of the acceleration slab launch_corr_fused needs as scratch (acc).
 long accel_slab_size(const Box& b);
 int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp,
@@ -31,14 +31,20 @@ int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables
                       const double* la, const double* rho, const SgCoef& sg, double beta,
                       double* upo, double* acc, double h, double dt, hipStream_t s,
                       const std::function<int()>& after_interior, const std::function<int()>& after_damping,
-                      int variant = 1, int kchunk = 32, int zfill = 0);
+                      int variant = 1, int kchunk = 32, int zfill = 0, const double* accf = nullptr);
+// Interior planes of the fused sweeps (below and above the SBP rows).
+void fused_planes(const Box& b, int nk, const int* onesided, int* k1, int* k2);
+// acc = (up - 2u + um)/dt^2 at the grid points of the sources (after the
+// sources have been added to up).
+int launch_accel_points(const Box& b, const SourceTable& st, const double* up, const double* u, const double* um,
+                        double dt, double* acc, hipStream_t s);
 // Rows next to a free surface and the ghost shell of the fused sweeps
 // (variant >= 1), after the interior sweep.  zfill (bit s): side s takes
 // zero ghost values in the shell's pass (Dirichlet or supergrid side with no
 // data, no periodic side); the boundary conditions then skip it.
 int launch_pred_tail(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp, const double* u,
                      const double* um, const double* mu, const double* la, const double* rho, const SgCoef& sg,
-                     double* up, double h, double dt, hipStream_t s, int zfill = 0);
+                     double* up, double h, double dt, hipStream_t s, int zfill = 0, double* accf = nullptr);
 int launch_corr_tail(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp, const double* up,
                      const double* u, const double* um, const double* mu, const double* la, const double* rho,
                      const SgCoef& sg, double beta, double* upo, double h, double dt, hipStream_t s,

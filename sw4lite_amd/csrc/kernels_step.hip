@@ -154,7 +154,8 @@ k_addsgd4_planes(Box b, int klo, int khi, double* __restrict__ up, const double*
 // predictor where L(u) and F vanish); mode 1: out = in.
 // ---------------------------------------------------------------------------
 __global__ void k_shell(Box b, int mode, const double* __restrict__ a, const double* __restrict__ u,
-                        const double* __restrict__ um, double* __restrict__ out, int zfill)
+                        const double* __restrict__ um, double* __restrict__ out, int zfill,
+                        double* __restrict__ acc = nullptr, double dt2i = 0)
 {
    const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
    int i, j, k;
@@ -165,6 +166,18 @@ __global__ void k_shell(Box b, int mode, const double* __restrict__ a, const dou
                      ((zfill & 4) && j <= b.jb + 1) || ((zfill & 8) && j >= b.je - 1) ||
                      ((zfill & 16) && k <= b.kb + 1) || ((zfill & 32) && k >= b.ke - 1);
    const long p = sidx(b, i, j, k);
+   if (mode == 0 && acc) {
+      // predictor with the stored acceleration: also (up - 2u + um)/dt^2 here,
+      // the expression the corrector stages
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+         const long v = p + b.npts * c;
+         const double uv = u[v], mv = um[v], o = zero ? 0.0 : 2 * uv - mv;
+         out[v] = o;
+         acc[v] = dt2i * (o - 2 * uv + mv);
+      }
+      return;
+   }
 #pragma unroll
    for (int c = 0; c < 3; c++) {
       const long v = p + b.npts * c;
@@ -201,6 +214,34 @@ __global__ void k_src_apply(Box b, SourceTable st, double t, int deriv2, double 
    up[p] += sc * f0;
    up[p + b.npts] += sc * f1;
    up[p + 2 * b.npts] += sc * f2;
+}
+
+__global__ void k_accel_points(Box b, SourceTable st, const double* __restrict__ up, const double* __restrict__ u,
+                               const double* __restrict__ um, double dt2i, double* __restrict__ acc)
+{
+   const int r = blockIdx.x * blockDim.x + threadIdx.x;
+   if (r >= st.nunique) return;
+   const int s0 = st.first[r];
+   const long p = sidx(b, st.ijk[3 * s0], st.ijk[3 * s0 + 1], st.ijk[3 * s0 + 2]);
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      const long q = p + b.npts * c;
+      acc[q] = dt2i * (up[q] - 2 * u[q] + um[q]);  // the expression the corrector stages
+   }
+}
+
+int launch_accel_points(const Box& b, const SourceTable& st, const double* up, const double* u, const double* um,
+                        double dt, double* acc, hipStream_t s)
+{
+   if (st.nunique <= 0) return OK;
+   k_accel_points<<<cdiv(st.nunique, 256), 256, 0, s>>>(b, st, up, u, um, 1.0 / (dt * dt), acc);
+   return check_hip(hipGetLastError(), "acceleration at source points");
+}
+
+void fused_planes(const Box& b, int nk, const int* onesided, int* k1, int* k2)
+{
+   *k1 = onesided[4] == 1 ? 7 : b.kb + 2;
+   *k2 = onesided[5] == 1 ? nk - 6 : b.ke - 2;
 }
 
 // ---------------------------------------------------------------------------
@@ -483,7 +524,7 @@ int launch_pred_fused(const Box& b, int nk, const int* onesided, const SbpTables
                       const double* u, const double* um, const double* mu, const double* la,
                       const double* rho, const SgCoef& sg, double* up, double h, double dt,
                       hipStream_t s, const std::function<int()>& after_interior, int variant, int kchunk,
-                      int zfill)
+                      int zfill, double* accf)
 {
    const double cof = 1.0 / (h * h), dt2 = dt * dt;
    const int k1 = onesided[4] == 1 ? 7 : b.kb + 2;
@@ -492,7 +533,7 @@ int launch_pred_fused(const Box& b, int nk, const int* onesided, const SbpTables
    dim3 blk(64, 4, 1);
    if (k2 >= k1 && variant >= 1) {
       const int rc = launch_sweep_lds(false, variant, b, k1, k2, kchunk, nullptr, u, um, mu, la, rho, sg, up, h, dt,
-                                      0.0, s);
+                                      0.0, s, accf);
       if (rc) return rc;
    } else if (k2 >= k1) {
       dim3 grd(cdiv(nii, 64), cdiv(njj, 4), k2 - k1 + 1);
@@ -502,7 +543,8 @@ int launch_pred_fused(const Box& b, int nk, const int* onesided, const SbpTables
       const int rc = after_interior();
       if (rc) return rc;
    }
-   if (variant >= 1) return launch_pred_tail(b, nk, onesided, d_sbp, u, um, mu, la, rho, sg, up, h, dt, s, zfill);
+   if (variant >= 1)
+      return launch_pred_tail(b, nk, onesided, d_sbp, u, um, mu, la, rho, sg, up, h, dt, s, zfill, accf);
    for (int side = 4; side <= 5; side++) {
       if (onesided[side] != 1) continue;
       dim3 grd(cdiv(nii, 64), cdiv(njj, 4), 6);
@@ -517,7 +559,7 @@ long accel_slab_size(const Box& b) { return 3 * b.nij * SLAB_PLANES; }
 
 int launch_pred_tail(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp, const double* u,
                      const double* um, const double* mu, const double* la, const double* rho, const SgCoef& sg,
-                     double* up, double h, double dt, hipStream_t s, int zfill)
+                     double* up, double h, double dt, hipStream_t s, int zfill, double* accf)
 {
    for (int side = 4; side <= 5; side++) {  // LDS slab of the six rows
       if (onesided[side] != 1) continue;
@@ -525,7 +567,7 @@ int launch_pred_tail(const Box& b, int nk, const int* onesided, const SbpTables*
       if (rc) return rc;
    }
    const long ns = shell_count(b);
-   k_shell<<<cdiv(ns, 256), 256, 0, s>>>(b, 0, nullptr, u, um, up, zfill);
+   k_shell<<<cdiv(ns, 256), 256, 0, s>>>(b, 0, nullptr, u, um, up, zfill, accf, 1.0 / (dt * dt));
    return check_hip(hipGetLastError(), "predictor rows and shell launch");
 }
 
@@ -549,7 +591,7 @@ int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables
                       const double* la, const double* rho, const SgCoef& sg, double beta,
                       double* upo, double* acc, double h, double dt, hipStream_t s,
                       const std::function<int()>& after_interior, const std::function<int()>& after_damping,
-                      int variant, int kchunk, int zfill)
+                      int variant, int kchunk, int zfill, const double* accf)
 {
    const double cof = 1.0 / (h * h), dt2i = 1.0 / (dt * dt), dt4i12 = dt * dt * dt * dt / 12;
    const int k1 = onesided[4] == 1 ? 7 : b.kb + 2;
@@ -557,7 +599,8 @@ int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables
    const int nii = b.ie - b.ib - 3, njj = b.je - b.jb - 3;
    dim3 blk(64, 4, 1);
    if (k2 >= k1 && variant >= 1) {
-      int rc = launch_sweep_lds(true, variant, b, k1, k2, kchunk, up, u, um, mu, la, rho, sg, upo, h, dt, beta, s);
+      int rc = launch_sweep_lds(true, variant, b, k1, k2, kchunk, up, u, um, mu, la, rho, sg, upo, h, dt, beta, s,
+                                const_cast<double*>(accf));
       if (rc) return rc;
       if (after_interior && (rc = after_interior())) return rc;
       if (beta != 0 && variant >= 6) {

@@ -239,7 +239,10 @@ class EW:
         register cap, 4 as 3 with the corrector's damping in the same sweep, 5
         corrector as 4 and predictor as 1, 6 as 1 with an LDS damping sweep,
         7 as 6 with the predictor's planes loaded asynchronously into LDS
-        where the row length is even, the default)."""
+        where the row length is even, 8 as 7 with the acceleration stored by
+        the predictor and loaded asynchronously by the corrector (the
+        default; one more field of device memory; even row lengths,
+        otherwise as 7))."""
         L.check(self.lib.sw4hip_ew_set_kernel_variant(self.h, int(variant), int(kchunk)))
 
     def set_material(self, rho, mu, la):

@@ -232,6 +232,12 @@ def test_full_size_fused_matches_reference_order(torch):
     (7, 5, [2, 2, 2, 2, 2, 2], 44),
     (7, 1, [2, 2, 2, 2, 0, 2], 44),
     (7, 64, [2, 2, 2, 2, 0, 2]),
+    # variant 8 stores the acceleration (even row lengths; otherwise it is 7)
+    (8, 64, [2, 2, 2, 2, 0, 2], 44),
+    (8, 7, [2, 2, 2, 2, 0, 0], 46),
+    (8, 5, [2, 2, 2, 2, 2, 2], 44),
+    (8, 7, [2, 2, 2, 2, 0, 2]),
+    (8, 7, [3, 3, 3, 3, 0, 1], 44),
 ]])
 def test_sweep_variants_match_oracle(torch, variant, kchunk, bc, nx):
     """The LDS-staged k-marching sweeps on a grid that is not a multiple of
@@ -254,6 +260,38 @@ def test_sweep_variants_match_oracle(torch, variant, kchunk, bc, nx):
     err = relerr(u, ref.U)
     print("sweep variant", variant, "kchunk", kchunk, "bc", bc, "nx", nx, "relerr", err, flush=True)
     assert err < TOL and relerr(um, ref.Um) < TOL
+
+
+# supergrid and Dirichlet sides: the ghost shell's acceleration is final;
+# periodic sides: the boundary conditions change the ghost frame afterwards
+@pytest.mark.parametrize("bc,nx,kchunk", [([2, 2, 2, 2, 0, 2], 44, 7), ([2, 2, 2, 2, 0, 0], 46, 64),
+                                          ([2, 2, 2, 2, 2, 2], 44, 5), ([3, 3, 3, 3, 0, 1], 44, 7),
+                                          ([1, 1, 2, 2, 0, 2], 46, 7)])
+def test_stored_acceleration_bitwise(torch, bc, nx, kchunk):
+    """Variant 8 stores the acceleration in the predictor sweep, completes it
+    where later stages change Up (sources, ghost frame, the planes around the
+    sweep) and lets the corrector stage it: every value the corrector reads
+    must be the one variant 7 forms on the fly, so the solutions agree to
+    the last bit."""
+    from sw4lite_amd import EW
+    ny, nz, h = 19, 41, 0.1
+    rho, mu, la = layered_material(nx, ny, nz, h, 6)
+    res = []
+    for variant in (7, 8):
+        e = EW(nx, ny, nz, h, bc, corder=1, sg_gp_thickness=6)
+        e.set_kernel_variant(variant, kchunk)
+        e.set_material(rho, mu, la)
+        for s in SOURCES:
+            e.add_moment_source(s["x"], s["y"], s["z"], s["m"], s["t0"], s["freq"], s["tfkind"])
+        e.setup(nsteps=8)
+        e.timesteploop(8, mode=0)
+        res.append(e.get_solution())
+        e.close()
+    (u7, um7), (u8, um8) = res
+    assert np.max(np.abs(u7)) > 0
+    ndiff = int(np.count_nonzero(u7 != u8)) + int(np.count_nonzero(um7 != um8))
+    print("stored acceleration: bc", bc, "nx", nx, "kchunk", kchunk, "points differing from variant 7:", ndiff, flush=True)
+    assert ndiff == 0
 
 
 def test_receivers_match_solution(torch):
