@@ -116,27 +116,46 @@ int Batch::build_tables()
    const int n = (int)m_shots.size();
    std::vector<SweepShot> h(9 * n);
    for (int ph = 0; ph < 3; ph++) {
-      int g0[3] = {0, 0, 0};
-      for (int q = 0; q < n; q++) {
-         EW* e = m_shots[q];
-         const GridSetup& g = e->m_opt.grid;
-         const Box& b = e->m_box;
-         double* um = m_base[3 * q + ph % 3];  // phase ph: (Um, U, Up2) = base rotated by ph
-         double* u = m_base[3 * q + (ph + 1) % 3];
-         double* up2 = m_base[3 * q + (ph + 2) % 3];
-         const SgCoef c = e->sg();
-         const int k1 = e->m_onesided[4] == 1 ? 7 : b.kb + 2;
-         const int k2 = e->m_onesided[5] == 1 ? g.nz - 6 : b.ke - 2;
-         SweepShot* t = &h[(ph * 3) * n];
-         g0[0] = fill_sweep_shot(false, b, k1, k2, m_kchunk, nullptr, u, um, e->mMu.dev, e->mLambda.dev, e->mRho.dev, c,
-                                 e->mUp.dev, g.h, e->mDt, 0.0, false, g0[0], &t[q]);
-         g0[1] = fill_sweep_shot(false, b, k1, k2, m_kchunk, e->mUp.dev, u, um, e->mMu.dev, e->mLambda.dev, e->mRho.dev,
-                                 c, up2, g.h, e->mDt, e->m_opt.beta, true, g0[1], &t[n + q]);
-         const bool damp = e->m_opt.beta != 0;
-         g0[2] = fill_sweep_shot(true, b, k1, damp ? k2 : k1 - 1, m_kchunk, nullptr, u, um, nullptr, nullptr, e->mRho.dev,
-                                 c, up2, g.h, e->mDt, e->m_opt.beta, true, g0[2], &t[2 * n + q]);
+      // kinds 0 and 2: the grids whose planes take the asynchronous loads
+      // first (their own launch, block ids from 0), then the others
+      int g0[3] = {0, 0, 0}, ga[3] = {0, 0, 0}, slot[3] = {0, 0, 0};
+      for (int pass = 0; pass < 2; pass++)
+         for (int q = 0; q < n; q++) {
+            EW* e = m_shots[q];
+            const GridSetup& g = e->m_opt.grid;
+            const Box& b = e->m_box;
+            double* um = m_base[3 * q + ph % 3];  // phase ph: (Um, U, Up2) = base rotated by ph
+            double* u = m_base[3 * q + (ph + 1) % 3];
+            double* up2 = m_base[3 * q + (ph + 2) % 3];
+            const SgCoef c = e->sg();
+            const int k1 = e->m_onesided[4] == 1 ? 7 : b.kb + 2;
+            const int k2 = e->m_onesided[5] == 1 ? g.nz - 6 : b.ke - 2;
+            SweepShot* t = &h[(ph * 3) * n];
+            const double* pa[3] = {u, e->mMu.dev, e->mLambda.dev};
+            const double* da[4] = {u, um, e->mRho.dev, up2};
+            const bool fa[3] = {async_loads_fit(b, pa, 3), false, async_loads_fit(b, da, 4)};
+            const bool damp = e->m_opt.beta != 0;
+            for (int k = 0; k < 3; k++) {
+               if (fa[k] != (pass == 0)) continue;
+               int& gc = fa[k] ? ga[k] : g0[k];
+               SweepShot* d = &t[k * n + slot[k]++];
+               if (k == 0)
+                  gc = fill_sweep_shot(false, b, k1, k2, m_kchunk, nullptr, u, um, e->mMu.dev, e->mLambda.dev,
+                                       e->mRho.dev, c, e->mUp.dev, g.h, e->mDt, 0.0, false, gc, d);
+               else if (k == 1)
+                  gc = fill_sweep_shot(false, b, k1, k2, m_kchunk, e->mUp.dev, u, um, e->mMu.dev, e->mLambda.dev,
+                                       e->mRho.dev, c, up2, g.h, e->mDt, e->m_opt.beta, true, gc, d);
+               else
+                  gc = fill_sweep_shot(true, b, k1, damp ? k2 : k1 - 1, m_kchunk, nullptr, u, um, nullptr, nullptr,
+                                       e->mRho.dev, c, up2, g.h, e->mDt, e->m_opt.beta, true, gc, d);
+            }
+            if (pass == 0)
+               for (int k = 0; k < 3; k++) m_nasync[ph][k] = slot[k];
+         }
+      for (int k = 0; k < 3; k++) {
+         m_nblk[ph][k] = g0[k];
+         m_nblk_async[ph][k] = ga[k];
       }
-      for (int k = 0; k < 3; k++) m_nblk[ph][k] = g0[k];
    }
    if (m_exec) {
       (void)hipGraphExecDestroy(m_exec);
@@ -150,7 +169,10 @@ int Batch::one_step(int ph)
    const int n = (int)m_shots.size();
    const SweepShot* tab = m_dtab + (ph * 3) * n;
    int rc;
-   if ((rc = launch_sweep_batch(0, tab, n, m_nblk[ph][0], m_s)) || (rc = fork())) return rc;
+   const int a0 = m_nasync[ph][0], a2 = m_nasync[ph][2];
+   if ((rc = launch_sweep_batch_async(0, tab, a0, m_nblk_async[ph][0], m_s)) ||
+       (rc = launch_sweep_batch(0, tab + a0, n - a0, m_nblk[ph][0], m_s)) || (rc = fork()))
+      return rc;
    for (int q = 0; q < n; q++) {
       EW* e = m_shots[q];
       e->m_stream = m_side[q % NSIDE];
@@ -168,7 +190,8 @@ int Batch::one_step(int ph)
          return rc;
    }
    if ((rc = join()) || (rc = launch_sweep_batch(1, tab + n, n, m_nblk[ph][1], m_s)) ||
-       (rc = launch_sweep_batch(2, tab + 2 * n, n, m_nblk[ph][2], m_s)) || (rc = fork()))
+       (rc = launch_sweep_batch_async(2, tab + 2 * n, a2, m_nblk_async[ph][2], m_s)) ||
+       (rc = launch_sweep_batch(2, tab + 2 * n + a2, n - a2, m_nblk[ph][2], m_s)) || (rc = fork()))
       return rc;
    for (int q = 0; q < n; q++) {
       EW* e = m_shots[q];

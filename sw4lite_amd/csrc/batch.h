@@ -38,7 +38,11 @@ private:
    std::vector<EW*> m_shots;
    hipStream_t m_s = nullptr;
    SweepShot* m_dtab = nullptr;     // [phase 0..2][kind 0..2][shot]
-   int m_nblk[3][3] = {{0}};
+   int m_nblk[3][3] = {{0}};        // workgroups of the register-staged launch of each kind
+   // predictor and damping: the first m_nasync shots of a table take the
+   // asynchronous-load kernels (their own launch of m_nblk_async workgroups)
+   int m_nasync[3][3] = {{0}};
+   int m_nblk_async[3][3] = {{0}};
    double* m_clk_t = nullptr;       // per shot: time, dt
    double* m_clk_dt = nullptr;
    int* m_clk_s = nullptr;          // per shot: sample count, maximum

@@ -33,6 +33,10 @@ int fill_sweep_shot(bool damping, const Box& b, int k1, int k2, int kc, const do
 // kind 0 predictor, 1 corrector, 2 damping sweeps of nshots grids in one
 // launch of nblocks workgroups (d_shots in device memory).
 int launch_sweep_batch(int kind, const SweepShot* d_shots, int nshots, int nblocks, hipStream_t s);
+// The same for kinds 0 and 2 with the asynchronous-load bodies, for grids
+// where async_loads_fit (even row length, 16-byte aligned arrays).
+int launch_sweep_batch_async(int kind, const SweepShot* d_shots, int nshots, int nblocks, hipStream_t s);
+bool async_loads_fit(const Box& b, const double* const* arrays, int n);
 
 // Interior planes k1..k2 of the predictor (corr = false: out = Up from u, um)
 // or corrector sweep (corr = true: out = Up' from upin = Up, u, um), chunks of

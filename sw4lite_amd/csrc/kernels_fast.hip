@@ -912,6 +912,41 @@ int fill_sweep_shot(bool damping, const Box& b, int k1, int k2, int kc, const do
    return grid0 + 8 * (int)cdiv(sh->map.ntiles, 8);
 }
 
+// Batched predictor (kind 0) and damping (kind 2) sweeps of grids whose
+// planes take the asynchronous global-to-LDS loads (async_loads_fit).
+__global__ void __launch_bounds__(NT, 2) k_sweep_dma_batch(const SweepShot* __restrict__ shots, int nshots)
+{
+   const SweepShot& h = shots[find_shot(shots, nshots, blockIdx.x)];
+   sweep_dma_body<DMA_PRED>(blockIdx.x - h.grid0, h.b, h.map, h.kbeg, h.kend, h.kc, h.u, h.um, h.mu, h.la, h.rho, h.sg,
+                            h.out, nullptr, h.cof, h.cupd, h.dt2i);
+}
+
+__global__ void __launch_bounds__(NT, 3) k_damp_dma_batch(const SweepShot* __restrict__ shots, int nshots)
+{
+   const SweepShot& h = shots[find_shot(shots, nshots, blockIdx.x)];
+   damp_dma_body(blockIdx.x - h.grid0, h.b, h.map, h.kbeg, h.kend, h.kc, h.out, h.u, h.um, h.rho, h.sg, h.beta);
+}
+
+bool async_loads_fit(const Box& b, const double* const* arrays, int n)
+{
+   if (b.ni % 2 != 0) return false;
+   for (int q = 0; q < n; q++)
+      if (((uintptr_t)arrays[q] & 15) != 0) return false;
+   return true;
+}
+
+int launch_sweep_batch_async(int kind, const SweepShot* d_shots, int nshots, int nblocks, hipStream_t s)
+{
+   if (nblocks <= 0) return OK;
+   if (kind == 0)
+      k_sweep_dma_batch<<<nblocks, NT, 0, s>>>(d_shots, nshots);
+   else if (kind == 2)
+      k_damp_dma_batch<<<nblocks, NT, 0, s>>>(d_shots, nshots);
+   else
+      return set_error(E_INVAL, "batched asynchronous-load sweeps: predictor or damping only");
+   return check_hip(hipGetLastError(), "batched asynchronous-load sweep launch");
+}
+
 int launch_sweep_batch(int kind, const SweepShot* d_shots, int nshots, int nblocks, hipStream_t s)
 {
    if (nblocks <= 0) return OK;
