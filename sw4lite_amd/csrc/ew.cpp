@@ -658,13 +658,14 @@ int EW::step_comp(int n, double* stage_ms)
       // the damping is part of the corrector sweep: stages 4 and 5 coincide
       if ((rc = mark(s, 0)) ||
           (rc = launch_comp_pred(m_box, g.nz, m_onesided, m_dsbpf, cs, cf, g.h, m_stream,
-                                 [&]() { return mark(s, 1); }, m_kchunk, dt, zfill)) ||
+                                 [&]() { return mark(s, 1); }, m_kchunk, dt, zfill, m_variant >= 7)) ||
           (rc = mark(s, 2)) || (rc = launch_comp_src(false, m_box, st, t, 1.0, cs, m_stream)) ||
           (rc = launch_comp_bc(false, m_box, m_wind, g.nx, g.ny, g.nz, g.bc, cs, g.h, dt, m_dsbp, mMu.dev,
                                mLambda.dev, c.strx, c.stry, m_stream, zfill)) ||
           (rc = mark(s, 3)) ||
           (rc = launch_comp_corr(m_box, g.nz, m_onesided, m_dsbpf, cs, cf, m_opt.beta, g.h, dt, m_stream,
-                                 [&]() { const int r = mark(s, 4); return r ? r : mark(s, 5); }, m_kchunk, zfill)) ||
+                                 [&]() { const int r = mark(s, 4); return r ? r : mark(s, 5); }, m_kchunk, zfill,
+                                 m_variant >= 7)) ||
           (rc = mark(s, 6)) ||
           (rc = launch_comp_src(true, m_box, st, t, dt * dt * dt * dt / 12, cs, m_stream)) ||
           (rc = launch_comp_bc(true, m_box, m_wind, g.nx, g.ny, g.nz, g.bc, cs, g.h, dt, m_dsbp, mMu.dev,

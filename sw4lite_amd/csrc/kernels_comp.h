@@ -50,11 +50,13 @@ int launch_to_float(long n, const double* src, float* dst, hipStream_t s);
 // when no side is periodic.
 int launch_comp_pred(const Box& b, int nk, const int* onesided, const SbpTablesT<float>* d_sbpf, const CompState& st,
                      const SgCoefF& sgf, double h, hipStream_t s, const std::function<int()>& after_interior,
-                     int kchunk, double dt = 0.0, int zfill = 0);
-// Corrector: Dn and the compensated (U, C) += Dn on every point.
+                     int kchunk, double dt = 0.0, int zfill = 0, bool async = true);
+// Corrector: Dn and the compensated (U, C) += Dn on every point.  async: the
+// interior sweeps bring their planes to LDS by asynchronous loads where the
+// rows allow it (ni % 4 == 0, aligned arrays); the same bits either way.
 int launch_comp_corr(const Box& b, int nk, const int* onesided, const SbpTablesT<float>* d_sbpf, const CompState& st,
                      const SgCoefF& sgf, double beta, double h, double dt, hipStream_t s,
-                     const std::function<int()>& after_interior, int kchunk, int zfill = 0);
+                     const std::function<int()>& after_interior, int kchunk, int zfill = 0, bool async = true);
 // Sources: predictor A += F(t)/rho; corrector Dn += coef/rho F_tt(t) and
 // (U, C) += the same increment.
 int launch_comp_src(bool corr, const Box& b, const SourceTable& st, double t, double coef, const CompState& cs,

@@ -308,6 +308,237 @@ k_comp_sweep(Box b, CompTiles map, int kbeg, int kend, int kc, const float* __re
    }
 }
 
+// The two sweeps above with their staged planes brought to LDS by
+// asynchronous global-to-LDS loads (global_load_lds_dwordx4: lane l of a
+// wave moves four floats to LDS at M0 + 16 l), as the FP64 sweeps of
+// kernels_fast.hip: the planes of k+3 and the material of plane k+1 land
+// while plane k is computed, one barrier per plane; the centre values of U,
+// C (or rho) are issued after the barrier and used after the stencil, the
+// column values one plane ahead.  Rings: the staged field in six slots
+// (planes k-2..k+3), D in four (k..k+3), mu, lambda (and rho) in two.  Same
+// arithmetic in the same order as k_comp_sweep.  Needs ni % 4 == 0 and
+// 16-byte aligned arrays (every quad of floats in one row, aligned).
+#define SW4_LDSF(p) ((__attribute__((address_space(3))) void*)(p))
+template <bool CORR>
+__global__ void __launch_bounds__(NT, 2)
+k_comp_dma(Box b, CompTiles map, int kbeg, int kend, int kc, const float* __restrict__ stg,
+           const float* __restrict__ mu, const float* __restrict__ la, const float* __restrict__ rho, SgCoefF sg,
+           const float* __restrict__ dold, float* __restrict__ dnew, float* __restrict__ uu,
+           float* __restrict__ cc, float* __restrict__ aout, float cof, float dt2, float c4, float beta)
+{
+   constexpr int QP = 3 * PL / 4;          // quads of a staged 3-component plane
+   constexpr int NQ = (QP + NT - 1) / NT;  // asynchronous loads per thread and plane
+   constexpr int QM = PL / 4;              // quads of a scalar plane
+   static_assert(HX % 4 == 0 && QM <= NT, "quads must tile the rows; one material load per thread");
+   __shared__ __align__(16) float su[6 * 3 * PL];              // plane kp in slot (kp - k0 + 2) % 6
+   __shared__ __align__(16) float sd[CORR ? 4 * 3 * PL : 4];   // D of plane kp in slot (kp - k0) % 4
+   __shared__ __align__(16) float sm[2 * PL], sl[2 * PL];      // plane kp in half (kp - k0) & 1
+   __shared__ __align__(16) float sr[CORR ? 2 * PL : 4];
+   __shared__ float ssx[HX], ssy[HY];
+   __shared__ float sgx[CORR ? 5 * TX : 1], sgy[CORR ? 5 * TY : 1];
+   const int per = (int)((map.ntiles + 7) / 8);
+   const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+   if (t >= map.ntiles) return;
+   const int tix = t % map.ntx, tr = t / map.ntx, tjy = tr % map.nty, tkz = tr / map.nty;
+   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+   const int ia = b.ib + tix * TX, ja = b.jb + tjy * TY;
+   const int i = ia + 2 + tx, j = ja + 2 + ty;
+   const bool active = i <= b.ie - 2 && j <= b.je - 2;
+   const int k0 = kbeg + tkz * kc;
+   const int k1 = min(kend, k0 + kc - 1);
+   const long ni = b.ni, nij = b.nij, npts = b.npts;
+   const long col = active ? (i - b.ib) + ni * (j - b.jb) : 0;
+   const int o = (ty + 2) * HX + tx + 2;
+
+   // quad m = tid + p NT of the staged (c, jj, ii) tile: offset of its first float
+   long goff[NQ];
+#pragma unroll
+   for (int p = 0; p < NQ; p++) {
+      const int e = 4 * (tid + p * NT);
+      const int c = e / PL, r = e - c * PL, jj = r / HX, ii = r - jj * HX;
+      const int gi = ia + ii, gj = ja + jj;
+      const bool ok = e < 3 * PL && gi <= b.ie && gj <= b.je;
+      goff[p] = ok ? (gi - b.ib) + ni * (gj - b.jb) + npts * c : 0;
+   }
+   long moff;
+   {
+      const int e = 4 * tid, jj = e / HX, ii = e - jj * HX;
+      const int gi = ia + ii, gj = ja + jj;
+      moff = e < PL && gi <= b.ie && gj <= b.je ? (gi - b.ib) + ni * (gj - b.jb) : 0;
+   }
+   auto dma3 = [&](const float* src, float* ring, int kp, int slot) {
+      const float* g = src + nij * (kp - b.kb);
+      float* dst = ring + slot * 3 * PL + 4 * 64 * wv;
+#pragma unroll
+      for (int p = 0; p < NQ; p++)
+         if (tid + p * NT < QP) __builtin_amdgcn_global_load_lds(g + goff[p], SW4_LDSF(dst + 4 * p * NT), 16, 0, 0);
+   };
+   auto dma_mat = [&](int kp, int half) {
+      if (tid < QM) {
+         const long pb = nij * (kp - b.kb);
+         __builtin_amdgcn_global_load_lds(mu + pb + moff, SW4_LDSF(sm + half * PL + 4 * 64 * wv), 16, 0, 0);
+         __builtin_amdgcn_global_load_lds(la + pb + moff, SW4_LDSF(sl + half * PL + 4 * 64 * wv), 16, 0, 0);
+         if (CORR)
+            __builtin_amdgcn_global_load_lds(rho + pb + moff, SW4_LDSF(sr + half * PL + 4 * 64 * wv), 16, 0, 0);
+      }
+   };
+
+   CompSten s;
+   float dcol[5][3], rkm = 0, rk0 = 0, nrkp = 0;
+   if (CORR) {
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+         for (int c = 0; c < 3; c++) {
+            const float v = dold[col + nij * (k0 - 2 + q - b.kb) + npts * c];
+            dcol[q][c] = active ? v : 0.0f;
+         }
+      const float v = rho[col + nij * (k0 - 1 - b.kb)];
+      rkm = active ? v : 0.0f;
+      nrkp = rho[col + nij * (k0 + 1 - b.kb)];
+      if (tid < TX) {
+         const int ii = ia + 2 + tid - b.ib;
+         const bool ok = ia + 2 + tid <= b.ie - 2;
+         sgx[0 * TX + tid] = ok ? sg.dcx[ii - 1] : 0.0f;
+         sgx[1 * TX + tid] = ok ? sg.dcx[ii] : 0.0f;
+         sgx[2 * TX + tid] = ok ? sg.dcx[ii + 1] : 0.0f;
+         sgx[3 * TX + tid] = ok ? sg.strx[ii] : 0.0f;
+         sgx[4 * TX + tid] = ok ? sg.cox[ii] : 0.0f;
+      } else if (tid >= 64 && tid < 64 + TY) {
+         const int q = tid - 64, jj = ja + 2 + q - b.jb;
+         const bool ok = ja + 2 + q <= b.je - 2;
+         sgy[0 * TY + q] = ok ? sg.dcy[jj - 1] : 0.0f;
+         sgy[1 * TY + q] = ok ? sg.dcy[jj] : 0.0f;
+         sgy[2 * TY + q] = ok ? sg.dcy[jj + 1] : 0.0f;
+         sgy[3 * TY + q] = ok ? sg.stry[jj] : 0.0f;
+         sgy[4 * TY + q] = ok ? sg.coy[jj] : 0.0f;
+      }
+   }
+   if (tid >= 128 && tid < 128 + HX) ssx[tid - 128] = ia + tid - 128 <= b.ie ? sg.strx[ia + tid - 128 - b.ib] : 1.0f;
+   if (tid >= 192 && tid < 192 + HY) ssy[tid - 192] = ja + tid - 192 <= b.je ? sg.stry[ja + tid - 192 - b.jb] : 1.0f;
+#pragma unroll
+   for (int q = 0; q < 5; q++) dma3(stg, su, k0 - 2 + q, q);
+   if (CORR) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) dma3(dold, sd, k0 + q, q);
+   }
+   dma_mat(k0, 0);
+#pragma unroll
+   for (int q = 0; q < 4; q++) {
+      const long pq = nij * (k0 - 2 + q - b.kb);
+      const float vm = mu[col + pq], vl = la[col + pq];
+      s.muc[q] = active ? vm : 0.0f;
+      s.lac[q] = active ? vl : 0.0f;
+      s.szv[q] = sg.strz[k0 - 2 + q - b.kb];
+   }
+   s.psx = ssx + tx + 2;
+   s.psy = ssy + ty + 2;
+   float cmu = mu[col + nij * (k0 + 2 - b.kb)], cla = la[col + nij * (k0 + 2 - b.kb)];
+   float csz = sg.strz[k0 + 2 - b.kb];
+
+   for (int k = k0; k <= k1; k++) {
+      const int sl0 = (k - k0) % 6, h = (k - k0) & 1;
+      const long pk = col + nij * (k - b.kb);
+      s.muc[4] = active ? cmu : 0.0f;
+      s.lac[4] = active ? cla : 0.0f;
+      s.szv[4] = csz;
+      const float rkp = nrkp;
+      __syncthreads();  // plane k+2 (field, D) and the material of plane k have landed; plane k-1 is done
+      // centre values, issued first and used after the stencil
+      float cu[3] = {0, 0, 0}, cl[3] = {0, 0, 0}, crho = 1.0f;
+      if (CORR) {
+#pragma unroll
+         for (int c = 0; c < 3; c++) {
+            cu[c] = uu[pk + npts * c];
+            cl[c] = cc[pk + npts * c];
+         }
+      } else {
+         const float v = rho[pk];
+         crho = active ? v : 1.0f;
+      }
+      if (k < k1) {
+         dma3(stg, su, k + 3, (sl0 + 5) % 6);
+         if (CORR) dma3(dold, sd, k + 3, (k + 3 - k0) % 4);
+         dma_mat(k + 1, h ^ 1);
+         cmu = mu[col + nij * (k + 3 - b.kb)];  // col = 0 where inactive: a valid address, value unused
+         cla = la[col + nij * (k + 3 - b.kb)];
+         csz = sg.strz[k + 3 - b.kb];
+         if (CORR) nrkp = rho[col + nij * (k + 2 - b.kb)];
+      }
+      const int kz = k - b.kb;
+      float gz[5] = {0, 0, 0, 0, 0};
+      if (CORR) {
+         gz[0] = sg.dcz[kz - 1]; gz[1] = sg.dcz[kz]; gz[2] = sg.dcz[kz + 1]; gz[3] = sg.strz[kz]; gz[4] = sg.coz[kz];
+      }
+      if (active) {
+         s.p0 = su + (sl0 * 3) * PL + o;
+         s.p1 = su + (((sl0 + 1) % 6) * 3) * PL + o;
+         s.p2 = su + (((sl0 + 2) % 6) * 3) * PL + o;
+         s.p3 = su + (((sl0 + 3) % 6) * 3) * PL + o;
+         s.p4 = su + (((sl0 + 4) % 6) * 3) * PL + o;
+         s.pm = sm + h * PL + o;
+         s.pl = sl + h * PL + o;
+         float r[3];
+         rhs4_interior_body(s, r);
+         if (!CORR) {
+            const float sc = cof / crho;
+#pragma unroll
+            for (int c = 0; c < 3; c++) aout[pk + npts * c] = sc * r[c];
+         } else {
+            const float* srh = sr + h * PL;
+            const float r0 = srh[o];
+            rk0 = r0;
+            const float* dpl = sd + ((k - k0) % 4) * 3 * PL + o;  // D of plane k at this point
+#pragma unroll
+            for (int c = 0; c < 3; c++) dcol[4][c] = sd[((k + 2 - k0) % 4) * 3 * PL + c * PL + o];
+            const float birho = beta / r0;
+            const float* gx = sgx + tx;
+            const float* gy = sgy + ty;
+            const float rxp = srh[o + 1] * gx[2 * TX], rx0 = r0 * gx[TX], rxm = srh[o - 1] * gx[0];
+            const float ryp = srh[o + HX] * gy[2 * TY], ry0 = r0 * gy[TY], rym = srh[o - HX] * gy[0];
+            const float rzp = rkp * gz[2], rz0 = r0 * gz[1], rzm = rkm * gz[0];
+            const float wx = gx[3 * TX] * gy[4 * TY] * gz[4];
+            const float wy = gy[3 * TY] * gx[4 * TX] * gz[4];
+            const float wz = gz[3] * gx[4 * TX] * gy[4 * TY];
+            const float sc = c4 / r0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+               const float* q = dpl + c * PL;
+               const float ax = rxp * (q[2] - 2 * q[1] + q[0]) - 2 * rx0 * (q[1] - 2 * q[0] + q[-1]) +
+                                rxm * (q[0] - 2 * q[-1] + q[-2]);
+               const float ay = ryp * (q[2 * HX] - 2 * q[HX] + q[0]) - 2 * ry0 * (q[HX] - 2 * q[0] + q[-HX]) +
+                                rym * (q[0] - 2 * q[-HX] + q[-2 * HX]);
+               const float az = rzp * (dcol[4][c] - 2 * dcol[3][c] + dcol[2][c]) -
+                                2 * rz0 * (dcol[3][c] - 2 * dcol[2][c] + dcol[1][c]) +
+                                rzm * (dcol[2][c] - 2 * dcol[1][c] + dcol[0][c]);
+               const float dmp = birho * (wx * ax + wy * ay + wz * az);
+               const float dn = ((q[0] + dt2 * s.p2[c * PL]) + sc * (cof * r[c])) - dmp;
+               float U = cu[c], C = cl[c];
+               comp_add(U, C, dn);
+               const long g = pk + npts * c;
+               dnew[g] = dn;
+               uu[g] = U;
+               cc[g] = C;
+            }
+         }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+         s.muc[q] = s.muc[q + 1];
+         s.lac[q] = s.lac[q + 1];
+         s.szv[q] = s.szv[q + 1];
+      }
+      if (CORR) {
+#pragma unroll
+         for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) dcol[q][c] = dcol[q + 1][c];
+         rkm = rk0;
+      }
+   }
+}
+
 // Damping increment of one point from the stored D (addsgd4_point on
 // D = U - Um, ew-cfromfort.C:985-1041), FP32.
 __device__ __forceinline__ void damp_point_f(const Box& b, int i, int j, int k, const float* __restrict__ d,
@@ -689,7 +920,7 @@ int launch_to_float(long n, const double* src, float* dst, hipStream_t s)
 
 namespace {
 int comp_sweep(bool corr, const Box& b, int k1, int k2, int kc, const CompState& st, const SgCoefF& sg, float cof,
-               float dt2, float c4, float beta, hipStream_t s)
+               float dt2, float c4, float beta, hipStream_t s, bool async)
 {
    if (k2 < k1) return OK;
    const long nx = b.ie - b.ib - 3, ny = b.je - b.jb - 3;
@@ -699,9 +930,18 @@ int comp_sweep(bool corr, const Box& b, int k1, int k2, int kc, const CompState&
    map.nty = (int)cdiv(ny, TY);
    map.ntiles = map.ntx * map.nty * (int)cdiv(k2 - k1 + 1, kc);
    const unsigned grd = 8 * cdiv(map.ntiles, 8);
-   if (corr)
+   // asynchronous loads where every staged quad of floats lies in one row, aligned
+   auto al = [](const float* p) { return ((uintptr_t)p & 15) == 0; };
+   const bool fits = async && b.ni % 4 == 0 && al(st.mu) && al(st.la) && al(st.rho) && (corr ? al(st.A) && al(st.D) : al(st.U));
+   if (corr && fits)
+      k_comp_dma<true><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, st.A, st.mu, st.la, st.rho, sg, st.D, st.Dn, st.U, st.C,
+                                          nullptr, cof, dt2, c4, beta);
+   else if (corr)
       k_comp_sweep<true><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, st.A, st.mu, st.la, st.rho, sg, st.D, st.Dn, st.U,
                                             st.C, nullptr, cof, dt2, c4, beta);
+   else if (fits)
+      k_comp_dma<false><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, st.U, st.mu, st.la, st.rho, sg, nullptr, nullptr,
+                                           nullptr, nullptr, st.A, cof, dt2, c4, beta);
    else
       k_comp_sweep<false><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, st.U, st.mu, st.la, st.rho, sg, nullptr, nullptr,
                                              nullptr, nullptr, st.A, cof, dt2, c4, beta);
@@ -725,12 +965,12 @@ int comp_rows(bool corr, const Box& b, int side, int nk, const SbpTablesT<float>
 
 int launch_comp_pred(const Box& b, int nk, const int* onesided, const SbpTablesT<float>* d_sbpf, const CompState& st,
                      const SgCoefF& sgf, double h, hipStream_t s, const std::function<int()>& after_interior,
-                     int kchunk, double dt, int zfill)
+                     int kchunk, double dt, int zfill, bool async)
 {
    const float cof = (float)(1.0 / (h * h));
    const int k1 = onesided[4] == 1 ? 7 : b.kb + 2;
    const int k2 = onesided[5] == 1 ? nk - 6 : b.ke - 2;
-   int rc = comp_sweep(false, b, k1, k2, kchunk, st, sgf, cof, 0.0f, 0.0f, 0.0f, s);
+   int rc = comp_sweep(false, b, k1, k2, kchunk, st, sgf, cof, 0.0f, 0.0f, 0.0f, s, async);
    if (rc || (after_interior && (rc = after_interior()))) return rc;
    for (int side = 4; side <= 5; side++)
       if (onesided[side] == 1 && (rc = comp_rows(false, b, side, nk, d_sbpf, st, sgf, cof, 0.0f, 0.0f, 0.0f, s)))
@@ -745,12 +985,12 @@ int launch_comp_pred(const Box& b, int nk, const int* onesided, const SbpTablesT
 
 int launch_comp_corr(const Box& b, int nk, const int* onesided, const SbpTablesT<float>* d_sbpf, const CompState& st,
                      const SgCoefF& sgf, double beta, double h, double dt, hipStream_t s,
-                     const std::function<int()>& after_interior, int kchunk, int zfill)
+                     const std::function<int()>& after_interior, int kchunk, int zfill, bool async)
 {
    const float cof = (float)(1.0 / (h * h)), dt2 = (float)(dt * dt), c4 = (float)(dt * dt * dt * dt / 12);
    const int k1 = onesided[4] == 1 ? 7 : b.kb + 2;
    const int k2 = onesided[5] == 1 ? nk - 6 : b.ke - 2;
-   int rc = comp_sweep(true, b, k1, k2, kchunk, st, sgf, cof, dt2, c4, (float)beta, s);
+   int rc = comp_sweep(true, b, k1, k2, kchunk, st, sgf, cof, dt2, c4, (float)beta, s, async);
    if (rc || (after_interior && (rc = after_interior()))) return rc;
    for (int side = 4; side <= 5; side++)
       if (onesided[side] == 1 && (rc = comp_rows(true, b, side, nk, d_sbpf, st, sgf, cof, dt2, c4, (float)beta, s)))

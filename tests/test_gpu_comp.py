@@ -190,3 +190,45 @@ def test_comp_full_size_matches_fp64(torch):
     err = relerr(b, a)
     print("512^3 FP32 comp vs FP64 after 3 steps: relerr", err, "max|u|", np.max(np.abs(a)), flush=True)
     assert err < FP32_TOL
+
+
+@pytest.mark.parametrize("bc,kchunk", [([2, 2, 2, 2, 0, 2], 7), ([2, 2, 2, 2, 0, 0], 64), ([3, 3, 3, 3, 0, 1], 5)])
+def test_comp_async_sweeps_bitwise(torch, bc, kchunk):
+    """From kernel variant 7 on, the FP32 sweeps bring their planes to LDS by
+    asynchronous loads where the row length allows it (nx + 4 divisible by 4,
+    as here).  The arithmetic is the same in the same order, so the state
+    after mode-2 steps equals variant 6's to the last bit; and it meets the
+    FP32 bar against the FP64 oracle."""
+    from sw4lite_amd import EW
+    nx, ny, nz, h = 44, 19, 41, 0.1
+    rho, mu, la = layered_material(nx, ny, nz, h, 6)
+    res = []
+    for variant in (6, 8):
+        e = EW(nx, ny, nz, h, bc, corder=1, sg_gp_thickness=6)
+        e.set_kernel_variant(variant, kchunk)
+        e.set_material(rho, mu, la)
+        for s in SOURCES:
+            e.add_moment_source(s["x"], s["y"], s["z"], s["m"], s["t0"], s["freq"], s["tfkind"])
+        e.setup(nsteps=10)
+        e.timesteploop(10, mode=2)
+        res.append(e.get_solution())
+        e.close()
+    (u6, um6), (u8, um8) = res
+    assert np.max(np.abs(u6)) > 0
+    ndiff = int(np.count_nonzero(u6 != u8)) + int(np.count_nonzero(um6 != um8))
+    print("FP32 asynchronous-load sweeps: bc", bc, "kchunk", kchunk, "points differing from variant 6:", ndiff,
+          flush=True)
+    assert ndiff == 0
+
+
+def test_comp_async_sweeps_match_oracle(torch):
+    e, ref = make_pair(44, 19, 41, 0.1, [2, 2, 2, 2, 0, 2], 6, 10)
+    e.set_kernel_variant(8, 7)
+    e.setup(nsteps=10)
+    e.timesteploop(10, mode=2)
+    ref.step(10)
+    u, um = e.get_solution()
+    e.close()
+    err = field_errors(u, um, ref)
+    print("FP32 asynchronous-load sweeps vs oracle", err, flush=True)
+    assert err["u"] < FP32_TOL and err["um"] < FP32_TOL
