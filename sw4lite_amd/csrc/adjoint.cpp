@@ -17,6 +17,7 @@
 
 #include "ew.h"
 #include "kernels_adj.h"
+#include "kernels_fast.h"
 
 namespace sw4amd {
 
@@ -83,13 +84,25 @@ int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double
    Sarray UpA = mU;
    UpA.dev = Up;
    int rc;
-   // forward: Up = BC(2U - Um + dt^2/rho (L(U) + F)), A = (Up - 2U + Um)/dt^2
+   // forward: Up = BC(2U - Um + dt^2/rho (L(U) + F)), A = (Up - 2U + Um)/dt^2.
+   // Kernel variant 8: the predictor sweep and its ghost shell store A, the
+   // planes outside the sweep, the side frame (when the boundary conditions
+   // change it) and the source points are completed from the final Up, in
+   // the expression k_dpdmt uses (the same bits); else one k_dpdmt pass.
    const int zf = zfill();
+   int ka = 0, kz = 0;
+   fused_planes(m_box, g.nz, m_onesided, &ka, &kz);
+   double* accf = m_variant == 8 && m_box.ni % 2 == 0 && kz >= ka ? Acc : nullptr;
    if ((rc = launch_pred_fused(m_box, g.nz, m_onesided, m_dsbp, u, um, mMu.dev, mLambda.dev, mRho.dev, c, Up, g.h, dt,
-                               m_stream, nullptr, m_variant, m_kchunk, zf)) ||
+                               m_stream, nullptr, m_variant, m_kchunk, zf, accf)) ||
        (rc = launch_src_apply(m_box, st, t, 0, dt2, mRho.dev, Up, m_stream)) ||
-       (rc = enforceBC(UpA, mMu, mLambda, t + dt, zf)) ||
-       (rc = launch_dpdmt(m_box, Up, u, um, Acc, 1.0 / dt2, m_stream)))
+       (rc = enforceBC(UpA, mMu, mLambda, t + dt, zf)))
+      return rc;
+   if (accf) {
+      if ((rc = launch_accel_fix(m_box, ka, kz, (zf & 15) != 15, true, Up, u, um, dt, Acc, m_stream)) ||
+          (rc = launch_accel_points(m_box, st, Up, u, um, dt, Acc, m_stream)))
+         return rc;
+   } else if ((rc = launch_dpdmt(m_box, Up, u, um, Acc, 1.0 / dt2, m_stream)))
       return rc;
    // corrector BC, transposed (acting on U^{n+1})
    if ((rc = launch_bcfortsg_T(m_box, m_wind, g.nx, g.ny, g.nz, g.bc, Y, u1, g.h, m_dsbp, mMu.dev, mLambda.dev, c.strx,

@@ -64,13 +64,13 @@ int launch_sweep_lds(bool corr, int variant, const Box& b, int k1, int k2, int k
                      const double* u, const double* um, const double* mu, const double* la, const double* rho,
                      const SgCoef& sg, double* out, double h, double dt, double beta, hipStream_t s,
                      double* acc = nullptr);
-// acc = (up - 2u + um)/dt^2 on the planes k1-2, k1-1, k2+1, k2+2 and (frame)
-// the two-point frame of columns of planes k1..k2: where the sweep of planes
-// k1..k2 does not form it.  The predictor's ghost shell stores the frame's
+// acc = (up - 2u + um)/dt^2 on the planes k1-2, k1-1, k2+1, k2+2 (all_planes:
+// every plane of the box outside k1..k2) and (frame) the two-point frame of
+// columns of planes k1..k2: where the sweep of planes k1..k2 does not form it.  The predictor's ghost shell stores the frame's
 // values; they stay final, and frame may be false, when the boundary
 // conditions change no ghost value of the four sides (all zero-filled).
-int launch_accel_fix(const Box& b, int k1, int k2, bool frame, const double* up, const double* u, const double* um,
-                     double dt, double* acc, hipStream_t s);
+int launch_accel_fix(const Box& b, int k1, int k2, bool frame, bool all_planes, const double* up, const double* u,
+                     const double* um, double dt, double* acc, hipStream_t s);
 // out = cof L(w)/S (S = strx stry strz) on planes k1..k2 of the box's
 // interior columns: the predictor sweep without the time update, for the
 // transposed operator.

@@ -960,23 +960,24 @@ int launch_sweep_batch(int kind, const SweepShot* d_shots, int nshots, int nbloc
 }
 
 // The acceleration where the predictor sweep of planes k1..k2 did not form
-// it: the four planes k1-2, k1-1, k2+1, k2+2 whole and (frame) on the planes
-// k1..k2 the two-point frame of columns next to the sides.  Thread q runs
+// it: the four planes k1-2, k1-1, k2+1, k2+2 whole (all_planes: every plane
+// outside k1..k2) and (frame) on the planes k1..k2 the two-point frame of
+// columns next to the sides.  Thread q runs
 // over the planes' points first, then over the frame's columns (the rows
 // jb, jb+1, je-1, je whole, then the four side columns of each row between,
 // on consecutive threads, so that a row's two cache lines are read once).
-__global__ void k_accel_fix(Box b, int k1, int k2, bool frame, const double* __restrict__ up,
+__global__ void k_accel_fix(Box b, int k1, int k2, int nlo, int nhi, bool frame, const double* __restrict__ up,
                             const double* __restrict__ u, const double* __restrict__ um, double dt2i,
                             double* __restrict__ acc)
 {
    const long ni = b.ni, nj = b.je - b.jb + 1, nij = b.nij, npts = b.npts;
-   const long nplane = 4 * nij;
+   const long nplane = (long)(nlo + nhi) * nij;  // planes k1-nlo..k1-1 and k2+1..k2+nhi
    const long nframe = 4 * (ni + nj - 4);  // columns of the frame in one plane
    const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
    long p;
    if (q < nplane) {
       const int s = (int)(q / nij);
-      const int k = s < 2 ? k1 - 2 + s : k2 - 1 + s;
+      const int k = s < nlo ? k1 - nlo + s : k2 + 1 + (s - nlo);
       p = nij * (k - b.kb) + q % nij;
    } else {
       const long r = q - nplane;
@@ -999,14 +1000,15 @@ __global__ void k_accel_fix(Box b, int k1, int k2, bool frame, const double* __r
    for (int c = 0; c < 3; c++) acc[p + npts * c] = accel_of(up[p + npts * c], u[p + npts * c], um[p + npts * c], dt2i);
 }
 
-int launch_accel_fix(const Box& b, int k1, int k2, bool frame, const double* up, const double* u, const double* um,
-                     double dt, double* acc, hipStream_t s)
+int launch_accel_fix(const Box& b, int k1, int k2, bool frame, bool all_planes, const double* up, const double* u,
+                     const double* um, double dt, double* acc, hipStream_t s)
 {
-   if (k2 < k1) return OK;
+   if (k2 < k1) return set_error(E_INVAL, "acceleration fix: no sweep planes");
    if (k1 - 2 < b.kb || k2 + 2 > b.ke || b.ni < 5 || b.je - b.jb + 1 < 5)
       return set_error(E_INVAL, "acceleration fix: planes or box out of range");
-   const long n = 4 * b.nij + (frame ? 4 * (b.ni + (b.je - b.jb + 1) - 4) * (long)(k2 - k1 + 1) : 0);
-   k_accel_fix<<<cdiv(n, 256), 256, 0, s>>>(b, k1, k2, frame, up, u, um, 1.0 / (dt * dt), acc);
+   const int nlo = all_planes ? k1 - b.kb : 2, nhi = all_planes ? b.ke - k2 : 2;
+   const long n = (long)(nlo + nhi) * b.nij + (frame ? 4 * (b.ni + (b.je - b.jb + 1) - 4) * (long)(k2 - k1 + 1) : 0);
+   k_accel_fix<<<cdiv(n, 256), 256, 0, s>>>(b, k1, k2, nlo, nhi, frame, up, u, um, 1.0 / (dt * dt), acc);
    return check_hip(hipGetLastError(), "acceleration fix launch");
 }
 

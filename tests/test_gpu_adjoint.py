@@ -267,6 +267,28 @@ def test_gradient_independent_of_checkpoints(torch):
         assert np.array_equal(a, c) and np.array_equal(b, c)
 
 
+@pytest.mark.parametrize("bc", [[2, 2, 2, 2, 0, 2], [1, 1, 2, 2, 2, 0]])
+def test_gradient_stored_acceleration_bitwise(torch, bc):
+    """Kernel variant 8 lets the recomputed predictor of every reverse step
+    store its acceleration (sweep, ghost shell, then the planes outside the
+    sweep, the side frame where the boundary conditions change it and the
+    source points) instead of a separate pass over the whole box: the
+    gradients must be variant 6's to the last bit."""
+    nx, ny, nz, h, nsteps = 20, 18, 16, 0.1, 12
+    rng = np.random.default_rng(47)
+    gtr = rng.standard_normal((nsteps + 1, len(RECS), 3))
+    res = []
+    for variant in (6, 8):
+        e, mat = make_ew(nx, ny, nz, h, bc, nsteps)
+        e.set_kernel_variant(variant, 7)
+        res.append(e.gradient(gtr, nsteps, ckpt=4))
+        e.close()
+    ndiff = [int(np.count_nonzero(a != b)) for a, b in zip(*res)]
+    print("gradient, stored acceleration: bc", bc, "points differing from variant 6 (rho, mu, lambda):", ndiff,
+          flush=True)
+    assert np.max(np.abs(res[0][0])) > 0 and ndiff == [0, 0, 0]
+
+
 def test_autograd_function(torch):
     """torch.autograd through sw4lite_amd.adjoint.seismograms: the backward
     pass equals the C-ABI gradient, and a gradient step lowers the misfit."""
