@@ -390,7 +390,8 @@ def bench_fp32(a, e, torch, n, fp64_value):
         "stage_ms_per_step": {k: v / a.steps for k, v in zip(STAGES, stage_ms.tolist())},
         "pred_interior_GBps": corr_pts * COMP_PRED_BYTES_PER_PT / (pred_ms * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "kernel": "k_comp_sweep<true>",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "kernel": "k_comp_dma<true>" if a.variant >= 7 and (n + 4) % 4 == 0 else "k_comp_sweep<true>",
                      "bytes_per_launch": corr_pts * COMP_CORR_BYTES_PER_PT, "avg_launch_ms": corr_ms},
     }
 
