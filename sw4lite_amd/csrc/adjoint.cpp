@@ -37,21 +37,31 @@ int EW::reset()
    return record(mU, mUm);
 }
 
-int EW::fused_once(double t)
+int EW::fused_once(double t, double* acc)
 {
    const GridSetup& g = m_opt.grid;
    const SgCoef c = sg();
    const SourceTable st = sources();
    const double dt = mDt;
    const int zf = zfill();
+   // kernel variant 8: the acceleration stored in acc (a work field) as in
+   // the time-step loop
+   int ka = 0, kz = 0;
+   fused_planes(m_box, g.nz, m_onesided, &ka, &kz);
+   double* af = acc && m_variant == 8 && m_box.ni % 2 == 0 && kz >= ka ? acc : nullptr;
    int rc;
    if ((rc = launch_pred_fused(m_box, g.nz, m_onesided, m_dsbp, mU.dev, mUm.dev, mMu.dev, mLambda.dev, mRho.dev, c,
-                               mUp.dev, g.h, dt, m_stream, nullptr, m_variant, m_kchunk, zf)) ||
+                               mUp.dev, g.h, dt, m_stream, nullptr, m_variant, m_kchunk, zf, af)) ||
        (rc = launch_src_apply(m_box, st, t, 0, dt * dt, mRho.dev, mUp.dev, m_stream)) ||
-       (rc = enforceBC(mUp, mMu, mLambda, t + dt, zf)) ||
-       (rc = launch_corr_fused(m_box, g.nz, m_onesided, m_dsbp, mUp.dev, mU.dev, mUm.dev, mMu.dev, mLambda.dev,
+       (rc = enforceBC(mUp, mMu, mLambda, t + dt, zf)))
+      return rc;
+   if (af && ((rc = launch_accel_fix(m_box, ka, kz, (zf & 15) != 15, false, mUp.dev, mU.dev, mUm.dev, dt, af,
+                                     m_stream)) ||
+              (rc = launch_accel_points(m_box, st, mUp.dev, mU.dev, mUm.dev, dt, af, m_stream))))
+      return rc;
+   if ((rc = launch_corr_fused(m_box, g.nz, m_onesided, m_dsbp, mUp.dev, mU.dev, mUm.dev, mMu.dev, mLambda.dev,
                                mRho.dev, c, m_opt.beta, mUp2.dev, m_dacc, g.h, dt, m_stream, nullptr, nullptr, m_variant,
-                               m_kchunk, zf)) ||
+                               m_kchunk, zf, af)) ||
        (rc = launch_src_apply(m_box, st, t, 1, dt * dt * dt * dt / 12, mRho.dev, mUp2.dev, m_stream)) ||
        (rc = enforceBC(mUp2, mMu, mLambda, t + dt, zf)))
       return rc;
@@ -326,7 +336,7 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
          mUm.dev = seg[m - 1];
          mUp.dev = tmp[0];
          mUp2.dev = seg[m + 1];
-         e = fused_once((s0 + m - 1) * mDt);
+         e = fused_once((s0 + m - 1) * mDt, tmp[1]);  // tmp[1]: the reverse step's acceleration, free here
       }
       mU.dev = sU;
       mUm.dev = sUm;

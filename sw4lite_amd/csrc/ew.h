@@ -156,7 +156,9 @@ private:
    int ensure_reference_arrays();
    SgCoef sg() const;
    SourceTable sources() const;
-   int fused_once(double t);  // one fused step U, Um -> Up2 (scratch Up), no rotation
+   // one fused step U, Um -> Up2 (scratch Up, acc: a work field for the stored
+   // acceleration of kernel variant 8, or null), no rotation
+   int fused_once(double t, double* acc = nullptr);
    int step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double* lam_b, double* lam_c,
                 double* grho, double* gmu, double* gla, double* const* tmp, double* plane, const AdjPad& pad);
    int comp_begin();
