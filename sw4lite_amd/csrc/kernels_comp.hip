@@ -317,10 +317,12 @@ k_comp_sweep(Box b, CompTiles map, int kbeg, int kend, int kc, const float* __re
 // column values one plane ahead.  Rings: the staged field in six slots
 // (planes k-2..k+3), D in four (k..k+3), mu, lambda (and rho) in two.  Same
 // arithmetic in the same order as k_comp_sweep.  Needs ni % 4 == 0 and
-// 16-byte aligned arrays (every quad of floats in one row, aligned).
+// 16-byte aligned arrays (every quad of floats in one row, aligned).  The
+// predictor runs at 4 waves/SIMD (128 registers, 2 of them spilled: faster
+// than 3 waves without the spill).
 #define SW4_LDSF(p) ((__attribute__((address_space(3))) void*)(p))
 template <bool CORR>
-__global__ void __launch_bounds__(NT, 2)
+__global__ void __launch_bounds__(NT, CORR ? 2 : 4)
 k_comp_dma(Box b, CompTiles map, int kbeg, int kend, int kc, const float* __restrict__ stg,
            const float* __restrict__ mu, const float* __restrict__ la, const float* __restrict__ rho, SgCoefF sg,
            const float* __restrict__ dold, float* __restrict__ dnew, float* __restrict__ uu,
