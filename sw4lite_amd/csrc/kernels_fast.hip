@@ -1063,7 +1063,7 @@ __global__ void __launch_bounds__(NT, 1)
 k_sbp_rows(Box b, int top, int nk, int ntx, const SbpTables* __restrict__ sbp, const double* __restrict__ upin,
            const double* __restrict__ u, const double* __restrict__ um, const double* __restrict__ mu,
            const double* __restrict__ la, const double* __restrict__ rho, SgCoef sg, double* __restrict__ out,
-           double cof, double dt2i, double cupd, double beta)
+           double cof, double dt2i, double cupd, double beta, bool dma)
 {
    constexpr int NU = (3 * PL + NT - 1) / NT;
    constexpr int NM = (PL + NT - 1) / NT;
@@ -1079,9 +1079,43 @@ k_sbp_rows(Box b, int top, int nk, int ntx, const SbpTables* __restrict__ sbp, c
    const int kbase = top ? 1 : nk;
    const int sgn = top ? 1 : -1;
    const int o = (ty + 2) * HX + tx + 2;
-   // stage: displacement (or acceleration) planes q = 0..8, material 1..8
+   // stage: displacement (or acceleration) planes q = 0..8, material 1..8.
+   // The predictor's raw planes go by asynchronous global-to-LDS loads when
+   // the rows allow it (dma: even ni, aligned arrays), all issued before one
+   // wait; pairs off the box load the plane's first point (read only by
+   // inactive points, as every staged value off the box).
+   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+   if (!CORR && dma) {
+      constexpr int PP = 3 * PL / 2, NP = (PP + NT - 1) / NT, MP = PL / 2;
+      long goff[NP], moff = 0;
+#pragma unroll
+      for (int p = 0; p < NP; p++) {
+         const int e = 2 * (tid + p * NT);
+         const int c = e / PL, r = e - c * PL, y = r / HX, x = r - y * HX;
+         const int gi = ia + x, gj = ja + y;
+         goff[p] = e < 3 * PL && gi <= b.ie && gj <= b.je ? (gi - b.ib) + ni * (gj - b.jb) + npts * c : 0;
+      }
+      {
+         const int e = 2 * tid, y = e / HX, x = e - y * HX;
+         const int gi = ia + x, gj = ja + y;
+         moff = e < PL && gi <= b.ie && gj <= b.je ? (gi - b.ib) + ni * (gj - b.jb) : 0;
+      }
 #pragma unroll 1
-   for (int q = 0; q <= 8; q++) {
+      for (int q = 0; q <= 8; q++) {
+         const long pb = nij * (kbase + sgn * (q - 1) - b.kb);
+         double* dst = su + q * 3 * PL + 2 * 64 * wv;
+#pragma unroll
+         for (int p = 0; p < NP; p++)
+            if (tid + p * NT < PP)
+               __builtin_amdgcn_global_load_lds(u + pb + goff[p], SW4_LDS(dst + 2 * p * NT), 16, 0, 0);
+         if (q >= 1 && tid < MP) {
+            __builtin_amdgcn_global_load_lds(mu + pb + moff, SW4_LDS(sm + (q - 1) * PL + 2 * 64 * wv), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(la + pb + moff, SW4_LDS(sl + (q - 1) * PL + 2 * 64 * wv), 16, 0, 0);
+         }
+      }
+   }
+#pragma unroll 1
+   for (int q = 0; q <= 8 && (CORR || !dma); q++) {
       const long pb = nij * (kbase + sgn * (q - 1) - b.kb);
 #pragma unroll
       for (int p = 0; p < NU; p++) {
@@ -1149,10 +1183,10 @@ int launch_sbp_rows(bool corr, const Box& b, int side, int nk, const SbpTables* 
    const double cof = 1.0 / (h * h), dt2i = 1.0 / (dt * dt);
    if (corr)
       k_sbp_rows<true><<<ntx * nty, NT, 0, s>>>(b, side == 4, nk, ntx, d_sbp, upin, u, um, mu, la, rho, sg, out, cof,
-                                                dt2i, dt * dt * dt * dt / 12, beta);
+                                                dt2i, dt * dt * dt * dt / 12, beta, false);
    else
       k_sbp_rows<false><<<ntx * nty, NT, 0, s>>>(b, side == 4, nk, ntx, d_sbp, nullptr, u, um, mu, la, rho, sg, out,
-                                                 cof, dt2i, dt * dt, beta);
+                                                 cof, dt2i, dt * dt, beta, dma_fits(b, u, mu, la));
    return check_hip(hipGetLastError(), "SBP rows launch");
 }
 
