@@ -55,7 +55,7 @@ int EW::fused_once(double t, double* acc)
        (rc = launch_src_apply(m_box, st, t, 0, dt * dt, mRho.dev, mUp.dev, m_stream)) ||
        (rc = enforceBC(mUp, mMu, mLambda, t + dt, zf)))
       return rc;
-   if (af && ((rc = launch_accel_fix(m_box, ka, kz, (zf & 15) != 15, false, mUp.dev, mU.dev, mUm.dev, dt, af,
+   if (af && ((rc = launch_accel_fix(m_box, ka, kz, (zf & 15) != 15, true, mUp.dev, mU.dev, mUm.dev, dt, af,
                                      m_stream)) ||
               (rc = launch_accel_points(m_box, st, mUp.dev, mU.dev, mUm.dev, dt, af, m_stream))))
       return rc;

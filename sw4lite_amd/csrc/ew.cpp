@@ -498,7 +498,8 @@ int EW::step_fused(int n, double* stage_ms)
       if (!accf) return OK;
       // the shell's values of the side frame are final when every side is zero-filled
       const bool frame = (zf & 15) != 15;
-      const int r = launch_accel_fix(m_box, ka, kz, frame, false, mUp.dev, mU.dev, mUm.dev, dt, accf, m_stream);
+      // every plane outside the sweep: the corrector's SBP rows stage it too
+      const int r = launch_accel_fix(m_box, ka, kz, frame, true, mUp.dev, mU.dev, mUm.dev, dt, accf, m_stream);
       return r ? r : launch_accel_points(m_box, st, mUp.dev, mU.dev, mUm.dev, dt, accf, m_stream);
    };
    for (int s = 0; s < n && rc == OK; s++) {

@@ -51,7 +51,8 @@ bool async_loads_fit(const Box& b, const double* const* arrays, int n);
 // LDS slab (corr: with the acceleration and the supergrid damping).
 int launch_sbp_rows(bool corr, const Box& b, int side, int nk, const SbpTables* d_sbp, const double* upin,
                     const double* u, const double* um, const double* mu, const double* la, const double* rho,
-                    const SgCoef& sg, double* out, double h, double dt, double beta, hipStream_t s);
+                    const SgCoef& sg, double* out, double h, double dt, double beta, hipStream_t s,
+                    const double* acc = nullptr);
 // up -= supergrid damping(u, um) on planes k1..k2, k-marching LDS sweep.
 int launch_damp_lds(const Box& b, int k1, int k2, int kc, double* up, const double* u, const double* um,
                     const double* rho, const SgCoef& sg, double beta, hipStream_t s);

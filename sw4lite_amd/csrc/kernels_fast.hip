@@ -1063,7 +1063,7 @@ __global__ void __launch_bounds__(NT, 1)
 k_sbp_rows(Box b, int top, int nk, int ntx, const SbpTables* __restrict__ sbp, const double* __restrict__ upin,
            const double* __restrict__ u, const double* __restrict__ um, const double* __restrict__ mu,
            const double* __restrict__ la, const double* __restrict__ rho, SgCoef sg, double* __restrict__ out,
-           double cof, double dt2i, double cupd, double beta, bool dma)
+           double cof, double dt2i, double cupd, double beta, bool dma, const double* __restrict__ acc)
 {
    constexpr int NU = (3 * PL + NT - 1) / NT;
    constexpr int NM = (PL + NT - 1) / NT;
@@ -1085,7 +1085,10 @@ k_sbp_rows(Box b, int top, int nk, int ntx, const SbpTables* __restrict__ sbp, c
    // wait; pairs off the box load the plane's first point (read only by
    // inactive points, as every staged value off the box).
    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-   if (!CORR && dma) {
+   // (the corrector's planes too when the acceleration is stored: acc)
+   const bool async = dma && (!CORR || acc);
+   const double* __restrict__ src = CORR ? acc : u;
+   if (async) {
       constexpr int PP = 3 * PL / 2, NP = (PP + NT - 1) / NT, MP = PL / 2;
       long goff[NP], moff = 0;
 #pragma unroll
@@ -1107,7 +1110,7 @@ k_sbp_rows(Box b, int top, int nk, int ntx, const SbpTables* __restrict__ sbp, c
 #pragma unroll
          for (int p = 0; p < NP; p++)
             if (tid + p * NT < PP)
-               __builtin_amdgcn_global_load_lds(u + pb + goff[p], SW4_LDS(dst + 2 * p * NT), 16, 0, 0);
+               __builtin_amdgcn_global_load_lds(src + pb + goff[p], SW4_LDS(dst + 2 * p * NT), 16, 0, 0);
          if (q >= 1 && tid < MP) {
             __builtin_amdgcn_global_load_lds(mu + pb + moff, SW4_LDS(sm + (q - 1) * PL + 2 * 64 * wv), 16, 0, 0);
             __builtin_amdgcn_global_load_lds(la + pb + moff, SW4_LDS(sl + (q - 1) * PL + 2 * 64 * wv), 16, 0, 0);
@@ -1115,7 +1118,7 @@ k_sbp_rows(Box b, int top, int nk, int ntx, const SbpTables* __restrict__ sbp, c
       }
    }
 #pragma unroll 1
-   for (int q = 0; q <= 8 && (CORR || !dma); q++) {
+   for (int q = 0; q <= 8 && !async; q++) {
       const long pb = nij * (kbase + sgn * (q - 1) - b.kb);
 #pragma unroll
       for (int p = 0; p < NU; p++) {
@@ -1176,17 +1179,17 @@ k_sbp_rows(Box b, int top, int nk, int ntx, const SbpTables* __restrict__ sbp, c
 
 int launch_sbp_rows(bool corr, const Box& b, int side, int nk, const SbpTables* d_sbp, const double* upin,
                     const double* u, const double* um, const double* mu, const double* la, const double* rho,
-                    const SgCoef& sg, double* out, double h, double dt, double beta, hipStream_t s)
+                    const SgCoef& sg, double* out, double h, double dt, double beta, hipStream_t s, const double* acc)
 {
    const long nx = b.ie - b.ib - 3, ny = b.je - b.jb - 3;
    const int ntx = (int)cdiv(nx, TX), nty = (int)cdiv(ny, TY);
    const double cof = 1.0 / (h * h), dt2i = 1.0 / (dt * dt);
    if (corr)
       k_sbp_rows<true><<<ntx * nty, NT, 0, s>>>(b, side == 4, nk, ntx, d_sbp, upin, u, um, mu, la, rho, sg, out, cof,
-                                                dt2i, dt * dt * dt * dt / 12, beta, false);
+                                                dt2i, dt * dt * dt * dt / 12, beta, acc && dma_fits(b, acc, mu, la), acc);
    else
       k_sbp_rows<false><<<ntx * nty, NT, 0, s>>>(b, side == 4, nk, ntx, d_sbp, nullptr, u, um, mu, la, rho, sg, out,
-                                                 cof, dt2i, dt * dt, beta, dma_fits(b, u, mu, la));
+                                                 cof, dt2i, dt * dt, beta, dma_fits(b, u, mu, la), nullptr);
    return check_hip(hipGetLastError(), "SBP rows launch");
 }
 

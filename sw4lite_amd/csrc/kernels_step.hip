@@ -574,11 +574,11 @@ int launch_pred_tail(const Box& b, int nk, const int* onesided, const SbpTables*
 int launch_corr_tail(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp, const double* up,
                      const double* u, const double* um, const double* mu, const double* la, const double* rho,
                      const SgCoef& sg, double beta, double* upo, double h, double dt, hipStream_t s,
-                     int zfill)
+                     int zfill, const double* accf)
 {
-   for (int side = 4; side <= 5; side++) {  // LDS slab, acceleration formed while staging
+   for (int side = 4; side <= 5; side++) {  // LDS slab, acceleration formed while staging (or stored: accf)
       if (onesided[side] != 1) continue;
-      const int rc = launch_sbp_rows(true, b, side, nk, d_sbp, up, u, um, mu, la, rho, sg, upo, h, dt, beta, s);
+      const int rc = launch_sbp_rows(true, b, side, nk, d_sbp, up, u, um, mu, la, rho, sg, upo, h, dt, beta, s, accf);
       if (rc) return rc;
    }
    const long ns = shell_count(b);
@@ -622,7 +622,7 @@ int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables
       if (rc) return rc;
    }
    if (variant >= 1)
-      return launch_corr_tail(b, nk, onesided, d_sbp, up, u, um, mu, la, rho, sg, beta, upo, h, dt, s, zfill);
+      return launch_corr_tail(b, nk, onesided, d_sbp, up, u, um, mu, la, rho, sg, beta, upo, h, dt, s, zfill, accf);
    for (int side = 4; side <= 5; side++) {
       if (onesided[side] != 1) continue;
       const int ks = side == 4 ? 0 : nk - SLAB_PLANES + 2;  // planes KK(0)..KK(8) of the rows
