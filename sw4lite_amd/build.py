@@ -1,7 +1,8 @@
 """Build the gfx950 shared library ``sw4lite_amd/libsw4hip.so`` in-tree.
 
-All HIP and host sources are compiled by one ``hipcc`` invocation for
-``--offload-arch=gfx950``.  Packed FP32 instructions are disabled
+Every HIP and host source is compiled by ``hipcc`` for
+``--offload-arch=gfx950`` into an object of its own (in parallel, under
+``sw4lite_amd/_build/``), then the objects are linked into the library.  Packed FP32 instructions are disabled
 (``-Xclang -target-feature -Xclang -packed-fp32-ops``).  Run as
 ``python -m sw4lite_amd.build`` or through ``__graft_entry__.build()``.
 """
@@ -34,7 +35,6 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
@@ -47,17 +47,33 @@ def hipcc():
     return exe
 
 
-def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + FLAGS + ["-I", CSRC, "-I", os.path.join(os.path.dirname(HERE), "include"),
-                               "-o", tmp] + srcs
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
+    return res.returncode
+
+
+def build(verbose=False, jobs=None):
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    inc = ["-I", CSRC, "-I", os.path.join(os.path.dirname(HERE), "include")]
+    cmds, objs = [], []
+    for s in SOURCES:
+        obj = os.path.join(objdir, os.path.splitext(s)[0] + ".o")
+        objs.append(obj)
+        cmds.append([hipcc()] + FLAGS + inc + ["-c", "-o", obj, os.path.join(CSRC, s)])
+    jobs = jobs or min(len(cmds), max(1, os.cpu_count() or 1), 16)
+    with ThreadPoolExecutor(jobs) as pool:
+        rcs = list(pool.map(lambda c: _run(c, verbose), cmds))
+    if any(rcs):
         raise RuntimeError("hipcc failed building libsw4hip.so")
+    tmp = LIB + ".tmp"
+    if _run([hipcc()] + FLAGS + ["-shared", "-o", tmp] + objs, verbose):
+        raise RuntimeError("hipcc failed linking libsw4hip.so")
     os.replace(tmp, LIB)
     return LIB
 

@@ -23,13 +23,17 @@ void Taper::define(bool l, double start, bool r, double end, double w)
    trans = 0.5 * w;  // full transition, no constant-damping core (SuperGrid.C:62-67)
 }
 
-// C5 step 0 -> 1 on [0,1] (SuperGrid.C:165-187)
+// C5 step 0 -> 1 on [0,1] (SuperGrid.C:165-187).  The products are formed
+// left to right as the reference forms them: near psi = 1 the stretching
+// 1 - (1 - eps) psi cancels to ~1e-4, so a different grouping of the
+// polynomial changes the profiles in their ninth digit.
 double Taper::psi0(double xi)
 {
    if (xi <= 0.) return 0.;
    if (xi >= 1.) return 1.;
-   const double x2 = xi * xi, x3 = x2 * xi, x6 = x3 * x3;
-   return x6 * (462 - 1980 * xi + 3465 * x2 - 3080 * x3 + 1386 * x2 * x2 - 252 * x2 * x3);
+   const double p6 = xi * xi * xi * xi * xi * xi;
+   return p6 * (462 - 1980 * xi + 3465 * xi * xi - 3080 * xi * xi * xi + 1386 * xi * xi * xi * xi -
+                252 * xi * xi * xi * xi * xi);
 }
 
 double Taper::ramp(double x, double w) const
