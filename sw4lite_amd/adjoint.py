@@ -15,7 +15,10 @@ import torch
 class Seismograms(torch.autograd.Function):
     @staticmethod
     def forward(ctx, rho, mu, la, ew, nsteps, ckpt):
-        ew.set_material(rho.detach().contiguous(), mu.detach().contiguous(), la.detach().contiguous())
+        if not (rho.is_cuda and mu.is_cuda and la.is_cuda):
+            raise ValueError("rho, mu and lambda must be device tensors")
+        f64 = [x.detach().to(torch.float64).contiguous() for x in (rho, mu, la)]
+        ew.set_material(*f64)
         ew.sync()
         ew.reset()
         ew.timesteploop(nsteps, mode=0)
@@ -28,11 +31,13 @@ class Seismograms(torch.autograd.Function):
     def backward(ctx, grad):
         rho, mu, la = ctx.saved_tensors
         ew = ctx.ew
-        ew.set_material(rho.detach().contiguous(), mu.detach().contiguous(), la.detach().contiguous())
+        ew.set_material(*[x.detach().to(torch.float64).contiguous() for x in (rho, mu, la)])
         ew.sync()
-        out = [torch.empty_like(rho), torch.empty_like(mu), torch.empty_like(la)]
+        n = ew.npoints()
+        out = [torch.empty(n, dtype=torch.float64, device=rho.device) for _ in range(3)]
         ew.gradient(grad.detach().to("cpu", torch.float64).numpy(), ctx.nsteps, ctx.ckpt, out=out)
-        return out[0], out[1], out[2], None, None, None
+        return (out[0].view_as(rho).to(rho.dtype), out[1].view_as(mu).to(mu.dtype), out[2].view_as(la).to(la.dtype),
+                None, None, None)
 
 
 def seismograms(rho, mu, la, ew, nsteps, ckpt=0):

@@ -16,16 +16,19 @@ namespace sw4amd {
 namespace {
 thread_local std::string g_last_error;
 
-struct SbpCache {
+// One immutable device copy per distinct SBP table set and device: a buffer
+// is written once, when it is created, and never again, so kernels still
+// running on a caller's stream keep reading the tables they were launched
+// with whatever tables later calls bring.
+struct SbpEntry {
    SbpTables host;
-   SbpTables* dev = nullptr;
-   bool valid = false;
+   SbpTables* dev;
 };
+constexpr int kMaxSbpSets = 32;  // distinct table sets per device
 std::mutex g_sbp_mutex;
-SbpCache g_sbp_cache[64];
+std::vector<SbpEntry> g_sbp_cache[64];
 
-// Device copy of an SBP table set on the current device; re-uploaded only
-// when the caller's tables differ from the cached ones.
+// Device copy of an SBP table set on the current device.
 int device_sbp(const SbpTables& t, const SbpTables** out)
 {
    int dev = 0;
@@ -33,18 +36,22 @@ int device_sbp(const SbpTables& t, const SbpTables** out)
    if (rc) return rc;
    if (dev < 0 || dev >= 64) return set_error(E_RANGE, "device ordinal beyond 63");
    std::lock_guard<std::mutex> lock(g_sbp_mutex);
-   SbpCache& c = g_sbp_cache[dev];
-   if (!c.dev) {
-      void* p = nullptr;
-      if (hipMalloc(&p, sizeof(SbpTables)) != hipSuccess) return set_error(E_NOMEM, "SBP table allocation failed");
-      c.dev = static_cast<SbpTables*>(p);
+   std::vector<SbpEntry>& c = g_sbp_cache[dev];
+   for (const SbpEntry& e : c)
+      if (std::memcmp(&e.host, &t, sizeof(t)) == 0) {
+         *out = e.dev;
+         return OK;
+      }
+   if ((int)c.size() >= kMaxSbpSets)
+      return set_error(E_RANGE, "more than 32 distinct SBP coefficient table sets on one device");
+   void* p = nullptr;
+   if (hipMalloc(&p, sizeof(SbpTables)) != hipSuccess) return set_error(E_NOMEM, "SBP table allocation failed");
+   if ((rc = check_hip(hipMemcpy(p, &t, sizeof(t), hipMemcpyHostToDevice), "SBP table upload"))) {
+      hipFree(p);
+      return rc;
    }
-   if (!c.valid || std::memcmp(&c.host, &t, sizeof(t)) != 0) {
-      if ((rc = check_hip(hipMemcpy(c.dev, &t, sizeof(t), hipMemcpyHostToDevice), "SBP table upload"))) return rc;
-      c.host = t;
-      c.valid = true;
-   }
-   *out = c.dev;
+   c.push_back(SbpEntry{t, static_cast<SbpTables*>(p)});
+   *out = c.back().dev;
    return OK;
 }
 

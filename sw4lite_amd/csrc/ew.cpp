@@ -489,10 +489,11 @@ int EW::step_fused(int n, double* stage_ms)
       return stage_ms ? check_hip(hipEventRecord(ev[(size_t)s * (kStages + 1) + q], m_stream), "hipEventRecord") : OK;
    };
    const int zf = zfill();
-   double* accf = nullptr;  // variant 8: the acceleration stored by the predictor, staged by the corrector
-   if (rc == OK) rc = accel_field(&accf);
    int ka = 0, kz = 0;
    fused_planes(m_box, g.nz, m_onesided, &ka, &kz);
+   double* accf = nullptr;  // variant 8: the acceleration stored by the predictor, staged by the corrector
+   // (a grid whose SBP rows leave the sweep no plane has nothing to store it)
+   if (rc == OK && kz >= ka) rc = accel_field(&accf);
    // completes the stored acceleration from the final Up (after sources and BC)
    auto accel_rest = [&](double dt) -> int {
       if (!accf) return OK;

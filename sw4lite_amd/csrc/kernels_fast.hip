@@ -1067,8 +1067,12 @@ k_sbp_rows(Box b, int top, int nk, int ntx, const SbpTables* __restrict__ sbp, c
 {
    constexpr int NU = (3 * PL + NT - 1) / NT;
    constexpr int NM = (PL + NT - 1) / NT;
-   __shared__ double su[9 * 3 * PL];  // planes q = 0..8, components
-   __shared__ double sm[8 * PL], sl[8 * PL];
+   // destinations of 16-byte asynchronous loads: every staged plane starts
+   // on a 16-byte boundary
+   static_assert((PL * sizeof(double)) % 16 == 0, "staged planes must stay 16-byte aligned");
+   __shared__ __align__(16) double su[9 * 3 * PL];  // planes q = 0..8, components
+   __shared__ __align__(16) double sm[8 * PL];
+   __shared__ __align__(16) double sl[8 * PL];
    __shared__ double ssx[HX], ssy[HY];
    const int tix = blockIdx.x % ntx, tjy = blockIdx.x / ntx;
    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;

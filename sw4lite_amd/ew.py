@@ -35,6 +35,43 @@ def _is_dev(a):
     return hasattr(a, "is_cuda") and a.is_cuda
 
 
+def _fields(n, *arrays):
+    """Check arrays handed to an EW method that copies n float64 values from
+    or into each: float64, exactly n elements, contiguous, and all of them on
+    one side (all device tensors or all host arrays).  Returns whether they
+    are on the device.  Raises ValueError otherwise."""
+    on_dev = None
+    for a in arrays:
+        if hasattr(a, "data_ptr"):
+            import torch
+            if a.dtype != torch.float64:
+                raise ValueError("expected a float64 tensor, got %s" % a.dtype)
+            if a.numel() != n:
+                raise ValueError("expected %d elements, got %d" % (n, a.numel()))
+            if not a.is_contiguous():
+                raise ValueError("tensor must be contiguous")
+            dev = bool(a.is_cuda)
+        elif isinstance(a, np.ndarray):
+            if a.dtype != np.float64:
+                raise ValueError("expected a float64 array, got %s" % a.dtype)
+            if a.size != n:
+                raise ValueError("expected %d elements, got %d" % (n, a.size))
+            if not a.flags["C_CONTIGUOUS"]:
+                raise ValueError("array must be C-contiguous")
+            dev = False
+        else:
+            raise TypeError("expected a torch tensor or numpy array")
+        if on_dev is None:
+            on_dev = dev
+        elif on_dev != dev:
+            raise ValueError("arrays must all be device tensors or all host arrays")
+    if on_dev:
+        devs = {a.device for a in arrays}
+        if len(devs) != 1:
+            raise ValueError("device tensors must be on one device, got %s" % sorted(map(str, devs)))
+    return bool(on_dev)
+
+
 def _stream(stream):
     if stream is not None:
         return C.c_void_p(stream)
@@ -245,8 +282,13 @@ class EW:
         otherwise as 7))."""
         L.check(self.lib.sw4hip_ew_set_kernel_variant(self.h, int(variant), int(kchunk)))
 
+    def npoints(self):
+        """Points of the box (two ghost points per side): the length of a
+        scalar field, a third of a displacement field."""
+        return int(np.prod(self.shape))
+
     def set_material(self, rho, mu, la):
-        on_dev = _is_dev(rho)
+        on_dev = _fields(self.npoints(), rho, mu, la)
         L.check(self.lib.sw4hip_ew_set_material(self.h, _ptr(rho), _ptr(mu), _ptr(la), int(on_dev)))
 
     def set_material_const(self, rho, cs, cp):
@@ -304,7 +346,7 @@ class EW:
 
     # state ---------------------------------------------------------------
     def set_solution(self, u, um):
-        on_dev = _is_dev(u)
+        on_dev = _fields(3 * self.npoints(), u, um)
         L.check(self.lib.sw4hip_ew_set_solution(self.h, _ptr(u), _ptr(um), int(on_dev)))
 
     def get_solution(self):
@@ -322,7 +364,8 @@ class EW:
     def set_topography(self, z):
         """Curvilinear grid of depth z over the box (numpy array or device
         tensor); before setup and receivers."""
-        L.check(self.lib.sw4hip_ew_set_topography(self.h, _ptr(z), int(_is_dev(z))))
+        on_dev = _fields(self.npoints(), z)
+        L.check(self.lib.sw4hip_ew_set_topography(self.h, _ptr(z), int(on_dev)))
 
     def timesteploop(self, nsteps=None, mode=0):
         """Advance nsteps (default: all remaining) steps; mode 0 fused sweeps,
@@ -358,12 +401,15 @@ class EW:
         g = np.ascontiguousarray(np.asarray(grad_traces, dtype=np.float64))
         if nsteps is None:
             nsteps = g.shape[0] - 1
+        if g.ndim != 3 or g.shape[0] < nsteps + 1 or g.shape[2] != 3:
+            raise ValueError("grad_traces must be [nsteps+1, nreceivers, 3], got %s" % (g.shape,))
         if out is None:
-            n = int(np.prod(self.shape))
+            n = self.npoints()
             res = [np.empty(n), np.empty(n), np.empty(n)]
             on_dev = 0
         else:
-            res, on_dev = list(out), 1
+            res = list(out)
+            on_dev = int(_fields(self.npoints(), *res))
         L.check(self.lib.sw4hip_ew_gradient(self.h, int(nsteps), int(ckpt), _ptr(g), _ptr(res[0]), _ptr(res[1]),
                                             _ptr(res[2]), on_dev))
         return tuple(res)

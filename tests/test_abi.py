@@ -55,3 +55,40 @@ def test_kernel_entry_rejects_bad_layout():
     rc = lib.sw4hip_predfort(2, -1, 10, -1, 10, -1, 10, None, None, None, None, None, None, 1.0, None)
     assert rc == 1
     assert b"corder" in lib.sw4hip_last_error()
+
+
+def test_field_checks_of_the_ew_bindings():
+    """EW.set_material/set_solution/gradient copy n float64 values per array:
+    other dtypes, sizes, layouts or mixed host/device inputs are refused
+    before any pointer reaches the library."""
+    import torch
+    n = 60
+    ok = np.zeros(n)
+    assert ew._fields(n, ok, ok.copy()) is False
+    assert ew._fields(n, torch.zeros(n, dtype=torch.float64)) is False
+    with pytest.raises(ValueError, match="float64"):
+        ew._fields(n, np.zeros(n, dtype=np.float32))
+    with pytest.raises(ValueError, match="float64"):
+        ew._fields(n, torch.zeros(n))                      # torch's default float32
+    with pytest.raises(ValueError, match="elements"):
+        ew._fields(n, np.zeros(n - 1))
+    with pytest.raises(ValueError, match="contiguous"):
+        ew._fields(n, np.zeros(2 * n)[::2])
+
+    class FakeDev:  # a device tensor as far as the checks look
+        dtype = torch.float64
+        device = "cuda:0"
+        is_cuda = True
+
+        def data_ptr(self):
+            return 0
+
+        def numel(self):
+            return n
+
+        def is_contiguous(self):
+            return True
+
+    with pytest.raises(ValueError, match="all be device tensors or all host"):
+        ew._fields(n, FakeDev(), ok)
+    assert ew._fields(n, FakeDev(), FakeDev()) is True

@@ -262,6 +262,32 @@ def test_sweep_variants_match_oracle(torch, variant, kchunk, bc, nx):
     assert err < TOL and relerr(um, ref.Um) < TOL
 
 
+@pytest.mark.parametrize("variant", [7, 8])
+def test_thin_grid_free_top_and_bottom(torch, variant):
+    """nz = 12 (the smallest grid accepted) with free surfaces at top and
+    bottom: the SBP rows of both ends cover every plane and the interior
+    sweep has none, so the stored acceleration of variant 8 has nowhere to
+    be stored and the step must run as variant 7 does."""
+    from oracle.loop import OracleRun
+    from sw4lite_amd import EW
+    nx, ny, nz, h, bc = 44, 19, 12, 0.1, [2, 2, 2, 2, 0, 0]
+    rho, mu, la = layered_material(nx, ny, nz, h, 8)
+    ref = OracleRun(nx, ny, nz, h, bc, rho, mu, la, SOURCES, corder=1, sg_gp=6, nsteps=6)
+    e = EW(nx, ny, nz, h, bc, corder=1, sg_gp_thickness=6)
+    e.set_kernel_variant(variant, 64)
+    e.set_material(rho, mu, la)
+    for s in SOURCES:
+        e.add_moment_source(s["x"], s["y"], s["z"], s["m"], s["t0"], s["freq"], s["tfkind"])
+    e.setup(nsteps=6)
+    e.timesteploop(6, mode=0)
+    ref.step(6)
+    u, um = e.get_solution()
+    e.close()
+    err = relerr(u, ref.U)
+    print("thin grid nz=12 variant", variant, "relerr", err, flush=True)
+    assert np.max(np.abs(ref.U)) > 0 and err < TOL and relerr(um, ref.Um) < TOL
+
+
 # supergrid and Dirichlet sides: the ghost shell's acceleration is final;
 # periodic sides: the boundary conditions change the ghost frame afterwards
 @pytest.mark.parametrize("bc,nx,kchunk", [([2, 2, 2, 2, 0, 2], 44, 7), ([2, 2, 2, 2, 0, 0], 46, 64),
