@@ -16,11 +16,15 @@ corrector + supergrid sweep, sources, BC).  The metric is interior grid-point
 updates per second: 512^3 * steps / seconds, inputs resident in HBM.
 
 The JSON line adds
-  roofline: the dominant kernel (corrector sweep, interior points), its
-    algorithmic bytes per launch (15 FP64 words per point: read Up, U, Um (3
-    components each), mu, lambda, rho, write Up') over its average launch
-    time from hipEvents on the stream it runs on, against 8 TB/s; traffic is
-    the HBM bytes per launch measured with rocprofv3 counters;
+  roofline: the dominant kernel (the slower interior sweep), its
+    algorithmic bytes per launch (predictor: 12 FP64 words per point, read
+    U, Um (3 components each), mu, lambda, rho, write Up) over its average
+    launch time from hipEvents on the stream it runs on, against 8 TB/s;
+    traffic is the HBM bytes per launch measured with rocprofv3 counters;
+    moved_* the bytes the kernel moves by design (with the stored
+    acceleration the predictor also writes A: 15 words);
+  step_roofline: the whole step on its 27 algorithmic words (216 B) per
+    point against 8 TB/s;
   cpu_baseline: the same step on the host cores for a bounded slab of the
     workload, with the stencil stages (rhs4sg_rev, addsgd4) from the
     reference's own sources (oracle/_ref) when they are present.
@@ -63,8 +67,8 @@ def parse():
     p.add_argument("--n", type=int, default=None, help="grid points per axis (default 512; 1024 for the precision workload)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-fp32", action="store_true", help="skip the FP32 compensated form")
-    p.add_argument("--cpu-nz", type=int, default=128, help="depth of the CPU-baseline slab")
-    p.add_argument("--cpu-steps", type=int, default=3)
+    p.add_argument("--cpu-nz", type=int, default=192, help="depth of the CPU-baseline slab")
+    p.add_argument("--cpu-steps", type=int, default=12)
     p.add_argument("--variant", type=int, default=8, help="interior sweep kernels: 0 thread/point, 1 LDS k-march, 2 with prefetch, "
                         "3 prefetch without register cap, 4 as 3 with the damping in the corrector sweep, "
                         "5 corrector as 4 and predictor as 1, 6 as 1 with an LDS damping sweep, "
@@ -157,7 +161,42 @@ def sources(n, h, seed, nsrc=4):
 def cpu_baseline(n, h, nz, steps, srcs):
     """Bounded CPU sample: an n x n x nz slab of the workload (same boundary
     types, layered material without the random perturbation, the sources that
-    fall inside), one warm-up step and `steps` timed steps."""
+    fall inside).  Preferred: the reference program itself (oracle/_ref,
+    built from the reference's sources) running the slab's deck on the host
+    cores, timed as the difference of a 1-step and a (1 + steps)-step run;
+    otherwise the loop of oracle/loop.py with the reference's stencils."""
+    import subprocess
+    import tempfile
+    from oracle import refbench
+    model, ncpu, avail = refbench.host_description()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or avail
+    inside = [s for s in srcs if s["z"] < (nz - 6) * h] or [dict(srcs[0], z=0.5 * (nz - 1) * h)]
+    host = "%s, nproc %d, %d usable, OMP threads %d" % (model, ncpu, avail, threads)
+    from oracle import refew
+    if refew.available():
+        with tempfile.TemporaryDirectory(prefix="cpubase_") as tmp:
+            fn = os.path.join(tmp, "srcs.json")
+            json.dump(inside, open(fn, "w"))
+            try:
+                r = subprocess.run([sys.executable, "-m", "oracle.refbench", str(n), str(n), str(nz), repr(h),
+                                    repr((n - 1) * h), str(steps), fn], capture_output=True, text=True,
+                                   timeout=300, cwd=ROOT)
+                res = json.loads(r.stdout.strip().splitlines()[-1]) if r.returncode == 0 else None
+            except (subprocess.TimeoutExpired, ValueError, IndexError):
+                res = None
+        if res is not None:
+            return {
+                "value": n * n * nz * steps / res["seconds"],
+                "unit": "grid-pt updates/s",
+                "cores": threads,
+                "kind": "reference",
+                "sample": "the reference program (its own time-step loop and OpenMP C kernels, built from its "
+                          "sources) on a %dx%dx%d slab of the workload (layered material as linear blocks, no "
+                          "perturbation, free top, supergrid 30, %d Ricker sources): %d steps timed as the "
+                          "difference of a %d-step and a %d-step run (%.1f s and %.1f s); host %s"
+                          % (n, n, nz, len(inside), steps, 1, 1 + steps, res["runs"][0], res["runs"][1], host),
+                "cpu_model": model, "nproc": ncpu,
+            }
     from oracle.loop import OracleRun
     from oracle import oracle as O
     from sw4lite_amd import lib as L
@@ -170,7 +209,6 @@ def cpu_baseline(n, h, nz, steps, srcs):
     rho = 1700.0 * (cp / 1000.0) ** 0.25
     mu = rho * cs * cs
     la = rho * cp * cp - 2 * mu
-    inside = [s for s in srcs if s["z"] < (nz - 6) * h] or [dict(srcs[0], z=0.5 * (nz - 1) * h)]
     bc = [L.BC_SUPERGRID] * 4 + [L.BC_FREE, L.BC_SUPERGRID]
     run = OracleRun(n, n, nz, h, bc, rho.ravel(), mu.ravel(), la.ravel(), inside, sg_gp=30, nsteps=steps + 1,
                     kernels=kernels)
@@ -178,17 +216,16 @@ def cpu_baseline(n, h, nz, steps, srcs):
     t0 = time.perf_counter()
     run.step(steps)
     dt = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
         "value": n * n * nz * steps / dt,
         "unit": "grid-pt updates/s",
         "cores": threads,
-        "kind": "reference" if kernels == "reference" else "port",
+        "kind": "port",
         "sample": "%dx%dx%d slab of the workload (layered material, no perturbation), %d timed steps after 1 "
-                  "warm-up, %.1f s; stencil stages %s, pointwise stages from the oracle restatement; "
-                  "OMP threads %d" % (n, n, nz, steps, dt,
-                                      "from the reference sources" if kernels == "reference" else "oracle",
-                                      threads),
+                  "warm-up, %.1f s; stencil stages %s, pointwise stages from the oracle restatement; host %s"
+                  % (n, n, nz, steps, dt, "from the reference sources" if kernels == "reference" else "oracle",
+                     host),
+        "cpu_model": model, "nproc": ncpu,
     }
 
 
@@ -675,21 +712,31 @@ def main():
     pts = float(n) ** 3
     value = pts * a.steps / wall
     # dominant kernel: the slower of the predictor and corrector sweeps over
-    # the interior planes k = 7..n
+    # the interior planes k = 7..n.  Its roofline counts the algorithmic bytes
+    # of the unit of work (a point of the sweep): the predictor reads U, Um,
+    # mu, lambda, rho and writes Up (12 words); a corrector that stages a
+    # stored acceleration moves 12 words (A, Up, mu, lambda, rho in; Up' out),
+    # one that forms it reads Up, U, Um, mu, lambda, rho (15 words).  The
+    # bytes the kernels actually move with the stored acceleration (the
+    # predictor writes A: 15 words) are reported beside it.
     corr_pts = float(n) * n * (n - 6)
     corr_ms = stage_ms[3] / a.steps
     pred_ms = stage_ms[0] / a.steps
     stored_acc = a.variant == 8 and (n + 4) % 2 == 0
     if stored_acc:
-        kern = {"pred": ("k_sweep_dma<0>", PRED_ACC_BYTES_PER_PT), "corr": ("k_sweep_dma<2>", CORR_ACC_BYTES_PER_PT)}
+        kern = {"pred": ("k_sweep_dma<0>", PRED_BYTES_PER_PT, PRED_ACC_BYTES_PER_PT),
+                "corr": ("k_sweep_dma<2>", CORR_ACC_BYTES_PER_PT, CORR_ACC_BYTES_PER_PT)}
     elif a.variant == 7 and (n + 4) % 2 == 0:
-        kern = {"pred": ("k_sweep_dma<0>", PRED_BYTES_PER_PT), "corr": ("k_sweep_lds<true,...>", CORR_BYTES_PER_PT)}
+        kern = {"pred": ("k_sweep_dma<0>", PRED_BYTES_PER_PT, PRED_BYTES_PER_PT),
+                "corr": ("k_sweep_lds<true,...>", CORR_BYTES_PER_PT, CORR_BYTES_PER_PT)}
     elif a.variant >= 1:
-        kern = {"pred": ("k_sweep_lds<false,...>", PRED_BYTES_PER_PT), "corr": ("k_sweep_lds<true,...>", CORR_BYTES_PER_PT)}
+        kern = {"pred": ("k_sweep_lds<false,...>", PRED_BYTES_PER_PT, PRED_BYTES_PER_PT),
+                "corr": ("k_sweep_lds<true,...>", CORR_BYTES_PER_PT, CORR_BYTES_PER_PT)}
     else:
-        kern = {"pred": ("k_pred_interior", PRED_BYTES_PER_PT), "corr": ("k_corr_interior", CORR_BYTES_PER_PT)}
+        kern = {"pred": ("k_pred_interior", PRED_BYTES_PER_PT, PRED_BYTES_PER_PT),
+                "corr": ("k_corr_interior", CORR_BYTES_PER_PT, CORR_BYTES_PER_PT)}
     dom = "pred" if pred_ms > corr_ms else "corr"
-    dom_kernel, dom_bpp = kern[dom]
+    dom_kernel, dom_bpp, dom_moved = kern[dom]
     dom_ms = pred_ms if dom == "pred" else corr_ms
     achieved = corr_pts * dom_bpp / (dom_ms * 1e-3) / 1e9
     traffic = None
@@ -722,8 +769,14 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": dom_kernel, "bytes_per_launch": corr_pts * dom_bpp,
+            "kernel": dom_kernel, "bytes_per_launch": corr_pts * dom_bpp, "bytes_per_pt": dom_bpp,
             "avg_launch_ms": dom_ms,
+            "moved_bytes_per_launch": corr_pts * dom_moved,
+            "moved_GBps": corr_pts * dom_moved / (dom_ms * 1e-3) / 1e9,
+        },
+        "step_roofline": {
+            "bound": "hbm", "bytes_per_pt": STEP_BYTES_PER_PT, "achieved": value * STEP_BYTES_PER_PT / 1e9,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": value * STEP_BYTES_PER_PT / (HBM_PEAK_GBS * 1e9),
         },
         "cpu_baseline": None,
     }
