@@ -151,6 +151,21 @@ int sw4hip_bcfortsg_adjoint(int ib, int ie, int jb, int je, int kb, int ke, cons
                             const double* mu, const double* lambda, const double* strx, const double* stry,
                             double* gmu, double* glambda, void* stream);
 
+/* Allocation-free variant of sw4hip_bcfortsg_adjoint (no counterpart in the
+ * reference): for callers that cannot allocate device memory, such as code
+ * under stream capture or a serving loop.  Same arguments and results; it
+ * calls no hipMalloc/hipFree and does not synchronise (stream-ordered on
+ * `stream`).  Its scratch (the SBP weights and one plane of 3 (nx+4)(ny+4)
+ * doubles) is a fixed device array of the library, one per GPU, held until
+ * the process ends: planes of more than SW4HIP_NOALLOC_MAX_PLANE_POINTS
+ * points are rejected (SW4HIP_ERANGE), and calls that overlap on one GPU
+ * (other streams, other threads) race on it. */
+#define SW4HIP_NOALLOC_MAX_PLANE_POINTS (1030L * 1030L)
+int sw4hip_bcfortsg_adjoint_noalloc(int ib, int ie, int jb, int je, int kb, int ke, const int* wind, int nx, int ny,
+                                    int nz, double* y, const double* fwd, double h, const int* bccnd,
+                                    const double* sbop, const double* mu, const double* lambda, const double* strx,
+                                    const double* stry, double* gmu, double* glambda, void* stream);
+
 /* Replaces EW::bcfortsg / bcfortsg_indrev (ew-cfromfort.C:205-747).
  * wind[36], bccnd[6], sbop[5] are HOST tables; bforce[6] is a host array of
  * device pointers (NULL entries mean zero forcing).  t, om, ph, cv of the

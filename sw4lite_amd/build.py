@@ -28,6 +28,7 @@ SOURCES = [
     "adjoint.cpp",
     "batch.cpp",
     "abi.cpp",
+    "noalloc.hip",
 ]
 
 FLAGS = [

@@ -104,6 +104,29 @@ int check_hip(hipError_t e, const char* what)
 
 }  // namespace sw4amd
 
+namespace sw4amd {
+// Arguments of sw4hip_bcfortsg_adjoint and its allocation-free variant.
+int check_bcfortsg_adjoint(int ib, int ie, int jb, int je, int kb, int ke, const int* wind, int nx, int ny, int nz,
+                           const double* y, const double* fwd, const int* bccnd, const double* sbop, const double* mu,
+                           const double* lambda, const double* strx, const double* stry, const double* gmu,
+                           const double* glambda)
+{
+   if (!box_ok(ib, ie, jb, je, kb, ke)) return set_error(E_INVAL, "box too small");
+   if (ib != -1 || jb != -1 || kb != -1 || ke != nz + 2)
+      return set_error(E_INVAL, "transposed kernels need the index box [-1, n+2] of one grid");
+   if (ie != nx + 2 || je != ny + 2) return set_error(E_INVAL, "bcfortsg_adjoint: box must be [-1, n+2]");
+   if (!wind || !y || !fwd || !bccnd || !sbop || !mu || !lambda || !strx || !stry || !gmu || !glambda)
+      return set_error(E_INVAL, "bcfortsg_adjoint: null argument");
+   for (int s = 0; s < 6; s++) {
+      const int* w = wind + 6 * s;
+      if (bccnd[s] < 0 || bccnd[s] > 2 || w[1] < w[0] || w[3] < w[2] || w[5] < w[4]) continue;
+      if (w[0] < ib || w[1] > ie || w[2] < jb || w[3] > je || w[4] < kb || w[5] > ke)
+         return set_error(E_INVAL, "bcfortsg_adjoint: window outside the array");
+   }
+   return OK;
+}
+}  // namespace sw4amd
+
 using namespace sw4amd;
 
 struct sw4hip_ew {
@@ -309,6 +332,7 @@ int rows_ok(int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast,
 }
 }  // namespace
 
+
 int sw4hip_rhs4sg_adjoint(int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast, int nk,
                           const int* onesided, const double* acof, const double* bope, const double* ghcof, double* out,
                           const double* z, const double* mu, const double* lambda, double h, const double* strx,
@@ -379,16 +403,9 @@ int sw4hip_bcfortsg_adjoint(int ib, int ie, int jb, int je, int kb, int ke, cons
                             double* gmu, double* glambda, void* stream)
 {
    int rc;
-   if ((rc = rows_ok(ib, ie, jb, je, kb, ke, nz, nullptr))) return rc;
-   if (ie != nx + 2 || je != ny + 2) return set_error(E_INVAL, "bcfortsg_adjoint: box must be [-1, n+2]");
-   if (!wind || !y || !fwd || !bccnd || !sbop || !mu || !lambda || !strx || !stry || !gmu || !glambda)
-      return set_error(E_INVAL, "bcfortsg_adjoint: null argument");
-   for (int s = 0; s < 6; s++) {
-      const int* w = wind + 6 * s;
-      if (bccnd[s] < 0 || bccnd[s] > 2 || w[1] < w[0] || w[3] < w[2] || w[5] < w[4]) continue;
-      if (w[0] < ib || w[1] > ie || w[2] < jb || w[3] > je || w[4] < kb || w[5] > ke)
-         return set_error(E_INVAL, "bcfortsg_adjoint: window outside the array");
-   }
+   if ((rc = sw4amd::check_bcfortsg_adjoint(ib, ie, jb, je, kb, ke, wind, nx, ny, nz, y, fwd, bccnd, sbop, mu, lambda, strx,
+                                    stry, gmu, glambda)))
+      return rc;
    SbpTables t = default_tables();
    std::memcpy(t.sbop, sbop, sizeof(t.sbop));
    const SbpTables* d = nullptr;

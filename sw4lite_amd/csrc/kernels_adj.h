@@ -73,6 +73,13 @@ int launch_bcfortsg_T(const Box& b, const int* wind, int nx, int ny, int nz, con
                       const double* strx, const double* stry, double* gmu, double* gla, double* plane,
                       hipStream_t s);
 
+// Argument checks shared by sw4hip_bcfortsg_adjoint and its allocation-free
+// variant (abi.cpp).
+int check_bcfortsg_adjoint(int ib, int ie, int jb, int je, int kb, int ke, const int* wind, int nx, int ny, int nz,
+                           const double* y, const double* fwd, const int* bccnd, const double* sbop, const double* mu,
+                           const double* lambda, const double* strx, const double* stry, const double* gmu,
+                           const double* glambda);
+
 // pointwise helper over n values
 int launch_axpy(long n, double a, const double* x, double* y, hipStream_t s);              // y += a x
 

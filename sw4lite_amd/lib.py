@@ -47,6 +47,7 @@ _SIGS = {
     "sw4hip_rhs4sg_matgrad": (I, [I] * 7 + [P] * 8 + [D, P, P, P, P, P, P]),
     "sw4hip_addsgd4_adjoint": (I, [I] * 6 + [P] * 13 + [D, P, P, P]),
     "sw4hip_bcfortsg_adjoint": (I, [I] * 6 + [P, I, I, I, P, P, D, P, P, P, P, P, P, P, P, P]),
+    "sw4hip_bcfortsg_adjoint_noalloc": (I, [I] * 6 + [P, I, I, I, P, P, D, P, P, P, P, P, P, P, P, P]),
     "sw4hip_ew_reset": (I, [P]),
     "sw4hip_batch_create": (I, [P, I, C.POINTER(P)]),
     "sw4hip_batch_destroy": (I, [P]),
