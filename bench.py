@@ -69,11 +69,12 @@ def parse():
     p.add_argument("--no-fp32", action="store_true", help="skip the FP32 compensated form")
     p.add_argument("--cpu-nz", type=int, default=192, help="depth of the CPU-baseline slab")
     p.add_argument("--cpu-steps", type=int, default=12)
-    p.add_argument("--variant", type=int, default=8, help="interior sweep kernels: 0 thread/point, 1 LDS k-march, 2 with prefetch, "
+    p.add_argument("--variant", type=int, default=9, help="interior sweep kernels: 0 thread/point, 1 LDS k-march, 2 with prefetch, "
                         "3 prefetch without register cap, 4 as 3 with the damping in the corrector sweep, "
                         "5 corrector as 4 and predictor as 1, 6 as 1 with an LDS damping sweep, "
                         "7 as 6 with the predictor planes loaded asynchronously into LDS (even row lengths), "
-                        "8 as 7 with the acceleration stored by the predictor and staged by the corrector")
+                        "8 as 7 with the acceleration stored by the predictor and staged by the corrector, "
+                        "9 as 8 with the damping sweep only over the supergrid layers")
     p.add_argument("--kchunk", type=int, default=64, help="planes per workgroup of the LDS sweeps")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also launch a plain device copy of 1 GiB (PMC scaling)")
@@ -722,7 +723,7 @@ def main():
     corr_pts = float(n) * n * (n - 6)
     corr_ms = stage_ms[3] / a.steps
     pred_ms = stage_ms[0] / a.steps
-    stored_acc = a.variant == 8 and (n + 4) % 2 == 0
+    stored_acc = a.variant >= 8 and (n + 4) % 2 == 0
     if stored_acc:
         kern = {"pred": ("k_sweep_dma<0>", PRED_BYTES_PER_PT, PRED_ACC_BYTES_PER_PT),
                 "corr": ("k_sweep_dma<2>", CORR_ACC_BYTES_PER_PT, CORR_ACC_BYTES_PER_PT)}

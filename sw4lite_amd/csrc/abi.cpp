@@ -593,7 +593,7 @@ int sw4hip_ew_set_stream(sw4hip_ew* h, void* stream)
 int sw4hip_ew_set_kernel_variant(sw4hip_ew* h, int variant, int kchunk)
 {
    EWCHK(h);
-   if (variant < 0 || variant > 8) return set_error(E_INVAL, "variant must be 0..8");
+   if (variant < 0 || variant > 9) return set_error(E_INVAL, "variant must be 0..9");
    if (kchunk < 1 || kchunk > 4096) return set_error(E_INVAL, "kchunk must be in [1, 4096]");
    h->ew->m_variant = variant;
    h->ew->m_kchunk = kchunk;

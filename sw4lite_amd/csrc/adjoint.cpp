@@ -48,8 +48,11 @@ int EW::fused_once(double t, double* acc)
    // the time-step loop
    int ka = 0, kz = 0;
    fused_planes(m_box, g.nz, m_onesided, &ka, &kz);
-   double* af = acc && m_variant == 8 && m_box.ni % 2 == 0 && kz >= ka ? acc : nullptr;
+   double* af = acc && m_variant >= 8 && m_box.ni % 2 == 0 && kz >= ka ? acc : nullptr;
+   const DampTile* dtiles = nullptr;
+   int ndtiles = 0;
    int rc;
+   if ((rc = damp_tile_list(ka, kz, &dtiles, &ndtiles))) return rc;
    if ((rc = launch_pred_fused(m_box, g.nz, m_onesided, m_dsbp, mU.dev, mUm.dev, mMu.dev, mLambda.dev, mRho.dev, c,
                                mUp.dev, g.h, dt, m_stream, nullptr, m_variant, m_kchunk, zf, af)) ||
        (rc = launch_src_apply(m_box, st, t, 0, dt * dt, mRho.dev, mUp.dev, m_stream)) ||
@@ -61,7 +64,7 @@ int EW::fused_once(double t, double* acc)
       return rc;
    if ((rc = launch_corr_fused(m_box, g.nz, m_onesided, m_dsbp, mUp.dev, mU.dev, mUm.dev, mMu.dev, mLambda.dev,
                                mRho.dev, c, m_opt.beta, mUp2.dev, m_dacc, g.h, dt, m_stream, nullptr, nullptr, m_variant,
-                               m_kchunk, zf, af)) ||
+                               m_kchunk, zf, af, dtiles, ndtiles)) ||
        (rc = launch_src_apply(m_box, st, t, 1, dt * dt * dt * dt / 12, mRho.dev, mUp2.dev, m_stream)) ||
        (rc = enforceBC(mUp2, mMu, mLambda, t + dt, zf)))
       return rc;
@@ -102,7 +105,7 @@ int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double
    const int zf = zfill();
    int ka = 0, kz = 0;
    fused_planes(m_box, g.nz, m_onesided, &ka, &kz);
-   double* accf = m_variant == 8 && m_box.ni % 2 == 0 && kz >= ka ? Acc : nullptr;
+   double* accf = m_variant >= 8 && m_box.ni % 2 == 0 && kz >= ka ? Acc : nullptr;
    if ((rc = launch_pred_fused(m_box, g.nz, m_onesided, m_dsbp, u, um, mMu.dev, mLambda.dev, mRho.dev, c, Up, g.h, dt,
                                m_stream, nullptr, m_variant, m_kchunk, zf, accf)) ||
        (rc = launch_src_apply(m_box, st, t, 0, dt2, mRho.dev, Up, m_stream)) ||

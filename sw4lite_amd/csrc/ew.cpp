@@ -78,7 +78,7 @@ int EW::init()
 int EW::accel_field(double** p)
 {
    *p = nullptr;
-   if (m_variant != 8 || m_box.ni % 2 != 0) return OK;
+   if (m_variant < 8 || m_box.ni % 2 != 0) return OK;
    if (!m_accf) {
       void* q = nullptr;
       if (hipMalloc(&q, sizeof(double) * 3 * m_box.npts) != hipSuccess) {
@@ -89,6 +89,38 @@ int EW::accel_field(double** p)
       m_accf = static_cast<double*>(q);
    }
    *p = m_accf;
+   return OK;
+}
+
+int EW::damp_tile_list(int k1, int k2, const DampTile** tiles, int* n)
+{
+   *tiles = nullptr;
+   *n = 0;
+   if (m_variant < 9 || k2 < k1) return OK;
+   if (!m_ddamp || m_damp_k1 != k1 || m_damp_k2 != k2 || m_damp_kc != m_kchunk) {
+      const std::vector<DampTile> v = damp_tiles(m_box, k1, k2, m_kchunk, m_sg_host.dcx.data(),
+                                                 m_sg_host.dcy.data(), m_sg_host.dcz.data());
+      // at least one element, so that an empty list is a valid pointer
+      const size_t bytes = sizeof(DampTile) * std::max<size_t>(v.size(), 1);
+      void* q = nullptr;
+      if (hipMalloc(&q, bytes) != hipSuccess) {
+         (void)hipGetLastError();
+         return set_error(E_NOMEM, "EW: damping tile list allocation failed");
+      }
+      m_allocs.push_back(q);
+      if (!v.empty()) {
+         const int rc = check_hip(hipMemcpy(q, v.data(), sizeof(DampTile) * v.size(), hipMemcpyHostToDevice),
+                                  "damping tile list upload");
+         if (rc) return rc;
+      }
+      m_ddamp = static_cast<DampTile*>(q);
+      m_ndamp = (int)v.size();
+      m_damp_k1 = k1;
+      m_damp_k2 = k2;
+      m_damp_kc = m_kchunk;
+   }
+   *tiles = m_ddamp;
+   *n = m_ndamp;
    return OK;
 }
 
@@ -494,6 +526,9 @@ int EW::step_fused(int n, double* stage_ms)
    double* accf = nullptr;  // variant 8: the acceleration stored by the predictor, staged by the corrector
    // (a grid whose SBP rows leave the sweep no plane has nothing to store it)
    if (rc == OK && kz >= ka) rc = accel_field(&accf);
+   const DampTile* dtiles = nullptr;
+   int ndtiles = 0;
+   if (rc == OK) rc = damp_tile_list(ka, kz, &dtiles, &ndtiles);
    // completes the stored acceleration from the final Up (after sources and BC)
    auto accel_rest = [&](double dt) -> int {
       if (!accf) return OK;
@@ -513,7 +548,7 @@ int EW::step_fused(int n, double* stage_ms)
           (rc = enforceBC(mUp, mMu, mLambda, t + dt, zf)) || (rc = accel_rest(dt)) || (rc = mark(s, 3)) ||
           (rc = launch_corr_fused(m_box, g.nz, m_onesided, m_dsbp, mUp.dev, mU.dev, mUm.dev, mMu.dev, mLambda.dev,
                                   mRho.dev, c, m_opt.beta, mUp2.dev, m_dacc, g.h, dt, m_stream, [&]() { return mark(s, 4); },
-                                  [&]() { return mark(s, 5); }, m_variant, m_kchunk, zf, accf)) ||
+                                  [&]() { return mark(s, 5); }, m_variant, m_kchunk, zf, accf, dtiles, ndtiles)) ||
           (rc = mark(s, 6)) ||
           (rc = launch_src_apply(m_box, st, t, 1, dt * dt * dt * dt / 12, mRho.dev, mUp2.dev, m_stream)) ||
           (rc = enforceBC(mUp2, mMu, mLambda, t + dt, zf)) || (rc = mark(s, 7)) || (rc = record(mUp2, mUm)))

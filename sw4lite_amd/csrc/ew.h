@@ -127,6 +127,11 @@ public:
    double* m_dacc = nullptr;  // acceleration on the planes the corrector's SBP rows read
    double* m_accf = nullptr;  // the whole acceleration (kernel variant 8), allocated at its first step
    int accel_field(double** p);
+   // kernel variant 9: the damping sweep's tiles (damp_tiles) for planes
+   // k1..k2 and the current k-chunk, made and uploaded on first use
+   int damp_tile_list(int k1, int k2, const DampTile** tiles, int* n);
+   DampTile* m_ddamp = nullptr;
+   int m_ndamp = 0, m_damp_k1 = 0, m_damp_k2 = -1, m_damp_kc = 0;
    double mDt = 0, mT = 0;
    int mNumberOfTimeSteps = 0, mCycle = 0;
    int m_onesided[6] = {0, 0, 0, 0, 0, 0};
@@ -139,7 +144,7 @@ public:
    std::vector<GridPointSource> m_gps;
    std::vector<int> m_first;
    hipStream_t m_stream = nullptr;
-   int m_variant = 8;   // interior sweep kernels: 0 thread per point, 1-8 LDS-staged k-march
+   int m_variant = 9;   // interior sweep kernels: 0 thread per point, 1-9 LDS-staged k-march
    int m_kchunk = 64;   // planes per workgroup of the LDS sweeps
 
 private:

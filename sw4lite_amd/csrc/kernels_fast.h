@@ -1,5 +1,7 @@
 // kernels_fast.h -- LDS-staged sweeps (kernels_fast.hip).
 #pragma once
+#include <vector>
+
 #include "kernels_common.h"
 
 namespace sw4amd {
@@ -46,16 +48,33 @@ bool async_loads_fit(const Box& b, const double* const* arrays, int n);
 // supergrid damping (beta != 0) applied in the same sweep; 5: corrector as 4,
 // predictor as 1; 6: both as 1, the damping by launch_damp_lds; 7: as 6, the
 // predictor by asynchronous global-to-LDS loads where the rows allow it; 8:
-// as 7, with acc the acceleration stored and staged (below).
+// as 7, with acc the acceleration stored and staged (below); 9: as 8, the
+// damping sweep restricted to the supergrid layers (damp_tiles).
 // The six SBP rows next to side 4 (top) or 5 (bottom) of one sweep from an
 // LDS slab (corr: with the acceleration and the supergrid damping).
 int launch_sbp_rows(bool corr, const Box& b, int side, int nk, const SbpTables* d_sbp, const double* upin,
                     const double* u, const double* um, const double* mu, const double* la, const double* rho,
                     const SgCoef& sg, double* out, double h, double dt, double beta, hipStream_t s,
                     const double* acc = nullptr);
-// up -= supergrid damping(u, um) on planes k1..k2, k-marching LDS sweep.
+// Column tile (tix, tjy) of the 32 x 8 sweeps and its planes k0..k1.
+struct DampTile {
+   int tix, tjy, k0, k1;
+};
+// The tiles of the damping sweep over planes k1..k2 (chunks of kc planes)
+// whose points can carry a nonzero damping term, from the host damping
+// profiles dcx, dcy, dcz (box-sized, from the box start): addsgd4 at a point
+// is a sum of terms each with a factor dc at the point or a neighbour, and dc
+// is exactly zero outside the supergrid layers, where the update subtracts
+// zero and leaves every bit of up.  Tiles whose columns meet an x or y layer
+// keep all their planes, the others only the planes of the z layers.
+std::vector<DampTile> damp_tiles(const Box& b, int k1, int k2, int kc, const double* dcx, const double* dcy,
+                                 const double* dcz);
+// up -= supergrid damping(u, um) on planes k1..k2, k-marching LDS sweep;
+// with tiles (device array of damp_tiles for the same k1, k2, kc) only those
+// tiles run (asynchronous-load sweep; otherwise the whole box).
 int launch_damp_lds(const Box& b, int k1, int k2, int kc, double* up, const double* u, const double* um,
-                    const double* rho, const SgCoef& sg, double beta, hipStream_t s);
+                    const double* rho, const SgCoef& sg, double beta, hipStream_t s,
+                    const DampTile* tiles = nullptr, int ntiles = 0);
 // Variant 8 with acc (3 npts doubles, even row length, aligned arrays): the
 // predictor also stores the acceleration (up - 2u + um)/dt^2 at the points
 // it computes, the corrector stages acc instead of forming it; between the

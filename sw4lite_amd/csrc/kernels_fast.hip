@@ -10,6 +10,8 @@
 // never stored.  The arithmetic is rhs4_interior_body, shared with the
 // global-memory kernels (same results to rounding).  The supergrid damping
 // of the corrector sweep runs as k_addsgd4_planes right after it.
+#include <algorithm>
+
 #include "kernels_common.h"
 #include "kernels_fast.h"
 #include "sbp_row_body.h"
@@ -693,27 +695,21 @@ __device__ __forceinline__ void damp_lds_body(unsigned bid, Box b, TileMap map, 
 // after the next barrier.  Two slots of 7 staged values per point (48 KB):
 // three workgroups per CU.  Same conditions as sweep_dma_body (even ni,
 // 16-byte aligned arrays).
-__device__ __forceinline__ void damp_dma_body(unsigned bid, Box b, TileMap map, int kbeg, int kend, int kc,
-                                              double* __restrict__ up, const double* __restrict__ u,
-                                              const double* __restrict__ um, const double* __restrict__ rho,
-                                              SgCoef sg, double beta)
+// Column tile (tix, tjy), planes k0..k1.
+__device__ __forceinline__ void damp_dma_tile(int tix, int tjy, int k0, int k1, Box b, double* __restrict__ up,
+                                              const double* __restrict__ u, const double* __restrict__ um,
+                                              const double* __restrict__ rho, SgCoef sg, double beta)
 {
    constexpr int PP = 3 * PL / 2, NP = (PP + NT - 1) / NT, MP = PL / 2;
    static_assert(HX % 2 == 0 && MP <= NT, "pairs must tile the rows; one rho load per thread");
    __shared__ __align__(16) double su[2 * 3 * PL];  // u of plane kp in half (kp - k0) & 1
    __shared__ __align__(16) double sq[2 * 3 * PL];  // um
    __shared__ __align__(16) double sr[2 * PL];      // rho
-   const int per = (int)((map.ntiles + 7) / 8);
-   const int t = (int)(bid % 8) * per + (int)(bid / 8);
-   if (t >= map.ntiles) return;
-   const int tix = t % map.ntx, tr = t / map.ntx, tjy = tr % map.nty, tkz = tr / map.nty;
    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
    const int ia = b.ib + tix * TX, ja = b.jb + tjy * TY;
    const int i = ia + 2 + tx, j = ja + 2 + ty;
    const bool active = i <= b.ie - 2 && j <= b.je - 2;
-   const int k0 = kbeg + tkz * kc;
-   const int k1 = min(kend, k0 + kc - 1);
    const long ni = b.ni, nij = b.nij, npts = b.npts;
    const long col = active ? (i - b.ib) + ni * (j - b.jb) : 0;
    const int o = (ty + 2) * HX + tx + 2;
@@ -842,11 +838,38 @@ __device__ __forceinline__ void damp_dma_body(unsigned bid, Box b, TileMap map, 
    }
 }
 
+__device__ __forceinline__ void damp_dma_body(unsigned bid, Box b, TileMap map, int kbeg, int kend, int kc,
+                                              double* __restrict__ up, const double* __restrict__ u,
+                                              const double* __restrict__ um, const double* __restrict__ rho,
+                                              SgCoef sg, double beta)
+{
+   const int per = (int)((map.ntiles + 7) / 8);
+   const int t = (int)(bid % 8) * per + (int)(bid / 8);
+   if (t >= map.ntiles) return;
+   const int tix = t % map.ntx, tr = t / map.ntx, tjy = tr % map.nty, tkz = tr / map.nty;
+   const int k0 = kbeg + tkz * kc;
+   damp_dma_tile(tix, tjy, k0, min(kend, k0 + kc - 1), b, up, u, um, rho, sg, beta);
+}
+
 __global__ void __launch_bounds__(NT, 3)
 k_damp_dma(Box b, TileMap map, int kbeg, int kend, int kc, double* __restrict__ up, const double* __restrict__ u,
            const double* __restrict__ um, const double* __restrict__ rho, SgCoef sg, double beta)
 {
    damp_dma_body(blockIdx.x, b, map, kbeg, kend, kc, up, u, um, rho, sg, beta);
+}
+
+// The damping sweep over a list of tiles (damp_tiles): the XCD of a block is
+// its id mod 8 and each XCD takes one contiguous run of the list.
+__global__ void __launch_bounds__(NT, 3)
+k_damp_dma_tiles(Box b, const DampTile* __restrict__ tiles, int ntiles, double* __restrict__ up,
+                 const double* __restrict__ u, const double* __restrict__ um, const double* __restrict__ rho,
+                 SgCoef sg, double beta)
+{
+   const int per = (ntiles + 7) / 8;
+   const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+   if (t >= ntiles) return;
+   const DampTile d = tiles[t];
+   damp_dma_tile(d.tix, d.tjy, d.k0, d.k1, b, up, u, um, rho, sg, beta);
 }
 
 template <int DTY>
@@ -1012,10 +1035,54 @@ int launch_accel_fix(const Box& b, int k1, int k2, bool frame, bool all_planes, 
    return check_hip(hipGetLastError(), "acceleration fix launch");
 }
 
+std::vector<DampTile> damp_tiles(const Box& b, int k1, int k2, int kc, const double* dcx, const double* dcy,
+                                 const double* dcz)
+{
+   if (kc <= 0) kc = 64;
+   // point q (offset from the box start) can be damped along an axis when the
+   // profile is nonzero at q-1, q or q+1 (the stencil of addsgd4)
+   auto damped = [](const double* dc, int n) {
+      std::vector<char> on(n, 0);
+      for (int q = 1; q + 1 < n; q++) on[q] = dc[q - 1] != 0.0 || dc[q] != 0.0 || dc[q + 1] != 0.0;
+      return on;
+   };
+   const int ni = b.ie - b.ib + 1, nj = b.je - b.jb + 1, nk = b.ke - b.kb + 1;
+   const std::vector<char> ox = damped(dcx, ni), oy = damped(dcy, nj), oz = damped(dcz, nk);
+   const long nx = b.ie - b.ib - 3, ny = b.je - b.jb - 3;
+   const int ntx = (int)cdiv(nx, TX), nty = (int)cdiv(ny, TY);
+   std::vector<DampTile> out;
+   std::vector<std::vector<DampTile>> per_chunk((size_t)cdiv(k2 - k1 + 1, kc));
+   for (int tjy = 0; tjy < nty; tjy++)
+      for (int tix = 0; tix < ntx; tix++) {
+         bool side = false;  // the tile's columns meet an x or y layer: every plane
+         for (int q = 2 + tix * TX; q < std::min(2 + (tix + 1) * TX, ni - 2); q++) side = side || ox[q];
+         for (int q = 2 + tjy * TY; q < std::min(2 + (tjy + 1) * TY, nj - 2); q++) side = side || oy[q];
+         for (int c = 0; c < (int)per_chunk.size(); c++) {
+            const int ka = k1 + c * kc, kb = std::min(k2, ka + kc - 1);
+            int lo = kb + 1, hi = ka - 1;  // planes of the chunk to run
+            for (int k = ka; k <= kb; k++)
+               if (side || oz[k - b.kb]) {
+                  lo = std::min(lo, k);
+                  hi = std::max(hi, k);
+               }
+            if (lo <= hi) per_chunk[c].push_back(DampTile{tix, tjy, lo, hi});
+         }
+      }
+   // chunk-major, then j, then i: neighbouring tiles stay neighbours in the list
+   for (auto& v : per_chunk) out.insert(out.end(), v.begin(), v.end());
+   return out;
+}
+
 int launch_damp_lds(const Box& b, int k1, int k2, int kc, double* up, const double* u, const double* um,
-                    const double* rho, const SgCoef& sg, double beta, hipStream_t s)
+                    const double* rho, const SgCoef& sg, double beta, hipStream_t s, const DampTile* tiles,
+                    int ntiles)
 {
    if (k2 < k1 || beta == 0) return OK;
+   if (tiles && dma_fits(b, u, um, rho) && ((uintptr_t)up & 15) == 0) {
+      if (ntiles > 0)
+         k_damp_dma_tiles<<<8 * cdiv(ntiles, 8), NT, 0, s>>>(b, tiles, ntiles, up, u, um, rho, sg, beta);
+      return check_hip(hipGetLastError(), "damping sweep launch");
+   }
    const long nx = b.ie - b.ib - 3, ny = b.je - b.jb - 3;
    if (kc <= 0) kc = 64;
    constexpr int DTY = 8;
@@ -1232,7 +1299,8 @@ int launch_sweep_lds(bool corr, int variant, const Box& b, int k1, int k2, int k
       break;
    case 7:  // as 6, the predictor's planes by asynchronous global-to-LDS loads where they fit
    case 8:  // as 7; with acc, the predictor also stores the acceleration and the corrector stages it
-      if (variant == 8 && acc) {
+   case 9:  // as 8 (the damping sweep runs only where the supergrid profiles are nonzero)
+      if (variant >= 8 && acc) {
          if (!dma_fits(b, acc, mu, la) || !dma_fits(b, corr ? upin : u, mu, la))
             return set_error(E_INVAL, "acceleration sweeps need an even row length and aligned arrays");
          if (corr)

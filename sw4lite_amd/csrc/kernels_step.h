@@ -4,6 +4,7 @@
 #include <functional>
 
 #include "kernels_common.h"
+#include "kernels_fast.h"
 
 namespace sw4amd {
 
@@ -31,7 +32,8 @@ int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables
                       const double* la, const double* rho, const SgCoef& sg, double beta,
                       double* upo, double* acc, double h, double dt, hipStream_t s,
                       const std::function<int()>& after_interior, const std::function<int()>& after_damping,
-                      int variant = 1, int kchunk = 32, int zfill = 0, const double* accf = nullptr);
+                      int variant = 1, int kchunk = 32, int zfill = 0, const double* accf = nullptr,
+                      const DampTile* dtiles = nullptr, int ndtiles = 0);
 // Interior planes of the fused sweeps (below and above the SBP rows).
 void fused_planes(const Box& b, int nk, const int* onesided, int* k1, int* k2);
 // acc = (up - 2u + um)/dt^2 at the grid points of the sources (after the

@@ -591,7 +591,8 @@ int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables
                       const double* la, const double* rho, const SgCoef& sg, double beta,
                       double* upo, double* acc, double h, double dt, hipStream_t s,
                       const std::function<int()>& after_interior, const std::function<int()>& after_damping,
-                      int variant, int kchunk, int zfill, const double* accf)
+                      int variant, int kchunk, int zfill, const double* accf, const DampTile* dtiles,
+                      int ndtiles)
 {
    const double cof = 1.0 / (h * h), dt2i = 1.0 / (dt * dt), dt4i12 = dt * dt * dt * dt / 12;
    const int k1 = onesided[4] == 1 ? 7 : b.kb + 2;
@@ -604,7 +605,9 @@ int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables
       if (rc) return rc;
       if (after_interior && (rc = after_interior())) return rc;
       if (beta != 0 && variant >= 6) {
-         if ((rc = launch_damp_lds(b, k1, k2, kchunk, upo, u, um, rho, sg, beta, s))) return rc;
+         if ((rc = launch_damp_lds(b, k1, k2, kchunk, upo, u, um, rho, sg, beta, s, variant >= 9 ? dtiles : nullptr,
+                                   ndtiles)))
+            return rc;
       } else if (beta != 0 && variant < 4) {  // variants 4, 5 damp inside the sweep
          dim3 grd(cdiv(nii, 64), cdiv(njj, 4), k2 - k1 + 1);
          k_addsgd4_planes<<<grd, blk, 0, s>>>(b, k1, k2, upo, u, um, rho, sg, beta);

@@ -277,9 +277,10 @@ class EW:
         corrector as 4 and predictor as 1, 6 as 1 with an LDS damping sweep,
         7 as 6 with the predictor's planes loaded asynchronously into LDS
         where the row length is even, 8 as 7 with the acceleration stored by
-        the predictor and loaded asynchronously by the corrector (the
-        default; one more field of device memory; even row lengths,
-        otherwise as 7))."""
+        the predictor and loaded asynchronously by the corrector (one more
+        field of device memory; even row lengths, otherwise as 7), 9 as 8
+        with the damping sweep only over the tiles of the supergrid layers,
+        where the damping profiles are nonzero (the default; same bits))."""
         L.check(self.lib.sw4hip_ew_set_kernel_variant(self.h, int(variant), int(kchunk)))
 
     def npoints(self):

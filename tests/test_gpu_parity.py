@@ -238,6 +238,10 @@ def test_full_size_fused_matches_reference_order(torch):
     (8, 5, [2, 2, 2, 2, 2, 2], 44),
     (8, 7, [2, 2, 2, 2, 0, 2]),
     (8, 7, [3, 3, 3, 3, 0, 1], 44),
+    # variant 9 runs the damping sweep only over the supergrid layers
+    (9, 64, [2, 2, 2, 2, 0, 2], 96),
+    (9, 7, [2, 2, 2, 2, 2, 2], 96),
+    (9, 5, [2, 2, 2, 2, 0, 0], 45),
 ]])
 def test_sweep_variants_match_oracle(torch, variant, kchunk, bc, nx):
     """The LDS-staged k-marching sweeps on a grid that is not a multiple of
@@ -317,6 +321,35 @@ def test_stored_acceleration_bitwise(torch, bc, nx, kchunk):
     assert np.max(np.abs(u7)) > 0
     ndiff = int(np.count_nonzero(u7 != u8)) + int(np.count_nonzero(um7 != um8))
     print("stored acceleration: bc", bc, "nx", nx, "kchunk", kchunk, "points differing from variant 7:", ndiff, flush=True)
+    assert ndiff == 0
+
+
+@pytest.mark.parametrize("bc,kchunk", [([2, 2, 2, 2, 0, 2], 7), ([2, 2, 2, 2, 2, 2], 5), ([1, 1, 2, 2, 0, 2], 64),
+                                       ([2, 2, 1, 1, 2, 0], 9), ([3, 3, 3, 3, 0, 1], 7)])
+def test_damping_layers_bitwise(torch, bc, kchunk):
+    """Variant 9 runs the damping sweep only on the tiles and planes where
+    a supergrid profile is nonzero (elsewhere the damping term is an exact
+    zero): on a grid of three tiles in x, five in y and k-chunks that cut
+    the z layers, the solution must equal variant 8's to the last bit
+    (periodic x, y and a Dirichlet bottom: no layer, nothing runs)."""
+    from sw4lite_amd import EW
+    nx, ny, nz, h = 96, 40, 41, 0.05
+    rho, mu, la = layered_material(nx, ny, nz, h, 9)
+    res = []
+    for variant in (8, 9):
+        e = EW(nx, ny, nz, h, bc, corder=1, sg_gp_thickness=6)
+        e.set_kernel_variant(variant, kchunk)
+        e.set_material(rho, mu, la)
+        for s in SOURCES:
+            e.add_moment_source(s["x"] * 0.5, s["y"] * 0.5, s["z"] * 0.5, s["m"], s["t0"], s["freq"], s["tfkind"])
+        e.setup(nsteps=8)
+        e.timesteploop(8, mode=0)
+        res.append(e.get_solution())
+        e.close()
+    (u8, um8), (u9, um9) = res
+    assert np.max(np.abs(u8)) > 0
+    ndiff = int(np.count_nonzero(u8 != u9)) + int(np.count_nonzero(um8 != um9))
+    print("damping layers: bc", bc, "kchunk", kchunk, "points differing from variant 8:", ndiff, flush=True)
     assert ndiff == 0
 
 
