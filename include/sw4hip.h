@@ -373,6 +373,57 @@ int sw4hip_batch_tune(sw4hip_batch* batch, int ncand, const int* cand, int* best
 int sw4hip_batch_step(sw4hip_batch* batch, int nsteps, int use_graph);
 int sw4hip_ew_sync(sw4hip_ew* ew);
 
+/* ---- Input decks, receiver files, checkpoints ---- */
+
+/* Replaces EW::EW(deck) + EW::timesteploop (EW.C:2171-2336, 2339-2928;
+ * main.C) for decks of one Cartesian grid: the commands grid, time, fileio,
+ * supergrid, developer, testpointsource, source (moment tensors; Ricker,
+ * Gaussian, C6SmoothBump), block, rec, checkpoint and restart.  Anything
+ * else (topography, point forces, other time functions) is refused with
+ * SW4HIP_EINVAL and a message naming the line.  Runs on the current GPU:
+ * mode 0 fused sweeps, 1 the reference stage order, 2 FP32 with the
+ * compensated update.  Writes the receiver files, checkpoints and (point
+ * source test) PointSourceErr.txt into the deck's fileio path, or into
+ * `path` when it is not NULL. */
+typedef struct {
+   int nx, ny, nz;
+   double h, dt, t;
+   int nsteps, cycle, nstations, ncheckpoints, point_source_test;
+   double linf, l2, solinf; /* point source test: errors at the end */
+} sw4hip_deck_report;
+int sw4hip_run_deck(const char* deck_file, const char* path, int mode, sw4hip_deck_report* report);
+/* Parse deck text only (no GPU): grid sizes and step settings in report
+ * (nsteps = steps= of the deck, -1 when it gives t=). */
+int sw4hip_parse_deck(const char* deck_text, sw4hip_deck_report* report);
+
+/* Replaces TimeSeries::writeFile (TimeSeries.C:416-1060, sacsubc.C): the
+ * station's binary SAC files (.x .y .z, velocity .xv .yv .zv, nsew .e .n .u /
+ * .ev .nv .uv) and/or its USGS text file (.txt) in path ("." for none).
+ * samples: nsamples x 3 (x, y, z) at t0 + i dt.  Geographic frame: the
+ * reference's defaults (origin 37 N 118 W, azimuth 0, 111319.5 m/degree). */
+typedef struct {
+   const char* file; /* file name prefix */
+   const char* sta;  /* station name (NULL: file) */
+   int mode;         /* 0 displacement, 1 velocity */
+   int nsew;         /* 1: east-west, north-south, up components */
+   int usgs, sac;    /* formats to write */
+   double x, y, z;   /* requested location */
+   double gx, gy, gz; /* the grid point recorded */
+   int utc[7];       /* reference time: year, month, day, hour, minute, second, ms */
+} sw4hip_station;
+int sw4hip_write_station(const sw4hip_station* station, const char* path, double t0, double dt, int nsamples,
+                         const double* samples);
+
+/* Replaces CheckPoint::write_checkpoint / read_checkpoint
+ * (CheckPoint.C:251-480) for one grid: header (precision 8, 1 grid, time,
+ * cycle, 1 nx 1 ny 1 nz), then Um and U on the interior points, the three
+ * components of a point together, i fastest.  um, u: host arrays over the
+ * box [-1, n+2], component-blocked (corder 1); read fills the interior. */
+int sw4hip_write_checkpoint(const char* file, double time, int cycle, int nx, int ny, int nz, const double* um,
+                            const double* u);
+int sw4hip_read_checkpoint(const char* file, int nx, int ny, int nz, double* time, int* cycle, double* um,
+                           double* u);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,8 @@ SOURCES = [
     "batch.cpp",
     "abi.cpp",
     "noalloc.hip",
+    "timeseries.cpp",
+    "deck.cpp",
 ]
 
 FLAGS = [

@@ -1,5 +1,7 @@
 // abi.cpp -- extern "C" entry points declared in include/sw4hip.h.
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <new>
 #include <mutex>
 #include <string>
@@ -7,6 +9,7 @@
 
 #include "../../include/sw4hip.h"
 #include "batch.h"
+#include "deck.h"
 #include "ew.h"
 #include "kernels_adj.h"
 #include "kernels_curv.h"
@@ -872,3 +875,104 @@ int sw4hip_ew_sync(sw4hip_ew* h)
 }
 
 }  // extern "C"
+
+namespace {
+void fill_report(const sw4amd::DeckResult& r, sw4hip_deck_report* rep)
+{
+   rep->nx = r.nx;
+   rep->ny = r.ny;
+   rep->nz = r.nz;
+   rep->h = r.h;
+   rep->dt = r.dt;
+   rep->t = r.t;
+   rep->nsteps = r.nsteps;
+   rep->cycle = r.cycle;
+   rep->nstations = r.nstations;
+   rep->ncheckpoints = r.ncheckpoints;
+   rep->point_source_test = r.point_source_test;
+   rep->linf = r.linf;
+   rep->l2 = r.l2;
+   rep->solinf = r.solinf;
+}
+}  // namespace
+
+extern "C" int sw4hip_parse_deck(const char* text, sw4hip_deck_report* rep)
+{
+   if (!text || !rep) return set_error(E_INVAL, "parse_deck: null argument");
+   Deck d;
+   int rc;
+   if ((rc = parse_deck(text, d))) return rc;
+   DeckResult r;
+   r.nx = d.nx;
+   r.ny = d.ny;
+   r.nz = d.nz;
+   r.h = d.h;
+   r.t = d.tmax;
+   r.nsteps = d.steps;
+   r.nstations = (int)d.stations.size();
+   r.ncheckpoints = (int)d.checkpoints.size();
+   r.point_source_test = d.point_source_test;
+   fill_report(r, rep);
+   return OK;
+}
+
+extern "C" int sw4hip_run_deck(const char* file, const char* path, int mode, sw4hip_deck_report* rep)
+{
+   if (!file || !rep) return set_error(E_INVAL, "run_deck: null argument");
+   std::ifstream in(file);
+   if (!in) return set_error(E_INVAL, std::string("run_deck: cannot open ") + file);
+   std::stringstream ss;
+   ss << in.rdbuf();
+   Deck d;
+   int rc;
+   if ((rc = parse_deck(ss.str(), d))) return rc;
+   int ndev = 0;
+   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+      (void)hipGetLastError();
+      return set_error(E_NODEV, "no HIP device available");
+   }
+   DeckResult r;
+   if ((rc = run_deck(d, path ? path : "", mode, r))) return rc;
+   fill_report(r, rep);
+   return OK;
+}
+
+extern "C" int sw4hip_write_station(const sw4hip_station* s, const char* path, double t0, double dt, int nsamples,
+                                    const double* samples)
+{
+   if (!s || !s->file || !path) return set_error(E_INVAL, "write_station: null argument");
+   Station st;
+   st.file = s->file;
+   st.sta = s->sta ? s->sta : s->file;
+   st.mode = s->mode;
+   st.xyz = s->nsew != 1;
+   st.usgs = s->usgs != 0;
+   st.sac = s->sac != 0;
+   st.x = s->x;
+   st.y = s->y;
+   st.z = s->z;
+   st.gx = s->gx;
+   st.gy = s->gy;
+   st.gz = s->gz;
+   for (int q = 0; q < 7; q++) st.utc[q] = s->utc[q];
+   std::string p = path;
+   if (p != "." && !p.empty() && p.back() != '/') p += '/';
+   return write_station(p, st, GeoFrame(), t0, dt, nsamples, samples);
+}
+
+extern "C" int sw4hip_write_checkpoint(const char* file, double time, int cycle, int nx, int ny, int nz,
+                                       const double* um, const double* u)
+{
+   if (!file || !um || !u) return set_error(E_INVAL, "write_checkpoint: null argument");
+   if (nx < 1 || ny < 1 || nz < 1) return set_error(E_INVAL, "write_checkpoint: empty grid");
+   return write_checkpoint(file, time, cycle, nx, ny, nz, um, u);
+}
+
+extern "C" int sw4hip_read_checkpoint(const char* file, int nx, int ny, int nz, double* time, int* cycle, double* um,
+                                      double* u)
+{
+   if (!file || !time || !cycle) return set_error(E_INVAL, "read_checkpoint: null argument");
+   if (nx < 1 || ny < 1 || nz < 1) return set_error(E_INVAL, "read_checkpoint: empty grid");
+   return read_checkpoint(file, nx, ny, nz, time, cycle, um, u);
+}
+

@@ -333,6 +333,65 @@ int EW::setup(double tmax, int nsteps)
    return OK;
 }
 
+int EW::host_solution(double* u, double* um)
+{
+   if (!m_setup_done) return set_error(E_STATE, "EW: solution before setup");
+   int rc;
+   if ((rc = comp_end())) return rc;
+   const long npts = mU.npts();
+   std::vector<double> tmp;
+   // component-blocked copies whatever the device layout
+   for (int w = 0; w < 2; w++) {
+      double* dst = w == 0 ? u : um;
+      const Sarray& a = w == 0 ? mU : mUm;
+      if (!dst) continue;
+      double* buf = dst;
+      if (!m_opt.corder) {
+         tmp.resize(3 * npts);
+         buf = tmp.data();
+      }
+      if ((rc = check_hip(hipMemcpyAsync(buf, a.dev, sizeof(double) * 3 * npts, hipMemcpyDeviceToHost, m_stream),
+                          "solution readback")) ||
+          (rc = check_hip(hipStreamSynchronize(m_stream), "solution readback")))
+         return rc;
+      if (!m_opt.corder)
+         for (long p = 0; p < npts; p++)
+            for (int c = 0; c < 3; c++) dst[p + npts * c] = tmp[3 * p + c];
+   }
+   return OK;
+}
+
+int EW::restart_from(const double* u, const double* um, double t, int cycle)
+{
+   if (!m_setup_done) return set_error(E_STATE, "EW: restart before setup");
+   if (cycle < 0 || cycle > mNumberOfTimeSteps)
+      return set_error(E_INVAL, "EW: restart cycle outside the run's " + std::to_string(mNumberOfTimeSteps) + " steps");
+   int rc;
+   if ((rc = comp_end())) return rc;
+   m_comp = false;
+   const long npts = mU.npts();
+   std::vector<double> tmp;
+   for (int w = 0; w < 2; w++) {
+      const double* src = w == 0 ? u : um;
+      Sarray& a = w == 0 ? mU : mUm;
+      if (!m_opt.corder) {
+         tmp.resize(3 * npts);
+         for (long p = 0; p < npts; p++)
+            for (int c = 0; c < 3; c++) tmp[3 * p + c] = src[p + npts * c];
+         src = tmp.data();
+      }
+      if ((rc = check_hip(hipMemcpyAsync(a.dev, src, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, m_stream),
+                          "restart upload")) ||
+          (rc = check_hip(hipStreamSynchronize(m_stream), "restart upload")))
+         return rc;
+   }
+   if ((rc = enforceBC(mU, mMu, mLambda, t)) || (rc = enforceBC(mUm, mMu, mLambda, t - mDt))) return rc;
+   mT = t;
+   mCycle = cycle;
+   m_nsamples = 0;
+   return record(mU, mUm);
+}
+
 int EW::add_receiver(double x, double y, double z, int mode, int* index)
 {
    if (m_setup_done) return set_error(E_STATE, "EW: receivers must be added before setup");

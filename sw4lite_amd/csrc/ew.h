@@ -97,6 +97,14 @@ public:
    // --- discrete adjoint of the FP64 step (adjoint.cpp) ---
    // Zero initial data at t = 0, receivers restarted with the t = 0 sample.
    int reset();
+   // U, Um (component-blocked host arrays over the box, corder 1) of the
+   // current state, waiting for the stream
+   int host_solution(double* u, double* um);
+   // Continue from a checkpoint: U, Um (host arrays as above) at time t after
+   // `cycle` steps; the boundary conditions of both time levels are applied
+   // as after CheckPoint::read_checkpoint (EW.C:2403-2414) and the receivers
+   // restart with this state as their first sample.
+   int restart_from(const double* u, const double* um, double t, int cycle);
    // Gradient of a receiver misfit with respect to rho, mu, lambda (whole
    // box): forward nsteps fused steps from zero data with checkpoints every
    // ckpt steps, then the reverse sweep.  gtr: host array

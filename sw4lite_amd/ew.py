@@ -481,3 +481,64 @@ class Batch:
         except Exception:
             pass
 
+
+
+# --------------------------------------------------------------------------
+# input decks, receiver files, checkpoints
+# --------------------------------------------------------------------------
+def _report(r):
+    return {f: getattr(r, f) for f, _ in L.DeckReport._fields_}
+
+
+def parse_deck(text):
+    """Grid and step settings of deck text (no GPU); raises on commands the
+    front end does not run."""
+    r = L.DeckReport()
+    L.check(L.load().sw4hip_parse_deck(text.encode(), C.byref(r)))
+    return _report(r)
+
+
+def run_deck(deck_file, path=None, mode=0):
+    """Run a reference input deck on the current GPU (sw4hip_run_deck):
+    receiver files, checkpoints and PointSourceErr.txt go to the deck's
+    fileio path or to `path`.  Returns the run's report as a dict."""
+    r = L.DeckReport()
+    L.check(L.load().sw4hip_run_deck(str(deck_file).encode(), None if path is None else str(path).encode(),
+                                     int(mode), C.byref(r)))
+    return _report(r)
+
+
+def write_station(path, file, samples, dt, t0=0.0, sta=None, mode=0, nsew=False, usgs=True, sac=True,
+                  loc=(0.0, 0.0, 0.0), grid_point=None, utc=(1970, 1, 1, 0, 0, 0, 0)):
+    """Receiver files of one station (TimeSeries::writeFile formats);
+    samples: array [n, 3] of x, y, z components."""
+    s = np.ascontiguousarray(np.asarray(samples, dtype=np.float64).reshape(-1, 3))
+    st = L.Station()
+    st.file = file.encode()
+    st.sta = None if sta is None else sta.encode()
+    st.mode, st.nsew, st.usgs, st.sac = int(mode), int(bool(nsew)), int(bool(usgs)), int(bool(sac))
+    st.x, st.y, st.z = loc
+    st.gx, st.gy, st.gz = loc if grid_point is None else grid_point
+    for q in range(7):
+        st.utc[q] = int(utc[q])
+    L.check(L.load().sw4hip_write_station(C.byref(st), str(path).encode(), float(t0), float(dt), s.shape[0], _ptr(s)))
+
+
+def write_checkpoint(file, time, cycle, nx, ny, nz, um, u):
+    """CheckPoint file of (Um, U), host arrays over the box [-1, n+2],
+    component-blocked."""
+    n = 3 * (nx + 4) * (ny + 4) * (nz + 4)
+    _fields(n, um, u)
+    L.check(L.load().sw4hip_write_checkpoint(str(file).encode(), float(time), int(cycle), nx, ny, nz, _ptr(um),
+                                             _ptr(u)))
+
+
+def read_checkpoint(file, nx, ny, nz):
+    """(time, cycle, um, u) of a CheckPoint file; um, u over the box
+    (ghost points zero)."""
+    n = 3 * (nx + 4) * (ny + 4) * (nz + 4)
+    um, u = np.zeros(n), np.zeros(n)
+    t, cyc = C.c_double(), C.c_int()
+    L.check(L.load().sw4hip_read_checkpoint(str(file).encode(), nx, ny, nz, C.byref(t), C.byref(cyc), _ptr(um),
+                                            _ptr(u)))
+    return t.value, cyc.value, um, u

@@ -25,6 +25,17 @@ class GridDesc(C.Structure):
                 ("bc", I * 6), ("sg_gp_thickness", I), ("sg_damping", D), ("cfl", D)]
 
 
+class DeckReport(C.Structure):
+    _fields_ = [("nx", I), ("ny", I), ("nz", I), ("h", D), ("dt", D), ("t", D), ("nsteps", I), ("cycle", I),
+                ("nstations", I), ("ncheckpoints", I), ("point_source_test", I), ("linf", D), ("l2", D),
+                ("solinf", D)]
+
+
+class Station(C.Structure):
+    _fields_ = [("file", C.c_char_p), ("sta", C.c_char_p), ("mode", I), ("nsew", I), ("usgs", I), ("sac", I),
+                ("x", D), ("y", D), ("z", D), ("gx", D), ("gy", D), ("gz", D), ("utc", I * 7)]
+
+
 class EWInfo(C.Structure):
     _fields_ = [("dt", D), ("nsteps", I), ("t", D), ("cycle", I), ("onesided", I * 6),
                 ("wind", I * 36), ("npts", C.c_long), ("nsources", I), ("nunique", I)]
@@ -85,6 +96,11 @@ _SIGS = {
     "sw4hip_ew_get_receivers": (I, [P, IP, P]),
     "sw4hip_ew_device_array": (I, [P, I, C.POINTER(P)]),
     "sw4hip_ew_sync": (I, [P]),
+    "sw4hip_run_deck": (I, [C.c_char_p, C.c_char_p, I, C.POINTER(DeckReport)]),
+    "sw4hip_parse_deck": (I, [C.c_char_p, C.POINTER(DeckReport)]),
+    "sw4hip_write_station": (I, [C.POINTER(Station), C.c_char_p, D, D, I, P]),
+    "sw4hip_write_checkpoint": (I, [C.c_char_p, D, I, I, I, I, P, P]),
+    "sw4hip_read_checkpoint": (I, [C.c_char_p, I, I, I, DP, IP, P, P]),
 }
 
 _lib = None

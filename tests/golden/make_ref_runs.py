@@ -11,6 +11,18 @@ Only numbers and small output files are stored.
                         (EW::exactSol + normOfDifference, EW.C:2889-2905),
                         at full precision and as the run's own
                         PointSourceErr.txt.
+  ref_stations/         the reference's receiver files (USGS text and binary
+                        SAC, TimeSeries::writeFile) of a small layered deck
+                        (STATIONS_DECK: two blocks, one Gaussian moment
+                        source, displacement and velocity stations, each in
+                        x/y/z and in east/north/up components) after 40 steps;
+                        the deck itself is ref_stations/stations.in.
+  ref_checkpoint/       the reference's checkpoint file
+                        (CheckPoint::write_checkpoint) of a small deck at
+                        cycle 6, gzip-compressed, run with the interleaved
+                        layout (corder=0) in which the reference extracts
+                        the fields it writes (Sarray.C:616-633); the deck is
+                        ref_checkpoint/checkpoint.in.
 
 Usage: python tests/golden/make_ref_runs.py [name ...]
 """
@@ -48,7 +60,57 @@ def c1():
     print("c1:", res["nsteps"], "steps, dt", res["dt"], "linf", li, "l2", l2, "solinf", xi)
 
 
-RUNS = {"c1": c1}
+STATIONS_DECK = """grid h=100 x=3000 y=3000 z=1700
+time steps=40
+supergrid gp=10
+block vp=6000 vs=3464 rho=2700
+block vp=4000 vs=2000 rho=2600 z2=1000
+source x=1700 y=1900 z=600 mxy=1e18 mxx=3e17 t0=0.36 freq=16.6667 type=Gaussian
+developer cfl=1.3 corder=1
+rec x=2100 y=2300 z=0 file=sta10 usgsformat=1 sacformat=1
+rec x=2100 y=2300 z=0 file=sta10e nsew=1 usgsformat=1 sacformat=1
+rec x=2150 y=1340 z=200 file=stb usgsformat=1 sacformat=1 variables=velocity
+rec x=2150 y=1340 z=200 file=stbe sta=STB nsew=1 usgsformat=1 sacformat=1 variables=velocity
+"""
+
+CHECKPOINT_DECK = """grid h=100 x=1300 y=1200 z=1100
+time steps=9
+supergrid gp=4
+block vp=6000 vs=3464 rho=2700
+block vp=4000 vs=2000 rho=2600 z2=500
+source x=600 y=650 z=400 mxy=1e18 t0=0.2 freq=16.6667 type=Gaussian
+developer cfl=1.3 corder=0
+checkpoint cycle=6 file=ck
+"""
+
+
+def stations():
+    import shutil
+    out = os.path.join(HERE, "ref_stations")
+    os.makedirs(out, exist_ok=True)
+    with R.RefEW(STATIONS_DECK) as e:
+        for f in sorted(os.listdir(e.path)):
+            shutil.copy(os.path.join(e.path, f), os.path.join(out, f))
+    with open(os.path.join(out, "stations.in"), "w") as f:
+        f.write(STATIONS_DECK)
+    print("stations:", sorted(os.listdir(out)))
+
+
+def checkpoint():
+    import gzip
+    out = os.path.join(HERE, "ref_checkpoint")
+    os.makedirs(out, exist_ok=True)
+    with R.RefEW(CHECKPOINT_DECK) as e:
+        names = [f for f in os.listdir(e.path) if f.endswith(".sw4checkpoint")]
+        for f in names:
+            with open(os.path.join(e.path, f), "rb") as src, gzip.open(os.path.join(out, f + ".gz"), "wb") as dst:
+                dst.write(src.read())
+    with open(os.path.join(out, "checkpoint.in"), "w") as f:
+        f.write(CHECKPOINT_DECK)
+    print("checkpoint:", names)
+
+
+RUNS = {"c1": c1, "stations": stations, "checkpoint": checkpoint}
 
 if __name__ == "__main__":
     O.build(ref=True)

@@ -1,0 +1,119 @@
+"""Input decks, receiver files and checkpoint files (no GPU).
+
+The reference program (compiled from its sources in the container that holds
+them, tests/golden/make_ref_runs.py) wrote the files under
+tests/golden/ref_stations (TimeSeries::writeFile: USGS text and binary SAC,
+displacement and velocity, x/y/z and east/north/up components) and
+tests/golden/ref_checkpoint (CheckPoint::write_checkpoint).  The library's
+writers must reproduce those files byte for byte from the same samples:
+the x/y/z samples are read back from the reference's own USGS files (17
+significant digits: every double exactly), the reference time from their
+Date line.  The checkpoint reader and writer must reproduce the reference's
+checkpoint file byte for byte.  The decks of the reference's test suite
+(tests/golden/decks) parse to the grids the reference sets up."""
+import gzip
+import os
+import re
+
+import numpy as np
+import pytest
+
+from common import ROOT
+from sw4lite_amd import ew
+from sw4lite_amd import lib as L
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+REF = os.path.join(GOLDEN, "ref_stations")
+
+
+def read_usgs(fn):
+    """(utc[7], samples [n, 4] with time first) of a USGS text file."""
+    text = open(fn).read()
+    m = re.search(r"# Date: UTC  (\d+)/(\d+)/(\d+):(\d+):(\d+):(\d+)\.(\d+)", text)
+    month, day, year, hour, minute, sec, ms = (int(v) for v in m.groups())
+    return (year, month, day, hour, minute, sec, ms), np.loadtxt(fn, comments="#")
+
+
+# (file, x/y/z source file of the samples, options) of the reference's stations deck
+STATIONS = [("sta10", "sta10", dict(mode=0, nsew=False, loc=(2100.0, 2300.0, 0.0), grid_point=(2100.0, 2300.0, 0.0))),
+            ("sta10e", "sta10", dict(mode=0, nsew=True, loc=(2100.0, 2300.0, 0.0), grid_point=(2100.0, 2300.0, 0.0))),
+            ("stb", "stb", dict(mode=1, nsew=False, loc=(2150.0, 1340.0, 200.0), grid_point=(2100.0, 1300.0, 200.0))),
+            ("stbe", "stb", dict(mode=1, nsew=True, sta="STB", loc=(2150.0, 1340.0, 200.0),
+                                 grid_point=(2100.0, 1300.0, 200.0)))]
+
+
+@pytest.mark.parametrize("name,src,opts", STATIONS)
+def test_station_files_match_reference(tmp_path, name, src, opts):
+    utc, data = read_usgs(os.path.join(REF, src + ".txt"))
+    rep = ew.parse_deck(open(os.path.join(REF, "stations.in")).read())
+    assert rep["nsteps"] == 40 and (rep["nx"], rep["ny"], rep["nz"]) == (31, 31, 18)
+    # dt of the deck (EW::computeDT: CFL h / sqrt(max (4 mu + lambda)/rho),
+    # the half-space block vp 6000, vs 3464, rho 2700 is the fastest)
+    rho, cs, cp = 2700.0, 3464.0, 6000.0
+    mu = rho * cs * cs
+    la = rho * cp * cp - 2 * mu
+    dt = 1.3 * 100.0 * (1 / np.sqrt((4 * mu + la) / rho))
+    sac_dt = np.frombuffer(open(os.path.join(REF, src + (".xv" if opts["mode"] else ".x")), "rb").read()[:4], "<f4")
+    assert np.float32(dt) == sac_dt[0]
+    ew.write_station(str(tmp_path), name, data[:, 1:], dt, utc=utc, usgs=True, sac=True, **opts)
+    exts = ["txt"] + ([c + ("v" if opts["mode"] else "") for c in ("enu" if opts["nsew"] else "xyz")])
+    for ext in exts:
+        got = open(os.path.join(tmp_path, name + "." + ext), "rb").read()
+        want = open(os.path.join(REF, name + "." + ext), "rb").read()
+        assert len(got) == len(want), ext
+        if got != want:
+            diff = [i for i in range(len(got)) if got[i] != want[i]]
+            raise AssertionError("%s.%s differs from the reference's file at bytes %s" % (name, ext, diff[:10]))
+
+
+def test_checkpoint_file_matches_reference(tmp_path):
+    deck = open(os.path.join(GOLDEN, "ref_checkpoint", "checkpoint.in")).read()
+    rep = ew.parse_deck(deck)
+    nx, ny, nz = rep["nx"], rep["ny"], rep["nz"]
+    assert (nx, ny, nz) == (14, 13, 12) and rep["ncheckpoints"] == 1
+    raw = gzip.open(os.path.join(GOLDEN, "ref_checkpoint", "ck.cycle=6.sw4checkpoint.gz"), "rb").read()
+    src = tmp_path / "ref.sw4checkpoint"
+    src.write_bytes(raw)
+    t, cyc, um, u = ew.read_checkpoint(str(src), nx, ny, nz)
+    assert cyc == 6 and t > 0
+    assert np.max(np.abs(u)) > 0 and np.max(np.abs(um)) > 0
+    out = tmp_path / "ours.sw4checkpoint"
+    ew.write_checkpoint(str(out), t, cyc, nx, ny, nz, um, u)
+    assert out.read_bytes() == raw
+    # header: precision 8, one grid, time, cycle, 1 nx 1 ny 1 nz
+    hdr = np.frombuffer(raw[:8], "<i4").tolist() + [float(np.frombuffer(raw[8:16], "<f8")[0])] + \
+        np.frombuffer(raw[16:44], "<i4").tolist()
+    assert hdr == [8, 1, t, 6, 1, nx, 1, ny, 1, nz]
+    assert len(raw) == 44 + 2 * 3 * nx * ny * nz * 8
+
+
+def test_checkpoint_rejects_other_grids(tmp_path):
+    n = 3 * 16 * 16 * 16
+    ew.write_checkpoint(str(tmp_path / "a"), 1.5, 3, 12, 12, 12, np.zeros(n), np.ones(n))
+    t, c, um, u = ew.read_checkpoint(str(tmp_path / "a"), 12, 12, 12)
+    assert (t, c) == (1.5, 3) and u.sum() == 3 * 12 ** 3 and um.sum() == 0
+    with pytest.raises(L.Sw4HipError, match="do not match"):
+        ew.read_checkpoint(str(tmp_path / "a"), 12, 13, 12)
+
+
+@pytest.mark.parametrize("deck,grid,steps,tmax", [
+    ("LOH.1-h100.in", (301, 301, 171, 100.0), -1, 9.0),
+    ("pointsource.in", (201, 201, 101, 0.04), -1, 0.6),
+])
+def test_reference_decks_parse(deck, grid, steps, tmax):
+    rep = ew.parse_deck(open(os.path.join(GOLDEN, "decks", deck)).read())
+    assert (rep["nx"], rep["ny"], rep["nz"], rep["h"]) == grid
+    assert rep["nsteps"] == steps and rep["t"] == tmax
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("grid h=1 x=20 y=20 z=20\ntime t=1\ntopography input=gaussian zmax=2\n", "topography"),
+    ("grid h=1 x=20 y=20 z=20\ntime t=1\nsource x=1 y=1 z=1 fx=1 type=Ricker\n", "point forces"),
+    ("grid h=1 x=20 y=20 z=20\ntime t=1\nsource x=1 y=1 z=1 mxx=1 type=Brune\n", "Brune"),
+    ("grid h=1 x=20 y=20 z=20\ntime t=1 steps=3\n", "both t and steps"),
+    ("grid h=1 x=20 y=20 z=20\ntime t=1\nimage mode=mag z=0\n", "command not supported: image"),
+    ("time t=1\n", "no grid"),
+])
+def test_decks_refused(text, msg):
+    with pytest.raises(L.Sw4HipError, match=msg):
+        ew.parse_deck(text)
