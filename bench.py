@@ -434,33 +434,33 @@ def bench_fp32(a, e, torch, n, fp64_value):
     }
 
 
-def main_precision(a, torch):
-    """The workload model at twice the resolution (h = 10 m, the smoothing
-    length of the perturbation kept at 170 m), run once with the FP64 fused
-    step and once with the FP32 form with the compensated update from the
-    same zero initial state; 64 seeded surface receivers record displacement
-    after every step.  The four double couples sit 200-800 m deep and the
-    receivers within 1.5 km of their epicentres, so that P and S arrivals
-    reach the receivers within the 800 steps (1.07 s).  Reported: per
-    component, the relative L2 error of the FP32 traces against the FP64
-    traces over all receivers and samples, and the maximum error relative to
-    the largest FP64 amplitude."""
+def precision_traces(torch, n, steps, t0=0.24, log=True, depth=(200.0, 800.0), radius=1500.0):
+    """The precision workload at n^3 (h = 10 m * 1024/n, the smoothing length
+    of the perturbation kept at 170 m), run once with the FP64 fused step and
+    once with the FP32 form with the compensated update from the same zero
+    initial state; 64 seeded surface receivers record displacement after
+    every step.  The four double couples (Ricker 5 Hz centred at t0) sit
+    depth[0]..depth[1] m deep (200-800) and the receivers within radius m
+    (1.5 km) of their epicentres.
+    Returns ({0: FP64 traces, 2: FP32 traces} [steps+1, 64, 3], seconds per
+    mode, dt, receivers)."""
     from sw4lite_amd import EW
     from sw4lite_amd import lib as L
-    n, h, steps = a.n, 10.0 * 1024 / a.n, a.steps
+    h = 10.0 * 1024 / n
     span = (n - 1) * h
     rng = np.random.default_rng(4)
     srcs = []
     for _ in range(4):
         x, y = rng.uniform(0.2 * span, 0.8 * span), rng.uniform(0.2 * span, 0.8 * span)
-        srcs.append(dict(x=x, y=y, z=rng.uniform(200.0, 800.0), m=double_couple(rng, 1e15), t0=0.24, freq=5.0,
+        srcs.append(dict(x=x, y=y, z=rng.uniform(*depth), m=double_couple(rng, 1e15), t0=t0, freq=5.0,
                          tfkind=0))
     recs = []
     for r in range(64):
         s0 = srcs[r % 4]
-        rad, ang = rng.uniform(0.0, 1500.0), rng.uniform(0.0, 2 * np.pi)
+        rad, ang = rng.uniform(0.0, radius), rng.uniform(0.0, 2 * np.pi)
         recs.append((s0["x"] + rad * np.cos(ang), s0["y"] + rad * np.sin(ang), 0.0))
     traces, secs = {}, {}
+    dt = None
     for mode in (0, 2):
         rho, mu, la = make_material(torch, n, h, seed=4, width=33)
         torch.cuda.empty_cache()
@@ -474,18 +474,31 @@ def main_precision(a, torch):
             e.add_receiver(x, y, z, 0)
         e.setup(nsteps=steps)
         dt = e.info().dt
-        done, t0 = 0, time.perf_counter()
+        done, c0 = 0, time.perf_counter()
         while done < steps:
             chunk = min(100, steps - done)
             e.timesteploop(chunk, mode=mode)
             e.sync()
             done += chunk
-            sys.stderr.write("precision: mode %d, %d/%d steps, %.1f s\n" % (mode, done, steps, time.perf_counter() - t0))
-            sys.stderr.flush()
-        secs[mode] = time.perf_counter() - t0
+            if log:
+                sys.stderr.write("precision: mode %d, %d/%d steps, %.1f s\n" % (mode, done, steps,
+                                                                              time.perf_counter() - c0))
+                sys.stderr.flush()
+        secs[mode] = time.perf_counter() - c0
         traces[mode] = e.receivers()
         e.close()
         torch.cuda.empty_cache()
+    return traces, secs, dt, recs
+
+
+def main_precision(a, torch):
+    """The precision workload (precision_traces at n^3, a.steps steps).
+    Reported: per component, the relative L2 error of the FP32 traces against
+    the FP64 traces over all receivers and samples, and the maximum error
+    relative to the largest FP64 amplitude."""
+    n, steps = a.n, a.steps
+    h = 10.0 * 1024 / n
+    traces, secs, dt, recs = precision_traces(torch, n, steps)
     ref, got = traces[0], traces[2]
     d = got - ref
     rel_l2 = [float(np.linalg.norm(d[..., c]) / np.linalg.norm(ref[..., c])) for c in range(3)]

@@ -97,6 +97,20 @@ int sw4hip_rhs4sg(int corder, int ifirst, int ilast, int jfirst, int jlast, int 
                   const double* lambda, double h, const double* strx, const double* stry,
                   const double* strz, void* stream);
 
+/* Replaces EW::Force (EW.C:3085-3123) with its GridPointSource::getFxyz /
+ * getFxyz_tt (GridPointSource.C:330-586): F = 0 on the box, then each of the
+ * nsrc grid-point sources adds force[3s..3s+2] * g(t - t0[s]) (tt != 0: g'')
+ * at (ijk[3s], ijk[3s+1], ijk[3s+2]); g the time function tfkind[s] (0
+ * Ricker, 1 Gaussian, 2 C6SmoothBump) of frequency freq[s].  The source
+ * table is HOST data (ijk, force, freq, t0, tfkind), summed per grid point
+ * in the order the reference's m_identsources gives (sorted by point,
+ * EW::sort_grid_point_sources, EW.C:6370).  The library keeps a device copy
+ * per distinct table (up to 16 per device; calls with new tables beyond
+ * that wait for the device and start a new set).  F: device array. */
+int sw4hip_force(int corder, int ib, int ie, int jb, int je, int kb, int ke, double t, int tt, int nsrc,
+                 const int* ijk, const double* force, const double* freq, const double* t0, const int* tfkind,
+                 double* F, void* stream);
+
 /* Replaces EW::predfort (ew-cfromfort.C:70-113): up = 2u - um + dt2/rho (lu + fo). */
 int sw4hip_predfort(int corder, int ib, int ie, int jb, int je, int kb, int ke, double* up,
                     const double* u, const double* um, const double* lu, const double* fo,

@@ -81,5 +81,20 @@ def build(verbose=False, jobs=None):
     return LIB
 
 
+def build_option_a(verbose=False):
+    """Compile the kernel-level drop-in test program tests/option_a/option_a
+    (C++ against include/sw4hip.h, linked to libsw4hip.so)."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "option_a", "option_a.cpp")
+    exe = os.path.join(root, "tests", "option_a", "option_a")
+    cmd = [hipcc(), "--offload-arch=gfx950", "-O2", "-std=c++17", "-I", os.path.join(root, "include"), src, "-o",
+           exe + ".tmp", "-L", HERE, "-lsw4hip", "-Wl,-rpath,$ORIGIN/../../sw4lite_amd"]
+    if _run(cmd, verbose):
+        raise RuntimeError("hipcc failed building tests/option_a/option_a")
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_option_a(verbose=True))

@@ -1,4 +1,5 @@
 // abi.cpp -- extern "C" entry points declared in include/sw4hip.h.
+#include <algorithm>
 #include <cstring>
 #include <fstream>
 #include <sstream>
@@ -55,6 +56,87 @@ int device_sbp(const SbpTables& t, const SbpTables** out)
    }
    c.push_back(SbpEntry{t, static_cast<SbpTables*>(p)});
    *out = c.back().dev;
+   return OK;
+}
+
+// Device copy of a grid-point source table, sorted by grid point as
+// EW::sort_grid_point_sources does (stable, so sources at one point keep
+// their order), packed as launch_force reads it.  One immutable buffer per
+// distinct table and device (up to 16; then the device is synchronised and
+// the set starts over).
+struct SrcEntry {
+   std::vector<char> key;
+   void* dev;
+   int nunique, n;
+};
+std::mutex g_src_mutex;
+std::vector<SrcEntry> g_src_cache[64];
+
+int device_source_table(const Box& b, int n, const int* ijk, const double* force, const double* freq,
+                        const double* t0, const int* tfkind, SourceTable* out)
+{
+   int dev = 0;
+   int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+   if (rc) return rc;
+   if (dev < 0 || dev >= 64) return set_error(E_RANGE, "device ordinal beyond 63");
+   *out = SourceTable{0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+   if (n == 0) return OK;
+   std::vector<int> order(n);
+   for (int q = 0; q < n; q++) order[q] = q;
+   auto key = [&](int q) { return sidx(b, ijk[3 * q], ijk[3 * q + 1], ijk[3 * q + 2]); };
+   std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return key(a) < key(c); });
+   std::vector<int> first;
+   for (int s = 0; s < n; s++)
+      if (s == 0 || key(order[s]) != key(order[s - 1])) first.push_back(s);
+   const int nu = (int)first.size();
+   first.push_back(n);
+   // packed: first[nu+1] | ijk[3n] | tfkind[n] | force[3n] | freq[n] | t0[n]
+   const size_t ibytes = sizeof(int) * (nu + 1 + 4 * n), ioff = (ibytes + 15) / 16 * 16;
+   std::vector<char> host(ioff + sizeof(double) * 5 * n, 0);
+   int* ip = reinterpret_cast<int*>(host.data());
+   std::memcpy(ip, first.data(), sizeof(int) * (nu + 1));
+   int* pijk = ip + nu + 1;
+   int* ptf = pijk + 3 * n;
+   double* dp = reinterpret_cast<double*>(host.data() + ioff);
+   for (int s = 0; s < n; s++) {
+      const int q = order[s];
+      for (int c = 0; c < 3; c++) {
+         pijk[3 * s + c] = ijk[3 * q + c];
+         dp[3 * s + c] = force[3 * q + c];
+      }
+      ptf[s] = tfkind[q];
+      dp[3 * n + s] = freq[q];
+      dp[4 * n + s] = t0[q];
+   }
+   std::lock_guard<std::mutex> lock(g_src_mutex);
+   std::vector<SrcEntry>& c = g_src_cache[dev];
+   const SrcEntry* hit = nullptr;
+   for (const SrcEntry& e : c)
+      if (e.key == host) hit = &e;
+   if (!hit) {
+      if (c.size() >= 16) {
+         if ((rc = check_hip(hipDeviceSynchronize(), "source table cache"))) return rc;
+         for (SrcEntry& e : c) (void)hipFree(e.dev);
+         c.clear();
+      }
+      void* p = nullptr;
+      if (hipMalloc(&p, host.size()) != hipSuccess) return set_error(E_NOMEM, "source table allocation failed");
+      if ((rc = check_hip(hipMemcpy(p, host.data(), host.size(), hipMemcpyHostToDevice), "source table upload"))) {
+         (void)hipFree(p);
+         return rc;
+      }
+      c.push_back(SrcEntry{host, p, nu, n});
+      hit = &c.back();
+   }
+   const int* di = static_cast<const int*>(hit->dev);
+   const double* dd = reinterpret_cast<const double*>(static_cast<const char*>(hit->dev) + ioff);
+   out->nunique = nu;
+   out->first = di;
+   out->ijk = di + nu + 1;
+   out->tfkind = di + nu + 1 + 3 * n;
+   out->force = dd;
+   out->freq = dd + 3 * n;
+   out->t0 = dd + 4 * n;
    return OK;
 }
 
@@ -236,6 +318,27 @@ int sw4hip_rhs4sg(int corder, int ifirst, int ilast, int jfirst, int jlast, int 
    if ((rc = device_sbp(t, &d))) return rc;
    return launch_rhs4sg(corder, make_box(ifirst, ilast, jfirst, jlast, kfirst, klast), nk, onesided, d, lu, u, mu,
                         lambda, h, strx, stry, strz, (hipStream_t)stream);
+}
+
+int sw4hip_force(int corder, int ib, int ie, int jb, int je, int kb, int ke, double t, int tt, int nsrc,
+                 const int* ijk, const double* force, const double* freq, const double* t0, const int* tfkind,
+                 double* F, void* stream)
+{
+   if (corder != 0 && corder != 1) return set_error(E_INVAL, "corder must be 0 or 1");
+   if (ie < ib || je < jb || ke < kb || !F) return set_error(E_INVAL, "force: bad box or null F");
+   if (nsrc < 0 || (nsrc > 0 && (!ijk || !force || !freq || !t0 || !tfkind)))
+      return set_error(E_INVAL, "force: null source table");
+   for (int q = 0; q < nsrc; q++) {
+      if (ijk[3 * q] < ib || ijk[3 * q] > ie || ijk[3 * q + 1] < jb || ijk[3 * q + 1] > je || ijk[3 * q + 2] < kb ||
+          ijk[3 * q + 2] > ke)
+         return set_error(E_INVAL, "force: grid-point source outside the box");
+      if (tfkind[q] < 0 || tfkind[q] > 2) return set_error(E_INVAL, "force: time function must be 0, 1 or 2");
+   }
+   const Box b = make_box(ib, ie, jb, je, kb, ke);
+   SourceTable st;
+   int rc;
+   if ((rc = device_source_table(b, nsrc, ijk, force, freq, t0, tfkind, &st))) return rc;
+   return launch_force(corder, b, F, st, t, tt ? 1 : 0, true, (hipStream_t)stream);
 }
 
 int sw4hip_predfort(int corder, int ib, int ie, int jb, int je, int kb, int ke, double* up, const double* u,

@@ -145,6 +145,22 @@ def rhs4sg(corder, box, nk, onesided, acof, bope, ghcof, lu, u, mu, la, h, strx,
                               _stream(stream)))
 
 
+def force(corder, box, t, tt, ijk, frc, freq, t0, tfkind, F, stream=None):
+    """EW::Force for one grid: F (device) = sum of the grid-point sources
+    (host arrays ijk [n,3], frc [n,3], freq, t0, tfkind [n]) at time t (tt:
+    second time derivative)."""
+    ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1)
+    frc = np.ascontiguousarray(frc, dtype=np.float64).reshape(-1)
+    n = ijk.size // 3
+    fq = np.ascontiguousarray(freq, dtype=np.float64)
+    t0_ = np.ascontiguousarray(t0, dtype=np.float64)
+    tf = np.ascontiguousarray(tfkind, dtype=np.int32)
+    if not (frc.size == 3 * n and fq.size == n and t0_.size == n and tf.size == n):
+        raise ValueError("source table arrays disagree in length")
+    L.check(L.load().sw4hip_force(corder, *box, float(t), int(bool(tt)), n, _ptr(ijk), _ptr(frc), _ptr(fq), _ptr(t0_),
+                                  _ptr(tf), _ptr(F), _stream(stream)))
+
+
 def predfort(corder, box, up, u, um, lu, fo, rho, dt2, stream=None):
     L.check(L.load().sw4hip_predfort(corder, *box, _ptr(up), _ptr(u), _ptr(um), _ptr(lu), _ptr(fo), _ptr(rho),
                                      dt2, _stream(stream)))

@@ -49,6 +49,7 @@ _SIGS = {
     "sw4hip_host_boundary_windows": (I, [C.POINTER(GridDesc), P, P]),
     "sw4hip_host_moment_source": (I, [C.POINTER(GridDesc), D, D, D, P, I, IP, P, P]),
     "sw4hip_rhs4sg": (I, [I] * 8 + [P, P, P, P, P, P, P, P, D, P, P, P, P]),
+    "sw4hip_force": (I, [I] * 7 + [D, I, I, P, P, P, P, P, P, P]),
     "sw4hip_predfort": (I, [I] * 7 + [P] * 6 + [D, P]),
     "sw4hip_corrfort": (I, [I] * 7 + [P] * 4 + [D, P]),
     "sw4hip_dpdmtfort": (I, [I] * 6 + [P] * 4 + [D, P]),

@@ -80,6 +80,35 @@ def test_pointwise_updates(torch, corder):
     assert relerr(host(ua), want) < TOL
 
 
+@pytest.mark.parametrize("corder,tt", [(0, 0), (1, 0), (1, 1)])
+def test_force(torch, corder, tt):
+    """sw4hip_force (EW::Force) against the oracle's: 40 grid-point sources
+    of the three time functions, several at the same grid points (summed in
+    the table's order), a second call with another table reusing none of
+    the first one's device copy, and the same table again."""
+    from sw4lite_amd import ew
+    box = (-1, 22, -1, 20, -1, 18)
+    npts = 24 * 22 * 20
+    for call, seed in enumerate((31, 32, 31)):
+        rng = np.random.default_rng(seed + 10 * corder + 100 * tt)
+        n = 40
+        ijk = np.stack([rng.integers(1, 21, n), rng.integers(1, 19, n), rng.integers(1, 17, n)], axis=1)
+        ijk[5:12] = ijk[4]  # a point with eight sources
+        frc = rng.uniform(-1, 1, (n, 3))
+        freq = rng.uniform(2.0, 9.0, n)
+        t0 = rng.uniform(0.0, 0.3, n)
+        tf = rng.integers(0, 3, n)
+        t = 0.21
+        want = np.zeros(3 * npts)
+        O.force(corder, box, want, ijk, frc, freq, t0, tf, t, tt)
+        got = dev(torch, np.full(3 * npts, 5.0))  # every value is overwritten
+        ew.force(corder, box, t, tt, ijk, frc, freq, t0, tf, got)
+        assert np.count_nonzero(want) > 0
+        err = relerr(host(got), want)
+        print("force corder", corder, "tt", tt, "call", call, "relerr", err, flush=True)
+        assert err < 1e-14
+
+
 @pytest.mark.parametrize("corder", [0, 1])
 def test_addsgd4(torch, corder):
     from sw4lite_amd import ew
