@@ -75,25 +75,30 @@ def parse():
                         "7 as 6 with the predictor planes loaded asynchronously into LDS (even row lengths), "
                         "8 as 7 with the acceleration stored by the predictor and staged by the corrector, "
                         "9 as 8 with the damping sweep only over the supergrid layers")
-    p.add_argument("--kchunk", type=int, default=64, help="planes per workgroup of the LDS sweeps")
+    p.add_argument("--kchunk", type=int, default=0,
+                   help="planes per workgroup of the LDS sweeps (0: the library's defaults, 128 for the predictor "
+                        "and corrector sweeps, 32 for the damping sweep)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also launch a plain device copy of 1 GiB (PMC scaling)")
-    p.add_argument("--workload", choices=["cartesian", "curvilinear", "precision", "adjoint", "batch"],
+    p.add_argument("--workload", choices=["cartesian", "curvilinear", "precision", "adjoint", "batch",
+                                          "batch-adjoint"],
                    default="cartesian",
                    help="cartesian: the full time step at N^3 (default); curvilinear: the curvilinear-grid "
                         "kernels (rhs4sgcurv, addsgd4c, free-surface closure) on a 384x384x256 grid under "
                         "Gaussian hills; precision: FP32 compensated against FP64 on 64 surface receivers "
                         "(1024^3, h=10 m, 800 steps); adjoint: gradient of a receiver misfit against the forward run "
                         "(256^3, h=25 m, 600 steps, 128 surface receivers); batch: 48 shots of N^3, N uniform in "
-                        "128..288, h=25 m, stepped alone, batched, batched with graph replay")
+                        "128..288, h=25 m, stepped alone, batched, batched with graph replay; batch-adjoint: the "
+                        "gradients of every shot of the batch workload against the batched forward run")
     p.add_argument("--ckpt", type=int, default=0,
                    help="adjoint workload: steps between checkpoints (0: from free HBM, the whole history when it fits)")
-    p.add_argument("--shots", type=int, default=48, help="batch workload: number of shots")
+    p.add_argument("--shots", type=int, default=48, help="batch workloads: number of shots")
+    p.add_argument("--concurrent", type=int, default=4, help="batch-adjoint workload: shots whose gradients run at once")
     a = p.parse_args()
     if a.n is None:
         a.n = {"precision": 1024, "adjoint": 256}.get(a.workload, 512)
     if a.steps is None:
-        a.steps = {"precision": 800, "adjoint": 600, "batch": 30}.get(a.workload, 100)
+        a.steps = {"precision": 800, "adjoint": 600, "batch": 30, "batch-adjoint": 100}.get(a.workload, 100)
     return a
 
 
@@ -679,6 +684,97 @@ def main_batch(a, torch):
     print(json.dumps(line), flush=True)
 
 
+def main_batch_adjoint(a, torch):
+    """The batched-adjoint workload: --shots grids of N^3 points, N drawn
+    uniformly from 128..288 (seeded), h = 25 m, workload material, one shallow
+    double couple (Ricker 5 Hz) and 128 surface receivers each.  The observed
+    data d_obs come from the same shots with the material raised by 3% in a
+    Gaussian blob (width 15% of the grid) at the centre; the misfit is
+    J = 1/2 |d - d_obs|^2 over all shots and samples.  Timed: the forward run
+    of all shots (batched, graph replay) and the gradient of J with respect
+    to rho, mu, lambda of every shot (sw4hip_batch_gradient: up to
+    --concurrent shots at once, checkpoints every --ckpt steps, default 50).
+    value = gradient seconds / forward seconds."""
+    from sw4lite_amd import EW
+    from sw4lite_amd import lib as L
+    from sw4lite_amd.ew import Batch
+    rng = np.random.default_rng(8)
+    h, steps = 25.0, a.steps
+    ckpt = a.ckpt if a.ckpt > 0 else 50
+    sizes = [int(v) for v in rng.integers(128, 289, size=a.shots)]
+    shots, mats = [], []
+    for q, n in enumerate(sizes):
+        rho, mu, la = make_material(torch, n, h, seed=100 + q)
+        e = EW(n, n, n, h, [L.BC_SUPERGRID] * 4 + [L.BC_FREE, L.BC_SUPERGRID], corder=1, sg_gp_thickness=30)
+        span = (n - 1) * h
+        e.add_moment_source(rng.uniform(0.3, 0.7) * span, rng.uniform(0.3, 0.7) * span, rng.uniform(200, 1000),
+                            double_couple(rng, 1e15), 0.24, 5.0, 0)
+        for _ in range(128):
+            e.add_receiver(rng.uniform(0.15, 0.85) * span, rng.uniform(0.15, 0.85) * span, 0.0, 0)
+        # the blob-perturbed material for the observed data
+        k = torch.arange(-1, n + 3, device="cuda", dtype=torch.float64)
+        x = (k - 1) * h - 0.5 * span
+        r2 = x.view(-1, 1, 1) ** 2 + x.view(1, -1, 1) ** 2 + x.view(1, 1, -1) ** 2
+        blob = (1 + 0.03 * torch.exp(-r2 / (2 * (0.15 * span) ** 2))).view(-1)
+        e.set_material(rho * blob, mu * blob, la * blob)
+        e.setup(nsteps=steps)
+        e.reset()
+        e.timesteploop(steps, mode=0)
+        dobs = e.receivers()[: steps + 1]
+        e.set_material(rho, mu, la)
+        e.sync()
+        del rho, mu, la, blob, r2
+        shots.append(e)
+        mats.append(dobs)
+        sys.stderr.write("batch-adjoint: shot %d/%d (%d^3) set up\n" % (q + 1, a.shots, n))
+        sys.stderr.flush()
+    torch.cuda.empty_cache()
+    pts = float(sum(n ** 3 for n in sizes))
+    b = Batch(shots)
+    b.set_kchunk(32)
+
+    def forward():
+        for e in shots:
+            e.reset()
+        b.step(steps, graph=True)
+        for e in shots:
+            e.sync()
+
+    forward()  # warm-up and graph capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    forward()
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    gtr = [e.receivers()[: steps + 1] - d for e, d in zip(shots, mats)]
+    J = 0.5 * sum(float(np.sum(g * g)) for g in gtr)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grads = b.gradient(gtr, steps, ckpt=ckpt, max_concurrent=a.concurrent)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    gmax = [max(float(np.max(np.abs(g[c]))) for g in grads) for c in range(3)]
+    b.close()
+    for e in shots:
+        e.close()
+    line = {
+        "metric": "batched adjoint/forward cost ratio (gradients of a receiver misfit w.r.t. rho, mu, lambda)",
+        "value": t_grad / t_fwd, "unit": "ratio", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": 1e3 * t_grad / steps, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {
+            "workload": "%d shots of N^3, N uniform in 128..288 (seeded), h=25 m, layered+lognormal material, free "
+                        "top, supergrid 30, one double couple, 128 surface receivers each; d_obs from the material "
+                        "+3%% in a central Gaussian blob" % a.shots,
+            "sizes": sizes, "total_points": pts, "checkpoint_every": ckpt, "concurrent_shots": a.concurrent,
+            "forward_s": t_fwd, "gradient_s": t_grad, "misfit": J,
+            "forward_grid_pt_updates_per_s": pts * steps / t_fwd,
+            "max_abs_gradient": {"rho": gmax[0], "mu": gmax[1], "lambda": gmax[2]},
+        },
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -698,6 +794,8 @@ def main():
         return main_adjoint(a, torch)
     if a.workload == "batch":
         return main_batch(a, torch)
+    if a.workload == "batch-adjoint":
+        return main_batch_adjoint(a, torch)
     n, h = a.n, 20.0
     srcs = sources(n, h, seed=2)
     rho, mu, la = make_material(torch, n, h, seed=2)

@@ -259,17 +259,21 @@ int sw4hip_ew_destroy(sw4hip_ew* ew);
 /* Stream used by every later call on this object (hipStream_t). */
 int sw4hip_ew_set_stream(sw4hip_ew* ew, void* stream);
 /* Interior kernels of the fused sweeps: variant 0 one thread per point;
- * variants 1-7 LDS-staged k-marching tiles with kchunk planes per workgroup
- * (default 64): 1 plain at 2 waves/SIMD, 2 register prefetch of the next
+ * variants 1-9 LDS-staged k-marching tiles with kchunk planes per workgroup:
+ * 1 plain at 2 waves/SIMD, 2 register prefetch of the next
  * plane, 3 prefetch without a register cap, 4 as 3 with the supergrid
  * damping inside the corrector sweep, 5 corrector as 4 and predictor as 1,
  * 6 as 1 with the damping as its own LDS sweep, 7 as 6 with the
  * predictor's planes brought to LDS by asynchronous loads one plane ahead
- * (even row lengths; otherwise as 6), 8 (the default) as 7 with the
+ * (even row lengths; otherwise as 6), 8 as 7 with the
  * acceleration (Up - 2U + Um)/dt^2 stored by the predictor (and its ghost
  * shell) and brought to LDS by the corrector the same way (one more field of
  * device memory, allocated at the first step; even row lengths, otherwise as
- * 7; the same bits as 7).  All give the same results to rounding. */
+ * 7; the same bits as 7), 9 (the default) as 8 with the damping sweep only
+ * over the tiles and planes of the supergrid layers (the same bits as 8).
+ * All give the same results to rounding.  kchunk > 0 sets the planes per
+ * workgroup of every sweep; kchunk <= 0 keeps the defaults (128 for the
+ * predictor and corrector sweeps, 32 for the damping sweep). */
 int sw4hip_ew_set_kernel_variant(sw4hip_ew* ew, int variant, int kchunk);
 /* rho, mu, lambda over the whole box incl. ghosts; on_device selects whether
  * the pointers are device (1) or host (0) arrays. */
@@ -383,6 +387,16 @@ int sw4hip_batch_set_kchunk(sw4hip_batch* batch, int kchunk);
  * 16, 24, 32, 40, 48, 64, 96), keep the fastest (*best, per-candidate ms in ms[] when
  * not NULL), then reset every shot to zero data at t = 0. */
 int sw4hip_batch_tune(sw4hip_batch* batch, int ncand, const int* cand, int* best, double* ms);
+/* Gradients of every shot's receiver misfit with respect to rho, mu,
+ * lambda (sw4hip_ew_gradient of each shot: grad_traces[q], grho[q], gmu[q],
+ * glambda[q] as there; every shot run from zero data).  Up to
+ * max_concurrent shots run at once, each on a stream of its own, so that
+ * the kernels of small grids share the device; a shot starts when its work
+ * memory fits beside the running ones and releases it when done.
+ * ckpt < 1: about sqrt(nsteps) steps between checkpoints. */
+int sw4hip_batch_gradient(sw4hip_batch* batch, int nsteps, int ckpt, const double* const* grad_traces,
+                          double* const* grho, double* const* gmu, double* const* glambda, int on_device,
+                          int max_concurrent);
 /* Advance every shot by nsteps; use_graph: replay the captured period. */
 int sw4hip_batch_step(sw4hip_batch* batch, int nsteps, int use_graph);
 int sw4hip_ew_sync(sw4hip_ew* ew);

@@ -22,6 +22,28 @@ def npts(box):
     return (ie - ib + 1) * (je - jb + 1) * (ke - kb + 1)
 
 
+def curvilinear_eigmax(met, jac, rho, mu, la, corder, offdiag=1.0):
+    """max over the points of -w0 / (4 rho), w0 the most negative eigenvalue
+    of the principal-part matrix of the curvilinear operator (EW::computeDT,
+    EW.C:5063-5136; the reference's LAPACK dspev here numpy's eigvalsh).
+    met: 4 components over the box (layout corder), jac, rho, mu, la over the
+    box.  dt = CFL / sqrt(this).  offdiag scales the off-diagonal entries
+    (1 in the scheme; tests use other values to show that they matter)."""
+    n = jac.size
+    m = met.reshape(4, n) if corder == 1 else met.reshape(n, 4).T
+    l2, ji = la + 2 * mu, 1 / jac
+    s1, s2, s3, s4 = (m[c] ** 2 for c in range(4))
+    A = np.empty((n, 3, 3))
+    A[:, 0, 0] = -4 * (s1 * l2 + s1 * mu + s2 * l2 + s3 * mu + s4 * mu) * ji
+    A[:, 0, 1] = A[:, 1, 0] = -4 * offdiag * m[1] * m[2] * (mu + la) * ji
+    A[:, 0, 2] = A[:, 2, 0] = -4 * offdiag * m[1] * m[3] * (mu + la) * ji
+    A[:, 1, 1] = -4 * (s1 * mu + s1 * l2 + s2 * mu + s3 * l2 + s4 * mu) * ji
+    A[:, 1, 2] = A[:, 2, 1] = -4 * offdiag * m[2] * m[3] * (mu + la) * ji
+    A[:, 2, 2] = -4 * (2 * s1 * mu + s2 * mu + s3 * mu + s4 * l2) * ji
+    w0 = np.linalg.eigvalsh(A)[:, 0]
+    return np.max(-w0 / (4 * rho))
+
+
 class OracleRun:
     def __init__(self, nx, ny, nz, h, bc, rho, mu, la, sources, corder=1, sg_gp=20, beta=0.02, cfl=1.3,
                  tmax=0.0, nsteps=0, kernels="oracle", zgrid=None):
@@ -60,19 +82,7 @@ class OracleRun:
             eig = np.max((4 * self.mu + self.la) / self.rho)
             dt = min(1e10, cfl * h * (1 / np.sqrt(eig)))
         else:
-            m = self.met.reshape(4, n) if corder == 1 else self.met.reshape(n, 4).T
-            la_, mu_ = self.la, self.mu
-            l2, ji = la_ + 2 * mu_, 1 / self.jac
-            s1, s2, s3, s4 = (m[c] ** 2 for c in range(4))
-            A = np.empty((n, 3, 3))
-            A[:, 0, 0] = -4 * (s1 * l2 + s1 * mu_ + s2 * l2 + s3 * mu_ + s4 * mu_) * ji
-            A[:, 0, 1] = A[:, 1, 0] = -4 * m[1] * m[2] * (mu_ + la_) * ji
-            A[:, 0, 2] = A[:, 2, 0] = -4 * m[1] * m[3] * (mu_ + la_) * ji
-            A[:, 1, 1] = -4 * (s1 * mu_ + s1 * l2 + s2 * mu_ + s3 * l2 + s4 * mu_) * ji
-            A[:, 1, 2] = A[:, 2, 1] = -4 * m[2] * m[3] * (mu_ + la_) * ji
-            A[:, 2, 2] = -4 * (2 * s1 * mu_ + s2 * mu_ + s3 * mu_ + s4 * l2) * ji
-            w0 = np.linalg.eigvalsh(A)[:, 0]
-            eig = np.max(-w0 / (4 * self.rho))
+            eig = curvilinear_eigmax(self.met, self.jac, self.rho, self.mu, self.la, corder)
             dt = min(1e10, cfl * (1 / np.sqrt(eig)))
         if tmax > 0:
             nst = int(tmax / dt + 0.5)

@@ -105,7 +105,7 @@ long refew_array(void* ew, int g, int which, R* out)
       case 9: a = &e->mJ; break;
       default: return -1;
    }
-   const long n = (long)a->m_npts * a->m_nc;
+   const long n = (long)a->m_npts;  // m_npts counts the components too (Sarray.C:755)
    if (out) std::memcpy(out, a->c_ptr(), sizeof(R) * n);
    return n;
 }

@@ -6,4 +6,4 @@ this package holds the build script and thin Python bindings.
 """
 from .lib import (BC_DIRICHLET, BC_FREE, BC_PERIODIC, BC_SUPERGRID, TF_C6SMOOTHBUMP,  # noqa: F401
                   TF_GAUSSIAN, TF_RICKER, Sw4HipError, load)
-from .ew import EW  # noqa: F401
+from .ew import EW, Batch  # noqa: F401

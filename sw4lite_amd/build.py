@@ -75,7 +75,7 @@ def build(verbose=False, jobs=None):
     if any(rcs):
         raise RuntimeError("hipcc failed building libsw4hip.so")
     tmp = LIB + ".tmp"
-    if _run([hipcc()] + FLAGS + ["-shared", "-o", tmp] + objs, verbose):
+    if _run([hipcc()] + FLAGS + ["-shared", "-pthread", "-o", tmp] + objs, verbose):
         raise RuntimeError("hipcc failed linking libsw4hip.so")
     os.replace(tmp, LIB)
     return LIB

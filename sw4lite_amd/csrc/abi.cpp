@@ -700,9 +700,10 @@ int sw4hip_ew_set_kernel_variant(sw4hip_ew* h, int variant, int kchunk)
 {
    EWCHK(h);
    if (variant < 0 || variant > 9) return set_error(E_INVAL, "variant must be 0..9");
-   if (kchunk < 1 || kchunk > 4096) return set_error(E_INVAL, "kchunk must be in [1, 4096]");
+   if (kchunk > 4096) return set_error(E_INVAL, "kchunk must be at most 4096");
    h->ew->m_variant = variant;
-   h->ew->m_kchunk = kchunk;
+   h->ew->m_kchunk = kchunk > 0 ? kchunk : EW::kSweepChunk;
+   h->ew->m_kchunk_damp = kchunk > 0 ? kchunk : EW::kDampChunk;
    return OK;
 }
 
@@ -963,6 +964,14 @@ int sw4hip_batch_tune(sw4hip_batch* h, int ncand, const int* cand, int* best, do
 {
    if (!h || !h->b) return set_error(E_INVAL, "null batch");
    return h->b->tune(best, ms, ncand, cand);
+}
+
+int sw4hip_batch_gradient(sw4hip_batch* h, int nsteps, int ckpt, const double* const* grad_traces,
+                          double* const* grho, double* const* gmu, double* const* glambda, int on_device,
+                          int max_concurrent)
+{
+   if (!h || !h->b) return set_error(E_INVAL, "null batch");
+   return h->b->gradient(nsteps, ckpt, grad_traces, grho, gmu, glambda, on_device != 0, max_concurrent);
 }
 
 int sw4hip_batch_step(sw4hip_batch* h, int nsteps, int use_graph)

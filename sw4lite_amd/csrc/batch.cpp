@@ -11,6 +11,13 @@
 #include "batch.h"
 
 #include <algorithm>
+#include <cmath>
+#include <condition_variable>
+#include <mutex>
+#include <string>
+#include <thread>
+
+extern "C" const char* sw4hip_last_error(void);
 
 namespace sw4amd {
 
@@ -344,6 +351,84 @@ int Batch::tune(int* best, double* ms, int ncand, const int* cand)
    if (best) *best = kbest;
    m_kchunk = kbest;
    return refresh();
+}
+
+int Batch::gradient(int nsteps, int ckpt, const double* const* gtr, double* const* grho, double* const* gmu,
+                    double* const* gla, bool on_device, int max_concurrent)
+{
+   const int n = (int)m_shots.size();
+   if (!gtr || !grho || !gmu || !gla) return set_error(E_INVAL, "batch gradient: null argument");
+   if (max_concurrent < 1) max_concurrent = 1;
+   int dev = 0;
+   int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+   if (rc) return rc;
+   // the batch's own work must be done before the shots' streams take over
+   if ((rc = check_hip(hipStreamSynchronize(m_s), "batch gradient"))) return rc;
+   // every shot keeps ckpt + 2 segment states and two per checkpoint: with no
+   // interval given, about sqrt(nsteps), which keeps that near its least
+   if (ckpt < 1) ckpt = std::max(1, (int)std::lround(std::sqrt((double)nsteps)));
+   std::vector<double> need(n, 0.0);
+   for (int q = 0; q < n; q++)
+      if ((rc = m_shots[q]->gradient_plan(nsteps, ckpt, nullptr, &need[q]))) return rc;
+   size_t freeb = 0, totalb = 0;
+   if ((rc = check_hip(hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo"))) return rc;
+   const double budget = (double)freeb - 4.0 * (1 << 30) - 0.02 * (double)totalb;
+   std::mutex mtx;
+   std::condition_variable cv;
+   double inflight = 0.0;
+   int running = 0, next = 0, first_rc = OK;
+   std::string first_msg;
+   auto worker = [&]() {
+      (void)hipSetDevice(dev);
+      hipStream_t st = nullptr;
+      if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+         std::lock_guard<std::mutex> lk(mtx);
+         if (first_rc == OK) {
+            first_rc = E_HIP;
+            first_msg = "batch gradient: stream creation failed";
+         }
+         return;
+      }
+      for (;;) {
+         int q;
+         {
+            std::unique_lock<std::mutex> lk(mtx);
+            // start the next shot when it fits beside the running ones (or runs alone)
+            cv.wait(lk, [&]() {
+               return next >= n || first_rc != OK || running == 0 || inflight + need[next] <= budget;
+            });
+            if (next >= n || first_rc != OK) break;
+            q = next++;
+            running++;
+            inflight += need[q];
+         }
+         EW* e = m_shots[q];
+         const hipStream_t saved = e->m_stream;
+         e->m_stream = st;
+         int r = e->gradient(nsteps, ckpt, gtr[q], grho[q], gmu[q], gla[q], on_device);
+         const int r2 = e->release_workspace();
+         e->m_stream = saved;
+         if (r == OK) r = r2;
+         {
+            std::lock_guard<std::mutex> lk(mtx);
+            running--;
+            inflight -= need[q];
+            if (r != OK && first_rc == OK) {
+               first_rc = r;
+               first_msg = "batch gradient, shot " + std::to_string(q) + ": " + sw4hip_last_error();
+            }
+         }
+         cv.notify_all();
+      }
+      (void)hipStreamDestroy(st);
+      cv.notify_all();
+   };
+   const int nthreads = std::min(max_concurrent, n);
+   std::vector<std::thread> pool;
+   for (int w = 0; w < nthreads; w++) pool.emplace_back(worker);
+   for (std::thread& t : pool) t.join();
+   if (first_rc != OK) return set_error(first_rc, first_msg);
+   return OK;
 }
 
 }  // namespace sw4amd

@@ -27,6 +27,14 @@ public:
    // period (the buffer rotation) where it fits.
    int step(int n, bool graph);
    hipStream_t stream() const { return m_s; }
+   // Gradients of every shot's receiver misfit (EW::gradient of each shot,
+   // gtr[q], grho[q], gmu[q], gla[q] as there) with up to max_concurrent
+   // shots in flight at once, each on a stream of its own driven by a host
+   // thread of its own, so that the kernels of small grids share the device;
+   // a shot starts when its work memory (EW::gradient_plan) fits beside the
+   // shots running.  Each shot's work memory is released when it is done.
+   int gradient(int nsteps, int ckpt, const double* const* gtr, double* const* grho, double* const* gmu,
+                double* const* gla, bool on_device, int max_concurrent);
 
 private:
    int refresh();  // base pointers and clocks from the shots, tables

@@ -97,8 +97,8 @@ int EW::damp_tile_list(int k1, int k2, const DampTile** tiles, int* n)
    *tiles = nullptr;
    *n = 0;
    if (m_variant < 9 || k2 < k1) return OK;
-   if (!m_ddamp || m_damp_k1 != k1 || m_damp_k2 != k2 || m_damp_kc != m_kchunk) {
-      const std::vector<DampTile> v = damp_tiles(m_box, k1, k2, m_kchunk, m_sg_host.dcx.data(),
+   if (!m_ddamp || m_damp_k1 != k1 || m_damp_k2 != k2 || m_damp_kc != m_kchunk_damp) {
+      const std::vector<DampTile> v = damp_tiles(m_box, k1, k2, m_kchunk_damp, m_sg_host.dcx.data(),
                                                  m_sg_host.dcy.data(), m_sg_host.dcz.data());
       // at least one element, so that an empty list is a valid pointer
       const size_t bytes = sizeof(DampTile) * std::max<size_t>(v.size(), 1);
@@ -117,7 +117,7 @@ int EW::damp_tile_list(int k1, int k2, const DampTile** tiles, int* n)
       m_ndamp = (int)v.size();
       m_damp_k1 = k1;
       m_damp_k2 = k2;
-      m_damp_kc = m_kchunk;
+      m_damp_kc = m_kchunk_damp;
    }
    *tiles = m_ddamp;
    *n = m_ndamp;

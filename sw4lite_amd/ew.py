@@ -286,7 +286,7 @@ class EW:
             L.check(self.lib.sw4hip_ew_set_stream(self.h, C.c_void_p(stream)))
 
     # set-up -------------------------------------------------------------
-    def set_kernel_variant(self, variant, kchunk=64):
+    def set_kernel_variant(self, variant, kchunk=0):
         """0: one thread per point; 1-7: LDS-staged k-marching tiles (1 plain
         at 2 waves/SIMD, 2 with register prefetch, 3 prefetch without a
         register cap, 4 as 3 with the corrector's damping in the same sweep, 5
@@ -296,7 +296,9 @@ class EW:
         the predictor and loaded asynchronously by the corrector (one more
         field of device memory; even row lengths, otherwise as 7), 9 as 8
         with the damping sweep only over the tiles of the supergrid layers,
-        where the damping profiles are nonzero (the default; same bits))."""
+        where the damping profiles are nonzero (the default; same bits)).
+        kchunk > 0: planes per workgroup of every sweep; 0: the library's
+        defaults (128 for the predictor and corrector, 32 for the damping)."""
         L.check(self.lib.sw4hip_ew_set_kernel_variant(self.h, int(variant), int(kchunk)))
 
     def npoints(self):
@@ -485,6 +487,35 @@ class Batch:
 
     def step(self, nsteps, graph=True):
         L.check(self.lib.sw4hip_batch_step(self.h, int(nsteps), int(bool(graph))))
+
+    def gradient(self, grad_traces, nsteps, ckpt=0, out=None, max_concurrent=4):
+        """Gradients of every shot's receiver misfit (EW.gradient per shot,
+        run concurrently on streams of their own).  grad_traces: one array
+        [nsteps+1, nreceivers, 3] per shot.  out: per shot three float64
+        device tensors of the shot's point count (else numpy arrays are
+        returned).  Returns a list of (grho, gmu, gla)."""
+        n = len(self.shots)
+        if len(grad_traces) != n:
+            raise ValueError("one trace array per shot")
+        gs = [np.ascontiguousarray(np.asarray(g, dtype=np.float64)) for g in grad_traces]
+        for g in gs:
+            if g.ndim != 3 or g.shape[0] < nsteps + 1 or g.shape[2] != 3:
+                raise ValueError("grad_traces must be [nsteps+1, nreceivers, 3]")
+        if out is None:
+            res = [(np.empty(s.npoints()), np.empty(s.npoints()), np.empty(s.npoints())) for s in self.shots]
+            on_dev = 0
+        else:
+            res = [tuple(o) for o in out]
+            on_dev = int(all(_fields(s.npoints(), *o) for s, o in zip(self.shots, res)))
+            if not on_dev:
+                raise ValueError("out must hold device tensors")
+        arr = lambda xs: (C.c_void_p * n)(*[_ptr(x).value for x in xs])  # noqa: E731
+        L.check(self.lib.sw4hip_batch_gradient(self.h, int(nsteps), int(ckpt), C.cast(arr(gs), C.c_void_p),
+                                               C.cast(arr([r[0] for r in res]), C.c_void_p),
+                                               C.cast(arr([r[1] for r in res]), C.c_void_p),
+                                               C.cast(arr([r[2] for r in res]), C.c_void_p), on_dev,
+                                               int(max_concurrent)))
+        return res
 
     def close(self):
         if self.h:

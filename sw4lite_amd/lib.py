@@ -66,6 +66,7 @@ _SIGS = {
     "sw4hip_batch_set_kchunk": (I, [P, I]),
     "sw4hip_batch_tune": (I, [P, I, P, IP, P]),
     "sw4hip_batch_step": (I, [P, I, I]),
+    "sw4hip_batch_gradient": (I, [P, I, I, P, P, P, P, I, I]),
     "sw4hip_ew_gradient": (I, [P, I, I, P, P, P, P, I]),
     "sw4hip_ew_gradient_plan": (I, [P, I, I, P, P]),
     "sw4hip_ew_release_workspace": (I, [P]),

@@ -312,3 +312,49 @@ def test_autograd_function(torch):
     print("autograd misfit", float(J), "after a descent step", float(J1), flush=True)
     assert float(J1) < float(J)
     e.close()
+
+
+@pytest.mark.parametrize("max_concurrent", [1, 3])
+def test_batch_gradient_matches_single_shots(torch, max_concurrent):
+    """sw4hip_batch_gradient over three shots of uneven grid size, run one
+    after another or concurrently on streams of their own: every shot's
+    gradients must be its single-shot gradients to the last bit, and the
+    smallest shot's must match central finite differences of its misfit."""
+    from sw4lite_amd import Batch
+    nsteps = 14
+    grids = [(24, 22, 20), (20, 18, 16), (28, 20, 18)]
+    rng = np.random.default_rng(53)
+    shots, mats = [], []
+    for nx, ny, nz in grids:
+        e, mat = make_ew(nx, ny, nz, 0.1, [2, 2, 2, 2, 0, 2], nsteps)
+        shots.append(e)
+        mats.append(mat)
+    gtrs = [rng.standard_normal((nsteps + 1, len(RECS), 3)) for _ in shots]
+    single = [e.gradient(g, nsteps, ckpt=4) for e, g in zip(shots, gtrs)]
+    for e in shots:
+        e.release_workspace()
+    b = Batch(shots)
+    got = b.gradient(gtrs, nsteps, ckpt=4, max_concurrent=max_concurrent)
+    for q, (a, w) in enumerate(zip(got, single)):
+        ndiff = [int(np.count_nonzero(x != y)) for x, y in zip(a, w)]
+        print("batch gradient, max_concurrent", max_concurrent, "shot", q, grids[q],
+              "points differing from the single shot:", ndiff, flush=True)
+        assert np.max(np.abs(w[0])) > 0 and ndiff == [0, 0, 0]
+    # finite differences on the smallest shot, misfit against a perturbed record
+    e, mat = shots[1], mats[1]
+    J0, tr = misfit(e, mat, nsteps, 0.0)
+    dobs = 0.7 * tr
+    J0, tr = misfit(e, mat, nsteps, dobs)
+    gb = b.gradient([np.zeros((nsteps + 1, len(RECS), 3)), tr - dobs, np.zeros((nsteps + 1, len(RECS), 3))],
+                    nsteps, ckpt=5, max_concurrent=max_concurrent)[1]
+    d = [rng.uniform(-1, 1, m.size) * m for m in mat]
+    eps = 1e-6
+    jp, _ = misfit(e, [m + eps * dd for m, dd in zip(mat, d)], nsteps, dobs)
+    jm, _ = misfit(e, [m - eps * dd for m, dd in zip(mat, d)], nsteps, dobs)
+    fd = (jp - jm) / (2 * eps)
+    ad = sum(float(np.dot(gq, dq)) for gq, dq in zip(gb, d))
+    print("batch gradient finite differences: fd", fd, "adjoint", ad, "rel", dot_rel(fd, ad), flush=True)
+    assert dot_rel(fd, ad) < FD_TOL
+    b.close()
+    for e in shots:
+        e.close()
