@@ -76,7 +76,7 @@ def parse():
                         "8 as 7 with the acceleration stored by the predictor and staged by the corrector, "
                         "9 as 8 with the damping sweep only over the supergrid layers")
     p.add_argument("--kchunk", type=int, default=0,
-                   help="planes per workgroup of the LDS sweeps (0: the library's defaults, 128 for the predictor "
+                   help="planes per workgroup of the LDS sweeps (0: the library's defaults, 64 for the predictor "
                         "and corrector sweeps, 32 for the damping sweep)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also launch a plain device copy of 1 GiB (PMC scaling)")

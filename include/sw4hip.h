@@ -272,7 +272,7 @@ int sw4hip_ew_set_stream(sw4hip_ew* ew, void* stream);
  * 7; the same bits as 7), 9 (the default) as 8 with the damping sweep only
  * over the tiles and planes of the supergrid layers (the same bits as 8).
  * All give the same results to rounding.  kchunk > 0 sets the planes per
- * workgroup of every sweep; kchunk <= 0 keeps the defaults (128 for the
+ * workgroup of every sweep; kchunk <= 0 keeps the defaults (64 for the
  * predictor and corrector sweeps, 32 for the damping sweep). */
 int sw4hip_ew_set_kernel_variant(sw4hip_ew* ew, int variant, int kchunk);
 /* rho, mu, lambda over the whole box incl. ghosts; on_device selects whether

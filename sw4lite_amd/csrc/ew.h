@@ -154,9 +154,9 @@ public:
    hipStream_t m_stream = nullptr;
    int m_variant = 9;   // interior sweep kernels: 0 thread per point, 1-9 LDS-staged k-march
    // planes per workgroup of the LDS sweeps: predictor and corrector, and
-   // the damping sweep over the supergrid layers (fastest measured at 512^3:
-   // 128 and 32 of 32..256)
-   static constexpr int kSweepChunk = 128, kDampChunk = 32;
+   // the damping sweep over the supergrid layers (fastest measured at 512^3
+   // on one device: 64 and 32 of 32..256)
+   static constexpr int kSweepChunk = 64, kDampChunk = 32;
    int m_kchunk = kSweepChunk;
    int m_kchunk_damp = kDampChunk;
 

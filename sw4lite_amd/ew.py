@@ -298,7 +298,7 @@ class EW:
         with the damping sweep only over the tiles of the supergrid layers,
         where the damping profiles are nonzero (the default; same bits)).
         kchunk > 0: planes per workgroup of every sweep; 0: the library's
-        defaults (128 for the predictor and corrector, 32 for the damping)."""
+        defaults (64 for the predictor and corrector, 32 for the damping)."""
         L.check(self.lib.sw4hip_ew_set_kernel_variant(self.h, int(variant), int(kchunk)))
 
     def npoints(self):
