@@ -121,8 +121,13 @@ int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double
    if ((rc = launch_bcfortsg_T(m_box, m_wind, g.nx, g.ny, g.nz, g.bc, Y, u1, g.h, m_dsbp, mMu.dev, mLambda.dev, c.strx,
                                c.stry, gmu, gla, plane, m_stream)))
       return rc;
-   // damping: U^{n+1} -= D(U - Um); corrector: Up + c4/rho (L(A) + F_tt)
-   if ((rc = launch_addsgd4_T_march(m_box, Y, u, um, mRho.dev, c, m_opt.beta, Db, grho, m_stream, false)) ||
+   // damping: U^{n+1} -= D(U - Um); corrector: Up + c4/rho (L(A) + F_tt).
+   // Kernel variant 9: the damping transpose runs over the supergrid layers
+   // only; Db is zero elsewhere from the start of the reverse sweep.
+   const DampTile* dT = nullptr;
+   int ndT = 0;
+   if ((rc = damp_T_tile_list(&dT, &ndT))) return rc;
+   if ((rc = launch_addsgd4_T_march(m_box, Y, u, um, mRho.dev, c, m_opt.beta, Db, grho, m_stream, false, dT, ndT)) ||
        (rc = launch_vjp_corr_pw(m_box, rm, c.strx, c.stry, c.strz, c4, mRho.dev, Y, u1, Up, Z, grho, pad, m_stream)) ||
        (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, nullptr, pad,
                              m_stream, true)) ||
@@ -372,6 +377,8 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
    // the reverse step writes W on the box points only: its pad stays zero
    if (rc == OK) rc = launch_zero(pad.w, 3 * pad.pb.npts, m_stream);
    for (int q = 0; q < 3 && rc == OK; q++) rc = launch_zero(gr[q], n1, m_stream);
+   // the damping transpose's output (tmp[2]) where its layer tiles do not reach
+   if (rc == OK && m_variant >= 9) rc = launch_zero(tmp[2], n3, m_stream);
    for (int q = 0; q < 2 && rc == OK; q++) rc = launch_zero(lam[q], n3, m_stream);
    double *la = lam[0], *lb = lam[1], *lc = lam[2];
    if (rc == OK) rc = add_direct(nsteps, la);

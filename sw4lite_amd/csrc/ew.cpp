@@ -124,6 +124,33 @@ int EW::damp_tile_list(int k1, int k2, const DampTile** tiles, int* n)
    return OK;
 }
 
+int EW::damp_T_tile_list(const DampTile** tiles, int* n)
+{
+   *tiles = nullptr;
+   *n = 0;
+   if (m_variant < 9) return OK;
+   if (!m_ddampT) {
+      const std::vector<DampTile> v =
+          damp_T_tiles(m_box, m_sg_host.dcx.data(), m_sg_host.dcy.data(), m_sg_host.dcz.data());
+      void* q = nullptr;
+      if (hipMalloc(&q, sizeof(DampTile) * std::max<size_t>(v.size(), 1)) != hipSuccess) {
+         (void)hipGetLastError();
+         return set_error(E_NOMEM, "EW: damping transpose tile list allocation failed");
+      }
+      m_allocs.push_back(q);
+      if (!v.empty()) {
+         const int rc = check_hip(hipMemcpy(q, v.data(), sizeof(DampTile) * v.size(), hipMemcpyHostToDevice),
+                                  "damping transpose tile list upload");
+         if (rc) return rc;
+      }
+      m_ddampT = static_cast<DampTile*>(q);
+      m_ndampT = (int)v.size();
+   }
+   *tiles = m_ddampT;
+   *n = m_ndampT;
+   return OK;
+}
+
 int EW::set_material(const double* rho, const double* mu, const double* la, bool on_device)
 {
    const size_t bytes = sizeof(double) * m_box.npts;

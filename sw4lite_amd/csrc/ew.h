@@ -140,6 +140,11 @@ public:
    int damp_tile_list(int k1, int k2, const DampTile** tiles, int* n);
    DampTile* m_ddamp = nullptr;
    int m_ndamp = 0, m_damp_k1 = 0, m_damp_k2 = -1, m_damp_kc = 0;
+   // kernel variant 9: the tiles of the damping transpose of the adjoint
+   // (damp_T_tiles), made and uploaded on first use
+   int damp_T_tile_list(const DampTile** tiles, int* n);
+   DampTile* m_ddampT = nullptr;
+   int m_ndampT = 0;
    double mDt = 0, mT = 0;
    int mNumberOfTimeSteps = 0, mCycle = 0;
    int m_onesided[6] = {0, 0, 0, 0, 0, 0};

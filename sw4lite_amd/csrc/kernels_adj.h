@@ -9,7 +9,10 @@
 //   G_mu(z, u)     d/dmu [z . L(u; mu, la)] on every point (L is linear in
 //   G_la(z, u)     mu and la, so these are exact bilinear forms).
 #pragma once
+#include <vector>
+
 #include "kernels_common.h"
+#include "kernels_fast.h"
 
 namespace sw4amd {
 
@@ -60,9 +63,15 @@ int launch_rhs4sg_matgrad(const Box& b, const RowMap& m, const SbpTables* d_sbp,
 // inv_rho = false leaves out the derivative through the 1/rho factor at the
 // damped point (y . D / rho), for callers that form it from state differences.
 // LDS k-march over column tiles.
+// tiles (damp_T_tiles of the same box and profiles): only those tiles run,
+// dbar keeps its values elsewhere (where the transposed damping vanishes).
 int launch_addsgd4_T_march(const Box& b, const double* y, const double* u, const double* um, const double* rho,
                            const SgCoef& sg, double beta, double* dbar, double* grho, hipStream_t s,
-                           bool inv_rho = true);
+                           bool inv_rho = true, const DampTile* tiles = nullptr, int ntiles = 0);
+// The column tiles and planes of the damping transpose whose outputs can be
+// nonzero (the damping profiles at a point or a neighbour nonzero), from the
+// host profiles (box-sized, from the box start).
+std::vector<DampTile> damp_T_tiles(const Box& b, const double* dcx, const double* dcy, const double* dcz);
 // Transpose of bcfortsg with zero boundary data on the adjoint y of its
 // output (sides in reverse order): Dirichlet/supergrid ghosts are dropped, a
 // free-surface ghost plane is pushed into the points its closure reads.  The

@@ -697,10 +697,13 @@ struct DampCtx {
 // dbar = -sum_ax w_ax D2(r D2(s_ax g)) and grho += -sum_ax dc_ax D2(d) w_ax
 // D2(s_ax g) at the point (r = rho dc_ax; D2 the centred second difference;
 // w_x = coy coz, w_y = cox coz, w_z = cox coy).
+// tiles: when given, block b runs tiles[b] (column tile and planes) instead
+// of the full tiling.
 __global__ void __launch_bounds__(GNT, 2) k_addsgd4_T_march(DampCtx D, int ntx, int nty, int kc,
                                                             const double* __restrict__ y, const double* __restrict__ u,
                                                             const double* __restrict__ um, double* __restrict__ dbar,
-                                                            double* __restrict__ grho)
+                                                            double* __restrict__ grho,
+                                                            const DampTile* __restrict__ tiles)
 {
    __shared__ double sgl[2][3 * GPL];
    __shared__ double sdl[2][3 * GPL];
@@ -709,13 +712,25 @@ __global__ void __launch_bounds__(GNT, 2) k_addsgd4_T_march(DampCtx D, int ntx, 
    const SgCoef& sg = D.sg;
    const double* rho = D.rho;
    const double beta = D.beta;
-   const int t = blockIdx.x;
-   const int tix = t % ntx, tr = t / ntx, tjy = tr % nty, tkz = tr / nty;
+   int tix, tjy, k0, k1;
+   if (tiles) {
+      const DampTile d = tiles[blockIdx.x];
+      tix = d.tix;
+      tjy = d.tjy;
+      k0 = d.k0;
+      k1 = d.k1;
+   } else {
+      const int t = blockIdx.x;
+      tix = t % ntx;
+      const int tr = t / ntx;
+      tjy = tr % nty;
+      k0 = b.kb + (tr / nty) * kc;
+      k1 = min(b.ke, k0 + kc - 1);
+   }
    const int tid = threadIdx.x, tx = tid % GTX, ty = tid / GTX;
    const int ia = b.ib + tix * GTX, ja = b.jb + tjy * GTY;
    const int i = ia + tx, j = ja + ty;
    const bool active = i <= b.ie && j <= b.je;
-   const int k0 = b.kb + tkz * kc, k1 = min(b.ke, k0 + kc - 1);
    const long n = b.npts;
    // staging slots: in-plane offset (-1 off the box), interior column
    int soff[GSL];
@@ -1176,13 +1191,52 @@ int launch_rhs4sg_matgrad(const Box& b, const RowMap& m, const SbpTables* d_sbp,
    return check_hip(hipGetLastError(), "rhs4sg material-derivative launch");
 }
 
+std::vector<DampTile> damp_T_tiles(const Box& b, const double* dcx, const double* dcy, const double* dcz)
+{
+   // the kernel reads the profiles at the point and its two neighbours,
+   // clamped to the box; its outputs at a point vanish where all three are 0
+   auto damped = [](const double* dc, int n) {
+      std::vector<char> on(n, 0);
+      for (int q = 0; q < n; q++)
+         on[q] = dc[std::max(q - 1, 0)] != 0.0 || dc[q] != 0.0 || dc[std::min(q + 1, n - 1)] != 0.0;
+      return on;
+   };
+   const int ni = (int)b.ni, nj = b.je - b.jb + 1, nk = b.ke - b.kb + 1;
+   const std::vector<char> ox = damped(dcx, ni), oy = damped(dcy, nj), oz = damped(dcz, nk);
+   const int ntx = (int)cdiv(ni, GTX), nty = (int)cdiv(nj, GTY);
+   const int kc = march_kchunk((long)ntx * nty, nk, 2);
+   std::vector<DampTile> out;
+   for (int kb = b.kb; kb <= b.ke; kb += kc) {
+      const int ke = std::min(b.ke, kb + kc - 1);
+      for (int tjy = 0; tjy < nty; tjy++)
+         for (int tix = 0; tix < ntx; tix++) {
+            bool side = false;
+            for (int q = tix * GTX; q < std::min((tix + 1) * GTX, ni); q++) side = side || ox[q];
+            for (int q = tjy * GTY; q < std::min((tjy + 1) * GTY, nj); q++) side = side || oy[q];
+            int lo = ke + 1, hi = kb - 1;
+            for (int k = kb; k <= ke; k++)
+               if (side || oz[k - b.kb]) {
+                  lo = std::min(lo, k);
+                  hi = std::max(hi, k);
+               }
+            if (lo <= hi) out.push_back(DampTile{tix, tjy, lo, hi});
+         }
+   }
+   return out;
+}
+
 int launch_addsgd4_T_march(const Box& b, const double* y, const double* u, const double* um, const double* rho,
-                           const SgCoef& sg, double beta, double* dbar, double* grho, hipStream_t s, bool inv_rho)
+                           const SgCoef& sg, double beta, double* dbar, double* grho, hipStream_t s, bool inv_rho,
+                           const DampTile* tiles, int ntiles)
 {
    const DampCtx D{b, b.ie - b.ib - 3, b.je - b.jb - 3, b.ke - b.kb - 3, sg, rho, beta};
    const int ntx = (int)cdiv(b.ni, GTX), nty = (int)cdiv(b.je - b.jb + 1, GTY);
    const int kc = march_kchunk((long)ntx * nty, b.ke - b.kb + 1, 2), nkc = (int)cdiv(b.ke - b.kb + 1, kc);
-   k_addsgd4_T_march<<<ntx * nty * nkc, GNT, 0, s>>>(D, ntx, nty, kc, y, u, um, dbar, grho);
+   if (tiles) {
+      if (ntiles > 0) k_addsgd4_T_march<<<ntiles, GNT, 0, s>>>(D, ntx, nty, kc, y, u, um, dbar, grho, tiles);
+   } else {
+      k_addsgd4_T_march<<<ntx * nty * nkc, GNT, 0, s>>>(D, ntx, nty, kc, y, u, um, dbar, grho, nullptr);
+   }
    if (inv_rho) {
       const dim3 blk(64, 4, 1);
       k_damp_invrho<<<dim3(cdiv(D.nx, 64), cdiv(D.ny, 4), D.nz), blk, 0, s>>>(D, y, u, um, grho);

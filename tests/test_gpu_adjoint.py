@@ -358,3 +358,33 @@ def test_batch_gradient_matches_single_shots(torch, max_concurrent):
     b.close()
     for e in shots:
         e.close()
+
+
+@pytest.mark.parametrize("bc", [[2, 2, 2, 2, 0, 2], [1, 2, 2, 1, 2, 0]])
+def test_gradient_damping_layers_bitwise(torch, bc):
+    """Kernel variant 9 runs the damping transpose of every reverse step only
+    over the tiles and planes of the supergrid layers (its outputs vanish
+    elsewhere): the gradients must be variant 8's to the last bit on a grid
+    of several tiles per axis."""
+    from sw4lite_amd import EW
+    nx, ny, nz, h, nsteps = 96, 40, 41, 0.05, 10
+    rho, mu, la = layered_material(nx, ny, nz, h, 9)
+    rng = np.random.default_rng(59)
+    recs = [(1.0, 0.9, 0.0, 0), (3.2, 1.4, 0.6, 1), (4.5, 0.3, 0.2, 0)]
+    gtr = rng.standard_normal((nsteps + 1, len(recs), 3))
+    res = []
+    for variant in (8, 9):
+        e = EW(nx, ny, nz, h, bc, corder=1, sg_gp_thickness=6)
+        e.set_kernel_variant(variant, 7)
+        e.set_material(rho, mu, la)
+        for s in SOURCES[:2]:
+            e.add_moment_source(s["x"] * 0.5, s["y"] * 0.5, s["z"] * 0.5, s["m"], s["t0"], s["freq"], s["tfkind"])
+        for x, y, z, m in recs:
+            e.add_receiver(x, y, z, m)
+        e.setup(nsteps=nsteps)
+        res.append(e.gradient(gtr, nsteps, ckpt=4))
+        e.close()
+    ndiff = [int(np.count_nonzero(a != b)) for a, b in zip(*res)]
+    print("gradient, damping transpose over the layers: bc", bc, "points differing from variant 8:", ndiff,
+          flush=True)
+    assert np.max(np.abs(res[0][0])) > 0 and ndiff == [0, 0, 0]
