@@ -84,6 +84,42 @@ int sw4hip_host_moment_source(const sw4hip_grid_desc* g, double x, double y, dou
 int sw4hip_host_moment_source_curv(const sw4hip_grid_desc* g, const double* zgrid, double x,
                                    double y, double z, const double m[6], int maxn, int* n,
                                    int* ijk, double* force);
+/* Topography input=gaussian (EW::processTopography, EW.C:535-725): a
+ * Gaussian hill of elevation amp exp(-((x-xc)/lx)^2 - ((y-yc)/ly)^2) over a
+ * curvilinear grid from the surface down to z = zmax, stacked on the
+ * Cartesian grid that continues to the domain depth zdomain. */
+typedef struct {
+   double zmax;       /* zmax= */
+   int order;         /* order= (2..7; reference default 4) */
+   double zeta_break; /* zetabreak= (default 0.95) */
+   double amp, xc, yc, lx, ly; /* gaussianAmp, gaussianXc, gaussianYc, gaussianLx, gaussianLy */
+} sw4hip_topography;
+
+/* Replaces EW::defineDimensionsZ + EW::generate_grid (EW.C:2011-2060,
+ * 6164-6205): the z sizes of both grids and, when zgrid is not NULL, z of
+ * the curvilinear grid over its box [-1,nx+2] x [-1,ny+2] x [-1,nz_curv+2]
+ * (i fastest). */
+int sw4hip_host_topography_grid(const sw4hip_topography* t, int nx, int ny, double h, double zdomain, int* nz_cart,
+                                int* nz_curv, double* zgrid);
+
+/* Replaces the receiver placement of TimeSeries::TimeSeries with
+ * EW::computeNearestGridPoint and EW::invert_grid_mapping (TimeSeries.C:
+ * 112-183, 307-330; EW.C:5149-5226, 5722-5790): grid 0 Cartesian or 1
+ * curvilinear and the grid point (i, j, k); grid -1 where the reference
+ * ignores the station (above the surface).  topodepth: z is the depth
+ * below the surface. */
+int sw4hip_host_topography_receiver(const sw4hip_topography* t, int nx, int ny, double h, double zdomain, double x,
+                                    double y, double z, int topodepth, int* grid, int ijk[3]);
+
+/* Replaces Source::adjust_zcoord + Source::set_grid_point_sources4 for a
+ * moment tensor in the stacked grids (Source.C:517-553, 1074-1960,
+ * 2191-2330): gijk[4n] = (grid, i, j, k), force[3n].  jac: the curvilinear
+ * grid's Jacobian over its box (sw4hip_metric).  *n = 0 where the
+ * reference ignores the source (above the surface). */
+int sw4hip_host_topography_source(const sw4hip_topography* t, int nx, int ny, double h, double zdomain,
+                                  const double* jac, double x, double y, double z, int topodepth, const double m[6],
+                                  int maxn, int* n, int* gijk, double* force);
+
 /* ----------------------------------------------------------------------- */
 /* Kernel-level entry points (device arrays)                                */
 /* ----------------------------------------------------------------------- */
@@ -404,20 +440,24 @@ int sw4hip_ew_sync(sw4hip_ew* ew);
 /* ---- Input decks, receiver files, checkpoints ---- */
 
 /* Replaces EW::EW(deck) + EW::timesteploop (EW.C:2171-2336, 2339-2928;
- * main.C) for decks of one Cartesian grid: the commands grid, time, fileio,
- * supergrid, developer, testpointsource, source (moment tensors; Ricker,
- * Gaussian, C6SmoothBump), block, rec, checkpoint and restart.  Anything
- * else (topography, point forces, other time functions) is refused with
- * SW4HIP_EINVAL and a message naming the line.  Runs on the current GPU:
- * mode 0 fused sweeps, 1 the reference stage order, 2 FP32 with the
- * compensated update.  Writes the receiver files, checkpoints and (point
- * source test) PointSourceErr.txt into the deck's fileio path, or into
- * `path` when it is not NULL. */
+ * main.C): the commands grid, time, fileio, supergrid, developer,
+ * testpointsource, source (moment tensors; Ricker, Gaussian, C6SmoothBump),
+ * block, rec, checkpoint, restart and topography (input=gaussian: a
+ * curvilinear grid under the hill stacked on the Cartesian grid below zmax,
+ * coupled by EW::enforceCartTopo; no checkpoints or point-source test with
+ * it).  Anything else (point forces, other time functions, other topography
+ * inputs) is refused with SW4HIP_EINVAL and a message naming the line.  Runs
+ * on the current GPU: mode 0 fused sweeps, 1 the reference stage order, 2
+ * FP32 with the compensated update; decks with topography step in the
+ * reference stage order in modes 0 and 1.  Writes the receiver files,
+ * checkpoints and (point source test) PointSourceErr.txt into the deck's
+ * fileio path, or into `path` when it is not NULL. */
 typedef struct {
-   int nx, ny, nz;
+   int nx, ny, nz;          /* nz: the Cartesian grid's */
    double h, dt, t;
    int nsteps, cycle, nstations, ncheckpoints, point_source_test;
    double linf, l2, solinf; /* point source test: errors at the end */
+   int ngrids, nz_curv;     /* with topography: 2, and the curvilinear grid's nz */
 } sw4hip_deck_report;
 int sw4hip_run_deck(const char* deck_file, const char* path, int mode, sw4hip_deck_report* report);
 /* Parse deck text only (no GPU): grid sizes and step settings in report

@@ -31,6 +31,8 @@ SOURCES = [
     "noalloc.hip",
     "timeseries.cpp",
     "deck.cpp",
+    "topo.cpp",
+    "stack.cpp",
 ]
 
 FLAGS = [

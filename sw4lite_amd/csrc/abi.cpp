@@ -11,6 +11,7 @@
 #include "../../include/sw4hip.h"
 #include "batch.h"
 #include "deck.h"
+#include "topo.h"
 #include "ew.h"
 #include "kernels_adj.h"
 #include "kernels_curv.h"
@@ -291,6 +292,87 @@ int sw4hip_host_moment_source_curv(const sw4hip_grid_desc* d, const double* zgri
       ijk[3 * k] = pts[k].i; ijk[3 * k + 1] = pts[k].j; ijk[3 * k + 2] = pts[k].k;
       for (int c = 0; c < 3; c++) force[3 * k + c] = pts[k].f[c];
    }
+   return OK;
+}
+
+namespace {
+int to_topo(const sw4hip_topography* t, int nx, int ny, double h, double zdomain, Topography& tp, StackedGrids& sg)
+{
+   if (!t) return set_error(E_INVAL, "null topography");
+   tp.zmax = t->zmax;
+   tp.order = t->order;
+   tp.zeta_break = t->zeta_break;
+   tp.amp = t->amp;
+   tp.xc = t->xc;
+   tp.yc = t->yc;
+   tp.lx = t->lx;
+   tp.ly = t->ly;
+   return stacked_grid_sizes(tp, nx, ny, h, zdomain, sg);
+}
+}  // namespace
+
+int sw4hip_host_topography_grid(const sw4hip_topography* t, int nx, int ny, double h, double zdomain, int* nz_cart,
+                                int* nz_curv, double* zgrid)
+{
+   Topography tp;
+   StackedGrids sg;
+   int rc;
+   if ((rc = to_topo(t, nx, ny, h, zdomain, tp, sg))) return rc;
+   if (nz_cart) *nz_cart = sg.nz_cart;
+   if (nz_curv) *nz_curv = sg.nz_curv;
+   if (zgrid) {
+      std::vector<double> z;
+      curvilinear_grid_z(tp, sg, z);
+      std::memcpy(zgrid, z.data(), sizeof(double) * z.size());
+   }
+   return OK;
+}
+
+int sw4hip_host_topography_receiver(const sw4hip_topography* t, int nx, int ny, double h, double zdomain, double x,
+                                    double y, double z, int topodepth, int* grid, int ijk[3])
+{
+   Topography tp;
+   StackedGrids sg;
+   int rc;
+   if ((rc = to_topo(t, nx, ny, h, zdomain, tp, sg))) return rc;
+   if (!grid || !ijk) return set_error(E_INVAL, "topography_receiver: null argument");
+   rc = stacked_receiver_point(tp, sg, x, y, z, topodepth != 0, *grid, ijk[0], ijk[1], ijk[2]);
+   if (rc == 1) *grid = -1;
+   return rc == 1 ? OK : rc;
+}
+
+int sw4hip_host_topography_source(const sw4hip_topography* t, int nx, int ny, double h, double zdomain,
+                                  const double* jac, double x, double y, double z, int topodepth, const double m[6],
+                                  int maxn, int* n, int* gijk, double* force)
+{
+   Topography tp;
+   StackedGrids sg;
+   int rc;
+   if ((rc = to_topo(t, nx, ny, h, zdomain, tp, sg))) return rc;
+   if (!jac || !m || !n || !gijk || !force) return set_error(E_INVAL, "topography_source: null argument");
+   MomentSource s{x, y, z, {m[0], m[1], m[2], m[3], m[4], m[5]}, 0.0, 1.0, TF_GAUSSIAN};
+   if (topodepth) {
+      double zt = 0;
+      if (!topo_surface_z(tp, sg, x, y, zt)) return set_error(E_INVAL, "topography_source: off the surface");
+      s.z += zt;
+   }
+   std::vector<GridPointSource> pc, pv;
+   rc = stacked_moment_source(tp, sg, jac, s, pc, pv);
+   *n = 0;
+   if (rc == 1) return OK;
+   if (rc) return rc;
+   if ((int)(pc.size() + pv.size()) > maxn) return set_error(E_RANGE, "more grid-point sources than maxn");
+   int q = 0;
+   for (int g = 0; g < 2; g++)
+      for (const GridPointSource& p : g == 0 ? pc : pv) {
+         gijk[4 * q] = g;
+         gijk[4 * q + 1] = p.i;
+         gijk[4 * q + 2] = p.j;
+         gijk[4 * q + 3] = p.k;
+         for (int c = 0; c < 3; c++) force[3 * q + c] = p.f[c];
+         q++;
+      }
+   *n = q;
    return OK;
 }
 
@@ -1005,6 +1087,8 @@ void fill_report(const sw4amd::DeckResult& r, sw4hip_deck_report* rep)
    rep->linf = r.linf;
    rep->l2 = r.l2;
    rep->solinf = r.solinf;
+   rep->ngrids = r.ngrids;
+   rep->nz_curv = r.nz_curv;
 }
 }  // namespace
 

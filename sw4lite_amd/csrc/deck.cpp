@@ -1,5 +1,7 @@
-// deck.cpp -- the reference's input-deck commands for one Cartesian grid
-// (EW::parseInputFile, EW.C:2171-2285), run through the EW object.
+// deck.cpp -- the reference's input-deck commands (EW::parseInputFile,
+// EW.C:2171-2285) for one Cartesian grid, or for a curvilinear grid under a
+// Gaussian-hill topography stacked on the Cartesian one, run through EW
+// objects.
 #include "deck.h"
 
 #include <fcntl.h>
@@ -17,6 +19,7 @@
 #include <sstream>
 
 #include "ew.h"
+#include "stack.h"
 
 namespace sw4amd {
 
@@ -149,7 +152,12 @@ int parse_source(const std::vector<std::string>& tk, int ln, Deck& d)
       else return bad(ln, "source: option not supported: " + t);
    }
    if (!zset && !depthset) return bad(ln, "source: depth or z must be given");
-   if (depthset) s.z = depth;  // no topography: depth is z
+   if (depthset) {
+      // depth below the surface (EW.C:1026-1034); without topography it is z
+      if (depth < 0) return bad(ln, "source: depth below the surface must be >= 0");
+      s.z = depth;
+      s.topodepth = true;
+   }
    if (s.type == "Ricker") s.tfkind = 0;
    else if (s.type == "Gaussian") s.tfkind = 1;
    else if (s.type == "C6SmoothBump") s.tfkind = 2;
@@ -163,13 +171,18 @@ int parse_receiver(const std::vector<std::string>& tk, int ln, Deck& d)
 {
    Station s;
    s.file = "station";
-   bool named = false, zset = false;
+   bool named = false, zset = false, topodepth = false;
    for (size_t q = 1; q < tk.size(); q++) {
       const std::string& t = tk[q];
       if (starts(t, "x=")) s.x = fval(t, 2);
       else if (starts(t, "y=")) s.y = fval(t, 2);
-      else if (starts(t, "z=")) { s.z = fval(t, 2); zset = true; }
-      else if (starts(t, "depth=")) { s.z = fval(t, 6); zset = true; }
+      else if (starts(t, "z=")) { s.z = fval(t, 2); zset = true; topodepth = false; }
+      else if (starts(t, "depth=") || starts(t, "topodepth=")) {
+         s.z = fval(t, starts(t, "depth=") ? 6 : 10);
+         if (s.z < 0) return bad(ln, "rec: depth must be >= 0");
+         zset = true;
+         topodepth = true;
+      }
       else if (starts(t, "file=")) s.file = sval(t, 5);
       else if (starts(t, "sta=")) { s.sta = sval(t, 4); named = true; }
       else if (starts(t, "nsew=")) s.xyz = ival(t, 5) != 1;
@@ -186,6 +199,7 @@ int parse_receiver(const std::vector<std::string>& tk, int ln, Deck& d)
    (void)zset;
    if (!named) s.sta = s.file;
    d.stations.push_back(s);
+   d.station_topodepth.push_back(topodepth);
    return OK;
 }
 
@@ -207,11 +221,50 @@ int parse_block(const std::vector<std::string>& tk, int ln, Deck& d)
       else if (starts(t, "y2=")) b.y2 = fval(t, 3);
       else if (starts(t, "z1=")) b.z1 = fval(t, 3);
       else if (starts(t, "z2=")) b.z2 = fval(t, 3);
+      else if (starts(t, "absdepth=")) b.absdepth = ival(t, 9) != 0;
       else if (starts(t, "Qs=") || starts(t, "qs=") || starts(t, "Qp=") || starts(t, "qp=")) continue;  // no attenuation
       else return bad(ln, "block: option not supported: " + t);
    }
    if (b.x2 < b.x1 || b.y2 < b.y1 || b.z2 < b.z1) return bad(ln, "block: upper bound below lower bound");
    d.blocks.push_back(b);
+   return OK;
+}
+
+// topography (EW::processTopography, EW.C:535-725): the Gaussian hill
+int parse_topography(const std::vector<std::string>& tk, int ln, Deck& d)
+{
+   Topography t;
+   bool gaussian = false, zmax = false;
+   for (size_t q = 1; q < tk.size(); q++) {
+      const std::string& v = tk[q];
+      if (starts(v, "zmax=")) { t.zmax = fval(v, 5); zmax = true; }
+      else if (starts(v, "order=")) {
+         t.order = ival(v, 6);
+         if (t.order < 2 || t.order > 7) return bad(ln, "topography: order must be 2, 3, 4, 5, 6 or 7");
+      } else if (starts(v, "zetabreak=")) {
+         t.zeta_break = fval(v, 10);
+         if (!(t.zeta_break > 0 && t.zeta_break <= 1)) return bad(ln, "topography: zetabreak must be in (0, 1]");
+      } else if (starts(v, "input=")) {
+         if (sval(v, 6) != "gaussian") return bad(ln, "topography: input=" + sval(v, 6) + " is not supported (gaussian)");
+         gaussian = true;
+      } else if (starts(v, "file=")) continue;  // not read for the analytic hill
+      else if (starts(v, "gaussianAmp=")) t.amp = fval(v, 12);
+      else if (starts(v, "gaussianXc=")) t.xc = fval(v, 11);
+      else if (starts(v, "gaussianYc=")) t.yc = fval(v, 11);
+      else if (starts(v, "gaussianLx=")) t.lx = fval(v, 11);
+      else if (starts(v, "gaussianLy=")) t.ly = fval(v, 11);
+      else if (starts(v, "analyticalMetric=")) {
+         const std::string a = sval(v, 17);
+         if (a == "1" || a == "true" || a == "yes")
+            return bad(ln, "topography: analyticalMetric (metricexgh) is not supported; the metric is computed from "
+                           "the grid");
+      } else return bad(ln, "topography: option not supported: " + v);
+   }
+   if (!gaussian) return bad(ln, "topography: input=gaussian must be given");
+   if (!zmax) return bad(ln, "topography: zmax must be given");
+   if (!(t.lx > 0 && t.ly > 0)) return bad(ln, "topography: gaussianLx and gaussianLy must be positive");
+   d.topography = true;
+   d.topo = t;
    return OK;
 }
 
@@ -265,7 +318,10 @@ int parse_deck(const std::string& text, Deck& d)
       } else if (c == "developer") {
          for (size_t q = 1; q < tk.size(); q++) {
             if (starts(tk[q], "cfl=")) d.cfl = fval(tk[q], 4);
-            else if (starts(tk[q], "corder=")) d.corder = ival(tk[q], 7) != 0;
+            else if (starts(tk[q], "corder=")) {
+               const std::string v = sval(tk[q], 7);  // EW.C:1364-1369
+               d.corder = v == "yes" || v == "1" || v == "on";
+            }
             else if (starts(tk[q], "checkfornan=") || starts(tk[q], "reporttiming=") || starts(tk[q], "trace=") ||
                      starts(tk[q], "thblocki=") || starts(tk[q], "thblockj=") || starts(tk[q], "thblockk="))
                continue;  // diagnostics and CPU thread blocking of the reference
@@ -310,8 +366,7 @@ int parse_deck(const std::string& text, Deck& d)
          }
          if (d.restart.empty()) return bad(ln, "restart: file must be given");
       } else if (c == "topography") {
-         return bad(ln, "topography needs a curvilinear grid stacked on the Cartesian one, which this front end "
-                        "does not build");
+         if ((rc = parse_topography(tk, ln, d))) return rc;
       } else {
          return bad(ln, "command not supported: " + c);
       }
@@ -440,42 +495,78 @@ int make_dirs(const std::string& path)
    return OK;
 }
 
+// One grid for the material blocks: where its points are and which
+// extrapolations the reference applies to it (EW::setup_materials,
+// EW.C:5443-5466).
+struct MatGrid {
+   int nx = 0, ny = 0, nz = 0;
+   double h = 0, zmin = 0;
+   const double* zgrid = nullptr;     // curvilinear grid: z over the box
+   const Topography* topo = nullptr;  // surface of a stacked Cartesian grid
+   const StackedGrids* stack = nullptr;
+   bool ext_low = true, ext_high = true;
+};
+
 // material of the blocks (MaterialBlock::set_material_properties,
-// MaterialBlock.C:115-190; EW::setup_materials, EW.C:5443-5532): interior
-// columns at every k, -1 where no block reaches; ghosts extrapolated in z,
-// then in x and y; then mu = rho cs^2, lambda = rho cp^2 - 2 mu
-int block_material(const Deck& d, std::vector<double>& rho, std::vector<double>& mu, std::vector<double>& la)
+// MaterialBlock.C:115-262): a Cartesian grid's interior columns at every k
+// (depth below the surface, or z with absdepth=1), every point of a
+// curvilinear grid (depth below the grid's top plane); -1 where no block
+// reaches; ghosts extrapolated in z (the ends the grid has), then in x and y
+// (EW::extrapolateInZ / extrapolateInXY, EW.C:5493-5550); then
+// mu = rho cs^2, lambda = rho cp^2 - 2 mu
+int block_material(const Deck& d, const MatGrid& g, std::vector<double>& rho, std::vector<double>& mu,
+                   std::vector<double>& la)
 {
-   const int nx = d.nx, ny = d.ny, nz = d.nz;
+   const int nx = g.nx, ny = g.ny, nz = g.nz;
    const long ni = nx + 4, nj = ny + 4, nk = nz + 4, nij = ni * nj, n = nij * nk;
    std::vector<double> cs(n, -1.0), cp(n, -1.0);
    rho.assign(n, -1.0);
    auto P = [&](int i, int j, int k) { return (i + 1) + ni * (j + 1) + nij * (k + 1); };
    const double tol = 1e-5;
+   const int ij0 = g.zgrid ? -1 : 1, i1 = g.zgrid ? nx + 2 : nx, j1 = g.zgrid ? ny + 2 : ny;
+   // surface z of each interior column of a stacked Cartesian grid
+   std::vector<double> ztop;
+   if (!g.zgrid && g.topo) {
+      ztop.assign(ni * nj, 0.0);
+      for (int j = 1; j <= ny; j++)
+         for (int i = 1; i <= nx; i++) {
+            double zt = 0;
+            if (!topo_surface_z(*g.topo, *g.stack, (i - 1) * g.h, (j - 1) * g.h, zt))
+               return set_error(E_INVAL, "deck: surface undefined over the grid");
+            ztop[(i + 1) + ni * (j + 1)] = zt;
+         }
+   }
    for (const DeckBlock& b : d.blocks)
       for (int k = -1; k <= nz + 2; k++)
-         for (int j = 1; j <= ny; j++)
-            for (int i = 1; i <= nx; i++) {
-               const double x = (i - 1) * d.h, y = (j - 1) * d.h, z = (k - 1) * d.h;
-               if (b.x1 - tol <= x && x <= b.x2 + tol && b.y1 - tol <= y && y <= b.y2 + tol && b.z1 - tol <= z &&
-                   z <= b.z2 + tol) {
+         for (int j = ij0; j <= j1; j++)
+            for (int i = ij0; i <= i1; i++) {
+               const double x = (i - 1) * g.h, y = (j - 1) * g.h;
+               double depth;
+               if (g.zgrid) {
+                  const double z = g.zgrid[P(i, j, k)];
+                  depth = b.absdepth ? z : z - g.zgrid[P(i, j, 1)];
+               } else {
+                  const double z = g.zmin + (k - 1) * g.h;
+                  depth = b.absdepth || !g.topo ? z : z - ztop[(i + 1) + ni * (j + 1)];
+               }
+               if (b.x1 - tol <= x && x <= b.x2 + tol && b.y1 - tol <= y && y <= b.y2 + tol &&
+                   b.z1 - tol <= depth && depth <= b.z2 + tol) {
                   const long p = P(i, j, k);
-                  if (b.rho != -1) rho[p] = b.rho + b.rhograd * z;
-                  if (b.vs != -1) cs[p] = b.vs + b.vsgrad * z;
-                  if (b.vp != -1) cp[p] = b.vp + b.vpgrad * z;
+                  if (b.rho != -1) rho[p] = b.rho + b.rhograd * depth;
+                  if (b.vs != -1) cs[p] = b.vs + b.vsgrad * depth;
+                  if (b.vp != -1) cp[p] = b.vp + b.vpgrad * depth;
                }
             }
    for (std::vector<double>* f : {&rho, &cp, &cs}) {
       std::vector<double>& a = *f;
-      // EW::extrapolateInZ (EW.C:5535-5550), both ends of the one grid
       for (int k = -1; k <= nz + 2; k++) {
          if (k >= 1 && k <= nz) continue;
+         if ((k < 1 && !g.ext_low) || (k > nz && !g.ext_high)) continue;
          const int kk = k < 1 ? 1 : nz;
          for (int j = -1; j <= ny + 2; j++)
             for (int i = -1; i <= nx + 2; i++)
                if (a[P(i, j, k)] == -1) a[P(i, j, k)] = a[P(i, j, kk)];
       }
-      // EW::extrapolateInXY (EW.C:5493-5532)
       for (int k = -1; k <= nz + 2; k++)
          for (int j = -1; j <= ny + 2; j++) {
             for (int i = -1; i < 1; i++)
@@ -499,13 +590,41 @@ int block_material(const Deck& d, std::vector<double>& rho, std::vector<double>&
             if (rho[p] == -1 || cs[p] == -1 || cp[p] == -1)
                return set_error(E_INVAL, "deck: the blocks leave the material undefined at grid point (" +
                                              std::to_string(i) + "," + std::to_string(j) + "," + std::to_string(k) +
-                                             ")");
+                                             ")" + (g.zgrid ? " of the curvilinear grid" : ""));
          }
    mu.resize(n);
    la.resize(n);
    for (long p = 0; p < n; p++) {
       mu[p] = rho[p] * cs[p] * cs[p];
       la[p] = rho[p] * cp[p] * cp[p] - 2 * mu[p];
+   }
+   return OK;
+}
+
+// The receiver files of the stations recorded by one EW object: station r
+// of st is receiver idx[r] of e (idx[r] < 0: not recorded).  Sample i is
+// labelled t = i dt from the start time 0, also on a restarted run
+// (TimeSeries::allocateRecordingArrays, m_shift = mTstart - m_t0 = 0).
+// Velocity stations record nothing for the initial data (EW.C:2488-2490):
+// their first sample is the centred difference of the first step, so the
+// object's initial sample is dropped.
+int write_recorded(const std::string& path, EW& e, const std::vector<Station>& st, const std::vector<int>& idx)
+{
+   const int nrec = (int)e.m_rec.size() / 4;
+   int ns = 0, rc;
+   if ((rc = e.get_receivers(&ns, nullptr))) return rc;
+   std::vector<double> rec((size_t)3 * nrec * ns);
+   if (ns > 0 && nrec > 0 && (rc = e.get_receivers(&ns, rec.data()))) return rc;
+   GeoFrame geo;
+   for (size_t r = 0; r < st.size(); r++) {
+      if (idx[r] < 0) continue;
+      const int skip = st[r].mode == 1 ? 1 : 0;
+      const int n = std::max(ns - skip, 0);
+      std::vector<double> s((size_t)3 * n);
+      for (int q = 0; q < n; q++)
+         for (int c = 0; c < 3; c++) s[3 * q + c] = rec[((size_t)(q + skip) * nrec + idx[r]) * 3 + c];
+      if ((rc = write_station(path == "./" ? std::string(".") : path, st[r], geo, 0.0, e.mDt, n, s.data())))
+         return rc;
    }
    return OK;
 }
@@ -533,11 +652,153 @@ struct CkptState {
 
 }  // namespace
 
+namespace {
+
+// A deck with topography: the curvilinear grid under the Gaussian hill
+// stacked on the Cartesian grid below topography zmax (EW::parseInputFile
+// and EW::setupRun with two grids, EW.C:2171-2336), stepped in the
+// reference stage order.
+int run_deck_topo(const Deck& d, const std::string& path, int mode, DeckResult& out)
+{
+   if (mode == 2) return set_error(E_INVAL, "deck: the FP32 form steps one Cartesian grid; topography needs mode 0 or 1");
+   if (d.point_source_test) return set_error(E_INVAL, "deck: testpointsource with topography is not supported");
+   if (!d.checkpoints.empty() || !d.restart.empty())
+      return set_error(E_INVAL, "deck: checkpoint and restart files of two grids are not supported");
+   StackedGrids sg;
+   int rc;
+   if ((rc = stacked_grid_sizes(d.topo, d.nx, d.ny, d.h, d.z, sg))) return rc;
+   EWOptions oc, ov;
+   oc.grid.nx = ov.grid.nx = d.nx;
+   oc.grid.ny = ov.grid.ny = d.ny;
+   oc.grid.h = ov.grid.h = d.h;
+   oc.grid.nz = sg.nz_cart;
+   oc.grid.zmin = sg.zmin_cart;
+   ov.grid.nz = sg.nz_curv;
+   ov.grid.zmin = 0.0;
+   // sides (EW::default_bcs, EW.C:6077-6082): supergrid around, the free
+   // surface on top of the curvilinear grid, the common plane between them
+   for (int q = 0; q < 6; q++) oc.grid.bc[q] = ov.grid.bc[q] = BC_SUPERGRID;
+   oc.grid.bc[4] = BC_CCINTERFACE;
+   ov.grid.bc[4] = BC_FREE;
+   ov.grid.bc[5] = BC_CCINTERFACE;
+   oc.grid.sg_gp_thickness = ov.grid.sg_gp_thickness = d.sg_gp;
+   oc.corder = ov.corder = mode == 1 ? d.corder : 1;
+   oc.beta = ov.beta = d.sg_dc;
+   oc.cfl = ov.cfl = d.cfl;
+   EW cart(oc), curv(ov);
+   if ((rc = cart.init()) || (rc = curv.init())) return rc;
+   std::vector<double> zc;
+   curvilinear_grid_z(d.topo, sg, zc);
+   if ((rc = curv.set_topography(zc.data(), false))) return rc;
+   {
+      std::vector<double> rho, mu, la;
+      MatGrid mc;
+      mc.nx = d.nx;
+      mc.ny = d.ny;
+      mc.nz = sg.nz_cart;
+      mc.h = d.h;
+      mc.zmin = sg.zmin_cart;
+      mc.topo = &d.topo;
+      mc.stack = &sg;
+      mc.ext_low = false;
+      if ((rc = block_material(d, mc, rho, mu, la)) ||
+          (rc = cart.set_material(rho.data(), mu.data(), la.data(), false)))
+         return rc;
+      MatGrid mv = mc;
+      mv.nz = sg.nz_curv;
+      mv.zmin = 0;
+      mv.zgrid = zc.data();
+      mv.ext_low = true;
+      mv.ext_high = false;
+      if ((rc = block_material(d, mv, rho, mu, la)) ||
+          (rc = curv.set_material(rho.data(), mu.data(), la.data(), false)))
+         return rc;
+   }
+   // sources: the curvilinear grid's Jacobian weights them (Source.C:1798-1801)
+   if (!d.sources.empty()) {
+      std::vector<double> jac(curv.mJ.npts());
+      if ((rc = check_hip(hipMemcpy(jac.data(), curv.mJ.dev, sizeof(double) * jac.size(), hipMemcpyDeviceToHost),
+                          "Jacobian readback")))
+         return rc;
+      for (const DeckSource& s : d.sources) {
+         if (s.x < 0 || s.x > d.x || s.y < 0 || s.y > d.y)
+            return set_error(E_INVAL, "deck: source positioned outside the grid");
+         MomentSource m;
+         m.x = s.x;
+         m.y = s.y;
+         m.z = s.z;
+         if (s.topodepth) {  // Source::adjust_zcoord (Source.C:517-531)
+            double zt = 0;
+            if (!topo_surface_z(d.topo, sg, s.x, s.y, zt)) return set_error(E_INVAL, "deck: source off the surface");
+            m.z += zt;
+         }
+         if (m.z > sg.zmax) return set_error(E_INVAL, "deck: source below the grid");
+         for (int q = 0; q < 6; q++) m.m[q] = s.m[q];
+         m.t0 = s.t0;
+         m.freq = s.freq;
+         m.tfkind = s.tfkind;
+         std::vector<GridPointSource> pc, pv;
+         rc = stacked_moment_source(d.topo, sg, jac.data(), m, pc, pv);
+         if (rc == 1) continue;  // above the surface: ignored as the reference does
+         if (rc) return rc;
+         for (const GridPointSource& p : pc)
+            if ((rc = cart.add_grid_point_source(p))) return rc;
+         for (const GridPointSource& p : pv)
+            if ((rc = curv.add_grid_point_source(p))) return rc;
+      }
+   }
+   std::vector<Station> st = d.stations;
+   std::vector<int> in_cart(st.size(), -1), in_curv(st.size(), -1);
+   int utc[7];
+   utc_now(utc);
+   for (size_t r = 0; r < st.size(); r++) {
+      Station& s = st[r];
+      if (s.x < 0 || s.x > d.x || s.y < 0 || s.y > d.y)
+         return set_error(E_INVAL, "deck: receiver " + s.file + " outside the grid");
+      int grid = 0, i = 0, j = 0, k = 0;
+      rc = stacked_receiver_point(d.topo, sg, s.x, s.y, s.z, d.station_topodepth[r] != 0, grid, i, j, k);
+      if (rc == 1) continue;  // above the surface: the reference writes no file
+      if (rc) return set_error(E_INVAL, "deck: receiver " + s.file + " outside the grid");
+      EW& e = grid == 0 ? cart : curv;
+      int index = 0;
+      if ((rc = e.add_receiver_ijk(i, j, k, s.mode, &index))) return rc;
+      (grid == 0 ? in_cart : in_curv)[r] = index;
+      // the grid point recorded (TimeSeries.C:172-187)
+      s.gx = (i - 1) * d.h;
+      s.gy = (j - 1) * d.h;
+      s.gz = grid == 0 ? sg.zmin_cart + (k - 1) * d.h
+                       : zc[(i + 1) + (long)(d.nx + 4) * (j + 1) + (long)(d.nx + 4) * (d.ny + 4) * (k + 1)];
+      std::memcpy(s.utc, utc, sizeof(utc));
+   }
+   GridStack stack(cart, curv);
+   if ((rc = stack.setup(d.tmax, d.steps >= 0 ? d.steps : 0))) return rc;
+   if ((rc = make_dirs(path))) return rc;
+   const int total = cart.mNumberOfTimeSteps;
+   if ((rc = stack.step_reference(total))) return rc;
+   out = DeckResult{};
+   out.nx = d.nx;
+   out.ny = d.ny;
+   out.nz = sg.nz_cart;
+   out.ngrids = 2;
+   out.nz_curv = sg.nz_curv;
+   out.h = d.h;
+   out.dt = cart.mDt;
+   out.t = cart.mT;
+   out.nsteps = total;
+   out.cycle = cart.mCycle;
+   out.nstations = (int)st.size();
+   if ((rc = write_recorded(path, cart, st, in_cart)) || (rc = write_recorded(path, curv, st, in_curv))) return rc;
+   return OK;
+}
+
+}  // namespace
+
 int run_deck(const Deck& d, const std::string& path_override, int mode, DeckResult& out)
 {
    if (mode < 0 || mode > 2) return set_error(E_INVAL, "deck: mode must be 0, 1 or 2");
    const std::string path = path_override.empty() ? d.path
                                                   : (path_override.back() == '/' ? path_override : path_override + "/");
+   if (d.topography) return run_deck_topo(d, path, mode, out);
    EWOptions o;
    o.grid.nx = d.nx;
    o.grid.ny = d.ny;
@@ -561,7 +822,12 @@ int run_deck(const Deck& d, const std::string& path_override, int mode, DeckResu
       if ((rc = e.set_material_const(d.tp_rho, d.tp_cs, d.tp_cp))) return rc;
    } else {
       std::vector<double> rho, mu, la;
-      if ((rc = block_material(d, rho, mu, la)) || (rc = e.set_material(rho.data(), mu.data(), la.data(), false)))
+      MatGrid mg;
+      mg.nx = d.nx;
+      mg.ny = d.ny;
+      mg.nz = d.nz;
+      mg.h = d.h;
+      if ((rc = block_material(d, mg, rho, mu, la)) || (rc = e.set_material(rho.data(), mu.data(), la.data(), false)))
          return rc;
    }
    for (const DeckSource& s : d.sources) {
@@ -651,26 +917,9 @@ int run_deck(const Deck& d, const std::string& path_override, int mode, DeckResu
       if (!es) return set_error(E_INVAL, "cannot write " + path + "PointSourceErr.txt");
    }
    if (!st.empty()) {
-      int ns = 0;
-      if ((rc = e.get_receivers(&ns, nullptr))) return rc;
-      std::vector<double> rec((size_t)3 * st.size() * ns);
-      if (ns > 0 && (rc = e.get_receivers(&ns, rec.data()))) return rc;
-      // sample i is labelled t = i dt from the start time 0, also on a
-      // restarted run (TimeSeries::allocateRecordingArrays, m_shift =
-      // mTstart - m_t0 = 0).  Velocity stations record nothing for the
-      // initial data (EW.C:2488-2490): their first sample is the centred
-      // difference of the first step, so the object's initial sample is
-      // dropped.
-      GeoFrame geo;
-      for (size_t r = 0; r < st.size(); r++) {
-         const int skip = st[r].mode == 1 ? 1 : 0;
-         const int n = std::max(ns - skip, 0);
-         std::vector<double> s((size_t)3 * n);
-         for (int q = 0; q < n; q++)
-            for (int c = 0; c < 3; c++) s[3 * q + c] = rec[((size_t)(q + skip) * st.size() + r) * 3 + c];
-         if ((rc = write_station(path == "./" ? std::string(".") : path, st[r], geo, 0.0, e.mDt, n, s.data())))
-            return rc;
-      }
+      std::vector<int> idx(st.size());
+      for (size_t r = 0; r < st.size(); r++) idx[r] = (int)r;
+      if ((rc = write_recorded(path, e, st, idx))) return rc;
    }
    return OK;
 }

@@ -1,18 +1,21 @@
-// deck.h -- input-deck front end: the reference's deck commands for one
-// Cartesian grid (EW::parseInputFile, EW.C:2171-2285) run through the EW
-// object, with receiver output (TimeSeries) and checkpoint/restart files
-// (CheckPoint.C) in the reference's formats.
+// deck.h -- input-deck front end: the reference's deck commands
+// (EW::parseInputFile, EW.C:2171-2285) for one Cartesian grid, or with a
+// Gaussian-hill topography for a curvilinear grid stacked on the Cartesian
+// one, run through EW objects, with receiver output (TimeSeries) and
+// checkpoint/restart files (CheckPoint.C) in the reference's formats.
 #pragma once
 #include <string>
 #include <vector>
 
 #include "timeseries.h"
+#include "topo.h"
 
 namespace sw4amd {
 
 struct DeckBlock {  // block (EW::processMaterialBlock, EW.C:1385-1600)
    double vp = -1, vs = -1, rho = -1, vpgrad = 0, vsgrad = 0, rhograd = 0;
    double x1 = -1e38, x2 = 1e38, y1 = -1e38, y2 = 1e38, z1 = -1e38, z2 = 1e38;
+   bool absdepth = false;  // absdepth=1: z bounds are z, not depth below the surface
 };
 
 struct DeckSource {  // source (EW::processSource, EW.C:951-1232), moment tensors
@@ -20,6 +23,7 @@ struct DeckSource {  // source (EW::processSource, EW.C:951-1232), moment tensor
    double m[6] = {0, 0, 0, 0, 0, 0};  // xx, xy, xz, yy, yz, zz
    int tfkind = -1;                   // 0 Ricker, 1 Gaussian, 2 C6SmoothBump
    std::string type = "RickerInt";
+   bool topodepth = false;            // depth= given: z below the surface
 };
 
 struct DeckCheckPoint {  // checkpoint (EW::processCheckPoint, EW.C:783-848)
@@ -49,6 +53,10 @@ struct Deck {
    std::vector<DeckBlock> blocks;
    std::vector<DeckSource> sources;
    std::vector<Station> stations;
+   std::vector<char> station_topodepth;  // per station: depth= given
+   // topography input=gaussian (EW::processTopography, EW.C:535-725)
+   bool topography = false;
+   Topography topo;
    std::vector<DeckCheckPoint> checkpoints;
    std::string restart;  // restart file=
 };
@@ -59,9 +67,10 @@ struct Deck {
 int parse_deck(const std::string& text, Deck& d);
 
 struct DeckResult {
-   int nx = 0, ny = 0, nz = 0;
+   int nx = 0, ny = 0, nz = 0;  // nz: the Cartesian grid's
    double h = 0, dt = 0, t = 0;
    int nsteps = 0, cycle = 0, nstations = 0, ncheckpoints = 0;
+   int ngrids = 1, nz_curv = 0;  // with topography: 2 and the curvilinear grid's nz
    bool point_source_test = false;
    double linf = 0, l2 = 0, solinf = 0;
 };

@@ -279,7 +279,40 @@ SourceTable EW::sources() const
    return st;
 }
 
-int EW::setup(double tmax, int nsteps)
+int EW::stable_dt(double* dt)
+{
+   if (!m_material_set) return set_error(E_STATE, "EW: set the material before computing dt");
+   // EW::computeDT (EW.C:5041-5146): Cartesian grid, or the curvilinear
+   // grid's principal-part eigenvalues (EW.C:5063-5136, dt = CFL/sqrt(max))
+   int rc;
+   if (m_curv)
+      rc = launch_maxeig_curv(m_opt.corder, m_box, mMetric.dev, mJ.dev, mMu.dev, mLambda.dev, mRho.dev, m_dred, m_nred,
+                              m_dred + 3 * m_nred, m_stream);
+   else
+      rc = launch_maxeig(m_box.npts, mMu.dev, mLambda.dev, mRho.dev, m_dred, m_nred, m_dred + 3 * m_nred, m_stream);
+   if (rc) return rc;
+   double eigmax = 0;
+   if ((rc = check_hip(hipMemcpyAsync(&eigmax, m_dred + 3 * m_nred, sizeof(double), hipMemcpyDeviceToHost, m_stream), "dt readback")) ||
+       (rc = check_hip(hipStreamSynchronize(m_stream), "dt readback")))
+      return rc;
+   if (!(eigmax > 0)) return set_error(E_INVAL, "EW: material gives a non-positive (4mu+lambda)/rho");
+   const double ieigmax = 1 / std::sqrt(eigmax);
+   *dt = std::min(1e10, m_opt.cfl * (m_curv ? 1.0 : m_opt.grid.h) * ieigmax);
+   return OK;
+}
+
+int EW::add_grid_point_source(const GridPointSource& p)
+{
+   if (m_setup_done) return set_error(E_STATE, "EW: sources must be added before setup");
+   const GridSetup& g = m_opt.grid;
+   if (p.i < -1 || p.i > g.nx + 2 || p.j < -1 || p.j > g.ny + 2 || p.k < 1 || p.k > g.nz)
+      return set_error(E_INVAL, "EW: grid-point source outside the grid");
+   if (p.tfkind < TF_RICKER || p.tfkind > TF_C6SMOOTHBUMP) return set_error(E_INVAL, "EW: unknown time function");
+   m_gps_extra.push_back(p);
+   return OK;
+}
+
+int EW::setup(double tmax, int nsteps, double dt0)
 {
    if (!m_material_set) return set_error(E_STATE, "EW: set the material before setup");
    if (m_setup_done) return set_error(E_STATE, "EW: setup called twice");
@@ -303,21 +336,10 @@ int EW::setup(double tmax, int nsteps)
          return rc;
    }
    make_boundary_windows(g, m_wind, m_nbcpts);
-   // EW::computeDT (EW.C:5041-5146): Cartesian grid, or the curvilinear
-   // grid's principal-part eigenvalues (EW.C:5063-5136, dt = CFL/sqrt(max))
-   if (m_curv)
-      rc = launch_maxeig_curv(m_opt.corder, m_box, mMetric.dev, mJ.dev, mMu.dev, mLambda.dev, mRho.dev, m_dred, m_nred,
-                              m_dred + 3 * m_nred, m_stream);
-   else
-      rc = launch_maxeig(m_box.npts, mMu.dev, mLambda.dev, mRho.dev, m_dred, m_nred, m_dred + 3 * m_nred, m_stream);
-   if (rc) return rc;
-   double eigmax = 0;
-   if ((rc = check_hip(hipMemcpyAsync(&eigmax, m_dred + 3 * m_nred, sizeof(double), hipMemcpyDeviceToHost, m_stream), "dt readback")) ||
-       (rc = check_hip(hipStreamSynchronize(m_stream), "dt readback")))
+   if (dt0 > 0)
+      mDt = dt0;
+   else if ((rc = stable_dt(&mDt)))
       return rc;
-   if (!(eigmax > 0)) return set_error(E_INVAL, "EW: material gives a non-positive (4mu+lambda)/rho");
-   const double ieigmax = 1 / std::sqrt(eigmax);
-   mDt = std::min(1e10, m_opt.cfl * (m_curv ? 1.0 : g.h) * ieigmax);
    if (tmax > 0) {
       mNumberOfTimeSteps = static_cast<int>(tmax / mDt + 0.5);
       if (mNumberOfTimeSteps == 0) mNumberOfTimeSteps = 1;
@@ -333,6 +355,7 @@ int EW::setup(double tmax, int nsteps)
       if (q == -2) return set_error(E_INVAL, "EW: source outside the curvilinear grid");
       if (q) return set_error(E_INVAL, "EW: folded grid at the source");
    }
+   m_gps.insert(m_gps.end(), m_gps_extra.begin(), m_gps_extra.end());
    sort_grid_point_sources(g, m_gps, m_first);
    if (!m_gps.empty() && (rc = upload_sources())) return rc;
    mT = 0;
@@ -439,6 +462,18 @@ int EW::add_receiver(double x, double y, double z, int mode, int* index)
       for (int kk = 2; kk <= g.nz; kk++)
          if (std::fabs(col[nij * (kk + 1)] - z) < std::fabs(col[nij * (k + 1)] - z)) k = kk;
    }
+   if (i < 1 || i > g.nx || j < 1 || j > g.ny || k < 1 || k > g.nz)
+      return set_error(E_INVAL, "EW: receiver outside the grid");
+   if (index) *index = (int)m_rec.size() / 4;
+   m_rec.insert(m_rec.end(), {i, j, k, mode});
+   return OK;
+}
+
+int EW::add_receiver_ijk(int i, int j, int k, int mode, int* index)
+{
+   if (m_setup_done) return set_error(E_STATE, "EW: receivers must be added before setup");
+   if (mode != 0 && mode != 1) return set_error(E_INVAL, "EW: receiver mode must be 0 (displacement) or 1 (velocity)");
+   const GridSetup& g = m_opt.grid;
    if (i < 1 || i > g.nx || j < 1 || j > g.ny || k < 1 || k > g.nz)
       return set_error(E_INVAL, "EW: receiver outside the grid");
    if (index) *index = (int)m_rec.size() / 4;

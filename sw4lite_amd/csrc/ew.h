@@ -39,9 +39,11 @@ struct EWOptions {
 };
 
 class Batch;
+class GridStack;
 
 class EW {
    friend class Batch;
+   friend class GridStack;
 
 public:
    explicit EW(const EWOptions& opt);
@@ -58,7 +60,16 @@ public:
    // Curvilinear grid (EW::generate_grid + setup_metric, EW.C:6164-6300):
    // z = grid depth over the box, increasing with k; before setup.
    int set_topography(const double* z, bool on_device);
-   int setup(double tmax, int nsteps);  // setupRun + computeDT
+   // setupRun + computeDT; dt0 > 0 replaces the grid's own stable step
+   // (two stacked grids step with the smaller of their two)
+   int setup(double tmax, int nsteps, double dt0 = 0);
+   // EW::computeDT (EW.C:5041-5146) for this grid alone: CFL h / sqrt(max
+   // (4 mu + lambda) / rho) on a Cartesian grid, CFL / sqrt(max eigenvalue)
+   // on a curvilinear one.  Needs the material (and the topography).
+   int stable_dt(double* dt);
+   // Grid-point sources made outside (curvilinear sources of stacked grids),
+   // added to those of the moment sources at setup
+   int add_grid_point_source(const GridPointSource& p);
 
    // --- reference step methods (EW.h:93-127), single Cartesian grid ---
    int Force(double t, Sarray& F, bool tt);
@@ -122,6 +133,7 @@ public:
 
    // receivers (TimeSeries, EW.C:5854-5875): mode 0 displacement, 1 velocity
    int add_receiver(double x, double y, double z, int mode, int* index);
+   int add_receiver_ijk(int i, int j, int k, int mode, int* index);  // at a given grid point
    int record(const Sarray& Up, const Sarray& Um);  // one sample, after a step
    int get_receivers(int* nsamples, double* out);   // out[sample][rec][3]
    std::vector<int> m_rec;                          // i, j, k, mode per receiver
@@ -155,6 +167,7 @@ public:
    SupergridArrays m_sg_host;
    std::vector<MomentSource> m_sources;
    std::vector<GridPointSource> m_gps;
+   std::vector<GridPointSource> m_gps_extra;
    std::vector<int> m_first;
    hipStream_t m_stream = nullptr;
    int m_variant = 9;   // interior sweep kernels: 0 thread per point, 1-9 LDS-staged k-march

@@ -55,12 +55,15 @@ double Taper::damping(double x) const { return ramp(x, trans) / stretching(x); }
 double Taper::corner(double x) const { return 1.0 - (1.0 - 0.33) * lin(x); }
 
 // EW::setup_supergrid + assign_supergrid_damping_arrays (EW.C:4636-4801),
-// one Cartesian grid.
+// one grid.  A curvilinear grid has neither supergrid side in z
+// (EW.C:4728-4737): its z arrays stay 0, 1, 1.
 void make_supergrid_arrays(const GridSetup& g, SupergridArrays& a)
 {
    const bool use = g.bc[0] == BC_SUPERGRID || g.bc[1] == BC_SUPERGRID || g.bc[2] == BC_SUPERGRID ||
                     g.bc[3] == BC_SUPERGRID || g.bc[4] == BC_SUPERGRID || g.bc[5] == BC_SUPERGRID;
-   const double xmax = (g.nx - 1) * g.h, ymax = (g.ny - 1) * g.h, zmax = (g.nz - 1) * g.h;
+   // the z taper ends at the bottom of this grid, zmin + (nz - 1) h (a
+   // Cartesian grid under a curvilinear one starts below z = 0)
+   const double xmax = (g.nx - 1) * g.h, ymax = (g.ny - 1) * g.h, zmax = g.zmin + (g.nz - 1) * g.h;
    const double w = g.sg_gp_thickness * g.h;
    Taper tx, ty, tz;
    tx.define(g.bc[0] == BC_SUPERGRID, 0.0, g.bc[1] == BC_SUPERGRID, xmax, w);
@@ -130,10 +133,9 @@ void make_boundary_windows(const GridSetup& g, int wind[36], int npts[6])
 // Source discretisation, Source::set_grid_point_sources4 (Source.C:1074-2026)
 // for a single Cartesian grid.
 // ---------------------------------------------------------------------------
-namespace {
 
 // Regularised delta, moments 0..4 exact (Source.C:553-584): values only.
-void delta_weights(double a, double w[6])
+void source_delta_weights(double a, double w[6])
 {
    const double a2 = a * a, a3 = a2 * a, a4 = a3 * a, a5 = a4 * a;
    const double p5 = a5 * (5.0 / 3 - 7.0 / 24 * a - 17 / 12.0 * a2 + 1.125 * a3 - 0.25 * a4);
@@ -146,7 +148,7 @@ void delta_weights(double a, double w[6])
 }
 
 // Regularised delta' (Source.C:587-618): values only.
-void ddelta_weights(double a, double w[6])
+void source_ddelta_weights(double a, double w[6])
 {
    const double a2 = a * a, a3 = a2 * a, a4 = a3 * a;
    const double p5 = a4 * (-25.0 / 12 - 0.75 * a + 59.0 / 12 * a2 - 4 * a3 + a4);
@@ -159,7 +161,7 @@ void ddelta_weights(double a, double w[6])
 }
 
 // Lower-boundary stencils used when kc==3 and ci<=0 (Source.C:621-680).
-void delta_weights_low(double a, double w[6])
+void source_delta_weights_low(double a, double w[6])
 {
    const double a2 = a * a, a3 = a2 * a, a4 = a3 * a;
    w[0] = (2 * a - a2 - 2 * a3 + a4) / 24;
@@ -170,7 +172,7 @@ void delta_weights_low(double a, double w[6])
    w[5] = 0;
 }
 
-void ddelta_weights_low(double a, double w[6])
+void source_ddelta_weights_low(double a, double w[6])
 {
    const double a2 = a * a, a3 = a2 * a;
    w[0] = (-1 + a + 3 * a2 - 2 * a3) / 12;
@@ -180,8 +182,6 @@ void ddelta_weights_low(double a, double w[6])
    w[4] = (1 + a - 3 * a2 - 2 * a3) / 12;
    w[5] = 0;
 }
-
-}  // namespace
 
 int moment_source_points(const GridSetup& g, const MomentSource& src, std::vector<GridPointSource>& out)
 {
@@ -202,15 +202,15 @@ int moment_source_points(const GridSetup& g, const MomentSource& src, std::vecto
    if (kc >= Nz - 2) kc = Nz - 3;
    const double ai = q0 - ic, bi = r0 - jc, ci = s0 - kc;
    double wi[6], wj[6], wk[6], dwi[6], dwj[6], dwk[6];
-   delta_weights(ai, wi);
-   delta_weights(bi, wj);
-   delta_weights(ci, wk);
-   ddelta_weights(ai, dwi);
-   ddelta_weights(bi, dwj);
-   ddelta_weights(ci, dwk);
+   source_delta_weights(ai, wi);
+   source_delta_weights(bi, wj);
+   source_delta_weights(ci, wk);
+   source_ddelta_weights(ai, dwi);
+   source_ddelta_weights(bi, dwj);
+   source_ddelta_weights(ci, dwk);
    if (kc == 3 && ci <= 0) {
-      delta_weights_low(ci, wk);
-      ddelta_weights_low(ci, dwk);
+      source_delta_weights_low(ci, wk);
+      source_ddelta_weights_low(ci, dwk);
    }
    // SBP norm weights near a one-sided boundary (Source.C:1085, 1411-1442)
    static const double normwgh[4] = {17.0 / 48.0, 59.0 / 48.0, 43.0 / 48.0, 49.0 / 48.0};
@@ -308,7 +308,7 @@ void column_at(const CurvGrid& G, const double wi[6], const double wj[6], int ic
 {
    const int kc = std::min(std::max((int)std::floor(s), 1), G.g.nz - 1);
    double wk[6];
-   delta_weights(s - kc, wk);
+   source_delta_weights(s - kc, wk);
    z = zs = 0;
    for (int c = 0; c < 6; c++) {
       const int k = kc - 2 + c;
@@ -339,10 +339,10 @@ int moment_source_points_curv(const GridSetup& g, const double* zgrid, const Mom
    if (jc >= Nj - 2) jc = Nj - 3;
    const double ai = q0 - ic, bi = r0 - jc;
    double wi[6], wj[6], dwi[6], dwj[6];
-   delta_weights(ai, wi);
-   delta_weights(bi, wj);
-   ddelta_weights(ai, dwi);
-   ddelta_weights(bi, dwj);
+   source_delta_weights(ai, wi);
+   source_delta_weights(bi, wj);
+   source_ddelta_weights(ai, dwi);
+   source_ddelta_weights(bi, dwj);
    // s0 with z(q0, r0, s0) = src.z: bracket on the interpolated column, then
    // Newton on the delta-weight interpolant (EW::invert_curvilinear_grid_mapping)
    double s0 = -1;
@@ -376,19 +376,19 @@ int moment_source_points_curv(const GridSetup& g, const double* zgrid, const Mom
    if (kc >= Nz - 2) kc = Nz - 3;
    const double ci = s0 - kc;
    double wk[6], dwk[6];
-   delta_weights(ci, wk);
-   ddelta_weights(ci, dwk);
+   source_delta_weights(ci, wk);
+   source_ddelta_weights(ci, dwk);
    if (kc == 3 && ci <= 0) {
-      delta_weights_low(ci, wk);
-      ddelta_weights_low(ci, dwk);
+      source_delta_weights_low(ci, wk);
+      source_ddelta_weights_low(ci, dwk);
    }
    // grid derivatives at the source (index units), interpolated before the
    // SBP norm corrections are applied to the k weights
    double zq = 0, zr = 0, zs = 0;
    {
       double wk0[6];
-      delta_weights(ci, wk0);
-      if (kc == 3 && ci <= 0) delta_weights_low(ci, wk0);
+      source_delta_weights(ci, wk0);
+      if (kc == 3 && ci <= 0) source_delta_weights_low(ci, wk0);
       for (int c = 0; c < 6; c++)
          for (int b = 0; b < 6; b++)
             for (int a = 0; a < 6; a++) {

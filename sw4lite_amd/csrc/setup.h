@@ -6,7 +6,9 @@ namespace sw4amd {
 
 // boundaryConditionType values of the reference (src/double/sw4.h:37-38);
 // they are part of the bcfortsg ABI.
-enum BcType { BC_FREE = 0, BC_DIRICHLET = 1, BC_SUPERGRID = 2, BC_PERIODIC = 3, BC_NONE = 8 };
+// BC_CCINTERFACE: the side of a grid that a curvilinear grid shares with
+// the Cartesian grid below it, set by EW::enforceCartTopo, not by bcfortsg.
+enum BcType { BC_FREE = 0, BC_DIRICHLET = 1, BC_SUPERGRID = 2, BC_PERIODIC = 3, BC_CCINTERFACE = 4, BC_NONE = 8 };
 
 // Time functions supported on the hot path (timeDep subset).
 enum TimeFn { TF_RICKER = 0, TF_GAUSSIAN = 1, TF_C6SMOOTHBUMP = 2 };
@@ -50,6 +52,13 @@ struct GridPointSource {
 };
 
 void make_supergrid_arrays(const GridSetup& g, SupergridArrays& a);
+// Weights of the regularised delta and its derivative on the 6-point source
+// stencil at offset a, and their one-sided forms next to a free surface
+// (Source::getsourcewgh / getsourcedwgh / ...low, Source.C:553-680).
+void source_delta_weights(double a, double w[6]);
+void source_ddelta_weights(double a, double w[6]);
+void source_delta_weights_low(double a, double w[6]);
+void source_ddelta_weights_low(double a, double w[6]);
 void make_boundary_windows(const GridSetup& g, int wind[36], int npts[6]);
 int moment_source_points(const GridSetup& g, const MomentSource& src, std::vector<GridPointSource>& out);
 // Curvilinear grid of depth zgrid (over the box, i fastest): 0, or -1 grid

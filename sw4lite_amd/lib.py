@@ -28,7 +28,12 @@ class GridDesc(C.Structure):
 class DeckReport(C.Structure):
     _fields_ = [("nx", I), ("ny", I), ("nz", I), ("h", D), ("dt", D), ("t", D), ("nsteps", I), ("cycle", I),
                 ("nstations", I), ("ncheckpoints", I), ("point_source_test", I), ("linf", D), ("l2", D),
-                ("solinf", D)]
+                ("solinf", D), ("ngrids", I), ("nz_curv", I)]
+
+
+class Topography(C.Structure):
+    _fields_ = [("zmax", D), ("order", I), ("zeta_break", D), ("amp", D), ("xc", D), ("yc", D), ("lx", D),
+                ("ly", D)]
 
 
 class Station(C.Structure):
@@ -71,6 +76,9 @@ _SIGS = {
     "sw4hip_ew_gradient_plan": (I, [P, I, I, P, P]),
     "sw4hip_ew_release_workspace": (I, [P]),
     "sw4hip_host_moment_source_curv": (I, [C.POINTER(GridDesc), P, D, D, D, P, I, IP, P, P]),
+    "sw4hip_host_topography_grid": (I, [C.POINTER(Topography), I, I, D, D, IP, IP, P]),
+    "sw4hip_host_topography_receiver": (I, [C.POINTER(Topography), I, I, D, D, D, D, D, I, IP, P]),
+    "sw4hip_host_topography_source": (I, [C.POINTER(Topography), I, I, D, D, P, D, D, D, I, P, I, IP, P, P]),
     "sw4hip_ew_set_topography": (I, [P, P, I]),
     "sw4hip_metric": (I, [I] * 7 + [P] * 5 + [P]),
     "sw4hip_rhs4sgcurv": (I, [I] * 7 + [P] * 12 + [P]),

@@ -17,6 +17,15 @@ Only numbers and small output files are stored.
                         source, displacement and velocity stations, each in
                         x/y/z and in east/north/up components) after 40 steps;
                         the deck itself is ref_stations/stations.in.
+  topo_curvilinear.npz  the stacked grids of the reference's topography
+                        deck (decks/curvilinear.in, the reference's
+                        pytest/reference/topo/curvilinear.in) before any
+                        step: grid sizes, z of the curvilinear grid on its
+                        surface and along one column, the grid-point sources
+                        of the moment source and the Jacobian at them.  The
+                        deck's own receiver files are the reference's fixture
+                        (topo_curvilinear/sta0{1,2,3}.txt); the reference
+                        program built here reproduces them to 4e-14 of 11.6.
   ref_checkpoint/       the reference's checkpoint file
                         (CheckPoint::write_checkpoint) of a small deck at
                         cycle 6, gzip-compressed, run with the interleaved
@@ -110,7 +119,33 @@ def checkpoint():
     print("checkpoint:", names)
 
 
-RUNS = {"c1": c1, "stations": stations, "checkpoint": checkpoint}
+def topo():
+    """The curvilinear grid of the reference's topography deck
+    (pytest/reference/topo/curvilinear.in, copied as decks/curvilinear.in)
+    with no time steps: grid sizes, z of the curvilinear grid on its surface
+    plane and along the column of receiver sta03, the grid-point sources of
+    its moment source (grid 0 Cartesian, 1 curvilinear) and the curvilinear
+    Jacobian at each of them."""
+    import numpy as np
+    deck = open(os.path.join(HERE, "decks", "curvilinear.in")).read()
+    deck = "\n".join(ln for ln in deck.splitlines() if not ln.strip().startswith("fileio"))
+    deck = deck.replace("time t=0.5", "time steps=0")
+    with R.RefEW(deck) as e:
+        (hc, zminc, nzc), (hv, _, nzv) = e.grid(0), e.grid(1)
+        ib, ie, jb, je, kb, ke = e.boxes[1]
+        ni, nj, nk = ie - ib + 1, je - jb + 1, ke - kb + 1
+        z = e.array(7, 1).reshape(nk, nj, ni)
+        jac = e.array(9, 1).reshape(nk, nj, ni)
+        ijkg, f, _, _ = e.point_sources()
+        gijk = np.stack([ijkg[:, 3], ijkg[:, 0], ijkg[:, 1], ijkg[:, 2]], axis=1).astype(np.int32)
+        jac_at = np.array([jac[k - kb, j - jb, i - ib] if g == 1 else 0.0 for g, i, j, k in gijk])
+        np.savez_compressed(os.path.join(HERE, "topo_curvilinear.npz"), h=hc, zmin_cart=zminc, nz_cart=nzc,
+                            nz_curv=nzv, box_curv=np.array(e.boxes[1]), z_surface=z[1 - kb], z_column=z[:, 26 - jb, 31 - ib],
+                            gijk=gijk, force=f, jac_at=jac_at)
+    print("topo:", nzc, nzv, len(gijk), "grid-point sources")
+
+
+RUNS = {"c1": c1, "stations": stations, "checkpoint": checkpoint, "topo": topo}
 
 if __name__ == "__main__":
     O.build(ref=True)

@@ -99,6 +99,7 @@ def test_checkpoint_rejects_other_grids(tmp_path):
 @pytest.mark.parametrize("deck,grid,steps,tmax", [
     ("LOH.1-h100.in", (301, 301, 171, 100.0), -1, 9.0),
     ("pointsource.in", (201, 201, 101, 0.04), -1, 0.6),
+    ("curvilinear.in", (65, 65, 251, 0.02), -1, 0.5),
 ])
 def test_reference_decks_parse(deck, grid, steps, tmax):
     rep = ew.parse_deck(open(os.path.join(GOLDEN, "decks", deck)).read())
@@ -107,7 +108,9 @@ def test_reference_decks_parse(deck, grid, steps, tmax):
 
 
 @pytest.mark.parametrize("text,msg", [
-    ("grid h=1 x=20 y=20 z=20\ntime t=1\ntopography input=gaussian zmax=2\n", "topography"),
+    ("grid h=1 x=20 y=20 z=20\ntime t=1\ntopography input=grid zmax=2 file=a.topo\n", "input=grid is not supported"),
+    ("grid h=1 x=20 y=20 z=20\ntime t=1\ntopography input=gaussian\n", "zmax must be given"),
+    ("grid h=1 x=20 y=20 z=20\ntime t=1\ntopography input=gaussian zmax=2 analyticalMetric=1\n", "analyticalMetric"),
     ("grid h=1 x=20 y=20 z=20\ntime t=1\nsource x=1 y=1 z=1 fx=1 type=Ricker\n", "point forces"),
     ("grid h=1 x=20 y=20 z=20\ntime t=1\nsource x=1 y=1 z=1 mxx=1 type=Brune\n", "Brune"),
     ("grid h=1 x=20 y=20 z=20\ntime t=1 steps=3\n", "both t and steps"),
