@@ -387,6 +387,13 @@ int sw4hip_ew_reset(sw4hip_ew* ew);
  * Outputs overwritten; device pointers when on_device. */
 int sw4hip_ew_gradient(sw4hip_ew* ew, int nsteps, int ckpt, const double* grad_traces, double* grho,
                        double* gmu, double* glambda, int on_device);
+/* The gradient of the FP32 form's misfit: the forward run and its
+ * recomputation step in FP32 with the compensated update (sw4hip_ew_step
+ * mode 2), checkpointing its float state (U, C, D: 36 bytes a point against
+ * 48 for (U, Um) in FP64); the reverse steps run in FP64 on that trajectory.
+ * Arguments as sw4hip_ew_gradient.  No counterpart in the reference. */
+int sw4hip_ew_gradient_comp(sw4hip_ew* ew, int nsteps, int ckpt, const double* grad_traces, double* grho,
+                            double* gmu, double* glambda, int on_device);
 /* The checkpoint interval sw4hip_ew_gradient would use for (nsteps, ckpt)
  * and the device bytes it needs (device_bytes may be NULL). */
 int sw4hip_ew_gradient_plan(sw4hip_ew* ew, int nsteps, int ckpt, int* used_ckpt, double* device_bytes);
@@ -433,6 +440,10 @@ int sw4hip_batch_tune(sw4hip_batch* batch, int ncand, const int* cand, int* best
 int sw4hip_batch_gradient(sw4hip_batch* batch, int nsteps, int ckpt, const double* const* grad_traces,
                           double* const* grho, double* const* gmu, double* const* glambda, int on_device,
                           int max_concurrent);
+/* The same with sw4hip_ew_gradient_comp for every shot. */
+int sw4hip_batch_gradient_comp(sw4hip_batch* batch, int nsteps, int ckpt, const double* const* grad_traces,
+                               double* const* grho, double* const* gmu, double* const* glambda, int on_device,
+                               int max_concurrent);
 /* Advance every shot by nsteps; use_graph: replay the captured period. */
 int sw4hip_batch_step(sw4hip_batch* batch, int nsteps, int use_graph);
 int sw4hip_ew_sync(sw4hip_ew* ew);

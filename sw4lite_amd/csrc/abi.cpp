@@ -944,6 +944,13 @@ int sw4hip_ew_gradient(sw4hip_ew* h, int nsteps, int ckpt, const double* grad_tr
    return h->ew->gradient(nsteps, ckpt, grad_traces, grho, gmu, glambda, on_device != 0);
 }
 
+int sw4hip_ew_gradient_comp(sw4hip_ew* h, int nsteps, int ckpt, const double* grad_traces, double* grho,
+                            double* gmu, double* glambda, int on_device)
+{
+   EWCHK(h);
+   return h->ew->gradient(nsteps, ckpt, grad_traces, grho, gmu, glambda, on_device != 0, true);
+}
+
 int sw4hip_ew_gradient_plan(sw4hip_ew* h, int nsteps, int ckpt, int* used_ckpt, double* device_bytes)
 {
    EWCHK(h);
@@ -1054,6 +1061,14 @@ int sw4hip_batch_gradient(sw4hip_batch* h, int nsteps, int ckpt, const double* c
 {
    if (!h || !h->b) return set_error(E_INVAL, "null batch");
    return h->b->gradient(nsteps, ckpt, grad_traces, grho, gmu, glambda, on_device != 0, max_concurrent);
+}
+
+int sw4hip_batch_gradient_comp(sw4hip_batch* h, int nsteps, int ckpt, const double* const* grad_traces,
+                               double* const* grho, double* const* gmu, double* const* glambda, int on_device,
+                               int max_concurrent)
+{
+   if (!h || !h->b) return set_error(E_INVAL, "null batch");
+   return h->b->gradient(nsteps, ckpt, grad_traces, grho, gmu, glambda, on_device != 0, max_concurrent, true);
 }
 
 int sw4hip_batch_step(sw4hip_batch* h, int nsteps, int use_graph)

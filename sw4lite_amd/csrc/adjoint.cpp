@@ -209,11 +209,15 @@ static double adjoint_fixed_doubles(const Box& b, int nsteps, int nrec)
 }
 
 // States held by a checkpoint interval c: the segment buffer (c + 2) and two
-// per checkpoint.  The reverse sweep recomputes every segment but the last,
-// nsteps - (length of the last segment) forward steps in all.
-static double adjoint_states(int nsteps, int c) { return (c + 2) + 2.0 * ((nsteps + c - 1) / c); }
+// per checkpoint (U, Um), or 1.5 for the float (U, C, D) of the FP32 form.
+// The reverse sweep recomputes every segment but the last, nsteps - (length
+// of the last segment) forward steps in all.
+static double adjoint_states(int nsteps, int c, bool fp32)
+{
+   return (c + 2) + (fp32 ? 1.5 : 2.0) * ((nsteps + c - 1) / c);
+}
 
-int EW::gradient_plan(int nsteps, int ckpt, int* used, double* bytes)
+int EW::gradient_plan(int nsteps, int ckpt, int* used, double* bytes, bool fp32)
 {
    if (nsteps < 1) return set_error(E_INVAL, "EW: gradient plan needs nsteps >= 1");
    const int nrec = (int)m_rec.size() / 4;
@@ -231,18 +235,20 @@ int EW::gradient_plan(int nsteps, int ckpt, int* used, double* bytes)
       const double room = ((double)freeb + workspace_bytes() - reserve) / 8.0 - fixed;
       c = 0;
       for (int q = nsteps; q >= 1 && room > 0; q--)
-         if (adjoint_states(nsteps, q) * n3 <= room) {
+         if (adjoint_states(nsteps, q, fp32) * n3 <= room) {
             c = q;
             break;
          }
       if (c < 1) c = std::max(1, (int)std::lround(std::sqrt((double)nsteps)));
    }
    if (used) *used = c;
-   if (bytes) *bytes = 8.0 * (fixed + adjoint_states(nsteps, c) * n3);
+   // the FP32 form's own state (18 floats a point) is held outside the pool
+   if (bytes) *bytes = 8.0 * (fixed + adjoint_states(nsteps, c, fp32) * n3) + (fp32 && !m_cbuf ? 72.0 * m_box.npts : 0.0);
    return OK;
 }
 
-int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* gmu, double* gla, bool on_device)
+int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* gmu, double* gla, bool on_device,
+                 bool fp32)
 {
    if (!m_setup_done) return set_error(E_STATE, "EW: gradient before setup");
    if (!m_opt.corder) return set_error(E_INVAL, "EW: the adjoint needs the component-blocked layout (corder=1)");
@@ -255,7 +261,7 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
       return set_error(E_INVAL, "EW: gradient needs receivers and 1 <= nsteps <= the set-up step count");
    if (!gtr || !grho || !gmu || !gla) return set_error(E_INVAL, "EW: null gradient argument");
    {
-      int rc0 = gradient_plan(nsteps, ckpt, &ckpt, nullptr);
+      int rc0 = gradient_plan(nsteps, ckpt, &ckpt, nullptr, fp32);
       if (rc0) return rc0;
    }
    const long n1 = m_box.npts, n3 = 3 * n1;
@@ -279,7 +285,11 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
    for (int q = 0; q < 3 && rc == OK; q++) rc = grab(n3, &lam[q]);
    for (int q = 0; q < 3 && rc == OK; q++) rc = grab(n1, &gr[q]);
    for (int q = 0; q <= ckpt + 1 && rc == OK; q++) rc = grab(n3, &seg[q]);
-   for (int q = 0; q < 2 * ncp && rc == OK; q++) rc = grab(n3, &cp[q]);
+   // checkpoints: (U^{s0}, U^{s0-1}) in FP64, or the float (U, C, D) of the
+   // FP32 form (cp[2 q], 9 floats a point) at the start of segment q
+   for (int q = 0; q < 2 * ncp && rc == OK; q++)
+      if (!fp32) rc = grab(n3, &cp[q]);
+      else if (q % 2 == 0) rc = grab((size_t)(4.5 * n1) + 1, &cp[q]);
    if (rc == OK) rc = grab(3 * m_box.nij, &plane);
    // receiver traces as adjoint sources, grouped by the grid value they act
    // on (component offsets fit an int for the grids this path is meant for)
@@ -352,14 +362,61 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
       mUp2.dev = sUp2;
       return e;
    };
+   // the FP32 form: the float state (U, C, D) is the checkpoint; each
+   // recomputed step's U + C is the FP64 state the reverse step reads
+   const size_t fbytes = sizeof(float) * n3;
+   auto save_state32 = [&](int cq) {
+      const CompState cs = comp_state();
+      float* dst = reinterpret_cast<float*>(cp[2 * cq]);
+      int e;
+      if ((e = check_hip(hipMemcpyAsync(dst, cs.U, fbytes, hipMemcpyDeviceToDevice, m_stream), "checkpoint")) ||
+          (e = check_hip(hipMemcpyAsync(dst + n3, cs.C, fbytes, hipMemcpyDeviceToDevice, m_stream), "checkpoint")))
+         return e;
+      return check_hip(hipMemcpyAsync(dst + 2 * n3, cs.D, fbytes, hipMemcpyDeviceToDevice, m_stream), "checkpoint");
+   };
+   auto fill_segment32 = [&](int cq) {
+      const int s0 = cq * ckpt, s1 = std::min(nsteps, s0 + ckpt);
+      m_cd = 0;
+      const CompState cs = comp_state();
+      const float* src = reinterpret_cast<const float*>(cp[2 * cq]);
+      int e;
+      if ((e = check_hip(hipMemcpyAsync(cs.U, src, fbytes, hipMemcpyDeviceToDevice, m_stream), "checkpoint restore")) ||
+          (e = check_hip(hipMemcpyAsync(cs.C, src + n3, fbytes, hipMemcpyDeviceToDevice, m_stream),
+                         "checkpoint restore")) ||
+          (e = check_hip(hipMemcpyAsync(cs.D, src + 2 * n3, fbytes, hipMemcpyDeviceToDevice, m_stream),
+                         "checkpoint restore")) ||
+          (e = launch_comp_flush(n3, cs.U, cs.C, cs.D, seg[1], seg[0], m_stream)))
+         return e;
+      mT = s0 * mDt;
+      mCycle = s0;
+      for (int m = 1; m <= s1 - s0 && e == OK; m++) {
+         if ((e = step_comp(1))) break;
+         const CompState c2 = comp_state();
+         e = launch_comp_flush(n3, c2.U, c2.C, c2.D, seg[m + 1], nullptr, m_stream);
+      }
+      return e;
+   };
    if ((rc = reset())) {
       release();
       return rc;
    }
-   rc = check_hip(hipMemcpyAsync(cp[0], mU.dev, sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream), "checkpoint");
-   if (rc == OK)
-      rc = check_hip(hipMemcpyAsync(cp[1], mUm.dev, sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream), "checkpoint");
+   if (fp32) {
+      m_norecord = true;
+      rc = comp_begin();
+      if (rc == OK) rc = save_state32(0);
+   } else {
+      rc = check_hip(hipMemcpyAsync(cp[0], mU.dev, sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream),
+                     "checkpoint");
+      if (rc == OK)
+         rc = check_hip(hipMemcpyAsync(cp[1], mUm.dev, sizeof(double) * n3, hipMemcpyDeviceToDevice, m_stream),
+                        "checkpoint");
+   }
    for (int cq = 0; cq < ncp && rc == OK; cq++) {
+      if (fp32) {
+         if ((rc = fill_segment32(cq))) break;
+         if (cq + 1 < ncp && (rc = save_state32(cq + 1))) break;
+         continue;
+      }
       if ((rc = fill_segment(cq))) break;
       if (cq + 1 < ncp) {
          const int len = std::min(nsteps, (cq + 1) * ckpt) - cq * ckpt;  // U^{s1} is seg[len+1]
@@ -385,7 +442,7 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
    if (rc == OK) rc = add_direct(nsteps - 1, lb);
    for (int cq = ncp - 1; cq >= 0 && rc == OK; cq--) {
       const int s0 = cq * ckpt, s1 = std::min(nsteps, s0 + ckpt);
-      if (cq < ncp - 1 && (rc = fill_segment(cq))) break;  // the last one is still in seg
+      if (cq < ncp - 1 && (rc = fp32 ? fill_segment32(cq) : fill_segment(cq))) break;  // the last one is still in seg
       for (int n = s1 - 1; n >= s0 && rc == OK; n--) {
          // lc: the adjoint of U^{n-1} starts here, then takes its direct source
          if ((rc = step_vjp(n, seg[n - s0 + 1], seg[n - s0], seg[n - s0 + 2], la, lb, lc, gr[0], gr[1], gr[2], tmp,
@@ -404,6 +461,15 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
    for (int q = 0; q < 3 && rc == OK; q++)
       rc = check_hip(hipMemcpyAsync(outs[q], gr[q], sizeof(double) * n1, kind, m_stream), "gradient copy");
    const int rs = check_hip(hipStreamSynchronize(m_stream), "gradient synchronize");
+   if (fp32) {
+      // the float state was the gradient's work: the solution stays the
+      // zero data of reset() (U, Um untouched), at t = 0
+      m_comp = false;
+      m_norecord = false;
+      m_cd = 0;
+      mT = 0;
+      mCycle = 0;
+   }
    release();
    return rc ? rc : rs;
 }

@@ -354,7 +354,7 @@ int Batch::tune(int* best, double* ms, int ncand, const int* cand)
 }
 
 int Batch::gradient(int nsteps, int ckpt, const double* const* gtr, double* const* grho, double* const* gmu,
-                    double* const* gla, bool on_device, int max_concurrent)
+                    double* const* gla, bool on_device, int max_concurrent, bool fp32)
 {
    const int n = (int)m_shots.size();
    if (!gtr || !grho || !gmu || !gla) return set_error(E_INVAL, "batch gradient: null argument");
@@ -369,7 +369,7 @@ int Batch::gradient(int nsteps, int ckpt, const double* const* gtr, double* cons
    if (ckpt < 1) ckpt = std::max(1, (int)std::lround(std::sqrt((double)nsteps)));
    std::vector<double> need(n, 0.0);
    for (int q = 0; q < n; q++)
-      if ((rc = m_shots[q]->gradient_plan(nsteps, ckpt, nullptr, &need[q]))) return rc;
+      if ((rc = m_shots[q]->gradient_plan(nsteps, ckpt, nullptr, &need[q], fp32))) return rc;
    size_t freeb = 0, totalb = 0;
    if ((rc = check_hip(hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo"))) return rc;
    const double budget = (double)freeb - 4.0 * (1 << 30) - 0.02 * (double)totalb;
@@ -405,7 +405,7 @@ int Batch::gradient(int nsteps, int ckpt, const double* const* gtr, double* cons
          EW* e = m_shots[q];
          const hipStream_t saved = e->m_stream;
          e->m_stream = st;
-         int r = e->gradient(nsteps, ckpt, gtr[q], grho[q], gmu[q], gla[q], on_device);
+         int r = e->gradient(nsteps, ckpt, gtr[q], grho[q], gmu[q], gla[q], on_device, fp32);
          const int r2 = e->release_workspace();
          e->m_stream = saved;
          if (r == OK) r = r2;

@@ -34,7 +34,7 @@ public:
    // a shot starts when its work memory (EW::gradient_plan) fits beside the
    // shots running.  Each shot's work memory is released when it is done.
    int gradient(int nsteps, int ckpt, const double* const* gtr, double* const* grho, double* const* gmu,
-                double* const* gla, bool on_device, int max_concurrent);
+                double* const* gla, bool on_device, int max_concurrent, bool fp32 = false);
 
 private:
    int refresh();  // base pointers and clocks from the shots, tables

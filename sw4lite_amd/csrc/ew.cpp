@@ -830,7 +830,7 @@ int EW::step_comp(int n, double* stage_ms)
                                mLambda.dev, c.strx, c.stry, m_stream, zfill)) ||
           (rc = mark(s, 7)))
          break;
-      if (nrec > 0 && m_drecdata && m_nsamples < m_maxsamples) {
+      if (!m_norecord && nrec > 0 && m_drecdata && m_nsamples < m_maxsamples) {
          if ((rc = launch_comp_record(m_box, nrec, m_drec, cs, dt, m_drecdata + (size_t)3 * nrec * m_nsamples,
                                       m_stream)))
             break;

@@ -121,10 +121,15 @@ public:
    // ckpt steps, then the reverse sweep.  gtr: host array
    // [nsteps+1][nreceivers][3], the misfit's derivative with respect to each
    // recorded sample.  Outputs are overwritten (device pointers if on_device).
-   int gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* gmu, double* gla, bool on_device);
+   // fp32: the forward run and its recomputation in the FP32 form with the
+   // compensated update (step_comp), checkpoints of its float state (36
+   // bytes a point instead of 48); the reverse steps in FP64 on that
+   // trajectory.  The gradient of the FP32 run's misfit.
+   int gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* gmu, double* gla, bool on_device,
+                bool fp32 = false);
    // checkpoint interval the gradient uses for ckpt (< 1: from free device
    // memory and the workspace already held) and the device bytes it needs
-   int gradient_plan(int nsteps, int ckpt, int* used, double* bytes);
+   int gradient_plan(int nsteps, int ckpt, int* used, double* bytes, bool fp32 = false);
    // The gradient's device buffers stay allocated between calls (a pool of
    // exact-size buffers, grown as needed) so that repeated gradients skip
    // allocating hundreds of GB; this frees them.
@@ -214,6 +219,7 @@ private:
    SbpTablesT<float>* m_dsbpf = nullptr;
    int m_cd = 0;             // D0 (0) or D1 (1) holds u^n - u^{n-1}
    bool m_comp = false;      // the float state holds the solution
+   bool m_norecord = false;  // steps record no receiver samples (the gradient's recomputation)
    bool m_setup_done = false, m_material_set = false, m_curv = false;
    std::vector<void*> m_allocs;
 };

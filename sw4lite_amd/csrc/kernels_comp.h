@@ -36,7 +36,7 @@ struct CompState {
    const float* rho;
 };
 
-// double (u, um) -> (U, C, D); (U, C, D) -> double (u, um)
+// double (u, um) -> (U, C, D); (U, C, D) -> double (u, um) (um may be null)
 int launch_comp_init(long n3, const double* u, const double* um, float* U, float* C, float* D, hipStream_t s);
 int launch_comp_flush(long n3, const float* U, const float* C, const float* D, double* u, double* um,
                       hipStream_t s);

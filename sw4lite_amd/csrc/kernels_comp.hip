@@ -889,7 +889,7 @@ __global__ void k_comp_flush(long n, const float* __restrict__ U, const float* _
    if (q >= n) return;
    const double a = (double)U[q] + (double)C[q];
    u[q] = a;
-   um[q] = a - (double)D[q];
+   if (um) um[q] = a - (double)D[q];
 }
 
 __global__ void k_to_float(long n, const double* __restrict__ src, float* __restrict__ dst)

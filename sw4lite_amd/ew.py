@@ -411,12 +411,15 @@ class EW:
         """Zero initial data at t = 0; receivers restart with the t = 0 sample."""
         L.check(self.lib.sw4hip_ew_reset(self.h))
 
-    def gradient(self, grad_traces, nsteps=None, ckpt=0, out=None):
+    def gradient(self, grad_traces, nsteps=None, ckpt=0, out=None, fp32=False):
         """Gradient of a receiver misfit with respect to (rho, mu, lambda) by
         the discrete adjoint of the fused FP64 step.  grad_traces: the
         misfit's derivative with respect to every recorded sample, array
         [nsteps+1, nreceivers, 3].  Runs the forward from zero data itself.
-        out: three device tensors to fill (else numpy arrays are returned)."""
+        out: three device tensors to fill (else numpy arrays are returned).
+        fp32: the forward and its recomputation in the FP32 form (step mode
+        2) with float checkpoints, the reverse in FP64: the gradient of the
+        FP32 run's misfit (sw4hip_ew_gradient_comp)."""
         g = np.ascontiguousarray(np.asarray(grad_traces, dtype=np.float64))
         if nsteps is None:
             nsteps = g.shape[0] - 1
@@ -429,8 +432,8 @@ class EW:
         else:
             res = list(out)
             on_dev = int(_fields(self.npoints(), *res))
-        L.check(self.lib.sw4hip_ew_gradient(self.h, int(nsteps), int(ckpt), _ptr(g), _ptr(res[0]), _ptr(res[1]),
-                                            _ptr(res[2]), on_dev))
+        fn = self.lib.sw4hip_ew_gradient_comp if fp32 else self.lib.sw4hip_ew_gradient
+        L.check(fn(self.h, int(nsteps), int(ckpt), _ptr(g), _ptr(res[0]), _ptr(res[1]), _ptr(res[2]), on_dev))
         return tuple(res)
 
     def gradient_plan(self, nsteps, ckpt=0):
@@ -488,12 +491,12 @@ class Batch:
     def step(self, nsteps, graph=True):
         L.check(self.lib.sw4hip_batch_step(self.h, int(nsteps), int(bool(graph))))
 
-    def gradient(self, grad_traces, nsteps, ckpt=0, out=None, max_concurrent=4):
+    def gradient(self, grad_traces, nsteps, ckpt=0, out=None, max_concurrent=4, fp32=False):
         """Gradients of every shot's receiver misfit (EW.gradient per shot,
         run concurrently on streams of their own).  grad_traces: one array
         [nsteps+1, nreceivers, 3] per shot.  out: per shot three float64
         device tensors of the shot's point count (else numpy arrays are
-        returned).  Returns a list of (grho, gmu, gla)."""
+        returned).  fp32: as EW.gradient.  Returns a list of (grho, gmu, gla)."""
         n = len(self.shots)
         if len(grad_traces) != n:
             raise ValueError("one trace array per shot")
@@ -510,11 +513,10 @@ class Batch:
             if not on_dev:
                 raise ValueError("out must hold device tensors")
         arr = lambda xs: (C.c_void_p * n)(*[_ptr(x).value for x in xs])  # noqa: E731
-        L.check(self.lib.sw4hip_batch_gradient(self.h, int(nsteps), int(ckpt), C.cast(arr(gs), C.c_void_p),
-                                               C.cast(arr([r[0] for r in res]), C.c_void_p),
-                                               C.cast(arr([r[1] for r in res]), C.c_void_p),
-                                               C.cast(arr([r[2] for r in res]), C.c_void_p), on_dev,
-                                               int(max_concurrent)))
+        fn = self.lib.sw4hip_batch_gradient_comp if fp32 else self.lib.sw4hip_batch_gradient
+        L.check(fn(self.h, int(nsteps), int(ckpt), C.cast(arr(gs), C.c_void_p),
+                   C.cast(arr([r[0] for r in res]), C.c_void_p), C.cast(arr([r[1] for r in res]), C.c_void_p),
+                   C.cast(arr([r[2] for r in res]), C.c_void_p), on_dev, int(max_concurrent)))
         return res
 
     def close(self):

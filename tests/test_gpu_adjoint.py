@@ -21,6 +21,10 @@ from test_gpu_parity import SOURCES, layered_material
 pytestmark = pytest.mark.gpu
 DOT_TOL = 1e-13
 FD_TOL = 1e-4
+# FP32 form: central differences of the FP32 run's misfit with a relative
+# step of 1e-3 carry its rounding (about 1e-7 of the misfit) over a change of
+# about 1e-3 of it, and the float material's rounding: 1e-4 of the slope
+FD32_TOL = 2e-3
 
 
 @pytest.fixture(scope="module")
@@ -211,10 +215,10 @@ def make_ew(nx, ny, nz, h, bc, nsteps, seed=5):
     return e, (rho, mu, la)
 
 
-def misfit(e, mat, nsteps, dobs):
+def misfit(e, mat, nsteps, dobs, mode=0):
     e.set_material(*mat)
     e.reset()
-    e.timesteploop(nsteps, mode=0)
+    e.timesteploop(nsteps, mode=mode)
     tr = e.receivers()[: nsteps + 1]
     return 0.5 * float(np.sum((tr - dobs) ** 2)), tr
 
@@ -388,3 +392,84 @@ def test_gradient_damping_layers_bitwise(torch, bc):
     print("gradient, damping transpose over the layers: bc", bc, "points differing from variant 8:", ndiff,
           flush=True)
     assert np.max(np.abs(res[0][0])) > 0 and ndiff == [0, 0, 0]
+
+
+def smooth_direction(rng, mat, shape):
+    """A random direction of (rho, mu, lambda), each a relative perturbation
+    made of a few random low-order cosine modes over the box."""
+    nk, nj, ni = shape
+    z, y, x = np.meshgrid(np.linspace(0, 1, nk), np.linspace(0, 1, nj), np.linspace(0, 1, ni), indexing="ij")
+    out = []
+    for m in mat:
+        f = np.full(shape, rng.uniform(-1, 1))
+        for _ in range(4):
+            a, b, c = rng.integers(0, 3, size=3)
+            f += rng.uniform(-1, 1) * np.cos(np.pi * (a * x + rng.uniform())) * np.cos(np.pi * (b * y + rng.uniform())) \
+                * np.cos(np.pi * (c * z + rng.uniform()))
+        out.append(f.ravel() * m)
+    return out
+
+
+def test_gradient_16_directions_fp64_and_fp32(torch):
+    """The gradient against central finite differences of the misfit along
+    16 random smooth directions (rho, mu, lambda perturbed together), for the
+    FP64 step and for the FP32 form (sw4hip_ew_gradient_comp: the FP32 run
+    recomputed from float checkpoints, the reverse in FP64).  The FP32
+    gradient also stays near the FP64 gradient of the same traces'
+    derivative, though not at the trajectories' 1e-7: the density term
+    weighs the corrector's small state differences U^{n+1} - Up (about
+    (w dt)^4/12 of the state), in which the float rounding of the
+    trajectory is a part in 1e3 (measured 1.4e-2 relative L2 for rho, 3e-3
+    for mu and 6e-3 for lambda)."""
+    nx, ny, nz, h, nsteps = 24, 22, 20, 0.1, 30
+    e, mat = make_ew(nx, ny, nz, h, [2, 2, 2, 2, 0, 2], nsteps)
+    rng = np.random.default_rng(61)
+    shape = (nz + 4, ny + 4, nx + 4)
+    grads = {}
+    for mode, fp32, eps, tol in ((0, False, 1e-6, FD_TOL), (2, True, 1e-3, FD32_TOL)):
+        J0, tr = misfit(e, mat, nsteps, 0.0, mode)
+        dobs = 0.7 * tr + 0.1 * np.max(np.abs(tr)) * rng.standard_normal(tr.shape)
+        J0, tr = misfit(e, mat, nsteps, dobs, mode)
+        g = e.gradient(tr - dobs, nsteps, ckpt=7, fp32=fp32)
+        grads[fp32] = (g, tr - dobs)
+        errs = []
+        for q in range(16):
+            d = smooth_direction(rng, mat, shape)
+            jp, _ = misfit(e, [m + eps * dd for m, dd in zip(mat, d)], nsteps, dobs, mode)
+            jm, _ = misfit(e, [m - eps * dd for m, dd in zip(mat, d)], nsteps, dobs, mode)
+            fd = (jp - jm) / (2 * eps)
+            ad = sum(float(np.dot(gq, dq)) for gq, dq in zip(g, d))
+            errs.append(dot_rel(fd, ad))
+        print("gradient, 16 directions,", "FP32 form" if fp32 else "FP64", ": worst relative difference",
+              max(errs), "median", float(np.median(errs)), flush=True)
+        assert max(errs) < tol
+    # the FP64 gradient of the FP32 run's trace derivative, against the FP32 one
+    g32, gtr32 = grads[True]
+    g64 = e.gradient(gtr32, nsteps, ckpt=7)
+    l2 = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(g32, g64)]
+    print("FP32-form gradient against the FP64 gradient of the same trace derivative: relative L2 (rho, mu, "
+          "lambda)", l2, flush=True)
+    assert max(l2) < 5e-2
+    e.close()
+
+
+def test_batch_gradient_fp32_matches_single_shots(torch):
+    """sw4hip_batch_gradient_comp: every shot's gradient must be its own
+    sw4hip_ew_gradient_comp to the last bit, with shots in flight together."""
+    from sw4lite_amd import Batch
+    nsteps = 12
+    grids = [(24, 22, 20), (20, 18, 16)]
+    rng = np.random.default_rng(67)
+    shots = [make_ew(nx, ny, nz, 0.1, [2, 2, 2, 2, 0, 2], nsteps)[0] for nx, ny, nz in grids]
+    gtrs = [rng.standard_normal((nsteps + 1, len(RECS), 3)) for _ in shots]
+    single = [e.gradient(g, nsteps, ckpt=5, fp32=True) for e, g in zip(shots, gtrs)]
+    b = Batch(shots)
+    got = b.gradient(gtrs, nsteps, ckpt=5, max_concurrent=2, fp32=True)
+    for q, (a, w) in enumerate(zip(got, single)):
+        ndiff = [int(np.count_nonzero(x != y)) for x, y in zip(a, w)]
+        print("batch FP32-form gradient, shot", q, grids[q], "points differing from the single shot:", ndiff,
+              flush=True)
+        assert np.max(np.abs(w[0])) > 0 and ndiff == [0, 0, 0]
+    b.close()
+    for e in shots:
+        e.close()
