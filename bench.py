@@ -592,6 +592,35 @@ def main_adjoint(a, torch):
         fixed = {"checkpoint_every": c50, "device_bytes": e.gradient_plan(steps, c50)[1], "gradient_s": t50,
                  "ratio": t50 / t_fwd,
                  "max_rel_diff_vs_first": max(float((x - y).abs().max() / y.abs().max()) for x, y in zip(out, g_auto))}
+    fp32 = {}
+    if not a.no_fp32:
+        # the FP32 form: its forward run, and its gradient (forward and
+        # recomputation in FP32 from float checkpoints, reverse in FP64)
+        e.release_workspace()
+        c32 = a.ckpt if a.ckpt > 0 else min(50, steps)
+        e.reset()
+        e.timesteploop(min(5, steps), mode=2)
+        e.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e.reset()
+        e.timesteploop(steps, mode=2)
+        e.sync()
+        t_fwd32 = time.perf_counter() - t0
+        tr32 = e.receivers()
+
+        def timed_gradient32():
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            e.gradient(tr32 - dobs, steps, ckpt=c32, out=out, fp32=True)
+            torch.cuda.synchronize()
+            return time.perf_counter() - t1
+
+        timed_gradient32()
+        t_g32 = timed_gradient32()
+        fp32 = {"checkpoint_every": c32, "forward_s": t_fwd32, "gradient_s": t_g32, "ratio": t_g32 / t_fwd32,
+                "gradient_over_fp64_forward": t_g32 / t_fwd,
+                "trace_rel_l2_vs_fp64": float(np.linalg.norm(tr32 - tr) / np.linalg.norm(tr))}
     e.close()
     pts = float(n) ** 3
     line = {
@@ -606,6 +635,7 @@ def main_adjoint(a, torch):
             "gradient_device_bytes": nbytes, "forward_s": t_fwd, "gradient_s": t_grad,
             "gradient_first_call_s": t_first,
             "fixed_checkpoints": fixed,
+            "fp32_form": fp32,
             "forward_grid_pt_updates_per_s": pts * steps / t_fwd,
             "max_abs_gradient": {"rho": gn[0], "mu": gn[1], "lambda": gn[2]},
         },
@@ -754,6 +784,20 @@ def main_batch_adjoint(a, torch):
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
     gmax = [max(float(np.max(np.abs(g[c]))) for g in grads) for c in range(3)]
+    del grads
+    fp32 = {}
+    if not a.no_fp32:
+        # the FP32 form's gradients of the same shots (forward and its
+        # recomputation in FP32 from float checkpoints, reverse in FP64),
+        # against the FP64 batched forward
+        for e in shots:
+            e.release_workspace()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.gradient(gtr, steps, ckpt=ckpt, max_concurrent=a.concurrent, fp32=True)
+        torch.cuda.synchronize()
+        t_g32 = time.perf_counter() - t0
+        fp32 = {"gradient_s": t_g32, "gradient_over_fp64_forward": t_g32 / t_fwd}
     b.close()
     for e in shots:
         e.close()
@@ -770,6 +814,7 @@ def main_batch_adjoint(a, torch):
             "forward_s": t_fwd, "gradient_s": t_grad, "misfit": J,
             "forward_grid_pt_updates_per_s": pts * steps / t_fwd,
             "max_abs_gradient": {"rho": gmax[0], "mu": gmax[1], "lambda": gmax[2]},
+            "fp32_form": fp32,
         },
     }
     print(json.dumps(line), flush=True)
