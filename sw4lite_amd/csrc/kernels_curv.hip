@@ -307,8 +307,11 @@ __device__ __forceinline__ void curv_point(const S& s, int i, int j, int k, cons
    }
 }
 
+// Component-blocked layout: capped at 256 registers for 2 waves/SIMD (2-4
+// spilled; 262 registers and 1 wave/SIMD without the cap).  The interleaved
+// layout's index arithmetic would spill ~120 registers under the cap.
 template <int CO, bool BND>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, CO ? 2 : 1)
 k_rhs4curv(Box b, int klo, int khi, const SbpTables* __restrict__ sbp, CurvG<CO> s,
            const double* __restrict__ jac, double* __restrict__ lu)
 {
