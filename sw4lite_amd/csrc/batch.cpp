@@ -82,6 +82,31 @@ int Batch::init()
    m_clk_dt = m_clk_t + n;
    m_clk_s = reinterpret_cast<int*>(m_clk_dt + n);
    m_clk_max = m_clk_s + n;
+   // kernel variant 9: each shot's damping sweep runs over the tiles that
+   // reach its supergrid layers (damp_tiles; zero damping elsewhere), in
+   // chunks of the single-grid damping sweep's planes
+   m_dtiles.assign(n, nullptr);
+   m_ndtiles.assign(n, 0);
+   for (int q = 0; q < n; q++) {
+      EW* e = m_shots[q];
+      if (e->m_variant < 9) continue;
+      const Box& b = e->m_box;
+      const GridSetup& g = e->m_opt.grid;
+      const int k1 = e->m_onesided[4] == 1 ? 7 : b.kb + 2;
+      const int k2 = e->m_onesided[5] == 1 ? g.nz - 6 : b.ke - 2;
+      const std::vector<DampTile> v = damp_tiles(b, k1, k2, EW::kDampChunk, e->m_sg_host.dcx.data(),
+                                                 e->m_sg_host.dcy.data(), e->m_sg_host.dcz.data());
+      if (hipMalloc(&p, sizeof(DampTile) * std::max<size_t>(v.size(), 1)) != hipSuccess) {
+         (void)hipGetLastError();
+         return set_error(E_NOMEM, "batch: damping tile list allocation failed");
+      }
+      m_allocs.push_back(p);
+      if (!v.empty() &&
+          (rc = check_hip(hipMemcpy(p, v.data(), sizeof(DampTile) * v.size(), hipMemcpyHostToDevice), "tile upload")))
+         return rc;
+      m_dtiles[q] = static_cast<DampTile*>(p);
+      m_ndtiles[q] = (int)v.size();
+   }
    return refresh();
 }
 
@@ -154,7 +179,8 @@ int Batch::build_tables()
                                        e->mRho.dev, c, up2, g.h, e->mDt, e->m_opt.beta, true, gc, d);
                else
                   gc = fill_sweep_shot(true, b, k1, damp ? k2 : k1 - 1, m_kchunk, nullptr, u, um, nullptr, nullptr,
-                                       e->mRho.dev, c, up2, g.h, e->mDt, e->m_opt.beta, true, gc, d);
+                                       e->mRho.dev, c, up2, g.h, e->mDt, e->m_opt.beta, true, gc, d, m_dtiles[q],
+                                       m_ndtiles[q]);
             }
             if (pass == 0)
                for (int k = 0; k < 3; k++) m_nasync[ph][k] = slot[k];

@@ -60,6 +60,8 @@ private:
    int m_kchunk = 64;
    hipGraphExec_t m_exec = nullptr;
    std::vector<void*> m_allocs;
+   std::vector<DampTile*> m_dtiles;  // per shot: its damping tile list (kernel variant 9), or null
+   std::vector<int> m_ndtiles;
    std::vector<hipStream_t> m_saved;
    // the per-grid boundary work of the shots runs on NSIDE streams (default
    // priority) forked from and joined back into m_s, so the small kernels of

@@ -12,9 +12,11 @@ struct TileMap {
 
 // One grid of a batched sweep: the arguments of its single-grid launch and
 // its first block id in the batched launch.
+struct DampTile;
 struct SweepShot {
    Box b;
    TileMap map;
+   const DampTile* tiles;  // damping: the tile list (map.ntiles entries), or null for the whole map
    int kbeg, kend, kc, grid0;
    const double* upin;
    const double* u;
@@ -27,11 +29,13 @@ struct SweepShot {
    double cof, dt2i, cupd, beta;
 };
 // Fill sh for planes k1..k2 of one grid (damping: the 32 x 8 damping sweep,
-// out = the array it updates; else the predictor (corr false) or corrector
-// sweep of variant 6); returns the next free block id.
+// out = the array it updates, over the tiles of damp_tiles when tiles is
+// given; else the predictor (corr false) or corrector sweep of variant 6);
+// returns the next free block id.
 int fill_sweep_shot(bool damping, const Box& b, int k1, int k2, int kc, const double* upin, const double* u,
                     const double* um, const double* mu, const double* la, const double* rho, const SgCoef& sg,
-                    double* out, double h, double dt, double beta, bool corr, int grid0, SweepShot* sh);
+                    double* out, double h, double dt, double beta, bool corr, int grid0, SweepShot* sh,
+                    const DampTile* tiles = nullptr, int ntiles = 0);
 // kind 0 predictor, 1 corrector, 2 damping sweeps of nshots grids in one
 // launch of nblocks workgroups (d_shots in device memory).
 int launch_sweep_batch(int kind, const SweepShot* d_shots, int nshots, int nblocks, hipStream_t s);

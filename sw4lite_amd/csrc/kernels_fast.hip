@@ -574,27 +574,21 @@ static bool dma_fits(const Box& b, const double* u, const double* mu, const doub
 // own column of d for k-2..k+2 and rho at k+-1 live in registers.  Small LDS
 // and register footprint, so several workgroups share a CU and hide latency.
 // Tiles of 32 x DTY columns (DTY * 32 threads); taller tiles re-read less halo.
+// Column tile (tix, tjy), planes k0..k1.
 template <int DTY>
-__device__ __forceinline__ void damp_lds_body(unsigned bid, Box b, TileMap map, int kbeg, int kend, int kc,
-                                              double* __restrict__ up, const double* __restrict__ u,
-                                              const double* __restrict__ um, const double* __restrict__ rho,
-                                              SgCoef sg, double beta)
+__device__ __forceinline__ void damp_lds_tile(int tix, int tjy, int k0, int k1, Box b, double* __restrict__ up,
+                                              const double* __restrict__ u, const double* __restrict__ um,
+                                              const double* __restrict__ rho, SgCoef sg, double beta)
 {
    constexpr int TY = DTY, HY = TY + 4, NT = TX * TY, PL = HX * HY;
    constexpr int NU = (3 * PL + NT - 1) / NT;
    constexpr int NM = (PL + NT - 1) / NT;
    __shared__ double sdp[3 * PL];
    __shared__ double srp[PL];
-   const int per = (int)((map.ntiles + 7) / 8);
-   const int t = (int)(bid % 8) * per + (int)(bid / 8);
-   if (t >= map.ntiles) return;
-   const int tix = t % map.ntx, tr = t / map.ntx, tjy = tr % map.nty, tkz = tr / map.nty;
    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
    const int ia = b.ib + tix * TX, ja = b.jb + tjy * TY;
    const int i = ia + 2 + tx, j = ja + 2 + ty;
    const bool active = i <= b.ie - 2 && j <= b.je - 2;
-   const int k0 = kbeg + tkz * kc;
-   const int k1 = min(kend, k0 + kc - 1);
    const long ni = b.ni, nij = b.nij, npts = b.npts;
    const long col = active ? (i - b.ib) + ni * (j - b.jb) : 0;
    const int o = (ty + 2) * HX + tx + 2;
@@ -685,6 +679,20 @@ __device__ __forceinline__ void damp_lds_body(unsigned bid, Box b, TileMap map, 
          for (int c = 0; c < 3; c++) dq[q][c] = dq[q + 1][c];
       rkm = r0;
    }
+}
+
+template <int DTY>
+__device__ __forceinline__ void damp_lds_body(unsigned bid, Box b, TileMap map, int kbeg, int kend, int kc,
+                                              double* __restrict__ up, const double* __restrict__ u,
+                                              const double* __restrict__ um, const double* __restrict__ rho,
+                                              SgCoef sg, double beta)
+{
+   const int per = (int)((map.ntiles + 7) / 8);
+   const int t = (int)(bid % 8) * per + (int)(bid / 8);
+   if (t >= map.ntiles) return;
+   const int tix = t % map.ntx, tr = t / map.ntx, tjy = tr % map.nty, tkz = tr / map.nty;
+   const int k0 = kbeg + tkz * kc;
+   damp_lds_tile<DTY>(tix, tjy, k0, min(kend, k0 + kc - 1), b, up, u, um, rho, sg, beta);
 }
 
 // The damping sweep with its planes brought to LDS by asynchronous
@@ -900,15 +908,43 @@ __global__ void __launch_bounds__(NT, 2) k_sweep_batch(const SweepShot* __restri
                                       h.la, h.rho, h.sg, h.out, h.cof, h.dt2i, h.cupd, h.beta);
 }
 
+// A shot's tile of the batched damping sweep: from its tile list where it
+// has one (entry t of the list, each XCD taking one contiguous run), else
+// from its tile map.  false: no tile for this block.
+__device__ __forceinline__ bool damp_batch_tile(const SweepShot& h, unsigned bid, int& tix, int& tjy, int& k0,
+                                                int& k1)
+{
+   const int per = (int)((h.map.ntiles + 7) / 8);
+   const int t = (int)(bid % 8) * per + (int)(bid / 8);
+   if (t >= h.map.ntiles) return false;
+   if (h.tiles) {
+      const DampTile d = h.tiles[t];
+      tix = d.tix;
+      tjy = d.tjy;
+      k0 = d.k0;
+      k1 = d.k1;
+   } else {
+      const int tr = t / h.map.ntx, tkz = tr / h.map.nty;
+      tix = t % h.map.ntx;
+      tjy = tr % h.map.nty;
+      k0 = h.kbeg + tkz * h.kc;
+      k1 = min(h.kend, k0 + h.kc - 1);
+   }
+   return true;
+}
+
 __global__ void __launch_bounds__(32 * 8, 4) k_damp_batch(const SweepShot* __restrict__ shots, int nshots)
 {
    const SweepShot& h = shots[find_shot(shots, nshots, blockIdx.x)];
-   damp_lds_body<8>(blockIdx.x - h.grid0, h.b, h.map, h.kbeg, h.kend, h.kc, h.out, h.u, h.um, h.rho, h.sg, h.beta);
+   int tix, tjy, k0, k1;
+   if (!damp_batch_tile(h, blockIdx.x - h.grid0, tix, tjy, k0, k1)) return;
+   damp_lds_tile<8>(tix, tjy, k0, k1, h.b, h.out, h.u, h.um, h.rho, h.sg, h.beta);
 }
 
 int fill_sweep_shot(bool damping, const Box& b, int k1, int k2, int kc, const double* upin, const double* u,
                     const double* um, const double* mu, const double* la, const double* rho, const SgCoef& sg,
-                    double* out, double h, double dt, double beta, bool corr, int grid0, SweepShot* sh)
+                    double* out, double h, double dt, double beta, bool corr, int grid0, SweepShot* sh,
+                    const DampTile* tiles, int ntiles)
 {
    const long nx = b.ie - b.ib - 3, ny = b.je - b.jb - 3;
    if (kc <= 0) kc = 64;
@@ -916,6 +952,8 @@ int fill_sweep_shot(bool damping, const Box& b, int k1, int k2, int kc, const do
    sh->map.ntx = (int)cdiv(nx, TX);
    sh->map.nty = (int)cdiv(ny, damping ? 8 : TY);
    sh->map.ntiles = k2 >= k1 ? sh->map.ntx * sh->map.nty * (int)cdiv(k2 - k1 + 1, kc) : 0;
+   sh->tiles = damping && tiles ? tiles : nullptr;
+   if (sh->tiles) sh->map.ntiles = k2 >= k1 ? ntiles : 0;
    sh->kbeg = k1;
    sh->kend = k2;
    sh->kc = kc;
@@ -947,7 +985,9 @@ __global__ void __launch_bounds__(NT, 2) k_sweep_dma_batch(const SweepShot* __re
 __global__ void __launch_bounds__(NT, 3) k_damp_dma_batch(const SweepShot* __restrict__ shots, int nshots)
 {
    const SweepShot& h = shots[find_shot(shots, nshots, blockIdx.x)];
-   damp_dma_body(blockIdx.x - h.grid0, h.b, h.map, h.kbeg, h.kend, h.kc, h.out, h.u, h.um, h.rho, h.sg, h.beta);
+   int tix, tjy, k0, k1;
+   if (!damp_batch_tile(h, blockIdx.x - h.grid0, tix, tjy, k0, k1)) return;
+   damp_dma_tile(tix, tjy, k0, k1, h.b, h.out, h.u, h.um, h.rho, h.sg, h.beta);
 }
 
 bool async_loads_fit(const Box& b, const double* const* arrays, int n)
