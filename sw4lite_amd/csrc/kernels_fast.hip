@@ -1268,6 +1268,13 @@ k_sbp_rows(Box b, int top, int nk, int ntx, const SbpTables* __restrict__ sbp, c
    if (!active) return;
    SlabSten st{su + o, sm + o, sl + o, ssx + tx + 2, ssy + ty + 2};
    const long colp = (i - b.ib) + ni * (j - b.jb);
+   // every term of the damping carries a damping coefficient at the point or
+   // a neighbour along its axis: outside the supergrid layers it subtracts
+   // exactly zero and is not evaluated
+   const int ii = i - b.ib, jj = j - b.jb;
+   const bool dxy = CORR && beta != 0 &&
+                    (sg.dcx[ii - 1] != 0 || sg.dcx[ii] != 0 || sg.dcx[ii + 1] != 0 || sg.dcy[jj - 1] != 0 ||
+                     sg.dcy[jj] != 0 || sg.dcy[jj + 1] != 0);
 #pragma unroll 1
    for (int kr = 1; kr <= 6; kr++) {
       const int k = kbase + sgn * (kr - 1);
@@ -1276,7 +1283,9 @@ k_sbp_rows(Box b, int top, int nk, int ntx, const SbpTables* __restrict__ sbp, c
       const long p = colp + nij * (k - b.kb);
       const double sc = cupd / rho[p];
       double d[3] = {0, 0, 0};
-      if (CORR && beta != 0) addsgd4_point<1>(b, i, j, k, u, um, rho, sg, beta, d);
+      const int kk = k - b.kb;
+      if (CORR && beta != 0 && (dxy || sg.dcz[kk - 1] != 0 || sg.dcz[kk] != 0 || sg.dcz[kk + 1] != 0))
+         addsgd4_point<1>(b, i, j, k, u, um, rho, sg, beta, d);
 #pragma unroll
       for (int c = 0; c < 3; c++) {
          const long q = p + npts * c;
