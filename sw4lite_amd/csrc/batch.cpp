@@ -107,6 +107,18 @@ int Batch::init()
       m_dtiles[q] = static_cast<DampTile*>(p);
       m_ndtiles[q] = (int)v.size();
    }
+   // kernel variant 8 and later: the predictor stores the acceleration in the
+   // shot's own field and the corrector stages it (where both sweeps take
+   // the asynchronous loads), as the single-grid step does
+   m_acc.assign(n, nullptr);
+   for (int q = 0; q < n; q++) {
+      EW* e = m_shots[q];
+      const Box& b = e->m_box;
+      const GridSetup& g = e->m_opt.grid;
+      const int k1 = e->m_onesided[4] == 1 ? 7 : b.kb + 2;
+      const int k2 = e->m_onesided[5] == 1 ? g.nz - 6 : b.ke - 2;
+      if (e->m_variant >= 8 && b.ni % 2 == 0 && k2 >= k1 && (rc = e->accel_field(&m_acc[q]))) return rc;
+   }
    return refresh();
 }
 
@@ -147,6 +159,7 @@ int Batch::build_tables()
 {
    const int n = (int)m_shots.size();
    std::vector<SweepShot> h(9 * n);
+   m_useacc.assign(3 * n, 0);
    for (int ph = 0; ph < 3; ph++) {
       // kinds 0 and 2: the grids whose planes take the asynchronous loads
       // first (their own launch, block ids from 0), then the others
@@ -165,7 +178,10 @@ int Batch::build_tables()
             SweepShot* t = &h[(ph * 3) * n];
             const double* pa[3] = {u, e->mMu.dev, e->mLambda.dev};
             const double* da[4] = {u, um, e->mRho.dev, up2};
-            const bool fa[3] = {async_loads_fit(b, pa, 3), false, async_loads_fit(b, da, 4)};
+            const double* ca[4] = {m_acc[q], e->mUp.dev, e->mMu.dev, e->mLambda.dev};
+            const bool acc = m_acc[q] && async_loads_fit(b, pa, 3) && async_loads_fit(b, ca, 4);
+            m_useacc[ph * n + q] = acc;
+            const bool fa[3] = {async_loads_fit(b, pa, 3), acc, async_loads_fit(b, da, 4)};
             const bool damp = e->m_opt.beta != 0;
             for (int k = 0; k < 3; k++) {
                if (fa[k] != (pass == 0)) continue;
@@ -173,10 +189,12 @@ int Batch::build_tables()
                SweepShot* d = &t[k * n + slot[k]++];
                if (k == 0)
                   gc = fill_sweep_shot(false, b, k1, k2, m_kchunk, nullptr, u, um, e->mMu.dev, e->mLambda.dev,
-                                       e->mRho.dev, c, e->mUp.dev, g.h, e->mDt, 0.0, false, gc, d);
+                                       e->mRho.dev, c, e->mUp.dev, g.h, e->mDt, 0.0, false, gc, d, nullptr, 0,
+                                       acc ? m_acc[q] : nullptr);
                else if (k == 1)
                   gc = fill_sweep_shot(false, b, k1, k2, m_kchunk, e->mUp.dev, u, um, e->mMu.dev, e->mLambda.dev,
-                                       e->mRho.dev, c, up2, g.h, e->mDt, e->m_opt.beta, true, gc, d);
+                                       e->mRho.dev, c, up2, g.h, e->mDt, e->m_opt.beta, true, gc, d, nullptr, 0,
+                                       acc ? m_acc[q] : nullptr);
                else
                   gc = fill_sweep_shot(true, b, k1, damp ? k2 : k1 - 1, m_kchunk, nullptr, u, um, nullptr, nullptr,
                                        e->mRho.dev, c, up2, g.h, e->mDt, e->m_opt.beta, true, gc, d, m_dtiles[q],
@@ -202,7 +220,7 @@ int Batch::one_step(int ph)
    const int n = (int)m_shots.size();
    const SweepShot* tab = m_dtab + (ph * 3) * n;
    int rc;
-   const int a0 = m_nasync[ph][0], a2 = m_nasync[ph][2];
+   const int a0 = m_nasync[ph][0], a1 = m_nasync[ph][1], a2 = m_nasync[ph][2];
    if ((rc = launch_sweep_batch_async(0, tab, a0, m_nblk_async[ph][0], m_s)) ||
        (rc = launch_sweep_batch(0, tab + a0, n - a0, m_nblk[ph][0], m_s)) || (rc = fork()))
       return rc;
@@ -216,13 +234,24 @@ int Batch::one_step(int ph)
       const double dt = e->mDt;
       hipStream_t st = e->m_stream;
       const int zf = e->zfill();
+      double* acc = m_useacc[ph * n + q] ? m_acc[q] : nullptr;
       if ((rc = launch_pred_tail(e->m_box, g.nz, e->m_onesided, e->m_dsbp, u, um, e->mMu.dev, e->mLambda.dev,
-                                 e->mRho.dev, c, e->mUp.dev, g.h, dt, st, zf)) ||
+                                 e->mRho.dev, c, e->mUp.dev, g.h, dt, st, zf, acc)) ||
           (rc = launch_src_apply(e->m_box, e->sources(), 0.0, 0, dt * dt, e->mRho.dev, e->mUp.dev, st, m_clk_t + q)) ||
           (rc = e->enforceBC(e->mUp, e->mMu, e->mLambda, 0.0, zf)))
          return rc;
+      if (acc) {
+         // the acceleration outside the sweep's planes and frame, and at the
+         // source points, from the final Up (as EW::step_fused)
+         int ka = 0, kz = 0;
+         fused_planes(e->m_box, g.nz, e->m_onesided, &ka, &kz);
+         if ((rc = launch_accel_fix(e->m_box, ka, kz, (zf & 15) != 15, true, e->mUp.dev, u, um, dt, acc, st)) ||
+             (rc = launch_accel_points(e->m_box, e->sources(), e->mUp.dev, u, um, dt, acc, st)))
+            return rc;
+      }
    }
-   if ((rc = join()) || (rc = launch_sweep_batch(1, tab + n, n, m_nblk[ph][1], m_s)) ||
+   if ((rc = join()) || (rc = launch_sweep_batch_async(1, tab + n, a1, m_nblk_async[ph][1], m_s)) ||
+       (rc = launch_sweep_batch(1, tab + n + a1, n - a1, m_nblk[ph][1], m_s)) ||
        (rc = launch_sweep_batch_async(2, tab + 2 * n, a2, m_nblk_async[ph][2], m_s)) ||
        (rc = launch_sweep_batch(2, tab + 2 * n + a2, n - a2, m_nblk[ph][2], m_s)) || (rc = fork()))
       return rc;
@@ -239,8 +268,9 @@ int Batch::one_step(int ph)
       Sarray v = e->mU;
       v.dev = up2;
       const int zf = e->zfill();
+      const double* acc = m_useacc[ph * n + q] ? m_acc[q] : nullptr;
       if ((rc = launch_corr_tail(e->m_box, g.nz, e->m_onesided, e->m_dsbp, e->mUp.dev, u, um, e->mMu.dev,
-                                 e->mLambda.dev, e->mRho.dev, c, e->m_opt.beta, up2, g.h, dt, st, zf)) ||
+                                 e->mLambda.dev, e->mRho.dev, c, e->m_opt.beta, up2, g.h, dt, st, zf, acc)) ||
           (rc = launch_src_apply(e->m_box, e->sources(), 0.0, 1, dt * dt * dt * dt / 12, e->mRho.dev, up2, st,
                                  m_clk_t + q)) ||
           (rc = e->enforceBC(v, e->mMu, e->mLambda, 0.0, zf)))

@@ -62,6 +62,8 @@ private:
    std::vector<void*> m_allocs;
    std::vector<DampTile*> m_dtiles;  // per shot: its damping tile list (kernel variant 9), or null
    std::vector<int> m_ndtiles;
+   std::vector<double*> m_acc;       // per shot: its acceleration field (kernel variant 8 and later), or null
+   std::vector<char> m_useacc;       // [phase][shot]: the sweeps store and stage the acceleration
    std::vector<hipStream_t> m_saved;
    // the per-grid boundary work of the shots runs on NSIDE streams (default
    // priority) forked from and joined back into m_s, so the small kernels of

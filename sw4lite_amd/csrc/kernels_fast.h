@@ -27,6 +27,9 @@ struct SweepShot {
    SgCoef sg;
    double* out;
    double cof, dt2i, cupd, beta;
+   // asynchronous-load predictor: where to store the acceleration (or null);
+   // asynchronous-load corrector: the stored acceleration it stages
+   double* acc;
 };
 // Fill sh for planes k1..k2 of one grid (damping: the 32 x 8 damping sweep,
 // out = the array it updates, over the tiles of damp_tiles when tiles is
@@ -35,12 +38,13 @@ struct SweepShot {
 int fill_sweep_shot(bool damping, const Box& b, int k1, int k2, int kc, const double* upin, const double* u,
                     const double* um, const double* mu, const double* la, const double* rho, const SgCoef& sg,
                     double* out, double h, double dt, double beta, bool corr, int grid0, SweepShot* sh,
-                    const DampTile* tiles = nullptr, int ntiles = 0);
+                    const DampTile* tiles = nullptr, int ntiles = 0, double* acc = nullptr);
 // kind 0 predictor, 1 corrector, 2 damping sweeps of nshots grids in one
 // launch of nblocks workgroups (d_shots in device memory).
 int launch_sweep_batch(int kind, const SweepShot* d_shots, int nshots, int nblocks, hipStream_t s);
-// The same for kinds 0 and 2 with the asynchronous-load bodies, for grids
-// where async_loads_fit (even row length, 16-byte aligned arrays).
+// The same with the asynchronous-load bodies, for grids where
+// async_loads_fit (even row length, 16-byte aligned arrays); kind 1 is the
+// corrector staging the acceleration its predictor stored (SweepShot::acc).
 int launch_sweep_batch_async(int kind, const SweepShot* d_shots, int nshots, int nblocks, hipStream_t s);
 bool async_loads_fit(const Box& b, const double* const* arrays, int n);
 

@@ -944,7 +944,7 @@ __global__ void __launch_bounds__(32 * 8, 4) k_damp_batch(const SweepShot* __res
 int fill_sweep_shot(bool damping, const Box& b, int k1, int k2, int kc, const double* upin, const double* u,
                     const double* um, const double* mu, const double* la, const double* rho, const SgCoef& sg,
                     double* out, double h, double dt, double beta, bool corr, int grid0, SweepShot* sh,
-                    const DampTile* tiles, int ntiles)
+                    const DampTile* tiles, int ntiles, double* acc)
 {
    const long nx = b.ie - b.ib - 3, ny = b.je - b.jb - 3;
    if (kc <= 0) kc = 64;
@@ -965,6 +965,7 @@ int fill_sweep_shot(bool damping, const Box& b, int k1, int k2, int kc, const do
    sh->rho = rho;
    sh->sg = sg;
    sh->out = out;
+   sh->acc = damping ? nullptr : acc;
    sh->cof = 1.0 / (h * h);
    sh->dt2i = 1.0 / (dt * dt);
    sh->cupd = corr ? dt * dt * dt * dt / 12 : dt * dt;
@@ -975,11 +976,21 @@ int fill_sweep_shot(bool damping, const Box& b, int k1, int k2, int kc, const do
 
 // Batched predictor (kind 0) and damping (kind 2) sweeps of grids whose
 // planes take the asynchronous global-to-LDS loads (async_loads_fit).
+// (the predictor stores the acceleration in acc where the shot has one)
 __global__ void __launch_bounds__(NT, 2) k_sweep_dma_batch(const SweepShot* __restrict__ shots, int nshots)
 {
    const SweepShot& h = shots[find_shot(shots, nshots, blockIdx.x)];
    sweep_dma_body<DMA_PRED>(blockIdx.x - h.grid0, h.b, h.map, h.kbeg, h.kend, h.kc, h.u, h.um, h.mu, h.la, h.rho, h.sg,
-                            h.out, nullptr, h.cof, h.cupd, h.dt2i);
+                            h.out, h.acc, h.cof, h.cupd, h.dt2i);
+}
+
+// Batched corrector staging the acceleration its predictor stored (acc),
+// for shots that store it.
+__global__ void __launch_bounds__(NT, 2) k_sweep_dma_corr_batch(const SweepShot* __restrict__ shots, int nshots)
+{
+   const SweepShot& h = shots[find_shot(shots, nshots, blockIdx.x)];
+   sweep_dma_body<DMA_CORR>(blockIdx.x - h.grid0, h.b, h.map, h.kbeg, h.kend, h.kc, h.acc, h.upin, h.mu, h.la, h.rho,
+                            h.sg, h.out, nullptr, h.cof, h.cupd, h.dt2i);
 }
 
 __global__ void __launch_bounds__(NT, 3) k_damp_dma_batch(const SweepShot* __restrict__ shots, int nshots)
@@ -1003,10 +1014,10 @@ int launch_sweep_batch_async(int kind, const SweepShot* d_shots, int nshots, int
    if (nblocks <= 0) return OK;
    if (kind == 0)
       k_sweep_dma_batch<<<nblocks, NT, 0, s>>>(d_shots, nshots);
-   else if (kind == 2)
-      k_damp_dma_batch<<<nblocks, NT, 0, s>>>(d_shots, nshots);
+   else if (kind == 1)
+      k_sweep_dma_corr_batch<<<nblocks, NT, 0, s>>>(d_shots, nshots);
    else
-      return set_error(E_INVAL, "batched asynchronous-load sweeps: predictor or damping only");
+      k_damp_dma_batch<<<nblocks, NT, 0, s>>>(d_shots, nshots);
    return check_hip(hipGetLastError(), "batched asynchronous-load sweep launch");
 }
 

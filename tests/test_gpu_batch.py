@@ -14,9 +14,10 @@ from test_gpu_parity import SOURCES, layered_material
 pytestmark = pytest.mark.gpu
 TOL = 1e-13
 
-# even and odd row lengths: the predictor and damping sweeps of the first and
-# last grids take the asynchronous-load kernels, the second grid's the
-# register-staged ones, in launches of their own
+# even and odd row lengths: the sweeps of the first and last grids take the
+# asynchronous-load kernels (the predictor storing the acceleration, the
+# corrector staging it), the second grid's the register-staged ones, in
+# launches of their own
 SHAPES = [(30, 28, 26, 0.08, [2, 2, 2, 2, 0, 2]), (25, 22, 20, 0.1, [2, 2, 2, 2, 2, 2]),
           (36, 20, 24, 0.09, [1, 2, 2, 1, 0, 0])]
 RECS = [(1.0, 1.05, 0.0, 0), (1.53, 0.77, 0.41, 1), (0.7, 0.6, 0.9, 0)]
