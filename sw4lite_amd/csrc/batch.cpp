@@ -270,7 +270,8 @@ int Batch::one_step(int ph)
       const int zf = e->zfill();
       const double* acc = m_useacc[ph * n + q] ? m_acc[q] : nullptr;
       if ((rc = launch_corr_tail(e->m_box, g.nz, e->m_onesided, e->m_dsbp, e->mUp.dev, u, um, e->mMu.dev,
-                                 e->mLambda.dev, e->mRho.dev, c, e->m_opt.beta, up2, g.h, dt, st, zf, acc)) ||
+                                 e->mLambda.dev, e->mRho.dev, c, e->m_opt.beta, up2, g.h, dt, st, zf, acc,
+                                 e->m_variant >= 9)) ||
           (rc = launch_src_apply(e->m_box, e->sources(), 0.0, 1, dt * dt * dt * dt / 12, e->mRho.dev, up2, st,
                                  m_clk_t + q)) ||
           (rc = e->enforceBC(v, e->mMu, e->mLambda, 0.0, zf)))

@@ -60,6 +60,11 @@ bool async_loads_fit(const Box& b, const double* const* arrays, int n);
 // damping sweep restricted to the supergrid layers (damp_tiles).
 // The six SBP rows next to side 4 (top) or 5 (bottom) of one sweep from an
 // LDS slab (corr: with the acceleration and the supergrid damping).
+// The corrector's damping on the six SBP rows of a side (4 top, 5 bottom),
+// for rows k_sbp_rows wrote without it (launch_sbp_rows with beta 0); the
+// same bits as the rows kernel with its damping.
+int launch_rows_damp(const Box& b, int side, int nk, double* up, const double* u, const double* um,
+                     const double* rho, const SgCoef& sg, double beta, hipStream_t s);
 int launch_sbp_rows(bool corr, const Box& b, int side, int nk, const SbpTables* d_sbp, const double* upin,
                     const double* u, const double* um, const double* mu, const double* la, const double* rho,
                     const SgCoef& sg, double* out, double h, double dt, double beta, hipStream_t s,

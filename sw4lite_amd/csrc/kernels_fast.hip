@@ -1175,8 +1175,10 @@ struct SlabSten {
 
 // CORR = false: up = 2u - um + dt2/rho cof L(u) on the six rows;
 // CORR = true:  upo = up + dt4/(12 rho) cof L(acc) - damping(u, um) with
-// acc = (up - 2u + um) dt2i staged in the slab.
-template <bool CORR>
+// acc = (up - 2u + um) dt2i staged in the slab; DAMP = false: without the
+// damping (a damping sweep over these rows follows; the damping's registers
+// would push this kernel's 256 into 130 accumulator-register copies).
+template <bool CORR, bool DAMP = CORR>
 __global__ void __launch_bounds__(NT, 1)
 k_sbp_rows(Box b, int top, int nk, int ntx, const SbpTables* __restrict__ sbp, const double* __restrict__ upin,
            const double* __restrict__ u, const double* __restrict__ um, const double* __restrict__ mu,
@@ -1283,7 +1285,7 @@ k_sbp_rows(Box b, int top, int nk, int ntx, const SbpTables* __restrict__ sbp, c
    // a neighbour along its axis: outside the supergrid layers it subtracts
    // exactly zero and is not evaluated
    const int ii = i - b.ib, jj = j - b.jb;
-   const bool dxy = CORR && beta != 0 &&
+   const bool dxy = DAMP && beta != 0 &&
                     (sg.dcx[ii - 1] != 0 || sg.dcx[ii] != 0 || sg.dcx[ii + 1] != 0 || sg.dcy[jj - 1] != 0 ||
                      sg.dcy[jj] != 0 || sg.dcy[jj + 1] != 0);
 #pragma unroll 1
@@ -1295,7 +1297,7 @@ k_sbp_rows(Box b, int top, int nk, int ntx, const SbpTables* __restrict__ sbp, c
       const double sc = cupd / rho[p];
       double d[3] = {0, 0, 0};
       const int kk = k - b.kb;
-      if (CORR && beta != 0 && (dxy || sg.dcz[kk - 1] != 0 || sg.dcz[kk] != 0 || sg.dcz[kk + 1] != 0))
+      if (DAMP && beta != 0 && (dxy || sg.dcz[kk - 1] != 0 || sg.dcz[kk] != 0 || sg.dcz[kk + 1] != 0))
          addsgd4_point<1>(b, i, j, k, u, um, rho, sg, beta, d);
 #pragma unroll
       for (int c = 0; c < 3; c++) {
@@ -1308,6 +1310,44 @@ k_sbp_rows(Box b, int top, int nk, int ntx, const SbpTables* __restrict__ sbp, c
    }
 }
 
+// The damping of the corrector on the six SBP rows of a side, after
+// k_sbp_rows<true, false> wrote them: up -= damping(u, um) where a damping
+// coefficient reaches the point (every term carries one; elsewhere the
+// term is an exact zero).  The same arithmetic as the damping inside
+// k_sbp_rows<true> (addsgd4_point, then the subtraction), so the same bits.
+__global__ void __launch_bounds__(256)
+k_rows_damp(Box b, int top, int nk, double* __restrict__ up, const double* __restrict__ u,
+            const double* __restrict__ um, const double* __restrict__ rho, SgCoef sg, double beta)
+{
+   const int i = b.ib + 2 + blockIdx.x * blockDim.x + threadIdx.x;
+   const int j = b.jb + 2 + blockIdx.y * blockDim.y + threadIdx.y;
+   if (i > b.ie - 2 || j > b.je - 2) return;
+   const int k = top ? 1 + (int)blockIdx.z : nk - (int)blockIdx.z;
+   const int ii = i - b.ib, jj = j - b.jb, kk = k - b.kb;
+   const bool damped = sg.dcx[ii - 1] != 0 || sg.dcx[ii] != 0 || sg.dcx[ii + 1] != 0 || sg.dcy[jj - 1] != 0 ||
+                       sg.dcy[jj] != 0 || sg.dcy[jj + 1] != 0 || sg.dcz[kk - 1] != 0 || sg.dcz[kk] != 0 ||
+                       sg.dcz[kk + 1] != 0;
+   if (!damped) return;
+   double d[3] = {0, 0, 0};
+   if (beta != 0) addsgd4_point<1>(b, i, j, k, u, um, rho, sg, beta, d);
+   const long p = sidx(b, i, j, k);
+#pragma unroll
+   for (int c = 0; c < 3; c++) {
+      const long q = p + b.npts * c;
+      up[q] = up[q] - d[c];
+   }
+}
+
+int launch_rows_damp(const Box& b, int side, int nk, double* up, const double* u, const double* um,
+                     const double* rho, const SgCoef& sg, double beta, hipStream_t s)
+{
+   if (beta == 0) return OK;
+   const long nx = b.ie - b.ib - 3, ny = b.je - b.jb - 3;
+   k_rows_damp<<<dim3(cdiv(nx, 64), cdiv(ny, 4), 6), dim3(64, 4, 1), 0, s>>>(b, side == 4, nk, up, u, um, rho, sg,
+                                                                             beta);
+   return check_hip(hipGetLastError(), "SBP rows damping launch");
+}
+
 int launch_sbp_rows(bool corr, const Box& b, int side, int nk, const SbpTables* d_sbp, const double* upin,
                     const double* u, const double* um, const double* mu, const double* la, const double* rho,
                     const SgCoef& sg, double* out, double h, double dt, double beta, hipStream_t s, const double* acc)
@@ -1315,7 +1355,11 @@ int launch_sbp_rows(bool corr, const Box& b, int side, int nk, const SbpTables* 
    const long nx = b.ie - b.ib - 3, ny = b.je - b.jb - 3;
    const int ntx = (int)cdiv(nx, TX), nty = (int)cdiv(ny, TY);
    const double cof = 1.0 / (h * h), dt2i = 1.0 / (dt * dt);
-   if (corr)
+   if (corr && beta == 0)
+      k_sbp_rows<true, false><<<ntx * nty, NT, 0, s>>>(b, side == 4, nk, ntx, d_sbp, upin, u, um, mu, la, rho, sg,
+                                                       out, cof, dt2i, dt * dt * dt * dt / 12, 0.0,
+                                                       acc && dma_fits(b, acc, mu, la), acc);
+   else if (corr)
       k_sbp_rows<true><<<ntx * nty, NT, 0, s>>>(b, side == 4, nk, ntx, d_sbp, upin, u, um, mu, la, rho, sg, out, cof,
                                                 dt2i, dt * dt * dt * dt / 12, beta, acc && dma_fits(b, acc, mu, la), acc);
    else

@@ -51,7 +51,7 @@ int launch_corr_tail(const Box& b, int nk, const int* onesided, const SbpTables*
                      const double* u, const double* um, const double* mu, const double* la, const double* rho,
                      const SgCoef& sg, double beta, double* upo, double h, double dt, hipStream_t s,
                      int zfill = 0,
-                     const double* accf = nullptr);
+                     const double* accf = nullptr, bool split_damp = false);
 // tdev: when not null the time is read from device memory (graph replay)
 int launch_src_apply(const Box& b, const SourceTable& st, double t, int deriv2, double coef,
                      const double* rho, double* up, hipStream_t s, const double* tdev = nullptr);

@@ -574,11 +574,13 @@ int launch_pred_tail(const Box& b, int nk, const int* onesided, const SbpTables*
 int launch_corr_tail(const Box& b, int nk, const int* onesided, const SbpTables* d_sbp, const double* up,
                      const double* u, const double* um, const double* mu, const double* la, const double* rho,
                      const SgCoef& sg, double beta, double* upo, double h, double dt, hipStream_t s,
-                     int zfill, const double* accf)
+                     int zfill, const double* accf, bool split_damp)
 {
    for (int side = 4; side <= 5; side++) {  // LDS slab, acceleration formed while staging (or stored: accf)
       if (onesided[side] != 1) continue;
-      const int rc = launch_sbp_rows(true, b, side, nk, d_sbp, up, u, um, mu, la, rho, sg, upo, h, dt, beta, s, accf);
+      int rc = launch_sbp_rows(true, b, side, nk, d_sbp, up, u, um, mu, la, rho, sg, upo, h, dt,
+                               split_damp ? 0.0 : beta, s, accf);
+      if (rc == OK && split_damp) rc = launch_rows_damp(b, side, nk, upo, u, um, rho, sg, beta, s);
       if (rc) return rc;
    }
    const long ns = shell_count(b);
@@ -624,8 +626,11 @@ int launch_corr_fused(const Box& b, int nk, const int* onesided, const SbpTables
       const int rc = after_damping();
       if (rc) return rc;
    }
+   // kernel variant 9: the rows' damping in a pass of its own after them
+   // (inside the rows kernel its registers cost that kernel's occupancy)
    if (variant >= 1)
-      return launch_corr_tail(b, nk, onesided, d_sbp, up, u, um, mu, la, rho, sg, beta, upo, h, dt, s, zfill, accf);
+      return launch_corr_tail(b, nk, onesided, d_sbp, up, u, um, mu, la, rho, sg, beta, upo, h, dt, s, zfill, accf,
+                              variant >= 9);
    for (int side = 4; side <= 5; side++) {
       if (onesided[side] != 1) continue;
       const int ks = side == 4 ? 0 : nk - SLAB_PLANES + 2;  // planes KK(0)..KK(8) of the rows
