@@ -779,6 +779,13 @@ def main_batch_adjoint(a, torch):
     gtr = [e.receivers()[: steps + 1] - d for e, d in zip(shots, mats)]
     J = 0.5 * sum(float(np.sum(g * g)) for g in gtr)
     torch.cuda.synchronize()
+    # the first call allocates the batch's work memory, which later calls reuse
+    print("batch-adjoint: first gradient call", flush=True)
+    t0 = time.perf_counter()
+    b.gradient(gtr, steps, ckpt=ckpt, max_concurrent=a.concurrent)
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    print("batch-adjoint: first gradient call %.1f s; timed call" % t_first, flush=True)
     t0 = time.perf_counter()
     grads = b.gradient(gtr, steps, ckpt=ckpt, max_concurrent=a.concurrent)
     torch.cuda.synchronize()
@@ -789,10 +796,10 @@ def main_batch_adjoint(a, torch):
     if not a.no_fp32:
         # the FP32 form's gradients of the same shots (forward and its
         # recomputation in FP32 from float checkpoints, reverse in FP64),
-        # against the FP64 batched forward
-        for e in shots:
-            e.release_workspace()
+        # against the FP64 batched forward (the work memory of the FP64
+        # gradients is reused)
         torch.cuda.synchronize()
+        print("batch-adjoint: FP32-form gradient call", flush=True)
         t0 = time.perf_counter()
         b.gradient(gtr, steps, ckpt=ckpt, max_concurrent=a.concurrent, fp32=True)
         torch.cuda.synchronize()
@@ -811,7 +818,7 @@ def main_batch_adjoint(a, torch):
                         "top, supergrid 30, one double couple, 128 surface receivers each; d_obs from the material "
                         "+3%% in a central Gaussian blob" % a.shots,
             "sizes": sizes, "total_points": pts, "checkpoint_every": ckpt, "concurrent_shots": a.concurrent,
-            "forward_s": t_fwd, "gradient_s": t_grad, "misfit": J,
+            "forward_s": t_fwd, "gradient_s": t_grad, "gradient_first_call_s": t_first, "misfit": J,
             "forward_grid_pt_updates_per_s": pts * steps / t_fwd,
             "max_abs_gradient": {"rho": gmax[0], "mu": gmax[1], "lambda": gmax[2]},
             "fp32_form": fp32,

@@ -397,6 +397,14 @@ int sw4hip_ew_gradient_comp(sw4hip_ew* ew, int nsteps, int ckpt, const double* g
 /* The checkpoint interval sw4hip_ew_gradient would use for (nsteps, ckpt)
  * and the device bytes it needs (device_bytes may be NULL). */
 int sw4hip_ew_gradient_plan(sw4hip_ew* ew, int nsteps, int ckpt, int* used_ckpt, double* device_bytes);
+/* Whether sw4hip_ew_gradient keeps, for every step of a recomputed segment,
+ * its predicted field and acceleration (two more states a step), so that
+ * the reverse steps do not recompute their predictor; the same gradients
+ * to the last bit.  store -1 (the default): when they fit in free device
+ * memory and, with ckpt <= 0, when that recomputes fewer steps; 0 never;
+ * 1 always.  The FP32 form's gradient never keeps them.  No counterpart in
+ * the reference. */
+int sw4hip_ew_set_gradient_store(sw4hip_ew* ew, int store);
 /* sw4hip_ew_gradient keeps its device buffers (states, work fields) between
  * calls and reuses them; this frees them (sw4hip_ew_destroy does too).  No
  * counterpart in the reference. */
@@ -435,7 +443,10 @@ int sw4hip_batch_tune(sw4hip_batch* batch, int ncand, const int* cand, int* best
  * glambda[q] as there; every shot run from zero data).  Up to
  * max_concurrent shots run at once, each on a stream of its own, so that
  * the kernels of small grids share the device; a shot starts when its work
- * memory fits beside the running ones and releases it when done.
+ * memory fits beside the running ones.  The largest shots run first; the
+ * shots take their work memory from one pool of the batch, which the later
+ * (smaller) shots reuse and which stays allocated for the next call
+ * (sw4hip_batch_release_workspace, sw4hip_batch_destroy free it).
  * ckpt < 1: about sqrt(nsteps) steps between checkpoints. */
 int sw4hip_batch_gradient(sw4hip_batch* batch, int nsteps, int ckpt, const double* const* grad_traces,
                           double* const* grho, double* const* gmu, double* const* glambda, int on_device,
@@ -444,6 +455,9 @@ int sw4hip_batch_gradient(sw4hip_batch* batch, int nsteps, int ckpt, const doubl
 int sw4hip_batch_gradient_comp(sw4hip_batch* batch, int nsteps, int ckpt, const double* const* grad_traces,
                                double* const* grho, double* const* gmu, double* const* glambda, int on_device,
                                int max_concurrent);
+/* Frees the work memory sw4hip_batch_gradient keeps.  No counterpart in
+ * the reference. */
+int sw4hip_batch_release_workspace(sw4hip_batch* batch);
 /* Advance every shot by nsteps; use_graph: replay the captured period. */
 int sw4hip_batch_step(sw4hip_batch* batch, int nsteps, int use_graph);
 int sw4hip_ew_sync(sw4hip_ew* ew);

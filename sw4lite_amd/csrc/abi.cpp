@@ -941,7 +941,8 @@ int sw4hip_ew_gradient(sw4hip_ew* h, int nsteps, int ckpt, const double* grad_tr
                        double* glambda, int on_device)
 {
    EWCHK(h);
-   return h->ew->gradient(nsteps, ckpt, grad_traces, grho, gmu, glambda, on_device != 0);
+   return h->ew->gradient(nsteps, ckpt, grad_traces, grho, gmu, glambda, on_device != 0, false,
+                          h->ew->m_adj_store);
 }
 
 int sw4hip_ew_gradient_comp(sw4hip_ew* h, int nsteps, int ckpt, const double* grad_traces, double* grho,
@@ -955,7 +956,15 @@ int sw4hip_ew_gradient_plan(sw4hip_ew* h, int nsteps, int ckpt, int* used_ckpt, 
 {
    EWCHK(h);
    if (!used_ckpt) return set_error(E_INVAL, "null output");
-   return h->ew->gradient_plan(nsteps, ckpt, used_ckpt, device_bytes);
+   return h->ew->gradient_plan(nsteps, ckpt, used_ckpt, device_bytes, false, h->ew->m_adj_store);
+}
+
+int sw4hip_ew_set_gradient_store(sw4hip_ew* h, int store)
+{
+   EWCHK(h);
+   if (store < -1 || store > 1) return set_error(E_INVAL, "gradient store must be -1, 0 or 1");
+   h->ew->m_adj_store = store;
+   return OK;
 }
 
 int sw4hip_ew_release_workspace(sw4hip_ew* h)
@@ -1069,6 +1078,12 @@ int sw4hip_batch_gradient_comp(sw4hip_batch* h, int nsteps, int ckpt, const doub
 {
    if (!h || !h->b) return set_error(E_INVAL, "null batch");
    return h->b->gradient(nsteps, ckpt, grad_traces, grho, gmu, glambda, on_device != 0, max_concurrent, true);
+}
+
+int sw4hip_batch_release_workspace(sw4hip_batch* h)
+{
+   if (!h || !h->b) return set_error(E_INVAL, "null batch");
+   return h->b->release_workspace();
 }
 
 int sw4hip_batch_step(sw4hip_batch* h, int nsteps, int use_graph)

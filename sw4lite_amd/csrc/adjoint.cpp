@@ -82,8 +82,13 @@ int EW::fused_once(double t, double* acc)
 // tmp: 5 work fields of 3 components; pad: padded material and the
 // transposed operator's work fields (kernels_adj.h), pad.w zero in the
 // pad.
+// up_st (with acc_st where the predictor stores the acceleration): Up (and
+// A) of this step kept from the segment's recomputation, the same bits as
+// the recomputation here would give; then nothing of the forward step is
+// recomputed.
 int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double* lam_b, double* lam_c,
-                 double* grho, double* gmu, double* gla, double* const* tmp, double* plane, const AdjPad& pad)
+                 double* grho, double* gmu, double* gla, double* const* tmp, double* plane, const AdjPad& pad,
+                 double* up_st, double* acc_st)
 {
    const GridSetup& g = m_opt.grid;
    const SgCoef c = sg();
@@ -106,17 +111,25 @@ int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double
    int ka = 0, kz = 0;
    fused_planes(m_box, g.nz, m_onesided, &ka, &kz);
    double* accf = m_variant >= 8 && m_box.ni % 2 == 0 && kz >= ka ? Acc : nullptr;
-   if ((rc = launch_pred_fused(m_box, g.nz, m_onesided, m_dsbp, u, um, mMu.dev, mLambda.dev, mRho.dev, c, Up, g.h, dt,
-                               m_stream, nullptr, m_variant, m_kchunk, zf, accf)) ||
-       (rc = launch_src_apply(m_box, st, t, 0, dt2, mRho.dev, Up, m_stream)) ||
-       (rc = enforceBC(UpA, mMu, mLambda, t + dt, zf)))
-      return rc;
-   if (accf) {
-      if ((rc = launch_accel_fix(m_box, ka, kz, (zf & 15) != 15, true, Up, u, um, dt, Acc, m_stream)) ||
-          (rc = launch_accel_points(m_box, st, Up, u, um, dt, Acc, m_stream)))
+   if (up_st) {
+      Up = up_st;
+      if (accf && acc_st)
+         Acc = acc_st;
+      else if ((rc = launch_dpdmt(m_box, Up, u, um, Acc, 1.0 / dt2, m_stream)))
          return rc;
-   } else if ((rc = launch_dpdmt(m_box, Up, u, um, Acc, 1.0 / dt2, m_stream)))
-      return rc;
+   } else {
+      if ((rc = launch_pred_fused(m_box, g.nz, m_onesided, m_dsbp, u, um, mMu.dev, mLambda.dev, mRho.dev, c, Up, g.h,
+                                  dt, m_stream, nullptr, m_variant, m_kchunk, zf, accf)) ||
+          (rc = launch_src_apply(m_box, st, t, 0, dt2, mRho.dev, Up, m_stream)) ||
+          (rc = enforceBC(UpA, mMu, mLambda, t + dt, zf)))
+         return rc;
+      if (accf) {
+         if ((rc = launch_accel_fix(m_box, ka, kz, (zf & 15) != 15, true, Up, u, um, dt, Acc, m_stream)) ||
+             (rc = launch_accel_points(m_box, st, Up, u, um, dt, Acc, m_stream)))
+            return rc;
+      } else if ((rc = launch_dpdmt(m_box, Up, u, um, Acc, 1.0 / dt2, m_stream)))
+         return rc;
+   }
    // corrector BC, transposed (acting on U^{n+1})
    if ((rc = launch_bcfortsg_T(m_box, m_wind, g.nx, g.ny, g.nz, g.bc, Y, u1, g.h, m_dsbp, mMu.dev, mLambda.dev, c.strx,
                                c.stry, gmu, gla, plane, m_stream)))
@@ -151,8 +164,76 @@ int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double
    return OK;
 }
 
+void* WorkPool::take(size_t bytes)
+{
+   {
+      std::lock_guard<std::mutex> lk(m_mtx);
+      Buf* best = nullptr;
+      for (Buf& b : m_bufs)
+         if (!b.used && b.bytes >= bytes && (!best || b.bytes < best->bytes)) best = &b;
+      if (best) {
+         best->used = true;
+         return best->p;
+      }
+   }
+   // allocations and frees run outside the lock, so that the other callers'
+   // takes and gives do not wait for them
+   void* p = nullptr;
+   if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      // make room: free the idle buffers (none fits this request), then try once more
+      std::vector<void*> idle;
+      {
+         std::lock_guard<std::mutex> lk(m_mtx);
+         std::vector<Buf> keep;
+         for (const Buf& b : m_bufs) {
+            if (b.used)
+               keep.push_back(b);
+            else
+               idle.push_back(b.p);
+         }
+         m_bufs.swap(keep);
+      }
+      for (void* q : idle) (void)hipFree(q);
+      if (hipMalloc(&p, bytes) != hipSuccess) {
+         (void)hipGetLastError();
+         return nullptr;
+      }
+   }
+   std::lock_guard<std::mutex> lk(m_mtx);
+   m_bufs.push_back(Buf{bytes, p, true});
+   return p;
+}
+
+void WorkPool::give(void* p)
+{
+   std::lock_guard<std::mutex> lk(m_mtx);
+   for (Buf& b : m_bufs)
+      if (b.p == p) b.used = false;
+}
+
+double WorkPool::held() const
+{
+   std::lock_guard<std::mutex> lk(m_mtx);
+   double t = 0;
+   for (const Buf& b : m_bufs) t += (double)b.bytes;
+   return t;
+}
+
+void WorkPool::release()
+{
+   std::lock_guard<std::mutex> lk(m_mtx);
+   for (const Buf& b : m_bufs) (void)hipFree(b.p);
+   m_bufs.clear();
+}
+
 void* EW::pool_take(size_t bytes)
 {
+   if (m_shared) {
+      void* p = m_shared->take(bytes);
+      if (p) m_taken.push_back(p);
+      return p;
+   }
    for (PoolBuf& b : m_pool)
       if (!b.used && b.bytes == bytes) {
          b.used = true;
@@ -182,6 +263,12 @@ void* EW::pool_take(size_t bytes)
 void EW::pool_return()
 {
    for (PoolBuf& b : m_pool) b.used = false;
+   if (!m_taken.empty()) {
+      // kernels of this stream may still read them (error paths)
+      (void)hipStreamSynchronize(m_stream);
+      for (void* p : m_taken) m_shared->give(p);
+      m_taken.clear();
+   }
 }
 
 int EW::release_workspace()
@@ -209,23 +296,27 @@ static double adjoint_fixed_doubles(const Box& b, int nsteps, int nrec)
 }
 
 // States held by a checkpoint interval c: the segment buffer (c + 2) and two
-// per checkpoint (U, Um), or 1.5 for the float (U, C, D) of the FP32 form.
-// The reverse sweep recomputes every segment but the last, nsteps - (length
-// of the last segment) forward steps in all.
-static double adjoint_states(int nsteps, int c, bool fp32)
+// per checkpoint (U, Um), or 1.5 for the float (U, C, D) of the FP32 form;
+// with store, the predicted field and the acceleration of every step of a
+// segment (2 c).  The reverse sweep recomputes every segment but the last,
+// nsteps - (length of the last segment) forward steps in all.
+static double adjoint_states(int nsteps, int c, bool fp32, bool store)
 {
-   return (c + 2) + (fp32 ? 1.5 : 2.0) * ((nsteps + c - 1) / c);
+   return (c + 2) + (store ? 2.0 * c : 0.0) + (fp32 ? 1.5 : 2.0) * ((nsteps + c - 1) / c);
 }
 
-int EW::gradient_plan(int nsteps, int ckpt, int* used, double* bytes, bool fp32)
+int EW::gradient_plan(int nsteps, int ckpt, int* used, double* bytes, bool fp32, int store, int* stored)
 {
    if (nsteps < 1) return set_error(E_INVAL, "EW: gradient plan needs nsteps >= 1");
    const int nrec = (int)m_rec.size() / 4;
    const double n3 = 3.0 * m_box.npts, fixed = adjoint_fixed_doubles(m_box, nsteps, nrec);
+   // the FP32 form recomputes float steps, which give the FP64 reverse step
+   // no predicted field to keep
+   const bool can_store = !fp32 && store != 0;
    int c = std::min(ckpt, nsteps);
-   if (ckpt < 1) {
-      // the largest interval whose states fit in free device memory (all
-      // of them when they fit: no recomputation), keeping a reserve
+   bool st = can_store && store == 1;
+   if (ckpt < 1 || (can_store && store < 0)) {
+      // free device memory (with the workspace already held), keeping a reserve
       size_t freeb = 0, totalb = 0;
       if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
          (void)hipGetLastError();
@@ -233,22 +324,38 @@ int EW::gradient_plan(int nsteps, int ckpt, int* used, double* bytes, bool fp32)
       }
       const double reserve = 6.0 * (1 << 30) + 0.02 * (double)totalb;
       const double room = ((double)freeb + workspace_bytes() - reserve) / 8.0 - fixed;
-      c = 0;
-      for (int q = nsteps; q >= 1 && room > 0; q--)
-         if (adjoint_states(nsteps, q, fp32) * n3 <= room) {
-            c = q;
-            break;
+      // the largest interval whose states fit (all of them when they fit: no recomputation)
+      auto largest = [&](bool s) {
+         for (int q = nsteps; q >= 1 && room > 0; q--)
+            if (adjoint_states(nsteps, q, fp32, s) * n3 <= room) return q;
+         return 0;
+      };
+      if (ckpt < 1) {
+         c = largest(false);
+         if (c < 1) c = std::max(1, (int)std::lround(std::sqrt((double)nsteps)));
+         if (can_store) {
+            // forward steps recomputed: every segment but the last; without
+            // the kept fields every reverse step recomputes its predictor as
+            // well, 0.4 of a forward step (256^3, MI355X)
+            auto recomputed = [&](int q) { return (double)((nsteps + q - 1) / q - 1) * q; };
+            const int cs = largest(true);
+            st = cs >= 1 && (store == 1 || recomputed(cs) < recomputed(c) + 0.4 * nsteps);
+            if (st) c = cs;
          }
-      if (c < 1) c = std::max(1, (int)std::lround(std::sqrt((double)nsteps)));
+      } else if (store < 0) {
+         st = adjoint_states(nsteps, c, fp32, true) * n3 <= room;
+      }
    }
    if (used) *used = c;
+   if (stored) *stored = st ? 1 : 0;
    // the FP32 form's own state (18 floats a point) is held outside the pool
-   if (bytes) *bytes = 8.0 * (fixed + adjoint_states(nsteps, c, fp32) * n3) + (fp32 && !m_cbuf ? 72.0 * m_box.npts : 0.0);
+   if (bytes)
+      *bytes = 8.0 * (fixed + adjoint_states(nsteps, c, fp32, st) * n3) + (fp32 && !m_cbuf ? 72.0 * m_box.npts : 0.0);
    return OK;
 }
 
 int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* gmu, double* gla, bool on_device,
-                 bool fp32)
+                 bool fp32, int store)
 {
    if (!m_setup_done) return set_error(E_STATE, "EW: gradient before setup");
    if (!m_opt.corder) return set_error(E_INVAL, "EW: the adjoint needs the component-blocked layout (corder=1)");
@@ -260,13 +367,15 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
    if (nsteps < 1 || nsteps + 1 > m_maxsamples || nrec == 0)
       return set_error(E_INVAL, "EW: gradient needs receivers and 1 <= nsteps <= the set-up step count");
    if (!gtr || !grho || !gmu || !gla) return set_error(E_INVAL, "EW: null gradient argument");
+   int keep = 0;  // the segment's predicted fields and accelerations are kept
    {
-      int rc0 = gradient_plan(nsteps, ckpt, &ckpt, nullptr, fp32);
+      int rc0 = gradient_plan(nsteps, ckpt, &ckpt, nullptr, fp32, store, &keep);
       if (rc0) return rc0;
    }
    const long n1 = m_box.npts, n3 = 3 * n1;
    const int ncp = (nsteps + ckpt - 1) / ckpt;
-   // work memory: 5 fields, 3 adjoint fields, ckpt+2 segment states, 2 per checkpoint, 3 gradients, pad, traces
+   // work memory: 5 fields, 3 adjoint fields, ckpt+2 segment states (and 2 ckpt
+   // kept fields), 2 per checkpoint, 3 gradients, pad, traces
    // buffers come from the workspace pool and go back to it at the end
    auto release = [&]() { pool_return(); };
    auto grab = [&](size_t n, double** out) {
@@ -285,6 +394,9 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
    for (int q = 0; q < 3 && rc == OK; q++) rc = grab(n3, &lam[q]);
    for (int q = 0; q < 3 && rc == OK; q++) rc = grab(n1, &gr[q]);
    for (int q = 0; q <= ckpt + 1 && rc == OK; q++) rc = grab(n3, &seg[q]);
+   // kept[2 m], kept[2 m + 1]: Up and the acceleration of the segment's step m
+   std::vector<double*> kept(keep ? 2 * ckpt : 0);
+   for (size_t q = 0; q < kept.size() && rc == OK; q++) rc = grab(n3, &kept[q]);
    // checkpoints: (U^{s0}, U^{s0-1}) in FP64, or the float (U, C, D) of the
    // FP32 form (cp[2 q], 9 floats a point) at the start of segment q
    for (int q = 0; q < 2 * ncp && rc == OK; q++)
@@ -352,9 +464,10 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
       for (int m = 1; m <= s1 - s0 && e == OK; m++) {
          mU.dev = seg[m];
          mUm.dev = seg[m - 1];
-         mUp.dev = tmp[0];
+         mUp.dev = keep ? kept[2 * (m - 1)] : tmp[0];
          mUp2.dev = seg[m + 1];
-         e = fused_once((s0 + m - 1) * mDt, tmp[1]);  // tmp[1]: the reverse step's acceleration, free here
+         // tmp[1]: the reverse step's acceleration, free here
+         e = fused_once((s0 + m - 1) * mDt, keep ? kept[2 * (m - 1) + 1] : tmp[1]);
       }
       mU.dev = sU;
       mUm.dev = sUm;
@@ -446,7 +559,7 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
       for (int n = s1 - 1; n >= s0 && rc == OK; n--) {
          // lc: the adjoint of U^{n-1} starts here, then takes its direct source
          if ((rc = step_vjp(n, seg[n - s0 + 1], seg[n - s0], seg[n - s0 + 2], la, lb, lc, gr[0], gr[1], gr[2], tmp,
-                            plane, pad)) ||
+                            plane, pad, keep ? kept[2 * (n - s0)] : nullptr, keep ? kept[2 * (n - s0) + 1] : nullptr)) ||
              (rc = add_direct(n - 1, lc)))
             break;
          // rotate: U^{n} becomes the complete adjoint; la was consumed

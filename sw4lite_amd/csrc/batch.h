@@ -32,9 +32,13 @@ public:
    // shots in flight at once, each on a stream of its own driven by a host
    // thread of its own, so that the kernels of small grids share the device;
    // a shot starts when its work memory (EW::gradient_plan) fits beside the
-   // shots running.  Each shot's work memory is released when it is done.
+   // shots running.  The shots run largest first and take their work
+   // memory from one pool (WorkPool), which the smaller shots reuse and
+   // which stays allocated for the next call.
    int gradient(int nsteps, int ckpt, const double* const* gtr, double* const* grho, double* const* gmu,
                 double* const* gla, bool on_device, int max_concurrent, bool fp32 = false);
+   // frees the gradients' pool
+   int release_workspace();
 
 private:
    int refresh();  // base pointers and clocks from the shots, tables
@@ -65,6 +69,7 @@ private:
    std::vector<double*> m_acc;       // per shot: its acceleration field (kernel variant 8 and later), or null
    std::vector<char> m_useacc;       // [phase][shot]: the sweeps store and stage the acceleration
    std::vector<hipStream_t> m_saved;
+   WorkPool m_wpool;  // the shots' gradient work memory
    // the per-grid boundary work of the shots runs on NSIDE streams (default
    // priority) forked from and joined back into m_s, so the small kernels of
    // different grids overlap; the fork/join is captured into the graph too

@@ -3,6 +3,7 @@
 // argument meaning follow EW.h:93-127 so the reference time loop can be kept
 // as is; the bodies launch gfx950 kernels instead of OpenMP loops.
 #pragma once
+#include <mutex>
 #include <vector>
 
 #include "kernels_adj.h"
@@ -40,6 +41,32 @@ struct EWOptions {
 
 class Batch;
 class GridStack;
+
+// Device buffers shared by the gradients of several EW objects run one
+// after another or at once (Batch::gradient): a buffer given back is
+// reused by the next request it is large enough for, so the shots' work
+// memory is allocated once for the largest shots and never freed between
+// them.  Thread-safe.
+class WorkPool {
+public:
+   ~WorkPool() { release(); }
+   // the smallest free buffer of at least bytes, else a new one (after
+   // freeing the idle buffers when the allocation fails; null when it
+   // still does)
+   void* take(size_t bytes);
+   void give(void* p);
+   double held() const;  // bytes allocated
+   void release();       // frees every buffer; none may be in use
+
+private:
+   struct Buf {
+      size_t bytes;
+      void* p;
+      bool used;
+   };
+   mutable std::mutex m_mtx;
+   std::vector<Buf> m_bufs;
+};
 
 class EW {
    friend class Batch;
@@ -125,11 +152,22 @@ public:
    // compensated update (step_comp), checkpoints of its float state (36
    // bytes a point instead of 48); the reverse steps in FP64 on that
    // trajectory.  The gradient of the FP32 run's misfit.
+   // store: keep the predicted field and the acceleration of every step of
+   // a recomputed segment (two states a step), so that the reverse steps do
+   // not recompute their predictor (FP64 only).  -1: when they fit in free
+   // device memory and, for an interval chosen here, cost fewer recomputed
+   // steps; 0: never; 1: always (the interval chosen for it).
    int gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* gmu, double* gla, bool on_device,
-                bool fp32 = false);
+                bool fp32 = false, int store = -1);
    // checkpoint interval the gradient uses for ckpt (< 1: from free device
-   // memory and the workspace already held) and the device bytes it needs
-   int gradient_plan(int nsteps, int ckpt, int* used, double* bytes, bool fp32 = false);
+   // memory and the workspace already held), the device bytes it needs and
+   // whether it keeps the fields above
+   int gradient_plan(int nsteps, int ckpt, int* used, double* bytes, bool fp32 = false, int store = -1,
+                     int* stored = nullptr);
+   int m_adj_store = -1;  // the store argument of the C-ABI gradient calls
+   // when set, the gradient's buffers come from this pool (and go back to
+   // it) instead of the object's own
+   WorkPool* m_shared = nullptr;
    // The gradient's device buffers stay allocated between calls (a pool of
    // exact-size buffers, grown as needed) so that repeated gradients skip
    // allocating hundreds of GB; this frees them.
@@ -190,6 +228,7 @@ private:
       bool used;
    };
    std::vector<PoolBuf> m_pool;  // gradient workspace
+   std::vector<void*> m_taken;    // buffers taken from m_shared
    void* pool_take(size_t bytes);
    void pool_return();
    int alloc(Sarray& a, int nc);
@@ -201,7 +240,8 @@ private:
    // acceleration of kernel variant 8, or null), no rotation
    int fused_once(double t, double* acc = nullptr);
    int step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double* lam_b, double* lam_c,
-                double* grho, double* gmu, double* gla, double* const* tmp, double* plane, const AdjPad& pad);
+                double* grho, double* gmu, double* gla, double* const* tmp, double* plane, const AdjPad& pad,
+                double* up_st = nullptr, double* acc_st = nullptr);
    int comp_begin();
    CompState comp_state() const;
    SgCoefF sgf() const;

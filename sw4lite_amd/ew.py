@@ -443,6 +443,13 @@ class EW:
         L.check(self.lib.sw4hip_ew_gradient_plan(self.h, int(nsteps), int(ckpt), C.byref(used), C.byref(nbytes)))
         return used.value, nbytes.value
 
+    def set_gradient_store(self, store):
+        """Whether gradient() keeps the predicted field and acceleration of
+        every step of a recomputed segment (two more states a step) so that
+        the reverse steps skip recomputing their predictor: -1 when it fits
+        and pays (default), 0 never, 1 always.  Same gradients either way."""
+        L.check(self.lib.sw4hip_ew_set_gradient_store(self.h, int(store)))
+
     def release_workspace(self):
         """Free the device buffers that gradient() keeps between calls."""
         L.check(self.lib.sw4hip_ew_release_workspace(self.h))
@@ -518,6 +525,10 @@ class Batch:
                    C.cast(arr([r[0] for r in res]), C.c_void_p), C.cast(arr([r[1] for r in res]), C.c_void_p),
                    C.cast(arr([r[2] for r in res]), C.c_void_p), on_dev, int(max_concurrent)))
         return res
+
+    def release_workspace(self):
+        """Free the device work memory that gradient() keeps between calls."""
+        L.check(self.lib.sw4hip_batch_release_workspace(self.h))
 
     def close(self):
         if self.h:

@@ -72,10 +72,12 @@ _SIGS = {
     "sw4hip_batch_tune": (I, [P, I, P, IP, P]),
     "sw4hip_batch_step": (I, [P, I, I]),
     "sw4hip_batch_gradient": (I, [P, I, I, P, P, P, P, I, I]),
+    "sw4hip_batch_release_workspace": (I, [P]),
     "sw4hip_ew_gradient": (I, [P, I, I, P, P, P, P, I]),
     "sw4hip_ew_gradient_comp": (I, [P, I, I, P, P, P, P, I]),
     "sw4hip_batch_gradient_comp": (I, [P, I, I, P, P, P, P, I, I]),
     "sw4hip_ew_gradient_plan": (I, [P, I, I, P, P]),
+    "sw4hip_ew_set_gradient_store": (I, [P, I]),
     "sw4hip_ew_release_workspace": (I, [P]),
     "sw4hip_host_moment_source_curv": (I, [C.POINTER(GridDesc), P, D, D, D, P, I, IP, P, P]),
     "sw4hip_host_topography_grid": (I, [C.POINTER(Topography), I, I, D, D, IP, IP, P]),

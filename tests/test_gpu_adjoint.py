@@ -293,6 +293,37 @@ def test_gradient_stored_acceleration_bitwise(torch, bc):
     assert np.max(np.abs(res[0][0])) > 0 and ndiff == [0, 0, 0]
 
 
+@pytest.mark.parametrize("variant,bc", [(9, [2, 2, 2, 2, 0, 2]), (9, [1, 1, 2, 2, 2, 0]), (6, [2, 2, 2, 2, 0, 2])])
+def test_gradient_kept_fields_bitwise(torch, variant, bc):
+    """Kept from the recomputation of each segment, the predicted field and
+    the acceleration of every step spare the reverse steps their own
+    predictor (variant 6 keeps the field only and forms the acceleration in
+    the reverse step): the gradients must be those of the recomputing
+    reverse steps to the last bit, for an interval that leaves three
+    recomputed segments and for the whole history."""
+    nx, ny, nz, h, nsteps = 20, 18, 16, 0.1, 13
+    rng = np.random.default_rng(53)
+    gtr = rng.standard_normal((nsteps + 1, len(RECS), 3))
+    e, mat = make_ew(nx, ny, nz, h, bc, nsteps)
+    e.set_kernel_variant(variant, 7)
+    for ckpt in (4, 0):
+        res = []
+        for store in (0, 1):
+            e.set_gradient_store(store)
+            res.append(e.gradient(gtr, nsteps, ckpt=ckpt))
+        ndiff = [int(np.count_nonzero(a != b)) for a, b in zip(*res)]
+        print("gradient, kept fields: variant", variant, "bc", bc, "ckpt", ckpt, "points differing:", ndiff,
+              flush=True)
+        assert np.max(np.abs(res[0][0])) > 0 and ndiff == [0, 0, 0]
+    # the kept fields' memory: two states per step of the interval
+    e.set_gradient_store(0)
+    b0 = e.gradient_plan(nsteps, 4)[1]
+    e.set_gradient_store(1)
+    b1 = e.gradient_plan(nsteps, 4)[1]
+    e.close()
+    assert b1 - b0 == pytest.approx(2 * 4 * 3 * 8 * (nx + 4) * (ny + 4) * (nz + 4))
+
+
 def test_autograd_function(torch):
     """torch.autograd through sw4lite_amd.adjoint.seismograms: the backward
     pass equals the C-ABI gradient, and a gradient step lowers the misfit."""
@@ -344,6 +375,13 @@ def test_batch_gradient_matches_single_shots(torch, max_concurrent):
         print("batch gradient, max_concurrent", max_concurrent, "shot", q, grids[q],
               "points differing from the single shot:", ndiff, flush=True)
         assert np.max(np.abs(w[0])) > 0 and ndiff == [0, 0, 0]
+    # again from the work memory the first call left in the batch's pool
+    # (the largest shots' buffers serve the smaller ones), then after freeing it
+    again = b.gradient(gtrs, nsteps, ckpt=4, max_concurrent=max_concurrent)
+    b.release_workspace()
+    after = b.gradient(gtrs, nsteps, ckpt=4, max_concurrent=max_concurrent)
+    for a, c, w in zip(again, after, single):
+        assert all(np.array_equal(x, y) and np.array_equal(z, y) for x, z, y in zip(a, c, w))
     # finite differences on the smallest shot, misfit against a perturbed record
     e, mat = shots[1], mats[1]
     J0, tr = misfit(e, mat, nsteps, 0.0)
