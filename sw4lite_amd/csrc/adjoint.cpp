@@ -79,7 +79,8 @@ int EW::fused_once(double t, double* acc)
 //   dt^2/rho (L(U) + F)        = Up - 2U + Um = dt^2 A,
 //   c4/rho (L(A) + F_tt) - D   = U^{n+1} - Up,
 // so d/drho [y . c4/rho (...)] + d/drho [-y . D]|_(1/rho) = -y . (U^{n+1} - Up)/rho.
-// tmp: 5 work fields of 3 components; pad: padded material and the
+// tmp: 4 work fields of 3 components on the box and one on the padded box
+// (pad_box); pad: padded material and the
 // transposed operator's work fields (kernels_adj.h), pad.w zero in the
 // pad.
 // up_st (with acc_st where the predictor stores the acceleration): Up (and
@@ -96,7 +97,11 @@ int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double
    const double dt = mDt, t = n * dt, dt2 = dt * dt, c4 = dt2 * dt2 / 12;
    const long n3 = 3 * m_box.npts;
    const RowMap rm = make_rowmap(m_box, g.nz, m_onesided);
-   double *Up = tmp[0], *Acc = tmp[1], *Db = tmp[2], *Z = tmp[3], *LT1 = tmp[4];
+   double *Up = tmp[0], *Acc = tmp[1], *Db = tmp[2], *Z = tmp[3];
+   // the second transposed operator's result goes to tmp[4] (on the padded
+   // box), so that the first one's stays in pad.o for the final pass
+   AdjPad pad2 = pad;
+   pad2.o = tmp[4];
    // Y: the adjoint of the corrected field, then (in place) of Up
    double* Y = lam_a;
    Sarray UpA = mU;
@@ -147,19 +152,19 @@ int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double
        (rc = launch_rhs4sg_matgrad(m_box, rm, m_dsbp, Z, Acc, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, gmu,
                                    gla, m_stream)))
       return rc;
-   // A = (Up - 2U + Um)/dt^2: the Up part (and L^T of the corrector kept for the U, Um parts)
-   if ((rc = launch_vjp_crop_axpy(m_box, 1.0 / dt2, pad, LT1, Y, m_stream))) return rc;
+   // A = (Up - 2U + Um)/dt^2: the Up part (L^T of the corrector stays in pad.o for the U, Um parts)
+   if ((rc = launch_vjp_axpy_pad(m_box, 1.0 / dt2, pad, Y, m_stream))) return rc;
    // predictor BC, transposed (acting on Up)
    if ((rc = launch_bcfortsg_T(m_box, m_wind, g.nx, g.ny, g.nz, g.bc, Y, Up, g.h, m_dsbp, mMu.dev, mLambda.dev, c.strx,
                                c.stry, gmu, gla, plane, m_stream)))
       return rc;
    // predictor: 2U - Um + dt^2/rho (L(U) + F); then the U, Um parts of every stage
    if ((rc = launch_vjp_pred_pw(m_box, rm, c.strx, c.stry, c.strz, dt2, mRho.dev, Y, Acc, Z, grho, pad, m_stream)) ||
-       (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, nullptr, pad,
+       (rc = launch_rhs4sg_T(m_box, rm, m_dsbp, Z, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, nullptr, pad2,
                              m_stream, true)) ||
        (rc = launch_rhs4sg_matgrad(m_box, rm, m_dsbp, Z, u, mMu.dev, mLambda.dev, g.h, c.strx, c.stry, c.strz, gmu,
                                    gla, m_stream)) ||
-       (rc = launch_vjp_lam(m_box, 1.0 / dt2, Db, LT1, Y, pad, lam_b, lam_c, m_stream)))
+       (rc = launch_vjp_lam(m_box, 1.0 / dt2, Db, pad, Y, pad2, lam_b, lam_c, m_stream)))
       return rc;
    return OK;
 }
@@ -287,12 +292,13 @@ double EW::workspace_bytes() const
 }
 
 // Device doubles of the gradient's work memory besides the stored states:
-// 5 work fields, 3 adjoint fields, 3 gradients, the padded material, one
-// plane of the boundary transposes and the receiver traces.
+// 4 work fields and one on the padded box, 3 adjoint fields, 3 gradients,
+// the padded material, one plane of the boundary transposes and the
+// receiver traces.
 static double adjoint_fixed_doubles(const Box& b, int nsteps, int nrec)
 {
-   return 8.0 * 3 * b.npts + 3.0 * b.npts + (double)adj_pad_doubles(b) + 3.0 * b.nij + 3.0 * (nsteps + 1) * nrec +
-          2.0 * (8 * nrec + 1);
+   return 7.0 * 3 * b.npts + 3.0 * pad_box(b).npts + 3.0 * b.npts + (double)adj_pad_doubles(b) + 3.0 * b.nij +
+          3.0 * (nsteps + 1) * nrec + 2.0 * (8 * nrec + 1);
 }
 
 // States held by a checkpoint interval c: the segment buffer (c + 2) and two
@@ -388,7 +394,8 @@ int EW::gradient(int nsteps, int ckpt, const double* gtr, double* grho, double* 
    double* tmp[5];
    double *lam[3], *gr[3], *plane = nullptr;
    std::vector<double*> seg(ckpt + 2), cp(2 * ncp);
-   for (int q = 0; q < 5 && rc == OK; q++) rc = grab(n3, &tmp[q]);
+   for (int q = 0; q < 4 && rc == OK; q++) rc = grab(n3, &tmp[q]);
+   if (rc == OK) rc = grab(3 * pad_box(m_box).npts, &tmp[4]);  // on the padded box
    double* padmem = nullptr;
    if (rc == OK) rc = grab(adj_pad_doubles(m_box), &padmem);
    for (int q = 0; q < 3 && rc == OK; q++) rc = grab(n3, &lam[q]);

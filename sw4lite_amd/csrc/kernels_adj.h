@@ -103,12 +103,13 @@ int launch_vjp_corr_pw(const Box& b, const RowMap& m, const double* strx, const 
 int launch_vjp_pred_pw(const Box& b, const RowMap& m, const double* strx, const double* stry, const double* strz,
                        double dt2, const double* rho, const double* yp, const double* acc, double* z, double* grho,
                        const AdjPad& pad, hipStream_t s);
-// lt1 = pad.o (the transposed operator's result) on the box, y += lt1/dt2;
-int launch_vjp_crop_axpy(const Box& b, double dt2i, const AdjPad& pad, double* lt1, double* y, hipStream_t s);
-// adjoints of U^n and U^{n-1} through one step, with a = lt1/dt2 and lt2 =
-// pad.o: lb += db - 2a + 2yp + lt2, lc = a - db - yp (lc is overwritten).
-int launch_vjp_lam(const Box& b, double dt2i, const double* db, const double* lt1, const double* yp,
-                   const AdjPad& pad, double* lb, double* lc, hipStream_t s);
+// y += pad.o/dt2 on the box (pad.o: the transposed operator's result);
+int launch_vjp_axpy_pad(const Box& b, double dt2i, const AdjPad& pad, double* y, hipStream_t s);
+// adjoints of U^n and U^{n-1} through one step, with a = p1.o/dt2 and o2 =
+// p2.o (the two transposed operators' results, on the padded box):
+// lb += db - 2a + 2yp + o2, lc = a - db - yp (lc is overwritten).
+int launch_vjp_lam(const Box& b, double dt2i, const double* db, const AdjPad& p1, const double* yp,
+                   const AdjPad& p2, double* lb, double* lc, hipStream_t s);
 // Receiver traces g[s][r][c] (s = 0..nsteps) as adjoint sources, grouped by
 // the grid value they act on: group q adds to lam[uidx[q]] the entries
 // ent[off[q] .. off[q+1]-1] (e = 3r + c) in that order, so receivers that

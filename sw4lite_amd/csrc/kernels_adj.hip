@@ -1069,9 +1069,8 @@ __global__ void k_vjp_pred_pw(AdjCtx A, AdjPad P, double dt2, const double* __re
    vjp_scaled_rows(A, P, p, dt2, dt2, rho, yp, acc, nullptr, z, grho);
 }
 
-// lt1 = the transposed operator's result (padded box) on the box; y += lt1/dt2
-__global__ void k_vjp_crop_axpy(Box b, Box pb, double dt2i, const double* __restrict__ o, double* __restrict__ lt1,
-                                double* __restrict__ y)
+// y += o1/dt2 on the box, o1 the transposed operator's result on the padded box
+__global__ void k_vjp_axpy_pad(Box b, Box pb, double dt2i, const double* __restrict__ o1, double* __restrict__ y)
 {
    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (p >= b.npts) return;
@@ -1080,16 +1079,12 @@ __global__ void k_vjp_crop_axpy(Box b, Box pb, double dt2i, const double* __rest
    const int k = b.kb + (int)(p / b.nij);
    const long q = sidx(pb, i, j, k);
 #pragma unroll
-   for (int c = 0; c < 3; c++) {
-      const double v = o[q + pb.npts * c];
-      lt1[p + b.npts * c] = v;
-      y[p + b.npts * c] += dt2i * v;
-   }
+   for (int c = 0; c < 3; c++) y[p + b.npts * c] += dt2i * o1[q + pb.npts * c];
 }
 
-// with a = lt1/dt2 and lt2 read on the padded box:
-// lb += db - 2a + 2yp + lt2, lc = a - db - yp
-__global__ void k_vjp_lam(Box b, Box pb, double dt2i, const double* __restrict__ db, const double* __restrict__ lt1,
+// with a = o1/dt2, o1 and o2 read on the padded box:
+// lb += db - 2a + 2yp + o2, lc = a - db - yp
+__global__ void k_vjp_lam(Box b, Box pb, double dt2i, const double* __restrict__ db, const double* __restrict__ o1,
                           const double* __restrict__ yp, const double* __restrict__ o2, double* __restrict__ lb,
                           double* __restrict__ lc)
 {
@@ -1102,7 +1097,7 @@ __global__ void k_vjp_lam(Box b, Box pb, double dt2i, const double* __restrict__
 #pragma unroll
    for (int c = 0; c < 3; c++) {
       const long v = p + b.npts * c;
-      const double a = dt2i * lt1[v], d = db[v], y = yp[v];
+      const double a = dt2i * o1[q + pb.npts * c], d = db[v], y = yp[v];
       lb[v] += d - 2 * a + 2 * y + o2[q + pb.npts * c];
       lc[v] = a - d - y;
    }
@@ -1294,16 +1289,16 @@ int launch_vjp_pred_pw(const Box& b, const RowMap& m, const double* strx, const 
    return check_hip(hipGetLastError(), "reverse predictor launch");
 }
 
-int launch_vjp_crop_axpy(const Box& b, double dt2i, const AdjPad& P, double* lt1, double* y, hipStream_t s)
+int launch_vjp_axpy_pad(const Box& b, double dt2i, const AdjPad& P, double* y, hipStream_t s)
 {
-   k_vjp_crop_axpy<<<cdiv(b.npts, 256), 256, 0, s>>>(b, P.pb, dt2i, P.o, lt1, y);
-   return check_hip(hipGetLastError(), "reverse step copy launch");
+   k_vjp_axpy_pad<<<cdiv(b.npts, 256), 256, 0, s>>>(b, P.pb, dt2i, P.o, y);
+   return check_hip(hipGetLastError(), "reverse step axpy launch");
 }
 
-int launch_vjp_lam(const Box& b, double dt2i, const double* db, const double* lt1, const double* yp, const AdjPad& P,
-                   double* lb, double* lc, hipStream_t s)
+int launch_vjp_lam(const Box& b, double dt2i, const double* db, const AdjPad& P1, const double* yp,
+                   const AdjPad& P2, double* lb, double* lc, hipStream_t s)
 {
-   k_vjp_lam<<<cdiv(b.npts, 256), 256, 0, s>>>(b, P.pb, dt2i, db, lt1, yp, P.o, lb, lc);
+   k_vjp_lam<<<cdiv(b.npts, 256), 256, 0, s>>>(b, P1.pb, dt2i, db, P1.o, yp, P2.o, lb, lc);
    return check_hip(hipGetLastError(), "reverse step adjoint launch");
 }
 
