@@ -221,23 +221,45 @@ int Batch::one_step(int ph)
    const SweepShot* tab = m_dtab + (ph * 3) * n;
    int rc;
    const int a0 = m_nasync[ph][0], a1 = m_nasync[ph][1], a2 = m_nasync[ph][2];
+   // each grid's SBP rows and ghost shell (which write only the planes and
+   // points the interior sweeps do not, from inputs the sweeps do not
+   // change) run on the side streams while the batched sweeps run, instead
+   // of after them (48 shots: 55.5 -> 55.1 ms per step with graph replay,
+   // 57.1 -> 56.4 ms without)
+   auto pred_tail = [&](int q) {
+      EW* e = m_shots[q];
+      const GridSetup& g = e->m_opt.grid;
+      double* acc = m_useacc[ph * n + q] ? m_acc[q] : nullptr;
+      return launch_pred_tail(e->m_box, g.nz, e->m_onesided, e->m_dsbp, m_base[3 * q + (ph + 1) % 3],
+                              m_base[3 * q + ph % 3], e->mMu.dev, e->mLambda.dev, e->mRho.dev, e->sg(), e->mUp.dev,
+                              g.h, e->mDt, m_side[q % NSIDE], e->zfill(), acc);
+   };
+   auto corr_tail = [&](int q) {
+      EW* e = m_shots[q];
+      const GridSetup& g = e->m_opt.grid;
+      const double* acc = m_useacc[ph * n + q] ? m_acc[q] : nullptr;
+      return launch_corr_tail(e->m_box, g.nz, e->m_onesided, e->m_dsbp, e->mUp.dev, m_base[3 * q + (ph + 1) % 3],
+                              m_base[3 * q + ph % 3], e->mMu.dev, e->mLambda.dev, e->mRho.dev, e->sg(),
+                              e->m_opt.beta, m_base[3 * q + (ph + 2) % 3], g.h, e->mDt, m_side[q % NSIDE], e->zfill(),
+                              acc, e->m_variant >= 9);
+   };
+   if ((rc = fork())) return rc;
+   for (int q = 0; q < n; q++)
+      if ((rc = pred_tail(q))) return rc;
    if ((rc = launch_sweep_batch_async(0, tab, a0, m_nblk_async[ph][0], m_s)) ||
-       (rc = launch_sweep_batch(0, tab + a0, n - a0, m_nblk[ph][0], m_s)) || (rc = fork()))
+       (rc = launch_sweep_batch(0, tab + a0, n - a0, m_nblk[ph][0], m_s)) || (rc = join()) || (rc = fork()))
       return rc;
    for (int q = 0; q < n; q++) {
       EW* e = m_shots[q];
       e->m_stream = m_side[q % NSIDE];
       const GridSetup& g = e->m_opt.grid;
-      const SgCoef c = e->sg();
       double* um = m_base[3 * q + ph % 3];
       double* u = m_base[3 * q + (ph + 1) % 3];
       const double dt = e->mDt;
       hipStream_t st = e->m_stream;
       const int zf = e->zfill();
       double* acc = m_useacc[ph * n + q] ? m_acc[q] : nullptr;
-      if ((rc = launch_pred_tail(e->m_box, g.nz, e->m_onesided, e->m_dsbp, u, um, e->mMu.dev, e->mLambda.dev,
-                                 e->mRho.dev, c, e->mUp.dev, g.h, dt, st, zf, acc)) ||
-          (rc = launch_src_apply(e->m_box, e->sources(), 0.0, 0, dt * dt, e->mRho.dev, e->mUp.dev, st, m_clk_t + q)) ||
+      if ((rc = launch_src_apply(e->m_box, e->sources(), 0.0, 0, dt * dt, e->mRho.dev, e->mUp.dev, st, m_clk_t + q)) ||
           (rc = e->enforceBC(e->mUp, e->mMu, e->mLambda, 0.0, zf)))
          return rc;
       if (acc) {
@@ -250,29 +272,25 @@ int Batch::one_step(int ph)
             return rc;
       }
    }
-   if ((rc = join()) || (rc = launch_sweep_batch_async(1, tab + n, a1, m_nblk_async[ph][1], m_s)) ||
+   if ((rc = join()) || (rc = fork())) return rc;
+   for (int q = 0; q < n; q++)
+      if ((rc = corr_tail(q))) return rc;
+   if ((rc = launch_sweep_batch_async(1, tab + n, a1, m_nblk_async[ph][1], m_s)) ||
        (rc = launch_sweep_batch(1, tab + n + a1, n - a1, m_nblk[ph][1], m_s)) ||
        (rc = launch_sweep_batch_async(2, tab + 2 * n, a2, m_nblk_async[ph][2], m_s)) ||
-       (rc = launch_sweep_batch(2, tab + 2 * n + a2, n - a2, m_nblk[ph][2], m_s)) || (rc = fork()))
+       (rc = launch_sweep_batch(2, tab + 2 * n + a2, n - a2, m_nblk[ph][2], m_s)) || (rc = join()) || (rc = fork()))
       return rc;
    for (int q = 0; q < n; q++) {
       EW* e = m_shots[q];
       hipStream_t st = m_side[q % NSIDE];
       e->m_stream = st;
-      const GridSetup& g = e->m_opt.grid;
-      const SgCoef c = e->sg();
       double* um = m_base[3 * q + ph % 3];
-      double* u = m_base[3 * q + (ph + 1) % 3];
       double* up2 = m_base[3 * q + (ph + 2) % 3];
       const double dt = e->mDt;
       Sarray v = e->mU;
       v.dev = up2;
       const int zf = e->zfill();
-      const double* acc = m_useacc[ph * n + q] ? m_acc[q] : nullptr;
-      if ((rc = launch_corr_tail(e->m_box, g.nz, e->m_onesided, e->m_dsbp, e->mUp.dev, u, um, e->mMu.dev,
-                                 e->mLambda.dev, e->mRho.dev, c, e->m_opt.beta, up2, g.h, dt, st, zf, acc,
-                                 e->m_variant >= 9)) ||
-          (rc = launch_src_apply(e->m_box, e->sources(), 0.0, 1, dt * dt * dt * dt / 12, e->mRho.dev, up2, st,
+      if ((rc = launch_src_apply(e->m_box, e->sources(), 0.0, 1, dt * dt * dt * dt / 12, e->mRho.dev, up2, st,
                                  m_clk_t + q)) ||
           (rc = e->enforceBC(v, e->mMu, e->mLambda, 0.0, zf)))
          return rc;

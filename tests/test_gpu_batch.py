@@ -63,6 +63,8 @@ def reference(nsteps, checkpoints):
     return out, tr
 
 
+# (the grids' SBP rows and ghost shells run on the side streams beside the
+# batched sweeps: the same kernels, the same results)
 @pytest.mark.parametrize("graph", [False, True])
 def test_batch_matches_single_shots(torch, graph):
     from sw4lite_amd.ew import Batch
