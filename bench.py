@@ -69,12 +69,13 @@ def parse():
     p.add_argument("--no-fp32", action="store_true", help="skip the FP32 compensated form")
     p.add_argument("--cpu-nz", type=int, default=192, help="depth of the CPU-baseline slab")
     p.add_argument("--cpu-steps", type=int, default=12)
-    p.add_argument("--variant", type=int, default=9, help="interior sweep kernels: 0 thread/point, 1 LDS k-march, 2 with prefetch, "
+    p.add_argument("--variant", type=int, default=10, help="interior sweep kernels: 0 thread/point, 1 LDS k-march, 2 with prefetch, "
                         "3 prefetch without register cap, 4 as 3 with the damping in the corrector sweep, "
                         "5 corrector as 4 and predictor as 1, 6 as 1 with an LDS damping sweep, "
                         "7 as 6 with the predictor planes loaded asynchronously into LDS (even row lengths), "
                         "8 as 7 with the acceleration stored by the predictor and staged by the corrector, "
-                        "9 as 8 with the damping sweep only over the supergrid layers")
+                        "9 as 8 with the damping sweep only over the supergrid layers, "
+                        "10 as 9 with rows of odd length staged by four-byte asynchronous loads")
     p.add_argument("--kchunk", type=int, default=0,
                    help="planes per workgroup of the LDS sweeps (0: the library's defaults, 64 for the predictor "
                         "and corrector sweeps, 32 for the damping sweep)")
@@ -886,10 +887,12 @@ def main():
     corr_pts = float(n) * n * (n - 6)
     corr_ms = stage_ms[3] / a.steps
     pred_ms = stage_ms[0] / a.steps
-    stored_acc = a.variant >= 8 and (n + 4) % 2 == 0
+    even = (n + 4) % 2 == 0
+    stored_acc = a.variant >= 8 and (even or a.variant >= 10)
     if stored_acc:
-        kern = {"pred": ("k_sweep_dma<0>", PRED_BYTES_PER_PT, PRED_ACC_BYTES_PER_PT),
-                "corr": ("k_sweep_dma<2>", CORR_ACC_BYTES_PER_PT, CORR_ACC_BYTES_PER_PT)}
+        dw = "" if even else "_dw"  # odd rows: four-byte asynchronous loads
+        kern = {"pred": ("k_sweep_dma%s<0>" % dw, PRED_BYTES_PER_PT, PRED_ACC_BYTES_PER_PT),
+                "corr": ("k_sweep_dma%s<2>" % dw, CORR_ACC_BYTES_PER_PT, CORR_ACC_BYTES_PER_PT)}
     elif a.variant == 7 and (n + 4) % 2 == 0:
         kern = {"pred": ("k_sweep_dma<0>", PRED_BYTES_PER_PT, PRED_BYTES_PER_PT),
                 "corr": ("k_sweep_lds<true,...>", CORR_BYTES_PER_PT, CORR_BYTES_PER_PT)}

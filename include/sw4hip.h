@@ -295,7 +295,7 @@ int sw4hip_ew_destroy(sw4hip_ew* ew);
 /* Stream used by every later call on this object (hipStream_t). */
 int sw4hip_ew_set_stream(sw4hip_ew* ew, void* stream);
 /* Interior kernels of the fused sweeps: variant 0 one thread per point;
- * variants 1-9 LDS-staged k-marching tiles with kchunk planes per workgroup:
+ * variants 1-10 LDS-staged k-marching tiles with kchunk planes per workgroup:
  * 1 plain at 2 waves/SIMD, 2 register prefetch of the next
  * plane, 3 prefetch without a register cap, 4 as 3 with the supergrid
  * damping inside the corrector sweep, 5 corrector as 4 and predictor as 1,
@@ -305,8 +305,10 @@ int sw4hip_ew_set_stream(sw4hip_ew* ew, void* stream);
  * acceleration (Up - 2U + Um)/dt^2 stored by the predictor (and its ghost
  * shell) and brought to LDS by the corrector the same way (one more field of
  * device memory, allocated at the first step; even row lengths, otherwise as
- * 7; the same bits as 7), 9 (the default) as 8 with the damping sweep only
- * over the tiles and planes of the supergrid layers (the same bits as 8).
+ * 7; the same bits as 7), 9 as 8 with the damping sweep only over the
+ * tiles and planes of the supergrid layers (the same bits as 8), 10 (the
+ * default) as 9 with rows of odd length staged by four-byte asynchronous
+ * loads and the acceleration stored for them too (the same bits as 9).
  * All give the same results to rounding.  kchunk > 0 sets the planes per
  * workgroup of every sweep; kchunk <= 0 keeps the defaults (64 for the
  * predictor and corrector sweeps, 32 for the damping sweep). */

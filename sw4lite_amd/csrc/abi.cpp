@@ -781,7 +781,7 @@ int sw4hip_ew_set_stream(sw4hip_ew* h, void* stream)
 int sw4hip_ew_set_kernel_variant(sw4hip_ew* h, int variant, int kchunk)
 {
    EWCHK(h);
-   if (variant < 0 || variant > 9) return set_error(E_INVAL, "variant must be 0..9");
+   if (variant < 0 || variant > 10) return set_error(E_INVAL, "variant must be 0..10");
    if (kchunk > 4096) return set_error(E_INVAL, "kchunk must be at most 4096");
    h->ew->m_variant = variant;
    h->ew->m_kchunk = kchunk > 0 ? kchunk : EW::kSweepChunk;

@@ -48,7 +48,7 @@ int EW::fused_once(double t, double* acc)
    // the time-step loop
    int ka = 0, kz = 0;
    fused_planes(m_box, g.nz, m_onesided, &ka, &kz);
-   double* af = acc && m_variant >= 8 && m_box.ni % 2 == 0 && kz >= ka ? acc : nullptr;
+   double* af = acc && m_variant >= 8 && (m_box.ni % 2 == 0 || m_variant >= 10) && kz >= ka ? acc : nullptr;
    const DampTile* dtiles = nullptr;
    int ndtiles = 0;
    int rc;
@@ -115,7 +115,7 @@ int EW::step_vjp(int n, double* u, double* um, double* u1, double* lam_a, double
    const int zf = zfill();
    int ka = 0, kz = 0;
    fused_planes(m_box, g.nz, m_onesided, &ka, &kz);
-   double* accf = m_variant >= 8 && m_box.ni % 2 == 0 && kz >= ka ? Acc : nullptr;
+   double* accf = m_variant >= 8 && (m_box.ni % 2 == 0 || m_variant >= 10) && kz >= ka ? Acc : nullptr;
    if (up_st) {
       Up = up_st;
       if (accf && acc_st)

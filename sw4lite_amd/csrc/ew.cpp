@@ -78,7 +78,7 @@ int EW::init()
 int EW::accel_field(double** p)
 {
    *p = nullptr;
-   if (m_variant < 8 || m_box.ni % 2 != 0) return OK;
+   if (m_variant < 8 || (m_box.ni % 2 != 0 && m_variant < 10)) return OK;
    if (!m_accf) {
       void* q = nullptr;
       if (hipMalloc(&q, sizeof(double) * 3 * m_box.npts) != hipSuccess) {

@@ -213,7 +213,7 @@ public:
    std::vector<GridPointSource> m_gps_extra;
    std::vector<int> m_first;
    hipStream_t m_stream = nullptr;
-   int m_variant = 9;   // interior sweep kernels: 0 thread per point, 1-9 LDS-staged k-march
+   int m_variant = 10;  // interior sweep kernels: 0 thread per point, 1-10 LDS-staged k-march
    // planes per workgroup of the LDS sweeps: predictor and corrector, and
    // the damping sweep over the supergrid layers (fastest measured at 512^3
    // on one device: 64 and 32 of 32..256)

@@ -394,8 +394,15 @@ __device__ __forceinline__ double accel_of(double up, double u, double um, doubl
 // 16-byte aligned: an even row length ni and 16-byte aligned arrays (the
 // launcher checks both).  Pairs off the box load from the plane's first
 // point instead (a valid address); only inactive points would read them.
+// DW: rows of any length and 8-byte aligned arrays, by four-byte loads: each
+// staged row of the tile (HX doubles, 2 HX dwords) is moved by one wave, lane
+// l taking dwords l and, for l < 2 HX - 64, 64 + l; doubles off the box load
+// the row's first one.  BATCH: the batched kernels' instantiations, the same
+// code kept apart from the single-grid kernels' (hipcc's host pass rejects a
+// specialization that a non-template kernel instantiated when a kernel
+// template uses it as well).
 #define SW4_LDS(p) ((__attribute__((address_space(3))) void*)(p))
-template <int MODE>
+template <int MODE, bool DW = false, bool BATCH = false>
 __device__ __forceinline__ void sweep_dma_body(unsigned bid, Box b, TileMap map, int kbeg, int kend, int kc,
                                                const double* __restrict__ u, const double* __restrict__ um,
                                                const double* __restrict__ mu, const double* __restrict__ la,
@@ -407,6 +414,8 @@ __device__ __forceinline__ void sweep_dma_body(unsigned bid, Box b, TileMap map,
    constexpr int NP = (PP + NT - 1) / NT;  // asynchronous loads per thread and plane
    constexpr int MP = PL / 2;              // pairs of a material plane
    static_assert(HX % 2 == 0 && MP <= NT, "pairs must tile the rows; one material load per thread");
+   static_assert(2 * HX > 64 && 2 * HX <= 128, "a row is moved by one wave in two loads");
+   constexpr int NW = NT / 64;  // waves of the workgroup
    __shared__ __align__(16) double su[6 * 3 * PL];  // plane kp in slot (kp - k0 + 2) % 6
    __shared__ __align__(16) double sm[2 * PL];      // mu of plane kp in half (kp - k0) & 1
    __shared__ __align__(16) double sl[2 * PL];
@@ -428,30 +437,66 @@ __device__ __forceinline__ void sweep_dma_body(unsigned bid, Box b, TileMap map,
 
    // pair m = tid + p NT of the staged (c, jj, ii) tile: offset of its first double
    long goff[NP];
+   long moff = 0;
+   // DW: this lane's dword offsets in a staged row (first and second load)
+   int dwo0 = 0, dwo1 = 0;
+   if constexpr (DW) {
+      const int lane = tid & 63, d0 = lane >> 1, d1 = 32 + (lane >> 1);
+      dwo0 = 2 * (ia + d0 <= b.ie ? d0 : 0) + (lane & 1);
+      dwo1 = 2 * (ia + d1 <= b.ie ? d1 : 0) + (lane & 1);
+   } else {
 #pragma unroll
-   for (int p = 0; p < NP; p++) {
-      const int e = 2 * (tid + p * NT);
-      const int c = e / PL, r = e - c * PL, jj = r / HX, ii = r - jj * HX;
-      const int gi = ia + ii, gj = ja + jj;
-      const bool ok = e < 3 * PL && gi <= b.ie && gj <= b.je;
-      goff[p] = ok ? (gi - b.ib) + ni * (gj - b.jb) + npts * c : 0;
-   }
-   long moff;
-   {
+      for (int p = 0; p < NP; p++) {
+         const int e = 2 * (tid + p * NT);
+         const int c = e / PL, r = e - c * PL, jj = r / HX, ii = r - jj * HX;
+         const int gi = ia + ii, gj = ja + jj;
+         const bool ok = e < 3 * PL && gi <= b.ie && gj <= b.je;
+         goff[p] = ok ? (gi - b.ib) + ni * (gj - b.jb) + npts * c : 0;
+      }
       const int e = 2 * tid, jj = e / HX, ii = e - jj * HX;
       const int gi = ia + ii, gj = ja + jj;
       moff = e < PL && gi <= b.ie && gj <= b.je ? (gi - b.ib) + ni * (gj - b.jb) : 0;
    }
+   // DW: one staged row from its first double (lds: the row's slot, wave-uniform)
+   auto dma_row = [&](const double* row, double* lds) {
+      const float* g = reinterpret_cast<const float*>(row);
+      __builtin_amdgcn_global_load_lds(g + dwo0, SW4_LDS(lds), 4, 0, 0);
+      if ((tid & 63) < 2 * HX - 64) __builtin_amdgcn_global_load_lds(g + dwo1, SW4_LDS(lds + 32), 4, 0, 0);
+   };
    auto dma_plane = [&](int kp, int slot) {
       const double* src = u + nij * (kp - b.kb);
-      double* dst = su + slot * 3 * PL + 2 * 64 * wv;
+      if constexpr (DW) {
+         // rows r = (c, jj) of the plane, r = wv, wv + NW, ...; rows off the box load the last one
+         double* dst = su + slot * 3 * PL;
 #pragma unroll
-      for (int p = 0; p < NP; p++)
-         if (tid + p * NT < PP) __builtin_amdgcn_global_load_lds(src + goff[p], SW4_LDS(dst + 2 * p * NT), 16, 0, 0);
+         for (int t = 0; t < (3 * HY + NW - 1) / NW; t++) {
+            const int r = wv + NW * t;
+            if (r < 3 * HY) {
+               const int c = r / HY, jj = r - c * HY;
+               dma_row(src + npts * c + ni * (min(ja + jj, b.je) - b.jb) + (ia - b.ib), dst + r * HX);
+            }
+         }
+      } else {
+         double* dst = su + slot * 3 * PL + 2 * 64 * wv;
+#pragma unroll
+         for (int p = 0; p < NP; p++)
+            if (tid + p * NT < PP)
+               __builtin_amdgcn_global_load_lds(src + goff[p], SW4_LDS(dst + 2 * p * NT), 16, 0, 0);
+      }
    };
    auto dma_mat = [&](int kp, int half) {
       const long pb = nij * (kp - b.kb);
-      if (tid < MP) {
+      if constexpr (DW) {
+#pragma unroll
+         for (int t = 0; t < (HY + NW - 1) / NW; t++) {
+            const int r = wv + NW * t;
+            if (r < HY) {
+               const long ro = pb + ni * (min(ja + r, b.je) - b.jb) + (ia - b.ib);
+               dma_row(mu + ro, sm + half * PL + r * HX);
+               dma_row(la + ro, sl + half * PL + r * HX);
+            }
+         }
+      } else if (tid < MP) {
          __builtin_amdgcn_global_load_lds(mu + pb + moff, SW4_LDS(sm + half * PL + 2 * 64 * wv), 16, 0, 0);
          __builtin_amdgcn_global_load_lds(la + pb + moff, SW4_LDS(sl + half * PL + 2 * 64 * wv), 16, 0, 0);
       }
@@ -559,6 +604,17 @@ k_sweep_dma(Box b, TileMap map, int kbeg, int kend, int kc, const double* __rest
             double cupd, double dt2i)
 {
    sweep_dma_body<MODE>(blockIdx.x, b, map, kbeg, kend, kc, u, um, mu, la, rho, sg, out, acc, cof, cupd, dt2i);
+}
+
+// The same sweep with four-byte loads: rows of any length (DW above).
+template <int MODE>
+__global__ void __launch_bounds__(NT, 2)
+k_sweep_dma_dw(Box b, TileMap map, int kbeg, int kend, int kc, const double* __restrict__ u,
+               const double* __restrict__ um, const double* __restrict__ mu, const double* __restrict__ la,
+               const double* __restrict__ rho, SgCoef sg, double* __restrict__ out, double* __restrict__ acc,
+               double cof, double cupd, double dt2i)
+{
+   sweep_dma_body<MODE, true>(blockIdx.x, b, map, kbeg, kend, kc, u, um, mu, la, rho, sg, out, acc, cof, cupd, dt2i);
 }
 
 // The asynchronous loads need whole 16-byte pairs: even row length, aligned arrays.
@@ -941,6 +997,20 @@ __global__ void __launch_bounds__(32 * 8, 4) k_damp_batch(const SweepShot* __res
    damp_lds_tile<8>(tix, tjy, k0, k1, h.b, h.out, h.u, h.um, h.rho, h.sg, h.beta);
 }
 
+// The register-staged damping over a tile list of one grid (entry t of the
+// list, each XCD taking one contiguous run), for rows of odd length.
+__global__ void __launch_bounds__(32 * 8, 4) k_damp_lds_tiles(Box b, const DampTile* __restrict__ tiles, int ntiles,
+                                                              double* __restrict__ up, const double* __restrict__ u,
+                                                              const double* __restrict__ um,
+                                                              const double* __restrict__ rho, SgCoef sg, double beta)
+{
+   const int per = (ntiles + 7) / 8;
+   const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+   if (t >= ntiles) return;
+   const DampTile d = tiles[t];
+   damp_lds_tile<8>(d.tix, d.tjy, d.k0, d.k1, b, up, u, um, rho, sg, beta);
+}
+
 int fill_sweep_shot(bool damping, const Box& b, int k1, int k2, int kc, const double* upin, const double* u,
                     const double* um, const double* mu, const double* la, const double* rho, const SgCoef& sg,
                     double* out, double h, double dt, double beta, bool corr, int grid0, SweepShot* sh,
@@ -980,8 +1050,8 @@ int fill_sweep_shot(bool damping, const Box& b, int k1, int k2, int kc, const do
 __global__ void __launch_bounds__(NT, 2) k_sweep_dma_batch(const SweepShot* __restrict__ shots, int nshots)
 {
    const SweepShot& h = shots[find_shot(shots, nshots, blockIdx.x)];
-   sweep_dma_body<DMA_PRED>(blockIdx.x - h.grid0, h.b, h.map, h.kbeg, h.kend, h.kc, h.u, h.um, h.mu, h.la, h.rho, h.sg,
-                            h.out, h.acc, h.cof, h.cupd, h.dt2i);
+   sweep_dma_body<DMA_PRED, false, true>(blockIdx.x - h.grid0, h.b, h.map, h.kbeg, h.kend, h.kc, h.u, h.um, h.mu,
+                                         h.la, h.rho, h.sg, h.out, h.acc, h.cof, h.cupd, h.dt2i);
 }
 
 // Batched corrector staging the acceleration its predictor stored (acc),
@@ -989,8 +1059,8 @@ __global__ void __launch_bounds__(NT, 2) k_sweep_dma_batch(const SweepShot* __re
 __global__ void __launch_bounds__(NT, 2) k_sweep_dma_corr_batch(const SweepShot* __restrict__ shots, int nshots)
 {
    const SweepShot& h = shots[find_shot(shots, nshots, blockIdx.x)];
-   sweep_dma_body<DMA_CORR>(blockIdx.x - h.grid0, h.b, h.map, h.kbeg, h.kend, h.kc, h.acc, h.upin, h.mu, h.la, h.rho,
-                            h.sg, h.out, nullptr, h.cof, h.cupd, h.dt2i);
+   sweep_dma_body<DMA_CORR, false, true>(blockIdx.x - h.grid0, h.b, h.map, h.kbeg, h.kend, h.kc, h.acc, h.upin,
+                                         h.mu, h.la, h.rho, h.sg, h.out, nullptr, h.cof, h.cupd, h.dt2i);
 }
 
 __global__ void __launch_bounds__(NT, 3) k_damp_dma_batch(const SweepShot* __restrict__ shots, int nshots)
@@ -1132,6 +1202,11 @@ int launch_damp_lds(const Box& b, int k1, int k2, int kc, double* up, const doub
    if (tiles && dma_fits(b, u, um, rho) && ((uintptr_t)up & 15) == 0) {
       if (ntiles > 0)
          k_damp_dma_tiles<<<8 * cdiv(ntiles, 8), NT, 0, s>>>(b, tiles, ntiles, up, u, um, rho, sg, beta);
+      return check_hip(hipGetLastError(), "damping sweep launch");
+   }
+   if (tiles) {  // odd rows: the register-staged tile over the listed tiles only
+      if (ntiles > 0)
+         k_damp_lds_tiles<<<8 * cdiv(ntiles, 8), 32 * 8, 0, s>>>(b, tiles, ntiles, up, u, um, rho, sg, beta);
       return check_hip(hipGetLastError(), "damping sweep launch");
    }
    const long nx = b.ie - b.ib - 3, ny = b.je - b.jb - 3;
@@ -1403,11 +1478,19 @@ int launch_sweep_lds(bool corr, int variant, const Box& b, int k1, int k2, int k
       break;
    case 7:  // as 6, the predictor's planes by asynchronous global-to-LDS loads where they fit
    case 8:  // as 7; with acc, the predictor also stores the acceleration and the corrector stages it
-   case 9:  // as 8 (the damping sweep runs only where the supergrid profiles are nonzero)
+   case 9:   // as 8 (the damping sweep runs only where the supergrid profiles are nonzero)
+   case 10:  // as 9, rows of odd length by four-byte asynchronous loads (acceleration stored there too)
       if (variant >= 8 && acc) {
-         if (!dma_fits(b, acc, mu, la) || !dma_fits(b, corr ? upin : u, mu, la))
+         const bool fit = dma_fits(b, acc, mu, la) && dma_fits(b, corr ? upin : u, mu, la);
+         if (!fit && variant < 10)
             return set_error(E_INVAL, "acceleration sweeps need an even row length and aligned arrays");
-         if (corr)
+         if (!fit && corr)
+            k_sweep_dma_dw<DMA_CORR><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, acc, upin, mu, la, rho, sg, out, nullptr,
+                                                        cof, c4, dt2i);
+         else if (!fit)
+            k_sweep_dma_dw<DMA_PRED><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, u, um, mu, la, rho, sg, out, acc, cof,
+                                                        c2, dt2i);
+         else if (corr)
             k_sweep_dma<DMA_CORR><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, acc, upin, mu, la, rho, sg, out, nullptr,
                                                      cof, c4, dt2i);
          else
@@ -1418,6 +1501,9 @@ int launch_sweep_lds(bool corr, int variant, const Box& b, int k1, int k2, int k
       else if (dma_fits(b, u, mu, la))
          k_sweep_dma<DMA_PRED><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, u, um, mu, la, rho, sg, out, nullptr, cof, c2,
                                                   dt2i);
+      else if (variant >= 10)
+         k_sweep_dma_dw<DMA_PRED><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, u, um, mu, la, rho, sg, out, nullptr, cof,
+                                                     c2, dt2i);
       else
          SW4_SWEEP(false, false, 2, false);
       break;
@@ -1440,12 +1526,13 @@ int launch_sweep_lds_T(const Box& b, int k1, int k2, int kc, const double* w, co
    map.nty = (int)cdiv(ny, TY);
    map.ntiles = map.ntx * map.nty * (int)cdiv(k2 - k1 + 1, kc);
    const unsigned grd = 8 * cdiv(map.ntiles, 8);
+   // rows of odd length by four-byte asynchronous loads (the same results)
    if (dma_fits(b, w, mu, la))
       k_sweep_dma<DMA_TR><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, w, nullptr, mu, la, nullptr, sg, out, nullptr,
                                              1.0 / (h * h), 1.0, 1.0);
    else
-      k_sweep_lds<false, false, 2, false, true><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, nullptr, w, nullptr, mu, la,
-                                                                  nullptr, sg, out, 1.0 / (h * h), 1.0, 1.0, 0.0);
+      k_sweep_dma_dw<DMA_TR><<<grd, NT, 0, s>>>(b, map, k1, k2, kc, w, nullptr, mu, la, nullptr, sg, out, nullptr,
+                                                1.0 / (h * h), 1.0, 1.0);
    return check_hip(hipGetLastError(), "transposed sweep launch");
 }
 

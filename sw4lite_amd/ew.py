@@ -287,7 +287,7 @@ class EW:
 
     # set-up -------------------------------------------------------------
     def set_kernel_variant(self, variant, kchunk=0):
-        """0: one thread per point; 1-7: LDS-staged k-marching tiles (1 plain
+        """0: one thread per point; 1-10: LDS-staged k-marching tiles (1 plain
         at 2 waves/SIMD, 2 with register prefetch, 3 prefetch without a
         register cap, 4 as 3 with the corrector's damping in the same sweep, 5
         corrector as 4 and predictor as 1, 6 as 1 with an LDS damping sweep,
@@ -296,7 +296,9 @@ class EW:
         the predictor and loaded asynchronously by the corrector (one more
         field of device memory; even row lengths, otherwise as 7), 9 as 8
         with the damping sweep only over the tiles of the supergrid layers,
-        where the damping profiles are nonzero (the default; same bits)).
+        where the damping profiles are nonzero (same bits), 10 as 9 with
+        rows of odd length staged by four-byte asynchronous loads and the
+        acceleration stored for them too (the default; same bits)).
         kchunk > 0: planes per workgroup of every sweep; 0: the library's
         defaults (64 for the predictor and corrector, 32 for the damping)."""
         L.check(self.lib.sw4hip_ew_set_kernel_variant(self.h, int(variant), int(kchunk)))

@@ -271,6 +271,11 @@ def test_full_size_fused_matches_reference_order(torch):
     (9, 64, [2, 2, 2, 2, 0, 2], 96),
     (9, 7, [2, 2, 2, 2, 2, 2], 96),
     (9, 5, [2, 2, 2, 2, 0, 0], 45),
+    # variant 10 takes four-byte asynchronous loads (and the stored
+    # acceleration) for odd row lengths
+    (10, 64, [2, 2, 2, 2, 0, 2], 45),
+    (10, 7, [2, 2, 2, 2, 2, 2], 45),
+    (10, 5, [2, 2, 2, 2, 0, 0], 44),
 ]])
 def test_sweep_variants_match_oracle(torch, variant, kchunk, bc, nx):
     """The LDS-staged k-marching sweeps on a grid that is not a multiple of
@@ -380,6 +385,34 @@ def test_damping_layers_bitwise(torch, bc, kchunk):
     ndiff = int(np.count_nonzero(u8 != u9)) + int(np.count_nonzero(um8 != um9))
     print("damping layers: bc", bc, "kchunk", kchunk, "points differing from variant 8:", ndiff, flush=True)
     assert ndiff == 0
+
+
+@pytest.mark.parametrize("bc,kchunk", [([2, 2, 2, 2, 0, 2], 64), ([2, 2, 2, 2, 2, 2], 7), ([3, 3, 3, 3, 0, 1], 5)])
+def test_odd_rows_bitwise(torch, bc, kchunk):
+    """A grid of odd row length (nx = 95): variant 8 runs the register-staged
+    sweeps and damps the whole box, variant 9 damps only the listed tiles of
+    the supergrid layers, variant 10 stages by four-byte asynchronous loads
+    and stores the acceleration; the solutions must agree to the last bit."""
+    from sw4lite_amd import EW
+    nx, ny, nz, h = 95, 40, 41, 0.05
+    rho, mu, la = layered_material(nx, ny, nz, h, 9)
+    res = []
+    for variant in (8, 9, 10):
+        e = EW(nx, ny, nz, h, bc, corder=1, sg_gp_thickness=6)
+        e.set_kernel_variant(variant, kchunk)
+        e.set_material(rho, mu, la)
+        for s in SOURCES:
+            e.add_moment_source(s["x"] * 0.5, s["y"] * 0.5, s["z"] * 0.5, s["m"], s["t0"], s["freq"], s["tfkind"])
+        e.setup(nsteps=8)
+        e.timesteploop(8, mode=0)
+        res.append(e.get_solution())
+        e.close()
+    (u8, um8) = res[0]
+    assert np.max(np.abs(u8)) > 0
+    for v, (u, um) in zip((9, 10), res[1:]):
+        ndiff = int(np.count_nonzero(u8 != u)) + int(np.count_nonzero(um8 != um))
+        print("odd rows: bc", bc, "kchunk", kchunk, "variant", v, "points differing from variant 8:", ndiff, flush=True)
+        assert ndiff == 0
 
 
 def test_receivers_match_solution(torch):
