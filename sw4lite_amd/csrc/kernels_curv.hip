@@ -339,6 +339,12 @@ k_addsgd4c(Box b, double* __restrict__ up, const double* __restrict__ u, const d
    const int k = b.kb + 2 + blockIdx.z;
    if (i > b.ie - 2 || j > b.je - 2) return;
    const int ii = i - b.ib, jj = j - b.jb;
+   // every term carries one of these damping coefficients: where all six
+   // vanish (the columns away from the x and y supergrid layers) the update
+   // subtracts zero, and the point is left as it is
+   if (sg.dcx[ii - 1] == 0 && sg.dcx[ii] == 0 && sg.dcx[ii + 1] == 0 && sg.dcy[jj - 1] == 0 && sg.dcy[jj] == 0 &&
+       sg.dcy[jj + 1] == 0)
+      return;
    const long p = sidx(b, i, j, k);
    const double w = beta / (rho[p] * jac[p]);
    double rx[3], ry[3];
