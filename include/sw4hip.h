@@ -162,7 +162,9 @@ int sw4hip_dpdmtfort(int ib, int ie, int jb, int je, int kb, int ke, const doubl
                      const double* u, const double* um, double* uacc, double dt2i, void* stream);
 
 /* Replaces EW::addsgd4fort / addsgd4fort_indrev (ew-cfromfort.C:748-849,
- * :952-1056).  1-D arrays are device arrays. */
+ * :952-1056).  1-D arrays are device arrays.  Points where dcx, dcy and dcz
+ * vanish at the point and both its neighbours are left untouched (the
+ * reference subtracts zero there, for finite data). */
 int sw4hip_addsgd4(int corder, int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast,
                    double* up, const double* u, const double* um, const double* rho,
                    const double* dcx, const double* dcy, const double* dcz, const double* strx,

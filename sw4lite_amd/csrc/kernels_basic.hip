@@ -98,6 +98,13 @@ k_addsgd4(Box b, double* __restrict__ up, const double* __restrict__ u,
    const int j = b.jb + 2 + blockIdx.y * blockDim.y + threadIdx.y;
    const int k = b.kb + 2 + blockIdx.z;
    if (i > b.ie - 2 || j > b.je - 2 || k > b.ke - 2) return;
+   // every term carries one of these damping coefficients: where all nine
+   // vanish (away from the supergrid layers) the update subtracts zero for
+   // finite data, and the point is left as it is
+   const int ii = i - b.ib, jj = j - b.jb, kk = k - b.kb;
+   if (sg.dcx[ii - 1] == 0 && sg.dcx[ii] == 0 && sg.dcx[ii + 1] == 0 && sg.dcy[jj - 1] == 0 && sg.dcy[jj] == 0 &&
+       sg.dcy[jj + 1] == 0 && sg.dcz[kk - 1] == 0 && sg.dcz[kk] == 0 && sg.dcz[kk + 1] == 0)
+      return;
    double d[3];
    addsgd4_point<CO>(b, i, j, k, u, um, rho, sg, beta, d);
    for (int c = 0; c < 3; c++) up[vidx<CO>(b, c, i, j, k)] -= d[c];
